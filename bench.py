@@ -1,0 +1,164 @@
+"""Benchmark of the particle step (one PICCloud::evolve() per step) on one MI355X.
+
+Workload (default): two-species thermalisation, 32^3 all-cyclic hex box, 64 electrons +
+64 singly charged ions (m_i = 1836 m_e) per cell = 4,194,304 parcels, Maxwellian 10 eV /
+1 eV at n = 1e18 m^-3, Boris push, Nanbu binary Coulomb collisions with Takizuka-Abe
+pairing (lnL = 10), no field solve, FP64 throughout.  Synthetic initial state generated on
+the device.  Prints one JSON line.
+
+Roofline: every kernel of the step is HBM-bound.  Algorithmic bytes per parcel:
+  move kernel  : read + write of the parcel state (3 f64 barycentric + 3 U + i32 tet)
+                 plus the i32 id that travels with it = 2 x 56 B = 112 B (FP64),
+                 2 x 44 B = 88 B (mixed: U in f32)
+  Coulomb      : read + write of U plus the id read = 24 + 24 + 4 = 52 B (FP64)
+achieved = bytes per launch / average launch time (HIP events on the context stream).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s + %HBM peak; Poisson ms/step learned-guess vs plain PCG"
+
+
+def state_bytes(precision):
+    u = 4 if precision == "mixed" else 8
+    return 3 * 8 + 3 * u + 4 + 4
+
+
+def cpu_baseline(ncell_sample=8, ppc=64, target_s=12.0):
+    """The oracle (FP64 C restatement, one core) on a bounded sample of the same workload:
+    ncell_sample^3 cells of the same size, same ppc, dt and models."""
+    from oracle import picref
+    from picfoam_amd import cases
+    mesh, species, models, ex = cases.thermalisation(ncell=ncell_sample, ppc=ppc)
+    rm = picref.RefMesh(mesh)
+    rng = np.random.default_rng(1)
+    nc = rm.n_cells
+    tc, _, _ = rm.tets()
+    starts = np.searchsorted(tc, np.arange(nc))
+    dt = ex["dt"]
+    parts = []
+    for s, T in ((0, ex["Te_K"]), (1, ex["Ti_K"])):
+        n = nc * ppc
+        tet = (starts.repeat(ppc) + rng.integers(0, 12, n)).astype(np.int32)
+        e = -np.log(1 - rng.random((n, 4)))
+        y = np.ascontiguousarray((e / e.sum(1, keepdims=True))[:, 1:])
+        U = rng.normal(size=(n, 3)) * np.sqrt(cases.KB * T / species[s]["mass"])
+        parts.append([y, tet, U, np.arange(n, dtype=np.uint32)])
+    sp = [(species[s]["mass"], species[s]["charge"], species[s]["nEquivalentParticles"]) for s in range(2)]
+    cl = np.zeros((8, 8)); cl[:2, :2] = 10.0
+    zeros = np.zeros((nc, 3))
+    Q0 = abs(sp[0][1] * sp[0][2])
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        for s in range(2):
+            y, tet, U, ids = parts[s]
+            y, U, tet, _, _ = rm.move("Boris", "CellAverage", y, U, tet, sp[s][1], sp[s][0], dt, zeros, None, zeros)
+            order, cs = rm.sort_by_cell(tet)
+            parts[s] = [y[order], tet[order], U[order], ids[order], cs]
+        Us, _ = picref.coulomb(rm, sp, [parts[0][2], parts[1][2]], [parts[0][3], parts[1][3]],
+                               [parts[0][4], parts[1][4]], dt, seed=1, step=steps, coulomb_log=cl)
+        for s in range(2):
+            parts[s][2] = Us[s]
+        rm.deposit("CellAverage", [(parts[s][0], parts[s][1], sp[s][1] * sp[s][2]) for s in range(2)], Q0)
+        for s in range(2):
+            parts[s] = parts[s][:4]
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= target_s:
+            break
+    n_total = 2 * nc * ppc
+    return {"value": n_total * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle, {ncell_sample}^3 cells x {ppc}+{ppc} ppc ({n_total} parcels), {steps} steps "
+                      f"in {el:.1f} s, same dt/models as the GPU workload"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--ncell", type=int, default=32)
+    ap.add_argument("--ppc", type=int, default=64)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU domain decomposition is a design, not code)", file=sys.stderr)
+        sys.exit(2)
+
+    from picfoam_amd import cases, picmi
+
+    mesh, species, models, ex = cases.thermalisation(ncell=args.ncell, ppc=args.ppc, precision=args.precision)
+    print(f"bench: building {args.ncell}^3 mesh, {2 * args.ncell ** 3 * args.ppc} parcels", file=sys.stderr,
+          flush=True)
+    cloud = picmi.Cloud(mesh, species, models)
+    cloud.init_maxwellian(0, args.ppc, ex["Te_K"], id0=0, seed=11)
+    cloud.init_maxwellian(1, args.ppc, ex["Ti_K"], id0=0, seed=12)
+    dt = ex["dt"]
+    n_total = cloud.count(0) + cloud.count(1)
+    for _ in range(args.warmup):
+        cloud.step(dt)
+    cloud.sync()
+    cloud.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cloud.step(dt)
+    cloud.sync()
+    t1 = time.perf_counter()
+    stages = cloud.stage_times()
+    cloud.profile(False)
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n_total * args.steps / elapsed
+    # dominant kernel and its roofline
+    sb = state_bytes(args.precision)
+    move_ms, move_n = stages["move"]
+    coul_ms, coul_n = stages["coulomb"]
+    per_launch_move = move_ms / max(move_n, 1)
+    per_launch_coul = coul_ms / max(coul_n, 1)
+    move_bytes = (n_total / 2) * 2 * sb          # one species per launch
+    coul_bytes = n_total * (2 * 3 * (4 if args.precision == "mixed" else 8) + 4)
+    cand = {"move": (per_launch_move, move_bytes), "coulomb": (per_launch_coul, coul_bytes)}
+    dom = max(cand, key=lambda k: cand[k][0] * (move_n if k == "move" else coul_n))
+    t_ms, b = cand[dom]
+    achieved = b / (t_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": {"move": "k_move", "coulomb": "k_coulomb"}[dom], "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": b, "avg_launch_ms": round(t_ms, 4)}
+    stage_ms = {k: round(v[0] / args.steps, 4) for k, v in stages.items() if v[1]}
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32 push / f64 track",
+        "data": "synthetic",
+        "config": {"workload": f"two-species thermalisation, {args.ncell}^3 cyclic hex, {args.ppc}+{args.ppc} ppc "
+                               f"({n_total} parcels), Boris + Nanbu/TakizukaAbe Coulomb (lnL 10), no field solve",
+                   "parcels": n_total, "dt": dt, "precision": args.precision},
+        "roofline": roof,
+        "stage_ms_per_step": stage_ms,
+    }
+    cloud.close()
+    if not args.no_cpu_baseline:
+        print("bench: timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(target_s=args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,205 @@
+/*
+ * picmi.h — C-ABI of the MI355X-native particle step (libpicmi.so).
+ *
+ * The reference (picFoam, OpenFOAM-9) selects its submodels at run time from
+ * constant/picProperties and calls them once per parcel through virtual functions.  A GPU
+ * cannot be driven per parcel, so this ABI keeps the same model names and dictionary
+ * keywords but is batch-granular: one call runs one stage of PICCloud::evolve() for every
+ * parcel on the device.  All pointers are plain host pointers; nothing is retained after a
+ * call returns.  Every entry point returns 0 (PIC_OK) or a negative PIC_ERR_* code and
+ * leaves a message for pic_last_error().  No C++ exception crosses this boundary.
+ */
+#ifndef PICMI_H
+#define PICMI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PICMI_ABI_VERSION 1
+
+typedef struct pic_ctx pic_ctx;
+
+/* status codes; the reference aborts the process for each of these (FatalError) */
+enum {
+    PIC_OK = 0,
+    PIC_ERR_INVALID = -1,        /* bad argument */
+    PIC_ERR_UNKNOWN_MODEL = -2,  /* e.g. ParticlePusherNew.C:47-55 (lists valid names) */
+    PIC_ERR_SUPERLUMINAL = -3,   /* |U| >= c, PICParcel.C:61-66 */
+    PIC_ERR_HIP = -4,            /* HIP runtime error */
+    PIC_ERR_CAPACITY = -5,       /* more parcels than the species capacity */
+    PIC_ERR_MESH = -6,           /* inconsistent polyMesh */
+    PIC_ERR_UNSUPPORTED = -7,    /* option outside the implemented set */
+    PIC_ERR_CELL_OVERFLOW = -8,  /* more parcels in one cell than a collision kernel holds */
+    PIC_ERR_NOT_CONVERGED = -9,  /* Poisson solve hit its iteration limit */
+    PIC_ERR_NO_DEVICE = -10      /* no HIP device */
+};
+
+/* constant/polyMesh/boundary types */
+enum { PIC_PATCH_PATCH = 0, PIC_PATCH_WALL = 1, PIC_PATCH_CYCLIC = 2 };
+/* phi boundary conditions (0/phiE) */
+enum { PIC_BC_ZERO_GRADIENT = 0, PIC_BC_FIXED_VALUE = 1 };
+/* arithmetic of the particle step */
+enum { PIC_PREC_FP64 = 0, PIC_PREC_MIXED = 1 };
+/* Poisson solver arithmetic */
+enum { PIC_POISSON_FP64 = 0, PIC_POISSON_MIXED_IR = 1 };
+/* cell fields for pic_set_cell_field / pic_get_cell_field */
+enum {
+    PIC_FIELD_E = 0,            /* [n_cells*3] electric field */
+    PIC_FIELD_B = 1,            /* [n_cells*3] magnetic field */
+    PIC_FIELD_PHI = 2,          /* [n_cells]   electric potential */
+    PIC_FIELD_RHO = 3,          /* [n_cells]   charge density (get only) */
+    PIC_FIELD_SPACE_CHARGE = 4, /* [n_cells]   background space charge density */
+    PIC_FIELD_E_POINT = 5,      /* [n_points*3] E interpolated to the mesh points (get only) */
+    PIC_FIELD_MCC_SIGMA_CR_MAX = 6, /* [n_cells] background (sigma cR)_max (picSigmaTcRMax) */
+    PIC_FIELD_MCC_REMAINDER = 7     /* [n_cells] background candidate remainder */
+};
+/* cross-section processes (CrossSectionModel.H:52-59), tabulated "List" model */
+enum { PIC_XS_ELASTIC = 0, PIC_XS_EXCITATION = 1, PIC_XS_IONIZATION = 2 };
+/* stage ids for pic_stage_times */
+enum {
+    PIC_STAGE_MOVE = 0, PIC_STAGE_SORT = 1, PIC_STAGE_COULOMB = 2, PIC_STAGE_MCC = 3,
+    PIC_STAGE_DEPOSIT = 4, PIC_STAGE_POISSON = 5, PIC_STAGE_FIELD = 6, PIC_STAGE_COUNT = 7
+};
+
+typedef struct {
+    int32_t type;            /* PIC_PATCH_* */
+    int32_t start, size;     /* face range, as in constant/polyMesh/boundary */
+    int32_t neighbour_patch; /* cyclic partner patch, -1 otherwise */
+    int32_t phi_bc;          /* PIC_BC_* for the Poisson solve */
+    double phi_value;        /* fixedValue */
+} pic_patch;
+
+/* OpenFOAM polyMesh: internal faces first (upper-triangular order), face normals out of
+ * the owner, face i of a cyclic patch matching face i of its partner. */
+typedef struct {
+    int32_t n_points, n_faces, n_internal_faces, n_cells, n_patches;
+    const double* points;        /* [n_points*3] */
+    const int32_t* face_offsets; /* [n_faces+1] */
+    const int32_t* face_points;  /* [face_offsets[n_faces]] */
+    const int32_t* owner;        /* [n_faces] */
+    const int32_t* neighbour;    /* [n_internal_faces] */
+    const pic_patch* patches;    /* [n_patches] */
+} pic_mesh;
+
+/* moleculeProperties.<species> (PICCloud.C:57-317) */
+typedef struct {
+    double mass;          /* kg */
+    double charge;        /* C, already multiplied by the charge modifier */
+    double n_equivalent;  /* nEquivalentParticles (weight) */
+    int32_t solve_movement;
+    int64_t capacity;     /* parcels held on the device for this species */
+} pic_species;
+
+/* constant/picProperties keywords; names as the reference's selection tables */
+typedef struct {
+    const char* particle_pusher;      /* ParticlePusher: Boris BorisNR Vay HigueraCary none */
+    const char* field_weighting;      /* SolverSettings.FieldWeigthing: CellAverage VolumeWeighting */
+    const char* charge_distribution;  /* SolverSettings.ChargeDistribution: CellAverage VolumeWeighting */
+    const char* maxwell_solver;       /* MaxwellSolver: ElectroStatic none */
+    const char* coulomb_collision;    /* CollisionModels.CoulombCollisionModel: Nanbu none */
+    const char* pairing_algorithm;    /* NanbuCoeffs.PairingAlgorithm: TakizukaAbe */
+    const char* weight_correction;    /* NanbuCoeffs.WeightCorrectionModel: none */
+    const char* electron_neutral;     /* CollisionModels.ElectronNeutralCollisionModel: NanbuElectronNeutral none */
+    const char* background_gas;       /* CollisionModels.BackgroundGasModel: uniformBackground none */
+    int32_t allow_intra_collision;    /* NanbuCoeffs.allowIntraCollision */
+    int32_t calculate_bmin;           /* NanbuCoeffs.DebyeLength.calculate_bmin */
+    double coulomb_log[8][8];         /* NanbuCoeffs.Collisions.*.coulombLog; 0 = from the Debye length */
+    int32_t electron_species;         /* SpeciesRelations.electronTypeId */
+    int32_t ion_species;              /* ion created by background ionization, -1 none */
+    double bg_number_density, bg_temperature, bg_mass, bg_n_equivalent, bg_velocity[3];
+    int32_t create_ion, create_electron;
+    int32_t precision;                /* PIC_PREC_* */
+    int32_t hc_reference_form;        /* 1: Higuera-Cary with the reference's (t.u)^2 term */
+    uint64_t seed;
+    double poisson_tolerance;         /* ||r||2/||b||2 */
+    int32_t poisson_max_iter;
+    int32_t poisson_mode;             /* PIC_POISSON_* */
+    int32_t sort_interval;            /* sort every k steps when no collision model needs it */
+} pic_models;
+
+typedef struct {
+    int32_t iterations;        /* PCG iterations (inner, summed over refinements) */
+    int32_t refinements;       /* outer FP64 refinements (mixed mode) */
+    double rel_residual;       /* final ||b - A phi||2 / ||b||2 in FP64 */
+    double ms;                 /* solve time on the device */
+} pic_poisson_stats;
+
+typedef struct {
+    int64_t n_parcels[8];
+    int64_t n_deleted;          /* left through non-coupled patches this step */
+    int64_t n_track_guard;      /* hit the tracking iteration guard */
+    int64_t coulomb_collisions;
+    int64_t mcc_candidates, mcc_elastic, mcc_excitation, mcc_ionization;
+    pic_poisson_stats poisson;
+} pic_step_stats;
+
+typedef struct {
+    /* TemperatureInfo.C:72-138 / KineticEnergyInfo.C:69-82 sums per species */
+    double sum_w[8], sum_wU[8][3], sum_wU2[8], kinetic_energy[8];
+    double field_energy;        /* FieldEnergyInfo.C:118-124: 0.5 eps0 sum |E|^2 V */
+    int64_t n_parcels[8];
+} pic_diag;
+
+const char* pic_version(void);
+int pic_device_count(int32_t* n);
+
+/* PICCloud::PICCloud + setModels (PICCloud.C:734-1000, :378-518) */
+int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_species, const pic_models* models,
+               pic_ctx** out);
+void pic_destroy(pic_ctx* ctx);
+const char* pic_last_error(const pic_ctx* ctx);
+
+/* mesh-derived tables (the tet decomposition every parcel refers to) */
+int pic_mesh_sizes(const pic_ctx* ctx, int64_t* n_tets, int32_t* n_cells, int32_t* n_points);
+int pic_mesh_tets(const pic_ctx* ctx, int32_t* tet_cell, int32_t* tet_face, int32_t* tet_pt);
+int pic_mesh_geometry(const pic_ctx* ctx, double* cell_centres, double* cell_volumes);
+
+/* fields */
+int pic_set_cell_field(pic_ctx* ctx, int32_t field, const double* values);
+int pic_get_cell_field(pic_ctx* ctx, int32_t field, double* values);
+/* ListCrossSection energies [eV] / values [m^2] (ListCrossSection.C:142-323) */
+int pic_set_cross_section(pic_ctx* ctx, int32_t process, int32_t n, const double* energies_ev,
+                          const double* sigma_m2, double threshold_ev);
+/* ElectronNeutralCollisionModel::initialize "FindMaximum" (ElectronNeutralCollisionModel.C:375-420) */
+int pic_mcc_initialize(pic_ctx* ctx, double maximum_energy_ev, double energy_step_ev);
+
+/* particles: barycentric y1..y3 w.r.t. (cell centre, base, A, B) of tet, velocity U, id */
+int pic_particles_set(pic_ctx* ctx, int32_t species, int64_t n, const double* bary, const int32_t* tet,
+                      const double* U, const uint32_t* id);
+/* positions + containing cell -> (tet, bary) located on the device */
+int pic_particles_set_positions(pic_ctx* ctx, int32_t species, int64_t n, const double* x, const int32_t* cell,
+                                const double* U, const uint32_t* id);
+int pic_particles_count(pic_ctx* ctx, int32_t species, int64_t* n);
+int pic_particles_get(pic_ctx* ctx, int32_t species, int64_t capacity, int64_t* n, double* bary, int32_t* tet,
+                      double* U, uint32_t* id);
+int pic_particles_positions(pic_ctx* ctx, int32_t species, int64_t capacity, double* x);
+/* picInitialise Uniform/Maxwellian: nppc parcels per cell, uniform in volume, Maxwellian
+ * at temperature [K] plus drift; ids start at id0 */
+int pic_init_maxwellian(pic_ctx* ctx, int32_t species, int32_t nppc, double temperature, const double* drift,
+                        uint32_t id0, uint64_t seed);
+
+/* stages of PICCloud::evolve (PICCloud.C:1426-1471) */
+int pic_push(pic_ctx* ctx, double dt);                 /* ParticlePusher::updateVelocity only */
+int pic_move(pic_ctx* ctx, double dt);                 /* Cloud::move: push + interpolate + track */
+int pic_sort(pic_ctx* ctx);                            /* buildCellOccupancy */
+int pic_collide_coulomb(pic_ctx* ctx, double dt);      /* CoulombCollisionModel::handleCollisions */
+int pic_collide_mcc(pic_ctx* ctx, double dt);          /* ElectronNeutralCollisionModel::handleCollisions */
+int pic_deposit(pic_ctx* ctx);                         /* calculateFields: charge density */
+int pic_poisson_solve(pic_ctx* ctx, pic_poisson_stats* stats); /* MaxwellSolver::solveFields */
+int pic_diag_reduce(pic_ctx* ctx, pic_diag* out);      /* DiagnosticsList::info sums */
+int pic_step(pic_ctx* ctx, double dt, pic_step_stats* stats);  /* one PICCloud::evolve() */
+int pic_sync(pic_ctx* ctx);
+int64_t pic_step_index(const pic_ctx* ctx);
+int pic_set_step_index(pic_ctx* ctx, int64_t step);
+
+/* device timing of the stages run since the last reset (HIP events on the context stream) */
+int pic_profile_enable(pic_ctx* ctx, int32_t on);
+int pic_stage_times(pic_ctx* ctx, double* ms /*[PIC_STAGE_COUNT]*/, int64_t* launches /*[PIC_STAGE_COUNT]*/);
+int pic_profile_reset(pic_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

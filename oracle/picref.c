@@ -1,0 +1,1389 @@
+/*
+ * picref.c — FP64 CPU oracle for picFoam's per-timestep particle loop.
+ *
+ * TEST INFRASTRUCTURE ONLY (see picref.h).  Plain C, compiled with -ffp-contract=off so
+ * that every product and sum is rounded as written.
+ *
+ * Parity status per function (the tests that pin each one are named in tests/):
+ *   geometry, tets, locate, positions ............ pinned: closed-form volumes/centres,
+ *                                                    barycentric round trips
+ *   pushers ...................................... pinned: |u| conservation, E-only kick,
+ *                                                    gyro period/radius, ExB drift, Boris=HC=Vay (v<<c)
+ *   tracking ..................................... pinned: straight-line endpoint x+U*dt with
+ *                                                    periodic wrap, brute-force point location
+ *   interpolation / deposit ...................... pinned: linear field reproduced exactly,
+ *                                                    total charge, hand-computed single particles
+ *   cell->point interpolation .................... pinned: constant/linear fields on interior points
+ *   Poisson PCG/DIC + grad ....................... pinned: dense solve, manufactured sin(kx)
+ *   Coulomb (Nanbu + TA pairing) ................. pinned: momentum/energy conservation per pair,
+ *                                                    <cos chi> = exp(-s)
+ *   MCC (null collision) ......................... pinned: collision frequency n*sigma*v,
+ *                                                    candidate count, elastic energy loss 2m/M
+ *   Philox4x32-10 ................................ pinned: Random123 known-answer vectors
+ *   OpenFOAM-9 Random/shuffle streams, tracking tie-breaks, volPointInterpolation at
+ *   non-coupled boundary points ................... parity unpinned (not reproducible offline)
+ */
+#include "picref.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define C_LIGHT 299792458.0
+#define EPS0 8.8541878128e-12
+#define QE 1.602176634e-19
+#define KB 1.380649e-23
+#define HPLANCK 6.62607015e-34
+#define PI_D 3.14159265358979323846
+#define MAX_TRACK_STEPS 1000
+
+typedef struct { double x, y, z; } v3;
+static inline v3 V(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vscale(double s, v3 a) { return V(s * a.x, s * a.y, s * a.z); }
+static inline v3 vdiv(v3 a, double s) { return V(a.x / s, a.y / s, a.z / s); }
+static inline double vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 vcross(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline double vmag(v3 a) { return sqrt(vdot(a, a)); }
+static inline v3 ld3(const double* p) { return V(p[0], p[1], p[2]); }
+static inline void st3(double* p, v3 a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
+
+struct ref_mesh {
+    int n_points, n_faces, n_internal, n_cells, n_patches;
+    double* points;
+    int *fo, *fp, *owner, *nbr;
+    int *ptype, *pstart, *psize, *pnbr;
+    double *Cf, *Sf, *C, *V;
+    int *cfo, *cf;               /* cell -> faces, OpenFOAM primitiveMesh::calcCells order */
+    int n_tets;
+    int *tet_cell, *tet_face, *tet_pt, *cell_tet0;
+    int* pmaster;                /* cyclic-coupled point groups -> smallest point id */
+    int* face_patch;             /* boundary face -> patch index (-1 for internal) */
+    double* psep;                /* per patch: x_partner = x + sep */
+    /* VolumeWeighting (ChargeDistribution) weights, per cell-point (ascending point id) */
+    int *cpo, *cp;
+    double* vw_w;
+    /* volPointInterpolation inverse-distance weights, same cell-point layout */
+    double* ip_w;
+};
+
+/* ------------------------------------------------------------------ geometry */
+
+/* OpenFOAM-9 primitiveMeshFaceCentresAndAreas: triangles direct, polygons by a fan around
+ * the point average with |n|-weighted triangle centroids. */
+static void face_geometry(ref_mesh* m) {
+    for (int f = 0; f < m->n_faces; ++f) {
+        int s = m->fo[f], n = m->fo[f + 1] - s;
+        const int* fp = m->fp + s;
+        if (n == 3) {
+            v3 p0 = ld3(m->points + 3 * fp[0]), p1 = ld3(m->points + 3 * fp[1]), p2 = ld3(m->points + 3 * fp[2]);
+            st3(m->Cf + 3 * f, vscale(1.0 / 3.0, vadd(vadd(p0, p1), p2)));
+            st3(m->Sf + 3 * f, vscale(0.5, vcross(vsub(p1, p0), vsub(p2, p0))));
+        } else {
+            v3 est = V(0, 0, 0);
+            for (int i = 0; i < n; ++i) est = vadd(est, ld3(m->points + 3 * fp[i]));
+            est = vdiv(est, (double)n);
+            v3 sumN = V(0, 0, 0), sumAc = V(0, 0, 0);
+            double sumA = 0.0;
+            for (int i = 0; i < n; ++i) {
+                v3 p = ld3(m->points + 3 * fp[i]), q = ld3(m->points + 3 * fp[(i + 1) % n]);
+                v3 c = vadd(vadd(p, q), est);
+                v3 nn = vcross(vsub(q, p), vsub(est, p));
+                double a = vmag(nn);
+                sumN = vadd(sumN, nn);
+                sumA += a;
+                sumAc = vadd(sumAc, vscale(a, c));
+            }
+            if (sumA < 1e-300) {
+                st3(m->Cf + 3 * f, est);
+                st3(m->Sf + 3 * f, V(0, 0, 0));
+            } else {
+                st3(m->Cf + 3 * f, vscale(1.0 / 3.0, vdiv(sumAc, sumA)));
+                st3(m->Sf + 3 * f, vscale(0.5, sumN));
+            }
+        }
+    }
+}
+
+/* OpenFOAM-9 primitiveMeshCellCentresAndVols: pyramid decomposition about the face-centre
+ * average. */
+static void cell_geometry(ref_mesh* m) {
+    int nc = m->n_cells;
+    v3* est = (v3*)calloc(nc, sizeof(v3));
+    int* nf = (int*)calloc(nc, sizeof(int));
+    for (int f = 0; f < m->n_faces; ++f) {
+        est[m->owner[f]] = vadd(est[m->owner[f]], ld3(m->Cf + 3 * f));
+        nf[m->owner[f]]++;
+    }
+    for (int f = 0; f < m->n_internal; ++f) {
+        est[m->nbr[f]] = vadd(est[m->nbr[f]], ld3(m->Cf + 3 * f));
+        nf[m->nbr[f]]++;
+    }
+    for (int c = 0; c < nc; ++c) est[c] = vdiv(est[c], (double)nf[c]);
+    v3* ctr = (v3*)calloc(nc, sizeof(v3));
+    double* vol = (double*)calloc(nc, sizeof(double));
+    for (int f = 0; f < m->n_faces; ++f) {
+        int o = m->owner[f];
+        v3 cf = ld3(m->Cf + 3 * f), sf = ld3(m->Sf + 3 * f);
+        double pv = vdot(sf, vsub(cf, est[o]));
+        if (pv < 1e-300) pv = 1e-300;
+        v3 pc = vadd(vscale(0.75, cf), vscale(0.25, est[o]));
+        ctr[o] = vadd(ctr[o], vscale(pv, pc));
+        vol[o] += pv;
+    }
+    for (int f = 0; f < m->n_internal; ++f) {
+        int o = m->nbr[f];
+        v3 cf = ld3(m->Cf + 3 * f), sf = ld3(m->Sf + 3 * f);
+        double pv = vdot(sf, vsub(est[o], cf));
+        if (pv < 1e-300) pv = 1e-300;
+        v3 pc = vadd(vscale(0.75, cf), vscale(0.25, est[o]));
+        ctr[o] = vadd(ctr[o], vscale(pv, pc));
+        vol[o] += pv;
+    }
+    for (int c = 0; c < nc; ++c) {
+        st3(m->C + 3 * c, vdiv(ctr[c], vol[c]));
+        m->V[c] = vol[c] / 3.0;
+    }
+    free(est); free(nf); free(ctr); free(vol);
+}
+
+static int uf_find(int* p, int a) {
+    while (p[a] != a) { p[a] = p[p[a]]; a = p[a]; }
+    return a;
+}
+
+ref_mesh* ref_mesh_create(int n_points, const double* points, int n_faces, const int* face_offsets,
+                          const int* face_points, const int* owner, int n_internal, const int* neighbour,
+                          int n_cells, int n_patches, const int* patch_type, const int* patch_start,
+                          const int* patch_size, const int* patch_nbr) {
+    ref_mesh* m = (ref_mesh*)calloc(1, sizeof(ref_mesh));
+    m->n_points = n_points; m->n_faces = n_faces; m->n_internal = n_internal; m->n_cells = n_cells;
+    m->n_patches = n_patches;
+    m->points = (double*)malloc(sizeof(double) * 3 * n_points);
+    memcpy(m->points, points, sizeof(double) * 3 * n_points);
+    m->fo = (int*)malloc(sizeof(int) * (n_faces + 1));
+    memcpy(m->fo, face_offsets, sizeof(int) * (n_faces + 1));
+    m->fp = (int*)malloc(sizeof(int) * face_offsets[n_faces]);
+    memcpy(m->fp, face_points, sizeof(int) * face_offsets[n_faces]);
+    m->owner = (int*)malloc(sizeof(int) * n_faces);
+    memcpy(m->owner, owner, sizeof(int) * n_faces);
+    m->nbr = (int*)malloc(sizeof(int) * (n_internal > 0 ? n_internal : 1));
+    if (n_internal) memcpy(m->nbr, neighbour, sizeof(int) * n_internal);
+    m->ptype = (int*)malloc(sizeof(int) * (n_patches + 1));
+    m->pstart = (int*)malloc(sizeof(int) * (n_patches + 1));
+    m->psize = (int*)malloc(sizeof(int) * (n_patches + 1));
+    m->pnbr = (int*)malloc(sizeof(int) * (n_patches + 1));
+    memcpy(m->ptype, patch_type, sizeof(int) * n_patches);
+    memcpy(m->pstart, patch_start, sizeof(int) * n_patches);
+    memcpy(m->psize, patch_size, sizeof(int) * n_patches);
+    memcpy(m->pnbr, patch_nbr, sizeof(int) * n_patches);
+    m->Cf = (double*)calloc(3 * n_faces, sizeof(double));
+    m->Sf = (double*)calloc(3 * n_faces, sizeof(double));
+    m->C = (double*)calloc(3 * n_cells, sizeof(double));
+    m->V = (double*)calloc(n_cells, sizeof(double));
+    face_geometry(m);
+    cell_geometry(m);
+
+    /* cells(): owned faces in face order, then neighboured faces in face order */
+    int* cnt = (int*)calloc(n_cells + 1, sizeof(int));
+    for (int f = 0; f < n_faces; ++f) cnt[owner[f]]++;
+    for (int f = 0; f < n_internal; ++f) cnt[neighbour[f]]++;
+    m->cfo = (int*)malloc(sizeof(int) * (n_cells + 1));
+    m->cfo[0] = 0;
+    for (int c = 0; c < n_cells; ++c) m->cfo[c + 1] = m->cfo[c] + cnt[c];
+    m->cf = (int*)malloc(sizeof(int) * m->cfo[n_cells]);
+    memset(cnt, 0, sizeof(int) * (n_cells + 1));
+    for (int f = 0; f < n_faces; ++f) { int c = owner[f]; m->cf[m->cfo[c] + cnt[c]++] = f; }
+    for (int f = 0; f < n_internal; ++f) { int c = neighbour[f]; m->cf[m->cfo[c] + cnt[c]++] = f; }
+    free(cnt);
+
+    /* tets: (cell, face, pt), pt = 1..n-2 (face base point 0) */
+    int nt = 0;
+    m->cell_tet0 = (int*)malloc(sizeof(int) * (n_cells + 1));
+    for (int c = 0; c < n_cells; ++c) {
+        m->cell_tet0[c] = nt;
+        for (int k = m->cfo[c]; k < m->cfo[c + 1]; ++k) { int f = m->cf[k]; nt += m->fo[f + 1] - m->fo[f] - 2; }
+    }
+    m->cell_tet0[n_cells] = nt;
+    m->n_tets = nt;
+    m->tet_cell = (int*)malloc(sizeof(int) * nt);
+    m->tet_face = (int*)malloc(sizeof(int) * nt);
+    m->tet_pt = (int*)malloc(sizeof(int) * nt);
+    nt = 0;
+    for (int c = 0; c < n_cells; ++c)
+        for (int k = m->cfo[c]; k < m->cfo[c + 1]; ++k) {
+            int f = m->cf[k], n = m->fo[f + 1] - m->fo[f];
+            for (int pt = 1; pt <= n - 2; ++pt) { m->tet_cell[nt] = c; m->tet_face[nt] = f; m->tet_pt[nt] = pt; ++nt; }
+        }
+
+    /* boundary face -> patch, cyclic separations, coupled point groups */
+    m->face_patch = (int*)malloc(sizeof(int) * n_faces);
+    for (int f = 0; f < n_faces; ++f) m->face_patch[f] = -1;
+    m->psep = (double*)calloc(3 * (n_patches + 1), sizeof(double));
+    m->pmaster = (int*)malloc(sizeof(int) * n_points);
+    for (int p = 0; p < n_points; ++p) m->pmaster[p] = p;
+    for (int pi = 0; pi < n_patches; ++pi) {
+        for (int f = m->pstart[pi]; f < m->pstart[pi] + m->psize[pi]; ++f) m->face_patch[f] = pi;
+        if (m->ptype[pi] == REF_PATCH_CYCLIC && m->psize[pi] > 0) {
+            int q = m->pnbr[pi];
+            int f0 = m->pstart[pi], g0 = m->pstart[q];
+            st3(m->psep + 3 * pi, vsub(ld3(m->Cf + 3 * g0), ld3(m->Cf + 3 * f0)));
+            for (int k = 0; k < m->psize[pi]; ++k) {
+                int f = f0 + k, g = g0 + k, n = m->fo[f + 1] - m->fo[f];
+                /* cyclic ordering: g[0] ~ f[0], g[j] ~ f[n-j]; confirmed by position */
+                for (int j = 0; j < n; ++j) {
+                    int a = m->fp[m->fo[f] + j];
+                    v3 img = vadd(ld3(m->points + 3 * a), ld3(m->psep + 3 * pi));
+                    int best = -1; double bd = 1e300;
+                    for (int i = 0; i < n; ++i) {
+                        int b = m->fp[m->fo[g] + i];
+                        double d = vmag(vsub(ld3(m->points + 3 * b), img));
+                        if (d < bd) { bd = d; best = b; }
+                    }
+                    int ra = uf_find(m->pmaster, a), rb = uf_find(m->pmaster, best);
+                    if (ra != rb) { if (ra < rb) m->pmaster[rb] = ra; else m->pmaster[ra] = rb; }
+                }
+            }
+        }
+    }
+    for (int p = 0; p < n_points; ++p) m->pmaster[p] = uf_find(m->pmaster, p);
+
+    /* cellPoints (ascending point id) + weights */
+    m->cpo = (int*)malloc(sizeof(int) * (n_cells + 1));
+    int cap = 0;
+    for (int c = 0; c < n_cells; ++c)
+        for (int k = m->cfo[c]; k < m->cfo[c + 1]; ++k) { int f = m->cf[k]; cap += m->fo[f + 1] - m->fo[f]; }
+    m->cp = (int*)malloc(sizeof(int) * (cap + 1));
+    int ncp = 0;
+    int* tmp = (int*)malloc(sizeof(int) * 512);
+    for (int c = 0; c < n_cells; ++c) {
+        m->cpo[c] = ncp;
+        int nt2 = 0;
+        for (int k = m->cfo[c]; k < m->cfo[c + 1]; ++k) {
+            int f = m->cf[k];
+            for (int j = m->fo[f]; j < m->fo[f + 1]; ++j) {
+                int p = m->fp[j], dup = 0;
+                for (int i = 0; i < nt2; ++i) if (tmp[i] == p) { dup = 1; break; }
+                if (!dup) tmp[nt2++] = p;
+            }
+        }
+        /* insertion sort ascending */
+        for (int i = 1; i < nt2; ++i) { int v = tmp[i], j = i - 1; while (j >= 0 && tmp[j] > v) { tmp[j + 1] = tmp[j]; --j; } tmp[j + 1] = v; }
+        for (int i = 0; i < nt2; ++i) m->cp[ncp++] = tmp[i];
+    }
+    m->cpo[n_cells] = ncp;
+    free(tmp);
+    m->vw_w = (double*)malloc(sizeof(double) * (ncp + 1));
+    m->ip_w = (double*)malloc(sizeof(double) * (ncp + 1));
+    double* sumd = (double*)calloc(n_points, sizeof(double));
+    double* sumi = (double*)calloc(n_points, sizeof(double));
+    for (int c = 0; c < n_cells; ++c)
+        for (int k = m->cpo[c]; k < m->cpo[c + 1]; ++k) {
+            int p = m->cp[k];
+            double d = vmag(vsub(ld3(m->points + 3 * p), ld3(m->C + 3 * c)));
+            m->vw_w[k] = d;          /* VolumeWeighting.C:66 distance weight */
+            m->ip_w[k] = 1.0 / d;    /* volPointInterpolation inverse distance */
+            sumd[m->pmaster[p]] += d;
+            sumi[m->pmaster[p]] += 1.0 / d;
+        }
+    for (int c = 0; c < n_cells; ++c)
+        for (int k = m->cpo[c]; k < m->cpo[c + 1]; ++k) {
+            int p = m->pmaster[m->cp[k]];
+            m->vw_w[k] /= sumd[p];
+            m->ip_w[k] /= sumi[p];
+        }
+    free(sumd); free(sumi);
+    return m;
+}
+
+void ref_mesh_destroy(ref_mesh* m) {
+    if (!m) return;
+    free(m->points); free(m->fo); free(m->fp); free(m->owner); free(m->nbr);
+    free(m->ptype); free(m->pstart); free(m->psize); free(m->pnbr);
+    free(m->Cf); free(m->Sf); free(m->C); free(m->V); free(m->cfo); free(m->cf);
+    free(m->tet_cell); free(m->tet_face); free(m->tet_pt); free(m->cell_tet0);
+    free(m->pmaster); free(m->face_patch); free(m->psep);
+    free(m->cpo); free(m->cp); free(m->vw_w); free(m->ip_w);
+    free(m);
+}
+
+int ref_mesh_n_tets(const ref_mesh* m) { return m->n_tets; }
+int ref_mesh_n_points(const ref_mesh* m) { return m->n_points; }
+
+void ref_mesh_geometry(const ref_mesh* m, double* cc, double* cv, double* fc, double* fa) {
+    if (cc) memcpy(cc, m->C, sizeof(double) * 3 * m->n_cells);
+    if (cv) memcpy(cv, m->V, sizeof(double) * m->n_cells);
+    if (fc) memcpy(fc, m->Cf, sizeof(double) * 3 * m->n_faces);
+    if (fa) memcpy(fa, m->Sf, sizeof(double) * 3 * m->n_faces);
+}
+
+void ref_mesh_tets(const ref_mesh* m, int* tc, int* tf, int* tp) {
+    memcpy(tc, m->tet_cell, sizeof(int) * m->n_tets);
+    memcpy(tf, m->tet_face, sizeof(int) * m->n_tets);
+    memcpy(tp, m->tet_pt, sizeof(int) * m->n_tets);
+}
+
+void ref_mesh_point_master(const ref_mesh* m, int* pm) { memcpy(pm, m->pmaster, sizeof(int) * m->n_points); }
+
+/* tetIndices::faceTriIs (OpenFOAM-9): (base, A, B) = (f[0], f[pt], f[pt+1]) for the owner,
+ * A and B swapped when the cell is the face's neighbour. */
+static void tet_points(const ref_mesh* m, int t, int vid[3]) {
+    int f = m->tet_face[t], pt = m->tet_pt[t], c = m->tet_cell[t];
+    const int* fp = m->fp + m->fo[f];
+    vid[0] = fp[0];
+    if (m->owner[f] == c) { vid[1] = fp[pt]; vid[2] = fp[pt + 1]; }
+    else { vid[1] = fp[pt + 1]; vid[2] = fp[pt]; }
+}
+
+static void tet_vertices(const ref_mesh* m, int t, v3 x[4], int vid[3]) {
+    tet_points(m, t, vid);
+    x[0] = ld3(m->C + 3 * m->tet_cell[t]);
+    for (int i = 0; i < 3; ++i) x[i + 1] = ld3(m->points + 3 * vid[i]);
+}
+
+void ref_positions(const ref_mesh* m, int n, const double* y, const int* tet, double* x) {
+    for (int i = 0; i < n; ++i) {
+        if (tet[i] < 0) { x[3 * i] = x[3 * i + 1] = x[3 * i + 2] = NAN; continue; }
+        v3 X[4]; int vid[3];
+        tet_vertices(m, tet[i], X, vid);
+        double y1 = y[3 * i], y2 = y[3 * i + 1], y3 = y[3 * i + 2];
+        double y0 = 1.0 - (y1 + y2 + y3);
+        v3 p = vadd(vadd(vscale(y0, X[0]), vscale(y1, X[1])), vadd(vscale(y2, X[2]), vscale(y3, X[3])));
+        st3(x + 3 * i, p);
+    }
+}
+
+/* barycentric coordinates of p in tet X: y_i = r_i.(p - x0)/det, r1 = e2 x e3, ... */
+static void bary_of(const v3 X[4], v3 p, double y[4]) {
+    v3 e1 = vsub(X[1], X[0]), e2 = vsub(X[2], X[0]), e3 = vsub(X[3], X[0]);
+    v3 r1 = vcross(e2, e3), r2 = vcross(e3, e1), r3 = vcross(e1, e2);
+    double det = vdot(e1, r1);
+    v3 d = vsub(p, X[0]);
+    y[1] = vdot(r1, d) / det; y[2] = vdot(r2, d) / det; y[3] = vdot(r3, d) / det;
+    y[0] = 1.0 - (y[1] + y[2] + y[3]);
+}
+
+int ref_locate(const ref_mesh* m, int n, const double* x, const int* cell, double* y, int* tet) {
+    int fails = 0;
+    for (int i = 0; i < n; ++i) {
+        int c = cell[i], best = -1;
+        double bestmin = -1e300, by[4] = {0, 0, 0, 0};
+        v3 p = ld3(x + 3 * i);
+        for (int t = m->cell_tet0[c]; t < m->cell_tet0[c + 1]; ++t) {
+            v3 X[4]; int vid[3]; double yy[4];
+            tet_vertices(m, t, X, vid);
+            bary_of(X, p, yy);
+            double mn = yy[0];
+            for (int k = 1; k < 4; ++k) if (yy[k] < mn) mn = yy[k];
+            if (mn > bestmin) { bestmin = mn; best = t; memcpy(by, yy, sizeof by); }
+        }
+        if (bestmin < -1e-9) ++fails;
+        tet[i] = best;
+        y[3 * i] = by[1]; y[3 * i + 1] = by[2]; y[3 * i + 2] = by[3];
+    }
+    return fails;
+}
+
+/* ------------------------------------------------------------------ pushers */
+
+/* BorisPusher.C:69-112 (relativistic; rotation Lorentz factor = mean of the old and the
+ * half-kicked gamma, :98), BorisNRPusher.C:69-99, VayPusher.C:54-100, HigueraCary.C:69-115. */
+void ref_push_one(int pusher, double* Up, const double* Ep, const double* Bp, double q, double mass, double dt,
+                  int hc_reference_form, int rotate) {
+    v3 U = ld3(Up), E = ld3(Ep), B = ld3(Bp);
+    const double c = C_LIGHT;
+    double dt2 = 0.5 * dt;
+    if (pusher == REF_PUSH_NONE) return;
+    if (pusher == REF_PUSH_BORIS) {
+        double mu = vmag(U) / c;
+        v3 rU = vdiv(U, sqrt(1.0 - mu * mu));
+        v3 oldrU = rU;
+        double k = q * dt2 / mass;
+        v3 rUsm = vscale(k, E);
+        rU = vadd(rU, rUsm);
+        if (rotate) {
+            double a = vmag(oldrU) / c, b = vmag(rU) / c;
+            double lF = (sqrt(1.0 + a * a) + sqrt(1.0 + b * b)) / 2.0;
+            v3 t = vdiv(vscale(k, B), lF);
+            v3 Ud = vadd(rU, vcross(rU, t));
+            rU = vadd(rU, vcross(Ud, vdiv(vscale(2.0, t), 1.0 + vdot(t, t))));
+        }
+        rU = vadd(rU, rUsm);
+        double g = vmag(rU) / c;
+        U = vdiv(rU, sqrt(1.0 + g * g));
+    } else if (pusher == REF_PUSH_BORISNR) {
+        double alpha = q * dt2 / mass;
+        U = vadd(U, vscale(alpha, E));
+        if (rotate) {
+            v3 t = vscale(alpha, B);
+            v3 Ud = vadd(U, vcross(U, t));
+            U = vadd(U, vcross(Ud, vdiv(vscale(2.0, t), 1.0 + vdot(t, t))));
+        }
+        U = vadd(U, vscale(alpha, E));
+    } else if (pusher == REF_PUSH_VAY) {
+        double mu = vmag(U) / c;
+        double lF = 1.0 / sqrt(1.0 - mu * mu);
+        v3 rU = vscale(lF, U);
+        v3 Ud = vadd(rU, vscale(q * dt / mass, E));
+        v3 T = vscale(q * dt2 / mass, B);
+        Ud = vadd(Ud, vcross(U, T));
+        double ud = vmag(Ud) / c;
+        double alpha = 1.0 + ud * ud;
+        double T2 = vdot(T, T);
+        double s = alpha - T2;
+        double w = vdot(Ud, T) / c;
+        double us2 = w * w;
+        alpha = 1.0 / sqrt(0.5 * (s + sqrt(s * s + 4.0 * (T2 + us2))));
+        T = vscale(alpha, T);
+        s = 1.0 / (1.0 + vdot(T, T));
+        alpha = vdot(Ud, T);
+        rU = vscale(s, vadd(vadd(Ud, vscale(alpha, T)), vcross(Ud, T)));
+        double g = vmag(rU) / c;
+        U = vdiv(rU, sqrt(1.0 + g * g));
+    } else if (pusher == REF_PUSH_HC) {
+        double mu = vmag(U) / c;
+        v3 rU = vdiv(U, sqrt(1.0 - mu * mu));
+        double k = q * dt2 / mass;
+        v3 rUsm = vscale(k, E);
+        rU = vadd(rU, rUsm);
+        if (rotate) {
+            double g = vmag(rU) / c;
+            double gfm2 = 1.0 + g * g;
+            v3 t = vscale(k, B);
+            double beta2 = vdot(t, t);
+            /* published Higuera-Cary form uses (t.u/c)^2; the reference writes (t.u)^2 */
+            double tu = hc_reference_form ? vdot(t, rU) : vdot(t, rU) / c;
+            double sg = gfm2 - beta2;
+            double iG = 1.0 / sqrt(0.5 * (sg + sqrt(sg * sg + 4.0 * (beta2 + tu * tu))));
+            t = vscale(iG, t);
+            v3 Ud = vadd(rU, vcross(rU, t));
+            rU = vadd(rU, vcross(Ud, vdiv(vscale(2.0, t), 1.0 + vdot(t, t))));
+        }
+        rU = vadd(rU, rUsm);
+        double g = vmag(rU) / c;
+        U = vdiv(rU, sqrt(1.0 + g * g));
+    }
+    st3(Up, U);
+}
+
+/* CellAverageFW.C:58-65 / VolumeWeightingFW.C:60-67 (interpolationCellPoint): cell value
+ * weighted by y0 plus the face-triangle point values weighted by y1..y3. */
+static v3 interp_one(const ref_mesh* m, int weighting, int t, const double* yy, const double* Ecell,
+                     const double* Epoint) {
+    int c = m->tet_cell[t];
+    if (weighting == REF_W_CELLAVERAGE) return ld3(Ecell + 3 * c);
+    int vid[3];
+    tet_points(m, t, vid);
+    double y1 = yy[0], y2 = yy[1], y3 = yy[2];
+    double y0 = 1.0 - (y1 + y2 + y3);
+    v3 e = vscale(y0, ld3(Ecell + 3 * c));
+    e = vadd(e, vscale(y1, ld3(Epoint + 3 * m->pmaster[vid[0]])));
+    e = vadd(e, vscale(y2, ld3(Epoint + 3 * m->pmaster[vid[1]])));
+    e = vadd(e, vscale(y3, ld3(Epoint + 3 * m->pmaster[vid[2]])));
+    return e;
+}
+
+void ref_interp_E(const ref_mesh* m, int weighting, int n, const double* y, const int* tet, const double* Ecell,
+                  const double* Epoint, double* Ep) {
+    for (int i = 0; i < n; ++i) st3(Ep + 3 * i, interp_one(m, weighting, tet[i], y + 3 * i, Ecell, Epoint));
+}
+
+void ref_push(const ref_mesh* m, int pusher, int weighting, int n, const double* y, double* U, const int* tet,
+              double q, double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell,
+              int hc_reference_form, int rotate) {
+    for (int i = 0; i < n; ++i) {
+        if (tet[i] < 0) continue;
+        v3 E = interp_one(m, weighting, tet[i], y + 3 * i, Ecell, Epoint);
+        double e[3] = {E.x, E.y, E.z};
+        ref_push_one(pusher, U + 3 * i, e, Bcell + 3 * m->tet_cell[tet[i]], q, mass, dt, hc_reference_form, rotate);
+    }
+}
+
+/* ------------------------------------------------------------------ tracking */
+
+/* Find the tet of cell c (other than t) whose point set contains points P and Q. */
+static int find_tet_with_edge(const ref_mesh* m, int c, int t, int P, int Q, int vid_out[3]) {
+    for (int s = m->cell_tet0[c]; s < m->cell_tet0[c + 1]; ++s) {
+        if (s == t) continue;
+        int v[3];
+        tet_points(m, s, v);
+        int hp = (v[0] == P || v[1] == P || v[2] == P), hq = (v[0] == Q || v[1] == Q || v[2] == Q);
+        if (hp && hq) { memcpy(vid_out, v, sizeof v); return s; }
+    }
+    return -1;
+}
+
+/* Cross tet face iH (opposite vertex iH) of tet t; y[4] is remapped in place.
+ * Returns the new tet, or -1 - patch for a particle leaving through a non-coupled patch. */
+static int cross_face(const ref_mesh* m, int t, int iH, double y[4]) {
+    int c = m->tet_cell[t];
+    int vid[3];
+    tet_points(m, t, vid);
+    if (iH != 0) {
+        /* internal tet face (cc, P, Q): the two face points other than vertex iH */
+        int P = -1, Q = -1, k = 0;
+        int idx[2];
+        for (int i = 1; i <= 3; ++i) if (i != iH) { idx[k] = i; if (k == 0) P = vid[i - 1]; else Q = vid[i - 1]; ++k; }
+        int nv[3];
+        int s = find_tet_with_edge(m, c, t, P, Q, nv);
+        if (s < 0) return -1000000;
+        double ny[4] = {y[0], 0.0, 0.0, 0.0};
+        for (int i = 0; i < 3; ++i) {
+            if (nv[i] == P) ny[i + 1] = y[idx[0]];
+            else if (nv[i] == Q) ny[i + 1] = y[idx[1]];
+        }
+        memcpy(y, ny, sizeof ny);
+        return s;
+    }
+    int f = m->tet_face[t], pt = m->tet_pt[t];
+    if (f < m->n_internal) {
+        int c2 = (m->owner[f] == c) ? m->nbr[f] : m->owner[f];
+        for (int s = m->cell_tet0[c2]; s < m->cell_tet0[c2 + 1]; ++s)
+            if (m->tet_face[s] == f && m->tet_pt[s] == pt) {
+                int nv[3];
+                tet_points(m, s, nv);
+                double ny[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) if (nv[j] == vid[i]) ny[j + 1] = y[i + 1];
+                memcpy(y, ny, sizeof ny);
+                return s;
+            }
+        return -1000000;
+    }
+    int pi = m->face_patch[f];
+    if (m->ptype[pi] != REF_PATCH_CYCLIC) return -1 - pi;
+    int q = m->pnbr[pi];
+    int g = m->pstart[q] + (f - m->pstart[pi]);
+    int c2 = m->owner[g];
+    v3 sep = ld3(m->psep + 3 * pi);
+    v3 img[3];
+    for (int i = 0; i < 3; ++i) img[i] = vadd(ld3(m->points + 3 * vid[i]), sep);
+    double scale = 0.0;
+    for (int i = 0; i < 3; ++i) scale += vmag(vsub(img[i], img[(i + 1) % 3]));
+    for (int s = m->cell_tet0[c2]; s < m->cell_tet0[c2 + 1]; ++s) {
+        if (m->tet_face[s] != g) continue;
+        int nv[3];
+        tet_points(m, s, nv);
+        int map[3] = {-1, -1, -1}, ok = 1;
+        for (int i = 0; i < 3 && ok; ++i) {
+            for (int j = 0; j < 3; ++j)
+                if (vmag(vsub(ld3(m->points + 3 * nv[j]), img[i])) < 1e-9 * scale) { map[i] = j; break; }
+            if (map[i] < 0) ok = 0;
+        }
+        if (!ok) continue;
+        double ny[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < 3; ++i) ny[map[i] + 1] = y[i + 1];
+        memcpy(y, ny, sizeof ny);
+        return s;
+    }
+    return -1000000;
+}
+
+/* PICParcel::moveForward (PICParcel.C:81-144) with OpenFOAM-9 barycentric tracking
+ * (particle::trackToFace / trackToStationaryTet re-derived): per tet, the displacement
+ * (1-f)*dt*U is taken to barycentric form Tx_i = r_i.d (det-scaled); the first face with
+ * Tx_i < 0 and 0 <= -y_i/Tx_i < 1/det is hit. */
+static int track_one(const ref_mesh* m, double* yy, double* Up, int* tetp, double dt) {
+    int t = *tetp;
+    double y[4];
+    y[1] = yy[0]; y[2] = yy[1]; y[3] = yy[2];
+    y[0] = 1.0 - (y[1] + y[2] + y[3]);
+    v3 U = ld3(Up);
+    double sf = 0.0;
+    int guard = 0, status = 0;
+    while (sf < 1.0) {
+        double f = 1.0 - sf;
+        v3 d = vscale(f * dt, U);
+        v3 X[4]; int vid[3];
+        tet_vertices(m, t, X, vid);
+        v3 e1 = vsub(X[1], X[0]), e2 = vsub(X[2], X[0]), e3 = vsub(X[3], X[0]);
+        v3 r1 = vcross(e2, e3), r2 = vcross(e3, e1), r3 = vcross(e1, e2);
+        double det = vdot(e1, r1);
+        double T[4];
+        T[1] = vdot(r1, d); T[2] = vdot(r2, d); T[3] = vdot(r3, d);
+        T[0] = -(T[1] + T[2] + T[3]);
+        double muH = (det > 0.0 && isnormal(det)) ? 1.0 / det : 1e300;
+        int iH = -1;
+        for (int i = 0; i < 4; ++i)
+            if (T[i] < 0.0) {
+                double mu = -y[i] / T[i];
+                if (mu >= 0.0 && mu < muH) { muH = mu; iH = i; }
+            }
+        for (int i = 0; i < 4; ++i) y[i] = y[i] + muH * T[i];
+        if (iH < 0) { sf = 1.0; break; }
+        y[iH] = 0.0;
+        double frac = muH * det;
+        if (frac < 0.0) frac = 0.0;
+        if (frac > 1.0) frac = 1.0;
+        sf = sf + f * frac;
+        int s = cross_face(m, t, iH, y);
+        if (s <= -1000000) { status = 2; break; }
+        if (s < 0) { t = -1; break; } /* left through a non-coupled patch: deleted (NoInflow) */
+        t = s;
+        if (++guard > MAX_TRACK_STEPS) { status = 1; break; }
+    }
+    *tetp = t;
+    yy[0] = y[1]; yy[1] = y[2]; yy[2] = y[3];
+    return status;
+}
+
+int ref_move(const ref_mesh* m, int pusher, int weighting, int n, double* y, double* U, int* tet, double q,
+             double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell, int do_push,
+             int hc_reference_form, int rotate, int* n_superluminal) {
+    int guards = 0, sl = 0;
+    for (int i = 0; i < n; ++i) {
+        if (tet[i] < 0) continue;
+        if (do_push && q != 0.0) {
+            v3 E = interp_one(m, weighting, tet[i], y + 3 * i, Ecell, Epoint);
+            double e[3] = {E.x, E.y, E.z};
+            ref_push_one(pusher, U + 3 * i, e, Bcell + 3 * m->tet_cell[tet[i]], q, mass, dt, hc_reference_form,
+                         rotate);
+        }
+        if (vmag(ld3(U + 3 * i)) >= C_LIGHT) ++sl; /* PICParcel.C:61-66 */
+        if (track_one(m, y + 3 * i, U + 3 * i, tet + i, dt)) ++guards;
+    }
+    if (n_superluminal) *n_superluminal = sl;
+    return guards;
+}
+
+/* ------------------------------------------------------------------ deposit */
+
+/* CellAverage.C:60-63 (q*w into the cell) and VolumeWeighting.C:105-124 (y0 to the cell
+ * centre, y1..y3 to the face-triangle points), in int64 fixed point. */
+void ref_deposit(const ref_mesh* m, int weighting, int n, const double* y, const int* tet, double scale,
+                 int64_t* acc_cell, int64_t* acc_point) {
+    const double s = scale * 4398046511104.0; /* 2^42 */
+    const int64_t whole = llrint(s);
+    for (int i = 0; i < n; ++i) {
+        int t = tet[i];
+        if (t < 0) continue;
+        int c = m->tet_cell[t];
+        if (weighting == REF_W_CELLAVERAGE) { acc_cell[c] += whole; continue; }
+        int vid[3];
+        tet_points(m, t, vid);
+        double y1 = y[3 * i], y2 = y[3 * i + 1], y3 = y[3 * i + 2];
+        double y0 = 1.0 - (y1 + y2 + y3);
+        acc_cell[c] += llrint(y0 * s);
+        acc_point[m->pmaster[vid[0]]] += llrint(y1 * s);
+        acc_point[m->pmaster[vid[1]]] += llrint(y2 * s);
+        acc_point[m->pmaster[vid[2]]] += llrint(y3 * s);
+    }
+}
+
+/* CellAverage.C:66-68 / VolumeWeighting.C:127-164 */
+void ref_deposit_finish(const ref_mesh* m, int weighting, const int64_t* acc_cell, const int64_t* acc_point,
+                        double Q0, double* rho) {
+    const double q = Q0 / 4398046511104.0;
+    for (int c = 0; c < m->n_cells; ++c) {
+        double V = m->V[c];
+        double r = ((double)acc_cell[c] * q) / V;
+        if (weighting == REF_W_VOLUMEWEIGHTING)
+            for (int k = m->cpo[c]; k < m->cpo[c + 1]; ++k)
+                r += m->vw_w[k] * ((double)acc_point[m->pmaster[m->cp[k]]] * q) / V;
+        rho[c] = r;
+    }
+}
+
+void ref_cell_to_point(const ref_mesh* m, const double* Ecell, double* Epoint) {
+    memset(Epoint, 0, sizeof(double) * 3 * m->n_points);
+    for (int c = 0; c < m->n_cells; ++c)
+        for (int k = m->cpo[c]; k < m->cpo[c + 1]; ++k) {
+            int p = m->pmaster[m->cp[k]];
+            double w = m->ip_w[k];
+            for (int d = 0; d < 3; ++d) Epoint[3 * p + d] += w * Ecell[3 * c + d];
+        }
+    /* non-master points carry the value of their master */
+    for (int p = 0; p < m->n_points; ++p)
+        if (m->pmaster[p] != p) for (int d = 0; d < 3; ++d) Epoint[3 * p + d] = Epoint[3 * m->pmaster[p] + d];
+}
+
+/* ------------------------------------------------------------------ Philox4x32-10 */
+
+/* Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11. */
+void ref_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline double u01(uint32_t a, uint32_t b) {
+    uint64_t x = (((uint64_t)a << 32) | b) >> 11;
+    return (double)x * (1.0 / 9007199254740992.0);
+}
+
+static inline void rng4(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t out[4]) {
+    uint32_t ctr[4] = {a, b, c, d}, key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    ref_philox4x32(ctr, key, out);
+}
+
+/* stream tags (top byte of counter word 3) */
+#define TAG_SHUFFLE 1u
+#define TAG_COULOMB 2u
+#define TAG_MCC 3u
+#define TAG_NEWID 4u
+
+/* ------------------------------------------------------------------ sort */
+
+int ref_sort_by_cell(const ref_mesh* m, int n, const int* tet, int* order, int* cell_start) {
+    int nc = m->n_cells;
+    memset(cell_start, 0, sizeof(int) * (nc + 1));
+    for (int i = 0; i < n; ++i) if (tet[i] >= 0) cell_start[m->tet_cell[tet[i]] + 1]++;
+    for (int c = 0; c < nc; ++c) cell_start[c + 1] += cell_start[c];
+    int* cur = (int*)malloc(sizeof(int) * (nc + 1));
+    memcpy(cur, cell_start, sizeof(int) * (nc + 1));
+    for (int i = 0; i < n; ++i) if (tet[i] >= 0) order[cur[m->tet_cell[tet[i]]]++] = i;
+    int live = cell_start[nc];
+    free(cur);
+    return live;
+}
+
+/* ------------------------------------------------------------------ Coulomb */
+
+typedef struct { uint32_t key, id; int idx; } keyrec;
+static int keyrec_cmp(const void* a, const void* b) {
+    const keyrec* x = (const keyrec*)a; const keyrec* y = (const keyrec*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* shuffle (OpenFOAM Random::shuffle, TakizukaAbePairing.C:110) restated as a sort by a
+ * counter-based random key of (particle id, step, stage, species) */
+static void keyed_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
+                       int* perm) {
+    keyrec* r = (keyrec*)malloc(sizeof(keyrec) * (n > 0 ? n : 1));
+    for (int k = 0; k < n; ++k) {
+        uint32_t o[4];
+        rng4(seed, id[first + k], (uint32_t)step, (uint32_t)((stage << 8) | species), TAG_SHUFFLE << 24, o);
+        r[k].key = o[0]; r[k].id = id[first + k]; r[k].idx = first + k;
+    }
+    qsort(r, n, sizeof(keyrec), keyrec_cmp);
+    for (int k = 0; k < n; ++k) perm[k] = r[k].idx;
+    free(r);
+}
+
+/* NanbuCollision.C:137-159 */
+static double cos_chi(double s, double U) {
+    double A, invA;
+    if (s < 0.1) {
+        if (U < 0.0001) U = 0.0001;
+        return 1.0 + s * log(U);
+    }
+    if (s < 3.0) {
+        invA = 0.00569578 + (0.95602 + (-0.508139 + (0.479139 + (-0.12789 + 0.0238957 * s) * s) * s) * s) * s;
+        A = 1.0 / invA;
+        return invA * log(exp(-A) + 2. * U * sinh(A));
+    }
+    if (s < 6.0) {
+        A = 3.0 * exp(-s);
+        return (1.0 / A) * log(exp(-A) + 2.0 * U * sinh(A));
+    }
+    return 2.0 * U - 1.0;
+}
+
+/* NanbuCollision::collide, NanbuCollision.C:60-134 (weight correction "none") */
+static void nanbu_collide(double* UP, double* UQ, double mP, double qP, double mQ, double qQ, double lnL_fixed,
+                          int calc_bmin, double debye, double nP, double nQ, double nPQ, double dt, double u1,
+                          double u2, double* stats) {
+    double reMass = mP * mQ / (mP + mQ);
+    v3 uP = ld3(UP), uQ = ld3(UQ);
+    v3 g = vsub(uP, uQ);
+    double gm = vmag(g);
+    double gp = sqrt(g.y * g.y + g.z * g.z);
+    if (!(gm > 0.0) || !(gp > 0.0)) return; /* degenerate relative velocity: the reference divides by zero */
+    double lnL;
+    if (lnL_fixed == 0.0) {
+        double b0 = qP * qQ / (2.0 * PI_D * EPS0 * reMass * gm * gm);
+        if (calc_bmin) { double hb = HPLANCK / (2.0 * (gm * reMass)); b0 = hb > fabs(b0) ? hb : fabs(b0); }
+        double l = 0.5 * log(1.0 + (debye * debye) / (b0 * b0));
+        lnL = l > 2.0 ? l : 2.0;
+    } else lnL = lnL_fixed;
+    double a = qP * qQ / (EPS0 * reMass);
+    double s = lnL / (4.0 * PI_D) * (a * a) * (nQ * nP / nPQ) * pow(gm, -3.0) * dt;
+    double cc = cos_chi(s, u1);
+    double sc = sqrt(1.0 - cc * cc);
+    double eps = 2.0 * PI_D * u2;
+    double ce = cos(eps), se = sin(eps);
+    v3 h = V(gp * ce, -1.0 * (g.y * g.x * ce + gm * g.z * se) / gp, -1.0 * (g.z * g.x * ce - gm * g.y * se) / gp);
+    v3 dv = vadd(vscale(1.0 - cc, g), vscale(sc, h));
+    uP = vsub(uP, vscale(mQ / (mP + mQ), dv));
+    uQ = vadd(uQ, vscale(mP / (mP + mQ), dv));
+    st3(UP, uP); st3(UQ, uQ);
+    if (stats) { stats[0] += 1.0; stats[1] += lnL; stats[2] += s; }
+}
+
+/* CoulombCollisionModel::debyeLength, CoulombCollisionModel.C:249-310 (all charged species,
+ * drift removed) */
+static double debye_length(const ref_coulomb_params* p, double* const* U, const int* const* cs, int c, double Vc) {
+    double debye = 0.0, nmax = 0.0;
+    for (int s = 0; s < p->n_species; ++s) {
+        if (p->charge[s] == 0.0) continue;
+        int a = cs[s][c], b = cs[s][c + 1];
+        double n = 0.0, ch = 0.0, vs = 0.0;
+        v3 v = V(0, 0, 0);
+        for (int k = a; k < b; ++k) {
+            v3 u = ld3(U[s] + 3 * k);
+            vs += vdot(u, u) * p->weight[s];
+            ch += p->charge[s] * p->weight[s];
+            v = vadd(v, vscale(p->weight[s], u));
+            n += p->weight[s];
+        }
+        if (n <= 0.0) continue;
+        v = vdiv(v, n); ch /= n; vs /= n; n /= Vc;
+        if (n > nmax) nmax = n;
+        double T = p->mass[s] / (3.0 * KB) * (vs - vdot(v, v));
+        if (T <= 0.0) continue;
+        debye += n * ch * ch / EPS0 * 1.0 / (T * KB);
+    }
+    if (debye <= 0.0) return 0.0;
+    double rmin = pow(4.0 * PI_D * nmax / 3.0, -1.0 / 3.0);
+    double d = sqrt(1.0 / debye);
+    return d > rmin ? d : rmin;
+}
+
+static void coll_rand(uint64_t seed, int64_t step, int cell, int stage, int k, double* u1, double* u2) {
+    uint32_t o[4];
+    rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)((stage << 24) | k), TAG_COULOMB << 24, o);
+    *u1 = u01(o[0], o[1]);
+    *u2 = u01(o[2], o[3]);
+}
+
+/* TakizukaAbePairing::pairANDcollide, TakizukaAbePairing.C:60-338: per cell, species pairs
+ * (i, j>=i) visited as i ascending, j descending; intra-species pairs in shuffled order
+ * (odd count: first three collide cyclically with n12 = (n+3)w/V); inter-species equal
+ * counts pair k with n-1-k; unequal counts use the two Takizuka-Abe groups. */
+void ref_coulomb(const ref_mesh* m, const ref_coulomb_params* p, double dt, double* const* U,
+                 const uint32_t* const* id, const int* const* cs, double* stats) {
+    int ns = p->n_species;
+    int maxn = 1;
+    for (int s = 0; s < ns; ++s)
+        for (int c = 0; c < m->n_cells; ++c) if (cs[s][c + 1] - cs[s][c] > maxn) maxn = cs[s][c + 1] - cs[s][c];
+    int* pa = (int*)malloc(sizeof(int) * maxn);
+    int* pb = (int*)malloc(sizeof(int) * maxn);
+    int calc_debye = 0;
+    for (int i = 0; i < ns; ++i)
+        for (int j = i; j < ns; ++j)
+            if (p->charge[i] != 0.0 && p->charge[j] != 0.0 && p->coulomb_log[i][j] <= 0.0) calc_debye = 1;
+    for (int c = 0; c < m->n_cells; ++c) {
+        double Vc = m->V[c];
+        double debye = calc_debye ? debye_length(p, U, cs, c, Vc) : 0.0;
+        if (stats) stats[3] += debye;
+        int stage = 0;
+        for (int i = 0; i < ns; ++i) {
+            if (p->charge[i] == 0.0) continue;
+            for (int j = ns - 1; j >= i; --j) {
+                if (p->charge[j] == 0.0) continue;
+                int st = stage++;
+                int a0 = cs[i][c], na = cs[i][c + 1] - a0;
+                int b0 = cs[j][c], nb = cs[j][c + 1] - b0;
+                double mA = p->mass[i], qA = p->charge[i], mB = p->mass[j], qB = p->charge[j];
+                double lnL = p->coulomb_log[i][j];
+                double w = p->weight[i];
+                double u1, u2;
+                if (i == j) {
+                    if (na <= 1 || !p->allow_intra) continue;
+                    keyed_perm(p->seed, p->step, st, i, id[i], a0, na, pa);
+                    double n = na * w / Vc;
+                    double* Ub = U[i];
+                    if (na % 2) {
+                        double n12 = (na + 3.0) * w / Vc;
+                        int P = pa[0], Q = pa[1], R = pa[2];
+                        coll_rand(p->seed, p->step, c, st, 0, &u1, &u2);
+                        nanbu_collide(Ub + 3 * P, Ub + 3 * Q, mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n12, dt, u1, u2, stats);
+                        coll_rand(p->seed, p->step, c, st, 1, &u1, &u2);
+                        nanbu_collide(Ub + 3 * Q, Ub + 3 * R, mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n12, dt, u1, u2, stats);
+                        coll_rand(p->seed, p->step, c, st, 2, &u1, &u2);
+                        nanbu_collide(Ub + 3 * P, Ub + 3 * R, mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n12, dt, u1, u2, stats);
+                        for (int x = 3; x < na - 1; x += 2) {
+                            coll_rand(p->seed, p->step, c, st, 3 + (x - 3) / 2, &u1, &u2);
+                            nanbu_collide(Ub + 3 * pa[x], Ub + 3 * pa[x + 1], mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n12, dt, u1, u2, stats);
+                        }
+                    } else {
+                        for (int x = 0; x < na - 1; x += 2) {
+                            coll_rand(p->seed, p->step, c, st, x / 2, &u1, &u2);
+                            nanbu_collide(Ub + 3 * pa[x], Ub + 3 * pa[x + 1], mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n, dt, u1, u2, stats);
+                        }
+                    }
+                } else {
+                    if (na == 0 || nb == 0) continue;
+                    keyed_perm(p->seed, p->step, st, i, id[i], a0, na, pa);
+                    keyed_perm(p->seed, p->step, st, j, id[j], b0, nb, pb);
+                    if (na == nb) {
+                        double nP = na * w / Vc, nQ = nb * w / Vc;
+                        for (int k = 0; k < na; ++k) {
+                            coll_rand(p->seed, p->step, c, st, k, &u1, &u2);
+                            nanbu_collide(U[i] + 3 * pa[k], U[j] + 3 * pb[na - k - 1], mA, qA, mB, qB, lnL, p->calculate_bmin, debye, nP, nQ, nP, dt, u1, u2, stats);
+                        }
+                    } else {
+                        /* min / max species (TakizukaAbePairing.C:197-308) */
+                        int minIsA = na < nb;
+                        int nMin = minIsA ? na : nb, nMax = minIsA ? nb : na;
+                        int* pmin = minIsA ? pa : pb;
+                        int* pmax = minIsA ? pb : pa;
+                        int smin = minIsA ? i : j, smax = minIsA ? j : i;
+                        double nMinS = nMin * w / Vc, nMaxS = nMax * w / Vc;
+                        int delta = nMax / nMin;
+                        double delta_r = (double)(nMax % nMin) / nMin;
+                        int nMinG1 = (int)(delta_r * nMin);
+                        int nMaxG1 = (int)((delta + 1) * delta_r * nMin);
+                        int nMinG2 = (int)((1.0 - delta_r) * nMin);
+                        int nMaxG2 = (int)(delta * (1.0 - delta_r) * nMin);
+                        double mMin = p->mass[smin], qMin = p->charge[smin], mMax = p->mass[smax], qMax = p->charge[smax];
+                        double lnLp = p->coulomb_log[smax][smin];
+                        if (nMinG1 > 0) {
+                            int s1 = 0, s2 = 0;
+                            for (int g1 = 0; g1 < nMaxG1; ++g1) {
+                                if (s1 >= nMinG1) s1 = 0;
+                                coll_rand(p->seed, p->step, c, st, g1, &u1, &u2);
+                                nanbu_collide(U[smax] + 3 * pmax[s2], U[smin] + 3 * pmin[s1], mMax, qMax, mMin, qMin, lnLp, p->calculate_bmin, debye, nMaxS, nMinS, nMaxS, dt, u1, u2, stats);
+                                ++s1; ++s2;
+                            }
+                        }
+                        int s1 = nMinG1, s2 = nMaxG1;
+                        for (int g2 = 0; g2 < nMaxG2; ++g2) {
+                            if (s1 >= nMinG2 + nMinG1) s1 = nMinG1;
+                            coll_rand(p->seed, p->step, c, st, nMaxG1 + g2, &u1, &u2);
+                            nanbu_collide(U[smax] + 3 * pmax[s2], U[smin] + 3 * pmin[s1], mMax, qMax, mMin, qMin, lnLp, p->calculate_bmin, debye, nMaxS, nMinS, nMaxS, dt, u1, u2, stats);
+                            ++s1; ++s2;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    free(pa); free(pb);
+}
+
+/* ------------------------------------------------------------------ MCC */
+
+double ref_interpolate_xy(double x, int n, const double* xs, const double* ys) {
+    int lo = 0;
+    for (lo = 0; lo < n && xs[lo] > x; ++lo) {}
+    int low = lo;
+    if (low < n)
+        for (int i = low; i < n; ++i) if (xs[i] > xs[lo] && xs[i] <= x) lo = i;
+    int hi = 0;
+    for (hi = 0; hi < n && xs[hi] < x; ++hi) {}
+    int high = hi;
+    if (high < n)
+        for (int i = high; i < n; ++i) if (xs[i] < xs[hi] && xs[i] >= x) hi = i;
+    if (lo < n && hi < n && lo != hi) return ys[lo] + ((x - xs[lo]) / (xs[hi] - xs[lo])) * (ys[hi] - ys[lo]);
+    if (lo == hi) return ys[lo];
+    if (lo == n) return ys[hi];
+    return ys[lo];
+}
+
+static double sigma_of(int n, const double* e, const double* s, double eV) {
+    if (n <= 0) return 0.0;
+    return ref_interpolate_xy(eV, n, e, s);
+}
+
+/* NanbuElectronNeutralCollision::updateVelocity (NanbuElectronNeutralCollision.C:98-130) */
+static void nanbu_en_scatter(double eV, v3* UE, v3* Un, double mE, double mN, double uphi, double uchi) {
+    double phi = 2.0 * PI_D * uphi;
+    double cc;
+    if (eV < 1e-30) cc = 1.0;
+    else cc = 1.0 / eV * (eV + 2. - 2. * pow(1. + eV, uchi));
+    double sc;
+    if (cc * cc > 1.0) { cc = 1.0; sc = 0.0; }
+    else sc = sqrt(1.0 - cc * cc);
+    v3 g = vsub(*UE, *Un);
+    double gm = vmag(g), gp = sqrt(g.y * g.y + g.z * g.z);
+    if (!(gp > 0.0)) return;
+    double cp = cos(phi), sp = sin(phi);
+    v3 h = V(gp * cp, -1.0 * (g.y * g.x * cp + gm * g.z * sp) / gp, -1.0 * (g.z * g.x * cp - gm * g.y * sp) / gp);
+    v3 dv = vadd(vscale(1.0 - cc, g), vscale(sc, h));
+    *UE = vsub(*UE, vscale(mN / (mN + mE), dv));
+    *Un = vadd(*Un, vscale(mE / (mN + mE), dv));
+}
+
+static inline void mcc_rand(uint64_t seed, int64_t step, int cell, int k, int j, double* a, double* b) {
+    uint32_t o[4];
+    rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)k, (TAG_MCC << 24) | (uint32_t)j, o);
+    *a = u01(o[0], o[1]);
+    *b = u01(o[2], o[3]);
+}
+
+static inline void box_muller(double a, double b, double* n1, double* n2) {
+    double r = sqrt(-2.0 * log(1.0 - a));
+    *n1 = r * cos(2.0 * PI_D * b);
+    *n2 = r * sin(2.0 * PI_D * b);
+}
+
+typedef struct { uint32_t id; int idx; } idrec;
+static int idrec_cmp(const void* a, const void* b) {
+    const idrec* x = (const idrec*)a; const idrec* y = (const idrec*)b;
+    if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+/* ElectronNeutralCollisionModel::performBackgroundCollisions (ElectronNeutralCollisionModel.C:703-829)
+ * with the elastic/excitation/ionization events (:504-700); candidates are drawn with
+ * replacement from the cell's electrons taken in ascending id order. */
+int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, double* UE, int* tetE,
+            uint32_t* idE, const int* cs, int* n_e, int cap_e, double* yI, double* UI, int* tetI, uint32_t* idI,
+            int* n_i, int cap_i, double* scrmax, double* rem, int64_t* counts) {
+    const double c = C_LIGHT;
+    int maxn = 1;
+    for (int cc = 0; cc < m->n_cells; ++cc) if (cs[cc + 1] - cs[cc] > maxn) maxn = cs[cc + 1] - cs[cc];
+    idrec* L = (idrec*)malloc(sizeof(idrec) * maxn);
+    double vth = sqrt(KB * p->TN / p->massN);
+    double wEwN = p->wE / p->wN, wNwE = 1.0 / wEwN;
+    int err = 0;
+    for (int cell = 0; cell < m->n_cells; ++cell) {
+        int a0 = cs[cell], eC = cs[cell + 1] - a0;
+        double smax = scrmax[cell];
+        double coll = rem[cell] + eC * p->nN * smax * dt;
+        int nc = (int)coll;
+        rem[cell] = coll - nc;
+        if (counts) counts[0] += nc;
+        if (eC == 0 || nc == 0) continue;
+        for (int k = 0; k < eC; ++k) { L[k].id = idE[a0 + k]; L[k].idx = a0 + k; }
+        qsort(L, eC, sizeof(idrec), idrec_cmp);
+        for (int k = 0; k < nc; ++k) {
+            double upick, uacc, ua, ub, n1, n2, n3, n4;
+            mcc_rand(p->seed, p->step, cell, k, 0, &upick, &uacc);
+            int pick = (int)(upick * eC);
+            if (pick >= eC) pick = eC - 1;
+            int ei = L[pick].idx;
+            mcc_rand(p->seed, p->step, cell, k, 1, &ua, &ub);
+            box_muller(ua, ub, &n1, &n2);
+            mcc_rand(p->seed, p->step, cell, k, 2, &ua, &ub);
+            box_muller(ua, ub, &n3, &n4);
+            v3 Un = vadd(vscale(vth, V(n1, n2, n3)), V(p->driftN[0], p->driftN[1], p->driftN[2]));
+            v3 Ue = ld3(UE + 3 * ei);
+            double cR = vmag(vsub(Ue, Un));
+            double gamma = 1.0 / sqrt(1.0 - ((cR * cR) / (c * c)));
+            double eV = (gamma - 1.0) * p->massE * c * c / QE;
+            double sigmaT = smax / cR;
+            double Qel = sigma_of(p->n_el, p->e_el, p->s_el, eV);
+            double Qex = sigma_of(p->n_ex, p->e_ex, p->s_ex, eV);
+            double Qiz = sigma_of(p->n_iz, p->e_iz, p->s_iz, eV);
+            double Qt = Qel + Qex + Qiz;
+            if (Qt * cR > scrmax[cell]) scrmax[cell] = Qt * cR;
+            double us, uc;
+            mcc_rand(p->seed, p->step, cell, k, 3, &us, &uc);
+            if (uacc <= Qel / sigmaT) {
+                nanbu_en_scatter(eV, &Ue, &Un, p->massE, p->massN, us, uc);
+                if (counts) counts[1]++;
+            } else if (eV >= p->th_ex && uacc <= (Qel + Qex) / sigmaT) {
+                Ue = vscale(sqrt(1.0 - p->th_ex / eV), Ue);
+                eV -= p->th_ex;
+                nanbu_en_scatter(eV, &Ue, &Un, p->massE, p->massN, us, uc);
+                if (counts) counts[2]++;
+            } else if (eV >= p->th_iz && uacc <= Qt / sigmaT) {
+                double uE, uR, up2, uc2;
+                mcc_rand(p->seed, p->step, cell, k, 4, &uE, &uR);
+                mcc_rand(p->seed, p->step, cell, k, 5, &up2, &uc2);
+                double E0 = 2.0 - 100.0 / (eV + 10.0);
+                double E1 = (eV - p->th_iz) / 2.0 - E0;
+                double alpha = 10.3;
+                double Ep = E0 + alpha * tan(uE * (atan(E1 / alpha) + atan(E0 / alpha)) - atan(E0 / alpha));
+                v3 preUe = Ue, preUn = Un;
+                Ue = vscale(sqrt(1.0 - (p->th_iz + Ep) / eV), Ue);
+                eV -= (p->th_iz + Ep);
+                nanbu_en_scatter(eV, &Ue, &Un, p->massE, p->massN, us, uc);
+                int created = 0;
+                if (uR <= wEwN && p->create_ion) {
+                    if (*n_i >= cap_i) { err = -1; }
+                    else {
+                        int ni = (*n_i)++;
+                        memcpy(yI + 3 * ni, yE + 3 * ei, sizeof(double) * 3);
+                        tetI[ni] = tetE[ei];
+                        st3(UI + 3 * ni, Un);
+                        uint32_t o[4];
+                        rng4(p->seed, idE[ei], (uint32_t)p->step, (uint32_t)k, (TAG_NEWID << 24) | 1u, o);
+                        idI[ni] = o[0];
+                        created = 1;
+                    }
+                }
+                if (uR > wNwE) Ue = preUe;
+                if (uR <= wEwN && p->create_electron) {
+                    if (*n_e >= cap_e) { err = -1; }
+                    else {
+                        int ne = (*n_e)++;
+                        memcpy(yE + 3 * ne, yE + 3 * ei, sizeof(double) * 3);
+                        tetE[ne] = tetE[ei];
+                        v3 Unew = vscale(sqrt(2.0 * Ep * QE / p->massE), vdiv(preUe, vmag(preUe)));
+                        v3 Un2 = preUn;
+                        nanbu_en_scatter(Ep, &Unew, &Un2, p->massE, p->massN, up2, uc2);
+                        st3(UE + 3 * ne, Unew);
+                        uint32_t o[4];
+                        rng4(p->seed, idE[ei], (uint32_t)p->step, (uint32_t)k, (TAG_NEWID << 24) | 2u, o);
+                        idE[ne] = o[0];
+                    }
+                }
+                if (counts && created) counts[3]++;
+            }
+            st3(UE + 3 * ei, Ue);
+        }
+    }
+    free(L);
+    return err;
+}
+
+/* ------------------------------------------------------------------ Poisson */
+
+struct ref_poisson {
+    const ref_mesh* m;
+    int n, K;               /* cells, max couplings per cell */
+    int* nb;                /* [n*K] neighbour cell (-1 unused) */
+    double* a;              /* [n*K] coupling coefficient |Sf|*deltaCoeff (>0) */
+    double* diag;
+    double* bsrc;           /* Dirichlet source per cell (a_b * value) */
+    int singular;
+    int* colour;
+    int ncol;
+    int* corder;            /* cells sorted by (colour, index) */
+    double* rD;             /* DIC reciprocal diagonal source (not inverted) */
+    int* bctype;
+    double* bcval;
+    /* face interpolation weights for Gauss grad */
+    double* w;              /* per face owner weight */
+};
+
+static void add_coupling(struct ref_poisson* P, int i, int j, double a) {
+    for (int k = 0; k < P->K; ++k) {
+        if (P->nb[i * P->K + k] == j) { P->a[i * P->K + k] += a; return; }
+        if (P->nb[i * P->K + k] < 0) { P->nb[i * P->K + k] = j; P->a[i * P->K + k] = a; return; }
+    }
+}
+
+/* fvm::laplacian with "Gauss linear corrected" (orthogonal part: nonOrthDeltaCoeffs =
+ * 1/max(nf.d, 0.05|d|)), cyclic coupling, fixedValue/zeroGradient patches; DIC is IC(0)
+ * in a greedy multicolour ordering (cells coloured in index order). */
+ref_poisson* ref_poisson_create(const ref_mesh* m, const int* bct, const double* bcv) {
+    ref_poisson* P = (ref_poisson*)calloc(1, sizeof(ref_poisson));
+    P->m = m;
+    int n = m->n_cells;
+    P->n = n;
+    int K = 0;
+    for (int c = 0; c < n; ++c) if (m->cfo[c + 1] - m->cfo[c] > K) K = m->cfo[c + 1] - m->cfo[c];
+    P->K = K;
+    P->nb = (int*)malloc(sizeof(int) * n * K);
+    for (int i = 0; i < n * K; ++i) P->nb[i] = -1;
+    P->a = (double*)calloc(n * K, sizeof(double));
+    P->diag = (double*)calloc(n, sizeof(double));
+    P->bsrc = (double*)calloc(n, sizeof(double));
+    P->w = (double*)calloc(m->n_faces, sizeof(double));
+    P->bctype = (int*)malloc(sizeof(int) * (m->n_patches + 1));
+    P->bcval = (double*)malloc(sizeof(double) * (m->n_patches + 1));
+    memcpy(P->bctype, bct, sizeof(int) * m->n_patches);
+    memcpy(P->bcval, bcv, sizeof(double) * m->n_patches);
+    int has_dirichlet = 0;
+    for (int f = 0; f < m->n_faces; ++f) {
+        v3 Sf = ld3(m->Sf + 3 * f), Cf = ld3(m->Cf + 3 * f);
+        double mS = vmag(Sf);
+        v3 nf = vdiv(Sf, mS);
+        int o = m->owner[f];
+        v3 Co = ld3(m->C + 3 * o);
+        if (f < m->n_internal) {
+            int nb = m->nbr[f];
+            v3 Cn = ld3(m->C + 3 * nb);
+            v3 d = vsub(Cn, Co);
+            double nd = vdot(nf, d), md = 0.05 * vmag(d);
+            double a = mS / (nd > md ? nd : md);
+            add_coupling(P, o, nb, a);
+            add_coupling(P, nb, o, a);
+            P->diag[o] += a; P->diag[nb] += a;
+            double so = fabs(vdot(Sf, vsub(Cf, Co))), sn = fabs(vdot(Sf, vsub(Cn, Cf)));
+            P->w[f] = sn / (so + sn);
+            continue;
+        }
+        int pi = m->face_patch[f];
+        if (m->ptype[pi] == REF_PATCH_CYCLIC) {
+            int q = m->pnbr[pi];
+            int g = m->pstart[q] + (f - m->pstart[pi]);
+            int nb = m->owner[g];
+            v3 Cn = vsub(ld3(m->C + 3 * nb), ld3(m->psep + 3 * pi));
+            v3 Cfn = Cf; /* this side's face centre */
+            double so = fabs(vdot(Sf, vsub(Cfn, Co))), sn = fabs(vdot(Sf, vsub(Cn, Cfn)));
+            P->w[f] = sn / (so + sn);
+            if (nb == o) continue; /* self-coupled: no net contribution */
+            v3 d = vsub(Cn, Co);
+            double nd = vdot(nf, d), md = 0.05 * vmag(d);
+            double a = mS / (nd > md ? nd : md);
+            add_coupling(P, o, nb, a);
+            P->diag[o] += a;
+        } else if (P->bctype[pi] == 1) {
+            v3 d = vsub(Cf, Co);
+            double a = mS / vdot(nf, d);
+            P->diag[o] += a;
+            P->bsrc[o] += a * P->bcval[pi];
+            has_dirichlet = 1;
+            P->w[f] = 1.0;
+        } else {
+            P->w[f] = 1.0;
+        }
+    }
+    P->singular = !has_dirichlet;
+    /* greedy colouring in cell order */
+    P->colour = (int*)malloc(sizeof(int) * n);
+    P->ncol = 0;
+    for (int i = 0; i < n; ++i) {
+        unsigned long long used = 0;
+        for (int k = 0; k < K; ++k) {
+            int j = P->nb[i * K + k];
+            if (j >= 0 && j < i) used |= 1ull << P->colour[j];
+        }
+        int col = 0;
+        while (used & (1ull << col)) ++col;
+        P->colour[i] = col;
+        if (col + 1 > P->ncol) P->ncol = col + 1;
+    }
+    P->corder = (int*)malloc(sizeof(int) * n);
+    int t = 0;
+    for (int col = 0; col < P->ncol; ++col)
+        for (int i = 0; i < n; ++i) if (P->colour[i] == col) P->corder[t++] = i;
+    P->rD = (double*)malloc(sizeof(double) * n);
+    for (int t2 = 0; t2 < n; ++t2) {
+        int i = P->corder[t2];
+        double r = P->diag[i];
+        for (int k = 0; k < K; ++k) {
+            int j = P->nb[i * K + k];
+            if (j >= 0 && P->colour[j] < P->colour[i]) r -= P->a[i * K + k] * P->a[i * K + k] / P->rD[j];
+        }
+        P->rD[i] = r;
+    }
+    return P;
+}
+
+void ref_poisson_destroy(ref_poisson* P) {
+    if (!P) return;
+    free(P->nb); free(P->a); free(P->diag); free(P->bsrc); free(P->colour); free(P->corder); free(P->rD);
+    free(P->bctype); free(P->bcval); free(P->w);
+    free(P);
+}
+
+int ref_poisson_n_colours(const ref_poisson* P) { return P->ncol; }
+void ref_poisson_colours(const ref_poisson* P, int* col) { memcpy(col, P->colour, sizeof(int) * P->n); }
+
+void ref_poisson_apply(const ref_poisson* P, const double* x, double* y) {
+    for (int i = 0; i < P->n; ++i) {
+        double s = P->diag[i] * x[i];
+        for (int k = 0; k < P->K; ++k) {
+            int j = P->nb[i * P->K + k];
+            if (j >= 0) s -= P->a[i * P->K + k] * x[j];
+        }
+        y[i] = s;
+    }
+}
+
+void ref_poisson_rhs(const ref_poisson* P, const double* rho, double* b) {
+    for (int i = 0; i < P->n; ++i) b[i] = rho[i] * P->m->V[i] / EPS0 + P->bsrc[i];
+    if (P->singular) {
+        double s = 0.0;
+        for (int i = 0; i < P->n; ++i) s += b[i];
+        s /= P->n;
+        for (int i = 0; i < P->n; ++i) b[i] -= s;
+    }
+}
+
+static void dic_apply(const ref_poisson* P, const double* r, double* z) {
+    int n = P->n, K = P->K;
+    for (int t = 0; t < n; ++t) {
+        int i = P->corder[t];
+        double s = r[i];
+        for (int k = 0; k < K; ++k) {
+            int j = P->nb[i * K + k];
+            if (j >= 0 && P->colour[j] < P->colour[i]) s += P->a[i * K + k] * z[j];
+        }
+        z[i] = s / P->rD[i];
+    }
+    for (int t = n - 1; t >= 0; --t) {
+        int i = P->corder[t];
+        double s = 0.0;
+        for (int k = 0; k < K; ++k) {
+            int j = P->nb[i * K + k];
+            if (j >= 0 && P->colour[j] > P->colour[i]) s += P->a[i * K + k] * z[j];
+        }
+        z[i] = z[i] + s / P->rD[i];
+    }
+}
+
+static double dotp(int n, const double* a, const double* b) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+int ref_poisson_solve(const ref_poisson* P, const double* rho, double* phi, double tol, int max_iter, int precond,
+                      double* final_rel) {
+    int n = P->n;
+    double* b = (double*)malloc(sizeof(double) * n);
+    double* r = (double*)malloc(sizeof(double) * n);
+    double* z = (double*)malloc(sizeof(double) * n);
+    double* pp = (double*)malloc(sizeof(double) * n);
+    double* q = (double*)malloc(sizeof(double) * n);
+    ref_poisson_rhs(P, rho, b);
+    ref_poisson_apply(P, phi, q);
+    for (int i = 0; i < n; ++i) r[i] = b[i] - q[i];
+    double bn = sqrt(dotp(n, b, b));
+    if (bn == 0.0) bn = 1.0;
+    double rn = sqrt(dotp(n, r, r));
+    int it = 0;
+    if (rn / bn > tol) {
+        if (precond) dic_apply(P, r, z); else memcpy(z, r, sizeof(double) * n);
+        memcpy(pp, z, sizeof(double) * n);
+        double rz = dotp(n, r, z);
+        for (it = 1; it <= max_iter; ++it) {
+            ref_poisson_apply(P, pp, q);
+            double alpha = rz / dotp(n, pp, q);
+            for (int i = 0; i < n; ++i) { phi[i] += alpha * pp[i]; r[i] -= alpha * q[i]; }
+            rn = sqrt(dotp(n, r, r));
+            if (rn / bn <= tol) break;
+            if (precond) dic_apply(P, r, z); else memcpy(z, r, sizeof(double) * n);
+            double rz2 = dotp(n, r, z);
+            double beta = rz2 / rz;
+            rz = rz2;
+            for (int i = 0; i < n; ++i) pp[i] = z[i] + beta * pp[i];
+        }
+        if (it > max_iter) it = max_iter;
+    }
+    if (P->singular) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += phi[i];
+        s /= n;
+        for (int i = 0; i < n; ++i) phi[i] -= s;
+    }
+    if (final_rel) *final_rel = rn / bn;
+    free(b); free(r); free(z); free(pp); free(q);
+    return it;
+}
+
+/* fvc::grad, Gauss linear (ElectroStaticSolver.C:68) */
+void ref_grad(const ref_poisson* P, const double* phi, double* gr) {
+    const ref_mesh* m = P->m;
+    memset(gr, 0, sizeof(double) * 3 * m->n_cells);
+    for (int f = 0; f < m->n_faces; ++f) {
+        v3 Sf = ld3(m->Sf + 3 * f);
+        int o = m->owner[f];
+        double pf;
+        if (f < m->n_internal) {
+            int nb = m->nbr[f];
+            pf = P->w[f] * phi[o] + (1.0 - P->w[f]) * phi[nb];
+            gr[3 * nb + 0] -= Sf.x * pf; gr[3 * nb + 1] -= Sf.y * pf; gr[3 * nb + 2] -= Sf.z * pf;
+        } else {
+            int pi = m->face_patch[f];
+            if (m->ptype[pi] == REF_PATCH_CYCLIC) {
+                int q = m->pnbr[pi];
+                int g = m->pstart[q] + (f - m->pstart[pi]);
+                pf = P->w[f] * phi[o] + (1.0 - P->w[f]) * phi[m->owner[g]];
+            } else if (P->bctype[pi] == 1) pf = P->bcval[pi];
+            else pf = phi[o];
+        }
+        gr[3 * o + 0] += Sf.x * pf; gr[3 * o + 1] += Sf.y * pf; gr[3 * o + 2] += Sf.z * pf;
+    }
+    for (int c = 0; c < m->n_cells; ++c)
+        for (int d = 0; d < 3; ++d) gr[3 * c + d] /= m->V[c];
+}

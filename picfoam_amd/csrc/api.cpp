@@ -1,0 +1,1199 @@
+// C-ABI of libpicmi: the context owns the mesh tables, fields and particle arrays on one
+// HIP device and runs the stages of PICCloud::evolve() (PICCloud.C:1426-1471) as batched
+// kernels on its stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/picmi.h"
+#include "internal.h"
+#include "kernels.h"
+
+using namespace picmi;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Species {
+    double mass = 0, charge = 0, weight = 0;
+    int solve_movement = 1;
+    int64_t capacity = 0;
+    int64_t n = 0;  // host mirror of the parcel count
+    // double-buffered SoA
+    void* y[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    void* u[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    int32_t* tet[2] = {nullptr, nullptr};
+    uint32_t* id[2] = {nullptr, nullptr};
+    int cur = 0;
+    int* count = nullptr;   // [n_cells+1]
+    int* start = nullptr;   // [n_cells+1]
+    int* cursor = nullptr;  // [n_cells]
+};
+
+struct StageEvent {
+    int stage;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct pic_ctx {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    HostMesh hm;
+    HostPoisson hp;
+    bool has_poisson = false;
+    // models
+    int pusher = kPushBoris, fw = kCellAverage, cd = kCellAverage;
+    bool electrostatic = false, coulomb = false, mcc = false, bg = false;
+    int precision = PIC_PREC_FP64;
+    int hc_ref = 0;
+    int allow_intra = 1, calc_bmin = 1;
+    double coulomb_log[kMaxSpecies][kMaxSpecies] = {};
+    int electron = -1, ion = -1;
+    double bg_n = 0, bg_T = 0, bg_m = 0, bg_w = 0, bg_v[3] = {0, 0, 0};
+    int create_ion = 1, create_electron = 1;
+    uint64_t seed = 0;
+    double ptol = 1e-8;
+    int pmaxit = 10000, pmode = 0;
+    int sort_interval = 1;
+    int64_t step = 0;
+    bool rotate = false;
+    bool sorted = false, counts_valid = false;
+    // species
+    std::vector<Species> sp;
+    double Q0 = 1.0;
+    // device mesh
+    TetGeo* d_tets = nullptr;
+    int32_t *d_cell_tet0 = nullptr, *d_cpo = nullptr, *d_cp_master = nullptr, *d_pco = nullptr, *d_pc_cell = nullptr,
+            *d_pmaster = nullptr;
+    double *d_C = nullptr, *d_V = nullptr, *d_points = nullptr, *d_vw_w = nullptr, *d_pc_w = nullptr;
+    // fields
+    double *d_E = nullptr, *d_B = nullptr, *d_phi = nullptr, *d_rho = nullptr, *d_space = nullptr,
+           *d_Ep = nullptr;
+    float *d_Ef = nullptr, *d_Epf = nullptr, *d_Bf = nullptr;
+    bool has_space = false;
+    unsigned long long *d_acc_cell = nullptr, *d_acc_point = nullptr;
+    uint32_t* d_flags = nullptr;
+    unsigned long long* d_stats = nullptr;  // 0 deleted, 1 guard, 2 coulomb, 3..6 mcc
+    int64_t* d_n = nullptr;                  // device parcel counts per species
+    int* d_maxc = nullptr;
+    void* d_scan_tmp = nullptr;
+    size_t scan_bytes = 0;
+    double* d_diag_part = nullptr;
+    double* d_diag_out = nullptr;
+    // MCC
+    double *d_smax = nullptr, *d_rem = nullptr;
+    double* d_xs_e[3] = {nullptr, nullptr, nullptr};
+    double* d_xs_s[3] = {nullptr, nullptr, nullptr};
+    int xs_n[3] = {0, 0, 0};
+    double xs_th[3] = {0, 0, 0};
+    std::vector<double> xs_he[3], xs_hs[3];
+    // Poisson
+    PoissonDev pd{};
+    PoissonWork pw{};
+    double* d_b = nullptr;
+    // gradient tables
+    int32_t *d_gfo = nullptr, *d_gkind = nullptr, *d_gother = nullptr;
+    double *d_gw = nullptr, *d_gval = nullptr, *d_gS = nullptr;
+    // profiling
+    bool profile = false;
+    std::vector<StageEvent> pending;
+    std::vector<hipEvent_t> pool;
+    double stage_ms[PIC_STAGE_COUNT] = {};
+    int64_t stage_n[PIC_STAGE_COUNT] = {};
+
+    int fail(int code, const std::string& m) {
+        err = m;
+        g_last_error = m;
+        return code;
+    }
+    template <class T>
+    T* alloc(size_t count) {
+        if (count == 0) count = 1;
+        void* p = nullptr;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) return nullptr;
+        hipMemsetAsync(p, 0, count * sizeof(T), stream);
+        allocs.push_back(p);
+        return (T*)p;
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(v.size());
+        if (p && !v.empty()) hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream);
+        return p;
+    }
+    hipEvent_t ev() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    struct Scope {
+        pic_ctx* c;
+        int stage;
+        hipEvent_t a = nullptr;
+        Scope(pic_ctx* c_, int s) : c(c_), stage(s) {
+            if (c->profile) { a = c->ev(); hipEventRecord(a, c->stream); }
+        }
+        ~Scope() {
+            if (c->profile) {
+                hipEvent_t b = c->ev();
+                hipEventRecord(b, c->stream);
+                c->pending.push_back({stage, a, b});
+            }
+        }
+    };
+    bool mixed() const { return precision == PIC_PREC_MIXED; }
+};
+
+namespace {
+
+int check_hip(pic_ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return PIC_OK;
+    return c->fail(PIC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class UT>
+PartSoA<UT> soa(const Species& s, int buf) {
+    PartSoA<UT> p;
+    p.y1 = (double*)s.y[buf][0]; p.y2 = (double*)s.y[buf][1]; p.y3 = (double*)s.y[buf][2];
+    p.ux = (UT*)s.u[buf][0]; p.uy = (UT*)s.u[buf][1]; p.uz = (UT*)s.u[buf][2];
+    p.tet = s.tet[buf];
+    p.id = s.id[buf];
+    return p;
+}
+
+int parse_model(pic_ctx* c, const char* name, const char* keyword, const std::vector<std::string>& valid,
+                int* out) {
+    std::string n = name ? name : "";
+    for (size_t i = 0; i < valid.size(); ++i)
+        if (valid[i] == n) { *out = (int)i; return PIC_OK; }
+    std::string m = "Unknown " + std::string(keyword) + " type " + n + "\n\nValid " + keyword + " types are:\n" +
+                    std::to_string(valid.size()) + "\n(\n";
+    for (auto& v : valid) m += v + "\n";
+    m += ")";
+    return c->fail(PIC_ERR_UNKNOWN_MODEL, m);
+}
+
+// error flags + counts after a synchronisation point
+int sync_state(pic_ctx* c) {
+    uint32_t flags = 0;
+    std::vector<int64_t> n(c->sp.size());
+    hipMemcpyAsync(&flags, c->d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(n.data(), c->d_n, sizeof(int64_t) * c->sp.size(), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return check_hip(c, e, "stream synchronize");
+    for (size_t s = 0; s < c->sp.size(); ++s) c->sp[s].n = std::min<int64_t>(n[s], c->sp[s].capacity);
+    if (flags) {
+        hipMemsetAsync(c->d_flags, 0, sizeof(uint32_t), c->stream);
+        if (flags & 1u) return c->fail(PIC_ERR_SUPERLUMINAL, "Particle velocity greater than speed of light (299792458)! Aborting...");
+        if (flags & 2u) return c->fail(PIC_ERR_CELL_OVERFLOW, "a cell holds more parcels than the collision kernel capacity");
+        if (flags & 4u) return c->fail(PIC_ERR_CAPACITY, "species capacity exceeded by created parcels");
+        if (flags & 8u) return c->fail(PIC_ERR_MESH, "tracking reached a tet face without a neighbour");
+    }
+    return PIC_OK;
+}
+
+void set_dev_n(pic_ctx* c, int s) {
+    hipMemcpyAsync(c->d_n + s, &c->sp[s].n, sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+}
+
+int alloc_species(pic_ctx* c, Species& s) {
+    const size_t cap = (size_t)s.capacity;
+    const size_t ub = c->mixed() ? sizeof(float) : sizeof(double);
+    for (int b = 0; b < 2; ++b) {
+        for (int k = 0; k < 3; ++k) {
+            s.y[b][k] = c->alloc<double>(cap);
+            if (!s.y[b][k]) return c->fail(PIC_ERR_HIP, "out of device memory (particles)");
+            void* p = nullptr;
+            if (hipMalloc(&p, std::max<size_t>(cap, 1) * ub) != hipSuccess) return c->fail(PIC_ERR_HIP, "out of device memory (particles)");
+            c->allocs.push_back(p);
+            s.u[b][k] = p;
+        }
+        s.tet[b] = c->alloc<int32_t>(cap);
+        s.id[b] = c->alloc<uint32_t>(cap);
+        if (!s.tet[b] || !s.id[b]) return c->fail(PIC_ERR_HIP, "out of device memory (particles)");
+    }
+    const int nc = c->hm.n_cells;
+    s.count = c->alloc<int>(nc + 1);
+    s.start = c->alloc<int>(nc + 1);
+    s.cursor = c->alloc<int>(nc + 1);
+    if (!s.count || !s.start || !s.cursor) return c->fail(PIC_ERR_HIP, "out of device memory (cell counts)");
+    return PIC_OK;
+}
+
+double fixed_scale(const pic_ctx* c, int s) { return c->sp[s].charge * c->sp[s].weight / c->Q0 * kFixedScale; }
+
+// zero counts and accumulators before a deposit pass
+void zero_deposit(pic_ctx* c) {
+    for (auto& s : c->sp) hipMemsetAsync(s.count, 0, sizeof(int) * (c->hm.n_cells + 1), c->stream);
+    if (c->cd == kVolumeWeighting) {
+        hipMemsetAsync(c->d_acc_cell, 0, sizeof(unsigned long long) * c->hm.n_cells, c->stream);
+        hipMemsetAsync(c->d_acc_point, 0, sizeof(unsigned long long) * c->hm.n_points, c->stream);
+    }
+}
+
+template <class UT>
+MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
+    Species& S = c->sp[s];
+    MoveArgs<UT> a{};
+    a.p = soa<UT>(S, S.cur);
+    a.n = S.n;
+    a.tets = c->d_tets;
+    if constexpr (sizeof(UT) == 4) {
+        a.Ecell = (const UT*)c->d_Ef; a.Epoint = (const UT*)c->d_Epf; a.Bcell = (const UT*)c->d_Bf;
+    } else {
+        a.Ecell = (const UT*)c->d_E; a.Epoint = (const UT*)c->d_Ep; a.Bcell = (const UT*)c->d_B;
+    }
+    a.k = (UT)(S.charge * (0.5 * dt) / S.mass);
+    a.kv = (UT)(S.charge * dt / S.mass);
+    a.dt = dt;
+    a.do_push = (S.charge != 0.0) ? 1 : 0;
+    a.rotate = c->rotate ? 1 : 0;
+    a.hc_ref = c->hc_ref;
+    a.count = S.count;
+    a.acc_cell = c->d_acc_cell;
+    a.acc_point = c->d_acc_point;
+    a.s42 = fixed_scale(c, s);
+    a.flags = c->d_flags;
+    a.stats = c->d_stats;
+    return a;
+}
+
+int do_move(pic_ctx* c, double dt) {
+    zero_deposit(c);
+    for (int s = 0; s < (int)c->sp.size(); ++s) {
+        Species& S = c->sp[s];
+        if (S.n == 0) continue;
+        pic_ctx::Scope sc(c, PIC_STAGE_MOVE);  // one timed launch per species
+        hipError_t e;
+        if (!S.solve_movement) {
+            // stationary species are still counted / deposited (PICParcel.C:51)
+            if (c->mixed()) e = launch_deposit(move_args<float>(c, s, dt), c->cd, c->stream);
+            else e = launch_deposit(move_args<double>(c, s, dt), c->cd, c->stream);
+        } else if (c->mixed()) {
+            e = launch_move(move_args<float>(c, s, dt), c->pusher, c->fw, c->cd, c->stream);
+        } else {
+            e = launch_move(move_args<double>(c, s, dt), c->pusher, c->fw, c->cd, c->stream);
+        }
+        if (int r = check_hip(c, e, "move kernel")) return r;
+    }
+    c->counts_valid = true;
+    c->sorted = false;
+    return PIC_OK;
+}
+
+int do_sort(pic_ctx* c, int* maxc_out) {
+    {
+        pic_ctx::Scope sc(c, PIC_STAGE_SORT);
+        const int nc = c->hm.n_cells;
+        if (!c->counts_valid) {
+            for (auto& s : c->sp) hipMemsetAsync(s.count, 0, sizeof(int) * (nc + 1), c->stream);
+            for (int s = 0; s < (int)c->sp.size(); ++s) {
+                hipError_t e = c->mixed() ? launch_deposit(move_args<float>(c, s, 0.0), kCellAverage, c->stream)
+                                          : launch_deposit(move_args<double>(c, s, 0.0), kCellAverage, c->stream);
+                if (int r = check_hip(c, e, "count kernel")) return r;
+            }
+            c->counts_valid = true;
+        }
+        hipMemsetAsync(c->d_maxc, 0, sizeof(int), c->stream);
+        for (int s = 0; s < (int)c->sp.size(); ++s) {
+            Species& S = c->sp[s];
+            hipError_t e = launch_scan(S.count, S.start, nc + 1, c->d_scan_tmp, c->scan_bytes, c->stream);
+            if (int r = check_hip(c, e, "scan")) return r;
+            launch_max_count(S.count, nc, c->d_maxc, c->stream);
+            hipMemsetAsync(S.cursor, 0, sizeof(int) * (nc + 1), c->stream);
+            const int nb = 1 - S.cur;
+            if (c->mixed()) e = launch_scatter(soa<float>(S, S.cur), soa<float>(S, nb), S.n, c->d_tets, S.start, S.cursor, c->stream);
+            else e = launch_scatter(soa<double>(S, S.cur), soa<double>(S, nb), S.n, c->d_tets, S.start, S.cursor, c->stream);
+            if (int r = check_hip(c, e, "scatter")) return r;
+            S.cur = nb;
+        }
+    }
+    // live counts after compaction: start[n_cells]
+    std::vector<int> nn(c->sp.size());
+    int maxc = 0;
+    for (size_t s = 0; s < c->sp.size(); ++s)
+        hipMemcpyAsync(&nn[s], c->sp[s].start + c->hm.n_cells, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(&maxc, c->d_maxc, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (int r = sync_state(c)) return r;
+    for (size_t s = 0; s < c->sp.size(); ++s) {
+        c->sp[s].n = nn[s];
+        set_dev_n(c, (int)s);
+    }
+    if (maxc_out) *maxc_out = maxc;
+    c->sorted = true;
+    return PIC_OK;
+}
+
+int pow2_at_least(int v) {
+    int p = 64;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+int do_coulomb(pic_ctx* c, double dt, int maxc) {
+    pic_ctx::Scope sc(c, PIC_STAGE_COULOMB);
+    const int ns = (int)c->sp.size();
+    CoulombArgs a{};
+    a.n_cells = c->hm.n_cells;
+    a.n_species = ns;
+    bool calc_debye = false;
+    for (int i = 0; i < ns; ++i) {
+        a.mass[i] = c->sp[i].mass;
+        a.charge[i] = c->sp[i].charge;
+        a.weight[i] = c->sp[i].weight;
+        a.start[i] = c->sp[i].start;
+        const int b = c->sp[i].cur;
+        a.ux[i] = c->sp[i].u[b][0]; a.uy[i] = c->sp[i].u[b][1]; a.uz[i] = c->sp[i].u[b][2];
+        a.id[i] = c->sp[i].id[b];
+        for (int j = 0; j < ns; ++j) {
+            a.coulomb_log[i][j] = c->coulomb_log[i][j];
+            if (j >= i && c->sp[i].charge != 0.0 && c->sp[j].charge != 0.0 && c->coulomb_log[i][j] <= 0.0) calc_debye = true;
+        }
+    }
+    a.calc_debye = calc_debye ? 1 : 0;
+    a.calculate_bmin = c->calc_bmin;
+    a.allow_intra = c->allow_intra;
+    a.V = c->d_V;
+    a.dt = dt;
+    a.seed = c->seed;
+    a.step = c->step;
+    a.maxp = pow2_at_least(maxc);
+    if (a.maxp > 8192) return c->fail(PIC_ERR_CELL_OVERFLOW, "more than 8192 parcels of one species in a cell");
+    a.flags = c->d_flags;
+    a.ncoll = c->d_stats + 2;
+    return check_hip(c, launch_coulomb(a, c->mixed(), c->stream), "coulomb kernel");
+}
+
+template <class UT>
+int do_mcc_t(pic_ctx* c, double dt, int maxc) {
+    Species& E = c->sp[c->electron];
+    MccArgs<UT> a{};
+    a.n_cells = c->hm.n_cells;
+    a.e = soa<UT>(E, E.cur);
+    a.start = E.start;
+    a.n_e = c->d_n + c->electron;
+    a.cap_e = E.capacity;
+    if (c->ion >= 0) {
+        Species& I = c->sp[c->ion];
+        a.ion = soa<UT>(I, I.cur);
+        a.n_i = c->d_n + c->ion;
+        a.cap_i = I.capacity;
+        a.count_i = I.count;
+        a.s42_i = fixed_scale(c, c->ion);
+    } else {
+        a.n_i = c->d_n;
+        a.cap_i = 0;
+        a.count_i = E.count;
+    }
+    a.smax = c->d_smax;
+    a.rem = c->d_rem;
+    for (int k = 0; k < 3; ++k) { a.xs_e[k] = c->d_xs_e[k]; a.xs_s[k] = c->d_xs_s[k]; a.xs_n[k] = c->xs_n[k]; }
+    a.th_ex = c->xs_th[1];
+    a.th_iz = c->xs_th[2];
+    a.massE = E.mass;
+    a.massN = c->bg_m;
+    a.nN = c->bg_n;
+    a.vthN = std::sqrt(kKB * c->bg_T / c->bg_m);
+    for (int d = 0; d < 3; ++d) a.drift[d] = c->bg_v[d];
+    a.wEwN = E.weight / c->bg_w;
+    a.wNwE = 1.0 / a.wEwN;
+    a.create_ion = (c->create_ion && c->ion >= 0) ? 1 : 0;
+    a.create_electron = c->create_electron;
+    a.dt = dt;
+    a.seed = c->seed;
+    a.step = c->step;
+    a.maxp = pow2_at_least(maxc);
+    if (a.maxp > 16384) return c->fail(PIC_ERR_CELL_OVERFLOW, "more than 16384 electrons in a cell");
+    a.flags = c->d_flags;
+    a.counts = c->d_stats + 3;
+    a.tets = c->d_tets;
+    a.count_e = E.count;
+    a.acc_cell = c->d_acc_cell;
+    a.acc_point = c->d_acc_point;
+    a.s42_e = fixed_scale(c, c->electron);
+    a.dep_vw = c->cd == kVolumeWeighting ? 1 : 0;
+    return check_hip(c, launch_mcc(a, c->stream), "mcc kernel");
+}
+
+int do_mcc(pic_ctx* c, double dt, int maxc) {
+    pic_ctx::Scope sc(c, PIC_STAGE_MCC);
+    if (c->xs_n[0] + c->xs_n[1] + c->xs_n[2] == 0) return c->fail(PIC_ERR_INVALID, "no cross sections set");
+    return c->mixed() ? do_mcc_t<float>(c, dt, maxc) : do_mcc_t<double>(c, dt, maxc);
+}
+
+int do_deposit_finish(pic_ctx* c) {
+    pic_ctx::Scope sc(c, PIC_STAGE_DEPOSIT);
+    DepositFinishArgs a{};
+    a.n_cells = c->hm.n_cells;
+    a.n_species = (int)c->sp.size();
+    for (int s = 0; s < a.n_species; ++s) {
+        a.counts[s] = c->sp[s].count;
+        a.whole[s] = llrint(fixed_scale(c, s));
+    }
+    a.acc_cell = c->d_acc_cell;
+    a.acc_point = c->d_acc_point;
+    a.cpo = c->d_cpo;
+    a.cp_master = c->d_cp_master;
+    a.vw_w = c->d_vw_w;
+    a.V = c->d_V;
+    a.space_charge = c->has_space ? c->d_space : nullptr;
+    a.q42 = c->Q0 / kFixedScale;
+    a.weighting = c->cd;
+    a.rho = c->d_rho;
+    return check_hip(c, launch_deposit_finish(a, c->stream), "deposit finish");
+}
+
+int update_field_derived(pic_ctx* c) {
+    pic_ctx::Scope sc(c, PIC_STAGE_FIELD);
+    const HostMesh& m = c->hm;
+    PointInterpArgs pa{};
+    pa.n_points = m.n_points;
+    pa.pco = c->d_pco;
+    pa.pc_cell = c->d_pc_cell;
+    pa.pc_w = c->d_pc_w;
+    pa.pmaster = c->d_pmaster;
+    pa.Ecell = c->d_E;
+    pa.Epoint = c->d_Ep;
+    pa.Epoint_f = c->d_Epf;
+    hipError_t e = hipSuccess;
+    if (c->fw == kVolumeWeighting) e = launch_point_interp(pa, c->stream);
+    if (e == hipSuccess) e = launch_to_float(c->d_E, c->d_Ef, 3 * (int64_t)m.n_cells, c->stream);
+    return check_hip(c, e, "field update");
+}
+
+int do_poisson(pic_ctx* c, pic_poisson_stats* st) {
+    if (!c->has_poisson) return c->fail(PIC_ERR_INVALID, "MaxwellSolver is not ElectroStatic");
+    hipEvent_t e0 = c->ev(), e1 = c->ev();
+    hipEventRecord(e0, c->stream);
+    int iters = 0, refs = 0, status;
+    double rel = 0.0;
+    {
+        pic_ctx::Scope sc(c, PIC_STAGE_POISSON);
+        hipError_t e = poisson_rhs(c->pd, c->d_rho, c->d_b, c->hp.singular, c->pw.red, c->stream);
+        if (int r = check_hip(c, e, "poisson rhs")) return r;
+        status = poisson_solve(c->pd, c->pw, c->d_phi, c->d_b, c->ptol, c->pmaxit, c->pmode, c->hp.singular, &iters,
+                               &refs, &rel, c->stream);
+        GradArgs g{};
+        g.n_cells = c->hm.n_cells;
+        g.gfo = c->d_gfo; g.g_kind = c->d_gkind; g.g_other = c->d_gother;
+        g.g_w = c->d_gw; g.g_val = c->d_gval; g.g_S = c->d_gS;
+        g.V = c->d_V;
+        g.phi = c->d_phi;
+        g.E = c->d_E;
+        g.E_f = c->d_Ef;
+        if (int r = check_hip(c, launch_grad(g, c->stream), "grad")) return r;
+    }
+    if (int r = update_field_derived(c)) return r;
+    hipEventRecord(e1, c->stream);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    c->pool.push_back(e0);
+    c->pool.push_back(e1);
+    if (st) { st->iterations = iters; st->refinements = refs; st->rel_residual = rel; st->ms = ms; }
+    if (status == -2) return c->fail(PIC_ERR_HIP, "poisson kernels failed");
+    if (status == -1) return c->fail(PIC_ERR_NOT_CONVERGED, "Poisson PCG did not reach the tolerance");
+    return PIC_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C-ABI
+
+extern "C" {
+
+const char* pic_version(void) { return "picmi 0.1 (gfx950)"; }
+
+int pic_device_count(int32_t* n) {
+    int d = 0;
+    if (hipGetDeviceCount(&d) != hipSuccess) d = 0;
+    if (n) *n = d;
+    return PIC_OK;
+}
+
+const char* pic_last_error(const pic_ctx* ctx) { return ctx ? ctx->err.c_str() : g_last_error.c_str(); }
+
+void pic_destroy(pic_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) hipFree(p);
+    for (auto& pe : c->pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
+    for (auto e : c->pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_species, const pic_models* models,
+               pic_ctx** out) {
+    if (!out) { g_last_error = "out is null"; return PIC_ERR_INVALID; }
+    *out = nullptr;
+    if (!mesh || !species || !models || n_species <= 0 || n_species > kMaxSpecies) {
+        g_last_error = "invalid arguments to pic_create";
+        return PIC_ERR_INVALID;
+    }
+    std::unique_ptr<pic_ctx, void (*)(pic_ctx*)> c(new pic_ctx(), pic_destroy);
+    int r;
+    int ms = 0;
+    if ((r = parse_model(c.get(), models->particle_pusher, "ParticlePusher",
+                         {"none", "Boris", "BorisNR", "Vay", "HigueraCary"}, &c->pusher)))
+        return r;
+    if ((r = parse_model(c.get(), models->field_weighting, "FieldWeigthing", {"CellAverage", "VolumeWeighting"}, &c->fw)))
+        return r;
+    if ((r = parse_model(c.get(), models->charge_distribution, "ChargeDistribution", {"CellAverage", "VolumeWeighting"},
+                         &c->cd)))
+        return r;
+    if ((r = parse_model(c.get(), models->maxwell_solver, "MaxwellSolver", {"none", "ElectroStatic"}, &ms))) return r;
+    c->electrostatic = ms == 1;
+    int cm = 0;
+    if ((r = parse_model(c.get(), models->coulomb_collision, "CoulombCollisionModel", {"none", "Nanbu"}, &cm))) return r;
+    c->coulomb = cm == 1;
+    if (c->coulomb) {
+        int pa = 0, wc = 0;
+        if ((r = parse_model(c.get(), models->pairing_algorithm ? models->pairing_algorithm : "TakizukaAbe",
+                             "PairingAlgorithm", {"TakizukaAbe"}, &pa)))
+            return r;
+        if ((r = parse_model(c.get(), models->weight_correction ? models->weight_correction : "none",
+                             "WeightCorrectionModel", {"none"}, &wc)))
+            return r;
+    }
+    int en = 0, bgm = 0;
+    if ((r = parse_model(c.get(), models->electron_neutral, "ElectronNeutralCollisionModel",
+                         {"none", "NanbuElectronNeutral"}, &en)))
+        return r;
+    if ((r = parse_model(c.get(), models->background_gas, "BackgroundGasModel", {"none", "uniformBackground"}, &bgm)))
+        return r;
+    c->mcc = en == 1;
+    c->bg = bgm == 1;
+    if (c->mcc && !c->bg) {
+        g_last_error = c->err = "ElectronNeutralCollisionModel needs the uniformBackground gas model (particle neutrals are not implemented)";
+        return PIC_ERR_UNSUPPORTED;
+    }
+    c->precision = models->precision;
+    if (c->precision != PIC_PREC_FP64 && c->precision != PIC_PREC_MIXED) return c->fail(PIC_ERR_INVALID, "precision");
+    c->hc_ref = models->hc_reference_form;
+    c->allow_intra = models->allow_intra_collision;
+    c->calc_bmin = models->calculate_bmin;
+    std::memcpy(c->coulomb_log, models->coulomb_log, sizeof(c->coulomb_log));
+    c->electron = models->electron_species;
+    c->ion = models->ion_species;
+    c->bg_n = models->bg_number_density;
+    c->bg_T = models->bg_temperature;
+    c->bg_m = models->bg_mass;
+    c->bg_w = models->bg_n_equivalent;
+    for (int d = 0; d < 3; ++d) c->bg_v[d] = models->bg_velocity[d];
+    c->create_ion = models->create_ion;
+    c->create_electron = models->create_electron;
+    c->seed = models->seed;
+    c->ptol = models->poisson_tolerance > 0 ? models->poisson_tolerance : 1e-8;
+    c->pmaxit = models->poisson_max_iter > 0 ? models->poisson_max_iter : 10000;
+    c->pmode = models->poisson_mode;
+    c->sort_interval = models->sort_interval > 0 ? models->sort_interval : 1;
+    if (c->mcc) {
+        if (c->electron < 0 || c->electron >= n_species) return c->fail(PIC_ERR_INVALID, "electronTypeId not set");
+        if (c->ion >= n_species) return c->fail(PIC_ERR_INVALID, "ion species out of range");
+        if (!(c->bg_m > 0 && c->bg_w > 0)) return c->fail(PIC_ERR_INVALID, "background gas mass / weight");
+        if (c->ion >= 0 && species[c->ion].n_equivalent != c->bg_w)
+            return c->fail(PIC_ERR_UNSUPPORTED, "created ions carry the background weight; the ion species weight must equal it");
+        if (species[c->electron].n_equivalent != c->bg_w && c->create_electron)
+            return c->fail(PIC_ERR_UNSUPPORTED, "created electrons carry the background weight; the electron weight must equal it");
+    }
+    // the model selection above is validated before any device is touched
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_last_error = "no HIP device available";
+        return PIC_ERR_NO_DEVICE;
+    }
+    hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_last_error = "hipStreamCreate failed";
+        return PIC_ERR_HIP;
+    }
+    // mesh
+    HostMesh& m = c->hm;
+    m.n_points = mesh->n_points; m.n_faces = mesh->n_faces; m.n_internal = mesh->n_internal_faces;
+    m.n_cells = mesh->n_cells; m.n_patches = mesh->n_patches;
+    if (m.n_points <= 0 || m.n_faces <= 0 || m.n_cells <= 0 || m.n_internal < 0 || m.n_internal > m.n_faces)
+        return c->fail(PIC_ERR_MESH, "empty or inconsistent mesh sizes");
+    m.points.assign(mesh->points, mesh->points + 3 * (size_t)m.n_points);
+    m.fo.assign(mesh->face_offsets, mesh->face_offsets + m.n_faces + 1);
+    if (m.fo[0] != 0) return c->fail(PIC_ERR_MESH, "face_offsets[0] must be 0");
+    m.fp.assign(mesh->face_points, mesh->face_points + m.fo[m.n_faces]);
+    m.owner.assign(mesh->owner, mesh->owner + m.n_faces);
+    m.nbr.assign(mesh->neighbour, mesh->neighbour + m.n_internal);
+    for (int p = 0; p < m.n_patches; ++p) {
+        const pic_patch& pp = mesh->patches[p];
+        if (pp.type == PIC_PATCH_WALL) return c->fail(PIC_ERR_UNSUPPORTED, "wall patches (WallReflectionModel) are not implemented");
+        m.ptype.push_back(pp.type); m.pstart.push_back(pp.start); m.psize.push_back(pp.size);
+        m.pnbr.push_back(pp.neighbour_patch); m.phi_bc.push_back(pp.phi_bc); m.phi_val.push_back(pp.phi_value);
+    }
+    std::string merr = build_host_mesh(m);
+    if (!merr.empty()) return c->fail(PIC_ERR_MESH, merr);
+    if (c->electrostatic) {
+        merr = build_host_poisson(m, c->hp);
+        if (!merr.empty()) return c->fail(PIC_ERR_MESH, merr);
+        c->has_poisson = true;
+    }
+    // species
+    double qmax = 0.0;
+    for (int s = 0; s < n_species; ++s) {
+        Species S;
+        S.mass = species[s].mass;
+        S.charge = species[s].charge;
+        S.weight = species[s].n_equivalent;
+        S.solve_movement = species[s].solve_movement;
+        S.capacity = species[s].capacity;
+        if (!(S.mass > 0) || !(S.weight > 0) || S.capacity < 0 || S.capacity > 2000000000LL)
+            return c->fail(PIC_ERR_INVALID, "species " + std::to_string(s) + ": mass, weight and capacity must be positive (capacity < 2^31)");
+        qmax = std::max(qmax, std::fabs(S.charge * S.weight));
+        c->sp.push_back(S);
+    }
+    if (c->coulomb) {
+        double w0 = -1;
+        for (auto& s : c->sp) if (s.charge != 0.0) { if (w0 < 0) w0 = s.weight; else if (s.weight != w0) return c->fail(PIC_ERR_UNSUPPORTED, "Model TakizukaAbe is only valid for equal weighted particles"); }
+    }
+    c->Q0 = qmax > 0 ? qmax : 1.0;
+    // device mesh
+    const int nc = m.n_cells, np = m.n_points;
+    c->d_tets = c->upload(m.tets);
+    c->d_cell_tet0 = c->upload(m.cell_tet0);
+    c->d_C = c->upload(m.C);
+    c->d_V = c->upload(m.V);
+    c->d_points = c->upload(m.points);
+    c->d_cpo = c->upload(m.cpo);
+    {
+        std::vector<int32_t> cpm(m.cp.size());
+        for (size_t k = 0; k < m.cp.size(); ++k) cpm[k] = m.pmaster[m.cp[k]];
+        c->d_cp_master = c->upload(cpm);
+    }
+    c->d_vw_w = c->upload(m.vw_w);
+    c->d_pco = c->upload(m.pco);
+    c->d_pc_cell = c->upload(m.pc_cell);
+    c->d_pc_w = c->upload(m.pc_w);
+    c->d_pmaster = c->upload(m.pmaster);
+    c->d_E = c->alloc<double>(3 * (size_t)nc);
+    c->d_B = c->alloc<double>(3 * (size_t)nc);
+    c->d_phi = c->alloc<double>(nc);
+    c->d_rho = c->alloc<double>(nc);
+    c->d_space = c->alloc<double>(nc);
+    c->d_Ep = c->alloc<double>(3 * (size_t)np);
+    c->d_Ef = c->alloc<float>(3 * (size_t)nc);
+    c->d_Epf = c->alloc<float>(3 * (size_t)np);
+    c->d_Bf = c->alloc<float>(3 * (size_t)nc);
+    c->d_acc_cell = c->alloc<unsigned long long>(nc);
+    c->d_acc_point = c->alloc<unsigned long long>(np);
+    c->d_flags = c->alloc<uint32_t>(1);
+    c->d_stats = c->alloc<unsigned long long>(16);
+    c->d_n = c->alloc<int64_t>(kMaxSpecies);
+    c->d_maxc = c->alloc<int>(1);
+    c->scan_bytes = scan_temp_bytes(nc + 1);
+    c->d_scan_tmp = c->alloc<char>(c->scan_bytes);
+    c->d_diag_part = c->alloc<double>(1024 * 8);
+    c->d_diag_out = c->alloc<double>(16);
+    if (!c->d_tets || !c->d_E || !c->d_diag_out) return c->fail(PIC_ERR_HIP, "out of device memory (mesh)");
+    if (c->mcc) {
+        c->d_smax = c->alloc<double>(nc);
+        c->d_rem = c->alloc<double>(nc);
+        // remainders start random in [0,1) (ElectronNeutralCollisionModel.C:286-296)
+        std::vector<double> rem(nc);
+        for (int i = 0; i < nc; ++i) {
+            uint64_t x = (uint64_t)i * 0x9E3779B97F4A7C15ull ^ (c->seed + 0x632BE59BD9B4E019ull);
+            x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
+            rem[i] = (double)(x >> 11) * (1.0 / 9007199254740992.0);
+        }
+        hipMemcpyAsync(c->d_rem, rem.data(), sizeof(double) * nc, hipMemcpyHostToDevice, c->stream);
+    }
+    for (auto& S : c->sp)
+        if ((r = alloc_species(c.get(), S))) return r;
+    // Poisson device tables
+    if (c->has_poisson) {
+        HostPoisson& P = c->hp;
+        std::vector<float> af(P.a.begin(), P.a.end()), df(P.diag.begin(), P.diag.end()), rdf(P.rD.begin(), P.rD.end());
+        c->pd.n = P.n; c->pd.K = P.K; c->pd.ncol = P.ncol;
+        c->pd.nb = c->upload(P.nb);
+        c->pd.a = c->upload(P.a);
+        c->pd.a_f = c->upload(af);
+        c->pd.diag = c->upload(P.diag);
+        c->pd.diag_f = c->upload(df);
+        c->pd.rD = c->upload(P.rD);
+        c->pd.rD_f = c->upload(rdf);
+        c->pd.colour = c->upload(P.colour);
+        c->pd.corder = c->upload(P.corder);
+        c->pd.col_start_h = P.col_start.data();
+        c->pd.bsrc = c->upload(P.bsrc);
+        c->pd.V = c->d_V;
+        c->pw.r = c->alloc<double>(nc); c->pw.z = c->alloc<double>(nc); c->pw.p = c->alloc<double>(nc);
+        c->pw.q = c->alloc<double>(nc); c->pw.x = c->alloc<double>(nc); c->pw.b = c->alloc<double>(nc);
+        c->pw.rf = c->alloc<float>(nc); c->pw.zf = c->alloc<float>(nc); c->pw.pf = c->alloc<float>(nc);
+        c->pw.qf = c->alloc<float>(nc); c->pw.xf = c->alloc<float>(nc);
+        c->pw.red = c->alloc<double>(1024 + 64);
+        c->pw.scal = c->alloc<double>(64);
+        c->d_b = c->alloc<double>(nc);
+        c->d_gfo = c->upload(P.gfo);
+        c->d_gkind = c->upload(P.g_kind);
+        c->d_gother = c->upload(P.g_other);
+        c->d_gw = c->upload(P.g_w);
+        c->d_gval = c->upload(P.g_val);
+        c->d_gS = c->upload(P.g_S);
+        if (!c->pw.scal || !c->d_gS) return c->fail(PIC_ERR_HIP, "out of device memory (poisson)");
+    }
+    if ((r = check_hip(c.get(), hipStreamSynchronize(c->stream), "create"))) return r;
+    *out = c.release();
+    return PIC_OK;
+}
+
+int pic_mesh_sizes(const pic_ctx* c, int64_t* nt, int32_t* nc, int32_t* np) {
+    if (!c) return PIC_ERR_INVALID;
+    if (nt) *nt = (int64_t)c->hm.tet_cell.size();
+    if (nc) *nc = c->hm.n_cells;
+    if (np) *np = c->hm.n_points;
+    return PIC_OK;
+}
+
+int pic_mesh_tets(const pic_ctx* c, int32_t* tc, int32_t* tf, int32_t* tp) {
+    if (!c) return PIC_ERR_INVALID;
+    const size_t n = c->hm.tet_cell.size();
+    if (tc) std::memcpy(tc, c->hm.tet_cell.data(), n * sizeof(int32_t));
+    if (tf) std::memcpy(tf, c->hm.tet_face.data(), n * sizeof(int32_t));
+    if (tp) std::memcpy(tp, c->hm.tet_pt.data(), n * sizeof(int32_t));
+    return PIC_OK;
+}
+
+int pic_mesh_geometry(const pic_ctx* c, double* cc, double* cv) {
+    if (!c) return PIC_ERR_INVALID;
+    if (cc) std::memcpy(cc, c->hm.C.data(), c->hm.C.size() * sizeof(double));
+    if (cv) std::memcpy(cv, c->hm.V.data(), c->hm.V.size() * sizeof(double));
+    return PIC_OK;
+}
+
+int pic_set_cell_field(pic_ctx* c, int32_t field, const double* v) {
+    if (!c || !v) return PIC_ERR_INVALID;
+    const int nc = c->hm.n_cells;
+    switch (field) {
+        case PIC_FIELD_E:
+            hipMemcpyAsync(c->d_E, v, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, c->stream);
+            if (int r = update_field_derived(c)) return r;
+            break;
+        case PIC_FIELD_B: {
+            hipMemcpyAsync(c->d_B, v, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, c->stream);
+            std::vector<float> f(3 * (size_t)nc);
+            bool any = false;
+            for (size_t i = 0; i < f.size(); ++i) { f[i] = (float)v[i]; if (v[i] != 0.0) any = true; }
+            hipMemcpyAsync(c->d_Bf, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice, c->stream);
+            // BorisPusher.C:42-55: the rotation is skipped when B is zero everywhere
+            c->rotate = any || c->pusher == kPushVay;
+            hipStreamSynchronize(c->stream);
+            break;
+        }
+        case PIC_FIELD_PHI:
+            hipMemcpyAsync(c->d_phi, v, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream);
+            break;
+        case PIC_FIELD_SPACE_CHARGE:
+            hipMemcpyAsync(c->d_space, v, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream);
+            c->has_space = true;
+            break;
+        case PIC_FIELD_MCC_SIGMA_CR_MAX:
+        case PIC_FIELD_MCC_REMAINDER:
+            if (!c->mcc) return c->fail(PIC_ERR_INVALID, "no electron-neutral collision model");
+            hipMemcpyAsync(field == PIC_FIELD_MCC_REMAINDER ? c->d_rem : c->d_smax, v, sizeof(double) * nc,
+                           hipMemcpyHostToDevice, c->stream);
+            break;
+        default:
+            return c->fail(PIC_ERR_INVALID, "field cannot be set");
+    }
+    return check_hip(c, hipStreamSynchronize(c->stream), "set field");
+}
+
+int pic_get_cell_field(pic_ctx* c, int32_t field, double* v) {
+    if (!c || !v) return PIC_ERR_INVALID;
+    const int nc = c->hm.n_cells;
+    const double* src = nullptr;
+    size_t n = 0;
+    switch (field) {
+        case PIC_FIELD_E: src = c->d_E; n = 3 * (size_t)nc; break;
+        case PIC_FIELD_B: src = c->d_B; n = 3 * (size_t)nc; break;
+        case PIC_FIELD_PHI: src = c->d_phi; n = nc; break;
+        case PIC_FIELD_RHO: src = c->d_rho; n = nc; break;
+        case PIC_FIELD_SPACE_CHARGE: src = c->d_space; n = nc; break;
+        case PIC_FIELD_E_POINT: src = c->d_Ep; n = 3 * (size_t)c->hm.n_points; break;
+        case PIC_FIELD_MCC_SIGMA_CR_MAX:
+        case PIC_FIELD_MCC_REMAINDER:
+            if (!c->mcc) return c->fail(PIC_ERR_INVALID, "no electron-neutral collision model");
+            src = field == PIC_FIELD_MCC_REMAINDER ? c->d_rem : c->d_smax;
+            n = nc;
+            break;
+        default: return c->fail(PIC_ERR_INVALID, "unknown field");
+    }
+    hipMemcpyAsync(v, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+    return check_hip(c, hipStreamSynchronize(c->stream), "get field");
+}
+
+int pic_set_cross_section(pic_ctx* c, int32_t process, int32_t n, const double* e, const double* s, double th) {
+    if (!c || process < 0 || process > 2 || n < 0 || (n > 0 && (!e || !s))) return PIC_ERR_INVALID;
+    for (int i = 1; i < n; ++i)
+        if (!(e[i] > e[i - 1])) return c->fail(PIC_ERR_INVALID, "cross-section energies must be strictly ascending");
+    c->xs_he[process].assign(e, e + n);
+    c->xs_hs[process].assign(s, s + n);
+    c->xs_n[process] = n;
+    c->xs_th[process] = th;
+    c->d_xs_e[process] = c->upload(c->xs_he[process]);
+    c->d_xs_s[process] = c->upload(c->xs_hs[process]);
+    return check_hip(c, hipStreamSynchronize(c->stream), "set cross section");
+}
+
+int pic_mcc_initialize(pic_ctx* c, double emax, double estep) {
+    if (!c || !c->mcc) return c ? c->fail(PIC_ERR_INVALID, "no electron-neutral model") : PIC_ERR_INVALID;
+    if (!(estep > 0)) return c->fail(PIC_ERR_INVALID, "energy step");
+    auto lookup = [&](int k, double x) -> double {
+        const int n = c->xs_n[k];
+        if (n == 0) return 0.0;
+        const double* xs = c->xs_he[k].data();
+        const double* ys = c->xs_hs[k].data();
+        if (x <= xs[0]) return ys[0];
+        if (x >= xs[n - 1]) return ys[n - 1];
+        int lo = (int)(std::upper_bound(xs, xs + n, x) - xs) - 1;
+        if (xs[lo] == x) return ys[lo];
+        return ys[lo] + ((x - xs[lo]) / (xs[lo + 1] - xs[lo])) * (ys[lo + 1] - ys[lo]);
+    };
+    const double mE = c->sp[c->electron].mass;
+    double smax = 0.0;
+    for (double eV = 0.0; eV < emax; eV += estep) {
+        double q = lookup(0, eV) + lookup(1, eV) + lookup(2, eV);
+        smax = std::max(smax, std::sqrt(2.0 * kQe * eV / mE) * q);
+    }
+    std::vector<double> v(c->hm.n_cells, smax);
+    hipMemcpyAsync(c->d_smax, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, c->stream);
+    return check_hip(c, hipStreamSynchronize(c->stream), "mcc initialize");
+}
+
+int pic_particles_set(pic_ctx* c, int32_t s, int64_t n, const double* bary, const int32_t* tet, const double* U,
+                      const uint32_t* id) {
+    if (!c || s < 0 || s >= (int)c->sp.size() || n < 0 || (n > 0 && (!bary || !tet || !U || !id))) return PIC_ERR_INVALID;
+    Species& S = c->sp[s];
+    if (n > S.capacity) return c->fail(PIC_ERR_CAPACITY, "more parcels than the species capacity");
+    const int64_t nt = (int64_t)c->hm.tet_cell.size();
+    for (int64_t i = 0; i < n; ++i)
+        if (tet[i] < -1 || tet[i] >= nt) return c->fail(PIC_ERR_INVALID, "tet index out of range");
+    std::vector<double> comp(n);
+    for (int k = 0; k < 3; ++k) {
+        for (int64_t i = 0; i < n; ++i) comp[i] = bary[3 * i + k];
+        hipMemcpy(S.y[S.cur][k], comp.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (c->mixed()) {
+            std::vector<float> f(n);
+            for (int64_t i = 0; i < n; ++i) f[i] = (float)U[3 * i + k];
+            hipMemcpy(S.u[S.cur][k], f.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+        } else {
+            for (int64_t i = 0; i < n; ++i) comp[i] = U[3 * i + k];
+            hipMemcpy(S.u[S.cur][k], comp.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+        }
+    }
+    hipMemcpy(S.tet[S.cur], tet, sizeof(int32_t) * n, hipMemcpyHostToDevice);
+    hipMemcpy(S.id[S.cur], id, sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+    S.n = n;
+    set_dev_n(c, s);
+    c->sorted = false;
+    c->counts_valid = false;
+    return check_hip(c, hipStreamSynchronize(c->stream), "particles set");
+}
+
+int pic_particles_set_positions(pic_ctx* c, int32_t s, int64_t n, const double* x, const int32_t* cell,
+                                const double* U, const uint32_t* id) {
+    if (!c || s < 0 || s >= (int)c->sp.size() || n < 0 || (n > 0 && (!x || !cell || !U || !id))) return PIC_ERR_INVALID;
+    Species& S = c->sp[s];
+    if (n > S.capacity) return c->fail(PIC_ERR_CAPACITY, "more parcels than the species capacity");
+    for (int64_t i = 0; i < n; ++i)
+        if (cell[i] < 0 || cell[i] >= c->hm.n_cells) return c->fail(PIC_ERR_INVALID, "cell index out of range");
+    double* dx = nullptr; int32_t* dc = nullptr; double* dU = nullptr; uint32_t* did = nullptr; unsigned* dfail = nullptr;
+    const size_t nn = std::max<int64_t>(n, 1);
+    if (hipMalloc(&dx, nn * 3 * sizeof(double)) || hipMalloc(&dc, nn * sizeof(int32_t)) ||
+        hipMalloc(&dU, nn * 3 * sizeof(double)) || hipMalloc(&did, nn * sizeof(uint32_t)) ||
+        hipMalloc(&dfail, sizeof(unsigned)))
+        return c->fail(PIC_ERR_HIP, "out of device memory (locate)");
+    hipMemcpy(dx, x, n * 3 * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(dc, cell, n * sizeof(int32_t), hipMemcpyHostToDevice);
+    hipMemcpy(dU, U, n * 3 * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(did, id, n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    hipMemset(dfail, 0, sizeof(unsigned));
+    hipError_t e;
+    if (c->mixed()) {
+        LocateArgs<float> a{soa<float>(S, S.cur), n, 0, dx, dc, dU, did, c->d_tets, c->d_cell_tet0, c->d_C, dfail};
+        e = launch_locate(a, c->stream);
+    } else {
+        LocateArgs<double> a{soa<double>(S, S.cur), n, 0, dx, dc, dU, did, c->d_tets, c->d_cell_tet0, c->d_C, dfail};
+        e = launch_locate(a, c->stream);
+    }
+    unsigned fails = 0;
+    hipStreamSynchronize(c->stream);
+    hipMemcpy(&fails, dfail, sizeof(unsigned), hipMemcpyDeviceToHost);
+    hipFree(dx); hipFree(dc); hipFree(dU); hipFree(did); hipFree(dfail);
+    if (int r = check_hip(c, e, "locate")) return r;
+    S.n = n;
+    set_dev_n(c, s);
+    c->sorted = false;
+    c->counts_valid = false;
+    if (fails) return c->fail(PIC_ERR_INVALID, std::to_string(fails) + " positions lie outside their given cell");
+    return check_hip(c, hipStreamSynchronize(c->stream), "particles set positions");
+}
+
+int pic_particles_count(pic_ctx* c, int32_t s, int64_t* n) {
+    if (!c || s < 0 || s >= (int)c->sp.size() || !n) return PIC_ERR_INVALID;
+    *n = c->sp[s].n;
+    return PIC_OK;
+}
+
+int pic_particles_get(pic_ctx* c, int32_t s, int64_t cap, int64_t* nout, double* bary, int32_t* tet, double* U,
+                      uint32_t* id) {
+    if (!c || s < 0 || s >= (int)c->sp.size()) return PIC_ERR_INVALID;
+    if (int r = check_hip(c, hipStreamSynchronize(c->stream), "particles get")) return r;
+    Species& S = c->sp[s];
+    const int64_t n = S.n;
+    if (nout) *nout = n;
+    if (n > cap) return c->fail(PIC_ERR_CAPACITY, "output buffer too small");
+    std::vector<double> comp(n);
+    if (bary)
+        for (int k = 0; k < 3; ++k) {
+            hipMemcpy(comp.data(), S.y[S.cur][k], sizeof(double) * n, hipMemcpyDeviceToHost);
+            for (int64_t i = 0; i < n; ++i) bary[3 * i + k] = comp[i];
+        }
+    if (U)
+        for (int k = 0; k < 3; ++k) {
+            if (c->mixed()) {
+                std::vector<float> f(n);
+                hipMemcpy(f.data(), S.u[S.cur][k], sizeof(float) * n, hipMemcpyDeviceToHost);
+                for (int64_t i = 0; i < n; ++i) U[3 * i + k] = f[i];
+            } else {
+                hipMemcpy(comp.data(), S.u[S.cur][k], sizeof(double) * n, hipMemcpyDeviceToHost);
+                for (int64_t i = 0; i < n; ++i) U[3 * i + k] = comp[i];
+            }
+        }
+    if (tet) hipMemcpy(tet, S.tet[S.cur], sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+    if (id) hipMemcpy(id, S.id[S.cur], sizeof(uint32_t) * n, hipMemcpyDeviceToHost);
+    return check_hip(c, hipGetLastError(), "particles get");
+}
+
+int pic_particles_positions(pic_ctx* c, int32_t s, int64_t cap, double* x) {
+    if (!c || s < 0 || s >= (int)c->sp.size() || !x) return PIC_ERR_INVALID;
+    Species& S = c->sp[s];
+    if (S.n > cap) return c->fail(PIC_ERR_CAPACITY, "output buffer too small");
+    double* dx = nullptr;
+    if (hipMalloc(&dx, std::max<int64_t>(S.n, 1) * 3 * sizeof(double))) return c->fail(PIC_ERR_HIP, "alloc");
+    hipError_t e = c->mixed() ? launch_positions(soa<float>(S, S.cur), S.n, c->d_tets, c->d_C, c->d_points, dx, c->stream)
+                              : launch_positions(soa<double>(S, S.cur), S.n, c->d_tets, c->d_C, c->d_points, dx, c->stream);
+    hipStreamSynchronize(c->stream);
+    hipMemcpy(x, dx, S.n * 3 * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(dx);
+    return check_hip(c, e, "positions");
+}
+
+int pic_init_maxwellian(pic_ctx* c, int32_t s, int32_t nppc, double T, const double* drift, uint32_t id0,
+                        uint64_t seed) {
+    if (!c || s < 0 || s >= (int)c->sp.size() || nppc < 0) return PIC_ERR_INVALID;
+    Species& S = c->sp[s];
+    const int64_t add = (int64_t)nppc * c->hm.n_cells;
+    if (S.n + add > S.capacity) return c->fail(PIC_ERR_CAPACITY, "more parcels than the species capacity");
+    hipError_t e;
+    if (c->mixed()) {
+        InitArgs<float> a{soa<float>(S, S.cur), add, S.n, nppc, c->d_tets, c->d_cell_tet0, std::sqrt(kKB * T / S.mass),
+                          {drift ? drift[0] : 0.0, drift ? drift[1] : 0.0, drift ? drift[2] : 0.0}, id0, seed};
+        e = launch_init(a, c->stream);
+    } else {
+        InitArgs<double> a{soa<double>(S, S.cur), add, S.n, nppc, c->d_tets, c->d_cell_tet0, std::sqrt(kKB * T / S.mass),
+                           {drift ? drift[0] : 0.0, drift ? drift[1] : 0.0, drift ? drift[2] : 0.0}, id0, seed};
+        e = launch_init(a, c->stream);
+    }
+    if (int r = check_hip(c, e, "init")) return r;
+    S.n += add;
+    set_dev_n(c, s);
+    c->sorted = false;
+    c->counts_valid = false;
+    return check_hip(c, hipStreamSynchronize(c->stream), "init");
+}
+
+int pic_push(pic_ctx* c, double dt) {
+    if (!c) return PIC_ERR_INVALID;
+    for (int s = 0; s < (int)c->sp.size(); ++s) {
+        if (!c->sp[s].solve_movement || c->sp[s].charge == 0.0) continue;
+        hipError_t e = c->mixed() ? launch_push(move_args<float>(c, s, dt), c->pusher, c->fw, c->stream)
+                                  : launch_push(move_args<double>(c, s, dt), c->pusher, c->fw, c->stream);
+        if (int r = check_hip(c, e, "push")) return r;
+    }
+    return sync_state(c);
+}
+
+int pic_move(pic_ctx* c, double dt) {
+    if (!c) return PIC_ERR_INVALID;
+    if (int r = do_move(c, dt)) return r;
+    return sync_state(c);
+}
+
+int pic_sort(pic_ctx* c) {
+    if (!c) return PIC_ERR_INVALID;
+    return do_sort(c, nullptr);
+}
+
+int pic_collide_coulomb(pic_ctx* c, double dt) {
+    if (!c || !c->coulomb) return c ? c->fail(PIC_ERR_INVALID, "CoulombCollisionModel is none") : PIC_ERR_INVALID;
+    int maxc = 0;
+    if (int r = do_sort(c, &maxc)) return r;
+    if (int r = do_coulomb(c, dt, maxc)) return r;
+    return sync_state(c);
+}
+
+int pic_collide_mcc(pic_ctx* c, double dt) {
+    if (!c || !c->mcc) return c ? c->fail(PIC_ERR_INVALID, "ElectronNeutralCollisionModel is none") : PIC_ERR_INVALID;
+    int maxc = 0;
+    if (int r = do_sort(c, &maxc)) return r;
+    if (int r = do_mcc(c, dt, maxc)) return r;
+    return sync_state(c);
+}
+
+int pic_deposit(pic_ctx* c) {
+    if (!c) return PIC_ERR_INVALID;
+    zero_deposit(c);
+    for (int s = 0; s < (int)c->sp.size(); ++s) {
+        hipError_t e = c->mixed() ? launch_deposit(move_args<float>(c, s, 0.0), c->cd, c->stream)
+                                  : launch_deposit(move_args<double>(c, s, 0.0), c->cd, c->stream);
+        if (int r = check_hip(c, e, "deposit")) return r;
+    }
+    c->counts_valid = true;
+    if (int r = do_deposit_finish(c)) return r;
+    return sync_state(c);
+}
+
+int pic_poisson_solve(pic_ctx* c, pic_poisson_stats* st) {
+    if (!c) return PIC_ERR_INVALID;
+    return do_poisson(c, st);
+}
+
+int pic_step(pic_ctx* c, double dt, pic_step_stats* st) {
+    if (!c) return PIC_ERR_INVALID;
+    hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * 16, c->stream);
+    if (int r = do_move(c, dt)) return r;
+    const bool need_sort = c->coulomb || c->mcc || ((c->step + 1) % c->sort_interval == 0);
+    int maxc = 0;
+    if (need_sort)
+        if (int r = do_sort(c, &maxc)) return r;
+    if (c->coulomb)
+        if (int r = do_coulomb(c, dt, maxc)) return r;
+    if (c->mcc) {
+        // cell capacity for the electron species only
+        if (int r = do_mcc(c, dt, maxc)) return r;
+    }
+    if (int r = do_deposit_finish(c)) return r;
+    pic_poisson_stats ps{};
+    if (c->electrostatic)
+        if (int r = do_poisson(c, &ps)) return r;
+    if (int r = sync_state(c)) return r;
+    c->step++;
+    if (st) {
+        unsigned long long h[16];
+        hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost);
+        std::memset(st, 0, sizeof(*st));
+        for (size_t s = 0; s < c->sp.size(); ++s) st->n_parcels[s] = c->sp[s].n;
+        st->n_deleted = (int64_t)h[0];
+        st->n_track_guard = (int64_t)h[1];
+        st->coulomb_collisions = (int64_t)h[2];
+        st->mcc_candidates = (int64_t)h[3];
+        st->mcc_elastic = (int64_t)h[4];
+        st->mcc_excitation = (int64_t)h[5];
+        st->mcc_ionization = (int64_t)h[6];
+        st->poisson = ps;
+    }
+    return PIC_OK;
+}
+
+int pic_diag_reduce(pic_ctx* c, pic_diag* out) {
+    if (!c || !out) return PIC_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    for (int s = 0; s < (int)c->sp.size(); ++s) {
+        Species& S = c->sp[s];
+        int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (S.n + 255) / 256));
+        hipError_t e = c->mixed() ? launch_diag(soa<float>(S, S.cur), S.n, c->d_diag_part, nb, c->d_diag_out, c->stream)
+                                  : launch_diag(soa<double>(S, S.cur), S.n, c->d_diag_part, nb, c->d_diag_out, c->stream);
+        if (int r = check_hip(c, e, "diag")) return r;
+        double h[6];
+        hipMemcpyAsync(h, c->d_diag_out, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+        hipStreamSynchronize(c->stream);
+        const double w = S.weight;
+        out->n_parcels[s] = (int64_t)h[0];
+        out->sum_w[s] = h[0] * w;
+        for (int d = 0; d < 3; ++d) out->sum_wU[s][d] = h[1 + d] * w;
+        out->sum_wU2[s] = h[4] * w;
+        out->kinetic_energy[s] = 0.5 * S.mass * h[4] * w;
+    }
+    {
+        int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (c->hm.n_cells + 255) / 256));
+        hipError_t e = launch_field_energy(c->d_E, c->d_V, c->hm.n_cells, c->d_diag_part, nb, c->d_diag_out, c->stream);
+        if (int r = check_hip(c, e, "field energy")) return r;
+        double fe = 0;
+        hipMemcpyAsync(&fe, c->d_diag_out, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        hipStreamSynchronize(c->stream);
+        out->field_energy = 0.5 * kEps0 * fe;
+    }
+    return PIC_OK;
+}
+
+int pic_sync(pic_ctx* c) {
+    if (!c) return PIC_ERR_INVALID;
+    return check_hip(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+int64_t pic_step_index(const pic_ctx* c) { return c ? c->step : -1; }
+int pic_set_step_index(pic_ctx* c, int64_t s) {
+    if (!c) return PIC_ERR_INVALID;
+    c->step = s;
+    return PIC_OK;
+}
+
+int pic_profile_enable(pic_ctx* c, int32_t on) {
+    if (!c) return PIC_ERR_INVALID;
+    c->profile = on != 0;
+    return PIC_OK;
+}
+
+int pic_profile_reset(pic_ctx* c) {
+    if (!c) return PIC_ERR_INVALID;
+    hipStreamSynchronize(c->stream);
+    for (auto& pe : c->pending) { c->pool.push_back(pe.a); c->pool.push_back(pe.b); }
+    c->pending.clear();
+    std::memset(c->stage_ms, 0, sizeof(c->stage_ms));
+    std::memset(c->stage_n, 0, sizeof(c->stage_n));
+    return PIC_OK;
+}
+
+int pic_stage_times(pic_ctx* c, double* ms, int64_t* n) {
+    if (!c) return PIC_ERR_INVALID;
+    hipStreamSynchronize(c->stream);
+    for (auto& pe : c->pending) {
+        float t = 0.f;
+        hipEventElapsedTime(&t, pe.a, pe.b);
+        c->stage_ms[pe.stage] += t;
+        c->stage_n[pe.stage] += 1;
+        c->pool.push_back(pe.a);
+        c->pool.push_back(pe.b);
+    }
+    c->pending.clear();
+    for (int k = 0; k < PIC_STAGE_COUNT; ++k) {
+        if (ms) ms[k] = c->stage_ms[k];
+        if (n) n[k] = c->stage_n[k];
+    }
+    return PIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// Device-side helpers shared by the libpicmi kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "internal.h"
+
+namespace picmi {
+
+template <class T>
+struct vec3 {
+    T x, y, z;
+};
+template <class T>
+__device__ __forceinline__ vec3<T> mk3(T x, T y, T z) { return {x, y, z}; }
+template <class T>
+__device__ __forceinline__ vec3<T> operator+(vec3<T> a, vec3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T>
+__device__ __forceinline__ vec3<T> operator-(vec3<T> a, vec3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T>
+__device__ __forceinline__ vec3<T> operator*(T s, vec3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class T>
+__device__ __forceinline__ vec3<T> operator/(vec3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class T>
+__device__ __forceinline__ T dot(vec3<T> a, vec3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T>
+__device__ __forceinline__ vec3<T> cross(vec3<T> a, vec3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T>
+__device__ __forceinline__ T mag(vec3<T> a) { return sqrt(dot(a, a)); }
+
+// Philox4x32-10 (Salmon et al., SC'11): counter (4 x u32), key (2 x u32).
+__device__ __forceinline__ uint4 philox4x32(uint4 c, uint2 k) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        uint4 n;
+        n.x = hi1 ^ c.y ^ k.x;
+        n.y = lo1;
+        n.z = hi0 ^ c.w ^ k.y;
+        n.w = lo0;
+        c = n;
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+
+__device__ __forceinline__ uint4 rng4(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    return philox4x32(make_uint4(a, b, c, d), make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
+}
+
+// 53-bit uniform in [0,1) from two words
+__device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
+    uint64_t x = ((((uint64_t)a) << 32) | b) >> 11;
+    return (double)x * (1.0 / 9007199254740992.0);
+}
+
+constexpr uint32_t kTagShuffle = 1u, kTagCoulomb = 2u, kTagMcc = 3u, kTagNewId = 4u, kTagInit = 5u;
+
+// error flags raised by kernels (bitmask in device memory)
+enum : uint32_t {
+    kFlagSuperluminal = 1u,
+    kFlagCellOverflow = 2u,
+    kFlagCapacity = 4u,
+    kFlagTrackBroken = 8u,
+};
+
+// Particle arrays of one species (structure of arrays).
+template <class UT>
+struct PartSoA {
+    double* y1;
+    double* y2;
+    double* y3;
+    UT* ux;
+    UT* uy;
+    UT* uz;
+    int32_t* tet;
+    uint32_t* id;
+};
+
+// Wave-aggregated atomic increment: lanes with equal keys (neighbouring lanes usually
+// share a cell because parcels are kept sorted by cell) issue one atomic per distinct key.
+// Returns this lane's slot (old value + rank among the equal-key lanes).  key < 0 lanes
+// take no part.  Must be called by all lanes of the wave.
+__device__ __forceinline__ int wave_aggregated_inc(int* counter, int key) {
+    const int lane = __lane_id();
+    uint64_t active = __ballot(key >= 0);
+    int result = -1;
+    while (active) {
+        int leader = __ffsll((unsigned long long)active) - 1;
+        int lk = __shfl(key, leader);
+        uint64_t mask = __ballot(key == lk) & active;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(counter + lk, __popcll(mask));
+        base = __shfl(base, leader);
+        if ((mask >> lane) & 1ull) {
+            uint64_t lower = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+            result = base + __popcll(lower);
+        }
+        active &= ~mask;
+    }
+    return result;
+}
+
+}  // namespace picmi

@@ -1,0 +1,254 @@
+// Grid-side kernels of libpicmi: exclusive scan of cell counts, charge-density
+// finalisation, Gauss gradient, cell->point interpolation, diagnostics reductions.
+#include "device.h"
+#include "kernels.h"
+
+namespace picmi {
+
+// ---------------------------------------------------------------- scan
+constexpr int kScanBlock = 1024;
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_block(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                           int* __restrict__ totals) {
+    __shared__ int s[kScanBlock];
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kScanBlock + tid;
+    int v = (i < n) ? in[i] : 0;
+    s[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        int add = (tid >= off) ? s[tid - off] : 0;
+        __syncthreads();
+        s[tid] += add;
+        __syncthreads();
+    }
+    if (i < n) out[i] = s[tid] - v;
+    if (tid == kScanBlock - 1) totals[blockIdx.x] = s[tid];
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_totals(int* totals, int nb) {
+    __shared__ int s[kScanBlock];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += kScanBlock) {
+        const int tid = threadIdx.x;
+        int v = (base + tid < nb) ? totals[base + tid] : 0;
+        s[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < kScanBlock; off <<= 1) {
+            int add = (tid >= off) ? s[tid - off] : 0;
+            __syncthreads();
+            s[tid] += add;
+            __syncthreads();
+        }
+        if (base + tid < nb) totals[base + tid] = carry + s[tid] - v;
+        __syncthreads();
+        if (tid == kScanBlock - 1) carry += s[tid];
+        __syncthreads();
+    }
+}
+
+__global__ void k_scan_add(int* out, int n, const int* totals) {
+    const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    if (i < n) out[i] += totals[blockIdx.x];
+}
+
+size_t scan_temp_bytes(int n) { return sizeof(int) * (size_t)((n + kScanBlock - 1) / kScanBlock + 1); }
+
+hipError_t launch_scan(const int* in, int* out, int n, void* temp, size_t temp_bytes, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int nb = (n + kScanBlock - 1) / kScanBlock;
+    if (temp_bytes < scan_temp_bytes(n)) return hipErrorInvalidValue;
+    int* totals = (int*)temp;
+    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, in, out, n, totals);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanBlock), 0, s, totals, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, s, out, n, totals);
+    return hipGetLastError();
+}
+
+__global__ void k_max_count(const int* __restrict__ c, int n, int* out) {
+    int m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, c[i]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off));
+    if (__lane_id() == 0) atomicMax(out, m);
+}
+
+hipError_t launch_max_count(const int* counts, int n, int* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int nb = (n + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(k_max_count, dim3(nb), dim3(256), 0, s, counts, n, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- charge density
+// CellAverage.C:66-68 and VolumeWeighting.C:127-164 from the int64 accumulators;
+// CellAverage sums are count * llrint(q w/Q0 2^42) per species, the same integers the
+// per-parcel accumulation produces.
+__global__ void k_deposit_finish(DepositFinishArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n_cells) return;
+    long long acc;
+    if (a.weighting == kCellAverage) {
+        acc = 0;
+        for (int s = 0; s < a.n_species; ++s)
+            if (a.counts[s]) acc += (long long)a.counts[s][c] * a.whole[s];
+    } else {
+        acc = (long long)a.acc_cell[c];
+    }
+    const double V = a.V[c];
+    double r = ((double)acc * a.q42) / V;
+    if (a.weighting == kVolumeWeighting)
+        for (int k = a.cpo[c]; k < a.cpo[c + 1]; ++k)
+            r = r + a.vw_w[k] * ((double)(long long)a.acc_point[a.cp_master[k]] * a.q42) / V;
+    if (a.space_charge) r = r + a.space_charge[c];
+    a.rho[c] = r;
+}
+
+hipError_t launch_deposit_finish(const DepositFinishArgs& a, hipStream_t s) {
+    if (a.n_cells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_deposit_finish, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- Gauss gradient
+// fvc::grad Gauss linear (ElectroStaticSolver.C:68): grad_c = (1/V) sum_f S_f phi_f, faces
+// in ascending order; E = -grad.
+__global__ void k_grad(GradArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n_cells) return;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    const double ps = a.phi[c];
+    for (int k = a.gfo[c]; k < a.gfo[c + 1]; ++k) {
+        double pf;
+        const int kind = a.g_kind[k];
+        if (kind == 0) pf = a.g_w[k] * ps + a.g_val[k] * a.phi[a.g_other[k]];
+        else if (kind == 1) pf = a.g_val[k];
+        else pf = ps;
+        gx = gx + a.g_S[3 * k] * pf;
+        gy = gy + a.g_S[3 * k + 1] * pf;
+        gz = gz + a.g_S[3 * k + 2] * pf;
+    }
+    const double V = a.V[c];
+    const double ex = -(gx / V), ey = -(gy / V), ez = -(gz / V);
+    a.E[3 * c] = ex; a.E[3 * c + 1] = ey; a.E[3 * c + 2] = ez;
+    if (a.E_f) { a.E_f[3 * c] = (float)ex; a.E_f[3 * c + 1] = (float)ey; a.E_f[3 * c + 2] = (float)ez; }
+}
+
+hipError_t launch_grad(const GradArgs& a, hipStream_t s) {
+    if (a.n_cells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grad, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// volPointInterpolation: inverse-distance weighted cell values summed over every cell of
+// the coupled point group (cyclic images share the master's value).
+__global__ void k_point_interp(PointInterpArgs a) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n_points) return;
+    const int m = a.pmaster[p];
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    for (int k = a.pco[m]; k < a.pco[m + 1]; ++k) {
+        const int c = a.pc_cell[k];
+        const double w = a.pc_w[k];
+        ex = ex + w * a.Ecell[3 * c];
+        ey = ey + w * a.Ecell[3 * c + 1];
+        ez = ez + w * a.Ecell[3 * c + 2];
+    }
+    a.Epoint[3 * p] = ex; a.Epoint[3 * p + 1] = ey; a.Epoint[3 * p + 2] = ez;
+    if (a.Epoint_f) { a.Epoint_f[3 * p] = (float)ex; a.Epoint_f[3 * p + 1] = (float)ey; a.Epoint_f[3 * p + 2] = (float)ez; }
+}
+
+hipError_t launch_point_interp(const PointInterpArgs& a, hipStream_t s) {
+    if (a.n_points <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_point_interp, dim3((a.n_points + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void k_to_float(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (float)s[i];
+}
+
+hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- diagnostics
+constexpr int kDiagVals = 6;
+
+template <class UT>
+__global__ void __launch_bounds__(256) k_diag_partial(PartSoA<UT> p, int64_t n, double* partial) {
+    __shared__ double s[kDiagVals][256];
+    double v[kDiagVals] = {0, 0, 0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (p.tet[i] < 0) continue;
+        double ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+        double u2 = ux * ux + uy * uy + uz * uz;
+        v[0] += 1.0; v[1] += ux; v[2] += uy; v[3] += uz; v[4] += u2;
+        double b = sqrt(u2) / kC;
+        v[5] += 1.0 / sqrt(1.0 - b * b) - 1.0;
+    }
+    for (int k = 0; k < kDiagVals; ++k) s[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < kDiagVals; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kDiagVals; ++k) partial[(int64_t)blockIdx.x * kDiagVals + k] = s[k][0];
+}
+
+__global__ void k_diag_final(const double* partial, int nb, int nv, double* out) {
+    const int k = threadIdx.x;
+    if (k >= nv) return;
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += partial[(int64_t)b * nv + k];
+    out[k] = acc;
+}
+
+template <class UT>
+static hipError_t launch_diag_t(const PartSoA<UT>& p, int64_t n, double* partial, int nblocks, double* out,
+                                hipStream_t s) {
+    hipLaunchKernelGGL((k_diag_partial<UT>), dim3(nblocks), dim3(256), 0, s, p, n, partial);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, partial, nblocks, kDiagVals, out);
+    return hipGetLastError();
+}
+hipError_t launch_diag(const PartSoA<double>& p, int64_t n, double* partial, int nblocks, double* out,
+                       hipStream_t s) {
+    return launch_diag_t(p, n, partial, nblocks, out, s);
+}
+hipError_t launch_diag(const PartSoA<float>& p, int64_t n, double* partial, int nblocks, double* out,
+                       hipStream_t s) {
+    return launch_diag_t(p, n, partial, nblocks, out, s);
+}
+
+__global__ void __launch_bounds__(256) k_energy_partial(const double* E, const double* V, int n, double* partial) {
+    __shared__ double s[256];
+    double v = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double ex = E[3 * i], ey = E[3 * i + 1], ez = E[3 * i + 2];
+        v += (ex * ex + ey * ey + ez * ez) * V[i];
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+hipError_t launch_field_energy(const double* E, const double* V, int n, double* partial, int nblocks, double* out,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_energy_partial, dim3(nblocks), dim3(256), 0, s, E, V, n, partial);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, partial, nblocks, 1, out);
+    return hipGetLastError();
+}
+
+}  // namespace picmi

@@ -1,0 +1,257 @@
+// Electron-neutral null-collision Monte Carlo with a uniform background gas
+// (ElectronNeutralCollisionModel::performBackgroundCollisions,
+// ElectronNeutralCollisionModel.C:703-829, events :504-700, Nanbu scattering
+// NanbuElectronNeutralCollision.C:98-130, ListCrossSection interpolateXY
+// ListCrossSection.C:32-92).
+//
+// One wave per cell.  The cell's electrons are ordered by id (bitonic sort in LDS) so that
+// a candidate index names the same parcel whatever order the sort scatter left them in.
+// Candidates are drawn with replacement as in the reference; lanes take one candidate
+// each and the lowest lane of a group of candidates that picked the same parcel replays
+// the group in candidate order, so every parcel sees its collisions in sequence.
+#include "device.h"
+#include "kernels.h"
+
+namespace picmi {
+
+namespace {
+
+// interpolateXY on an ascending table: clamped linear interpolation
+__device__ double xs_lookup(const double* __restrict__ e, const double* __restrict__ s, int n, double x) {
+    if (n <= 0) return 0.0;
+    if (!(x > e[0])) return (x == e[0] || x < e[0]) ? s[0] : s[0];
+    if (x >= e[n - 1]) return s[n - 1];
+    int lo = 0, hi = n - 1;  // e[lo] < x < e[hi] or equality found
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (e[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    if (e[lo] == x) return s[lo];
+    return s[lo] + ((x - e[lo]) / (e[hi] - e[lo])) * (s[hi] - s[lo]);
+}
+
+__device__ __forceinline__ void mcc_rand(uint64_t seed, int64_t step, int cell, int k, int j, double& a, double& b) {
+    uint4 o = rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)k, (kTagMcc << 24) | (uint32_t)j);
+    a = u01(o.x, o.y);
+    b = u01(o.z, o.w);
+}
+
+__device__ __forceinline__ void box_muller(double a, double b, double& n1, double& n2) {
+    const double r = sqrt(-2.0 * log(1.0 - a));
+    n1 = r * cos(2.0 * kPi * b);
+    n2 = r * sin(2.0 * kPi * b);
+}
+
+// NanbuElectronNeutralCollision::updateVelocity with a sampled partner velocity
+__device__ void nanbu_scatter(double eV, vec3<double>& UE, vec3<double>& Un, double mE, double mN, double uphi,
+                              double uchi) {
+    const double phi = 2.0 * kPi * uphi;
+    double cc;
+    if (eV < 1e-30) cc = 1.0;
+    else cc = 1.0 / eV * (eV + 2. - 2. * pow(1. + eV, uchi));
+    double sc;
+    if (cc * cc > 1.0) { cc = 1.0; sc = 0.0; }
+    else sc = sqrt(1.0 - cc * cc);
+    const vec3<double> g = UE - Un;
+    const double gm = mag(g), gp = sqrt(g.y * g.y + g.z * g.z);
+    if (!(gp > 0.0)) return;
+    const double cp = cos(phi), sp = sin(phi);
+    const vec3<double> h = mk3<double>(gp * cp, -1.0 * (g.y * g.x * cp + gm * g.z * sp) / gp,
+                                       -1.0 * (g.z * g.x * cp - gm * g.y * sp) / gp);
+    const vec3<double> dv = (1.0 - cc) * g + sc * h;
+    UE = UE - (mN / (mN + mE)) * dv;
+    Un = Un + (mE / (mN + mE)) * dv;
+}
+
+__device__ void bitonic_sort_u64(uint64_t* key, int* val, int npow2) {
+    for (int k = 2; k <= npow2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < npow2; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    const uint64_t a = key[i], b = key[l];
+                    if ((a > b) == up) {
+                        key[i] = b; key[l] = a;
+                        const int t = val[i]; val[i] = val[l]; val[l] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+template <class UT>
+__device__ void deposit_new(const MccArgs<UT>& a, int t, double y1, double y2, double y3, int* count, double s42) {
+    const TetGeo& g = a.tets[t];
+    atomicAdd(count + g.cell, 1);
+    if (a.dep_vw) {
+        const double y0 = 1.0 - (y1 + y2 + y3);
+        atomicAdd(a.acc_cell + g.cell, (unsigned long long)llrint(y0 * s42));
+        atomicAdd(a.acc_point + g.vm[0], (unsigned long long)llrint(y1 * s42));
+        atomicAdd(a.acc_point + g.vm[1], (unsigned long long)llrint(y2 * s42));
+        atomicAdd(a.acc_point + g.vm[2], (unsigned long long)llrint(y3 * s42));
+    }
+}
+
+// one candidate (ElectronNeutralCollisionModel.C:747-800 and the event handlers)
+template <class UT>
+__device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double smax_local, double& qtcr,
+                          unsigned long long* cnt) {
+    double ua, ub, n1, n2, n3, n4, upick, uacc;
+    mcc_rand(a.seed, a.step, cell, k, 0, upick, uacc);
+    mcc_rand(a.seed, a.step, cell, k, 1, ua, ub);
+    box_muller(ua, ub, n1, n2);
+    mcc_rand(a.seed, a.step, cell, k, 2, ua, ub);
+    box_muller(ua, ub, n3, n4);
+    vec3<double> Un = mk3<double>(a.vthN * n1 + a.drift[0], a.vthN * n2 + a.drift[1], a.vthN * n3 + a.drift[2]);
+    vec3<double> Ue = mk3<double>(a.e.ux[ei], a.e.uy[ei], a.e.uz[ei]);
+    const double c = kC;
+    const double cR = mag(Ue - Un);
+    const double gamma = 1.0 / sqrt(1.0 - ((cR * cR) / (c * c)));
+    double eV = (gamma - 1.0) * a.massE * c * c / kQe;
+    const double sigmaT = smax_local / cR;
+    const double Qel = xs_lookup(a.xs_e[0], a.xs_s[0], a.xs_n[0], eV);
+    const double Qex = xs_lookup(a.xs_e[1], a.xs_s[1], a.xs_n[1], eV);
+    const double Qiz = xs_lookup(a.xs_e[2], a.xs_s[2], a.xs_n[2], eV);
+    const double Qt = Qel + Qex + Qiz;
+    qtcr = fmax(qtcr, Qt * cR);
+    double us, uc;
+    mcc_rand(a.seed, a.step, cell, k, 3, us, uc);
+    if (uacc <= Qel / sigmaT) {
+        nanbu_scatter(eV, Ue, Un, a.massE, a.massN, us, uc);
+        cnt[1]++;
+    } else if (eV >= a.th_ex && uacc <= (Qel + Qex) / sigmaT) {
+        Ue = sqrt(1.0 - a.th_ex / eV) * Ue;
+        eV -= a.th_ex;
+        nanbu_scatter(eV, Ue, Un, a.massE, a.massN, us, uc);
+        cnt[2]++;
+    } else if (eV >= a.th_iz && uacc <= Qt / sigmaT) {
+        double uE, uR, up2, uc2;
+        mcc_rand(a.seed, a.step, cell, k, 4, uE, uR);
+        mcc_rand(a.seed, a.step, cell, k, 5, up2, uc2);
+        const double E0 = 2.0 - 100.0 / (eV + 10.0);
+        const double E1 = (eV - a.th_iz) / 2.0 - E0;
+        const double alpha = 10.3;
+        const double Ep = E0 + alpha * tan(uE * (atan(E1 / alpha) + atan(E0 / alpha)) - atan(E0 / alpha));
+        const vec3<double> preUe = Ue, preUn = Un;
+        Ue = sqrt(1.0 - (a.th_iz + Ep) / eV) * Ue;
+        eV -= (a.th_iz + Ep);
+        nanbu_scatter(eV, Ue, Un, a.massE, a.massN, us, uc);
+        const int t = a.e.tet[ei];
+        const double y1 = a.e.y1[ei], y2 = a.e.y2[ei], y3 = a.e.y3[ei];
+        bool created = false;
+        if (uR <= a.wEwN && a.create_ion) {
+            const int64_t slot = atomicAdd((unsigned long long*)a.n_i, 1ull);
+            if (slot >= a.cap_i) {
+                atomicOr(a.flags, kFlagCapacity);
+            } else {
+                a.ion.y1[slot] = y1; a.ion.y2[slot] = y2; a.ion.y3[slot] = y3;
+                a.ion.tet[slot] = t;
+                a.ion.ux[slot] = (UT)Un.x; a.ion.uy[slot] = (UT)Un.y; a.ion.uz[slot] = (UT)Un.z;
+                uint4 o = rng4(a.seed, a.e.id[ei], (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 1u);
+                a.ion.id[slot] = o.x;
+                deposit_new(a, t, y1, y2, y3, a.count_i, a.s42_i);
+                created = true;
+            }
+        }
+        if (uR > a.wNwE) Ue = preUe;
+        if (uR <= a.wEwN && a.create_electron) {
+            const int64_t slot = atomicAdd((unsigned long long*)a.n_e, 1ull);
+            if (slot >= a.cap_e) {
+                atomicOr(a.flags, kFlagCapacity);
+            } else {
+                const double sp = sqrt(2.0 * Ep * kQe / a.massE);
+                vec3<double> Unew = sp * (preUe / mag(preUe));
+                vec3<double> Un2 = preUn;
+                nanbu_scatter(Ep, Unew, Un2, a.massE, a.massN, up2, uc2);
+                a.e.y1[slot] = y1; a.e.y2[slot] = y2; a.e.y3[slot] = y3;
+                a.e.tet[slot] = t;
+                a.e.ux[slot] = (UT)Unew.x; a.e.uy[slot] = (UT)Unew.y; a.e.uz[slot] = (UT)Unew.z;
+                uint4 o = rng4(a.seed, a.e.id[ei], (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 2u);
+                a.e.id[slot] = o.x;
+                deposit_new(a, t, y1, y2, y3, a.count_e, a.s42_e);
+            }
+        }
+        if (created) cnt[3]++;
+    }
+    a.e.ux[ei] = (UT)Ue.x; a.e.uy[ei] = (UT)Ue.y; a.e.uz[ei] = (UT)Ue.z;
+}
+
+}  // namespace
+
+template <class UT>
+__global__ void __launch_bounds__(64) k_mcc(MccArgs<UT> a) {
+    extern __shared__ uint64_t lds[];
+    const int maxp = a.maxp;
+    uint64_t* key = lds;
+    int* val = (int*)(lds + maxp);
+    __shared__ int picks[64];
+    const int cell = blockIdx.x;
+    const int a0 = a.start[cell], eC = a.start[cell + 1] - a0;
+    if (eC > maxp) {
+        if (threadIdx.x == 0) atomicOr(a.flags, kFlagCellOverflow);
+        return;
+    }
+    const double smax = a.smax[cell];
+    const double coll = a.rem[cell] + eC * a.nN * smax * a.dt;
+    const int nc = (int)coll;
+    __syncthreads();
+    if (threadIdx.x == 0) a.rem[cell] = coll - nc;
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    if (threadIdx.x == 0) cnt[0] = (unsigned long long)nc;
+    double qtcr = 0.0;
+    if (eC > 0 && nc > 0) {
+        int np2 = 1;
+        while (np2 < eC) np2 <<= 1;
+        for (int k = threadIdx.x; k < np2; k += blockDim.x) {
+            key[k] = (k < eC) ? (((uint64_t)a.e.id[a0 + k] << 32) | (uint32_t)k) : ~0ull;
+            val[k] = (k < eC) ? a0 + k : -1;
+        }
+        __syncthreads();
+        bitonic_sort_u64(key, val, np2);
+        for (int base = 0; base < nc; base += 64) {
+            const int k = base + (int)threadIdx.x;
+            int pick = -1;
+            if (k < nc) {
+                double upick, uacc;
+                mcc_rand(a.seed, a.step, cell, k, 0, upick, uacc);
+                pick = (int)(upick * eC);
+                if (pick >= eC) pick = eC - 1;
+            }
+            picks[threadIdx.x] = pick;
+            __syncthreads();
+            if (k < nc) {
+                bool first = true;
+                for (int l = 0; l < (int)threadIdx.x; ++l)
+                    if (picks[l] == pick) { first = false; break; }
+                if (first) {
+                    const int ei = val[pick];
+                    for (int l = (int)threadIdx.x; l < 64 && base + l < nc; ++l)
+                        if (picks[l] == pick) candidate(a, cell, base + l, ei, smax, qtcr, cnt);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) qtcr = fmax(qtcr, __shfl_xor(qtcr, off));
+    if (threadIdx.x == 0 && qtcr > smax) a.smax[cell] = qtcr;
+    for (int q = 0; q < 4; ++q) {
+        unsigned long long v = cnt[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (threadIdx.x == 0 && v) atomicAdd(a.counts + q, v);
+    }
+}
+
+template <class UT>
+static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
+    if (a.n_cells <= 0) return hipSuccess;
+    const size_t lds = (size_t)a.maxp * (sizeof(uint64_t) + sizeof(int));
+    hipLaunchKernelGGL((k_mcc<UT>), dim3(a.n_cells), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_mcc(const MccArgs<double>& a, hipStream_t s) { return launch_mcc_t(a, s); }
+hipError_t launch_mcc(const MccArgs<float>& a, hipStream_t s) { return launch_mcc_t(a, s); }
+
+}  // namespace picmi

@@ -1,0 +1,497 @@
+// Particle kernels of libpicmi: push + E interpolation + barycentric tet tracking +
+// charge deposit / cell count (one lane per parcel, parcels kept sorted by cell), push
+// only, point location, positions, Maxwellian initialisation and the counting-sort
+// scatter.
+#include "device.h"
+#include "kernels.h"
+
+namespace picmi {
+
+// ---------------------------------------------------------------- pushers
+// BorisPusher.C:69-112, BorisNRPusher.C:69-99, VayPusher.C:54-100, HigueraCary.C:69-115.
+// k = q*(dt/2)/m and kv = q*dt/m are precomputed on the host in FP64.
+template <class T, int P>
+__device__ __forceinline__ vec3<T> push_u(vec3<T> U, vec3<T> E, vec3<T> B, T k, T kv, bool rotate, bool hc_ref) {
+    const T c = (T)kC;
+    if constexpr (P == kPushBoris) {
+        T mu = mag(U) / c;
+        vec3<T> rU = U / sqrt((T)1 - mu * mu);
+        vec3<T> oldrU = rU;
+        vec3<T> rUsm = k * E;
+        rU = rU + rUsm;
+        if (rotate) {
+            T a = mag(oldrU) / c, b = mag(rU) / c;
+            T lF = (sqrt((T)1 + a * a) + sqrt((T)1 + b * b)) / (T)2;
+            vec3<T> t = (k * B) / lF;
+            vec3<T> Ud = rU + cross(rU, t);
+            rU = rU + cross(Ud, ((T)2 * t) / ((T)1 + dot(t, t)));
+        }
+        rU = rU + rUsm;
+        T g = mag(rU) / c;
+        return rU / sqrt((T)1 + g * g);
+    } else if constexpr (P == kPushBorisNR) {
+        U = U + k * E;
+        if (rotate) {
+            vec3<T> t = k * B;
+            vec3<T> Ud = U + cross(U, t);
+            U = U + cross(Ud, ((T)2 * t) / ((T)1 + dot(t, t)));
+        }
+        return U + k * E;
+    } else if constexpr (P == kPushVay) {
+        T mu = mag(U) / c;
+        T lF = (T)1 / sqrt((T)1 - mu * mu);
+        vec3<T> rU = lF * U;
+        vec3<T> Ud = rU + kv * E;
+        vec3<T> Tv = k * B;
+        Ud = Ud + cross(U, Tv);
+        T ud = mag(Ud) / c;
+        T alpha = (T)1 + ud * ud;
+        T T2 = dot(Tv, Tv);
+        T s = alpha - T2;
+        T w = dot(Ud, Tv) / c;
+        T us2 = w * w;
+        alpha = (T)1 / sqrt((T)0.5 * (s + sqrt(s * s + (T)4 * (T2 + us2))));
+        Tv = alpha * Tv;
+        s = (T)1 / ((T)1 + dot(Tv, Tv));
+        alpha = dot(Ud, Tv);
+        rU = s * ((Ud + alpha * Tv) + cross(Ud, Tv));
+        T g = mag(rU) / c;
+        return rU / sqrt((T)1 + g * g);
+    } else if constexpr (P == kPushHC) {
+        T mu = mag(U) / c;
+        vec3<T> rU = U / sqrt((T)1 - mu * mu);
+        vec3<T> rUsm = k * E;
+        rU = rU + rUsm;
+        if (rotate) {
+            T g = mag(rU) / c;
+            T gfm2 = (T)1 + g * g;
+            vec3<T> t = k * B;
+            T beta2 = dot(t, t);
+            // published Higuera-Cary: (t.u/c)^2; the reference writes (t.u)^2
+            T tu = hc_ref ? dot(t, rU) : dot(t, rU) / c;
+            T sg = gfm2 - beta2;
+            T iG = (T)1 / sqrt((T)0.5 * (sg + sqrt(sg * sg + (T)4 * (beta2 + tu * tu))));
+            t = iG * t;
+            vec3<T> Ud = rU + cross(rU, t);
+            rU = rU + cross(Ud, ((T)2 * t) / ((T)1 + dot(t, t)));
+        }
+        rU = rU + rUsm;
+        T g = mag(rU) / c;
+        return rU / sqrt((T)1 + g * g);
+    } else {
+        return U;
+    }
+}
+
+// CellAverageFW.C:58-65 / VolumeWeightingFW.C:60-67 (interpolationCellPoint)
+template <class T, int W>
+__device__ __forceinline__ vec3<T> interp_E(const TetGeo& g, double y1, double y2, double y3, const T* Ec,
+                                            const T* Ep) {
+    const int c = g.cell;
+    vec3<T> ec = mk3<T>(Ec[3 * c], Ec[3 * c + 1], Ec[3 * c + 2]);
+    if constexpr (W == kCellAverage) {
+        return ec;
+    } else {
+        T b0 = (T)(1.0 - (y1 + y2 + y3));
+        vec3<T> e = b0 * ec;
+        const int a = g.vm[0], b = g.vm[1], d = g.vm[2];
+        e = e + (T)y1 * mk3<T>(Ep[3 * a], Ep[3 * a + 1], Ep[3 * a + 2]);
+        e = e + (T)y2 * mk3<T>(Ep[3 * b], Ep[3 * b + 1], Ep[3 * b + 2]);
+        e = e + (T)y3 * mk3<T>(Ep[3 * d], Ep[3 * d + 1], Ep[3 * d + 2]);
+        return e;
+    }
+}
+
+// Barycentric tracking through the tet decomposition (PICParcel::moveForward,
+// PICParcel.C:81-144, with OpenFOAM-9 particle::trackToFace re-derived): per tet the
+// remaining displacement (1-sf)*dt*U becomes Tx_i = r_i.d (det-scaled barycentric);
+// the first face with Tx_i < 0 and 0 <= -y_i/Tx_i < 1/det is hit and the parcel
+// continues in the neighbouring tet with its coordinates permuted by the adjacency table.
+// Returns the final tet (-1 = left through a non-coupled patch, deleted as NoInflow does);
+// *status = 1 when the iteration guard stopped the track, 2 when the table had no
+// neighbour.
+__device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, double& y1, double& y2, double& y3,
+                                     double Ux, double Uy, double Uz, double dt, int& status) {
+    double Y0 = 1.0 - (y1 + y2 + y3), Y1 = y1, Y2 = y2, Y3 = y3;
+    double sf = 0.0;
+    int guard = 0;
+    status = 0;
+    while (sf < 1.0) {
+        const double f = 1.0 - sf;
+        const double fdt = f * dt;
+        const double dx = fdt * Ux, dy = fdt * Uy, dz = fdt * Uz;
+        const TetGeo& g = tets[t];
+        const double T1 = g.r[0] * dx + g.r[1] * dy + g.r[2] * dz;
+        const double T2 = g.r[3] * dx + g.r[4] * dy + g.r[5] * dz;
+        const double T3 = g.r[6] * dx + g.r[7] * dy + g.r[8] * dz;
+        const double T0 = -(T1 + T2 + T3);
+        const double det = g.det;
+        double muH = (det > 0.0 && __builtin_isnormal(det)) ? 1.0 / det : 1e300;
+        int iH = -1;
+        if (T0 < 0.0) { double mu = -Y0 / T0; if (mu >= 0.0 && mu < muH) { muH = mu; iH = 0; } }
+        if (T1 < 0.0) { double mu = -Y1 / T1; if (mu >= 0.0 && mu < muH) { muH = mu; iH = 1; } }
+        if (T2 < 0.0) { double mu = -Y2 / T2; if (mu >= 0.0 && mu < muH) { muH = mu; iH = 2; } }
+        if (T3 < 0.0) { double mu = -Y3 / T3; if (mu >= 0.0 && mu < muH) { muH = mu; iH = 3; } }
+        Y0 = Y0 + muH * T0;
+        Y1 = Y1 + muH * T1;
+        Y2 = Y2 + muH * T2;
+        Y3 = Y3 + muH * T3;
+        if (iH < 0) break;
+        if (iH == 0) Y0 = 0.0;
+        if (iH == 1) Y1 = 0.0;
+        if (iH == 2) Y2 = 0.0;
+        if (iH == 3) Y3 = 0.0;
+        double frac = muH * det;
+        frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+        sf = sf + f * frac;
+        const int nb = g.nbr[iH];
+        if (nb < 0) {
+            if (nb <= -1000000) status = 2;
+            t = -1;
+            break;
+        }
+        const uint32_t p = (g.perm >> (8 * iH)) & 0xffu;
+        const uint32_t p0 = p & 3u, p1 = (p >> 2) & 3u, p2 = (p >> 4) & 3u;
+        // new slot p_k receives Y_k; written as selects so that nothing leaves registers
+        double n0 = (p0 == 0) ? Y0 : (p1 == 0) ? Y1 : (p2 == 0) ? Y2 : Y3;
+        double n1 = (p0 == 1) ? Y0 : (p1 == 1) ? Y1 : (p2 == 1) ? Y2 : Y3;
+        double n2 = (p0 == 2) ? Y0 : (p1 == 2) ? Y1 : (p2 == 2) ? Y2 : Y3;
+        double n3 = (p0 == 3) ? Y0 : (p1 == 3) ? Y1 : (p2 == 3) ? Y2 : Y3;
+        Y0 = n0; Y1 = n1; Y2 = n2; Y3 = n3;
+        t = nb;
+        if (++guard > kMaxTrackSteps) { status = 1; break; }
+    }
+    y1 = Y1; y2 = Y2; y3 = Y3;
+    return t;
+}
+
+// One lane per parcel: push (if charged and moving) + track + deposit/count.
+template <class UT, int P, int W, int DEP>
+__global__ void __launch_bounds__(256) k_move(MoveArgs<UT> a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int newcell = -1;
+    double y1 = 0, y2 = 0, y3 = 0;
+    int t = -1;
+    int status = 0;
+    bool deleted = false;
+    if (i < a.n) {
+        t = a.p.tet[i];
+        if (t >= 0) {
+            y1 = a.p.y1[i]; y2 = a.p.y2[i]; y3 = a.p.y3[i];
+            vec3<UT> U = mk3<UT>(a.p.ux[i], a.p.uy[i], a.p.uz[i]);
+            if (a.do_push) {
+                const TetGeo& g = a.tets[t];
+                vec3<UT> E = interp_E<UT, W>(g, y1, y2, y3, a.Ecell, a.Epoint);
+                vec3<UT> B = mk3<UT>(0, 0, 0);
+                if (a.rotate) {
+                    const int c = g.cell;
+                    B = mk3<UT>(a.Bcell[3 * c], a.Bcell[3 * c + 1], a.Bcell[3 * c + 2]);
+                }
+                U = push_u<UT, P>(U, E, B, a.k, a.kv, a.rotate != 0, a.hc_ref != 0);
+                a.p.ux[i] = U.x; a.p.uy[i] = U.y; a.p.uz[i] = U.z;
+            }
+            const double Ux = (double)U.x, Uy = (double)U.y, Uz = (double)U.z;
+            if (sqrt(Ux * Ux + Uy * Uy + Uz * Uz) >= kC) atomicOr(a.flags, kFlagSuperluminal);
+            t = track(a.tets, t, y1, y2, y3, Ux, Uy, Uz, a.dt, status);
+            a.p.y1[i] = y1; a.p.y2[i] = y2; a.p.y3[i] = y3;
+            a.p.tet[i] = t;
+            if (t >= 0) newcell = a.tets[t].cell;
+            else deleted = (status != 2);
+        }
+    }
+    // statistics: deleted parcels and guard hits (one atomic per wave)
+    {
+        const uint64_t del = __ballot(deleted);
+        const uint64_t gd = __ballot(status == 1);
+        const uint64_t br = __ballot(status == 2);
+        if (__lane_id() == 0) {
+            if (del) atomicAdd(a.stats + 0, (unsigned long long)__popcll(del));
+            if (gd) atomicAdd(a.stats + 1, (unsigned long long)__popcll(gd));
+            if (br) atomicOr(a.flags, kFlagTrackBroken);
+        }
+    }
+    // cell count (sort histogram; for CellAverage also the deposit)
+    wave_aggregated_inc(a.count, newcell);
+    if constexpr (DEP == kVolumeWeighting) {
+        if (newcell >= 0) {
+            const TetGeo& g = a.tets[t];
+            const double s = a.s42;
+            const double y0 = 1.0 - (y1 + y2 + y3);
+            atomicAdd(a.acc_cell + newcell, (unsigned long long)llrint(y0 * s));
+            atomicAdd(a.acc_point + g.vm[0], (unsigned long long)llrint(y1 * s));
+            atomicAdd(a.acc_point + g.vm[1], (unsigned long long)llrint(y2 * s));
+            atomicAdd(a.acc_point + g.vm[2], (unsigned long long)llrint(y3 * s));
+        }
+    }
+}
+
+template <class UT, int P, int W>
+__global__ void __launch_bounds__(256) k_push(MoveArgs<UT> a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int t = a.p.tet[i];
+    if (t < 0) return;
+    const double y1 = a.p.y1[i], y2 = a.p.y2[i], y3 = a.p.y3[i];
+    vec3<UT> U = mk3<UT>(a.p.ux[i], a.p.uy[i], a.p.uz[i]);
+    const TetGeo& g = a.tets[t];
+    vec3<UT> E = interp_E<UT, W>(g, y1, y2, y3, a.Ecell, a.Epoint);
+    vec3<UT> B = mk3<UT>(0, 0, 0);
+    if (a.rotate) {
+        const int c = g.cell;
+        B = mk3<UT>(a.Bcell[3 * c], a.Bcell[3 * c + 1], a.Bcell[3 * c + 2]);
+    }
+    U = push_u<UT, P>(U, E, B, a.k, a.kv, a.rotate != 0, a.hc_ref != 0);
+    a.p.ux[i] = U.x; a.p.uy[i] = U.y; a.p.uz[i] = U.z;
+}
+
+// Deposit / count only (calculateFields without a preceding move).
+template <class UT, int DEP>
+__global__ void __launch_bounds__(256) k_deposit(MoveArgs<UT> a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int cell = -1, t = -1;
+    double y1 = 0, y2 = 0, y3 = 0;
+    if (i < a.n) {
+        t = a.p.tet[i];
+        if (t >= 0) { cell = a.tets[t].cell; y1 = a.p.y1[i]; y2 = a.p.y2[i]; y3 = a.p.y3[i]; }
+    }
+    wave_aggregated_inc(a.count, cell);
+    if constexpr (DEP == kVolumeWeighting) {
+        if (cell >= 0) {
+            const TetGeo& g = a.tets[t];
+            const double s = a.s42;
+            const double y0 = 1.0 - (y1 + y2 + y3);
+            atomicAdd(a.acc_cell + cell, (unsigned long long)llrint(y0 * s));
+            atomicAdd(a.acc_point + g.vm[0], (unsigned long long)llrint(y1 * s));
+            atomicAdd(a.acc_point + g.vm[1], (unsigned long long)llrint(y2 * s));
+            atomicAdd(a.acc_point + g.vm[2], (unsigned long long)llrint(y3 * s));
+        }
+    }
+}
+
+// Point location inside the given cell: the tet with the largest minimum barycentric
+// coordinate (y_i = r_i.(x - C)/det, the tracking kernel's own transform).
+template <class UT>
+__global__ void k_locate(LocateArgs<UT> a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int c = a.cell[i];
+    const double px = a.x[3 * i], py = a.x[3 * i + 1], pz = a.x[3 * i + 2];
+    const double cx = a.C[3 * c], cy = a.C[3 * c + 1], cz = a.C[3 * c + 2];
+    const double dx = px - cx, dy = py - cy, dz = pz - cz;
+    int best = -1;
+    double bestmin = -1e300, b1 = 0, b2 = 0, b3 = 0;
+    for (int t = a.cell_tet0[c]; t < a.cell_tet0[c + 1]; ++t) {
+        const TetGeo& g = a.tets[t];
+        double v1 = (g.r[0] * dx + g.r[1] * dy + g.r[2] * dz) / g.det;
+        double v2 = (g.r[3] * dx + g.r[4] * dy + g.r[5] * dz) / g.det;
+        double v3_ = (g.r[6] * dx + g.r[7] * dy + g.r[8] * dz) / g.det;
+        double v0 = 1.0 - (v1 + v2 + v3_);
+        double mn = fmin(fmin(v0, v1), fmin(v2, v3_));
+        if (mn > bestmin) { bestmin = mn; best = t; b1 = v1; b2 = v2; b3 = v3_; }
+    }
+    if (bestmin < -1e-9) atomicAdd(a.fails, 1u);
+    const int64_t o = a.offset + i;
+    a.p.y1[o] = b1; a.p.y2[o] = b2; a.p.y3[o] = b3;
+    a.p.tet[o] = best;
+    a.p.ux[o] = (UT)a.U[3 * i]; a.p.uy[o] = (UT)a.U[3 * i + 1]; a.p.uz[o] = (UT)a.U[3 * i + 2];
+    a.p.id[o] = a.id[i];
+}
+
+template <class UT>
+__global__ void k_positions(PartSoA<UT> p, int64_t n, const TetGeo* __restrict__ tets, const double* __restrict__ C,
+                            const double* __restrict__ pts, double* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int t = p.tet[i];
+    if (t < 0) { x[3 * i] = x[3 * i + 1] = x[3 * i + 2] = __builtin_nan(""); return; }
+    const TetGeo& g = tets[t];
+    const double y1 = p.y1[i], y2 = p.y2[i], y3 = p.y3[i];
+    const double y0 = 1.0 - (y1 + y2 + y3);
+    for (int d = 0; d < 3; ++d) {
+        double v = y0 * C[3 * g.cell + d];
+        v = v + y1 * pts[3 * g.vid[0] + d];
+        v = v + y2 * pts[3 * g.vid[1] + d];
+        v = v + y3 * pts[3 * g.vid[2] + d];
+        x[3 * i + d] = v;
+    }
+}
+
+// picInitialise Uniform + Maxwellian: parcel j of cell c sits in a tet chosen with
+// probability proportional to its volume, uniformly inside it (normalised exponential
+// spacings), with velocity sqrt(kT/m) N(0,1)^3 + drift.
+template <class UT>
+__global__ void k_init(InitArgs<UT> a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int c = (int)(i / a.nppc);
+    const uint32_t j = (uint32_t)(i - (int64_t)c * a.nppc);
+    uint4 r0 = rng4(a.seed, (uint32_t)c, j, 0u, (kTagInit << 24));
+    uint4 r1 = rng4(a.seed, (uint32_t)c, j, 1u, (kTagInit << 24));
+    uint4 r2 = rng4(a.seed, (uint32_t)c, j, 2u, (kTagInit << 24));
+    const int t0 = a.cell_tet0[c], t1 = a.cell_tet0[c + 1];
+    double vol = 0.0;
+    for (int t = t0; t < t1; ++t) vol += a.tets[t].det;
+    double pick = u01(r0.x, r0.y) * vol, acc = 0.0;
+    int tt = t1 - 1;
+    for (int t = t0; t < t1; ++t) { acc += a.tets[t].det; if (pick < acc) { tt = t; break; } }
+    double e0 = -log(1.0 - u01(r0.z, r0.w)), e1 = -log(1.0 - u01(r1.x, r1.y));
+    double e2 = -log(1.0 - u01(r1.z, r1.w)), e3 = -log(1.0 - u01(r2.x, r2.y));
+    double s = (e0 + e1) + (e2 + e3);
+    const int64_t o = a.offset + i;
+    a.p.y1[o] = e1 / s; a.p.y2[o] = e2 / s; a.p.y3[o] = e3 / s;
+    a.p.tet[o] = tt;
+    uint4 r3 = rng4(a.seed, (uint32_t)c, j, 3u, (kTagInit << 24));
+    double ua = u01(r2.z, r2.w), ub = u01(r3.x, r3.y), uc = u01(r3.z, r3.w);
+    uint4 r4 = rng4(a.seed, (uint32_t)c, j, 4u, (kTagInit << 24));
+    double ud = u01(r4.x, r4.y);
+    double ra = sqrt(-2.0 * log(1.0 - ua)), rb = sqrt(-2.0 * log(1.0 - uc));
+    double n1 = ra * cos(2.0 * kPi * ub), n2 = ra * sin(2.0 * kPi * ub), n3 = rb * cos(2.0 * kPi * ud);
+    a.p.ux[o] = (UT)(a.vth * n1 + a.drift[0]);
+    a.p.uy[o] = (UT)(a.vth * n2 + a.drift[1]);
+    a.p.uz[o] = (UT)(a.vth * n3 + a.drift[2]);
+    a.p.id[o] = a.id0 + (uint32_t)i;
+}
+
+// Counting-sort scatter (buildCellOccupancy, PICCloud.C:327-354): live parcels go to
+// start[cell] + slot; the slot comes from a wave-aggregated atomic, so the order inside a
+// cell is not fixed; every consumer that depends on it orders the cell's parcels by id.
+template <class UT>
+__global__ void __launch_bounds__(256) k_scatter(PartSoA<UT> src, PartSoA<UT> dst, int64_t n,
+                                                  const TetGeo* __restrict__ tets, const int* __restrict__ start,
+                                                  int* __restrict__ cursor) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int cell = -1, t = -1;
+    if (i < n) {
+        t = src.tet[i];
+        if (t >= 0) cell = tets[t].cell;
+    }
+    const int slot = wave_aggregated_inc(cursor, cell);
+    if (cell >= 0) {
+        const int64_t o = (int64_t)start[cell] + slot;
+        dst.y1[o] = src.y1[i]; dst.y2[o] = src.y2[i]; dst.y3[o] = src.y3[i];
+        dst.ux[o] = src.ux[i]; dst.uy[o] = src.uy[i]; dst.uz[o] = src.uz[i];
+        dst.tet[o] = t;
+        dst.id[o] = src.id[i];
+    }
+}
+
+// ---------------------------------------------------------------- instantiation / launch
+
+template <class UT>
+static hipError_t launch_move_t(const MoveArgs<UT>& a, int pusher, int weighting, int dep, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    const dim3 blk(256), grd((unsigned)((a.n + 255) / 256));
+#define PICMI_MOVE_CASE(P, W, D)                                                     \
+    if (pusher == P && weighting == W && dep == D) {                                 \
+        hipLaunchKernelGGL((k_move<UT, P, W, D>), grd, blk, 0, s, a);                \
+        return hipGetLastError();                                                    \
+    }
+#define PICMI_MOVE_P(P)                                                              \
+    PICMI_MOVE_CASE(P, kCellAverage, kCellAverage)                                   \
+    PICMI_MOVE_CASE(P, kCellAverage, kVolumeWeighting)                               \
+    PICMI_MOVE_CASE(P, kVolumeWeighting, kCellAverage)                               \
+    PICMI_MOVE_CASE(P, kVolumeWeighting, kVolumeWeighting)
+    PICMI_MOVE_P(kPushNone)
+    PICMI_MOVE_P(kPushBoris)
+    PICMI_MOVE_P(kPushBorisNR)
+    PICMI_MOVE_P(kPushVay)
+    PICMI_MOVE_P(kPushHC)
+#undef PICMI_MOVE_P
+#undef PICMI_MOVE_CASE
+    return hipErrorInvalidValue;
+}
+
+template <class UT>
+static hipError_t launch_push_t(const MoveArgs<UT>& a, int pusher, int weighting, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    const dim3 blk(256), grd((unsigned)((a.n + 255) / 256));
+#define PICMI_PUSH_CASE(P, W)                                                        \
+    if (pusher == P && weighting == W) {                                             \
+        hipLaunchKernelGGL((k_push<UT, P, W>), grd, blk, 0, s, a);                   \
+        return hipGetLastError();                                                    \
+    }
+    PICMI_PUSH_CASE(kPushNone, kCellAverage) PICMI_PUSH_CASE(kPushNone, kVolumeWeighting)
+    PICMI_PUSH_CASE(kPushBoris, kCellAverage) PICMI_PUSH_CASE(kPushBoris, kVolumeWeighting)
+    PICMI_PUSH_CASE(kPushBorisNR, kCellAverage) PICMI_PUSH_CASE(kPushBorisNR, kVolumeWeighting)
+    PICMI_PUSH_CASE(kPushVay, kCellAverage) PICMI_PUSH_CASE(kPushVay, kVolumeWeighting)
+    PICMI_PUSH_CASE(kPushHC, kCellAverage) PICMI_PUSH_CASE(kPushHC, kVolumeWeighting)
+#undef PICMI_PUSH_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_move(const MoveArgs<double>& a, int pusher, int weighting, int dep, hipStream_t s) {
+    return launch_move_t(a, pusher, weighting, dep, s);
+}
+hipError_t launch_move(const MoveArgs<float>& a, int pusher, int weighting, int dep, hipStream_t s) {
+    return launch_move_t(a, pusher, weighting, dep, s);
+}
+hipError_t launch_push(const MoveArgs<double>& a, int pusher, int weighting, hipStream_t s) {
+    return launch_push_t(a, pusher, weighting, s);
+}
+hipError_t launch_push(const MoveArgs<float>& a, int pusher, int weighting, hipStream_t s) {
+    return launch_push_t(a, pusher, weighting, s);
+}
+
+template <class UT>
+static hipError_t launch_deposit_t(const MoveArgs<UT>& a, int dep, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    const dim3 blk(256), grd((unsigned)((a.n + 255) / 256));
+    if (dep == kVolumeWeighting) hipLaunchKernelGGL((k_deposit<UT, kVolumeWeighting>), grd, blk, 0, s, a);
+    else hipLaunchKernelGGL((k_deposit<UT, kCellAverage>), grd, blk, 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_deposit(const MoveArgs<double>& a, int dep, hipStream_t s) { return launch_deposit_t(a, dep, s); }
+hipError_t launch_deposit(const MoveArgs<float>& a, int dep, hipStream_t s) { return launch_deposit_t(a, dep, s); }
+
+template <class UT>
+static hipError_t launch_locate_t(const LocateArgs<UT>& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_locate<UT>), dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_locate(const LocateArgs<double>& a, hipStream_t s) { return launch_locate_t(a, s); }
+hipError_t launch_locate(const LocateArgs<float>& a, hipStream_t s) { return launch_locate_t(a, s); }
+
+template <class UT>
+static hipError_t launch_positions_t(const PartSoA<UT>& p, int64_t n, const TetGeo* tets, const double* C,
+                                     const double* pts, double* x, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_positions<UT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, tets, C, pts, x);
+    return hipGetLastError();
+}
+hipError_t launch_positions(const PartSoA<double>& p, int64_t n, const TetGeo* tets, const double* C,
+                            const double* pts, double* x, hipStream_t s) {
+    return launch_positions_t(p, n, tets, C, pts, x, s);
+}
+hipError_t launch_positions(const PartSoA<float>& p, int64_t n, const TetGeo* tets, const double* C,
+                            const double* pts, double* x, hipStream_t s) {
+    return launch_positions_t(p, n, tets, C, pts, x, s);
+}
+
+template <class UT>
+static hipError_t launch_init_t(const InitArgs<UT>& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_init<UT>), dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_init(const InitArgs<double>& a, hipStream_t s) { return launch_init_t(a, s); }
+hipError_t launch_init(const InitArgs<float>& a, hipStream_t s) { return launch_init_t(a, s); }
+
+template <class UT>
+static hipError_t launch_scatter_t(const PartSoA<UT>& src, const PartSoA<UT>& dst, int64_t n, const TetGeo* tets,
+                                   const int* start, int* cursor, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, tets, start,
+                       cursor);
+    return hipGetLastError();
+}
+hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const TetGeo* tets,
+                          const int* start, int* cursor, hipStream_t s) {
+    return launch_scatter_t(src, dst, n, tets, start, cursor, s);
+}
+hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const TetGeo* tets,
+                          const int* start, int* cursor, hipStream_t s) {
+    return launch_scatter_t(src, dst, n, tets, start, cursor, s);
+}
+
+}  // namespace picmi

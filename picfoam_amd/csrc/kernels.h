@@ -1,0 +1,224 @@
+// Kernel argument blocks and launch wrappers of libpicmi.
+#pragma once
+#include "device.h"
+
+namespace picmi {
+
+template <class UT>
+struct MoveArgs {
+    PartSoA<UT> p;
+    int64_t n;
+    const TetGeo* tets;
+    const UT* Ecell;
+    const UT* Epoint;
+    const UT* Bcell;
+    UT k, kv;          // q*(dt/2)/m, q*dt/m
+    double dt;
+    int do_push, rotate, hc_ref;
+    int* count;                      // per-cell parcel count of this species
+    unsigned long long* acc_cell;    // VolumeWeighting fixed-point accumulators
+    unsigned long long* acc_point;
+    double s42;                      // q*w/Q0 * 2^42
+    uint32_t* flags;
+    unsigned long long* stats;       // [0] deleted, [1] guard hits
+};
+
+template <class UT>
+struct LocateArgs {
+    PartSoA<UT> p;
+    int64_t n, offset;
+    const double* x;
+    const int32_t* cell;
+    const double* U;
+    const uint32_t* id;
+    const TetGeo* tets;
+    const int32_t* cell_tet0;
+    const double* C;
+    unsigned* fails;
+};
+
+template <class UT>
+struct InitArgs {
+    PartSoA<UT> p;
+    int64_t n, offset;
+    int nppc;
+    const TetGeo* tets;
+    const int32_t* cell_tet0;
+    double vth;
+    double drift[3];
+    uint32_t id0;
+    uint64_t seed;
+};
+
+hipError_t launch_move(const MoveArgs<double>& a, int pusher, int weighting, int dep, hipStream_t s);
+hipError_t launch_move(const MoveArgs<float>& a, int pusher, int weighting, int dep, hipStream_t s);
+hipError_t launch_push(const MoveArgs<double>& a, int pusher, int weighting, hipStream_t s);
+hipError_t launch_push(const MoveArgs<float>& a, int pusher, int weighting, hipStream_t s);
+hipError_t launch_deposit(const MoveArgs<double>& a, int dep, hipStream_t s);
+hipError_t launch_deposit(const MoveArgs<float>& a, int dep, hipStream_t s);
+hipError_t launch_locate(const LocateArgs<double>& a, hipStream_t s);
+hipError_t launch_locate(const LocateArgs<float>& a, hipStream_t s);
+hipError_t launch_positions(const PartSoA<double>& p, int64_t n, const TetGeo* tets, const double* C,
+                            const double* pts, double* x, hipStream_t s);
+hipError_t launch_positions(const PartSoA<float>& p, int64_t n, const TetGeo* tets, const double* C,
+                            const double* pts, double* x, hipStream_t s);
+hipError_t launch_init(const InitArgs<double>& a, hipStream_t s);
+hipError_t launch_init(const InitArgs<float>& a, hipStream_t s);
+hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const TetGeo* tets,
+                          const int* start, int* cursor, hipStream_t s);
+hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const TetGeo* tets,
+                          const int* start, int* cursor, hipStream_t s);
+
+// ---------------------------------------------------------------- fields (k_field.hip)
+// exclusive scan of n+1 ints (last = total); temp storage sized by scan_temp_bytes
+size_t scan_temp_bytes(int n);
+hipError_t launch_scan(const int* in, int* out, int n, void* temp, size_t temp_bytes, hipStream_t s);
+// max over n ints into *out (single block reduce + atomicMax)
+hipError_t launch_max_count(const int* counts, int n, int* out, hipStream_t s);
+
+struct DepositFinishArgs {
+    int n_cells;
+    int n_species;
+    const int* counts[kMaxSpecies];   // per species per cell
+    long long whole[kMaxSpecies];     // llrint(q*w/Q0 * 2^42) per species (CellAverage)
+    const unsigned long long* acc_cell;
+    const unsigned long long* acc_point;
+    const int32_t* cpo;               // cell -> cell points
+    const int32_t* cp_master;
+    const double* vw_w;
+    const double* V;
+    const double* space_charge;
+    double q42;                       // Q0 / 2^42
+    int weighting;
+    double* rho;
+};
+hipError_t launch_deposit_finish(const DepositFinishArgs& a, hipStream_t s);
+
+// E = -grad(phi) (Gauss linear) then point interpolation; Ecell_f / Epoint_f are the FP32
+// copies used by the mixed-precision push (may be null)
+struct GradArgs {
+    int n_cells;
+    const int32_t* gfo;
+    const int32_t* g_kind;
+    const int32_t* g_other;
+    const double* g_w;
+    const double* g_val;
+    const double* g_S;
+    const double* V;
+    const double* phi;
+    double* E;
+    float* E_f;
+};
+hipError_t launch_grad(const GradArgs& a, hipStream_t s);
+
+struct PointInterpArgs {
+    int n_points;
+    const int32_t* pco;
+    const int32_t* pc_cell;
+    const double* pc_w;
+    const int32_t* pmaster;
+    const double* Ecell;
+    double* Epoint;
+    float* Epoint_f;
+};
+hipError_t launch_point_interp(const PointInterpArgs& a, hipStream_t s);
+hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t s);
+
+// per-species moment sums for diagnostics, deterministic two-stage reduction:
+// out[0]=sum w, [1..3]=sum w U, [4]=sum w U^2, [5]=relativistic KE / (m c^2)
+hipError_t launch_diag(const PartSoA<double>& p, int64_t n, double* partial, int nblocks, double* out,
+                       hipStream_t s);
+hipError_t launch_diag(const PartSoA<float>& p, int64_t n, double* partial, int nblocks, double* out,
+                       hipStream_t s);
+hipError_t launch_field_energy(const double* E, const double* V, int n, double* partial, int nblocks, double* out,
+                               hipStream_t s);
+
+// ---------------------------------------------------------------- Coulomb (k_coulomb.hip)
+struct CoulombArgs {
+    int n_cells;
+    int n_species;
+    double mass[kMaxSpecies], charge[kMaxSpecies], weight[kMaxSpecies];
+    double coulomb_log[kMaxSpecies][kMaxSpecies];
+    int calc_debye, calculate_bmin, allow_intra;
+    const int* start[kMaxSpecies];    // cell_start per species [n_cells+1]
+    void* ux[kMaxSpecies];
+    void* uy[kMaxSpecies];
+    void* uz[kMaxSpecies];
+    const uint32_t* id[kMaxSpecies];
+    const double* V;
+    double dt;
+    uint64_t seed;
+    int64_t step;
+    int maxp;                         // LDS capacity per species per cell
+    uint32_t* flags;
+    unsigned long long* ncoll;
+};
+hipError_t launch_coulomb(const CoulombArgs& a, bool fp32_velocity, hipStream_t s);
+
+// ---------------------------------------------------------------- MCC (k_mcc.hip)
+template <class UT>
+struct MccArgs {
+    int n_cells;
+    PartSoA<UT> e;            // electrons, sorted by cell
+    const int* start;         // electron cell_start
+    int64_t* n_e;             // device count of electrons (appends)
+    int64_t cap_e;
+    PartSoA<UT> ion;          // created ions (may have null pointers when no ion species)
+    int64_t* n_i;
+    int64_t cap_i;
+    double* smax;             // sigma*cR max per cell
+    double* rem;              // candidate remainder per cell
+    const double* xs_e[3];    // energies per process
+    const double* xs_s[3];    // values
+    int xs_n[3];
+    double th_ex, th_iz;
+    double massE, massN, nN, vthN, drift[3], wEwN, wNwE;
+    int create_ion, create_electron;
+    double dt;
+    uint64_t seed;
+    int64_t step;
+    int maxp;
+    uint32_t* flags;
+    unsigned long long* counts;   // candidates, elastic, excitation, ionization
+    // deposit of created parcels
+    const TetGeo* tets;
+    int* count_e;                 // per-cell counts (CellAverage / sort histogram)
+    int* count_i;
+    unsigned long long* acc_cell;
+    unsigned long long* acc_point;
+    double s42_e, s42_i;
+    int dep_vw;
+};
+hipError_t launch_mcc(const MccArgs<double>& a, hipStream_t s);
+hipError_t launch_mcc(const MccArgs<float>& a, hipStream_t s);
+
+// ---------------------------------------------------------------- Poisson (k_poisson.hip)
+struct PoissonDev {
+    int n, K, ncol;
+    const int32_t* nb;
+    const double* a;
+    const float* a_f;
+    const double* diag;
+    const float* diag_f;
+    const double* rD;      // DIC diagonal
+    const float* rD_f;
+    const int32_t* colour;
+    const int32_t* corder;
+    const int32_t* col_start_h;  // host copy of colour starts
+    const double* bsrc;
+    const double* V;
+};
+hipError_t poisson_rhs(const PoissonDev& P, const double* rho, double* b, bool singular, double* scratch,
+                       hipStream_t s);
+// PCG with DIC in FP64 (mode 0) or FP32-inner CG inside FP64 iterative refinement (mode 1)
+struct PoissonWork {
+    double *r, *z, *p, *q, *x, *b;
+    float *rf, *zf, *pf, *qf, *xf;
+    double* red;          // reduction scratch (>= 4*nblocks + 16)
+    double* scal;         // device scalars
+    int nblocks;
+};
+int poisson_solve(const PoissonDev& P, PoissonWork& W, double* phi, const double* b, double tol, int max_iter,
+                  int mode, bool singular, int* iters, int* refinements, double* rel, hipStream_t s);
+
+}  // namespace picmi

@@ -1,0 +1,398 @@
+"""Python host mirror of the libpicmi C-ABI (include/picmi.h).
+
+``Cloud`` plays the role of picFoam's ``PICCloud``: it is built from a polyMesh, the
+``moleculeProperties`` species and the ``constant/picProperties`` model keywords, and
+exposes the stages of ``PICCloud::evolve()`` (PICCloud.C:1426-1471).  Every call goes
+through the HIP library; there is no CPU fallback.  If ``libpicmi.so`` has not been built
+the import raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpicmi.so")
+
+# status codes (picmi.h)
+PIC_OK = 0
+ERRORS = {
+    -1: "PIC_ERR_INVALID", -2: "PIC_ERR_UNKNOWN_MODEL", -3: "PIC_ERR_SUPERLUMINAL", -4: "PIC_ERR_HIP",
+    -5: "PIC_ERR_CAPACITY", -6: "PIC_ERR_MESH", -7: "PIC_ERR_UNSUPPORTED", -8: "PIC_ERR_CELL_OVERFLOW",
+    -9: "PIC_ERR_NOT_CONVERGED", -10: "PIC_ERR_NO_DEVICE",
+}
+PATCH_TYPES = {"patch": 0, "wall": 1, "cyclic": 2}
+PREC_FP64, PREC_MIXED = 0, 1
+POISSON_FP64, POISSON_MIXED_IR = 0, 1
+FIELD_E, FIELD_B, FIELD_PHI, FIELD_RHO, FIELD_SPACE_CHARGE, FIELD_E_POINT, FIELD_MCC_SMAX, FIELD_MCC_REM = range(8)
+XS_ELASTIC, XS_EXCITATION, XS_IONIZATION = range(3)
+STAGES = ("move", "sort", "coulomb", "mcc", "deposit", "poisson", "field")
+
+# exported entry points (every one declared in include/picmi.h)
+EXPORTS = (
+    "pic_version", "pic_device_count", "pic_create", "pic_destroy", "pic_last_error", "pic_mesh_sizes",
+    "pic_mesh_tets", "pic_mesh_geometry", "pic_set_cell_field", "pic_get_cell_field", "pic_set_cross_section",
+    "pic_mcc_initialize", "pic_particles_set", "pic_particles_set_positions", "pic_particles_count",
+    "pic_particles_get", "pic_particles_positions", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
+    "pic_collide_coulomb", "pic_collide_mcc", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
+    "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
+    "pic_profile_reset",
+)
+
+
+class PicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class pic_patch(C.Structure):
+    _fields_ = [("type", C.c_int32), ("start", C.c_int32), ("size", C.c_int32), ("neighbour_patch", C.c_int32),
+                ("phi_bc", C.c_int32), ("phi_value", C.c_double)]
+
+
+class pic_mesh(C.Structure):
+    _fields_ = [("n_points", C.c_int32), ("n_faces", C.c_int32), ("n_internal_faces", C.c_int32),
+                ("n_cells", C.c_int32), ("n_patches", C.c_int32), ("points", C.c_void_p),
+                ("face_offsets", C.c_void_p), ("face_points", C.c_void_p), ("owner", C.c_void_p),
+                ("neighbour", C.c_void_p), ("patches", C.POINTER(pic_patch))]
+
+
+class pic_species(C.Structure):
+    _fields_ = [("mass", C.c_double), ("charge", C.c_double), ("n_equivalent", C.c_double),
+                ("solve_movement", C.c_int32), ("capacity", C.c_int64)]
+
+
+class pic_models(C.Structure):
+    _fields_ = [("particle_pusher", C.c_char_p), ("field_weighting", C.c_char_p),
+                ("charge_distribution", C.c_char_p), ("maxwell_solver", C.c_char_p),
+                ("coulomb_collision", C.c_char_p), ("pairing_algorithm", C.c_char_p),
+                ("weight_correction", C.c_char_p), ("electron_neutral", C.c_char_p),
+                ("background_gas", C.c_char_p), ("allow_intra_collision", C.c_int32),
+                ("calculate_bmin", C.c_int32), ("coulomb_log", (C.c_double * 8) * 8),
+                ("electron_species", C.c_int32), ("ion_species", C.c_int32),
+                ("bg_number_density", C.c_double), ("bg_temperature", C.c_double), ("bg_mass", C.c_double),
+                ("bg_n_equivalent", C.c_double), ("bg_velocity", C.c_double * 3), ("create_ion", C.c_int32),
+                ("create_electron", C.c_int32), ("precision", C.c_int32), ("hc_reference_form", C.c_int32),
+                ("seed", C.c_uint64), ("poisson_tolerance", C.c_double), ("poisson_max_iter", C.c_int32),
+                ("poisson_mode", C.c_int32), ("sort_interval", C.c_int32)]
+
+
+class pic_poisson_stats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("refinements", C.c_int32), ("rel_residual", C.c_double),
+                ("ms", C.c_double)]
+
+
+class pic_step_stats(C.Structure):
+    _fields_ = [("n_parcels", C.c_int64 * 8), ("n_deleted", C.c_int64), ("n_track_guard", C.c_int64),
+                ("coulomb_collisions", C.c_int64), ("mcc_candidates", C.c_int64), ("mcc_elastic", C.c_int64),
+                ("mcc_excitation", C.c_int64), ("mcc_ionization", C.c_int64), ("poisson", pic_poisson_stats)]
+
+
+class pic_diag(C.Structure):
+    _fields_ = [("sum_w", C.c_double * 8), ("sum_wU", (C.c_double * 3) * 8), ("sum_wU2", C.c_double * 8),
+                ("kinetic_energy", C.c_double * 8), ("field_energy", C.c_double), ("n_parcels", C.c_int64 * 8)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libpicmi.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m picfoam_amd.build` "
+                          "(the particle step has no CPU fallback)")
+    lib = C.CDLL(path)
+    vp, i32, i64, dbl, u32 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint32
+    sig = {
+        "pic_version": (C.c_char_p, []),
+        "pic_device_count": (C.c_int, [C.POINTER(i32)]),
+        "pic_create": (C.c_int, [C.POINTER(pic_mesh), C.POINTER(pic_species), i32, C.POINTER(pic_models),
+                                 C.POINTER(vp)]),
+        "pic_destroy": (None, [vp]),
+        "pic_last_error": (C.c_char_p, [vp]),
+        "pic_mesh_sizes": (C.c_int, [vp, C.POINTER(i64), C.POINTER(i32), C.POINTER(i32)]),
+        "pic_mesh_tets": (C.c_int, [vp, vp, vp, vp]),
+        "pic_mesh_geometry": (C.c_int, [vp, vp, vp]),
+        "pic_set_cell_field": (C.c_int, [vp, i32, vp]),
+        "pic_get_cell_field": (C.c_int, [vp, i32, vp]),
+        "pic_set_cross_section": (C.c_int, [vp, i32, i32, vp, vp, dbl]),
+        "pic_mcc_initialize": (C.c_int, [vp, dbl, dbl]),
+        "pic_particles_set": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
+        "pic_particles_set_positions": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
+        "pic_particles_count": (C.c_int, [vp, i32, C.POINTER(i64)]),
+        "pic_particles_get": (C.c_int, [vp, i32, i64, C.POINTER(i64), vp, vp, vp, vp]),
+        "pic_particles_positions": (C.c_int, [vp, i32, i64, vp]),
+        "pic_init_maxwellian": (C.c_int, [vp, i32, i32, dbl, vp, u32, C.c_uint64]),
+        "pic_push": (C.c_int, [vp, dbl]),
+        "pic_move": (C.c_int, [vp, dbl]),
+        "pic_sort": (C.c_int, [vp]),
+        "pic_collide_coulomb": (C.c_int, [vp, dbl]),
+        "pic_collide_mcc": (C.c_int, [vp, dbl]),
+        "pic_deposit": (C.c_int, [vp]),
+        "pic_poisson_solve": (C.c_int, [vp, C.POINTER(pic_poisson_stats)]),
+        "pic_diag_reduce": (C.c_int, [vp, C.POINTER(pic_diag)]),
+        "pic_step": (C.c_int, [vp, dbl, C.POINTER(pic_step_stats)]),
+        "pic_sync": (C.c_int, [vp]),
+        "pic_step_index": (i64, [vp]),
+        "pic_set_step_index": (C.c_int, [vp, i64]),
+        "pic_profile_enable": (C.c_int, [vp, i32]),
+        "pic_stage_times": (C.c_int, [vp, vp, vp]),
+        "pic_profile_reset": (C.c_int, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = C.c_int32(0)
+    lib.pic_device_count(C.byref(n))
+    return n.value
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Mesh:
+    """Keeps the polyMesh arrays alive for the C struct."""
+
+    def __init__(self, mesh: dict, phi_bc: dict | None = None):
+        self.d = mesh
+        self.points = np.ascontiguousarray(mesh["points"], dtype=np.float64)
+        self.fo = np.ascontiguousarray(mesh["face_offsets"], dtype=np.int32)
+        self.fp = np.ascontiguousarray(mesh["face_points"], dtype=np.int32)
+        self.owner = np.ascontiguousarray(mesh["owner"], dtype=np.int32)
+        self.neighbour = np.ascontiguousarray(mesh["neighbour"], dtype=np.int32)
+        self.n_cells = int(mesh["n_cells"])
+        phi_bc = phi_bc or {}
+        pts = (pic_patch * max(1, len(mesh["patches"])))()
+        for i, p in enumerate(mesh["patches"]):
+            bc = phi_bc.get(p["name"], ("zeroGradient", 0.0))
+            pts[i].type = PATCH_TYPES[p["type"]]
+            pts[i].start = p["start"]
+            pts[i].size = p["size"]
+            pts[i].neighbour_patch = p["neighbour_patch"]
+            pts[i].phi_bc = 1 if bc[0] == "fixedValue" else 0
+            pts[i].phi_value = float(bc[1])
+        self._patches = pts
+        self.c = pic_mesh(len(self.points), len(self.owner), len(self.neighbour), self.n_cells,
+                          len(mesh["patches"]), _ptr(self.points), _ptr(self.fo), _ptr(self.fp), _ptr(self.owner),
+                          _ptr(self.neighbour), pts)
+
+
+def _enc(s):
+    return None if s is None else str(s).encode()
+
+
+class Cloud:
+    """PICCloud counterpart on one HIP device."""
+
+    def __init__(self, mesh: dict, species: list[dict], models: dict, phi_bc: dict | None = None):
+        self.lib = load_library()
+        self.mesh = Mesh(mesh, phi_bc)
+        self.species = species
+        sp = (pic_species * len(species))()
+        for i, s in enumerate(species):
+            sp[i].mass = s["mass"]
+            sp[i].charge = s["charge"]
+            sp[i].n_equivalent = s["nEquivalentParticles"]
+            sp[i].solve_movement = 1 if s.get("solveMovement", True) else 0
+            sp[i].capacity = int(s["capacity"])
+        m = pic_models()
+        keep = []
+        for key, field, default in (("ParticlePusher", "particle_pusher", "Boris"),
+                                    ("FieldWeigthing", "field_weighting", "CellAverage"),
+                                    ("ChargeDistribution", "charge_distribution", "CellAverage"),
+                                    ("MaxwellSolver", "maxwell_solver", "none"),
+                                    ("CoulombCollisionModel", "coulomb_collision", "none"),
+                                    ("PairingAlgorithm", "pairing_algorithm", "TakizukaAbe"),
+                                    ("WeightCorrectionModel", "weight_correction", "none"),
+                                    ("ElectronNeutralCollisionModel", "electron_neutral", "none"),
+                                    ("BackgroundGasModel", "background_gas", "none")):
+            v = _enc(models.get(key, default))
+            keep.append(v)
+            setattr(m, field, v)
+        self._keep = keep
+        m.allow_intra_collision = 1 if models.get("allowIntraCollision", True) else 0
+        m.calculate_bmin = 1 if models.get("calculate_bmin", True) else 0
+        cl = np.zeros((8, 8))
+        for (i, j), v in models.get("coulombLog", {}).items():
+            cl[i, j] = cl[j, i] = v
+        for i in range(8):
+            for j in range(8):
+                m.coulomb_log[i][j] = cl[i, j]
+        m.electron_species = int(models.get("electronTypeId", -1))
+        m.ion_species = int(models.get("ionTypeId", -1))
+        bg = models.get("uniformBackground", {})
+        m.bg_number_density = bg.get("numberDensity", 0.0)
+        m.bg_temperature = bg.get("temperature", 0.0)
+        m.bg_mass = bg.get("mass", 0.0)
+        m.bg_n_equivalent = bg.get("nEquivalentParticles", 0.0)
+        for d, v in enumerate(bg.get("velocity", (0.0, 0.0, 0.0))):
+            m.bg_velocity[d] = v
+        m.create_ion = 1 if models.get("createIon", True) else 0
+        m.create_electron = 1 if models.get("createElectron", True) else 0
+        m.precision = {"fp64": PREC_FP64, "mixed": PREC_MIXED}[models.get("precision", "fp64")]
+        m.hc_reference_form = 1 if models.get("hcReferenceForm", False) else 0
+        m.seed = int(models.get("seed", 12345))
+        m.poisson_tolerance = float(models.get("poissonTolerance", 1e-8))
+        m.poisson_max_iter = int(models.get("poissonMaxIter", 20000))
+        m.poisson_mode = {"fp64": POISSON_FP64, "mixed": POISSON_MIXED_IR}[models.get("poissonMode", "fp64")]
+        m.sort_interval = int(models.get("sortInterval", 1))
+        self._models = m
+        h = C.c_void_p()
+        rc = self.lib.pic_create(C.byref(self.mesh.c), sp, len(species), C.byref(m), C.byref(h))
+        if rc != PIC_OK:
+            raise PicError(rc, self.lib.pic_last_error(None).decode())
+        self.h = h
+        nt, nc, npnt = C.c_int64(), C.c_int32(), C.c_int32()
+        self.lib.pic_mesh_sizes(self.h, C.byref(nt), C.byref(nc), C.byref(npnt))
+        self.n_tets, self.n_cells, self.n_points = nt.value, nc.value, npnt.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != PIC_OK:
+            raise PicError(rc, self.lib.pic_last_error(self.h).decode())
+
+    # mesh
+    def tets(self):
+        tc = np.empty(self.n_tets, np.int32); tf = np.empty(self.n_tets, np.int32); tp = np.empty(self.n_tets, np.int32)
+        self._check(self.lib.pic_mesh_tets(self.h, _ptr(tc), _ptr(tf), _ptr(tp)))
+        return tc, tf, tp
+
+    def geometry(self):
+        cc = np.empty((self.n_cells, 3)); cv = np.empty(self.n_cells)
+        self._check(self.lib.pic_mesh_geometry(self.h, _ptr(cc), _ptr(cv)))
+        return cc, cv
+
+    # fields
+    def set_field(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(self.lib.pic_set_cell_field(self.h, field, _ptr(v)))
+
+    def get_field(self, field):
+        n = {FIELD_E: 3 * self.n_cells, FIELD_B: 3 * self.n_cells, FIELD_PHI: self.n_cells,
+             FIELD_RHO: self.n_cells, FIELD_SPACE_CHARGE: self.n_cells, FIELD_E_POINT: 3 * self.n_points,
+             FIELD_MCC_SMAX: self.n_cells, FIELD_MCC_REM: self.n_cells}[field]
+        out = np.empty(n)
+        self._check(self.lib.pic_get_cell_field(self.h, field, _ptr(out)))
+        return out.reshape(-1, 3) if field in (FIELD_E, FIELD_B, FIELD_E_POINT) else out
+
+    def set_cross_section(self, process, energies, values, threshold=0.0):
+        e = np.ascontiguousarray(energies, dtype=np.float64); s = np.ascontiguousarray(values, dtype=np.float64)
+        self._check(self.lib.pic_set_cross_section(self.h, process, len(e), _ptr(e), _ptr(s), float(threshold)))
+
+    def mcc_initialize(self, maximum_energy, energy_step):
+        self._check(self.lib.pic_mcc_initialize(self.h, maximum_energy, energy_step))
+
+    # particles
+    def set_particles(self, s, bary, tet, U, ids):
+        bary = np.ascontiguousarray(bary, dtype=np.float64); tet = np.ascontiguousarray(tet, dtype=np.int32)
+        U = np.ascontiguousarray(U, dtype=np.float64); ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        self._check(self.lib.pic_particles_set(self.h, s, len(tet), _ptr(bary), _ptr(tet), _ptr(U), _ptr(ids)))
+
+    def set_positions(self, s, x, cell, U, ids):
+        x = np.ascontiguousarray(x, dtype=np.float64); cell = np.ascontiguousarray(cell, dtype=np.int32)
+        U = np.ascontiguousarray(U, dtype=np.float64); ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        self._check(self.lib.pic_particles_set_positions(self.h, s, len(cell), _ptr(x), _ptr(cell), _ptr(U),
+                                                         _ptr(ids)))
+
+    def count(self, s):
+        n = C.c_int64()
+        self._check(self.lib.pic_particles_count(self.h, s, C.byref(n)))
+        return n.value
+
+    def get_particles(self, s):
+        n = self.count(s)
+        bary = np.empty((n, 3)); tet = np.empty(n, np.int32); U = np.empty((n, 3)); ids = np.empty(n, np.uint32)
+        nout = C.c_int64()
+        self._check(self.lib.pic_particles_get(self.h, s, n, C.byref(nout), _ptr(bary), _ptr(tet), _ptr(U),
+                                               _ptr(ids)))
+        return bary, tet, U, ids
+
+    def positions(self, s):
+        n = self.count(s)
+        x = np.empty((n, 3))
+        self._check(self.lib.pic_particles_positions(self.h, s, n, _ptr(x)))
+        return x
+
+    def init_maxwellian(self, s, nppc, temperature, drift=(0.0, 0.0, 0.0), id0=0, seed=1):
+        d = np.ascontiguousarray(drift, dtype=np.float64)
+        self._check(self.lib.pic_init_maxwellian(self.h, s, nppc, temperature, _ptr(d), id0, seed))
+
+    # stages
+    def push(self, dt):
+        self._check(self.lib.pic_push(self.h, dt))
+
+    def move(self, dt):
+        self._check(self.lib.pic_move(self.h, dt))
+
+    def sort(self):
+        self._check(self.lib.pic_sort(self.h))
+
+    def collide_coulomb(self, dt):
+        self._check(self.lib.pic_collide_coulomb(self.h, dt))
+
+    def collide_mcc(self, dt):
+        self._check(self.lib.pic_collide_mcc(self.h, dt))
+
+    def deposit(self):
+        self._check(self.lib.pic_deposit(self.h))
+        return self.get_field(FIELD_RHO)
+
+    def poisson_solve(self):
+        st = pic_poisson_stats()
+        self._check(self.lib.pic_poisson_solve(self.h, C.byref(st)))
+        return {"iterations": st.iterations, "refinements": st.refinements, "rel_residual": st.rel_residual,
+                "ms": st.ms}
+
+    def step(self, dt):
+        st = pic_step_stats()
+        self._check(self.lib.pic_step(self.h, dt, C.byref(st)))
+        return st
+
+    def diag(self):
+        d = pic_diag()
+        self._check(self.lib.pic_diag_reduce(self.h, C.byref(d)))
+        return d
+
+    def sync(self):
+        self._check(self.lib.pic_sync(self.h))
+
+    @property
+    def step_index(self):
+        return self.lib.pic_step_index(self.h)
+
+    @step_index.setter
+    def step_index(self, v):
+        self._check(self.lib.pic_set_step_index(self.h, int(v)))
+
+    def profile(self, on=True):
+        self._check(self.lib.pic_profile_enable(self.h, 1 if on else 0))
+        self._check(self.lib.pic_profile_reset(self.h))
+
+    def stage_times(self):
+        ms = np.zeros(len(STAGES)); n = np.zeros(len(STAGES), np.int64)
+        self._check(self.lib.pic_stage_times(self.h, _ptr(ms), _ptr(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(STAGES)}

@@ -1,0 +1,347 @@
+"""HIP path against the FP64 CPU oracle on identical inputs (through the C-ABI).
+
+FP64 push, track and deposit: 1e-12 relative (positions relative to the box size);
+int64 deposit: bit-exact on identical coordinates; mesh tables: exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import ME, MP, QE, C_LIGHT, EPS0, periodic_diff, random_bary, random_parcels, rel_err, small_hex, \
+    small_tet, sort_by_id
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def cloud(mesh, models, species=None, phi_bc=None):
+    from picfoam_amd import picmi
+    if species is None:
+        species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": 1e5, "capacity": 20000}]
+    return picmi.Cloud(mesh, species, models, phi_bc)
+
+
+MESHES = {"hex": lambda: small_hex(), "tet": lambda: small_tet(n=3, L=1e-3, jitter=0.2)}
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet"])
+def test_mesh_tables_match_oracle(oracle, kind):
+    say("mesh tables", kind)
+    m = MESHES[kind]()
+    c = cloud(m, {"ParticlePusher": "Boris"})
+    rm = oracle.RefMesh(m)
+    for a, b in zip(c.tets(), rm.tets()):
+        assert np.array_equal(a, b)
+    cc, cv = c.geometry()
+    rcc, rcv, _, _ = rm.geometry()
+    assert np.array_equal(cc, rcc) and np.array_equal(cv, rcv)
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet"])
+@pytest.mark.parametrize("pusher", ["Boris", "BorisNR", "Vay", "HigueraCary"])
+@pytest.mark.parametrize("weighting", ["CellAverage", "VolumeWeighting"])
+def test_push_fp64_parity(oracle, kind, pusher, weighting):
+    say("push parity", kind, pusher, weighting)
+    rng = np.random.default_rng(100)
+    m = MESHES[kind]()
+    c = cloud(m, {"ParticlePusher": pusher, "FieldWeigthing": weighting, "ChargeDistribution": weighting})
+    rm = oracle.RefMesh(m)
+    n = 5000
+    y, tet, U, ids = random_parcels(rng, rm.n_tets, n, 0.5 * C_LIGHT)
+    E = rng.normal(size=(rm.n_cells, 3)) * 1e6
+    B = rng.normal(size=(rm.n_cells, 3)) * 0.5
+    c.set_field(0, E); c.set_field(1, B)
+    c.set_particles(0, y, tet, U, ids)
+    dt = 2e-12
+    c.push(dt)
+    _, _, Ug, _ = c.get_particles(0)
+    Ep = rm.cell_to_point(E)
+    Ur = rm.push(pusher, weighting, y, U, tet, -QE, ME, dt, E, Ep, B)
+    assert rel_err(Ug, Ur) < TOL
+    if weighting == "VolumeWeighting":
+        assert rel_err(c.get_field(5), Ep) < TOL
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet"])
+@pytest.mark.parametrize("weighting", ["CellAverage", "VolumeWeighting"])
+def test_move_fp64_parity(oracle, kind, weighting):
+    """push + interpolation + tracking across several cells and periodic boundaries"""
+    say("move parity", kind, weighting)
+    rng = np.random.default_rng(101)
+    m = MESHES[kind]()
+    c = cloud(m, {"ParticlePusher": "Boris", "FieldWeigthing": weighting, "ChargeDistribution": weighting})
+    rm = oracle.RefMesh(m)
+    L = np.array(m["lengths"])
+    n = 8000
+    dt = 1e-10
+    y, tet, U, ids = random_parcels(rng, rm.n_tets, n, 2.0 * L.min() / dt)
+    E = rng.normal(size=(rm.n_cells, 3)) * 1e4
+    B = rng.normal(size=(rm.n_cells, 3)) * 0.05
+    c.set_field(0, E); c.set_field(1, B)
+    c.set_particles(0, y, tet, U, ids)
+    c.move(dt)
+    yg, tg, Ug, ig = c.get_particles(0)
+    assert np.array_equal(ig, ids)  # move keeps the order
+    Ep = rm.cell_to_point(E)
+    yr, Ur, tr, guards, sl = rm.move("Boris", weighting, y, U, tet, -QE, ME, dt, E, Ep, B)
+    assert guards == 0
+    assert rel_err(Ug, Ur) < TOL
+    xg = rm.positions(yg, tg)
+    xr = rm.positions(yr, tr)
+    d = periodic_diff(xg, xr, L)
+    assert np.max(np.abs(d)) / L.max() < TOL
+    # same cell almost everywhere (a parcel exactly on a face may sit on either side)
+    tc, _, _ = rm.tets()
+    assert np.mean(tc[tg] == tc[tr]) > 0.999
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet"])
+@pytest.mark.parametrize("weighting", ["CellAverage", "VolumeWeighting"])
+def test_deposit_bit_exact(oracle, kind, weighting):
+    say("deposit parity", kind, weighting)
+    rng = np.random.default_rng(102)
+    m = MESHES[kind]()
+    species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": 3e5, "capacity": 20000},
+               {"mass": 40 * MP, "charge": QE, "nEquivalentParticles": 1e5, "capacity": 20000}]
+    c = cloud(m, {"ParticlePusher": "Boris", "FieldWeigthing": weighting, "ChargeDistribution": weighting}, species)
+    rm = oracle.RefMesh(m)
+    parts = []
+    for s in range(2):
+        y, tet, U, ids = random_parcels(rng, rm.n_tets, 7000 + 1000 * s, 1.0)
+        c.set_particles(s, y, tet, U, ids)
+        parts.append((y, tet, species[s]["charge"] * species[s]["nEquivalentParticles"]))
+    rho_g = c.deposit()
+    Q0 = max(abs(p[2]) for p in parts)
+    ac, ap = rm.deposit(weighting, parts, Q0)
+    rho_r = rm.deposit_finish(weighting, ac, ap, Q0)
+    assert np.array_equal(rho_g, rho_r), rel_err(rho_g, rho_r)
+
+
+def test_sort_groups_by_cell(oracle):
+    say("sort")
+    rng = np.random.default_rng(103)
+    m = small_tet()
+    c = cloud(m, {"ParticlePusher": "Boris"})
+    rm = oracle.RefMesh(m)
+    y, tet, U, ids = random_parcels(rng, rm.n_tets, 9000, 1.0)
+    tet[::7] = -1  # deleted parcels are compacted away
+    c.set_particles(0, y, tet, U, ids)
+    c.sort()
+    yg, tg, Ug, ig = c.get_particles(0)
+    tc, _, _ = rm.tets()
+    cells = tc[tg]
+    assert np.all(np.diff(cells) >= 0)
+    order, cs = rm.sort_by_cell(tet)
+    assert len(ig) == len(order)
+    for cidx in range(rm.n_cells):
+        a = np.sort(ig[cells == cidx]); b = np.sort(ids[order[cs[cidx]:cs[cidx + 1]]])
+        assert np.array_equal(a, b)
+    # payload travelled with the id
+    o = np.argsort(ig)
+    live = np.nonzero(tet >= 0)[0]
+    assert np.array_equal(yg[o], y[live]) and np.array_equal(Ug[o], U[live]) and np.array_equal(tg[o], tet[live])
+
+
+def _two_species_cell_setup(rng, m, rm, ne_per, ni_per, Te=10.0, Ti=1.0, mi=1836 * ME):
+    tc, _, _ = rm.tets()
+    first_tet = np.array([np.nonzero(tc == c)[0][0] for c in range(rm.n_cells)], np.int32)
+
+    def make(counts, T, mass, id0):
+        tet = np.concatenate([np.full(k, first_tet[c], np.int32) for c, k in enumerate(counts)])
+        n = len(tet)
+        y = random_bary(rng, n)
+        U = rng.normal(size=(n, 3)) * np.sqrt(T * QE / mass)
+        return y, tet, U, (np.arange(n) + id0).astype(np.uint32)
+
+    return make(ne_per, Te, ME, 0), make(ni_per, Ti, mi, 0)
+
+
+@pytest.mark.parametrize("counts", ["equal", "ragged"])
+def test_coulomb_fp64_parity(oracle, counts):
+    say("coulomb parity", counts)
+    rng = np.random.default_rng(104)
+    m = small_hex(n=(3, 2, 2), L=(1e-4, 1e-4, 1e-4))
+    rm = oracle.RefMesh(m)
+    nc = rm.n_cells
+    if counts == "equal":
+        ne = np.full(nc, 64); ni = np.full(nc, 64)
+    else:
+        ne = rng.integers(0, 90, nc); ni = rng.integers(0, 90, nc)
+        ne[0], ni[0] = 7, 3
+        ne[1], ni[1] = 1, 0
+        ne[2], ni[2] = 33, 33
+    mi = 1836 * ME
+    w = 1e6
+    species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": w, "capacity": 20000},
+               {"mass": mi, "charge": QE, "nEquivalentParticles": w, "capacity": 20000}]
+    for lnl in (10.0, 0.0):
+        cl = {} if lnl == 0 else {(0, 0): lnl, (0, 1): lnl, (1, 1): lnl}
+        c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu", "coulombLog": cl, "seed": 77},
+                  species)
+        (ye, te, Ue, ie), (yi, ti, Ui, ii) = _two_species_cell_setup(rng, m, rm, ne, ni, mi=mi)
+        c.set_particles(0, ye, te, Ue, ie)
+        c.set_particles(1, yi, ti, Ui, ii)
+        c.step_index = 5
+        dt = 1e-9
+        c.collide_coulomb(dt)
+        _, _, Ueg, ieg = c.get_particles(0)
+        _, _, Uig, iig = c.get_particles(1)
+        # oracle on arrays sorted by cell (setup is already cell-ordered)
+        cse = np.concatenate([[0], np.cumsum(ne)]).astype(np.int32)
+        csi = np.concatenate([[0], np.cumsum(ni)]).astype(np.int32)
+        clm = np.zeros((8, 8))
+        if lnl:
+            clm[:2, :2] = lnl
+        Us, st = oracle.coulomb(rm, [(ME, -QE, w), (mi, QE, w)], [Ue, Ui], [ie, ii], [cse, csi], dt, seed=77,
+                                step=5, coulomb_log=clm)
+        (Ueg_s,) = sort_by_id(ieg, Ueg)
+        (Uig_s,) = sort_by_id(iig, Uig)
+        assert rel_err(Ueg_s, Us[0]) < TOL
+        assert rel_err(Uig_s, Us[1]) < TOL
+        assert np.any(Us[0] != Ue)
+
+
+def test_mcc_fp64_parity(oracle):
+    say("mcc parity")
+    rng = np.random.default_rng(105)
+    from picfoam_amd import cases
+    m = small_hex(n=(3, 3, 2), L=(1e-3, 1e-3, 1e-3))
+    rm = oracle.RefMesh(m)
+    nc = rm.n_cells
+    e, el, ex, iz = cases.argon_like_cross_sections()
+    mAr = 39.948 * cases.AMU
+    w = 1.0
+    nN = 1e21
+    species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": w, "capacity": 40000},
+               {"mass": mAr, "charge": QE, "nEquivalentParticles": w, "capacity": 40000, "solveMovement": False}]
+    models = {"ParticlePusher": "Boris", "ElectronNeutralCollisionModel": "NanbuElectronNeutral",
+              "BackgroundGasModel": "uniformBackground", "electronTypeId": 0, "ionTypeId": 1,
+              "uniformBackground": {"numberDensity": nN, "temperature": 300.0, "mass": mAr,
+                                    "nEquivalentParticles": w}, "seed": 99}
+    c = cloud(m, models, species)
+    for k, (s_, th) in enumerate(((el, 0.0), (ex, 11.5), (iz, 15.76))):
+        c.set_cross_section(k, e, s_, th)
+    c.mcc_initialize(1000.0, 0.01)
+    ne = rng.integers(50, 150, nc)
+    (ye, te, Ue, ie), _ = _two_species_cell_setup(rng, m, rm, ne, np.zeros(nc, int), Te=30.0)
+    c.set_particles(0, ye, te, Ue, ie)
+    c.step_index = 3
+    dt = 2e-11
+    smax0 = c.get_field(6).copy()
+    rem0 = c.get_field(7).copy()
+    assert np.all(smax0 > 0) and np.all((rem0 >= 0) & (rem0 < 1))
+    c.collide_mcc(dt)
+    _, teg, Ueg, ieg = c.get_particles(0)
+    _, tig, Uig, iig = c.get_particles(1)
+    cs = np.concatenate([[0], np.cumsum(ne)]).astype(np.int32)
+    P = {"xs": [(e, el), (e, ex), (e, iz)], "massE": ME, "massN": mAr, "nN": nN, "TN": 300.0, "wE": w, "wN": w,
+         "th_ex": 11.5, "th_iz": 15.76, "seed": 99, "step": 3}
+    out = oracle.mcc(rm, P, dt, ye, Ue, te, ie, cs, 40000, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.int32),
+                     np.zeros(0, np.uint32), 40000, smax0, rem0)
+    assert out["err"] == 0
+    assert len(ieg) == len(out["idE"]) and len(iig) == len(out["idI"])
+    (Ueg_s, ieg_s) = sort_by_id(ieg, Ueg, ieg)
+    (Uer_s, ier_s) = sort_by_id(out["idE"], out["UE"], out["idE"])
+    assert np.array_equal(ieg_s, ier_s)
+    assert rel_err(Ueg_s, Uer_s) < 1e-11
+    (Uig_s,) = sort_by_id(iig, Uig)
+    (Uir_s,) = sort_by_id(out["idI"], out["UI"])
+    assert rel_err(Uig_s, Uir_s) < 1e-11
+    assert out["counts"][0] > 0
+    assert np.array_equal(c.get_field(7), out["rem"])
+    assert np.array_equal(c.get_field(6), out["smax"])
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_poisson_parity(oracle, mode):
+    say("poisson parity", mode)
+    rng = np.random.default_rng(106)
+    m = small_hex(n=(8, 6, 4), L=(1.0, 1.0, 1.0), cyclic=(False, True, True))
+    bc = {"xmin": ("fixedValue", 0.0), "xmax": ("fixedValue", 1.0)}
+    c = cloud(m, {"ParticlePusher": "Boris", "MaxwellSolver": "ElectroStatic", "poissonMode": mode,
+                  "poissonTolerance": 1e-10}, phi_bc=bc)
+    rm = oracle.RefMesh(m)
+    P = oracle.RefPoisson(rm, bc)
+    rho = rng.normal(size=rm.n_cells) * EPS0
+    c.set_field(4, rho)  # space charge only, no parcels
+    c.deposit()
+    st = c.poisson_solve()
+    phi_g = c.get_field(2)
+    phi_r, it, rel = P.solve(rho, tol=1e-12)
+    assert st["rel_residual"] <= 1e-10
+    assert rel_err(phi_g, phi_r) < 1e-8
+    Eg = c.get_field(0)
+    Er = -P.grad(phi_r)
+    assert rel_err(Eg, Er) < 1e-7
+    # gradient parity on identical phi
+    c.set_field(2, phi_r)
+
+
+def test_grad_and_point_interp_parity(oracle):
+    say("grad parity")
+    rng = np.random.default_rng(107)
+    m = small_hex(n=(4, 4, 6), L=(1.0, 1.0, 1.5), cyclic=(False, True, True))
+    bc = {"xmin": ("fixedValue", 0.5), "xmax": ("fixedValue", -1.0)}
+    c = cloud(m, {"ParticlePusher": "Boris", "MaxwellSolver": "ElectroStatic", "FieldWeigthing": "VolumeWeighting",
+                  "poissonTolerance": 1e-12}, phi_bc=bc)
+    rm = oracle.RefMesh(m)
+    P = oracle.RefPoisson(rm, bc)
+    rho = rng.normal(size=rm.n_cells) * EPS0
+    c.set_field(4, rho)
+    c.deposit()
+    c.poisson_solve()
+    phi = c.get_field(2)
+    Er = -P.grad(phi)
+    assert rel_err(c.get_field(0), Er) < 1e-13
+    assert rel_err(c.get_field(5), rm.cell_to_point(Er)) < 1e-13
+
+
+def test_step_equals_stage_sequence(oracle):
+    say("step composition")
+    rng = np.random.default_rng(108)
+    m = small_hex(n=(4, 4, 4), L=(1e-4, 1e-4, 1e-4))
+    mi = 1836 * ME
+    w = 1e6
+    species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": w, "capacity": 20000},
+               {"mass": mi, "charge": QE, "nEquivalentParticles": w, "capacity": 20000}]
+    models = {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu",
+              "coulombLog": {(0, 0): 10.0, (0, 1): 10.0, (1, 1): 10.0}, "seed": 3}
+    rm = oracle.RefMesh(m)
+    (ye, te, Ue, ie), (yi, ti, Ui, ii) = _two_species_cell_setup(rng, m, rm, np.full(64, 40), np.full(64, 40), mi=mi)
+    res = []
+    for mode in ("step", "stages"):
+        c = cloud(m, models, species)
+        c.set_particles(0, ye, te, Ue, ie); c.set_particles(1, yi, ti, Ui, ii)
+        dt = 1e-11
+        if mode == "step":
+            c.step(dt)
+        else:
+            c.move(dt); c.collide_coulomb(dt); c.step_index = 1
+        out = []
+        for s in range(2):
+            y, t, U, i = c.get_particles(s)
+            out.append(sort_by_id(i, y, t, U))
+        res.append(out)
+    for s in range(2):
+        for a, b in zip(res[0][s], res[1][s]):
+            assert np.array_equal(a, b)
+
+
+def test_superluminal_is_an_error():
+    say("superluminal")
+    from picfoam_amd import picmi
+    m = small_hex(n=(2, 2, 2), L=(1.0, 1.0, 1.0))
+    c = cloud(m, {"ParticlePusher": "BorisNR"})
+    c.set_field(0, np.tile([0.0, 0.0, 1e12], (8, 1)))
+    c.set_particles(0, np.full((1, 3), 0.1), np.array([0], np.int32), np.array([[0.0, 0.0, 0.9 * C_LIGHT]]),
+                    np.array([1], np.uint32))
+    with pytest.raises(picmi.PicError) as ei:
+        c.move(1e-6)
+    assert ei.value.code == -3
