@@ -199,6 +199,20 @@ int pic_profile_enable(pic_ctx* ctx, int32_t on);
 int pic_stage_times(pic_ctx* ctx, double* ms /*[PIC_STAGE_COUNT]*/, int64_t* launches /*[PIC_STAGE_COUNT]*/);
 int pic_profile_reset(pic_ctx* ctx);
 
+/* learned initial guess for the Poisson PCG (no counterpart in the reference): a
+ * message-passing network over the cell graph with BF16 MFMA kernels, trained on the
+ * device on (rho, phi_prev) -> phi snapshots the solver itself records */
+int pic_gnn_init(pic_ctx* ctx, int32_t layers, int32_t width, uint64_t seed);
+int pic_gnn_n_params(pic_ctx* ctx, int64_t* n);
+int pic_gnn_params(pic_ctx* ctx, int64_t n, float* get, const float* set);
+int pic_gnn_record(pic_ctx* ctx, int32_t capacity);   /* record every Poisson solve into a ring buffer */
+int pic_gnn_snapshot_count(pic_ctx* ctx, int32_t* n);
+int pic_gnn_snapshot(pic_ctx* ctx, int32_t index, float* features, float* target, double* norm);
+int pic_gnn_grad(pic_ctx* ctx, int32_t index, float* grad, double* loss);
+int pic_gnn_train(pic_ctx* ctx, int32_t steps, double lr, uint64_t seed, double* loss_first, double* loss_last);
+int pic_gnn_eval(pic_ctx* ctx, int32_t index, double* loss);
+int pic_gnn_use(pic_ctx* ctx, int32_t on);            /* start the PCG from the network's guess */
+
 #ifdef __cplusplus
 }
 #endif

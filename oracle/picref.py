@@ -334,3 +334,44 @@ class RefPoisson:
         g = np.empty((self.rm.n_cells, 3))
         lib().ref_grad(self.h, _p(phi), _p(g))
         return g
+
+
+def gnn_n_params(fin, H, L):
+    L_ = lib()
+    L_.ref_gnn_n_params.restype = C.c_int
+    L_.ref_gnn_n_params.argtypes = [C.c_int, C.c_int, C.c_int]
+    return L_.ref_gnn_n_params(fin, H, L)
+
+
+def gnn_loss_grad(nb, x, t, theta, H, L, want_grad=True):
+    """FP64 oracle of the learned-guess network: loss, gradient and output."""
+    L_ = lib()
+    L_.ref_gnn_loss_grad.restype = C.c_double
+    L_.ref_gnn_loss_grad.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    nb = np.ascontiguousarray(nb, np.int32)
+    n, K = nb.shape
+    x = np.ascontiguousarray(x, np.float64); t = np.ascontiguousarray(t, np.float64)
+    theta = np.ascontiguousarray(theta, np.float64)
+    g = np.zeros_like(theta) if want_grad else None
+    out = np.empty(n)
+    loss = L_.ref_gnn_loss_grad(n, K, _p(nb), x.shape[1], _p(x), _p(t), H, L, _p(theta),
+                                _p(g) if want_grad else None, _p(out))
+    return loss, g, out
+
+
+def gnn_forward_backward(nb, x, theta, H, L, dout=None):
+    """FP64 oracle forward; with dout (dloss/dout) also the parameter gradient."""
+    L_ = lib()
+    L_.ref_gnn_forward_backward.restype = None
+    L_.ref_gnn_forward_backward.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    nb = np.ascontiguousarray(nb, np.int32)
+    n, K = nb.shape
+    x = np.ascontiguousarray(x, np.float64); theta = np.ascontiguousarray(theta, np.float64)
+    out = np.empty(n)
+    g = np.zeros_like(theta) if dout is not None else None
+    d = np.ascontiguousarray(dout, np.float64) if dout is not None else None
+    L_.ref_gnn_forward_backward(n, K, _p(nb), x.shape[1], _p(x), H, L, _p(theta), _p(d) if d is not None else None,
+                                _p(g) if g is not None else None, _p(out))
+    return out, g

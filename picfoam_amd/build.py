@@ -22,6 +22,7 @@ SOURCES = [
     ("k_coulomb.hip", []),
     ("k_mcc.hip", []),
     ("k_poisson.hip", []),
+    ("k_gnn.hip", []),
 ]
 
 

@@ -11,6 +11,7 @@
 
 #include "../../include/picmi.h"
 #include "internal.h"
+#include "gnn.h"
 #include "kernels.h"
 
 using namespace picmi;
@@ -108,6 +109,10 @@ struct pic_ctx {
     // gradient tables
     int32_t *d_gfo = nullptr, *d_gkind = nullptr, *d_gother = nullptr;
     double *d_gw = nullptr, *d_gval = nullptr, *d_gS = nullptr;
+    // learned initial guess
+    std::unique_ptr<Gnn> gnn;
+    bool gnn_use = false, gnn_rec = false;
+    double* d_phi_prev = nullptr;
     // profiling
     bool profile = false;
     std::vector<StageEvent> pending;
@@ -373,7 +378,8 @@ int do_coulomb(pic_ctx* c, double dt, int maxc) {
     a.seed = c->seed;
     a.step = c->step;
     a.maxp = pow2_at_least(maxc);
-    if (a.maxp > 8192) return c->fail(PIC_ERR_CELL_OVERFLOW, "more than 8192 parcels of one species in a cell");
+    // two key/index buffers per species pair in LDS: 24 B per slot, at most 48 KiB
+    if (a.maxp > 2048) return c->fail(PIC_ERR_CELL_OVERFLOW, "more than 2048 parcels of one species in a cell");
     a.flags = c->d_flags;
     a.ncoll = c->d_stats + 2;
     return check_hip(c, launch_coulomb(a, c->mixed(), c->stream), "coulomb kernel");
@@ -418,7 +424,8 @@ int do_mcc_t(pic_ctx* c, double dt, int maxc) {
     a.seed = c->seed;
     a.step = c->step;
     a.maxp = pow2_at_least(maxc);
-    if (a.maxp > 16384) return c->fail(PIC_ERR_CELL_OVERFLOW, "more than 16384 electrons in a cell");
+    // one key/index buffer in LDS: 12 B per slot, at most 48 KiB
+    if (a.maxp > 4096) return c->fail(PIC_ERR_CELL_OVERFLOW, "more than 4096 electrons in a cell");
     a.flags = c->d_flags;
     a.counts = c->d_stats + 3;
     a.tets = c->d_tets;
@@ -486,8 +493,30 @@ int do_poisson(pic_ctx* c, pic_poisson_stats* st) {
         pic_ctx::Scope sc(c, PIC_STAGE_POISSON);
         hipError_t e = poisson_rhs(c->pd, c->d_rho, c->d_b, c->hp.singular, c->pw.red, c->stream);
         if (int r = check_hip(c, e, "poisson rhs")) return r;
+        const bool learned = c->gnn && (c->gnn_use || c->gnn_rec);
+        if (learned) {
+            // features of the warm start: r = b - A phi_prev, b, phi_prev
+            hipMemcpyAsync(c->d_phi_prev, c->d_phi, sizeof(double) * c->hm.n_cells, hipMemcpyDeviceToDevice, c->stream);
+            poisson_residual(c->pd, c->pw, c->d_phi_prev, c->d_b, c->pw.r, c->stream);
+            c->gnn->features(c->pw.r, c->d_b, c->d_phi_prev);
+            if (c->gnn_use) {
+                c->gnn->forward(c->gnn->x, false);
+                c->gnn->guess(c->d_phi_prev, c->d_phi);
+            }
+            if (int r = check_hip(c, hipGetLastError(), "learned guess")) return r;
+        }
         status = poisson_solve(c->pd, c->pw, c->d_phi, c->d_b, c->ptol, c->pmaxit, c->pmode, c->hp.singular, &iters,
                                &refs, &rel, c->stream);
+        if (c->gnn && c->gnn_rec && status == 0) {
+            Gnn& G = *c->gnn;
+            const int slot = G.snap_next;
+            const size_t n = (size_t)c->hm.n_cells;
+            G.target(c->d_phi, c->d_phi_prev, G.snap_t + slot * n);
+            hipMemcpyAsync(G.snap_x + slot * n * 3, G.x, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, c->stream);
+            G.snap_norm[slot] = gnn_anorm(G, G.snap_t + slot * n, c->pd.nb, c->pd.a, c->pd.diag, c->pd.K);
+            G.snap_next = (slot + 1) % G.snap_cap;
+            G.snap_count = std::min(G.snap_count + 1, G.snap_cap);
+        }
         GradArgs g{};
         g.n_cells = c->hm.n_cells;
         g.gfo = c->d_gfo; g.g_kind = c->d_gkind; g.g_other = c->d_gother;
@@ -531,6 +560,7 @@ const char* pic_last_error(const pic_ctx* ctx) { return ctx ? ctx->err.c_str() :
 void pic_destroy(pic_ctx* c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->gnn) c->gnn->release();
     for (void* p : c->allocs) hipFree(p);
     for (auto& pe : c->pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
     for (auto e : c->pool) hipEventDestroy(e);
@@ -1193,6 +1223,150 @@ int pic_stage_times(pic_ctx* c, double* ms, int64_t* n) {
         if (ms) ms[k] = c->stage_ms[k];
         if (n) n[k] = c->stage_n[k];
     }
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------- learned initial guess
+
+int pic_gnn_init(pic_ctx* c, int32_t layers, int32_t width, uint64_t seed) {
+    if (!c) return PIC_ERR_INVALID;
+    if (!c->has_poisson) return c->fail(PIC_ERR_INVALID, "the learned guess needs MaxwellSolver ElectroStatic");
+    if (width != 64) return c->fail(PIC_ERR_UNSUPPORTED, "hidden width must be 64 (MFMA tiles)");
+    if (layers < 1 || layers > 8) return c->fail(PIC_ERR_INVALID, "layers must be 1..8");
+    if (c->gnn) { c->gnn->release(); c->gnn.reset(); }
+    c->gnn.reset(new Gnn());
+    Gnn& G = *c->gnn;
+    G.nb = c->pd.nb;
+    if (int r = check_hip(c, G.alloc(c->hm.n_cells, c->pd.K, layers, c->stream), "gnn alloc")) return r;
+    double ds = 0.0;
+    for (double d : c->hp.diag) ds += d;
+    G.dbar = ds / c->hp.diag.size();
+    if (!c->d_phi_prev) c->d_phi_prev = c->alloc<double>(c->hm.n_cells);
+    // He-style initialisation from a splitmix stream; residual layers start small
+    std::vector<float> th(G.np, 0.f);
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto normal = [&]() {
+        auto u = [&]() {
+            st += 0x9E3779B97F4A7C15ull;
+            uint64_t z = st;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        };
+        double a = u(), b = u();
+        return std::sqrt(-2.0 * std::log(a)) * std::cos(2.0 * kPi * b);
+    };
+    size_t o = 0;
+    for (int k = 0; k < 192; ++k) th[o++] = (float)(normal() * std::sqrt(2.0 / 3.0));
+    o += 64;
+    for (int l = 0; l < layers; ++l) {
+        for (int k = 0; k < 8192; ++k) th[o++] = (float)(normal() * 0.5 * std::sqrt(2.0 / 128.0));
+        o += 64;
+    }
+    for (int k = 0; k < 64; ++k) th[o++] = (float)(normal() * 0.1 * std::sqrt(1.0 / 64.0));
+    hipMemcpyAsync(G.theta, th.data(), sizeof(float) * G.np, hipMemcpyHostToDevice, c->stream);
+    G.refresh_weights();
+    return check_hip(c, hipStreamSynchronize(c->stream), "gnn init");
+}
+
+int pic_gnn_n_params(pic_ctx* c, int64_t* n) {
+    if (!c || !c->gnn || !n) return c ? c->fail(PIC_ERR_INVALID, "pic_gnn_init first") : PIC_ERR_INVALID;
+    *n = c->gnn->np;
+    return PIC_OK;
+}
+
+int pic_gnn_params(pic_ctx* c, int64_t n, float* get, const float* set) {
+    if (!c || !c->gnn) return c ? c->fail(PIC_ERR_INVALID, "pic_gnn_init first") : PIC_ERR_INVALID;
+    Gnn& G = *c->gnn;
+    if (n != G.np) return c->fail(PIC_ERR_INVALID, "parameter count mismatch");
+    if (set) {
+        hipMemcpyAsync(G.theta, set, sizeof(float) * n, hipMemcpyHostToDevice, c->stream);
+        G.refresh_weights();
+    }
+    if (get) hipMemcpyAsync(get, G.theta, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream);
+    return check_hip(c, hipStreamSynchronize(c->stream), "gnn params");
+}
+
+int pic_gnn_record(pic_ctx* c, int32_t capacity) {
+    if (!c || !c->gnn) return c ? c->fail(PIC_ERR_INVALID, "pic_gnn_init first") : PIC_ERR_INVALID;
+    Gnn& G = *c->gnn;
+    if (capacity <= 0) { c->gnn_rec = false; return PIC_OK; }
+    if (capacity != G.snap_cap) {
+        const size_t n = (size_t)c->hm.n_cells;
+        G.snap_x = c->alloc<float>(3 * n * capacity);
+        G.snap_t = c->alloc<float>(n * capacity);
+        if (!G.snap_x || !G.snap_t) return c->fail(PIC_ERR_HIP, "out of device memory (snapshots)");
+        G.snap_cap = capacity;
+        G.snap_count = 0;
+        G.snap_next = 0;
+        G.snap_norm.assign(capacity, 1.0);
+    }
+    c->gnn_rec = true;
+    return PIC_OK;
+}
+
+int pic_gnn_snapshot_count(pic_ctx* c, int32_t* n) {
+    if (!c || !c->gnn || !n) return PIC_ERR_INVALID;
+    *n = c->gnn->snap_count;
+    return PIC_OK;
+}
+
+int pic_gnn_snapshot(pic_ctx* c, int32_t idx, float* x, float* t, double* norm) {
+    if (!c || !c->gnn || idx < 0 || idx >= c->gnn->snap_count) return c ? c->fail(PIC_ERR_INVALID, "snapshot index") : PIC_ERR_INVALID;
+    Gnn& G = *c->gnn;
+    const size_t n = (size_t)c->hm.n_cells;
+    if (x) hipMemcpyAsync(x, G.snap_x + idx * n * 3, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream);
+    if (t) hipMemcpyAsync(t, G.snap_t + idx * n, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream);
+    if (norm) *norm = G.snap_norm[idx];
+    return check_hip(c, hipStreamSynchronize(c->stream), "gnn snapshot");
+}
+
+int pic_gnn_grad(pic_ctx* c, int32_t idx, float* grad, double* loss) {
+    if (!c || !c->gnn || idx < 0 || idx >= c->gnn->snap_count) return c ? c->fail(PIC_ERR_INVALID, "snapshot index") : PIC_ERR_INVALID;
+    Gnn& G = *c->gnn;
+    const size_t n = (size_t)c->hm.n_cells;
+    G.forward(G.snap_x + idx * n * 3, true);
+    double l = 0.0;
+    G.loss_backward(G.snap_t + idx * n, G.snap_norm[idx], c->pd.nb, c->pd.a, c->pd.diag, c->pd.K, &l);
+    if (grad) hipMemcpyAsync(grad, G.grad, sizeof(float) * G.np, hipMemcpyDeviceToHost, c->stream);
+    if (loss) *loss = l;
+    return check_hip(c, hipStreamSynchronize(c->stream), "gnn grad");
+}
+
+int pic_gnn_train(pic_ctx* c, int32_t steps, double lr, uint64_t seed, double* loss_first, double* loss_last) {
+    if (!c || !c->gnn) return c ? c->fail(PIC_ERR_INVALID, "pic_gnn_init first") : PIC_ERR_INVALID;
+    Gnn& G = *c->gnn;
+    if (G.snap_count == 0) return c->fail(PIC_ERR_INVALID, "no snapshots recorded");
+    const size_t n = (size_t)c->hm.n_cells;
+    double first = 0.0, last = 0.0;
+    int nlast = 0;
+    const int tail = std::max(1, steps / 10);
+    for (int s = 0; s < steps; ++s) {
+        uint64_t z = (seed + 1) * 0x9E3779B97F4A7C15ull + (uint64_t)s * 0xBF58476D1CE4E5B9ull;
+        z ^= z >> 31; z *= 0x94D049BB133111EBull; z ^= z >> 29;
+        const int idx = (int)(z % (uint64_t)G.snap_count);
+        G.forward(G.snap_x + idx * n * 3, true);
+        const bool want = (s == 0) || (s >= steps - tail);
+        double l = 0.0;
+        G.loss_backward(G.snap_t + idx * n, G.snap_norm[idx], c->pd.nb, c->pd.a, c->pd.diag, c->pd.K,
+                        want ? &l : nullptr);
+        if (s == 0) first = l;
+        if (s >= steps - tail) { last += l; ++nlast; }
+        G.adam((float)lr, ++G.adam_t);
+    }
+    if (loss_first) *loss_first = first;
+    if (loss_last) *loss_last = nlast ? last / nlast : 0.0;
+    return check_hip(c, hipStreamSynchronize(c->stream), "gnn train");
+}
+
+int pic_gnn_eval(pic_ctx* c, int32_t idx, double* loss) {
+    return pic_gnn_grad(c, idx, nullptr, loss);
+}
+
+int pic_gnn_use(pic_ctx* c, int32_t on) {
+    if (!c || !c->gnn) return c ? c->fail(PIC_ERR_INVALID, "pic_gnn_init first") : PIC_ERR_INVALID;
+    c->gnn_use = on != 0;
     return PIC_OK;
 }
 

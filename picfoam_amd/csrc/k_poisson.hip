@@ -286,6 +286,13 @@ hipError_t poisson_rhs(const PoissonDev& P, const double* rho, double* b, bool s
     return hipGetLastError();
 }
 
+hipError_t poisson_residual(const PoissonDev& P, PoissonWork& W, const double* x, const double* b, double* r,
+                            hipStream_t s) {
+    const int nb = nblk(P.n);
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, P.n, P.K, P.nb, P.a, P.diag, x, b, r, W.red);
+    return hipGetLastError();
+}
+
 int poisson_solve(const PoissonDev& P, PoissonWork& W, double* phi, const double* b, double tol, int max_iter,
                   int mode, bool singular, int* iters, int* refinements, double* rel, hipStream_t s) {
     const int n = P.n, nb = nblk(n);

@@ -218,6 +218,9 @@ struct PoissonWork {
     double* scal;         // device scalars
     int nblocks;
 };
+// r = b - A x (FP64)
+hipError_t poisson_residual(const PoissonDev& P, PoissonWork& W, const double* x, const double* b, double* r,
+                            hipStream_t s);
 int poisson_solve(const PoissonDev& P, PoissonWork& W, double* phi, const double* b, double tol, int max_iter,
                   int mode, bool singular, int* iters, int* refinements, double* rel, hipStream_t s);
 

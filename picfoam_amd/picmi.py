@@ -38,7 +38,8 @@ EXPORTS = (
     "pic_particles_get", "pic_particles_positions", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
     "pic_collide_coulomb", "pic_collide_mcc", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
     "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
-    "pic_profile_reset",
+    "pic_profile_reset", "pic_gnn_init", "pic_gnn_n_params", "pic_gnn_params", "pic_gnn_record",
+    "pic_gnn_snapshot_count", "pic_gnn_snapshot", "pic_gnn_grad", "pic_gnn_train", "pic_gnn_eval", "pic_gnn_use",
 )
 
 
@@ -144,6 +145,16 @@ def load_library(path: str = LIB_PATH):
         "pic_profile_enable": (C.c_int, [vp, i32]),
         "pic_stage_times": (C.c_int, [vp, vp, vp]),
         "pic_profile_reset": (C.c_int, [vp]),
+        "pic_gnn_init": (C.c_int, [vp, i32, i32, C.c_uint64]),
+        "pic_gnn_n_params": (C.c_int, [vp, C.POINTER(i64)]),
+        "pic_gnn_params": (C.c_int, [vp, i64, vp, vp]),
+        "pic_gnn_record": (C.c_int, [vp, i32]),
+        "pic_gnn_snapshot_count": (C.c_int, [vp, C.POINTER(i32)]),
+        "pic_gnn_snapshot": (C.c_int, [vp, i32, vp, vp, C.POINTER(dbl)]),
+        "pic_gnn_grad": (C.c_int, [vp, i32, vp, C.POINTER(dbl)]),
+        "pic_gnn_train": (C.c_int, [vp, i32, dbl, C.c_uint64, C.POINTER(dbl), C.POINTER(dbl)]),
+        "pic_gnn_eval": (C.c_int, [vp, i32, C.POINTER(dbl)]),
+        "pic_gnn_use": (C.c_int, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -396,3 +407,47 @@ class Cloud:
         ms = np.zeros(len(STAGES)); n = np.zeros(len(STAGES), np.int64)
         self._check(self.lib.pic_stage_times(self.h, _ptr(ms), _ptr(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(STAGES)}
+
+    # learned initial guess
+    def gnn_init(self, layers=3, width=64, seed=1):
+        self._check(self.lib.pic_gnn_init(self.h, layers, width, seed))
+        n = C.c_int64()
+        self._check(self.lib.pic_gnn_n_params(self.h, C.byref(n)))
+        self.gnn_np = n.value
+
+    def gnn_params(self, set_values=None):
+        out = np.empty(self.gnn_np, np.float32)
+        sv = None if set_values is None else np.ascontiguousarray(set_values, np.float32)
+        self._check(self.lib.pic_gnn_params(self.h, self.gnn_np, _ptr(out), _ptr(sv) if sv is not None else None))
+        return out
+
+    def gnn_record(self, capacity):
+        self._check(self.lib.pic_gnn_record(self.h, capacity))
+
+    def gnn_snapshot_count(self):
+        n = C.c_int32()
+        self._check(self.lib.pic_gnn_snapshot_count(self.h, C.byref(n)))
+        return n.value
+
+    def gnn_snapshot(self, idx):
+        x = np.empty((self.n_cells, 3), np.float32); t = np.empty(self.n_cells, np.float32); nrm = C.c_double()
+        self._check(self.lib.pic_gnn_snapshot(self.h, idx, _ptr(x), _ptr(t), C.byref(nrm)))
+        return x, t, nrm.value
+
+    def gnn_grad(self, idx):
+        g = np.empty(self.gnn_np, np.float32); loss = C.c_double()
+        self._check(self.lib.pic_gnn_grad(self.h, idx, _ptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def gnn_train(self, steps, lr=1e-3, seed=1):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.lib.pic_gnn_train(self.h, steps, lr, seed, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def gnn_eval(self, idx):
+        loss = C.c_double()
+        self._check(self.lib.pic_gnn_eval(self.h, idx, C.byref(loss)))
+        return loss.value
+
+    def gnn_use(self, on=True):
+        self._check(self.lib.pic_gnn_use(self.h, 1 if on else 0))
