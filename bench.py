@@ -84,6 +84,29 @@ def cpu_baseline(ncell_sample=8, ppc=64, target_s=12.0):
                       f"in {el:.1f} s, same dt/models as the GPU workload"}
 
 
+def run_steps(args, precision, steps, warmup, profile):
+    from picfoam_amd import cases, picmi
+    mesh, species, models, ex = cases.thermalisation(ncell=args.ncell, ppc=args.ppc, precision=precision)
+    cloud = picmi.Cloud(mesh, species, models)
+    cloud.init_maxwellian(0, args.ppc, ex["Te_K"], id0=0, seed=11)
+    cloud.init_maxwellian(1, args.ppc, ex["Ti_K"], id0=0, seed=12)
+    dt = ex["dt"]
+    n_total = cloud.count(0) + cloud.count(1)
+    for _ in range(warmup):
+        cloud.step(dt)
+    cloud.sync()
+    if profile:
+        cloud.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cloud.step(dt)
+    cloud.sync()
+    t1 = time.perf_counter()
+    stages = cloud.stage_times() if profile else None
+    cloud.close()
+    return n_total, t1 - t0, stages, dt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,34 +117,16 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU domain decomposition is a design, not code)", file=sys.stderr)
         sys.exit(2)
 
-    from picfoam_amd import cases, picmi
-
-    mesh, species, models, ex = cases.thermalisation(ncell=args.ncell, ppc=args.ppc, precision=args.precision)
-    print(f"bench: building {args.ncell}^3 mesh, {2 * args.ncell ** 3 * args.ppc} parcels", file=sys.stderr,
+    print(f"bench: {args.ncell}^3 mesh, {2 * args.ncell ** 3 * args.ppc} parcels, {args.precision}", file=sys.stderr,
           flush=True)
-    cloud = picmi.Cloud(mesh, species, models)
-    cloud.init_maxwellian(0, args.ppc, ex["Te_K"], id0=0, seed=11)
-    cloud.init_maxwellian(1, args.ppc, ex["Ti_K"], id0=0, seed=12)
-    dt = ex["dt"]
-    n_total = cloud.count(0) + cloud.count(1)
-    for _ in range(args.warmup):
-        cloud.step(dt)
-    cloud.sync()
-    cloud.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        cloud.step(dt)
-    cloud.sync()
-    t1 = time.perf_counter()
-    stages = cloud.stage_times()
-    cloud.profile(False)
-    elapsed = t1 - t0
+    n_total, elapsed, stages, dt = run_steps(args, args.precision, args.steps, args.warmup, True)
     ms_per_step = 1e3 * elapsed / args.steps
     value = n_total * args.steps / elapsed
     # dominant kernel and its roofline
@@ -151,7 +156,11 @@ def main():
         "roofline": roof,
         "stage_ms_per_step": stage_ms,
     }
-    cloud.close()
+    if not args.no_mixed and args.precision == "fp64":
+        print("bench: mixed-precision run of the same workload", file=sys.stderr, flush=True)
+        n2, el2, _, _ = run_steps(args, "mixed", args.steps, args.warmup, False)
+        out["mixed_precision"] = {"value": n2 * args.steps / el2, "ms_per_step": 1e3 * el2 / args.steps,
+                                  "speedup_vs_fp64": (n2 * args.steps / el2) / value}
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(target_s=args.cpu_seconds)

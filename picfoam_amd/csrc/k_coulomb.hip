@@ -12,24 +12,30 @@ namespace picmi {
 
 namespace {
 
-// NanbuCollision.C:137-159
-__device__ double cos_chi(double s, double U) {
-    double A, invA;
-    if (s < 0.1) {
-        if (U < 0.0001) U = 0.0001;
-        return 1.0 + s * log(U);
+// NanbuCollision.C:137-159, in the arithmetic M of the collision (FP64, or FP32 in the
+// mixed-precision mode)
+template <class M>
+__device__ M cos_chi(M s, M U) {
+    M A, invA;
+    if (s < (M)0.1) {
+        if (U < (M)0.0001) U = (M)0.0001;
+        return (M)1 + s * log(U);
     }
-    if (s < 3.0) {
-        invA = 0.00569578 + (0.95602 + (-0.508139 + (0.479139 + (-0.12789 + 0.0238957 * s) * s) * s) * s) * s;
-        A = 1.0 / invA;
-        return invA * log(exp(-A) + 2. * U * sinh(A));
+    if (s < (M)3) {
+        invA = (M)0.00569578 +
+               ((M)0.95602 + ((M)-0.508139 + ((M)0.479139 + ((M)-0.12789 + (M)0.0238957 * s) * s) * s) * s) * s;
+        A = (M)1 / invA;
+        return invA * log(exp(-A) + (M)2 * U * sinh(A));
     }
-    if (s < 6.0) {
-        A = 3.0 * exp(-s);
-        return (1.0 / A) * log(exp(-A) + 2.0 * U * sinh(A));
+    if (s < (M)6) {
+        A = (M)3 * exp(-s);
+        return ((M)1 / A) * log(exp(-A) + (M)2 * U * sinh(A));
     }
-    return 2.0 * U - 1.0;
+    return (M)2 * U - (M)1;
 }
+
+__device__ __forceinline__ void sincos_m(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ void sincos_m(float x, float* s, float* c) { sincosf(x, s, c); }
 
 template <class UT>
 struct Vel {
@@ -38,43 +44,47 @@ struct Vel {
     UT* z;
 };
 
-// NanbuCollision::collide (weight correction "none")
+// NanbuCollision::collide (weight correction "none"); arithmetic in M = the velocity type
 template <class UT>
 __device__ void nanbu_collide(Vel<UT> vp, int64_t ip, Vel<UT> vq, int64_t iq, double mP, double qP, double mQ,
                               double qQ, double lnL_fixed, int calc_bmin, double debye, double nP, double nQ,
-                              double nPQ, double dt, double u1, double u2) {
-    const double reMass = mP * mQ / (mP + mQ);
-    vec3<double> uP = mk3<double>(vp.x[ip], vp.y[ip], vp.z[ip]);
-    vec3<double> uQ = mk3<double>(vq.x[iq], vq.y[iq], vq.z[iq]);
-    vec3<double> g = uP - uQ;
-    const double gm = mag(g);
-    const double gp = sqrt(g.y * g.y + g.z * g.z);
-    if (!(gm > 0.0) || !(gp > 0.0)) return;
-    double lnL;
+                              double nPQ, double dt, double u1d, double u2d) {
+    using M = UT;
+    const M reMass = (M)(mP * mQ / (mP + mQ));
+    vec3<M> uP = mk3<M>(vp.x[ip], vp.y[ip], vp.z[ip]);
+    vec3<M> uQ = mk3<M>(vq.x[iq], vq.y[iq], vq.z[iq]);
+    vec3<M> g = uP - uQ;
+    const M gm = mag(g);
+    const M gp = sqrt(g.y * g.y + g.z * g.z);
+    if (!(gm > (M)0) || !(gp > (M)0)) return;
+    M lnL;
     if (lnL_fixed == 0.0) {
-        double b0 = qP * qQ / (2.0 * kPi * kEps0 * reMass * gm * gm);
+        M b0 = (M)(qP * qQ / (2.0 * kPi * kEps0)) / (reMass * gm * gm);
         if (calc_bmin) {
-            double hb = kH / (2.0 * (gm * reMass));
+            M hb = (M)kH / ((M)2 * (gm * reMass));
             b0 = hb > fabs(b0) ? hb : fabs(b0);
         }
-        double l = 0.5 * log(1.0 + (debye * debye) / (b0 * b0));
-        lnL = l > 2.0 ? l : 2.0;
+        M l = (M)0.5 * log((M)1 + ((M)debye * (M)debye) / (b0 * b0));
+        lnL = l > (M)2 ? l : (M)2;
     } else {
-        lnL = lnL_fixed;
+        lnL = (M)lnL_fixed;
     }
-    const double a = qP * qQ / (kEps0 * reMass);
-    const double s = lnL / (4.0 * kPi) * (a * a) * (nQ * nP / nPQ) * pow(gm, -3.0) * dt;
-    const double cc = cos_chi(s, u1);
-    const double sc = sqrt(1.0 - cc * cc);
-    const double eps = 2.0 * kPi * u2;
-    const double ce = cos(eps), se = sin(eps);
-    vec3<double> h = mk3<double>(gp * ce, -1.0 * (g.y * g.x * ce + gm * g.z * se) / gp,
-                                 -1.0 * (g.z * g.x * ce - gm * g.y * se) / gp);
-    vec3<double> dv = (1.0 - cc) * g + sc * h;
-    uP = uP - (mQ / (mP + mQ)) * dv;
-    uQ = uQ + (mP / (mP + mQ)) * dv;
-    vp.x[ip] = (UT)uP.x; vp.y[ip] = (UT)uP.y; vp.z[ip] = (UT)uP.z;
-    vq.x[iq] = (UT)uQ.x; vq.y[iq] = (UT)uQ.y; vq.z[iq] = (UT)uQ.z;
+    const double ad = qP * qQ / (kEps0 * (mP * mQ / (mP + mQ)));
+    // s = lnL/(4 pi) (qP qQ / eps0 mu)^2 (nP nQ / nPQ) g^-3 dt, the constant part in FP64
+    const M cst = (M)(1.0 / (4.0 * kPi) * (ad * ad) * (nQ * nP / nPQ) * dt);
+    const M s = lnL * cst * ((M)1 / (gm * gm * gm));
+    const M cc = cos_chi<M>(s, (M)u1d);
+    const M sc = sqrt((M)1 - cc * cc);
+    const M eps = (M)(2.0 * kPi) * (M)u2d;
+    M se, ce;
+    sincos_m(eps, &se, &ce);
+    const M igp = (M)1 / gp;
+    vec3<M> h = mk3<M>(gp * ce, -(g.y * g.x * ce + gm * g.z * se) * igp, -(g.z * g.x * ce - gm * g.y * se) * igp);
+    vec3<M> dv = ((M)1 - cc) * g + sc * h;
+    uP = uP - (M)(mQ / (mP + mQ)) * dv;
+    uQ = uQ + (M)(mP / (mP + mQ)) * dv;
+    vp.x[ip] = uP.x; vp.y[ip] = uP.y; vp.z[ip] = uP.z;
+    vq.x[iq] = uQ.x; vq.y[iq] = uQ.y; vq.z[iq] = uQ.z;
 }
 
 __device__ __forceinline__ void coll_rand(uint64_t seed, int64_t step, int cell, int stage, int k, double& u1,
@@ -104,9 +114,30 @@ __device__ void bitonic_sort(uint64_t* key, int* val, int npow2) {
     }
 }
 
-// the cell's parcels of one species ordered by the random key of (id, step, stage, species)
+// the cell's parcels of one species ordered by the random key of (id, step, stage,
+// species).  Up to 256 parcels the order is found by ranking (each key counts the keys
+// below it, no barriers inside); larger cells use a bitonic sort.
 __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
                            uint64_t* key, int* val) {
+    if (n <= 256) {
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            const uint32_t pid = id[first + k];
+            uint4 o = rng4(seed, pid, (uint32_t)step, ((uint32_t)stage << 8) | (uint32_t)species, kTagShuffle << 24);
+            key[k] = ((uint64_t)o.x << 32) | pid;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            const uint64_t kk = key[k];
+            int r = 0;
+            for (int j = 0; j < n; ++j) {
+                const uint64_t kj = key[j];
+                r += (kj < kk) || (kj == kk && j < k);
+            }
+            val[r] = first + k;
+        }
+        __syncthreads();
+        return;
+    }
     int np2 = 1;
     while (np2 < n) np2 <<= 1;
     for (int k = threadIdx.x; k < np2; k += blockDim.x) {
@@ -194,13 +225,22 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
             debye = d > rmin ? d : rmin;
         }
     }
-    int stage = 0;
-    unsigned long long ncoll = 0;
+    // stages in the reference order: species i ascending, partner j descending from the
+    // last species down to i.  Two consecutive intra-species stages of different species
+    // touch disjoint parcels, so they run together on one set of lanes.
+    int st_i[kMaxSpecies * kMaxSpecies], st_j[kMaxSpecies * kMaxSpecies];
+    int nst = 0;
     for (int i = 0; i < ns; ++i) {
         if (a.charge[i] == 0.0) continue;
         for (int j = ns - 1; j >= i; --j) {
             if (a.charge[j] == 0.0) continue;
-            const int st = stage++;
+            st_i[nst] = i; st_j[nst] = j; ++nst;
+        }
+    }
+    unsigned long long ncoll = 0;
+    for (int st = 0; st < nst; ++st) {
+        {
+            const int i = st_i[st], j = st_j[st];
             const int a0 = a.start[i][cell], na = a.start[i][cell + 1] - a0;
             const int b0 = a.start[j][cell], nb = a.start[j][cell + 1] - b0;
             const double mA = a.mass[i], qA = a.charge[i], mB = a.mass[j], qB = a.charge[j];
@@ -210,38 +250,55 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
             Vel<UT> VB{(UT*)a.ux[j], (UT*)a.uy[j], (UT*)a.uz[j]};
             double u1, u2;
             if (i == j) {
-                if (na <= 1 || !a.allow_intra) continue;
-                keyed_perm(a.seed, a.step, st, i, a.id[i], a0, na, keyA, valA);
-                const double n = na * w / Vc;
-                if (na % 2) {
-                    const double n12 = (na + 3.0) * w / Vc;
-                    if (threadIdx.x == 0) {
-                        const int P = valA[0], Q = valA[1], R = valA[2];
-                        coll_rand(a.seed, a.step, cell, st, 0, u1, u2);
-                        nanbu_collide(VA, P, VA, Q, mA, qA, mA, qA, lnL, a.calculate_bmin, debye, n, n, n12, a.dt, u1, u2);
-                        coll_rand(a.seed, a.step, cell, st, 1, u1, u2);
-                        nanbu_collide(VA, Q, VA, R, mA, qA, mA, qA, lnL, a.calculate_bmin, debye, n, n, n12, a.dt, u1, u2);
-                        coll_rand(a.seed, a.step, cell, st, 2, u1, u2);
-                        nanbu_collide(VA, P, VA, R, mA, qA, mA, qA, lnL, a.calculate_bmin, debye, n, n, n12, a.dt, u1, u2);
-                    }
-                    __syncthreads();
-                    const int npairs = (na - 3) / 2;
-                    for (int q = threadIdx.x; q < npairs; q += blockDim.x) {
-                        const int x = 3 + 2 * q;
-                        coll_rand(a.seed, a.step, cell, st, 3 + q, u1, u2);
-                        nanbu_collide(VA, valA[x], VA, valA[x + 1], mA, qA, mA, qA, lnL, a.calculate_bmin, debye, n, n,
-                                      n12, a.dt, u1, u2);
-                    }
-                    ncoll += 3 + npairs;
-                } else {
-                    const int npairs = na / 2;
-                    for (int q = threadIdx.x; q < npairs; q += blockDim.x) {
-                        coll_rand(a.seed, a.step, cell, st, q, u1, u2);
-                        nanbu_collide(VA, valA[2 * q], VA, valA[2 * q + 1], mA, qA, mA, qA, lnL, a.calculate_bmin,
-                                      debye, n, n, n, a.dt, u1, u2);
-                    }
-                    ncoll += npairs;
+                // this intra stage, and the next one when it is intra for another species
+                const bool pair2 = (st + 1 < nst) && st_i[st + 1] == st_j[st + 1] && st_i[st + 1] != i;
+                int sp[2] = {i, pair2 ? st_i[st + 1] : -1};
+                int sg[2] = {st, st + 1};
+                int nn[2] = {0, 0}, items[2] = {0, 0};
+                for (int u = 0; u < (pair2 ? 2 : 1); ++u) {
+                    const int spc = sp[u];
+                    const int f0 = a.start[spc][cell];
+                    const int cnt = a.start[spc][cell + 1] - f0;
+                    if (cnt <= 1 || !a.allow_intra) continue;
+                    nn[u] = cnt;
+                    keyed_perm(a.seed, a.step, sg[u], spc, a.id[spc], f0, cnt, u == 0 ? keyA : keyB, u == 0 ? valA : valB);
+                    items[u] = (cnt % 2) ? 1 + (cnt - 3) / 2 : cnt / 2;
+                    ncoll += (cnt % 2) ? 3 + (cnt - 3) / 2 : cnt / 2;
                 }
+                const int total = items[0] + items[1];
+                for (int t = threadIdx.x; t < total; t += blockDim.x) {
+                    const int u = t < items[0] ? 0 : 1;
+                    const int it = u == 0 ? t : t - items[0];
+                    const int spc = sp[u], cnt = nn[u], stg = sg[u];
+                    const int* v = u == 0 ? valA : valB;
+                    Vel<UT> Vs{(UT*)a.ux[spc], (UT*)a.uy[spc], (UT*)a.uz[spc]};
+                    const double ms = a.mass[spc], qs = a.charge[spc];
+                    const double lnLs = a.coulomb_log[spc][spc];
+                    const double n = cnt * a.weight[spc] / Vc;
+                    if (cnt % 2) {
+                        const double n12 = (cnt + 3.0) * a.weight[spc] / Vc;
+                        if (it == 0) {
+                            // odd count: the first three collide cyclically (TakizukaAbePairing.C:115-139)
+                            const int P = v[0], Q = v[1], R = v[2];
+                            coll_rand(a.seed, a.step, cell, stg, 0, u1, u2);
+                            nanbu_collide(Vs, P, Vs, Q, ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n, n, n12, a.dt, u1, u2);
+                            coll_rand(a.seed, a.step, cell, stg, 1, u1, u2);
+                            nanbu_collide(Vs, Q, Vs, R, ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n, n, n12, a.dt, u1, u2);
+                            coll_rand(a.seed, a.step, cell, stg, 2, u1, u2);
+                            nanbu_collide(Vs, P, Vs, R, ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n, n, n12, a.dt, u1, u2);
+                        } else {
+                            const int q = it - 1, x = 3 + 2 * q;
+                            coll_rand(a.seed, a.step, cell, stg, 3 + q, u1, u2);
+                            nanbu_collide(Vs, v[x], Vs, v[x + 1], ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n, n,
+                                          n12, a.dt, u1, u2);
+                        }
+                    } else {
+                        coll_rand(a.seed, a.step, cell, stg, it, u1, u2);
+                        nanbu_collide(Vs, v[2 * it], Vs, v[2 * it + 1], ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n,
+                                      n, n, a.dt, u1, u2);
+                    }
+                }
+                if (pair2) ++st;
             } else {
                 if (na == 0 || nb == 0) continue;
                 keyed_perm(a.seed, a.step, st, i, a.id[i], a0, na, keyA, valA);
