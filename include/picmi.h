@@ -212,6 +212,8 @@ int pic_gnn_grad(pic_ctx* ctx, int32_t index, float* grad, double* loss);
 int pic_gnn_train(pic_ctx* ctx, int32_t steps, double lr, uint64_t seed, double* loss_first, double* loss_last);
 int pic_gnn_eval(pic_ctx* ctx, int32_t index, double* loss);
 int pic_gnn_use(pic_ctx* ctx, int32_t on);            /* start the PCG from the network's guess */
+/* training loss: 0 = A-norm error e^T A e / d^T A d, 1 = residual |A e|^2 / |A d|^2, 2 = their sum */
+int pic_gnn_loss(pic_ctx* ctx, int32_t kind);
 
 #ifdef __cplusplus
 }

@@ -514,6 +514,7 @@ int do_poisson(pic_ctx* c, pic_poisson_stats* st) {
             G.target(c->d_phi, c->d_phi_prev, G.snap_t + slot * n);
             hipMemcpyAsync(G.snap_x + slot * n * 3, G.x, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, c->stream);
             G.snap_norm[slot] = gnn_anorm(G, G.snap_t + slot * n, c->pd.nb, c->pd.a, c->pd.diag, c->pd.K);
+            G.snap_rnorm[slot] = gnn_rnorm(G, G.snap_t + slot * n, c->pd.nb, c->pd.a, c->pd.diag, c->pd.K);
             G.snap_next = (slot + 1) % G.snap_cap;
             G.snap_count = std::min(G.snap_count + 1, G.snap_cap);
         }
@@ -1301,6 +1302,7 @@ int pic_gnn_record(pic_ctx* c, int32_t capacity) {
         G.snap_count = 0;
         G.snap_next = 0;
         G.snap_norm.assign(capacity, 1.0);
+        G.snap_rnorm.assign(capacity, 1.0);
     }
     c->gnn_rec = true;
     return PIC_OK;
@@ -1328,7 +1330,8 @@ int pic_gnn_grad(pic_ctx* c, int32_t idx, float* grad, double* loss) {
     const size_t n = (size_t)c->hm.n_cells;
     G.forward(G.snap_x + idx * n * 3, true);
     double l = 0.0;
-    G.loss_backward(G.snap_t + idx * n, G.snap_norm[idx], c->pd.nb, c->pd.a, c->pd.diag, c->pd.K, &l);
+    G.loss_backward2(G.snap_t + idx * n, G.snap_norm[idx], G.snap_rnorm[idx], c->pd.nb, c->pd.a, c->pd.diag, c->pd.K,
+                     &l);
     if (grad) hipMemcpyAsync(grad, G.grad, sizeof(float) * G.np, hipMemcpyDeviceToHost, c->stream);
     if (loss) *loss = l;
     return check_hip(c, hipStreamSynchronize(c->stream), "gnn grad");
@@ -1349,8 +1352,8 @@ int pic_gnn_train(pic_ctx* c, int32_t steps, double lr, uint64_t seed, double* l
         G.forward(G.snap_x + idx * n * 3, true);
         const bool want = (s == 0) || (s >= steps - tail);
         double l = 0.0;
-        G.loss_backward(G.snap_t + idx * n, G.snap_norm[idx], c->pd.nb, c->pd.a, c->pd.diag, c->pd.K,
-                        want ? &l : nullptr);
+        G.loss_backward2(G.snap_t + idx * n, G.snap_norm[idx], G.snap_rnorm[idx], c->pd.nb, c->pd.a, c->pd.diag,
+                         c->pd.K, want ? &l : nullptr);
         if (s == 0) first = l;
         if (s >= steps - tail) { last += l; ++nlast; }
         G.adam((float)lr, ++G.adam_t);
@@ -1362,6 +1365,13 @@ int pic_gnn_train(pic_ctx* c, int32_t steps, double lr, uint64_t seed, double* l
 
 int pic_gnn_eval(pic_ctx* c, int32_t idx, double* loss) {
     return pic_gnn_grad(c, idx, nullptr, loss);
+}
+
+int pic_gnn_loss(pic_ctx* c, int32_t kind) {
+    if (!c || !c->gnn) return c ? c->fail(PIC_ERR_INVALID, "pic_gnn_init first") : PIC_ERR_INVALID;
+    if (kind < 0 || kind > 2) return c->fail(PIC_ERR_INVALID, "loss kind 0 (A-norm), 1 (residual) or 2 (both)");
+    c->gnn->loss_kind = kind;
+    return PIC_OK;
 }
 
 int pic_gnn_use(pic_ctx* c, int32_t on) {

@@ -20,13 +20,15 @@ struct Gnn {
     const float* cur_x = nullptr;    // features of the last forward pass
     uint16_t *h = nullptr, *mm = nullptr, *z = nullptr, *dz = nullptr;
     float *out = nullptr, *g = nullptr, *dm = nullptr, *part = nullptr;
-    double *q = nullptr, *red = nullptr, *scal = nullptr;
+    double *q = nullptr, *q2 = nullptr, *red = nullptr, *scal = nullptr;
+    int loss_kind = 0;               // 0: A-norm, 1: residual norm, 2: both
     std::vector<void*> bufs;
     // snapshots: features, targets, normalisation
     int snap_cap = 0, snap_count = 0, snap_next = 0;
     float* snap_x = nullptr;         // [cap][n][3]
     float* snap_t = nullptr;         // [cap][n]
     std::vector<double> snap_norm;   // d^T A d
+    std::vector<double> snap_rnorm;  // |A d|^2
     int64_t adam_t = 0;
 
     static int n_params(int L);
@@ -44,6 +46,8 @@ struct Gnn {
     // A-norm loss against t, gradient into grad; loss_out = e^T A e / norm
     hipError_t loss_backward(const float* t, double norm, const int32_t* anb, const double* aa, const double* ad,
                              int aK, double* loss_out);
+    hipError_t loss_backward2(const float* t, double norm_a, double norm_r, const int32_t* anb, const double* aa,
+                              const double* ad, int aK, double* loss_out);
     hipError_t adam(float lr, int64_t t);
 };
 
@@ -51,5 +55,6 @@ struct Gnn {
 
 namespace picmi {
 // t^T A t on the device (synchronises)
+double gnn_rnorm(Gnn& G, const float* t, const int32_t* anb, const double* aa, const double* ad, int aK);
 double gnn_anorm(Gnn& G, const float* t, const int32_t* anb, const double* aa, const double* ad, int aK);
 }  // namespace picmi

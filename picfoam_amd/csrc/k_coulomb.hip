@@ -226,21 +226,21 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
         }
     }
     // stages in the reference order: species i ascending, partner j descending from the
-    // last species down to i.  Two consecutive intra-species stages of different species
-    // touch disjoint parcels, so they run together on one set of lanes.
-    int st_i[kMaxSpecies * kMaxSpecies], st_j[kMaxSpecies * kMaxSpecies];
-    int nst = 0;
+    // last charged species down to i.  An intra-species stage (i, i) is followed by the
+    // intra stage of the last charged species when that is the next charged species; the
+    // two touch disjoint parcels and run together on one set of lanes.
+    int last = -1;
+    for (int s = 0; s < ns; ++s)
+        if (a.charge[s] != 0.0) last = s;
+    unsigned long long ncoll = 0;
+    int st = 0;
+    bool skip_last_intra = false;
     for (int i = 0; i < ns; ++i) {
         if (a.charge[i] == 0.0) continue;
-        for (int j = ns - 1; j >= i; --j) {
+        for (int j = last; j >= i; --j) {
             if (a.charge[j] == 0.0) continue;
-            st_i[nst] = i; st_j[nst] = j; ++nst;
-        }
-    }
-    unsigned long long ncoll = 0;
-    for (int st = 0; st < nst; ++st) {
-        {
-            const int i = st_i[st], j = st_j[st];
+            const int stage_now = st++;
+            if (i == j && i == last && skip_last_intra) continue;
             const int a0 = a.start[i][cell], na = a.start[i][cell + 1] - a0;
             const int b0 = a.start[j][cell], nb = a.start[j][cell + 1] - b0;
             const double mA = a.mass[i], qA = a.charge[i], mB = a.mass[j], qB = a.charge[j];
@@ -250,10 +250,14 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
             Vel<UT> VB{(UT*)a.ux[j], (UT*)a.uy[j], (UT*)a.uz[j]};
             double u1, u2;
             if (i == j) {
-                // this intra stage, and the next one when it is intra for another species
-                const bool pair2 = (st + 1 < nst) && st_i[st + 1] == st_j[st + 1] && st_i[st + 1] != i;
-                int sp[2] = {i, pair2 ? st_i[st + 1] : -1};
-                int sg[2] = {st, st + 1};
+                // this intra stage, and the last species' intra stage when it comes next
+                int nxt = -1;
+                for (int s2 = i + 1; s2 < ns; ++s2)
+                    if (a.charge[s2] != 0.0) { nxt = s2; break; }
+                const bool pair2 = (nxt >= 0) && (nxt == last);
+                if (pair2) skip_last_intra = true;
+                int sp[2] = {i, pair2 ? last : -1};
+                int sg[2] = {stage_now, stage_now + 1};
                 int nn[2] = {0, 0}, items[2] = {0, 0};
                 for (int u = 0; u < (pair2 ? 2 : 1); ++u) {
                     const int spc = sp[u];
@@ -298,15 +302,14 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
                                       n, n, a.dt, u1, u2);
                     }
                 }
-                if (pair2) ++st;
             } else {
                 if (na == 0 || nb == 0) continue;
-                keyed_perm(a.seed, a.step, st, i, a.id[i], a0, na, keyA, valA);
-                keyed_perm(a.seed, a.step, st, j, a.id[j], b0, nb, keyB, valB);
+                keyed_perm(a.seed, a.step, stage_now, i, a.id[i], a0, na, keyA, valA);
+                keyed_perm(a.seed, a.step, stage_now, j, a.id[j], b0, nb, keyB, valB);
                 if (na == nb) {
                     const double nP = na * w / Vc, nQ = nb * w / Vc;
                     for (int k = threadIdx.x; k < na; k += blockDim.x) {
-                        coll_rand(a.seed, a.step, cell, st, k, u1, u2);
+                        coll_rand(a.seed, a.step, cell, stage_now, k, u1, u2);
                         nanbu_collide(VA, valA[k], VB, valB[na - k - 1], mA, qA, mB, qB, lnL, a.calculate_bmin, debye,
                                       nP, nQ, nP, a.dt, u1, u2);
                     }
@@ -331,7 +334,7 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
                     if (nMinG1 > 0)
                         for (int m = threadIdx.x; m < nMinG1; m += blockDim.x)
                             for (int g1 = m; g1 < nMaxG1; g1 += nMinG1) {
-                                coll_rand(a.seed, a.step, cell, st, g1, u1, u2);
+                                coll_rand(a.seed, a.step, cell, stage_now, g1, u1, u2);
                                 nanbu_collide(Vmax, pmax[g1], Vmin, pmin[m], mMax, qMax, mMin, qMin, lnLp,
                                               a.calculate_bmin, debye, nMaxS, nMinS, nMaxS, a.dt, u1, u2);
                             }
@@ -339,7 +342,7 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
                     const int cyc = nMinG2 > 0 ? nMinG2 : 1;
                     for (int m = threadIdx.x; m < cyc; m += blockDim.x)
                         for (int g2 = m; g2 < nMaxG2; g2 += cyc) {
-                            coll_rand(a.seed, a.step, cell, st, nMaxG1 + g2, u1, u2);
+                            coll_rand(a.seed, a.step, cell, stage_now, nMaxG1 + g2, u1, u2);
                             nanbu_collide(Vmax, pmax[nMaxG1 + g2], Vmin, pmin[nMinG1 + m], mMax, qMax, mMin, qMin, lnLp,
                                           a.calculate_bmin, debye, nMaxS, nMinS, nMaxS, a.dt, u1, u2);
                         }

@@ -187,6 +187,30 @@ __global__ void k_loss_q(int n, int K, const int32_t* nb, const double* a, const
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
+// q2 = A q ; partial = q.q   (the residual A e of the guess is q)
+__global__ void k_loss_res(int n, int K, const int32_t* nb, const double* a, const double* diag, const double* q,
+                           double* q2, double* partial) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double s = diag[i] * q[i];
+        for (int k = 0; k < K; ++k) {
+            const int j = nb[(int64_t)i * K + k];
+            if (j >= 0) s -= a[(int64_t)i * K + k] * q[j];
+        }
+        q2[i] = s;
+        acc += q[i] * q[i];
+    }
+    acc = block_sum_d(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// q <- wa q + wr q2  (dloss/dout / 2)
+__global__ void k_combine(int n, double* q, const double* q2, double wa, double wr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) q[i] = wa * q[i] + wr * q2[i];
+}
+
 // ---------------------------------------------------------------- backward
 // dout = 2 q / norm ; g = dout w_dec (f32) ; partials of dw_dec, db_dec
 __global__ void __launch_bounds__(256) k_decode_bwd(int n, const double* q, double norm, const uint16_t* h,
@@ -418,6 +442,7 @@ hipError_t Gnn::alloc(int n_, int K_, int L_, hipStream_t s) {
     dm = (float*)A(sizeof(float) * 64 * (size_t)n);
     dz = (uint16_t*)A(sizeof(uint16_t) * 64 * (size_t)n);
     q = (double*)A(sizeof(double) * (size_t)n);
+    q2 = (double*)A(sizeof(double) * (size_t)n);
     red = (double*)A(sizeof(double) * 4096);
     scal = (double*)A(sizeof(double) * 16);
     part = (float*)A(sizeof(float) * 1024 * 8192);
@@ -484,10 +509,24 @@ hipError_t Gnn::forward(const float* xin, bool train) {
 
 hipError_t Gnn::loss_backward(const float* t, double norm, const int32_t* anb, const double* aa, const double* ad,
                               int aK, double* loss_out) {
+    return loss_backward2(t, norm, 0.0, anb, aa, ad, aK, loss_out);
+}
+
+hipError_t Gnn::loss_backward2(const float* t, double norm_a, double norm_r, const int32_t* anb, const double* aa,
+                               const double* ad, int aK, double* loss_out) {
     const size_t NH = (size_t)n * 64;
     const int nbq = nblk(n, 512);
+    // A-norm part: e^T A e / norm_a ; residual part: |A e|^2 / norm_r
+    const double wa = (loss_kind == 1) ? 0.0 : 1.0 / norm_a;
+    const double wr = (loss_kind == 0 || norm_r <= 0.0) ? 0.0 : 1.0 / norm_r;
     hipLaunchKernelGGL(k_loss_q, dim3(nbq), dim3(256), 0, stream, n, aK, anb, aa, ad, out, t, q, red);
     hipLaunchKernelGGL(k_reduce_n, dim3(1), dim3(64), 0, stream, red, nbq, 1, scal + 4);
+    if (wr != 0.0) {
+        hipLaunchKernelGGL(k_loss_res, dim3(nbq), dim3(256), 0, stream, n, aK, anb, aa, ad, q, q2, red);
+        hipLaunchKernelGGL(k_reduce_n, dim3(1), dim3(64), 0, stream, red, nbq, 1, scal + 6);
+    }
+    hipLaunchKernelGGL(k_combine, dim3((n + 255) / 256), dim3(256), 0, stream, n, q, q2, wa, wr);
+    const double norm = 1.0;
     const float* lay = theta + 256;
     const float* wdec = lay + L * (8192 + 64);
     float* glay = grad + 256;
@@ -513,10 +552,10 @@ hipError_t Gnn::loss_backward(const float* t, double norm, const int32_t* anb, c
     hipLaunchKernelGGL(k_encode_bwd, dim3(nb), dim3(256), 0, stream, n, g, z, cur_x, part);
     hipLaunchKernelGGL(k_reduce_f, dim3(1), dim3(256), 0, stream, part, nb, 256, grad, 1.f);
     if (loss_out) {
-        double v = 0.0;
-        hipMemcpyAsync(&v, scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream);
+        double v[3] = {0.0, 0.0, 0.0};
+        hipMemcpyAsync(v, scal + 4, sizeof(double) * 3, hipMemcpyDeviceToHost, stream);
         hipStreamSynchronize(stream);
-        *loss_out = v / norm;
+        *loss_out = wa * v[0] + wr * v[2];
     }
     return hipGetLastError();
 }
@@ -550,6 +589,19 @@ __global__ void k_anorm(int n, int K, const int32_t* nb, const double* a, const 
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 }  // namespace
+
+double gnn_rnorm(Gnn& G, const float* t, const int32_t* anb, const double* aa, const double* ad, int aK) {
+    // |A t|^2 : q = A t (via the loss kernel with out = t, target 0), then q.q
+    const int nb = nblk(G.n, 512);
+    hipMemsetAsync(G.dm, 0, sizeof(float) * G.n, G.stream);
+    hipLaunchKernelGGL(k_loss_q, dim3(nb), dim3(256), 0, G.stream, G.n, aK, anb, aa, ad, t, G.dm, G.q, G.red);
+    hipLaunchKernelGGL(k_loss_res, dim3(nb), dim3(256), 0, G.stream, G.n, aK, anb, aa, ad, G.q, G.q2, G.red);
+    hipLaunchKernelGGL(k_reduce_n, dim3(1), dim3(64), 0, G.stream, G.red, nb, 1, G.scal + 7);
+    double v = 0.0;
+    hipMemcpyAsync(&v, G.scal + 7, sizeof(double), hipMemcpyDeviceToHost, G.stream);
+    hipStreamSynchronize(G.stream);
+    return v;
+}
 
 double gnn_anorm(Gnn& G, const float* t, const int32_t* anb, const double* aa, const double* ad, int aK) {
     const int nb = nblk(G.n, 512);

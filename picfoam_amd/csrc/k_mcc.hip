@@ -98,7 +98,7 @@ __device__ void deposit_new(const MccArgs<UT>& a, int t, double y1, double y2, d
 // one candidate (ElectronNeutralCollisionModel.C:747-800 and the event handlers)
 template <class UT>
 __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double smax_local, double& qtcr,
-                          unsigned long long* cnt) {
+                          unsigned& c_el, unsigned& c_ex, unsigned& c_iz) {
     double ua, ub, n1, n2, n3, n4, upick, uacc;
     mcc_rand(a.seed, a.step, cell, k, 0, upick, uacc);
     mcc_rand(a.seed, a.step, cell, k, 1, ua, ub);
@@ -121,12 +121,12 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
     mcc_rand(a.seed, a.step, cell, k, 3, us, uc);
     if (uacc <= Qel / sigmaT) {
         nanbu_scatter(eV, Ue, Un, a.massE, a.massN, us, uc);
-        cnt[1]++;
+        ++c_el;
     } else if (eV >= a.th_ex && uacc <= (Qel + Qex) / sigmaT) {
         Ue = sqrt(1.0 - a.th_ex / eV) * Ue;
         eV -= a.th_ex;
         nanbu_scatter(eV, Ue, Un, a.massE, a.massN, us, uc);
-        cnt[2]++;
+        ++c_ex;
     } else if (eV >= a.th_iz && uacc <= Qt / sigmaT) {
         double uE, uR, up2, uc2;
         mcc_rand(a.seed, a.step, cell, k, 4, uE, uR);
@@ -174,7 +174,7 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
                 deposit_new(a, t, y1, y2, y3, a.count_e, a.s42_e);
             }
         }
-        if (created) cnt[3]++;
+        if (created) ++c_iz;
     }
     a.e.ux[ei] = (UT)Ue.x; a.e.uy[ei] = (UT)Ue.y; a.e.uz[ei] = (UT)Ue.z;
 }
@@ -199,8 +199,7 @@ __global__ void __launch_bounds__(64) k_mcc(MccArgs<UT> a) {
     const int nc = (int)coll;
     __syncthreads();
     if (threadIdx.x == 0) a.rem[cell] = coll - nc;
-    unsigned long long cnt[4] = {0, 0, 0, 0};
-    if (threadIdx.x == 0) cnt[0] = (unsigned long long)nc;
+    unsigned c_el = 0, c_ex = 0, c_iz = 0;
     double qtcr = 0.0;
     if (eC > 0 && nc > 0) {
         int np2 = 1;
@@ -229,7 +228,7 @@ __global__ void __launch_bounds__(64) k_mcc(MccArgs<UT> a) {
                 if (first) {
                     const int ei = val[pick];
                     for (int l = (int)threadIdx.x; l < 64 && base + l < nc; ++l)
-                        if (picks[l] == pick) candidate(a, cell, base + l, ei, smax, qtcr, cnt);
+                        if (picks[l] == pick) candidate(a, cell, base + l, ei, smax, qtcr, c_el, c_ex, c_iz);
                 }
             }
             __syncthreads();
@@ -237,10 +236,17 @@ __global__ void __launch_bounds__(64) k_mcc(MccArgs<UT> a) {
     }
     for (int off = 32; off > 0; off >>= 1) qtcr = fmax(qtcr, __shfl_xor(qtcr, off));
     if (threadIdx.x == 0 && qtcr > smax) a.smax[cell] = qtcr;
-    for (int q = 0; q < 4; ++q) {
-        unsigned long long v = cnt[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (threadIdx.x == 0 && v) atomicAdd(a.counts + q, v);
+    unsigned v1 = c_el, v2 = c_ex, v3 = c_iz;
+    for (int off = 32; off > 0; off >>= 1) {
+        v1 += __shfl_xor(v1, off);
+        v2 += __shfl_xor(v2, off);
+        v3 += __shfl_xor(v3, off);
+    }
+    if (threadIdx.x == 0) {
+        if (nc) atomicAdd(a.counts + 0, (unsigned long long)nc);
+        if (v1) atomicAdd(a.counts + 1, (unsigned long long)v1);
+        if (v2) atomicAdd(a.counts + 2, (unsigned long long)v2);
+        if (v3) atomicAdd(a.counts + 3, (unsigned long long)v3);
     }
 }
 

@@ -40,6 +40,7 @@ EXPORTS = (
     "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
     "pic_profile_reset", "pic_gnn_init", "pic_gnn_n_params", "pic_gnn_params", "pic_gnn_record",
     "pic_gnn_snapshot_count", "pic_gnn_snapshot", "pic_gnn_grad", "pic_gnn_train", "pic_gnn_eval", "pic_gnn_use",
+    "pic_gnn_loss",
 )
 
 
@@ -155,6 +156,7 @@ def load_library(path: str = LIB_PATH):
         "pic_gnn_train": (C.c_int, [vp, i32, dbl, C.c_uint64, C.POINTER(dbl), C.POINTER(dbl)]),
         "pic_gnn_eval": (C.c_int, [vp, i32, C.POINTER(dbl)]),
         "pic_gnn_use": (C.c_int, [vp, i32]),
+        "pic_gnn_loss": (C.c_int, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -451,3 +453,7 @@ class Cloud:
 
     def gnn_use(self, on=True):
         self._check(self.lib.pic_gnn_use(self.h, 1 if on else 0))
+
+    def gnn_loss(self, kind):
+        """0: A-norm error, 1: residual norm, 2: both"""
+        self._check(self.lib.pic_gnn_loss(self.h, int(kind)))
