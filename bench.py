@@ -84,6 +84,29 @@ def cpu_baseline(ncell_sample=8, ppc=64, target_s=12.0):
                       f"in {el:.1f} s, same dt/models as the GPU workload"}
 
 
+def copy_calibration(mib):
+    """hipMemcpyDeviceToDevice of mib MiB, three times: the blit kernel reads and writes
+    exactly mib MiB per dispatch.  Uses the HIP runtime libpicmi is linked against."""
+    import ctypes
+    from picfoam_amd import picmi
+    picmi.load_library()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    n = ctypes.c_size_t(mib * (1 << 20))
+    x, y = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (x, y):
+        if hip.hipMalloc(ctypes.byref(p), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(x, 1, n) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(y, x, n, 3) != 0:
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(x)
+    hip.hipFree(y)
+    print(f"bench: copy calibration {mib} MiB x3 done", file=sys.stderr, flush=True)
+
+
 def run_steps(args, precision, steps, warmup, profile):
     from picfoam_amd import cases, picmi
     mesh, species, models, ex = cases.thermalisation(ncell=args.ncell, ppc=args.ppc, precision=precision)
@@ -118,12 +141,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device three times (a plain copy of known size whose "
+                         "FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic figures)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU domain decomposition is a design, not code)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
     print(f"bench: {args.ncell}^3 mesh, {2 * args.ncell ** 3 * args.ppc} parcels, {args.precision}", file=sys.stderr,
           flush=True)
     n_total, elapsed, stages, dt = run_steps(args, args.precision, args.steps, args.warmup, True)

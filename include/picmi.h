@@ -98,10 +98,11 @@ typedef struct {
     const char* field_weighting;      /* SolverSettings.FieldWeigthing: CellAverage VolumeWeighting */
     const char* charge_distribution;  /* SolverSettings.ChargeDistribution: CellAverage VolumeWeighting */
     const char* maxwell_solver;       /* MaxwellSolver: ElectroStatic none */
-    const char* coulomb_collision;    /* CollisionModels.CoulombCollisionModel: Nanbu none */
-    const char* pairing_algorithm;    /* NanbuCoeffs.PairingAlgorithm: TakizukaAbe */
-    const char* weight_correction;    /* NanbuCoeffs.WeightCorrectionModel: none */
-    const char* electron_neutral;     /* CollisionModels.ElectronNeutralCollisionModel: NanbuElectronNeutral none */
+    const char* coulomb_collision;    /* CollisionModels.CoulombCollisionModel: Nanbu Perez none */
+    const char* pairing_algorithm;    /* <model>Coeffs.PairingAlgorithm: TakizukaAbe NanbuYonemura */
+    const char* weight_correction;    /* <model>Coeffs.WeightCorrectionModel: none Nanbu */
+    const char* electron_neutral;     /* CollisionModels.ElectronNeutralCollisionModel: NanbuElectronNeutral
+                                         RelativisticElectronNeutral none */
     const char* background_gas;       /* CollisionModels.BackgroundGasModel: uniformBackground none */
     int32_t allow_intra_collision;    /* NanbuCoeffs.allowIntraCollision */
     int32_t calculate_bmin;           /* NanbuCoeffs.DebyeLength.calculate_bmin */

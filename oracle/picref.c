@@ -749,7 +749,7 @@ int ref_sort_by_cell(const ref_mesh* m, int n, const int* tet, int* order, int* 
 
 /* ------------------------------------------------------------------ Coulomb */
 
-typedef struct { uint32_t key, id; int idx; } keyrec;
+typedef struct { uint64_t key; uint32_t id; int idx; } keyrec;
 static int keyrec_cmp(const void* a, const void* b) {
     const keyrec* x = (const keyrec*)a; const keyrec* y = (const keyrec*)b;
     if (x->key != y->key) return x->key < y->key ? -1 : 1;
@@ -757,22 +757,24 @@ static int keyrec_cmp(const void* a, const void* b) {
     return x->idx < y->idx ? -1 : (x->idx > y->idx);
 }
 
-/* shuffle (OpenFOAM Random::shuffle, TakizukaAbePairing.C:110) restated as a sort by a
- * counter-based random key of (particle id, step, stage, species) */
+/* shuffle (OpenFOAM Random::shuffle, TakizukaAbePairing.C:110, NanbuYonemuraPairing.C:124)
+ * restated as a sort by a counter-based 56-bit random key of (particle id, step, stage,
+ * species); ties (equal ids) fall back to the id, then the array position */
 static void keyed_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
                        int* perm) {
     keyrec* r = (keyrec*)malloc(sizeof(keyrec) * (n > 0 ? n : 1));
     for (int k = 0; k < n; ++k) {
         uint32_t o[4];
         rng4(seed, id[first + k], (uint32_t)step, (uint32_t)((stage << 8) | species), TAG_SHUFFLE << 24, o);
-        r[k].key = o[0]; r[k].id = id[first + k]; r[k].idx = first + k;
+        r[k].key = ((uint64_t)o[0] << 24) | (o[1] >> 8);
+        r[k].id = id[first + k]; r[k].idx = first + k;
     }
     qsort(r, n, sizeof(keyrec), keyrec_cmp);
     for (int k = 0; k < n; ++k) perm[k] = r[k].idx;
     free(r);
 }
 
-/* NanbuCollision.C:137-159 */
+/* NanbuCollision.C:137-159 (PerezRelativisticCollision.C:198-219 is the same function) */
 static double cos_chi(double s, double U) {
     double A, invA;
     if (s < 0.1) {
@@ -791,12 +793,12 @@ static double cos_chi(double s, double U) {
     return 2.0 * U - 1.0;
 }
 
-/* NanbuCollision::collide, NanbuCollision.C:60-134 (weight correction "none") */
-static void nanbu_collide(double* UP, double* UQ, double mP, double qP, double mQ, double qQ, double lnL_fixed,
+/* NanbuCollision::collide, NanbuCollision.C:60-134, on the two velocities */
+static void nanbu_collide(v3* UP, v3* UQ, double mP, double qP, double mQ, double qQ, double lnL_fixed,
                           int calc_bmin, double debye, double nP, double nQ, double nPQ, double dt, double u1,
                           double u2, double* stats) {
     double reMass = mP * mQ / (mP + mQ);
-    v3 uP = ld3(UP), uQ = ld3(UQ);
+    v3 uP = *UP, uQ = *UQ;
     v3 g = vsub(uP, uQ);
     double gm = vmag(g);
     double gp = sqrt(g.y * g.y + g.z * g.z);
@@ -816,10 +818,77 @@ static void nanbu_collide(double* UP, double* UQ, double mP, double qP, double m
     double ce = cos(eps), se = sin(eps);
     v3 h = V(gp * ce, -1.0 * (g.y * g.x * ce + gm * g.z * se) / gp, -1.0 * (g.z * g.x * ce - gm * g.y * se) / gp);
     v3 dv = vadd(vscale(1.0 - cc, g), vscale(sc, h));
-    uP = vsub(uP, vscale(mQ / (mP + mQ), dv));
-    uQ = vadd(uQ, vscale(mP / (mP + mQ), dv));
-    st3(UP, uP); st3(UQ, uQ);
-    if (stats) { stats[0] += 1.0; stats[1] += lnL; stats[2] += s; }
+    *UP = vsub(uP, vscale(mQ / (mP + mQ), dv));
+    *UQ = vadd(uQ, vscale(mP / (mP + mQ), dv));
+    if (stats) { stats[1] += lnL; stats[2] += s; }
+}
+
+/* PerezRelativisticCollision::collide, PerezRelativisticCollision.C:62-196: the scattering
+ * is done on the momentum of P in the centre-of-momentum frame */
+static void perez_collide(v3* UP, v3* UQ, double mP, double qP, double mQ, double qQ, double lnL_fixed,
+                          int calc_bmin, double debye, double nP, double nQ, double nPQ, double dt, double u1,
+                          double u2, double* stats) {
+    const double c = C_LIGHT, c2 = C_LIGHT * C_LIGHT;
+    v3 uP = *UP, uQ = *UQ;
+    double bP = vmag(uP) / c, bQ = vmag(uQ) / c;
+    double gP = 1.0 / sqrt(1.0 - bP * bP), gQ = 1.0 / sqrt(1.0 - bQ * bQ);
+    v3 pP = vscale(mP * gP, uP), pQ = vscale(mQ * gQ, uQ);
+    double msum = mP * gP + mQ * gQ;
+    v3 vc = vdiv(vadd(pP, pQ), msum);
+    double vc2 = vdot(vc, vc);
+    double gc, gcs, gPs, gQs;
+    v3 pPs;
+    if (vc2 == 0.0) {
+        gc = 1.0; gcs = 0.5; pPs = pP; gPs = gP; gQs = gQ;
+    } else {
+        gc = 1.0 / sqrt(1.0 - vc2 / c2);
+        gcs = (gc - 1.0) / vc2;
+        pPs = vadd(pP, vscale((gcs * vdot(vc, uP) - gc) * mP * gP, vc));
+        gPs = (1.0 - vdot(vc, uP) / c2) * gP * gc;
+        gQs = (1.0 - vdot(vc, uQ) / c2) * gQ * gc;
+    }
+    double pm = vmag(pPs);
+    if (pm == 0.0) return;
+    double lnL;
+    if (lnL_fixed == 0.0) {
+        double b0 = qP * qQ / (4.0 * PI_D * EPS0 * c2) * gc / msum * (1.0 + mP * gPs * mQ * gQs * c2 / (pm * pm));
+        if (calc_bmin) { double hb = HPLANCK / (2.0 * pm); b0 = hb > fabs(b0) ? hb : fabs(b0); }
+        double l = 0.5 * log(1.0 + (debye * debye) / (b0 * b0));
+        lnL = l > 2.0 ? l : 2.0;
+    } else lnL = lnL_fixed;
+    double s1 = gc * pm / msum;
+    double t2 = 1.0 + mP * gPs * mQ * gQs * c2 / (pm * pm);
+    double s2 = t2 * t2;
+    double s = (nP * nQ / nPQ) * dt * lnL * (qP * qQ) * (qP * qQ) /
+               (4.0 * PI_D * EPS0 * EPS0 * (c2 * c2) * mP * gP * mQ * gQ) * s1 * s2;
+    double vrel = pm * msum / (mP * gPs * mQ * gQs * gc);
+    double dmax = mP * pow(nP, 2.0 / 3.0), dq = mQ * pow(nQ, 2.0 / 3.0);
+    if (dq > dmax) dmax = dq;
+    double smax = pow(4.0 * PI_D / 3.0, 1.0 / 3.0) * (nP * nQ / nPQ) * dt * (mP + mQ) / dmax * vrel;
+    if (smax < s) s = smax;
+    double cc = cos_chi(s, u1);
+    double sc = sqrt(1.0 - cc * cc);
+    double eps = 2.0 * PI_D * u2;
+    double sce = sc * cos(eps), scs = sc * sin(eps);
+    v3 np;
+    double pperp = sqrt(pPs.x * pPs.x + pPs.y * pPs.y);
+    if (pperp > 1.e-10 * pm) {
+        double ip = 1.0 / pperp;
+        np = V((pPs.x * pPs.z * sce - pPs.y * pm * scs) * ip + pPs.x * cc,
+               (pPs.y * pPs.z * sce + pPs.x * pm * scs) * ip + pPs.y * cc, -pperp * sce + pPs.z * cc);
+    } else {
+        np = V(pm * sce, pm * scs, pm * cc);
+    }
+    v3 newP = vadd(np, vscale(gcs * vdot(vc, np) + mP * gPs * gc, vc));
+    double aP = vmag(newP) / (mP * c);
+    double ngP = sqrt(1.0 + aP * aP);
+    v3 mnp = vscale(-1.0, np);
+    v3 newQ = vadd(mnp, vscale(gcs * vdot(vc, mnp) + mQ * gQs * gc, vc));
+    double aQ = vmag(newQ) / (mQ * c);
+    double ngQ = sqrt(1.0 + aQ * aQ);
+    *UP = vdiv(newP, mP * ngP);
+    *UQ = vdiv(newQ, mQ * ngQ);
+    if (stats) { stats[1] += lnL; stats[2] += s; }
 }
 
 /* CoulombCollisionModel::debyeLength, CoulombCollisionModel.C:249-310 (all charged species,
@@ -851,17 +920,55 @@ static double debye_length(const ref_coulomb_params* p, double* const* U, const 
     return d > rmin ? d : rmin;
 }
 
-static void coll_rand(uint64_t seed, int64_t step, int cell, int stage, int k, double* u1, double* u2) {
+typedef struct {
+    const ref_coulomb_params* p;
+    double* const* U;
+    int cell, stage;
+    double debye, dt;
+    double* stats;
+} coll_ctx;
+
+/* one collision of parcel iP of species sP with parcel iQ of species sQ: two uniforms
+ * (cell, step, stage, k) for cos(chi) and the azimuth, a third for the weight correction */
+static void collide(const coll_ctx* x, int sP, int iP, int sQ, int iQ, double lnL, double nP, double nQ, double nPQ,
+                    int k) {
+    const ref_coulomb_params* p = x->p;
     uint32_t o[4];
-    rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)((stage << 24) | k), TAG_COULOMB << 24, o);
-    *u1 = u01(o[0], o[1]);
-    *u2 = u01(o[2], o[3]);
+    rng4(p->seed, (uint32_t)x->cell, (uint32_t)p->step, (uint32_t)((x->stage << 24) | k), TAG_COULOMB << 24, o);
+    double u1 = u01(o[0], o[1]), u2 = u01(o[2], o[3]);
+    v3 uP = ld3(x->U[sP] + 3 * iP), uQ = ld3(x->U[sQ] + 3 * iQ);
+    v3 preP = uP, preQ = uQ;
+    if (p->model == 1)
+        perez_collide(&uP, &uQ, p->mass[sP], p->charge[sP], p->mass[sQ], p->charge[sQ], lnL, p->calculate_bmin,
+                      x->debye, nP, nQ, nPQ, x->dt, u1, u2, x->stats);
+    else
+        nanbu_collide(&uP, &uQ, p->mass[sP], p->charge[sP], p->mass[sQ], p->charge[sQ], lnL, p->calculate_bmin,
+                      x->debye, nP, nQ, nPQ, x->dt, u1, u2, x->stats);
+    if (p->weight_correction) {
+        /* NanbuWeightCorrectionModel::correctVelocity, NanbuCorrection.C:50-69 */
+        rng4(p->seed, (uint32_t)x->cell, (uint32_t)p->step, (uint32_t)((x->stage << 24) | k),
+             (TAG_COULOMB << 24) | 1u, o);
+        double r = u01(o[0], o[1]);
+        double wP = p->weight[sP], wQ = p->weight[sQ];
+        if (r > wP / wQ) uQ = preQ;
+        if (r > wQ / wP) uP = preP;
+    }
+    st3(x->U[sP] + 3 * iP, uP);
+    st3(x->U[sQ] + 3 * iQ, uQ);
+    if (x->stats) x->stats[0] += 1.0;
 }
 
-/* TakizukaAbePairing::pairANDcollide, TakizukaAbePairing.C:60-338: per cell, species pairs
- * (i, j>=i) visited as i ascending, j descending; intra-species pairs in shuffled order
- * (odd count: first three collide cyclically with n12 = (n+3)w/V); inter-species equal
- * counts pair k with n-1-k; unequal counts use the two Takizuka-Abe groups. */
+static double dmin(double a, double b) { return a < b ? a : b; }
+
+/* TakizukaAbePairing::pairANDcollide (TakizukaAbePairing.C:60-338) and
+ * NanbuYonemuraPairing::pairANDcollide (NanbuYonemuraPairing.C:56-322): per cell, species
+ * pairs (i, j>=i) visited as i ascending, j descending, one stage each.
+ *   intra, even count: consecutive pairs of the shuffled list, n12 = n.
+ *   intra, odd count: Takizuka-Abe collides the first three cyclically (n12 = (n+3)w/V);
+ *     Nanbu-Yonemura collides the last with the first after the pairs (n12 = (n+1)w/V).
+ *   inter, equal counts: shuffled k with shuffled n-1-k.
+ *   inter, unequal counts: the two Takizuka-Abe groups, or Nanbu-Yonemura's cycling of
+ *     the smaller species over the larger one. */
 void ref_coulomb(const ref_mesh* m, const ref_coulomb_params* p, double dt, double* const* U,
                  const uint32_t* const* id, const int* const* cs, double* stats) {
     int ns = p->n_species;
@@ -874,87 +981,89 @@ void ref_coulomb(const ref_mesh* m, const ref_coulomb_params* p, double dt, doub
     for (int i = 0; i < ns; ++i)
         for (int j = i; j < ns; ++j)
             if (p->charge[i] != 0.0 && p->charge[j] != 0.0 && p->coulomb_log[i][j] <= 0.0) calc_debye = 1;
+    const int ny = p->pairing == 1;
+    coll_ctx x;
+    x.p = p; x.U = U; x.dt = dt; x.stats = stats;
     for (int c = 0; c < m->n_cells; ++c) {
         double Vc = m->V[c];
-        double debye = calc_debye ? debye_length(p, U, cs, c, Vc) : 0.0;
-        if (stats) stats[3] += debye;
+        x.cell = c;
+        x.debye = calc_debye ? debye_length(p, U, cs, c, Vc) : 0.0;
+        if (stats) stats[3] += x.debye;
         int stage = 0;
         for (int i = 0; i < ns; ++i) {
             if (p->charge[i] == 0.0) continue;
             for (int j = ns - 1; j >= i; --j) {
                 if (p->charge[j] == 0.0) continue;
                 int st = stage++;
+                x.stage = st;
                 int a0 = cs[i][c], na = cs[i][c + 1] - a0;
                 int b0 = cs[j][c], nb = cs[j][c + 1] - b0;
-                double mA = p->mass[i], qA = p->charge[i], mB = p->mass[j], qB = p->charge[j];
                 double lnL = p->coulomb_log[i][j];
-                double w = p->weight[i];
-                double u1, u2;
+                double wi = p->weight[i], wj = p->weight[j];
                 if (i == j) {
                     if (na <= 1 || !p->allow_intra) continue;
                     keyed_perm(p->seed, p->step, st, i, id[i], a0, na, pa);
-                    double n = na * w / Vc;
-                    double* Ub = U[i];
-                    if (na % 2) {
-                        double n12 = (na + 3.0) * w / Vc;
-                        int P = pa[0], Q = pa[1], R = pa[2];
-                        coll_rand(p->seed, p->step, c, st, 0, &u1, &u2);
-                        nanbu_collide(Ub + 3 * P, Ub + 3 * Q, mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n12, dt, u1, u2, stats);
-                        coll_rand(p->seed, p->step, c, st, 1, &u1, &u2);
-                        nanbu_collide(Ub + 3 * Q, Ub + 3 * R, mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n12, dt, u1, u2, stats);
-                        coll_rand(p->seed, p->step, c, st, 2, &u1, &u2);
-                        nanbu_collide(Ub + 3 * P, Ub + 3 * R, mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n12, dt, u1, u2, stats);
-                        for (int x = 3; x < na - 1; x += 2) {
-                            coll_rand(p->seed, p->step, c, st, 3 + (x - 3) / 2, &u1, &u2);
-                            nanbu_collide(Ub + 3 * pa[x], Ub + 3 * pa[x + 1], mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n12, dt, u1, u2, stats);
-                        }
+                    double n = na * wi / Vc;
+                    if (na % 2 == 0) {
+                        for (int x2 = 0; x2 < na - 1; x2 += 2)
+                            collide(&x, i, pa[x2], i, pa[x2 + 1], lnL, n, n, n, x2 / 2);
+                    } else if (!ny) {
+                        double n12 = (na + 3.0) * wi / Vc;
+                        collide(&x, i, pa[0], i, pa[1], lnL, n, n, n12, 0);
+                        collide(&x, i, pa[1], i, pa[2], lnL, n, n, n12, 1);
+                        collide(&x, i, pa[0], i, pa[2], lnL, n, n, n12, 2);
+                        for (int x2 = 3; x2 < na - 1; x2 += 2)
+                            collide(&x, i, pa[x2], i, pa[x2 + 1], lnL, n, n, n12, 3 + (x2 - 3) / 2);
                     } else {
-                        for (int x = 0; x < na - 1; x += 2) {
-                            coll_rand(p->seed, p->step, c, st, x / 2, &u1, &u2);
-                            nanbu_collide(Ub + 3 * pa[x], Ub + 3 * pa[x + 1], mA, qA, mA, qA, lnL, p->calculate_bmin, debye, n, n, n, dt, u1, u2, stats);
-                        }
+                        double n12 = 2.0 * wi * ((na - 1) / 2 + 1) / Vc;
+                        for (int x2 = 0; x2 < na - 2; x2 += 2)
+                            collide(&x, i, pa[x2], i, pa[x2 + 1], lnL, n, n, n12, x2 / 2);
+                        collide(&x, i, pa[na - 1], i, pa[0], lnL, n, n, n12, (na - 1) / 2);
                     }
                 } else {
                     if (na == 0 || nb == 0) continue;
                     keyed_perm(p->seed, p->step, st, i, id[i], a0, na, pa);
                     keyed_perm(p->seed, p->step, st, j, id[j], b0, nb, pb);
                     if (na == nb) {
-                        double nP = na * w / Vc, nQ = nb * w / Vc;
-                        for (int k = 0; k < na; ++k) {
-                            coll_rand(p->seed, p->step, c, st, k, &u1, &u2);
-                            nanbu_collide(U[i] + 3 * pa[k], U[j] + 3 * pb[na - k - 1], mA, qA, mB, qB, lnL, p->calculate_bmin, debye, nP, nQ, nP, dt, u1, u2, stats);
-                        }
+                        double nP = na * wi / Vc, nQ = nb * wj / Vc;
+                        double nPQ = ny ? na * dmin(wi, wj) / Vc : nP;
+                        for (int k = 0; k < na; ++k)
+                            collide(&x, i, pa[k], j, pb[na - k - 1], lnL, nP, nQ, nPQ, k);
                     } else {
-                        /* min / max species (TakizukaAbePairing.C:197-308) */
                         int minIsA = na < nb;
                         int nMin = minIsA ? na : nb, nMax = minIsA ? nb : na;
                         int* pmin = minIsA ? pa : pb;
                         int* pmax = minIsA ? pb : pa;
                         int smin = minIsA ? i : j, smax = minIsA ? j : i;
-                        double nMinS = nMin * w / Vc, nMaxS = nMax * w / Vc;
-                        int delta = nMax / nMin;
-                        double delta_r = (double)(nMax % nMin) / nMin;
-                        int nMinG1 = (int)(delta_r * nMin);
-                        int nMaxG1 = (int)((delta + 1) * delta_r * nMin);
-                        int nMinG2 = (int)((1.0 - delta_r) * nMin);
-                        int nMaxG2 = (int)(delta * (1.0 - delta_r) * nMin);
-                        double mMin = p->mass[smin], qMin = p->charge[smin], mMax = p->mass[smax], qMax = p->charge[smax];
+                        double nMinS = nMin * p->weight[smin] / Vc, nMaxS = nMax * p->weight[smax] / Vc;
                         double lnLp = p->coulomb_log[smax][smin];
-                        if (nMinG1 > 0) {
-                            int s1 = 0, s2 = 0;
-                            for (int g1 = 0; g1 < nMaxG1; ++g1) {
-                                if (s1 >= nMinG1) s1 = 0;
-                                coll_rand(p->seed, p->step, c, st, g1, &u1, &u2);
-                                nanbu_collide(U[smax] + 3 * pmax[s2], U[smin] + 3 * pmin[s1], mMax, qMax, mMin, qMin, lnLp, p->calculate_bmin, debye, nMaxS, nMinS, nMaxS, dt, u1, u2, stats);
+                        if (ny) {
+                            /* NanbuYonemuraPairing.C:253-300: min parcel l mod nMin with max parcel l */
+                            double n12 = nMax * dmin(wi, wj) / Vc;
+                            for (int l = 0; l < nMax; ++l)
+                                collide(&x, smax, pmax[l], smin, pmin[l % nMin], lnLp, nMaxS, nMinS, n12, l);
+                        } else {
+                            /* TakizukaAbePairing.C:197-308 */
+                            int delta = nMax / nMin;
+                            double delta_r = (double)(nMax % nMin) / nMin;
+                            int nMinG1 = (int)(delta_r * nMin);
+                            int nMaxG1 = (int)((delta + 1) * delta_r * nMin);
+                            int nMinG2 = (int)((1.0 - delta_r) * nMin);
+                            int nMaxG2 = (int)(delta * (1.0 - delta_r) * nMin);
+                            if (nMinG1 > 0) {
+                                int s1 = 0, s2 = 0;
+                                for (int g1 = 0; g1 < nMaxG1; ++g1) {
+                                    if (s1 >= nMinG1) s1 = 0;
+                                    collide(&x, smax, pmax[s2], smin, pmin[s1], lnLp, nMaxS, nMinS, nMaxS, g1);
+                                    ++s1; ++s2;
+                                }
+                            }
+                            int s1 = nMinG1, s2 = nMaxG1;
+                            for (int g2 = 0; g2 < nMaxG2; ++g2) {
+                                if (s1 >= nMinG2 + nMinG1) s1 = nMinG1;
+                                collide(&x, smax, pmax[s2], smin, pmin[s1], lnLp, nMaxS, nMinS, nMaxS, nMaxG1 + g2);
                                 ++s1; ++s2;
                             }
-                        }
-                        int s1 = nMinG1, s2 = nMaxG1;
-                        for (int g2 = 0; g2 < nMaxG2; ++g2) {
-                            if (s1 >= nMinG2 + nMinG1) s1 = nMinG1;
-                            coll_rand(p->seed, p->step, c, st, nMaxG1 + g2, &u1, &u2);
-                            nanbu_collide(U[smax] + 3 * pmax[s2], U[smin] + 3 * pmin[s1], mMax, qMax, mMin, qMin, lnLp, p->calculate_bmin, debye, nMaxS, nMinS, nMaxS, dt, u1, u2, stats);
-                            ++s1; ++s2;
                         }
                     }
                 }

@@ -85,7 +85,9 @@ void ref_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4
  * and cell_start[n_cells+1]; dead particles (tet<0) are dropped. */
 int ref_sort_by_cell(const ref_mesh* m, int n, const int* tet, int* order, int* cell_start);
 
-/* Coulomb collisions, Nanbu + Takizuka-Abe pairing on sorted species arrays. */
+/* Coulomb collisions on sorted species arrays: Nanbu (model 0) or Perez (model 1)
+ * scattering, Takizuka-Abe (pairing 0) or Nanbu-Yonemura (pairing 1) pairing, optional
+ * Nanbu weight correction.  stats: [0] collisions, [1] sum lnL, [2] sum s, [3] sum Debye. */
 typedef struct {
     int n_species;
     double mass[8], charge[8], weight[8];
@@ -94,6 +96,9 @@ typedef struct {
     int allow_intra;
     uint64_t seed;
     int64_t step;
+    int model;
+    int pairing;
+    int weight_correction;
 } ref_coulomb_params;
 void ref_coulomb(const ref_mesh* m, const ref_coulomb_params* p, double dt, double* const* U,
                  const uint32_t* const* id, const int* const* cell_start, double* stats);

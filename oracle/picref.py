@@ -210,12 +210,18 @@ def push_one(pusher, U, E, B, q, mass, dt, hc_ref=False, rotate=True):
 class CoulombParams(C.Structure):
     _fields_ = [("n_species", C.c_int), ("mass", C.c_double * 8), ("charge", C.c_double * 8),
                 ("weight", C.c_double * 8), ("coulomb_log", (C.c_double * 8) * 8), ("calculate_bmin", C.c_int),
-                ("allow_intra", C.c_int), ("seed", C.c_uint64), ("step", C.c_int64)]
+                ("allow_intra", C.c_int), ("seed", C.c_uint64), ("step", C.c_int64), ("model", C.c_int),
+                ("pairing", C.c_int), ("weight_correction", C.c_int)]
+
+
+COULOMB_MODELS = {"Nanbu": 0, "Perez": 1}
+PAIRINGS = {"TakizukaAbe": 0, "NanbuYonemura": 1}
 
 
 def coulomb(rm: RefMesh, species, U_list, id_list, cs_list, dt, seed, step, coulomb_log=None, calculate_bmin=True,
-            allow_intra=True):
-    """species: [(mass, charge, weight)]; U_list: sorted-by-cell velocity arrays (modified copies returned)."""
+            allow_intra=True, model="Nanbu", pairing="TakizukaAbe", weight_correction=False):
+    """species: [(mass, charge, weight)]; U_list: sorted-by-cell velocity arrays (modified copies
+    returned).  stats: [0] collisions, [1] sum lnL, [2] sum s, [3] sum of Debye lengths."""
     p = CoulombParams()
     p.n_species = len(species)
     for i, (m, q, w) in enumerate(species):
@@ -226,6 +232,7 @@ def coulomb(rm: RefMesh, species, U_list, id_list, cs_list, dt, seed, step, coul
             p.coulomb_log[i][j] = float(cl[i][j]) if i < len(cl) and j < len(cl[i]) else 0.0
     p.calculate_bmin = int(calculate_bmin); p.allow_intra = int(allow_intra)
     p.seed = seed; p.step = step
+    p.model = COULOMB_MODELS[model]; p.pairing = PAIRINGS[pairing]; p.weight_correction = int(weight_correction)
     Us = [np.ascontiguousarray(u, np.float64).copy() for u in U_list]
     ids = [np.ascontiguousarray(i, np.uint32) for i in id_list]
     css = [np.ascontiguousarray(c, np.int32) for c in cs_list]

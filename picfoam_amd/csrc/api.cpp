@@ -59,6 +59,8 @@ struct pic_ctx {
     // models
     int pusher = kPushBoris, fw = kCellAverage, cd = kCellAverage;
     bool electrostatic = false, coulomb = false, mcc = false, bg = false;
+    int coulomb_model = 0, pairing = 0, weight_corr = 0;
+    int en_relativistic = 0;
     int precision = PIC_PREC_FP64;
     int hc_ref = 0;
     int allow_intra = 1, calc_bmin = 1;
@@ -378,6 +380,9 @@ int do_coulomb(pic_ctx* c, double dt, int maxc) {
     a.seed = c->seed;
     a.step = c->step;
     a.maxp = pow2_at_least(maxc);
+    a.model = c->coulomb_model;
+    a.pairing = c->pairing;
+    a.weight_correction = c->weight_corr;
     // two key/index buffers per species pair in LDS: 24 B per slot, at most 48 KiB
     if (a.maxp > 2048) return c->fail(PIC_ERR_CELL_OVERFLOW, "more than 2048 parcels of one species in a cell");
     a.flags = c->d_flags;
@@ -591,24 +596,29 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     if ((r = parse_model(c.get(), models->maxwell_solver, "MaxwellSolver", {"none", "ElectroStatic"}, &ms))) return r;
     c->electrostatic = ms == 1;
     int cm = 0;
-    if ((r = parse_model(c.get(), models->coulomb_collision, "CoulombCollisionModel", {"none", "Nanbu"}, &cm))) return r;
-    c->coulomb = cm == 1;
+    if ((r = parse_model(c.get(), models->coulomb_collision, "CoulombCollisionModel", {"none", "Nanbu", "Perez"}, &cm)))
+        return r;
+    c->coulomb = cm != 0;
+    c->coulomb_model = cm == 2 ? 1 : 0;
     if (c->coulomb) {
         int pa = 0, wc = 0;
         if ((r = parse_model(c.get(), models->pairing_algorithm ? models->pairing_algorithm : "TakizukaAbe",
-                             "PairingAlgorithm", {"TakizukaAbe"}, &pa)))
+                             "PairingAlgorithm", {"TakizukaAbe", "NanbuYonemura"}, &pa)))
             return r;
         if ((r = parse_model(c.get(), models->weight_correction ? models->weight_correction : "none",
-                             "WeightCorrectionModel", {"none"}, &wc)))
+                             "WeightCorrectionModel", {"none", "Nanbu"}, &wc)))
             return r;
+        c->pairing = pa;
+        c->weight_corr = wc;
     }
     int en = 0, bgm = 0;
     if ((r = parse_model(c.get(), models->electron_neutral, "ElectronNeutralCollisionModel",
-                         {"none", "NanbuElectronNeutral"}, &en)))
+                         {"none", "NanbuElectronNeutral", "RelativisticElectronNeutral"}, &en)))
         return r;
     if ((r = parse_model(c.get(), models->background_gas, "BackgroundGasModel", {"none", "uniformBackground"}, &bgm)))
         return r;
-    c->mcc = en == 1;
+    c->mcc = en != 0;
+    c->en_relativistic = en == 2 ? 1 : 0;
     c->bg = bgm == 1;
     if (c->mcc && !c->bg) {
         g_last_error = c->err = "ElectronNeutralCollisionModel needs the uniformBackground gas model (particle neutrals are not implemented)";
@@ -643,6 +653,16 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         if (species[c->electron].n_equivalent != c->bg_w && c->create_electron)
             return c->fail(PIC_ERR_UNSUPPORTED, "created electrons carry the background weight; the electron weight must equal it");
     }
+    // nParticleEqWeight (PICCloud.C:86-93, :414-417): zero unless every species and the
+    // background gas carry the same weight
+    double eqw = species[0].n_equivalent;
+    for (int s = 1; s < n_species; ++s)
+        if (species[s].n_equivalent != eqw) eqw = 0.0;
+    if (c->bg && c->bg_w != eqw) eqw = 0.0;
+    if (c->coulomb && c->pairing == 0 && eqw == 0.0)
+        return c->fail(PIC_ERR_UNSUPPORTED, "Model TakizukaAbe is only valid for equal weighted particles");
+    // WeightCorrectionModel::active (WeightCorrectionModel.C:91-95)
+    c->weight_corr = (c->weight_corr && eqw == 0.0) ? 1 : 0;
     // the model selection above is validated before any device is touched
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
@@ -692,10 +712,6 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
             return c->fail(PIC_ERR_INVALID, "species " + std::to_string(s) + ": mass, weight and capacity must be positive (capacity < 2^31)");
         qmax = std::max(qmax, std::fabs(S.charge * S.weight));
         c->sp.push_back(S);
-    }
-    if (c->coulomb) {
-        double w0 = -1;
-        for (auto& s : c->sp) if (s.charge != 0.0) { if (w0 < 0) w0 = s.weight; else if (s.weight != w0) return c->fail(PIC_ERR_UNSUPPORTED, "Model TakizukaAbe is only valid for equal weighted particles"); }
     }
     c->Q0 = qmax > 0 ? qmax : 1.0;
     // device mesh

@@ -1,10 +1,19 @@
-// Binary Coulomb collisions: Nanbu scattering (NanbuCollision.C:60-159) with
-// Takizuka-Abe pairing (TakizukaAbePairing.C:60-338).  One wave per cell; parcels are
-// contiguous per species because the arrays are sorted by cell.  The reference's
-// shuffle (OpenFOAM Random::shuffle) becomes a sort of the cell's parcels by a
-// counter-based random key of (id, step, stage, species), so the pairing does not depend
-// on the order the parcels sit in memory; every collision draws its two uniforms from a
-// counter (cell, step, stage, collision index).
+// Binary Coulomb collisions (CoulombCollisionModel::handleCollisions,
+// CoulombCollisionModel.C:241-245): Nanbu (NanbuCollision.C:60-159) or Perez
+// (PerezRelativisticCollision.C:62-219) scattering, Takizuka-Abe (TakizukaAbePairing.C:60-338)
+// or Nanbu-Yonemura (NanbuYonemuraPairing.C:56-322) pairing, optional Nanbu weight
+// correction (NanbuCorrection.C:50-69).
+//
+// One wave per cell; parcels are contiguous per species because the arrays are sorted by
+// cell.  The reference's shuffle (OpenFOAM Random::shuffle) becomes a ranking of the cell's
+// parcels by a counter-based random key of (id, step, stage, species), so the pairing does
+// not depend on the order the parcels sit in memory; every collision draws its uniforms
+// from a counter (cell, step, stage, collision index).
+//
+// Each pairing stage is described by a small table in LDS and split into work units: a
+// unit is a run of collisions that must happen in order (the Takizuka-Abe odd triple, a
+// parcel of the smaller species meeting several parcels of the larger one); units touch
+// disjoint parcels and run on separate lanes.  All collisions go through one call site.
 #include "device.h"
 #include "kernels.h"
 
@@ -12,86 +21,230 @@ namespace picmi {
 
 namespace {
 
-// NanbuCollision.C:137-159, in the arithmetic M of the collision (FP64, or FP32 in the
-// mixed-precision mode)
+// NanbuCollision.C:137-159 / PerezRelativisticCollision.C:198-219 in the arithmetic M.  The
+// three regimes are evaluated for every lane with one log, so lanes of different s do not
+// diverge: exp(-A) + 2 U sinh(A) = e^-A (1 - U) + U e^A.
 template <class M>
-__device__ M cos_chi(M s, M U) {
-    M A, invA;
-    if (s < (M)0.1) {
-        if (U < (M)0.0001) U = (M)0.0001;
-        return (M)1 + s * log(U);
-    }
-    if (s < (M)3) {
-        invA = (M)0.00569578 +
-               ((M)0.95602 + ((M)-0.508139 + ((M)0.479139 + ((M)-0.12789 + (M)0.0238957 * s) * s) * s) * s) * s;
-        A = (M)1 / invA;
-        return invA * log(exp(-A) + (M)2 * U * sinh(A));
-    }
-    if (s < (M)6) {
-        A = (M)3 * exp(-s);
-        return ((M)1 / A) * log(exp(-A) + (M)2 * U * sinh(A));
-    }
-    return (M)2 * U - (M)1;
+__device__ __forceinline__ M cos_chi(M s, M U) {
+    const bool small = s < (M)0.1, mid = s < (M)3, big = s < (M)6;
+    const M invA = (M)0.00569578 +
+                   ((M)0.95602 + ((M)-0.508139 + ((M)0.479139 + ((M)-0.12789 + (M)0.0238957 * s) * s) * s) * s) * s;
+    M A = mid ? (M)1 / invA : (M)3 * exp(-s);
+    const M coef = mid ? invA : (M)1 / A;
+    if (small || !big) A = (M)1;  // keeps the unused branch finite
+    const M eA = exp(A);
+    const M Us = U < (M)0.0001 ? (M)0.0001 : U;
+    const M arg = small ? Us : ((M)1 - U) / eA + U * eA;
+    const M L = log(arg);
+    const M cc = small ? (M)1 + s * L : (big ? coef * L : (M)2 * U - (M)1);
+    // rounding can put coef * L an ulp outside [-1, 1] (U near 0 or 1), which would make
+    // sin(chi) = sqrt(1 - cos^2) a NaN; the reference formula has the same edge
+    return cc > (M)1 ? (M)1 : (cc < (M)-1 ? (M)-1 : cc);
 }
 
-__device__ __forceinline__ void sincos_m(double x, double* s, double* c) { sincos(x, s, c); }
-__device__ __forceinline__ void sincos_m(float x, float* s, float* c) { sincosf(x, s, c); }
+__device__ __forceinline__ void sincos_2pi(double u, double* s, double* c) { sincospi(2.0 * u, s, c); }
+__device__ __forceinline__ void sincos_2pi(float u, float* s, float* c) { sincospif(2.f * u, s, c); }
 
-template <class UT>
-struct Vel {
-    UT* x;
-    UT* y;
-    UT* z;
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+enum : int { kIntraEven = 0, kIntraOddTA = 1, kIntraOddNY = 2, kInterEqual = 3, kCyclic = 4 };
+
+// one pairing stage: P and Q species velocity arrays, the shuffled index lists (LDS) and
+// the densities of the collide() call.  kCyclic (unequal inter-species counts): P is the
+// larger species; unit m < g1_units is parcel g1_yo + m of Q meeting P parcels
+// g1_xo + l, l = m, m + g1_stride, ... < g1_lmax; the remaining units form group 2 alike.
+struct StageK {
+    void* u[6];
+    const int* permP;
+    const int* permQ;
+    double mP, qP, mQ, qQ, wP, wQ;
+    double nP, nQ, nPQ, lnL;
+    int kind, n, units, stage;
+    int g1_units, g1_lmax, g1_stride, g1_xo, g1_yo, g1_ko;
+    int g2_lmax, g2_stride, g2_xo, g2_yo, g2_ko;
 };
 
-// NanbuCollision::collide (weight correction "none"); arithmetic in M = the velocity type
-template <class UT>
-__device__ void nanbu_collide(Vel<UT> vp, int64_t ip, Vel<UT> vq, int64_t iq, double mP, double qP, double mQ,
-                              double qQ, double lnL_fixed, int calc_bmin, double debye, double nP, double nQ,
-                              double nPQ, double dt, double u1d, double u2d) {
-    using M = UT;
-    const M reMass = (M)(mP * mQ / (mP + mQ));
-    vec3<M> uP = mk3<M>(vp.x[ip], vp.y[ip], vp.z[ip]);
-    vec3<M> uQ = mk3<M>(vq.x[iq], vq.y[iq], vq.z[iq]);
-    vec3<M> g = uP - uQ;
-    const M gm = mag(g);
-    const M gp = sqrt(g.y * g.y + g.z * g.z);
-    if (!(gm > (M)0) || !(gp > (M)0)) return;
-    M lnL;
-    if (lnL_fixed == 0.0) {
-        M b0 = (M)(qP * qQ / (2.0 * kPi * kEps0)) / (reMass * gm * gm);
-        if (calc_bmin) {
-            M hb = (M)kH / ((M)2 * (gm * reMass));
-            b0 = hb > fabs(b0) ? hb : fabs(b0);
+// collision t of unit u; false once the unit is exhausted
+__device__ __forceinline__ bool decode(const StageK& d, int u, int t, int& iP, int& iQ, int& k) {
+    const int* P = d.permP;
+    switch (d.kind) {
+        case kIntraEven:
+            if (t) return false;
+            iP = P[2 * u]; iQ = P[2 * u + 1]; k = u;
+            return true;
+        case kIntraOddTA:  // TakizukaAbePairing.C:115-139: the first three collide cyclically
+            if (u == 0) {
+                if (t >= 3) return false;
+                iP = P[t == 1 ? 1 : 0]; iQ = P[t == 0 ? 1 : 2]; k = t;
+                return true;
+            }
+            if (t) return false;
+            iP = P[2 * u + 1]; iQ = P[2 * u + 2]; k = u + 2;
+            return true;
+        case kIntraOddNY:  // NanbuYonemuraPairing.C:127-152: pairs, then last with first
+            if (t == 0) { iP = P[2 * u]; iQ = P[2 * u + 1]; k = u; return true; }
+            if (u == 0 && t == 1) { iP = P[d.n - 1]; iQ = P[0]; k = (d.n - 1) >> 1; return true; }
+            return false;
+        case kInterEqual:
+            if (t) return false;
+            iP = P[u]; iQ = d.permQ[d.n - 1 - u]; k = u;
+            return true;
+        default: {
+            int l, m, xo, yo, ko;
+            if (u < d.g1_units) {
+                m = u; l = u + t * d.g1_stride;
+                if (l >= d.g1_lmax) return false;
+                xo = d.g1_xo; yo = d.g1_yo; ko = d.g1_ko;
+            } else {
+                m = u - d.g1_units; l = m + t * d.g2_stride;
+                if (l >= d.g2_lmax) return false;
+                xo = d.g2_xo; yo = d.g2_yo; ko = d.g2_ko;
+            }
+            iP = P[xo + l]; iQ = d.permQ[yo + m]; k = ko + l;
+            return true;
         }
-        M l = (M)0.5 * log((M)1 + ((M)debye * (M)debye) / (b0 * b0));
-        lnL = l > (M)2 ? l : (M)2;
-    } else {
-        lnL = (M)lnL_fixed;
     }
-    const double ad = qP * qQ / (kEps0 * (mP * mQ / (mP + mQ)));
-    // s = lnL/(4 pi) (qP qQ / eps0 mu)^2 (nP nQ / nPQ) g^-3 dt, the constant part in FP64
-    const M cst = (M)(1.0 / (4.0 * kPi) * (ad * ad) * (nQ * nP / nPQ) * dt);
-    const M s = lnL * cst * ((M)1 / (gm * gm * gm));
-    const M cc = cos_chi<M>(s, (M)u1d);
-    const M sc = sqrt((M)1 - cc * cc);
-    const M eps = (M)(2.0 * kPi) * (M)u2d;
-    M se, ce;
-    sincos_m(eps, &se, &ce);
-    const M igp = (M)1 / gp;
-    vec3<M> h = mk3<M>(gp * ce, -(g.y * g.x * ce + gm * g.z * se) * igp, -(g.z * g.x * ce - gm * g.y * se) * igp);
-    vec3<M> dv = ((M)1 - cc) * g + sc * h;
-    uP = uP - (M)(mQ / (mP + mQ)) * dv;
-    uQ = uQ + (M)(mP / (mP + mQ)) * dv;
-    vp.x[ip] = uP.x; vp.y[ip] = uP.y; vp.z[ip] = uP.z;
-    vq.x[iq] = uQ.x; vq.y[iq] = uQ.y; vq.z[iq] = uQ.z;
 }
 
-__device__ __forceinline__ void coll_rand(uint64_t seed, int64_t step, int cell, int stage, int k, double& u1,
-                                          double& u2) {
-    uint4 o = rng4(seed, (uint32_t)cell, (uint32_t)step, ((uint32_t)stage << 24) | (uint32_t)k, kTagCoulomb << 24);
-    u1 = u01(o.x, o.y);
-    u2 = u01(o.z, o.w);
+// NanbuCollision::collide (NanbuCollision.C:60-134) on two velocities, arithmetic M
+template <class M>
+__device__ __forceinline__ void nanbu_scatter(vec3<M>& uP, vec3<M>& uQ, const StageK& d, int calc_bmin, double debye,
+                                              double dt, double u1, double u2) {
+    const double mP = d.mP, mQ = d.mQ;
+    const double reMass_d = mP * mQ / (mP + mQ);
+    const M reMass = (M)reMass_d;
+    const vec3<M> g = uP - uQ;
+    const M gm = mag(g);
+    const M gp = sqrt(g.y * g.y + g.z * g.z);
+    if (!(gm > (M)0) || !(gp > (M)0)) return;  // the reference divides by zero here
+    M lnL;
+    if (d.lnL == 0.0) {
+        M b0 = (M)(d.qP * d.qQ / (2.0 * kPi * kEps0)) / (reMass * gm * gm);
+        if (calc_bmin) {
+            const M hb = (M)kH / ((M)2 * (gm * reMass));
+            b0 = hb > fabs(b0) ? hb : fabs(b0);
+        }
+        const M l = (M)0.5 * log((M)1 + ((M)debye * (M)debye) / (b0 * b0));
+        lnL = l > (M)2 ? l : (M)2;
+    } else {
+        lnL = (M)d.lnL;
+    }
+    const double ad = d.qP * d.qQ / (kEps0 * reMass_d);
+    // s = lnL/(4 pi) (qP qQ / eps0 mu)^2 (nP nQ / nPQ) g^-3 dt, the constant part in FP64
+    const M cst = (M)(1.0 / (4.0 * kPi) * (ad * ad) * (d.nQ * d.nP / d.nPQ) * dt);
+    const M s = lnL * cst * ((M)1 / (gm * gm * gm));
+    const M cc = cos_chi<M>(s, (M)u1);
+    const M sc = sqrt((M)1 - cc * cc);
+    M se, ce;
+    sincos_2pi((M)u2, &se, &ce);
+    const M igp = (M)1 / gp;
+    const vec3<M> h = mk3<M>(gp * ce, -(g.y * g.x * ce + gm * g.z * se) * igp, -(g.z * g.x * ce - gm * g.y * se) * igp);
+    const vec3<M> dv = ((M)1 - cc) * g + sc * h;
+    uP = uP - (M)(mQ / (mP + mQ)) * dv;
+    uQ = uQ + (M)(mP / (mP + mQ)) * dv;
+}
+
+// PerezRelativisticCollision::collide (PerezRelativisticCollision.C:62-196), FP64
+__device__ __forceinline__ void perez_scatter(vec3<double>& UP, vec3<double>& UQ, const StageK& d, int calc_bmin,
+                                              double debye, double dt, double u1, double u2) {
+    const double c2 = kC * kC;
+    const double mP = d.mP, mQ = d.mQ;
+    const double bP = mag(UP) / kC, bQ = mag(UQ) / kC;
+    const double gP = 1.0 / sqrt(1.0 - bP * bP), gQ = 1.0 / sqrt(1.0 - bQ * bQ);
+    const vec3<double> pP = (mP * gP) * UP, pQ = (mQ * gQ) * UQ;
+    const double msum = mP * gP + mQ * gQ;
+    const vec3<double> vc = (1.0 / msum) * (pP + pQ);
+    const double vc2 = dot(vc, vc);
+    double gc, gcs, gPs, gQs;
+    vec3<double> pPs;
+    if (vc2 == 0.0) {
+        gc = 1.0; gcs = 0.5; pPs = pP; gPs = gP; gQs = gQ;
+    } else {
+        gc = 1.0 / sqrt(1.0 - vc2 / c2);
+        gcs = (gc - 1.0) / vc2;
+        pPs = pP + ((gcs * dot(vc, UP) - gc) * mP * gP) * vc;
+        gPs = (1.0 - dot(vc, UP) / c2) * gP * gc;
+        gQs = (1.0 - dot(vc, UQ) / c2) * gQ * gc;
+    }
+    const double pm = mag(pPs);
+    if (pm == 0.0) return;
+    const double t2 = 1.0 + mP * gPs * mQ * gQs * c2 / (pm * pm);
+    double lnL;
+    if (d.lnL == 0.0) {
+        double b0 = d.qP * d.qQ / (4.0 * kPi * kEps0 * c2) * gc / msum * t2;
+        if (calc_bmin) {
+            const double hb = kH / (2.0 * pm);
+            b0 = hb > fabs(b0) ? hb : fabs(b0);
+        }
+        const double l = 0.5 * log(1.0 + (debye * debye) / (b0 * b0));
+        lnL = l > 2.0 ? l : 2.0;
+    } else {
+        lnL = d.lnL;
+    }
+    const double qq = d.qP * d.qQ;
+    double s = (d.nP * d.nQ / d.nPQ) * dt * lnL * qq * qq / (4.0 * kPi * kEps0 * kEps0 * (c2 * c2) * mP * gP * mQ * gQ) *
+               (gc * pm / msum) * (t2 * t2);
+    // low-temperature limit of s; 1.61199... = (4 pi / 3)^(1/3)
+    const double vrel = pm * msum / (mP * gPs * mQ * gQs * gc);
+    const double dP = mP * pow(d.nP, 2.0 / 3.0), dQ = mQ * pow(d.nQ, 2.0 / 3.0);
+    const double smax = 1.6119919540164696 * (d.nP * d.nQ / d.nPQ) * dt * (mP + mQ) / (dP > dQ ? dP : dQ) * vrel;
+    if (smax < s) s = smax;
+    const double cc = cos_chi<double>(s, u1);
+    const double sc = sqrt(1.0 - cc * cc);
+    double se, ce;
+    sincos_2pi(u2, &se, &ce);
+    const double sce = sc * ce, scs = sc * se;
+    vec3<double> np;
+    const double pperp = sqrt(pPs.x * pPs.x + pPs.y * pPs.y);
+    if (pperp > 1.e-10 * pm) {
+        const double ip = 1.0 / pperp;
+        np = mk3<double>((pPs.x * pPs.z * sce - pPs.y * pm * scs) * ip + pPs.x * cc,
+                         (pPs.y * pPs.z * sce + pPs.x * pm * scs) * ip + pPs.y * cc, -pperp * sce + pPs.z * cc);
+    } else {
+        np = mk3<double>(pm * sce, pm * scs, pm * cc);
+    }
+    const vec3<double> newP = np + (gcs * dot(vc, np) + mP * gPs * gc) * vc;
+    const double aP = mag(newP) / (mP * kC);
+    const vec3<double> mnp = -1.0 * np;
+    const vec3<double> newQ = mnp + (gcs * dot(vc, mnp) + mQ * gQs * gc) * vc;
+    const double aQ = mag(newQ) / (mQ * kC);
+    UP = (1.0 / (mP * sqrt(1.0 + aP * aP))) * newP;
+    UQ = (1.0 / (mQ * sqrt(1.0 + aQ * aQ))) * newQ;
+}
+
+// one collision of parcel iP (species of d's P side) with parcel iQ; MODEL 0 Nanbu, 1 Perez
+template <class UT, int MODEL>
+__device__ __forceinline__ void collide(const CoulombArgs& a, const StageK& d, int cell, int iP, int iQ, int k,
+                                        double debye) {
+    uint4 o = rng4(a.seed, (uint32_t)cell, (uint32_t)a.step, ((uint32_t)d.stage << 24) | (uint32_t)k,
+                   kTagCoulomb << 24);
+    const double u1 = u01(o.x, o.y), u2 = u01(o.z, o.w);
+    UT* pxP = (UT*)d.u[0]; UT* pyP = (UT*)d.u[1]; UT* pzP = (UT*)d.u[2];
+    UT* pxQ = (UT*)d.u[3]; UT* pyQ = (UT*)d.u[4]; UT* pzQ = (UT*)d.u[5];
+    const vec3<UT> preP = mk3<UT>(pxP[iP], pyP[iP], pzP[iP]);
+    const vec3<UT> preQ = mk3<UT>(pxQ[iQ], pyQ[iQ], pzQ[iQ]);
+    vec3<UT> nP = preP, nQ = preQ;
+    if (MODEL == 1) {
+        vec3<double> dP = mk3<double>(preP.x, preP.y, preP.z), dQ = mk3<double>(preQ.x, preQ.y, preQ.z);
+        perez_scatter(dP, dQ, d, a.calculate_bmin, debye, a.dt, u1, u2);
+        nP = mk3<UT>((UT)dP.x, (UT)dP.y, (UT)dP.z);
+        nQ = mk3<UT>((UT)dQ.x, (UT)dQ.y, (UT)dQ.z);
+    } else {
+        nanbu_scatter<UT>(nP, nQ, d, a.calculate_bmin, debye, a.dt, u1, u2);
+    }
+    if (a.weight_correction) {
+        // NanbuWeightCorrectionModel::correctVelocity
+        o = rng4(a.seed, (uint32_t)cell, (uint32_t)a.step, ((uint32_t)d.stage << 24) | (uint32_t)k,
+                 (kTagCoulomb << 24) | 1u);
+        const double r = u01(o.x, o.y);
+        if (r > d.wP / d.wQ) nQ = preQ;
+        if (r > d.wQ / d.wP) nP = preP;
+    }
+    pxP[iP] = nP.x; pyP[iP] = nP.y; pzP[iP] = nP.z;
+    pxQ[iQ] = nQ.x; pyQ[iQ] = nQ.y; pzQ[iQ] = nQ.z;
 }
 
 // bitonic sort of npow2 64-bit keys with int payload in LDS by one wave
@@ -102,9 +255,9 @@ __device__ void bitonic_sort(uint64_t* key, int* val, int npow2) {
                 const int l = i ^ j;
                 if (l > i) {
                     const bool up = (i & k) == 0;
-                    const uint64_t a = key[i], b = key[l];
-                    if ((a > b) == up) {
-                        key[i] = b; key[l] = a;
+                    const uint64_t x = key[i], y = key[l];
+                    if ((x > y) == up) {
+                        key[i] = y; key[l] = x;
                         const int t = val[i]; val[i] = val[l]; val[l] = t;
                     }
                 }
@@ -114,26 +267,33 @@ __device__ void bitonic_sort(uint64_t* key, int* val, int npow2) {
     }
 }
 
-// the cell's parcels of one species ordered by the random key of (id, step, stage,
-// species).  Up to 256 parcels the order is found by ranking (each key counts the keys
-// below it, no barriers inside); larger cells use a bitonic sort.
+__device__ __forceinline__ uint64_t shuffle_key(uint64_t seed, int64_t step, int stage, int species, uint32_t pid) {
+    const uint4 o = rng4(seed, pid, (uint32_t)step, ((uint32_t)stage << 8) | (uint32_t)species, kTagShuffle << 24);
+    return ((uint64_t)o.x << 24) | (o.y >> 8);
+}
+
+// val[0..n) = array indices of the cell's parcels of one species ordered by their 56-bit
+// random key.  Up to 256 parcels the key carries the cell-local index in its low byte, so
+// keys are distinct and each parcel's place is the number of smaller keys (no barriers in
+// the count; eight keys per LDS round trip).  Larger cells use a bitonic sort.
 __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
                            uint64_t* key, int* val) {
     if (n <= 256) {
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            const uint32_t pid = id[first + k];
-            uint4 o = rng4(seed, pid, (uint32_t)step, ((uint32_t)stage << 8) | (uint32_t)species, kTagShuffle << 24);
-            key[k] = ((uint64_t)o.x << 32) | pid;
-        }
+        const int n8 = (n + 7) & ~7;
+        for (int k = threadIdx.x; k < n8; k += blockDim.x)
+            key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k : ~0ull;
         __syncthreads();
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            const uint64_t kk = key[k];
+        for (int k0 = 0; k0 < n; k0 += blockDim.x) {
+            const int k = k0 + (int)threadIdx.x;
+            const uint64_t kk = k < n ? key[k] : 0ull;
             int r = 0;
-            for (int j = 0; j < n; ++j) {
-                const uint64_t kj = key[j];
-                r += (kj < kk) || (kj == kk && j < k);
+            for (int j = 0; j < n8; j += 8) {
+                const uint64_t* q = key + j;
+                const uint64_t x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3], x4 = q[4], x5 = q[5], x6 = q[6], x7 = q[7];
+                r += (int)(x0 < kk) + (int)(x1 < kk) + (int)(x2 < kk) + (int)(x3 < kk) + (int)(x4 < kk) +
+                     (int)(x5 < kk) + (int)(x6 < kk) + (int)(x7 < kk);
             }
-            val[r] = first + k;
+            if (k < n) val[r] = first + k;
         }
         __syncthreads();
         return;
@@ -142,9 +302,7 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
     while (np2 < n) np2 <<= 1;
     for (int k = threadIdx.x; k < np2; k += blockDim.x) {
         if (k < n) {
-            const uint32_t pid = id[first + k];
-            uint4 o = rng4(seed, pid, (uint32_t)step, ((uint32_t)stage << 8) | (uint32_t)species, kTagShuffle << 24);
-            key[k] = ((uint64_t)o.x << 32) | pid;
+            key[k] = (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)(k & 255);
             val[k] = first + k;
         } else {
             key[k] = ~0ull;
@@ -155,21 +313,20 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
     bitonic_sort(key, val, np2);
 }
 
-__device__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
+__device__ __forceinline__ double dmin(double x, double y) { return x < y ? x : y; }
 
 }  // namespace
 
-template <class UT>
+template <class UT, int MODEL>
 __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
     extern __shared__ uint64_t lds[];
+    __shared__ StageK sk[2];
     const int cell = blockIdx.x;
     const int maxp = a.maxp;
+    const int kstride = maxp + 8;
     uint64_t* keyA = lds;
-    uint64_t* keyB = lds + maxp;
-    int* valA = (int*)(lds + 2 * maxp);
+    uint64_t* keyB = lds + kstride;
+    int* valA = (int*)(lds + 2 * kstride);
     int* valB = valA + maxp;
     const int ns = a.n_species;
     for (int s = 0; s < ns; ++s) {
@@ -210,28 +367,29 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
             }
             vs = wave_sum(vs); vx = wave_sum(vx); vy = wave_sum(vy); vz = wave_sum(vz);
             double nn = n * w;
-            double ch = a.charge[s];
-            double mvx = vx / nn, mvy = vy / nn, mvz = vz / nn;
-            double mvs = vs / nn;
+            const double ch = a.charge[s];
+            const double mvx = vx / nn, mvy = vy / nn, mvz = vz / nn;
+            const double mvs = vs / nn;
             nn /= Vc;
             if (nn > nmax) nmax = nn;
-            double T = a.mass[s] / (3.0 * kKB) * (mvs - (mvx * mvx + mvy * mvy + mvz * mvz));
+            const double T = a.mass[s] / (3.0 * kKB) * (mvs - (mvx * mvx + mvy * mvy + mvz * mvz));
             if (T > 0.0) deb += nn * ch * ch / kEps0 * 1.0 / (T * kKB);
             __syncthreads();
         }
         if (deb > 0.0) {
-            double rmin = pow(4.0 * kPi * nmax / 3.0, -1.0 / 3.0);
-            double d = sqrt(1.0 / deb);
-            debye = d > rmin ? d : rmin;
+            const double rmin = pow(4.0 * kPi * nmax / 3.0, -1.0 / 3.0);
+            const double dl = sqrt(1.0 / deb);
+            debye = dl > rmin ? dl : rmin;
         }
     }
     // stages in the reference order: species i ascending, partner j descending from the
     // last charged species down to i.  An intra-species stage (i, i) is followed by the
     // intra stage of the last charged species when that is the next charged species; the
-    // two touch disjoint parcels and run together on one set of lanes.
+    // two touch disjoint parcels and run together.
     int last = -1;
     for (int s = 0; s < ns; ++s)
         if (a.charge[s] != 0.0) last = s;
+    const bool ny = a.pairing == 1;
     unsigned long long ncoll = 0;
     int st = 0;
     bool skip_last_intra = false;
@@ -241,125 +399,139 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
             if (a.charge[j] == 0.0) continue;
             const int stage_now = st++;
             if (i == j && i == last && skip_last_intra) continue;
-            const int a0 = a.start[i][cell], na = a.start[i][cell + 1] - a0;
-            const int b0 = a.start[j][cell], nb = a.start[j][cell + 1] - b0;
-            const double mA = a.mass[i], qA = a.charge[i], mB = a.mass[j], qB = a.charge[j];
-            const double lnL = a.coulomb_log[i][j];
-            const double w = a.weight[i];
-            Vel<UT> VA{(UT*)a.ux[i], (UT*)a.uy[i], (UT*)a.uz[i]};
-            Vel<UT> VB{(UT*)a.ux[j], (UT*)a.uy[j], (UT*)a.uz[j]};
-            double u1, u2;
+            int nd = 0;        // stage tables filled
             if (i == j) {
-                // this intra stage, and the last species' intra stage when it comes next
                 int nxt = -1;
                 for (int s2 = i + 1; s2 < ns; ++s2)
                     if (a.charge[s2] != 0.0) { nxt = s2; break; }
                 const bool pair2 = (nxt >= 0) && (nxt == last);
                 if (pair2) skip_last_intra = true;
-                int sp[2] = {i, pair2 ? last : -1};
-                int sg[2] = {stage_now, stage_now + 1};
-                int nn[2] = {0, 0}, items[2] = {0, 0};
-                for (int u = 0; u < (pair2 ? 2 : 1); ++u) {
-                    const int spc = sp[u];
+                for (int v = 0; v < (pair2 ? 2 : 1); ++v) {
+                    const int spc = v == 0 ? i : last;
                     const int f0 = a.start[spc][cell];
                     const int cnt = a.start[spc][cell + 1] - f0;
                     if (cnt <= 1 || !a.allow_intra) continue;
-                    nn[u] = cnt;
-                    keyed_perm(a.seed, a.step, sg[u], spc, a.id[spc], f0, cnt, u == 0 ? keyA : keyB, u == 0 ? valA : valB);
-                    items[u] = (cnt % 2) ? 1 + (cnt - 3) / 2 : cnt / 2;
-                    ncoll += (cnt % 2) ? 3 + (cnt - 3) / 2 : cnt / 2;
-                }
-                const int total = items[0] + items[1];
-                for (int t = threadIdx.x; t < total; t += blockDim.x) {
-                    const int u = t < items[0] ? 0 : 1;
-                    const int it = u == 0 ? t : t - items[0];
-                    const int spc = sp[u], cnt = nn[u], stg = sg[u];
-                    const int* v = u == 0 ? valA : valB;
-                    Vel<UT> Vs{(UT*)a.ux[spc], (UT*)a.uy[spc], (UT*)a.uz[spc]};
-                    const double ms = a.mass[spc], qs = a.charge[spc];
-                    const double lnLs = a.coulomb_log[spc][spc];
-                    const double n = cnt * a.weight[spc] / Vc;
-                    if (cnt % 2) {
-                        const double n12 = (cnt + 3.0) * a.weight[spc] / Vc;
-                        if (it == 0) {
-                            // odd count: the first three collide cyclically (TakizukaAbePairing.C:115-139)
-                            const int P = v[0], Q = v[1], R = v[2];
-                            coll_rand(a.seed, a.step, cell, stg, 0, u1, u2);
-                            nanbu_collide(Vs, P, Vs, Q, ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n, n, n12, a.dt, u1, u2);
-                            coll_rand(a.seed, a.step, cell, stg, 1, u1, u2);
-                            nanbu_collide(Vs, Q, Vs, R, ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n, n, n12, a.dt, u1, u2);
-                            coll_rand(a.seed, a.step, cell, stg, 2, u1, u2);
-                            nanbu_collide(Vs, P, Vs, R, ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n, n, n12, a.dt, u1, u2);
+                    uint64_t* key = nd == 0 ? keyA : keyB;
+                    int* val = nd == 0 ? valA : valB;
+                    keyed_perm(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, cnt, key, val);
+                    if (threadIdx.x == 0) {
+                        StageK& d = sk[nd];
+                        d.u[0] = d.u[3] = a.ux[spc]; d.u[1] = d.u[4] = a.uy[spc]; d.u[2] = d.u[5] = a.uz[spc];
+                        d.permP = d.permQ = val;
+                        d.mP = d.mQ = a.mass[spc]; d.qP = d.qQ = a.charge[spc]; d.wP = d.wQ = a.weight[spc];
+                        d.lnL = a.coulomb_log[spc][spc];
+                        d.n = cnt;
+                        d.stage = stage_now + v;
+                        const double n = cnt * a.weight[spc] / Vc;
+                        d.nP = d.nQ = n;
+                        if (cnt % 2 == 0) {
+                            d.kind = kIntraEven; d.units = cnt / 2; d.nPQ = n;
+                        } else if (!ny) {
+                            d.kind = kIntraOddTA; d.units = 1 + (cnt - 3) / 2; d.nPQ = (cnt + 3.0) * a.weight[spc] / Vc;
                         } else {
-                            const int q = it - 1, x = 3 + 2 * q;
-                            coll_rand(a.seed, a.step, cell, stg, 3 + q, u1, u2);
-                            nanbu_collide(Vs, v[x], Vs, v[x + 1], ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n, n,
-                                          n12, a.dt, u1, u2);
+                            d.kind = kIntraOddNY; d.units = (cnt - 1) / 2;
+                            d.nPQ = 2.0 * a.weight[spc] * ((cnt - 1) / 2 + 1) / Vc;
                         }
-                    } else {
-                        coll_rand(a.seed, a.step, cell, stg, it, u1, u2);
-                        nanbu_collide(Vs, v[2 * it], Vs, v[2 * it + 1], ms, qs, ms, qs, lnLs, a.calculate_bmin, debye, n,
-                                      n, n, a.dt, u1, u2);
                     }
+                    ++nd;
                 }
             } else {
-                if (na == 0 || nb == 0) continue;
-                keyed_perm(a.seed, a.step, stage_now, i, a.id[i], a0, na, keyA, valA);
-                keyed_perm(a.seed, a.step, stage_now, j, a.id[j], b0, nb, keyB, valB);
-                if (na == nb) {
-                    const double nP = na * w / Vc, nQ = nb * w / Vc;
-                    for (int k = threadIdx.x; k < na; k += blockDim.x) {
-                        coll_rand(a.seed, a.step, cell, stage_now, k, u1, u2);
-                        nanbu_collide(VA, valA[k], VB, valB[na - k - 1], mA, qA, mB, qB, lnL, a.calculate_bmin, debye,
-                                      nP, nQ, nP, a.dt, u1, u2);
-                    }
-                    ncoll += na;
-                } else {
-                    const bool minIsA = na < nb;
-                    const int nMin = minIsA ? na : nb, nMax = minIsA ? nb : na;
-                    const int* pmin = minIsA ? valA : valB;
-                    const int* pmax = minIsA ? valB : valA;
-                    const int smin = minIsA ? i : j, smax = minIsA ? j : i;
-                    Vel<UT> Vmin = minIsA ? VA : VB, Vmax = minIsA ? VB : VA;
-                    const double nMinS = nMin * w / Vc, nMaxS = nMax * w / Vc;
-                    const int delta = nMax / nMin;
-                    const double delta_r = (double)(nMax % nMin) / nMin;
-                    const int nMinG1 = (int)(delta_r * nMin);
-                    const int nMaxG1 = (int)((delta + 1) * delta_r * nMin);
-                    const int nMinG2 = (int)((1.0 - delta_r) * nMin);
-                    const int nMaxG2 = (int)(delta * (1.0 - delta_r) * nMin);
-                    const double mMin = a.mass[smin], qMin = a.charge[smin], mMax = a.mass[smax], qMax = a.charge[smax];
-                    const double lnLp = a.coulomb_log[smax][smin];
-                    // group 1: min parcel m collides with max parcels m, m+nMinG1, ... (in order)
-                    if (nMinG1 > 0)
-                        for (int m = threadIdx.x; m < nMinG1; m += blockDim.x)
-                            for (int g1 = m; g1 < nMaxG1; g1 += nMinG1) {
-                                coll_rand(a.seed, a.step, cell, stage_now, g1, u1, u2);
-                                nanbu_collide(Vmax, pmax[g1], Vmin, pmin[m], mMax, qMax, mMin, qMin, lnLp,
-                                              a.calculate_bmin, debye, nMaxS, nMinS, nMaxS, a.dt, u1, u2);
+                const int a0 = a.start[i][cell], na = a.start[i][cell + 1] - a0;
+                const int b0 = a.start[j][cell], nb = a.start[j][cell + 1] - b0;
+                if (na > 0 && nb > 0) {
+                    keyed_perm(a.seed, a.step, stage_now, i, a.id[i], a0, na, keyA, valA);
+                    keyed_perm(a.seed, a.step, stage_now, j, a.id[j], b0, nb, keyB, valB);
+                    if (threadIdx.x == 0) {
+                        StageK& d = sk[0];
+                        d.stage = stage_now;
+                        const double wi = a.weight[i], wj = a.weight[j];
+                        if (na == nb) {
+                            d.kind = kInterEqual; d.n = na; d.units = na;
+                            d.u[0] = a.ux[i]; d.u[1] = a.uy[i]; d.u[2] = a.uz[i];
+                            d.u[3] = a.ux[j]; d.u[4] = a.uy[j]; d.u[5] = a.uz[j];
+                            d.permP = valA; d.permQ = valB;
+                            d.mP = a.mass[i]; d.qP = a.charge[i]; d.wP = wi;
+                            d.mQ = a.mass[j]; d.qQ = a.charge[j]; d.wQ = wj;
+                            d.lnL = a.coulomb_log[i][j];
+                            d.nP = na * wi / Vc; d.nQ = nb * wj / Vc;
+                            d.nPQ = ny ? na * dmin(wi, wj) / Vc : d.nP;
+                        } else {
+                            const bool minIsA = na < nb;
+                            const int nMin = minIsA ? na : nb, nMax = minIsA ? nb : na;
+                            const int smin = minIsA ? i : j, smax = minIsA ? j : i;
+                            d.kind = kCyclic; d.n = nMax;
+                            d.u[0] = a.ux[smax]; d.u[1] = a.uy[smax]; d.u[2] = a.uz[smax];
+                            d.u[3] = a.ux[smin]; d.u[4] = a.uy[smin]; d.u[5] = a.uz[smin];
+                            d.permP = minIsA ? valB : valA;
+                            d.permQ = minIsA ? valA : valB;
+                            d.mP = a.mass[smax]; d.qP = a.charge[smax]; d.wP = a.weight[smax];
+                            d.mQ = a.mass[smin]; d.qQ = a.charge[smin]; d.wQ = a.weight[smin];
+                            d.lnL = a.coulomb_log[smax][smin];
+                            d.nP = nMax * a.weight[smax] / Vc;
+                            d.nQ = nMin * a.weight[smin] / Vc;
+                            if (ny) {
+                                // NanbuYonemuraPairing.C:253-300
+                                d.nPQ = nMax * dmin(wi, wj) / Vc;
+                                d.g1_units = nMin; d.g1_lmax = nMax; d.g1_stride = nMin;
+                                d.g1_xo = 0; d.g1_yo = 0; d.g1_ko = 0;
+                                d.units = nMin;
+                                d.g2_lmax = 0; d.g2_stride = 1; d.g2_xo = 0; d.g2_yo = 0; d.g2_ko = 0;
+                            } else {
+                                // TakizukaAbePairing.C:197-308: two groups
+                                d.nPQ = d.nP;
+                                const int delta = nMax / nMin;
+                                const double delta_r = (double)(nMax % nMin) / nMin;
+                                const int nMinG1 = (int)(delta_r * nMin);
+                                const int nMaxG1 = (int)((delta + 1) * delta_r * nMin);
+                                const int nMinG2 = (int)((1.0 - delta_r) * nMin);
+                                const int nMaxG2 = (int)(delta * (1.0 - delta_r) * nMin);
+                                const int cyc = nMinG2 > 0 ? nMinG2 : 1;
+                                d.g1_units = nMinG1 > 0 ? nMinG1 : 0;
+                                d.g1_lmax = nMaxG1; d.g1_stride = nMinG1 > 0 ? nMinG1 : 1;
+                                d.g1_xo = 0; d.g1_yo = 0; d.g1_ko = 0;
+                                d.g2_lmax = nMaxG2; d.g2_stride = cyc;
+                                d.g2_xo = nMaxG1; d.g2_yo = nMinG1; d.g2_ko = nMaxG1;
+                                d.units = d.g1_units + (nMaxG2 > 0 ? cyc : 0);
                             }
-                    // group 2: min parcels nMinG1.. cycle over max parcels nMaxG1..
-                    const int cyc = nMinG2 > 0 ? nMinG2 : 1;
-                    for (int m = threadIdx.x; m < cyc; m += blockDim.x)
-                        for (int g2 = m; g2 < nMaxG2; g2 += cyc) {
-                            coll_rand(a.seed, a.step, cell, stage_now, nMaxG1 + g2, u1, u2);
-                            nanbu_collide(Vmax, pmax[nMaxG1 + g2], Vmin, pmin[nMinG1 + m], mMax, qMax, mMin, qMin, lnLp,
-                                          a.calculate_bmin, debye, nMaxS, nMinS, nMaxS, a.dt, u1, u2);
                         }
-                    ncoll += (unsigned long long)(nMaxG1 > 0 && nMinG1 > 0 ? nMaxG1 : 0) + (unsigned long long)(nMaxG2 > 0 ? nMaxG2 : 0);
+                    }
+                    nd = 1;
+                }
+            }
+            if (nd == 0) continue;
+            __syncthreads();
+            const int units0 = sk[0].units;
+            const int total = units0 + (nd > 1 ? sk[1].units : 0);
+            for (int u = threadIdx.x; u < total; u += blockDim.x) {
+                const StageK& d = sk[u < units0 ? 0 : 1];
+                const int uu = u < units0 ? u : u - units0;
+                int iP, iQ, k;
+                for (int t = 0; decode(d, uu, t, iP, iQ, k); ++t) {
+                    collide<UT, MODEL>(a, d, cell, iP, iQ, k, debye);
+                    ++ncoll;
                 }
             }
             __syncthreads();
         }
     }
+    for (int off = 32; off > 0; off >>= 1) ncoll += __shfl_xor(ncoll, off);
     if (threadIdx.x == 0 && ncoll) atomicAdd(a.ncoll, ncoll);
+}
+
+size_t coulomb_lds_bytes(int maxp) {
+    return (size_t)(2 * (maxp + 8)) * sizeof(uint64_t) + (size_t)(2 * maxp) * sizeof(int);
 }
 
 hipError_t launch_coulomb(const CoulombArgs& a, bool fp32_velocity, hipStream_t s) {
     if (a.n_cells <= 0) return hipSuccess;
-    const size_t lds = (size_t)a.maxp * (2 * sizeof(uint64_t) + 2 * sizeof(int));
-    if (fp32_velocity) hipLaunchKernelGGL(k_coulomb<float>, dim3(a.n_cells), dim3(64), lds, s, a);
-    else hipLaunchKernelGGL(k_coulomb<double>, dim3(a.n_cells), dim3(64), lds, s, a);
+    const size_t lds = coulomb_lds_bytes(a.maxp);
+    if (a.model == 1) {
+        if (fp32_velocity) hipLaunchKernelGGL((k_coulomb<float, 1>), dim3(a.n_cells), dim3(64), lds, s, a);
+        else hipLaunchKernelGGL((k_coulomb<double, 1>), dim3(a.n_cells), dim3(64), lds, s, a);
+    } else {
+        if (fp32_velocity) hipLaunchKernelGGL((k_coulomb<float, 0>), dim3(a.n_cells), dim3(64), lds, s, a);
+        else hipLaunchKernelGGL((k_coulomb<double, 0>), dim3(a.n_cells), dim3(64), lds, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -149,10 +149,14 @@ struct CoulombArgs {
     double dt;
     uint64_t seed;
     int64_t step;
-    int maxp;                         // LDS capacity per species per cell
+    int maxp;                         // LDS capacity per species per cell (power of two >= 64)
+    int model;                        // 0 Nanbu, 1 Perez
+    int pairing;                      // 0 TakizukaAbe, 1 NanbuYonemura
+    int weight_correction;            // Nanbu weight correction active (unequal weights)
     uint32_t* flags;
     unsigned long long* ncoll;
 };
+size_t coulomb_lds_bytes(int maxp);
 hipError_t launch_coulomb(const CoulombArgs& a, bool fp32_velocity, hipStream_t s);
 
 // ---------------------------------------------------------------- MCC (k_mcc.hip)

@@ -56,3 +56,38 @@ def test_no_device_is_reported_not_faked():
     with pytest.raises(picmi.PicError) as ei:
         picmi.Cloud(m, sp, {"ParticlePusher": "Boris"})
     assert ei.value.code == -10
+
+
+def test_takizuka_abe_rejects_unequal_weights():
+    """TakizukaAbePairing.C:44-46: the pairing needs equal parcel weights; checked before any
+    device is touched."""
+    from picfoam_amd import picmi
+    m = small_hex(n=(2, 2, 2), L=(1, 1, 1))
+    sp = [{"mass": 9.1e-31, "charge": -1.6e-19, "nEquivalentParticles": 1.0, "capacity": 10},
+          {"mass": 1.7e-27, "charge": 1.6e-19, "nEquivalentParticles": 2.0, "capacity": 10}]
+    with pytest.raises(picmi.PicError, match="TakizukaAbe"):
+        picmi.Cloud(m, sp, {"CoulombCollisionModel": "Nanbu", "PairingAlgorithm": "TakizukaAbe"})
+    with pytest.raises(picmi.PicError, match="TakizukaAbe"):
+        picmi.Cloud(m, sp, {"CoulombCollisionModel": "Perez"})
+
+
+@pytest.mark.parametrize("key,good", [("CoulombCollisionModel", "Perez"), ("PairingAlgorithm", "NanbuYonemura"),
+                                      ("WeightCorrectionModel", "Nanbu"),
+                                      ("ElectronNeutralCollisionModel", "RelativisticElectronNeutral")])
+def test_new_model_names_are_accepted(key, good):
+    """The selectors know every model name of the particle step; without a device the
+    selection passes and the context creation stops at the device check."""
+    from picfoam_amd import picmi
+    if picmi.device_count() > 0:
+        pytest.skip("a device is present")
+    m = small_hex(n=(2, 2, 2), L=(1, 1, 1))
+    sp = [{"mass": 9.1e-31, "charge": -1.6e-19, "nEquivalentParticles": 1.0, "capacity": 10},
+          {"mass": 1.7e-27, "charge": 1.6e-19, "nEquivalentParticles": 1.0, "capacity": 10}]
+    models = {"CoulombCollisionModel": "Nanbu", key: good}
+    if key == "ElectronNeutralCollisionModel":
+        models.update({"BackgroundGasModel": "uniformBackground", "electronTypeId": 0, "ionTypeId": 1,
+                       "uniformBackground": {"mass": 6.6e-26, "numberDensity": 1e21, "temperature": 300.0,
+                                             "nEquivalentParticles": 1.0}})
+    with pytest.raises(picmi.PicError) as ei:
+        picmi.Cloud(m, sp, models)
+    assert ei.value.code == -10, str(ei.value)

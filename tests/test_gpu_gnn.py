@@ -126,3 +126,60 @@ def test_mfma_layer_matches_bf16_matmul(oracle):
     loss_ref = e @ A @ e / norm
     say(f"one-layer loss gpu {loss:.6f} oracle {loss_ref:.6f}")
     assert abs(loss - loss_ref) / loss_ref < 2e-2
+
+
+def test_gnn_residual_loss_gradient_matches_oracle(oracle):
+    """Residual-norm training loss |A e|^2 / |A d|^2: value and gradient against the oracle
+    network driven with dloss/dout = 2 A A e / |A d|^2 (the Laplacian is symmetric)."""
+    say("gnn residual-loss gradient vs oracle")
+    c, dt = _plasma_cloud(n=(8, 6, 4), ppc=20)
+    c.gnn_init(layers=3, width=64, seed=3)
+    c.gnn_loss(1)
+    c.gnn_record(4)
+    for _ in range(3):
+        c.step(dt)
+    x, t, _ = c.gnn_snapshot(2)
+    rng = np.random.default_rng(1)
+    theta = c.gnn_params().astype(np.float64) + rng.normal(size=c.gnn_np) * 0.05
+    c.gnn_params(theta.astype(np.float32))
+    theta = c.gnn_params().astype(np.float64)
+    g_gpu, loss_gpu = c.gnn_grad(2)
+    nb, A = _graph_from_cloud(c)
+    t64 = t.astype(np.float64)
+    out, _ = oracle.gnn_forward_backward(nb, x.astype(np.float64), theta, 64, 3)
+    e = out - t64
+    norm_r = np.sum((A @ t64) ** 2)
+    loss_ref = np.sum((A @ e) ** 2) / norm_r
+    dout = 2 * (A @ (A @ e)) / norm_r
+    _, g_ref = oracle.gnn_forward_backward(nb, x.astype(np.float64), theta, 64, 3, dout)
+    cos = g_gpu @ g_ref / (np.linalg.norm(g_gpu) * np.linalg.norm(g_ref))
+    rel = np.linalg.norm(g_gpu - g_ref) / np.linalg.norm(g_ref)
+    say(f"residual loss gpu {loss_gpu:.6f} oracle {loss_ref:.6f}; gradient cosine {cos:.5f}, rel err {rel:.3e}")
+    assert abs(loss_gpu - loss_ref) / loss_ref < 2e-2
+    assert cos > 0.995 and rel < 0.1
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_gnn_training_other_losses(kind):
+    """Residual-norm (1) and combined (2) losses train down and the guess does not cost
+    PCG iterations."""
+    say(f"gnn training, loss kind {kind}")
+    c, dt = _plasma_cloud()
+    c.gnn_init(layers=3, width=64, seed=5)
+    c.gnn_loss(kind)
+    c.gnn_record(64)
+    iters_plain = []
+    for k in range(80):
+        st = c.step(dt)
+        if k >= 16:
+            iters_plain.append(st.poisson.iterations)
+    l0 = np.mean([c.gnn_eval(i) for i in range(8)])
+    first, last = c.gnn_train(600, lr=2e-3, seed=2)
+    l1 = np.mean([c.gnn_eval(i) for i in range(8)])
+    c.gnn_record(0)
+    c.gnn_use(True)
+    iters_gnn = [c.step(dt).poisson.iterations for _ in range(40)]
+    say(f"loss kind {kind}: before {l0:.4f} after {l1:.4f}; PCG iterations warm start {np.mean(iters_plain):.2f}, "
+        f"learned guess {np.mean(iters_gnn):.2f}")
+    assert l1 < l0
+    assert np.mean(iters_gnn) <= np.mean(iters_plain) + 0.5

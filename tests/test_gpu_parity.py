@@ -163,9 +163,21 @@ def _two_species_cell_setup(rng, m, rm, ne_per, ni_per, Te=10.0, Ti=1.0, mi=1836
     return make(ne_per, Te, ME, 0), make(ni_per, Ti, mi, 0)
 
 
+COULOMB_VARIANTS = [
+    # (collision model, pairing, ion weight / electron weight, weight correction)
+    ("Nanbu", "TakizukaAbe", 1.0, "none"),
+    ("Nanbu", "NanbuYonemura", 1.0, "none"),
+    ("Perez", "TakizukaAbe", 1.0, "none"),
+    ("Perez", "NanbuYonemura", 3.0, "Nanbu"),
+    ("Nanbu", "NanbuYonemura", 0.25, "Nanbu"),
+]
+
+
+@pytest.mark.parametrize("variant", COULOMB_VARIANTS, ids=lambda v: "-".join(str(x) for x in v))
 @pytest.mark.parametrize("counts", ["equal", "ragged"])
-def test_coulomb_fp64_parity(oracle, counts):
-    say("coulomb parity", counts)
+def test_coulomb_fp64_parity(oracle, counts, variant):
+    model, pairing, wratio, wcorr = variant
+    say("coulomb parity", counts, variant)
     rng = np.random.default_rng(104)
     m = small_hex(n=(3, 2, 2), L=(1e-4, 1e-4, 1e-4))
     rm = oracle.RefMesh(m)
@@ -177,14 +189,17 @@ def test_coulomb_fp64_parity(oracle, counts):
         ne[0], ni[0] = 7, 3
         ne[1], ni[1] = 1, 0
         ne[2], ni[2] = 33, 33
+        ne[3], ni[3] = 300, 5      # above 256 parcels: the sorted (not ranked) shuffle
+        ne[4], ni[4] = 3, 11
     mi = 1836 * ME
     w = 1e6
     species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": w, "capacity": 20000},
-               {"mass": mi, "charge": QE, "nEquivalentParticles": w, "capacity": 20000}]
+               {"mass": mi, "charge": QE, "nEquivalentParticles": w * wratio, "capacity": 20000}]
+    sp = [(ME, -QE, w), (mi, QE, w * wratio)]
     for lnl in (10.0, 0.0):
         cl = {} if lnl == 0 else {(0, 0): lnl, (0, 1): lnl, (1, 1): lnl}
-        c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu", "coulombLog": cl, "seed": 77},
-                  species)
+        c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": model, "PairingAlgorithm": pairing,
+                      "WeightCorrectionModel": wcorr, "coulombLog": cl, "seed": 77}, species)
         (ye, te, Ue, ie), (yi, ti, Ui, ii) = _two_species_cell_setup(rng, m, rm, ne, ni, mi=mi)
         c.set_particles(0, ye, te, Ue, ie)
         c.set_particles(1, yi, ti, Ui, ii)
@@ -199,8 +214,8 @@ def test_coulomb_fp64_parity(oracle, counts):
         clm = np.zeros((8, 8))
         if lnl:
             clm[:2, :2] = lnl
-        Us, st = oracle.coulomb(rm, [(ME, -QE, w), (mi, QE, w)], [Ue, Ui], [ie, ii], [cse, csi], dt, seed=77,
-                                step=5, coulomb_log=clm)
+        Us, st = oracle.coulomb(rm, sp, [Ue, Ui], [ie, ii], [cse, csi], dt, seed=77, step=5, coulomb_log=clm,
+                                model=model, pairing=pairing, weight_correction=(wcorr == "Nanbu" and wratio != 1))
         (Ueg_s,) = sort_by_id(ieg, Ueg)
         (Uig_s,) = sort_by_id(iig, Uig)
         assert rel_err(Ueg_s, Us[0]) < TOL

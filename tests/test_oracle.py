@@ -338,8 +338,9 @@ def _thermal_cell_setup(rng, n_e, n_i, Te_eV=10.0, Ti_eV=1.0, mi=1836 * ME):
     return Ue, Ui
 
 
+@pytest.mark.parametrize("pairing", ["TakizukaAbe", "NanbuYonemura"])
 @pytest.mark.parametrize("counts", [(64, 64), (33, 33), (40, 17), (17, 40), (7, 3)])
-def test_coulomb_conserves_momentum_energy(oracle, counts):
+def test_coulomb_conserves_momentum_energy(oracle, counts, pairing):
     rng = np.random.default_rng(9)
     m = small_hex(n=(1, 1, 1), L=(1e-4, 1e-4, 1e-4))
     rm = oracle.RefMesh(m)
@@ -350,7 +351,8 @@ def test_coulomb_conserves_momentum_energy(oracle, counts):
     sp = [(ME, -QE, w), (mi, QE, w)]
     cl = np.zeros((8, 8)); cl[:2, :2] = 10.0
     Us, st = oracle.coulomb(rm, sp, [Ue, Ui], [np.arange(ne), np.arange(ni) + 1000],
-                            [np.array([0, ne]), np.array([0, ni])], 1e-9, seed=5, step=0, coulomb_log=cl)
+                            [np.array([0, ne]), np.array([0, ni])], 1e-9, seed=5, step=0, coulomb_log=cl,
+                            pairing=pairing)
     p0 = ME * Ue.sum(0) + mi * Ui.sum(0)
     p1 = ME * Us[0].sum(0) + mi * Us[1].sum(0)
     e0 = ME * (Ue ** 2).sum() + mi * (Ui ** 2).sum()
@@ -365,7 +367,7 @@ def test_coulomb_conserves_momentum_energy(oracle, counts):
 def test_coulomb_mean_cos_chi(oracle):
     """Nanbu's angular distribution has <cos chi> = exp(-s).  Two equal-mass species of
     opposite charge stream through each other at relative speed g, so every inter-species
-    pair has |g| fixed; intra-species pairs have zero relative velocity and do nothing."""
+    pair has |g| fixed; intra-species collisions are switched off."""
     m = small_hex(n=(1, 1, 1), L=(1e-4, 1e-4, 1e-4))
     rm = oracle.RefMesh(m)
     n = 4000
@@ -384,11 +386,154 @@ def test_coulomb_mean_cos_chi(oracle):
     for step in range(5):
         Us, st = oracle.coulomb(rm, [(ME, -QE, w), (ME, QE, w)], [UA, UB],
                                 [np.arange(n, dtype=np.uint32), np.arange(n, dtype=np.uint32)],
-                                [np.array([0, n]), np.array([0, n])], dt, seed=7, step=step, coulomb_log=cl)
+                                [np.array([0, n]), np.array([0, n])], dt, seed=7, step=step, coulomb_log=cl,
+                                allow_intra=False)
         dv2 = ((Us[0] - UA) ** 2).sum(1)
         coss.append(1 - 2 * dv2 / g ** 2)
     mean = np.mean(coss)
     assert abs(mean - np.exp(-s)) < 0.01
+
+
+def _one_cell(oracle):
+    return oracle.RefMesh(small_hex(n=(1, 1, 1), L=(1e-4, 1e-4, 1e-4)))
+
+
+@pytest.mark.parametrize("pairing", ["TakizukaAbe", "NanbuYonemura"])
+def test_perez_conserves_relativistic_momentum_energy(oracle, pairing):
+    """Perez scattering is a rotation of P's momentum in the centre-of-momentum frame:
+    sum m gamma v and sum m gamma c^2 are conserved per collision (equal weights)."""
+    rng = np.random.default_rng(21)
+    rm = _one_cell(oracle)
+    ne, ni = 23, 30
+    mi = 4 * ME
+    # mildly relativistic: |v| up to ~0.6 c
+    Ue = rng.normal(size=(ne, 3)) * 0.3 * C_LIGHT / np.sqrt(3)
+    Ui = rng.normal(size=(ni, 3)) * 0.2 * C_LIGHT / np.sqrt(3)
+    for U in (Ue, Ui):
+        sp = np.linalg.norm(U, axis=1)
+        U[sp > 0.9 * C_LIGHT] *= 0.5
+    w = 1e3
+    cl = np.zeros((8, 8)); cl[:2, :2] = 10.0
+    Us, st = oracle.coulomb(rm, [(ME, -QE, w), (mi, QE, w)], [Ue, Ui], [np.arange(ne), np.arange(ni) + 500],
+                            [np.array([0, ne]), np.array([0, ni])], 1e-12, seed=3, step=1, coulomb_log=cl,
+                            model="Perez", pairing=pairing)
+
+    def gam(U):
+        return 1.0 / np.sqrt(1.0 - (np.linalg.norm(U, axis=1) / C_LIGHT) ** 2)
+
+    def mom(Ua, Ub):
+        return ME * (gam(Ua)[:, None] * Ua).sum(0) + mi * (gam(Ub)[:, None] * Ub).sum(0)
+
+    def ene(Ua, Ub):
+        return ME * gam(Ua).sum() + mi * gam(Ub).sum()
+
+    scale = ME * (gam(Ue)[:, None] * np.abs(Ue)).sum()
+    assert np.allclose(mom(*Us), mom(Ue, Ui), rtol=0, atol=1e-12 * scale)
+    assert abs(ene(*Us) - ene(Ue, Ui)) / ene(Ue, Ui) < 1e-13
+    assert st[0] > 0 and np.all(np.isfinite(Us[0])) and np.all(np.isfinite(Us[1]))
+    assert np.all(np.any(Us[0] != Ue, axis=1))
+
+
+def _rotation_cos(Ua, Ub, Va, Vb):
+    g0 = Ua - Ub
+    g1 = Va - Vb
+    return (g0 * g1).sum(1) / (np.linalg.norm(g0, axis=1) * np.linalg.norm(g1, axis=1))
+
+
+def test_perez_reduces_to_nanbu_at_low_speed(oracle):
+    """At |v| << c the Perez centre-of-momentum rotation angle equals Nanbu's for the same
+    uniforms: both give cos(theta) = cosChi(s, U) with the same s (Perez's s tends to
+    lnL/(4 pi) (qq/(eps0 mu))^2 n dt / g^3).  One e-i pair per call, many calls."""
+    rm = _one_cell(oracle)
+    rng = np.random.default_rng(5)
+    w = 1e2
+    cl = np.zeros((8, 8)); cl[:2, :2] = 10.0
+    mi = 3 * ME
+    diffs, svals = [], []
+    for k in range(40):
+        Ue = rng.normal(size=(1, 3)) * 3e4
+        Ui = rng.normal(size=(1, 3)) * 1e4
+        args = (rm, [(ME, -QE, w), (mi, QE, w)], [Ue, Ui], [np.array([k]), np.array([k + 100])],
+                [np.array([0, 1]), np.array([0, 1])])
+        g = np.linalg.norm(Ue - Ui)
+        mu = ME * mi / (ME + mi)
+        s_n = 10.0 / (4 * np.pi) * (QE * QE / (EPS0 * mu)) ** 2 * (w / 1e-12) * g ** -3
+        dt = 0.7 / s_n          # s = 0.7 inside the polynomial branch
+        Un, _ = oracle.coulomb(*args, dt, seed=9, step=k, coulomb_log=cl, model="Nanbu")
+        Up, _ = oracle.coulomb(*args, dt, seed=9, step=k, coulomb_log=cl, model="Perez")
+        cn = _rotation_cos(Ue, Ui, Un[0], Un[1])[0]
+        cp = _rotation_cos(Ue, Ui, Up[0], Up[1])[0]
+        diffs.append(abs(cn - cp))
+        svals.append(cn)
+    assert max(diffs) < 1e-6
+    assert np.std(svals) > 0.1      # the angles vary: U is really used
+
+
+def test_perez_mean_cos_chi(oracle):
+    """<cos chi> = exp(-s) for the Perez model in the non-relativistic limit (the streams of
+    test_coulomb_mean_cos_chi)."""
+    rm = _one_cell(oracle)
+    n = 3000
+    g = 1e6
+    UA = np.zeros((n, 3)); UA[:, 0] = g / 2; UA[:, 1] = 1e-3
+    UB = np.zeros((n, 3)); UB[:, 0] = -g / 2; UB[:, 1] = -1e-3
+    w = 1e2
+    cl = np.zeros((8, 8)); cl[:2, :2] = 10.0
+    dt = 1e-7
+    mu = ME / 2
+    s = 10.0 / (4 * np.pi) * (QE * QE / (EPS0 * mu)) ** 2 * (n * w / 1e-12) * g ** -3 * dt
+    coss = []
+    for step in range(4):
+        Us, _ = oracle.coulomb(rm, [(ME, -QE, w), (ME, QE, w)], [UA, UB],
+                               [np.arange(n, dtype=np.uint32), np.arange(n, dtype=np.uint32)],
+                               [np.array([0, n]), np.array([0, n])], dt, seed=8, step=step, coulomb_log=cl,
+                               model="Perez", allow_intra=False)
+        # equal masses: |dU_A| = |g - g'| / 2, so cos(theta) = 1 - 2 |dU_A|^2 / g^2 per pair
+        coss.append(1 - 2 * ((Us[0] - UA) ** 2).sum(1) / g ** 2)
+    assert abs(np.mean(coss) - np.exp(-s)) < 0.012
+
+
+@pytest.mark.parametrize("ne,ni", [(7, 0), (8, 0), (5, 9), (12, 12), (3, 11)])
+def test_nanbu_yonemura_collision_counts(oracle, ne, ni):
+    """Collisions per cell of NanbuYonemuraPairing.C:123-300: intra n even -> n/2, n odd ->
+    1 + n/2 (the extra last-with-first collision); inter -> max(n_alpha, n_beta)."""
+    rng = np.random.default_rng(ne * 100 + ni)
+    rm = _one_cell(oracle)
+    mi = 50 * ME
+    Ue, Ui = _thermal_cell_setup(rng, ne, ni, mi=mi)
+    cl = np.zeros((8, 8)); cl[:2, :2] = 10.0
+    Us, st = oracle.coulomb(rm, [(ME, -QE, 1e3), (mi, QE, 1e3)], [Ue, Ui], [np.arange(ne), np.arange(ni) + 99],
+                            [np.array([0, ne]), np.array([0, ni])], 1e-9, seed=2, step=0, coulomb_log=cl,
+                            pairing="NanbuYonemura")
+
+    def intra(n):
+        return 0 if n <= 1 else (n // 2 if n % 2 == 0 else 1 + n // 2)
+    expect = intra(ne) + intra(ni) + (max(ne, ni) if min(ne, ni) > 0 else 0)
+    assert st[0] == expect
+    # every parcel collided
+    assert np.all(np.any(Us[0] != Ue, axis=1))
+    if ni > 1 or (ni == 1 and ne > 0):
+        assert np.all(np.any(Us[1] != Ui, axis=1))
+
+
+def test_nanbu_weight_correction_statistics(oracle):
+    """NanbuCorrection.C:50-69: with w_P < w_Q the heavier-weighted parcel Q keeps its new
+    velocity with probability w_P / w_Q and P always does."""
+    rm = _one_cell(oracle)
+    rng = np.random.default_rng(17)
+    n = 4000
+    wA, wB = 1e3, 4e3
+    UA = rng.normal(size=(n, 3)) * 1e6
+    UB = rng.normal(size=(n, 3)) * 1e6
+    cl = np.zeros((8, 8)); cl[:2, :2] = 10.0
+    Us, st = oracle.coulomb(rm, [(ME, -QE, wA), (ME, QE, wB)], [UA, UB], [np.arange(n), np.arange(n)],
+                            [np.array([0, n]), np.array([0, n])], 1e-9, seed=4, step=0, coulomb_log=cl,
+                            pairing="NanbuYonemura", weight_correction=True, allow_intra=False)
+    changedA = np.mean(np.any(Us[0] != UA, axis=1))
+    changedB = np.mean(np.any(Us[1] != UB, axis=1))
+    assert st[0] == n
+    assert changedA == 1.0
+    assert abs(changedB - wA / wB) < 4 * np.sqrt(0.25 * 0.75 / n)
 
 
 # ------------------------------------------------------------------ MCC
