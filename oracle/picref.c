@@ -823,71 +823,91 @@ static void nanbu_collide(v3* UP, v3* UQ, double mP, double qP, double mQ, doubl
     if (stats) { stats[1] += lnL; stats[2] += s; }
 }
 
+/* Lorentz transformation to the centre-of-momentum frame of two parcels and back, the
+ * common part of PerezRelativisticCollision.C:84-196 and
+ * RelativisticElectronNeutralCollision.C:84-158 */
+typedef struct {
+    double gP, gQ, msum, gc, gcs, gPs, gQs, pm;
+    v3 vc, pPs;
+} com_frame;
+
+static void com_enter(v3 uP, v3 uQ, double mP, double mQ, com_frame* f) {
+    const double c = C_LIGHT, c2 = C_LIGHT * C_LIGHT;
+    double bP = vmag(uP) / c, bQ = vmag(uQ) / c;
+    f->gP = 1.0 / sqrt(1.0 - bP * bP);
+    f->gQ = 1.0 / sqrt(1.0 - bQ * bQ);
+    v3 pP = vscale(mP * f->gP, uP), pQ = vscale(mQ * f->gQ, uQ);
+    f->msum = mP * f->gP + mQ * f->gQ;
+    f->vc = vdiv(vadd(pP, pQ), f->msum);
+    double vc2 = vdot(f->vc, f->vc);
+    if (vc2 == 0.0) {
+        f->gc = 1.0; f->gcs = 0.5; f->pPs = pP; f->gPs = f->gP; f->gQs = f->gQ;
+    } else {
+        f->gc = 1.0 / sqrt(1.0 - vc2 / c2);
+        f->gcs = (f->gc - 1.0) / vc2;
+        f->pPs = vadd(pP, vscale((f->gcs * vdot(f->vc, uP) - f->gc) * mP * f->gP, f->vc));
+        f->gPs = (1.0 - vdot(f->vc, uP) / c2) * f->gP * f->gc;
+        f->gQs = (1.0 - vdot(f->vc, uQ) / c2) * f->gQ * f->gc;
+    }
+    f->pm = vmag(f->pPs);
+}
+
+/* rotate P's centre-of-momentum momentum by (chi, eps) and transform both parcels back */
+static void com_leave(const com_frame* f, double mP, double mQ, double cc, double sc, double eps, v3* UP, v3* UQ) {
+    const double c = C_LIGHT;
+    double sce = sc * cos(eps), scs = sc * sin(eps);
+    v3 p = f->pPs;
+    double pm = f->pm;
+    v3 np;
+    double pperp = sqrt(p.x * p.x + p.y * p.y);
+    if (pperp > 1.e-10 * pm) {
+        double ip = 1.0 / pperp;
+        np = V((p.x * p.z * sce - p.y * pm * scs) * ip + p.x * cc, (p.y * p.z * sce + p.x * pm * scs) * ip + p.y * cc,
+               -pperp * sce + p.z * cc);
+    } else {
+        np = V(pm * sce, pm * scs, pm * cc);
+    }
+    v3 newP = vadd(np, vscale(f->gcs * vdot(f->vc, np) + mP * f->gPs * f->gc, f->vc));
+    double aP = vmag(newP) / (mP * c);
+    double ngP = sqrt(1.0 + aP * aP);
+    v3 mnp = vscale(-1.0, np);
+    v3 newQ = vadd(mnp, vscale(f->gcs * vdot(f->vc, mnp) + mQ * f->gQs * f->gc, f->vc));
+    double aQ = vmag(newQ) / (mQ * c);
+    double ngQ = sqrt(1.0 + aQ * aQ);
+    *UP = vdiv(newP, mP * ngP);
+    *UQ = vdiv(newQ, mQ * ngQ);
+}
+
 /* PerezRelativisticCollision::collide, PerezRelativisticCollision.C:62-196: the scattering
  * is done on the momentum of P in the centre-of-momentum frame */
 static void perez_collide(v3* UP, v3* UQ, double mP, double qP, double mQ, double qQ, double lnL_fixed,
                           int calc_bmin, double debye, double nP, double nQ, double nPQ, double dt, double u1,
                           double u2, double* stats) {
-    const double c = C_LIGHT, c2 = C_LIGHT * C_LIGHT;
-    v3 uP = *UP, uQ = *UQ;
-    double bP = vmag(uP) / c, bQ = vmag(uQ) / c;
-    double gP = 1.0 / sqrt(1.0 - bP * bP), gQ = 1.0 / sqrt(1.0 - bQ * bQ);
-    v3 pP = vscale(mP * gP, uP), pQ = vscale(mQ * gQ, uQ);
-    double msum = mP * gP + mQ * gQ;
-    v3 vc = vdiv(vadd(pP, pQ), msum);
-    double vc2 = vdot(vc, vc);
-    double gc, gcs, gPs, gQs;
-    v3 pPs;
-    if (vc2 == 0.0) {
-        gc = 1.0; gcs = 0.5; pPs = pP; gPs = gP; gQs = gQ;
-    } else {
-        gc = 1.0 / sqrt(1.0 - vc2 / c2);
-        gcs = (gc - 1.0) / vc2;
-        pPs = vadd(pP, vscale((gcs * vdot(vc, uP) - gc) * mP * gP, vc));
-        gPs = (1.0 - vdot(vc, uP) / c2) * gP * gc;
-        gQs = (1.0 - vdot(vc, uQ) / c2) * gQ * gc;
-    }
-    double pm = vmag(pPs);
+    const double c2 = C_LIGHT * C_LIGHT;
+    com_frame f;
+    com_enter(*UP, *UQ, mP, mQ, &f);
+    double pm = f.pm;
     if (pm == 0.0) return;
     double lnL;
+    double t2 = 1.0 + mP * f.gPs * mQ * f.gQs * c2 / (pm * pm);
     if (lnL_fixed == 0.0) {
-        double b0 = qP * qQ / (4.0 * PI_D * EPS0 * c2) * gc / msum * (1.0 + mP * gPs * mQ * gQs * c2 / (pm * pm));
+        double b0 = qP * qQ / (4.0 * PI_D * EPS0 * c2) * f.gc / f.msum * t2;
         if (calc_bmin) { double hb = HPLANCK / (2.0 * pm); b0 = hb > fabs(b0) ? hb : fabs(b0); }
         double l = 0.5 * log(1.0 + (debye * debye) / (b0 * b0));
         lnL = l > 2.0 ? l : 2.0;
     } else lnL = lnL_fixed;
-    double s1 = gc * pm / msum;
-    double t2 = 1.0 + mP * gPs * mQ * gQs * c2 / (pm * pm);
+    double s1 = f.gc * pm / f.msum;
     double s2 = t2 * t2;
     double s = (nP * nQ / nPQ) * dt * lnL * (qP * qQ) * (qP * qQ) /
-               (4.0 * PI_D * EPS0 * EPS0 * (c2 * c2) * mP * gP * mQ * gQ) * s1 * s2;
-    double vrel = pm * msum / (mP * gPs * mQ * gQs * gc);
+               (4.0 * PI_D * EPS0 * EPS0 * (c2 * c2) * mP * f.gP * mQ * f.gQ) * s1 * s2;
+    double vrel = pm * f.msum / (mP * f.gPs * mQ * f.gQs * f.gc);
     double dmax = mP * pow(nP, 2.0 / 3.0), dq = mQ * pow(nQ, 2.0 / 3.0);
     if (dq > dmax) dmax = dq;
     double smax = pow(4.0 * PI_D / 3.0, 1.0 / 3.0) * (nP * nQ / nPQ) * dt * (mP + mQ) / dmax * vrel;
     if (smax < s) s = smax;
     double cc = cos_chi(s, u1);
     double sc = sqrt(1.0 - cc * cc);
-    double eps = 2.0 * PI_D * u2;
-    double sce = sc * cos(eps), scs = sc * sin(eps);
-    v3 np;
-    double pperp = sqrt(pPs.x * pPs.x + pPs.y * pPs.y);
-    if (pperp > 1.e-10 * pm) {
-        double ip = 1.0 / pperp;
-        np = V((pPs.x * pPs.z * sce - pPs.y * pm * scs) * ip + pPs.x * cc,
-               (pPs.y * pPs.z * sce + pPs.x * pm * scs) * ip + pPs.y * cc, -pperp * sce + pPs.z * cc);
-    } else {
-        np = V(pm * sce, pm * scs, pm * cc);
-    }
-    v3 newP = vadd(np, vscale(gcs * vdot(vc, np) + mP * gPs * gc, vc));
-    double aP = vmag(newP) / (mP * c);
-    double ngP = sqrt(1.0 + aP * aP);
-    v3 mnp = vscale(-1.0, np);
-    v3 newQ = vadd(mnp, vscale(gcs * vdot(vc, mnp) + mQ * gQs * gc, vc));
-    double aQ = vmag(newQ) / (mQ * c);
-    double ngQ = sqrt(1.0 + aQ * aQ);
-    *UP = vdiv(newP, mP * ngP);
-    *UQ = vdiv(newQ, mQ * ngQ);
+    com_leave(&f, mP, mQ, cc, sc, 2.0 * PI_D * u2, UP, UQ);
     if (stats) { stats[1] += lnL; stats[2] += s; }
 }
 
@@ -1116,6 +1136,38 @@ static void nanbu_en_scatter(double eV, v3* UE, v3* Un, double mE, double mN, do
     *Un = vadd(*Un, vscale(mE / (mN + mE), dv));
 }
 
+/* RelativisticElectronNeutralCollision::updateVelocity with a sampled partner velocity
+ * (RelativisticElectronNeutralCollision.C:161-240).  cos(chi) is clamped to [-1, 1] as in
+ * the two-parcel variant (:120-126); the sampled variant would produce a NaN there. */
+static void rel_en_scatter(double eV, v3* UE, v3* Un, double mE, double mN, double ueps, double uchi) {
+    com_frame f;
+    com_enter(*UE, *Un, mE, mN, &f);
+    if (f.pm == 0.0) return;
+    double cc, sc;
+    if (eV < 1e-30) { cc = 1.0; sc = 0.0; }
+    else {
+        cc = 1.0 / eV * (eV + 2. - 2. * pow(1. + eV, uchi));
+        if (cc * cc > 1.0) { cc = 1.0; sc = 0.0; }
+        else sc = sqrt(1.0 - cc * cc);
+    }
+    com_leave(&f, mE, mN, cc, sc, 2.0 * PI_D * ueps, UE, Un);
+}
+
+static void en_scatter(const ref_mcc_params* p, double eV, v3* UE, v3* Un, double us, double uc) {
+    if (p->relativistic_scatter) rel_en_scatter(eV, UE, Un, p->massE, p->massN, us, uc);
+    else nanbu_en_scatter(eV, UE, Un, p->massE, p->massN, us, uc);
+}
+
+/* one electron-neutral scattering on (UE, Un) in place: Nanbu (relativistic = 0) or the
+ * relativistic centre-of-momentum variant (1) */
+void ref_en_scatter(int relativistic, double eV, double* UE, double* Un, double mE, double mN, double ueps,
+                    double uchi) {
+    v3 e = ld3(UE), n = ld3(Un);
+    if (relativistic) rel_en_scatter(eV, &e, &n, mE, mN, ueps, uchi);
+    else nanbu_en_scatter(eV, &e, &n, mE, mN, ueps, uchi);
+    st3(UE, e); st3(Un, n);
+}
+
 static inline void mcc_rand(uint64_t seed, int64_t step, int cell, int k, int j, double* a, double* b) {
     uint32_t o[4];
     rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)k, (TAG_MCC << 24) | (uint32_t)j, o);
@@ -1183,12 +1235,12 @@ int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, d
             double us, uc;
             mcc_rand(p->seed, p->step, cell, k, 3, &us, &uc);
             if (uacc <= Qel / sigmaT) {
-                nanbu_en_scatter(eV, &Ue, &Un, p->massE, p->massN, us, uc);
+                en_scatter(p, eV, &Ue, &Un, us, uc);
                 if (counts) counts[1]++;
             } else if (eV >= p->th_ex && uacc <= (Qel + Qex) / sigmaT) {
                 Ue = vscale(sqrt(1.0 - p->th_ex / eV), Ue);
                 eV -= p->th_ex;
-                nanbu_en_scatter(eV, &Ue, &Un, p->massE, p->massN, us, uc);
+                en_scatter(p, eV, &Ue, &Un, us, uc);
                 if (counts) counts[2]++;
             } else if (eV >= p->th_iz && uacc <= Qt / sigmaT) {
                 double uE, uR, up2, uc2;
@@ -1201,7 +1253,7 @@ int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, d
                 v3 preUe = Ue, preUn = Un;
                 Ue = vscale(sqrt(1.0 - (p->th_iz + Ep) / eV), Ue);
                 eV -= (p->th_iz + Ep);
-                nanbu_en_scatter(eV, &Ue, &Un, p->massE, p->massN, us, uc);
+                en_scatter(p, eV, &Ue, &Un, us, uc);
                 int created = 0;
                 if (uR <= wEwN && p->create_ion) {
                     if (*n_i >= cap_i) { err = -1; }
@@ -1225,7 +1277,7 @@ int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, d
                         tetE[ne] = tetE[ei];
                         v3 Unew = vscale(sqrt(2.0 * Ep * QE / p->massE), vdiv(preUe, vmag(preUe)));
                         v3 Un2 = preUn;
-                        nanbu_en_scatter(Ep, &Unew, &Un2, p->massE, p->massN, up2, uc2);
+                        en_scatter(p, Ep, &Unew, &Un2, up2, uc2);
                         st3(UE + 3 * ne, Unew);
                         uint32_t o[4];
                         rng4(p->seed, idE[ei], (uint32_t)p->step, (uint32_t)k, (TAG_NEWID << 24) | 2u, o);

@@ -119,6 +119,12 @@ int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, d
             uint32_t* idE, const int* cell_start, int* n_e, int cap_e, double* yI, double* UI, int* tetI,
             uint32_t* idI, int* n_i, int cap_i, double* sigma_cr_max, double* remainder, int64_t* counts);
 
+/* electron-neutral scattering with a sampled partner velocity, in place:
+ * NanbuElectronNeutralCollision.C:98-130 (relativistic 0) or
+ * RelativisticElectronNeutralCollision.C:161-240 (relativistic 1) */
+void ref_en_scatter(int relativistic, double eV, double* UE, double* Un, double mE, double mN, double ueps,
+                    double uchi);
+
 /* ListCrossSection interpolateXY (ListCrossSection.C:32-92) */
 double ref_interpolate_xy(double x, int n, const double* xs, const double* ys);
 

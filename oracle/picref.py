@@ -66,6 +66,8 @@ def lib():
         _lib.ref_mcc.restype = C.c_int
         _lib.ref_interpolate_xy.argtypes = [C.c_double, C.c_int, vp, vp]
         _lib.ref_interpolate_xy.restype = C.c_double
+        _lib.ref_en_scatter.argtypes = [C.c_int, C.c_double, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double]
+        _lib.ref_en_scatter.restype = None
         _lib.ref_poisson_create.restype = vp
         _lib.ref_poisson_create.argtypes = [vp, vp, vp]
         _lib.ref_poisson_destroy.argtypes = [vp]
@@ -245,6 +247,13 @@ def coulomb(rm: RefMesh, species, U_list, id_list, cs_list, dt, seed, step, coul
     return Us, stats
 
 
+def en_scatter(relativistic, eV, UE, Un, mE, mN, ueps, uchi):
+    """one electron-neutral scattering; returns the new (UE, Un)"""
+    e = np.array(UE, np.float64); n = np.array(Un, np.float64)
+    lib().ref_en_scatter(int(relativistic), float(eV), _p(e), _p(n), float(mE), float(mN), float(ueps), float(uchi))
+    return e, n
+
+
 class MccParams(C.Structure):
     _fields_ = [("massE", C.c_double), ("massN", C.c_double), ("nN", C.c_double), ("TN", C.c_double),
                 ("wE", C.c_double), ("wN", C.c_double), ("driftN", C.c_double * 3), ("n_el", C.c_int),
@@ -269,7 +278,7 @@ def mcc(rm: RefMesh, params: dict, dt, yE, UE, tetE, idE, cs, cap_e, yI, UI, tet
     for d in range(3):
         p.driftN[d] = params.get("driftN", (0, 0, 0))[d]
     p.create_ion = int(params.get("create_ion", 1)); p.create_electron = int(params.get("create_electron", 1))
-    p.relativistic_scatter = 0
+    p.relativistic_scatter = int(params.get("relativistic", 0))
     p.seed = params["seed"]; p.step = params["step"]
     ne, ni = len(tetE), len(tetI)
 

@@ -416,6 +416,7 @@ int do_mcc_t(pic_ctx* c, double dt, int maxc) {
     for (int k = 0; k < 3; ++k) { a.xs_e[k] = c->d_xs_e[k]; a.xs_s[k] = c->d_xs_s[k]; a.xs_n[k] = c->xs_n[k]; }
     a.th_ex = c->xs_th[1];
     a.th_iz = c->xs_th[2];
+    a.relativistic = c->en_relativistic;
     a.massE = E.mass;
     a.massN = c->bg_m;
     a.nN = c->bg_n;

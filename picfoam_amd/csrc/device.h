@@ -105,4 +105,58 @@ __device__ __forceinline__ int wave_aggregated_inc(int* counter, int key) {
     return result;
 }
 
+// Lorentz transformation of two parcels to their centre-of-momentum frame and back, the
+// common part of PerezRelativisticCollision.C:84-196 and
+// RelativisticElectronNeutralCollision.C:84-158 (FP64)
+struct ComFrame {
+    double gP, gQ, msum, gc, gcs, gPs, gQs, pm;
+    vec3<double> vc, pPs;
+};
+
+__device__ __forceinline__ void com_enter(vec3<double> uP, vec3<double> uQ, double mP, double mQ, ComFrame& f) {
+    const double c2 = kC * kC;
+    const double bP = mag(uP) / kC, bQ = mag(uQ) / kC;
+    f.gP = 1.0 / sqrt(1.0 - bP * bP);
+    f.gQ = 1.0 / sqrt(1.0 - bQ * bQ);
+    const vec3<double> pP = (mP * f.gP) * uP, pQ = (mQ * f.gQ) * uQ;
+    f.msum = mP * f.gP + mQ * f.gQ;
+    f.vc = (1.0 / f.msum) * (pP + pQ);
+    const double vc2 = dot(f.vc, f.vc);
+    if (vc2 == 0.0) {
+        f.gc = 1.0; f.gcs = 0.5; f.pPs = pP; f.gPs = f.gP; f.gQs = f.gQ;
+    } else {
+        f.gc = 1.0 / sqrt(1.0 - vc2 / c2);
+        f.gcs = (f.gc - 1.0) / vc2;
+        f.pPs = pP + ((f.gcs * dot(f.vc, uP) - f.gc) * mP * f.gP) * f.vc;
+        f.gPs = (1.0 - dot(f.vc, uP) / c2) * f.gP * f.gc;
+        f.gQs = (1.0 - dot(f.vc, uQ) / c2) * f.gQ * f.gc;
+    }
+    f.pm = mag(f.pPs);
+}
+
+// rotate P's centre-of-momentum momentum by chi (cos cc, sin sc) and azimuth (sin se,
+// cos ce), then transform both parcels back
+__device__ __forceinline__ void com_leave(const ComFrame& f, double mP, double mQ, double cc, double sc, double se,
+                                          double ce, vec3<double>& UP, vec3<double>& UQ) {
+    const double sce = sc * ce, scs = sc * se;
+    const vec3<double> p = f.pPs;
+    const double pm = f.pm;
+    vec3<double> np;
+    const double pperp = sqrt(p.x * p.x + p.y * p.y);
+    if (pperp > 1.e-10 * pm) {
+        const double ip = 1.0 / pperp;
+        np = mk3<double>((p.x * p.z * sce - p.y * pm * scs) * ip + p.x * cc,
+                         (p.y * p.z * sce + p.x * pm * scs) * ip + p.y * cc, -pperp * sce + p.z * cc);
+    } else {
+        np = mk3<double>(pm * sce, pm * scs, pm * cc);
+    }
+    const vec3<double> newP = np + (f.gcs * dot(f.vc, np) + mP * f.gPs * f.gc) * f.vc;
+    const double aP = mag(newP) / (mP * kC);
+    const vec3<double> mnp = -1.0 * np;
+    const vec3<double> newQ = mnp + (f.gcs * dot(f.vc, mnp) + mQ * f.gQs * f.gc) * f.vc;
+    const double aQ = mag(newQ) / (mQ * kC);
+    UP = (1.0 / (mP * sqrt(1.0 + aP * aP))) * newP;
+    UQ = (1.0 / (mQ * sqrt(1.0 + aQ * aQ))) * newQ;
+}
+
 }  // namespace picmi

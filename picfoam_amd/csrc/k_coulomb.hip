@@ -152,29 +152,14 @@ __device__ __forceinline__ void perez_scatter(vec3<double>& UP, vec3<double>& UQ
                                               double debye, double dt, double u1, double u2) {
     const double c2 = kC * kC;
     const double mP = d.mP, mQ = d.mQ;
-    const double bP = mag(UP) / kC, bQ = mag(UQ) / kC;
-    const double gP = 1.0 / sqrt(1.0 - bP * bP), gQ = 1.0 / sqrt(1.0 - bQ * bQ);
-    const vec3<double> pP = (mP * gP) * UP, pQ = (mQ * gQ) * UQ;
-    const double msum = mP * gP + mQ * gQ;
-    const vec3<double> vc = (1.0 / msum) * (pP + pQ);
-    const double vc2 = dot(vc, vc);
-    double gc, gcs, gPs, gQs;
-    vec3<double> pPs;
-    if (vc2 == 0.0) {
-        gc = 1.0; gcs = 0.5; pPs = pP; gPs = gP; gQs = gQ;
-    } else {
-        gc = 1.0 / sqrt(1.0 - vc2 / c2);
-        gcs = (gc - 1.0) / vc2;
-        pPs = pP + ((gcs * dot(vc, UP) - gc) * mP * gP) * vc;
-        gPs = (1.0 - dot(vc, UP) / c2) * gP * gc;
-        gQs = (1.0 - dot(vc, UQ) / c2) * gQ * gc;
-    }
-    const double pm = mag(pPs);
+    ComFrame f;
+    com_enter(UP, UQ, mP, mQ, f);
+    const double pm = f.pm;
     if (pm == 0.0) return;
-    const double t2 = 1.0 + mP * gPs * mQ * gQs * c2 / (pm * pm);
+    const double t2 = 1.0 + mP * f.gPs * mQ * f.gQs * c2 / (pm * pm);
     double lnL;
     if (d.lnL == 0.0) {
-        double b0 = d.qP * d.qQ / (4.0 * kPi * kEps0 * c2) * gc / msum * t2;
+        double b0 = d.qP * d.qQ / (4.0 * kPi * kEps0 * c2) * f.gc / f.msum * t2;
         if (calc_bmin) {
             const double hb = kH / (2.0 * pm);
             b0 = hb > fabs(b0) ? hb : fabs(b0);
@@ -185,10 +170,10 @@ __device__ __forceinline__ void perez_scatter(vec3<double>& UP, vec3<double>& UQ
         lnL = d.lnL;
     }
     const double qq = d.qP * d.qQ;
-    double s = (d.nP * d.nQ / d.nPQ) * dt * lnL * qq * qq / (4.0 * kPi * kEps0 * kEps0 * (c2 * c2) * mP * gP * mQ * gQ) *
-               (gc * pm / msum) * (t2 * t2);
+    double s = (d.nP * d.nQ / d.nPQ) * dt * lnL * qq * qq /
+               (4.0 * kPi * kEps0 * kEps0 * (c2 * c2) * mP * f.gP * mQ * f.gQ) * (f.gc * pm / f.msum) * (t2 * t2);
     // low-temperature limit of s; 1.61199... = (4 pi / 3)^(1/3)
-    const double vrel = pm * msum / (mP * gPs * mQ * gQs * gc);
+    const double vrel = pm * f.msum / (mP * f.gPs * mQ * f.gQs * f.gc);
     const double dP = mP * pow(d.nP, 2.0 / 3.0), dQ = mQ * pow(d.nQ, 2.0 / 3.0);
     const double smax = 1.6119919540164696 * (d.nP * d.nQ / d.nPQ) * dt * (mP + mQ) / (dP > dQ ? dP : dQ) * vrel;
     if (smax < s) s = smax;
@@ -196,23 +181,7 @@ __device__ __forceinline__ void perez_scatter(vec3<double>& UP, vec3<double>& UQ
     const double sc = sqrt(1.0 - cc * cc);
     double se, ce;
     sincos_2pi(u2, &se, &ce);
-    const double sce = sc * ce, scs = sc * se;
-    vec3<double> np;
-    const double pperp = sqrt(pPs.x * pPs.x + pPs.y * pPs.y);
-    if (pperp > 1.e-10 * pm) {
-        const double ip = 1.0 / pperp;
-        np = mk3<double>((pPs.x * pPs.z * sce - pPs.y * pm * scs) * ip + pPs.x * cc,
-                         (pPs.y * pPs.z * sce + pPs.x * pm * scs) * ip + pPs.y * cc, -pperp * sce + pPs.z * cc);
-    } else {
-        np = mk3<double>(pm * sce, pm * scs, pm * cc);
-    }
-    const vec3<double> newP = np + (gcs * dot(vc, np) + mP * gPs * gc) * vc;
-    const double aP = mag(newP) / (mP * kC);
-    const vec3<double> mnp = -1.0 * np;
-    const vec3<double> newQ = mnp + (gcs * dot(vc, mnp) + mQ * gQs * gc) * vc;
-    const double aQ = mag(newQ) / (mQ * kC);
-    UP = (1.0 / (mP * sqrt(1.0 + aP * aP))) * newP;
-    UQ = (1.0 / (mQ * sqrt(1.0 + aQ * aQ))) * newQ;
+    com_leave(f, mP, mQ, cc, sc, se, ce, UP, UQ);
 }
 
 // one collision of parcel iP (species of d's P side) with parcel iQ; MODEL 0 Nanbu, 1 Perez

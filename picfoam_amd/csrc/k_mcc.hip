@@ -64,6 +64,33 @@ __device__ void nanbu_scatter(double eV, vec3<double>& UE, vec3<double>& Un, dou
     Un = Un + (mE / (mN + mE)) * dv;
 }
 
+// RelativisticElectronNeutralCollision::updateVelocity with a sampled partner velocity
+// (RelativisticElectronNeutralCollision.C:161-240).  cos(chi) is clamped to [-1, 1] as in
+// the two-parcel variant (:120-126); the sampled variant would produce a NaN there.
+__device__ void rel_scatter(double eV, vec3<double>& UE, vec3<double>& Un, double mE, double mN, double ueps,
+                            double uchi) {
+    ComFrame f;
+    com_enter(UE, Un, mE, mN, f);
+    if (f.pm == 0.0) return;
+    double cc, sc;
+    if (eV < 1e-30) {
+        cc = 1.0; sc = 0.0;
+    } else {
+        cc = 1.0 / eV * (eV + 2. - 2. * pow(1. + eV, uchi));
+        if (cc * cc > 1.0) { cc = 1.0; sc = 0.0; }
+        else sc = sqrt(1.0 - cc * cc);
+    }
+    const double phi = 2.0 * kPi * ueps;
+    com_leave(f, mE, mN, cc, sc, sin(phi), cos(phi), UE, Un);
+}
+
+template <class UT>
+__device__ __forceinline__ void en_scatter(const MccArgs<UT>& a, double eV, vec3<double>& UE, vec3<double>& Un,
+                                           double us, double uc) {
+    if (a.relativistic) rel_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
+    else nanbu_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
+}
+
 __device__ void bitonic_sort_u64(uint64_t* key, int* val, int npow2) {
     for (int k = 2; k <= npow2; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -120,12 +147,12 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
     double us, uc;
     mcc_rand(a.seed, a.step, cell, k, 3, us, uc);
     if (uacc <= Qel / sigmaT) {
-        nanbu_scatter(eV, Ue, Un, a.massE, a.massN, us, uc);
+        en_scatter(a, eV, Ue, Un, us, uc);
         ++c_el;
     } else if (eV >= a.th_ex && uacc <= (Qel + Qex) / sigmaT) {
         Ue = sqrt(1.0 - a.th_ex / eV) * Ue;
         eV -= a.th_ex;
-        nanbu_scatter(eV, Ue, Un, a.massE, a.massN, us, uc);
+        en_scatter(a, eV, Ue, Un, us, uc);
         ++c_ex;
     } else if (eV >= a.th_iz && uacc <= Qt / sigmaT) {
         double uE, uR, up2, uc2;
@@ -138,7 +165,7 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
         const vec3<double> preUe = Ue, preUn = Un;
         Ue = sqrt(1.0 - (a.th_iz + Ep) / eV) * Ue;
         eV -= (a.th_iz + Ep);
-        nanbu_scatter(eV, Ue, Un, a.massE, a.massN, us, uc);
+        en_scatter(a, eV, Ue, Un, us, uc);
         const int t = a.e.tet[ei];
         const double y1 = a.e.y1[ei], y2 = a.e.y2[ei], y3 = a.e.y3[ei];
         bool created = false;
@@ -165,7 +192,7 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
                 const double sp = sqrt(2.0 * Ep * kQe / a.massE);
                 vec3<double> Unew = sp * (preUe / mag(preUe));
                 vec3<double> Un2 = preUn;
-                nanbu_scatter(Ep, Unew, Un2, a.massE, a.massN, up2, uc2);
+                en_scatter(a, Ep, Unew, Un2, up2, uc2);
                 a.e.y1[slot] = y1; a.e.y2[slot] = y2; a.e.y3[slot] = y3;
                 a.e.tet[slot] = t;
                 a.e.ux[slot] = (UT)Unew.x; a.e.uy[slot] = (UT)Unew.y; a.e.uz[slot] = (UT)Unew.z;

@@ -178,6 +178,7 @@ struct MccArgs {
     double th_ex, th_iz;
     double massE, massN, nN, vthN, drift[3], wEwN, wNwE;
     int create_ion, create_electron;
+    int relativistic;         // RelativisticElectronNeutral scattering instead of Nanbu
     double dt;
     uint64_t seed;
     int64_t step;

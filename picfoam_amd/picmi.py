@@ -316,6 +316,19 @@ class Cloud:
         e = np.ascontiguousarray(energies, dtype=np.float64); s = np.ascontiguousarray(values, dtype=np.float64)
         self._check(self.lib.pic_set_cross_section(self.h, process, len(e), _ptr(e), _ptr(s), float(threshold)))
 
+    def set_cross_section_model(self, process, model, **coeffs):
+        """CrossSectionModel selection per process (0 elastic, 1 excitation, 2 ionization):
+        "List" (energies, values, threshold; ListCrossSection.C), "FixedValue" (value,
+        threshold; FixedValueCrossSection.C: the same sigma at every energy) or "none"."""
+        if model == "List":
+            self.set_cross_section(process, coeffs["energies"], coeffs["values"], coeffs.get("threshold", 0.0))
+        elif model == "FixedValue":
+            self.set_cross_section(process, [0.0], [float(coeffs["value"])], coeffs.get("threshold", 0.0))
+        elif model == "none":
+            self.set_cross_section(process, [], [], 0.0)
+        else:
+            raise PicError(-2, f"Unknown CrossSectionModel type {model}. Valid types: List FixedValue none")
+
     def mcc_initialize(self, maximum_energy, energy_step):
         self._check(self.lib.pic_mcc_initialize(self.h, maximum_energy, energy_step))
 

@@ -223,8 +223,10 @@ def test_coulomb_fp64_parity(oracle, counts, variant):
         assert np.any(Us[0] != Ue)
 
 
-def test_mcc_fp64_parity(oracle):
-    say("mcc parity")
+@pytest.mark.parametrize("model,xs_model", [("NanbuElectronNeutral", "List"), ("RelativisticElectronNeutral", "List"),
+                                            ("NanbuElectronNeutral", "FixedValue")])
+def test_mcc_fp64_parity(oracle, model, xs_model):
+    say("mcc parity", model, xs_model)
     rng = np.random.default_rng(105)
     from picfoam_amd import cases
     m = small_hex(n=(3, 3, 2), L=(1e-3, 1e-3, 1e-3))
@@ -236,13 +238,21 @@ def test_mcc_fp64_parity(oracle):
     nN = 1e21
     species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": w, "capacity": 40000},
                {"mass": mAr, "charge": QE, "nEquivalentParticles": w, "capacity": 40000, "solveMovement": False}]
-    models = {"ParticlePusher": "Boris", "ElectronNeutralCollisionModel": "NanbuElectronNeutral",
+    models = {"ParticlePusher": "Boris", "ElectronNeutralCollisionModel": model,
               "BackgroundGasModel": "uniformBackground", "electronTypeId": 0, "ionTypeId": 1,
               "uniformBackground": {"numberDensity": nN, "temperature": 300.0, "mass": mAr,
                                     "nEquivalentParticles": w}, "seed": 99}
     c = cloud(m, models, species)
-    for k, (s_, th) in enumerate(((el, 0.0), (ex, 11.5), (iz, 15.76))):
-        c.set_cross_section(k, e, s_, th)
+    tables = [(e, el), (e, ex), (e, iz)]
+    if xs_model == "FixedValue":
+        # FixedValueCrossSection: one value at every energy (elastic and excitation)
+        tables = [(np.array([0.0]), np.array([1.5e-19])), (np.array([0.0]), np.array([2e-21])), (e, iz)]
+        c.set_cross_section_model(0, "FixedValue", value=1.5e-19)
+        c.set_cross_section_model(1, "FixedValue", value=2e-21, threshold=11.5)
+        c.set_cross_section_model(2, "List", energies=e, values=iz, threshold=15.76)
+    else:
+        for k, (s_, th) in enumerate(((el, 0.0), (ex, 11.5), (iz, 15.76))):
+            c.set_cross_section(k, e, s_, th)
     c.mcc_initialize(1000.0, 0.01)
     ne = rng.integers(50, 150, nc)
     (ye, te, Ue, ie), _ = _two_species_cell_setup(rng, m, rm, ne, np.zeros(nc, int), Te=30.0)
@@ -256,8 +266,9 @@ def test_mcc_fp64_parity(oracle):
     _, teg, Ueg, ieg = c.get_particles(0)
     _, tig, Uig, iig = c.get_particles(1)
     cs = np.concatenate([[0], np.cumsum(ne)]).astype(np.int32)
-    P = {"xs": [(e, el), (e, ex), (e, iz)], "massE": ME, "massN": mAr, "nN": nN, "TN": 300.0, "wE": w, "wN": w,
-         "th_ex": 11.5, "th_iz": 15.76, "seed": 99, "step": 3}
+    P = {"xs": tables, "massE": ME, "massN": mAr, "nN": nN, "TN": 300.0, "wE": w, "wN": w,
+         "th_ex": 11.5, "th_iz": 15.76, "seed": 99, "step": 3,
+         "relativistic": int(model == "RelativisticElectronNeutral")}
     out = oracle.mcc(rm, P, dt, ye, Ue, te, ie, cs, 40000, np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0, np.int32),
                      np.zeros(0, np.uint32), 40000, smax0, rem0)
     assert out["err"] == 0

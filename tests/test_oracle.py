@@ -614,3 +614,37 @@ def test_mcc_elastic_energy_loss(oracle):
     mean_1mc = np.mean(1 - (2 + eV - 2 * (1 + eV) ** Rs) / eV)
     expect = 2 * ME / M * mean_1mc
     assert abs(loss_per_collision - expect) / expect < 0.05
+
+
+def test_relativistic_en_scatter_conserves_four_momentum(oracle):
+    """RelativisticElectronNeutralCollision: the scattering is a rotation of the electron's
+    centre-of-momentum momentum, so sum m gamma v and sum m gamma are conserved."""
+    rng = np.random.default_rng(31)
+    mE, mN = ME, 6.63e-26
+    for _ in range(50):
+        UE = rng.normal(size=3) * 0.4 * C_LIGHT / np.sqrt(3)
+        Un = rng.normal(size=3) * 400.0
+        g = lambda u: 1.0 / np.sqrt(1.0 - (np.linalg.norm(u) / C_LIGHT) ** 2)
+        e, n = oracle.en_scatter(1, 50.0, UE, Un, mE, mN, rng.random(), rng.random())
+        p0 = mE * g(UE) * UE + mN * g(Un) * Un
+        p1 = mE * g(e) * e + mN * g(n) * n
+        assert np.allclose(p1, p0, rtol=0, atol=1e-12 * mE * g(UE) * np.linalg.norm(UE))
+        assert abs(mE * g(e) + mN * g(n) - (mE * g(UE) + mN * g(Un))) < 1e-13 * mN
+
+
+def test_relativistic_en_scatter_matches_nanbu_at_low_speed(oracle):
+    """At |v| << c both electron-neutral models rotate the relative velocity by the same
+    angle: cos(theta) = (2 + E - 2 (1 + E)^R) / E for the same uniform R."""
+    rng = np.random.default_rng(32)
+    mE, mN = ME, 40 * MP
+    for _ in range(40):
+        UE = rng.normal(size=3) * 1e5
+        Un = rng.normal(size=3) * 300.0
+        E, R, P = 5.0 * rng.random() + 0.1, rng.random(), rng.random()
+        cos_expect = (E + 2.0 - 2.0 * (1.0 + E) ** R) / E
+        for rel in (0, 1):
+            e, n = oracle.en_scatter(rel, E, UE, Un, mE, mN, P, R)
+            g0, g1 = UE - Un, e - n
+            c = g0 @ g1 / (np.linalg.norm(g0) * np.linalg.norm(g1))
+            assert abs(c - cos_expect) < 1e-7, (rel, c, cos_expect)
+            assert abs(np.linalg.norm(g1) / np.linalg.norm(g0) - 1) < 1e-7
