@@ -107,7 +107,11 @@ def copy_calibration(mib):
     print(f"bench: copy calibration {mib} MiB x3 done", file=sys.stderr, flush=True)
 
 
-def run_steps(args, precision, steps, warmup, profile):
+def run_steps(args, precision, steps, warmup, profile, sync_steps=0):
+    """warmup steps, then `steps` steps through pic_run (one host synchronisation at the end),
+    timed with the host clock around the call; the per-stage HIP events recorded on the
+    context stream give each stage's GPU time.  sync_steps > 0 adds that many pic_step
+    calls (synchronous, one round trip each) for comparison."""
     from picfoam_amd import cases, picmi
     mesh, species, models, ex = cases.thermalisation(ncell=args.ncell, ppc=args.ppc, precision=precision)
     cloud = picmi.Cloud(mesh, species, models)
@@ -115,19 +119,29 @@ def run_steps(args, precision, steps, warmup, profile):
     cloud.init_maxwellian(1, args.ppc, ex["Ti_K"], id0=0, seed=12)
     dt = ex["dt"]
     n_total = cloud.count(0) + cloud.count(1)
-    for _ in range(warmup):
-        cloud.step(dt)
+    if warmup:
+        cloud.run(dt, warmup)
     cloud.sync()
     if profile:
         cloud.profile(True)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        cloud.step(dt)
+    st = cloud.run(dt, steps)
     cloud.sync()
     t1 = time.perf_counter()
     stages = cloud.stage_times() if profile else None
+    if profile:
+        cloud.profile(False)
+    sync_ms = None
+    if sync_steps:
+        cloud.sync()
+        t2 = time.perf_counter()
+        for _ in range(sync_steps):
+            cloud.step(dt)
+        sync_ms = 1e3 * (time.perf_counter() - t2) / sync_steps
+    if st.coulomb_collisions <= 0:
+        raise RuntimeError("no Coulomb collisions happened in the timed steps")
     cloud.close()
-    return n_total, t1 - t0, stages, dt
+    return n_total, t1 - t0, stages, dt, sync_ms
 
 
 def main():
@@ -154,7 +168,8 @@ def main():
         copy_calibration(args.copy_calibration)
     print(f"bench: {args.ncell}^3 mesh, {2 * args.ncell ** 3 * args.ppc} parcels, {args.precision}", file=sys.stderr,
           flush=True)
-    n_total, elapsed, stages, dt = run_steps(args, args.precision, args.steps, args.warmup, True)
+    n_total, elapsed, stages, dt, sync_ms = run_steps(args, args.precision, args.steps, args.warmup, True,
+                                                      sync_steps=10)
     ms_per_step = 1e3 * elapsed / args.steps
     value = n_total * args.steps / elapsed
     # dominant kernel and its roofline
@@ -183,10 +198,11 @@ def main():
                    "parcels": n_total, "dt": dt, "precision": args.precision},
         "roofline": roof,
         "stage_ms_per_step": stage_ms,
+        "ms_per_step_synchronous_api": sync_ms,
     }
     if not args.no_mixed and args.precision == "fp64":
         print("bench: mixed-precision run of the same workload", file=sys.stderr, flush=True)
-        n2, el2, _, _ = run_steps(args, "mixed", args.steps, args.warmup, False)
+        n2, el2, _, _, _ = run_steps(args, "mixed", args.steps, args.warmup, False)
         out["mixed_precision"] = {"value": n2 * args.steps / el2, "ms_per_step": 1e3 * el2 / args.steps,
                                   "speedup_vs_fp64": (n2 * args.steps / el2) / value}
     if not args.no_cpu_baseline:

@@ -191,6 +191,11 @@ int pic_deposit(pic_ctx* ctx);                         /* calculateFields: charg
 int pic_poisson_solve(pic_ctx* ctx, pic_poisson_stats* stats); /* MaxwellSolver::solveFields */
 int pic_diag_reduce(pic_ctx* ctx, pic_diag* out);      /* DiagnosticsList::info sums */
 int pic_step(pic_ctx* ctx, double dt, pic_step_stats* stats);  /* one PICCloud::evolve() */
+/* n_steps evolve() calls, the time loop of picFoam.C, with one host synchronisation at the
+ * end: stats sum the counters over the steps (poisson: the last solve); an error raised on
+ * the device (superluminal parcel, capacity, cell overflow) is reported at the end and
+ * leaves the state of the step it happened in */
+int pic_run(pic_ctx* ctx, double dt, int64_t n_steps, pic_step_stats* stats);
 int pic_sync(pic_ctx* ctx);
 int64_t pic_step_index(const pic_ctx* ctx);
 int pic_set_step_index(pic_ctx* ctx, int64_t step);

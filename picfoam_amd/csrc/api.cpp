@@ -91,8 +91,9 @@ struct pic_ctx {
     unsigned long long *d_acc_cell = nullptr, *d_acc_point = nullptr;
     uint32_t* d_flags = nullptr;
     unsigned long long* d_stats = nullptr;  // 0 deleted, 1 guard, 2 coulomb, 3..6 mcc
-    int64_t* d_n = nullptr;                  // device parcel counts per species
-    int* d_maxc = nullptr;
+    int64_t* d_n = nullptr;                  // device parcel counts per species (authoritative)
+    int* d_big_list = nullptr;               // cells left to the large-cell collision passes
+    int* d_big_n = nullptr;
     void* d_scan_tmp = nullptr;
     size_t scan_bytes = 0;
     double* d_diag_part = nullptr;
@@ -259,6 +260,7 @@ MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
     MoveArgs<UT> a{};
     a.p = soa<UT>(S, S.cur);
     a.n = S.n;
+    a.n_live = c->d_n + s;
     a.tets = c->d_tets;
     if constexpr (sizeof(UT) == 4) {
         a.Ecell = (const UT*)c->d_Ef; a.Epoint = (const UT*)c->d_Epf; a.Bcell = (const UT*)c->d_Bf;
@@ -303,45 +305,37 @@ int do_move(pic_ctx* c, double dt) {
     return PIC_OK;
 }
 
-int do_sort(pic_ctx* c, int* maxc_out) {
-    {
-        pic_ctx::Scope sc(c, PIC_STAGE_SORT);
-        const int nc = c->hm.n_cells;
-        if (!c->counts_valid) {
-            for (auto& s : c->sp) hipMemsetAsync(s.count, 0, sizeof(int) * (nc + 1), c->stream);
-            for (int s = 0; s < (int)c->sp.size(); ++s) {
-                hipError_t e = c->mixed() ? launch_deposit(move_args<float>(c, s, 0.0), kCellAverage, c->stream)
-                                          : launch_deposit(move_args<double>(c, s, 0.0), kCellAverage, c->stream);
-                if (int r = check_hip(c, e, "count kernel")) return r;
-            }
-            c->counts_valid = true;
-        }
-        hipMemsetAsync(c->d_maxc, 0, sizeof(int), c->stream);
+// buildCellOccupancy (PICCloud.C:327-354): counting sort of every species by cell.  No
+// host round trip: the live count after the sort (start[n_cells]) stays on the device and
+// the host keeps S.n as an upper bound for launch sizes until the next synchronisation.
+int do_sort(pic_ctx* c) {
+    pic_ctx::Scope sc(c, PIC_STAGE_SORT);
+    const int nc = c->hm.n_cells;
+    if (!c->counts_valid) {
+        for (auto& s : c->sp) hipMemsetAsync(s.count, 0, sizeof(int) * (nc + 1), c->stream);
         for (int s = 0; s < (int)c->sp.size(); ++s) {
-            Species& S = c->sp[s];
-            hipError_t e = launch_scan(S.count, S.start, nc + 1, c->d_scan_tmp, c->scan_bytes, c->stream);
-            if (int r = check_hip(c, e, "scan")) return r;
-            launch_max_count(S.count, nc, c->d_maxc, c->stream);
-            hipMemsetAsync(S.cursor, 0, sizeof(int) * (nc + 1), c->stream);
-            const int nb = 1 - S.cur;
-            if (c->mixed()) e = launch_scatter(soa<float>(S, S.cur), soa<float>(S, nb), S.n, c->d_tets, S.start, S.cursor, c->stream);
-            else e = launch_scatter(soa<double>(S, S.cur), soa<double>(S, nb), S.n, c->d_tets, S.start, S.cursor, c->stream);
-            if (int r = check_hip(c, e, "scatter")) return r;
-            S.cur = nb;
+            hipError_t e = c->mixed() ? launch_deposit(move_args<float>(c, s, 0.0), kCellAverage, c->stream)
+                                      : launch_deposit(move_args<double>(c, s, 0.0), kCellAverage, c->stream);
+            if (int r = check_hip(c, e, "count kernel")) return r;
         }
+        c->counts_valid = true;
     }
-    // live counts after compaction: start[n_cells]
-    std::vector<int> nn(c->sp.size());
-    int maxc = 0;
-    for (size_t s = 0; s < c->sp.size(); ++s)
-        hipMemcpyAsync(&nn[s], c->sp[s].start + c->hm.n_cells, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-    hipMemcpyAsync(&maxc, c->d_maxc, sizeof(int), hipMemcpyDeviceToHost, c->stream);
-    if (int r = sync_state(c)) return r;
-    for (size_t s = 0; s < c->sp.size(); ++s) {
-        c->sp[s].n = nn[s];
-        set_dev_n(c, (int)s);
+    for (int s = 0; s < (int)c->sp.size(); ++s) {
+        Species& S = c->sp[s];
+        hipError_t e = launch_scan(S.count, S.start, nc + 1, c->d_scan_tmp, c->scan_bytes, c->stream);
+        if (int r = check_hip(c, e, "scan")) return r;
+        hipMemsetAsync(S.cursor, 0, sizeof(int) * (nc + 1), c->stream);
+        const int nb = 1 - S.cur;
+        if (c->mixed())
+            e = launch_scatter(soa<float>(S, S.cur), soa<float>(S, nb), S.n, c->d_n + s, c->d_tets, S.start, S.cursor,
+                               c->stream);
+        else
+            e = launch_scatter(soa<double>(S, S.cur), soa<double>(S, nb), S.n, c->d_n + s, c->d_tets, S.start,
+                               S.cursor, c->stream);
+        if (int r = check_hip(c, e, "scatter")) return r;
+        if (int r = check_hip(c, launch_store_count(S.start + nc, c->d_n + s, c->stream), "count")) return r;
+        S.cur = nb;
     }
-    if (maxc_out) *maxc_out = maxc;
     c->sorted = true;
     return PIC_OK;
 }
@@ -352,7 +346,7 @@ int pow2_at_least(int v) {
     return p;
 }
 
-int do_coulomb(pic_ctx* c, double dt, int maxc) {
+int do_coulomb(pic_ctx* c, double dt) {
     pic_ctx::Scope sc(c, PIC_STAGE_COULOMB);
     const int ns = (int)c->sp.size();
     CoulombArgs a{};
@@ -379,19 +373,23 @@ int do_coulomb(pic_ctx* c, double dt, int maxc) {
     a.dt = dt;
     a.seed = c->seed;
     a.step = c->step;
-    a.maxp = pow2_at_least(maxc);
+    // index lists in LDS: 256 parcels of a species per cell in the per-cell pass (6 KiB),
+    // 2048 in the large-cell pass (48 KiB)
+    a.maxp = kCoulombSlots;
+    a.maxp_big = kCoulombSlotsBig;
+    a.big_list = c->d_big_list;
+    a.big_n = c->d_big_n;
+    hipMemsetAsync(c->d_big_n, 0, sizeof(int), c->stream);
     a.model = c->coulomb_model;
     a.pairing = c->pairing;
     a.weight_correction = c->weight_corr;
-    // two key/index buffers per species pair in LDS: 24 B per slot, at most 48 KiB
-    if (a.maxp > 2048) return c->fail(PIC_ERR_CELL_OVERFLOW, "more than 2048 parcels of one species in a cell");
     a.flags = c->d_flags;
     a.ncoll = c->d_stats + 2;
     return check_hip(c, launch_coulomb(a, c->mixed(), c->stream), "coulomb kernel");
 }
 
 template <class UT>
-int do_mcc_t(pic_ctx* c, double dt, int maxc) {
+int do_mcc_t(pic_ctx* c, double dt) {
     Species& E = c->sp[c->electron];
     MccArgs<UT> a{};
     a.n_cells = c->hm.n_cells;
@@ -429,9 +427,13 @@ int do_mcc_t(pic_ctx* c, double dt, int maxc) {
     a.dt = dt;
     a.seed = c->seed;
     a.step = c->step;
-    a.maxp = pow2_at_least(maxc);
-    // one key/index buffer in LDS: 12 B per slot, at most 48 KiB
-    if (a.maxp > 4096) return c->fail(PIC_ERR_CELL_OVERFLOW, "more than 4096 electrons in a cell");
+    // one key/index list in LDS: 512 electrons per cell in the per-cell pass (6 KiB), 4096
+    // in the large-cell pass (48 KiB)
+    a.maxp = kMccSlots;
+    a.maxp_big = kMccSlotsBig;
+    a.big_list = c->d_big_list;
+    a.big_n = c->d_big_n;
+    hipMemsetAsync(c->d_big_n, 0, sizeof(int), c->stream);
     a.flags = c->d_flags;
     a.counts = c->d_stats + 3;
     a.tets = c->d_tets;
@@ -443,10 +445,15 @@ int do_mcc_t(pic_ctx* c, double dt, int maxc) {
     return check_hip(c, launch_mcc(a, c->stream), "mcc kernel");
 }
 
-int do_mcc(pic_ctx* c, double dt, int maxc) {
+int do_mcc(pic_ctx* c, double dt) {
     pic_ctx::Scope sc(c, PIC_STAGE_MCC);
     if (c->xs_n[0] + c->xs_n[1] + c->xs_n[2] == 0) return c->fail(PIC_ERR_INVALID, "no cross sections set");
-    return c->mixed() ? do_mcc_t<float>(c, dt, maxc) : do_mcc_t<double>(c, dt, maxc);
+    const int r = c->mixed() ? do_mcc_t<float>(c, dt) : do_mcc_t<double>(c, dt);
+    // created parcels are appended on the device: until the next synchronisation the
+    // launch bound of the species that can grow is their capacity
+    if (c->create_electron) c->sp[c->electron].n = c->sp[c->electron].capacity;
+    if (c->create_ion && c->ion >= 0) c->sp[c->ion].n = c->sp[c->ion].capacity;
+    return r;
 }
 
 int do_deposit_finish(pic_ctx* c) {
@@ -747,7 +754,8 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->d_flags = c->alloc<uint32_t>(1);
     c->d_stats = c->alloc<unsigned long long>(16);
     c->d_n = c->alloc<int64_t>(kMaxSpecies);
-    c->d_maxc = c->alloc<int>(1);
+    c->d_big_list = c->alloc<int>(nc);
+    c->d_big_n = c->alloc<int>(1);
     c->scan_bytes = scan_temp_bytes(nc + 1);
     c->d_scan_tmp = c->alloc<char>(c->scan_bytes);
     c->d_diag_part = c->alloc<double>(1024 * 8);
@@ -1092,22 +1100,21 @@ int pic_move(pic_ctx* c, double dt) {
 
 int pic_sort(pic_ctx* c) {
     if (!c) return PIC_ERR_INVALID;
-    return do_sort(c, nullptr);
+    if (int r = do_sort(c)) return r;
+    return sync_state(c);
 }
 
 int pic_collide_coulomb(pic_ctx* c, double dt) {
     if (!c || !c->coulomb) return c ? c->fail(PIC_ERR_INVALID, "CoulombCollisionModel is none") : PIC_ERR_INVALID;
-    int maxc = 0;
-    if (int r = do_sort(c, &maxc)) return r;
-    if (int r = do_coulomb(c, dt, maxc)) return r;
+    if (int r = do_sort(c)) return r;
+    if (int r = do_coulomb(c, dt)) return r;
     return sync_state(c);
 }
 
 int pic_collide_mcc(pic_ctx* c, double dt) {
     if (!c || !c->mcc) return c ? c->fail(PIC_ERR_INVALID, "ElectronNeutralCollisionModel is none") : PIC_ERR_INVALID;
-    int maxc = 0;
-    if (int r = do_sort(c, &maxc)) return r;
-    if (int r = do_mcc(c, dt, maxc)) return r;
+    if (int r = do_sort(c)) return r;
+    if (int r = do_mcc(c, dt)) return r;
     return sync_state(c);
 }
 
@@ -1129,40 +1136,57 @@ int pic_poisson_solve(pic_ctx* c, pic_poisson_stats* st) {
     return do_poisson(c, st);
 }
 
+// PICCloud::evolve (PICCloud.C:1426-1471) enqueued on the context stream without a host
+// round trip, except inside the Poisson solve (convergence checks) and GNN recording
+int step_body(pic_ctx* c, double dt, pic_poisson_stats* ps) {
+    if (int r = do_move(c, dt)) return r;
+    const bool need_sort = c->coulomb || c->mcc || ((c->step + 1) % c->sort_interval == 0);
+    if (need_sort)
+        if (int r = do_sort(c)) return r;
+    if (c->coulomb)
+        if (int r = do_coulomb(c, dt)) return r;
+    if (c->mcc)
+        if (int r = do_mcc(c, dt)) return r;
+    if (int r = do_deposit_finish(c)) return r;
+    if (c->electrostatic)
+        if (int r = do_poisson(c, ps)) return r;
+    c->step++;
+    return PIC_OK;
+}
+
+void read_step_stats(pic_ctx* c, const pic_poisson_stats& ps, pic_step_stats* st) {
+    unsigned long long h[16];
+    hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost);
+    std::memset(st, 0, sizeof(*st));
+    for (size_t s = 0; s < c->sp.size(); ++s) st->n_parcels[s] = c->sp[s].n;
+    st->n_deleted = (int64_t)h[0];
+    st->n_track_guard = (int64_t)h[1];
+    st->coulomb_collisions = (int64_t)h[2];
+    st->mcc_candidates = (int64_t)h[3];
+    st->mcc_elastic = (int64_t)h[4];
+    st->mcc_excitation = (int64_t)h[5];
+    st->mcc_ionization = (int64_t)h[6];
+    st->poisson = ps;
+}
+
 int pic_step(pic_ctx* c, double dt, pic_step_stats* st) {
     if (!c) return PIC_ERR_INVALID;
     hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * 16, c->stream);
-    if (int r = do_move(c, dt)) return r;
-    const bool need_sort = c->coulomb || c->mcc || ((c->step + 1) % c->sort_interval == 0);
-    int maxc = 0;
-    if (need_sort)
-        if (int r = do_sort(c, &maxc)) return r;
-    if (c->coulomb)
-        if (int r = do_coulomb(c, dt, maxc)) return r;
-    if (c->mcc) {
-        // cell capacity for the electron species only
-        if (int r = do_mcc(c, dt, maxc)) return r;
-    }
-    if (int r = do_deposit_finish(c)) return r;
     pic_poisson_stats ps{};
-    if (c->electrostatic)
-        if (int r = do_poisson(c, &ps)) return r;
+    if (int r = step_body(c, dt, &ps)) return r;
     if (int r = sync_state(c)) return r;
-    c->step++;
-    if (st) {
-        unsigned long long h[16];
-        hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost);
-        std::memset(st, 0, sizeof(*st));
-        for (size_t s = 0; s < c->sp.size(); ++s) st->n_parcels[s] = c->sp[s].n;
-        st->n_deleted = (int64_t)h[0];
-        st->n_track_guard = (int64_t)h[1];
-        st->coulomb_collisions = (int64_t)h[2];
-        st->mcc_candidates = (int64_t)h[3];
-        st->mcc_elastic = (int64_t)h[4];
-        st->mcc_excitation = (int64_t)h[5];
-        st->mcc_ionization = (int64_t)h[6];
-        st->poisson = ps;
-    }
+    if (st) read_step_stats(c, ps, st);
+    return PIC_OK;
+}
+
+int pic_run(pic_ctx* c, double dt, int64_t n_steps, pic_step_stats* st) {
+    if (!c || n_steps < 0) return PIC_ERR_INVALID;
+    hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * 16, c->stream);
+    pic_poisson_stats ps{};
+    for (int64_t k = 0; k < n_steps; ++k)
+        if (int r = step_body(c, dt, &ps)) return r;
+    if (int r = sync_state(c)) return r;
+    if (st) read_step_stats(c, ps, st);
     return PIC_OK;
 }
 

@@ -286,25 +286,26 @@ __device__ __forceinline__ double dmin(double x, double y) { return x < y ? x : 
 
 }  // namespace
 
+// largest charged-species parcel count of the cell
+__device__ __forceinline__ int cell_max_count(const CoulombArgs& a, int cell) {
+    int m = 0;
+    for (int s = 0; s < a.n_species; ++s) {
+        if (a.charge[s] == 0.0) continue;
+        const int n = a.start[s][cell + 1] - a.start[s][cell];
+        m = n > m ? n : m;
+    }
+    return m;
+}
+
+// every pairing stage of one cell by one wave; maxp = LDS slots per index list
 template <class UT, int MODEL>
-__global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
-    extern __shared__ uint64_t lds[];
-    __shared__ StageK sk[2];
-    const int cell = blockIdx.x;
-    const int maxp = a.maxp;
+__device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t* lds, StageK* sk) {
     const int kstride = maxp + 8;
     uint64_t* keyA = lds;
     uint64_t* keyB = lds + kstride;
     int* valA = (int*)(lds + 2 * kstride);
     int* valB = valA + maxp;
     const int ns = a.n_species;
-    for (int s = 0; s < ns; ++s) {
-        if (a.charge[s] == 0.0) continue;
-        if (a.start[s][cell + 1] - a.start[s][cell] > maxp) {
-            if (threadIdx.x == 0) atomicOr(a.flags, kFlagCellOverflow);
-            return;
-        }
-    }
     const double Vc = a.V[cell];
     // CoulombCollisionModel::debyeLength (CoulombCollisionModel.C:249-310), parcels summed
     // in ascending id order
@@ -487,19 +488,57 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
     if (threadIdx.x == 0 && ncoll) atomicAdd(a.ncoll, ncoll);
 }
 
+// one wave per cell with small index lists in LDS; cells with more parcels of a species
+// than a.maxp go to the list of the large-cell pass
+template <class UT, int MODEL>
+__global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ StageK sk[2];
+    const int cell = blockIdx.x;
+    if (cell_max_count(a, cell) > a.maxp) {
+        if (threadIdx.x == 0) a.big_list[atomicAdd(a.big_n, 1)] = cell;
+        return;
+    }
+    coulomb_cell<UT, MODEL>(a, cell, a.maxp, lds, sk);
+}
+
+// the listed large cells, up to a.maxp_big parcels of a species; larger cells raise the
+// overflow flag and are left alone
+template <class UT, int MODEL>
+__global__ void __launch_bounds__(64) k_coulomb_big(CoulombArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ StageK sk[2];
+    const int nbig = *a.big_n;
+    for (int i = blockIdx.x; i < nbig; i += gridDim.x) {
+        const int cell = a.big_list[i];
+        if (cell_max_count(a, cell) > a.maxp_big) {
+            if (threadIdx.x == 0) atomicOr(a.flags, kFlagCellOverflow);
+            continue;
+        }
+        coulomb_cell<UT, MODEL>(a, cell, a.maxp_big, lds, sk);
+        __syncthreads();
+    }
+}
+
 size_t coulomb_lds_bytes(int maxp) {
     return (size_t)(2 * (maxp + 8)) * sizeof(uint64_t) + (size_t)(2 * maxp) * sizeof(int);
 }
 
+template <class UT, int MODEL>
+static void launch_coulomb_t(const CoulombArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_coulomb<UT, MODEL>), dim3(a.n_cells), dim3(64), coulomb_lds_bytes(a.maxp), s, a);
+    const int nb = a.n_cells < 512 ? a.n_cells : 512;
+    hipLaunchKernelGGL((k_coulomb_big<UT, MODEL>), dim3(nb), dim3(64), coulomb_lds_bytes(a.maxp_big), s, a);
+}
+
 hipError_t launch_coulomb(const CoulombArgs& a, bool fp32_velocity, hipStream_t s) {
     if (a.n_cells <= 0) return hipSuccess;
-    const size_t lds = coulomb_lds_bytes(a.maxp);
     if (a.model == 1) {
-        if (fp32_velocity) hipLaunchKernelGGL((k_coulomb<float, 1>), dim3(a.n_cells), dim3(64), lds, s, a);
-        else hipLaunchKernelGGL((k_coulomb<double, 1>), dim3(a.n_cells), dim3(64), lds, s, a);
+        if (fp32_velocity) launch_coulomb_t<float, 1>(a, s);
+        else launch_coulomb_t<double, 1>(a, s);
     } else {
-        if (fp32_velocity) hipLaunchKernelGGL((k_coulomb<float, 0>), dim3(a.n_cells), dim3(64), lds, s, a);
-        else hipLaunchKernelGGL((k_coulomb<double, 0>), dim3(a.n_cells), dim3(64), lds, s, a);
+        if (fp32_velocity) launch_coulomb_t<float, 0>(a, s);
+        else launch_coulomb_t<double, 0>(a, s);
     }
     return hipGetLastError();
 }

@@ -208,19 +208,12 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
 
 }  // namespace
 
+// the candidates of one cell by one wave; maxp = LDS slots for the cell's electrons
 template <class UT>
-__global__ void __launch_bounds__(64) k_mcc(MccArgs<UT> a) {
-    extern __shared__ uint64_t lds[];
-    const int maxp = a.maxp;
+__device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds, int* picks) {
     uint64_t* key = lds;
     int* val = (int*)(lds + maxp);
-    __shared__ int picks[64];
-    const int cell = blockIdx.x;
     const int a0 = a.start[cell], eC = a.start[cell + 1] - a0;
-    if (eC > maxp) {
-        if (threadIdx.x == 0) atomicOr(a.flags, kFlagCellOverflow);
-        return;
-    }
     const double smax = a.smax[cell];
     const double coll = a.rem[cell] + eC * a.nN * smax * a.dt;
     const int nc = (int)coll;
@@ -277,11 +270,43 @@ __global__ void __launch_bounds__(64) k_mcc(MccArgs<UT> a) {
     }
 }
 
+// one wave per cell; cells with more electrons than a.maxp go to the large-cell pass
+template <class UT>
+__global__ void __launch_bounds__(64) k_mcc(MccArgs<UT> a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ int picks[64];
+    const int cell = blockIdx.x;
+    if (a.start[cell + 1] - a.start[cell] > a.maxp) {
+        if (threadIdx.x == 0) a.big_list[atomicAdd(a.big_n, 1)] = cell;
+        return;
+    }
+    mcc_cell(a, cell, a.maxp, lds, picks);
+}
+
+template <class UT>
+__global__ void __launch_bounds__(64) k_mcc_big(MccArgs<UT> a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ int picks[64];
+    const int nbig = *a.big_n;
+    for (int i = blockIdx.x; i < nbig; i += gridDim.x) {
+        const int cell = a.big_list[i];
+        if (a.start[cell + 1] - a.start[cell] > a.maxp_big) {
+            if (threadIdx.x == 0) atomicOr(a.flags, kFlagCellOverflow);
+            continue;
+        }
+        mcc_cell(a, cell, a.maxp_big, lds, picks);
+        __syncthreads();
+    }
+}
+
 template <class UT>
 static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
     if (a.n_cells <= 0) return hipSuccess;
     const size_t lds = (size_t)a.maxp * (sizeof(uint64_t) + sizeof(int));
     hipLaunchKernelGGL((k_mcc<UT>), dim3(a.n_cells), dim3(64), lds, s, a);
+    const size_t lds_big = (size_t)a.maxp_big * (sizeof(uint64_t) + sizeof(int));
+    const int nb = a.n_cells < 512 ? a.n_cells : 512;
+    hipLaunchKernelGGL((k_mcc_big<UT>), dim3(nb), dim3(64), lds_big, s, a);
     return hipGetLastError();
 }
 hipError_t launch_mcc(const MccArgs<double>& a, hipStream_t s) { return launch_mcc_t(a, s); }

@@ -169,12 +169,13 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
 template <class UT, int P, int W, int DEP>
 __global__ void __launch_bounds__(256) k_move(MoveArgs<UT> a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = a.n_live ? min(*a.n_live, a.n) : a.n;
     int newcell = -1;
     double y1 = 0, y2 = 0, y3 = 0;
     int t = -1;
     int status = 0;
     bool deleted = false;
-    if (i < a.n) {
+    if (i < n) {
         t = a.p.tet[i];
         if (t >= 0) {
             y1 = a.p.y1[i]; y2 = a.p.y2[i]; y3 = a.p.y3[i];
@@ -228,7 +229,7 @@ __global__ void __launch_bounds__(256) k_move(MoveArgs<UT> a) {
 template <class UT, int P, int W>
 __global__ void __launch_bounds__(256) k_push(MoveArgs<UT> a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    if (i >= (a.n_live ? min(*a.n_live, a.n) : a.n)) return;
     const int t = a.p.tet[i];
     if (t < 0) return;
     const double y1 = a.p.y1[i], y2 = a.p.y2[i], y3 = a.p.y3[i];
@@ -248,9 +249,10 @@ __global__ void __launch_bounds__(256) k_push(MoveArgs<UT> a) {
 template <class UT, int DEP>
 __global__ void __launch_bounds__(256) k_deposit(MoveArgs<UT> a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = a.n_live ? min(*a.n_live, a.n) : a.n;
     int cell = -1, t = -1;
     double y1 = 0, y2 = 0, y3 = 0;
-    if (i < a.n) {
+    if (i < n) {
         t = a.p.tet[i];
         if (t >= 0) { cell = a.tets[t].cell; y1 = a.p.y1[i]; y2 = a.p.y2[i]; y3 = a.p.y3[i]; }
     }
@@ -356,10 +358,12 @@ __global__ void k_init(InitArgs<UT> a) {
 // start[cell] + slot; the slot comes from a wave-aggregated atomic, so the order inside a
 // cell is not fixed; every consumer that depends on it orders the cell's parcels by id.
 template <class UT>
-__global__ void __launch_bounds__(256) k_scatter(PartSoA<UT> src, PartSoA<UT> dst, int64_t n,
-                                                  const TetGeo* __restrict__ tets, const int* __restrict__ start,
-                                                  int* __restrict__ cursor) {
+__global__ void __launch_bounds__(256) k_scatter(PartSoA<UT> src, PartSoA<UT> dst, int64_t n_bound,
+                                                  const int64_t* __restrict__ n_live, const TetGeo* __restrict__ tets,
+                                                  const int* __restrict__ start, int* __restrict__ cursor) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the device count can exceed the arrays after a capacity overflow (flagged)
+    const int64_t n = min(*n_live, n_bound);
     int cell = -1, t = -1;
     if (i < n) {
         t = src.tet[i];
@@ -478,20 +482,20 @@ hipError_t launch_init(const InitArgs<double>& a, hipStream_t s) { return launch
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s) { return launch_init_t(a, s); }
 
 template <class UT>
-static hipError_t launch_scatter_t(const PartSoA<UT>& src, const PartSoA<UT>& dst, int64_t n, const TetGeo* tets,
-                                   const int* start, int* cursor, hipStream_t s) {
+static hipError_t launch_scatter_t(const PartSoA<UT>& src, const PartSoA<UT>& dst, int64_t n, const int64_t* n_live,
+                                   const TetGeo* tets, const int* start, int* cursor, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, tets, start,
-                       cursor);
+    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, n_live, tets,
+                       start, cursor);
     return hipGetLastError();
 }
-hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const TetGeo* tets,
-                          const int* start, int* cursor, hipStream_t s) {
-    return launch_scatter_t(src, dst, n, tets, start, cursor, s);
+hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int64_t* n_live,
+                          const TetGeo* tets, const int* start, int* cursor, hipStream_t s) {
+    return launch_scatter_t(src, dst, n, n_live, tets, start, cursor, s);
 }
-hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const TetGeo* tets,
-                          const int* start, int* cursor, hipStream_t s) {
-    return launch_scatter_t(src, dst, n, tets, start, cursor, s);
+hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int64_t* n_live,
+                          const TetGeo* tets, const int* start, int* cursor, hipStream_t s) {
+    return launch_scatter_t(src, dst, n, n_live, tets, start, cursor, s);
 }
 
 }  // namespace picmi

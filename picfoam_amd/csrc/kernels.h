@@ -7,7 +7,8 @@ namespace picmi {
 template <class UT>
 struct MoveArgs {
     PartSoA<UT> p;
-    int64_t n;
+    int64_t n;                       // launch bound (>= the live count)
+    const int64_t* n_live;           // live parcel count on the device (null: n is exact)
     const TetGeo* tets;
     const UT* Ecell;
     const UT* Epoint;
@@ -64,17 +65,18 @@ hipError_t launch_positions(const PartSoA<float>& p, int64_t n, const TetGeo* te
                             const double* pts, double* x, hipStream_t s);
 hipError_t launch_init(const InitArgs<double>& a, hipStream_t s);
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s);
-hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const TetGeo* tets,
-                          const int* start, int* cursor, hipStream_t s);
-hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const TetGeo* tets,
-                          const int* start, int* cursor, hipStream_t s);
+// n: launch bound; n_live: live source count on the device
+hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int64_t* n_live,
+                          const TetGeo* tets, const int* start, int* cursor, hipStream_t s);
+hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int64_t* n_live,
+                          const TetGeo* tets, const int* start, int* cursor, hipStream_t s);
 
 // ---------------------------------------------------------------- fields (k_field.hip)
 // exclusive scan of n+1 ints (last = total); temp storage sized by scan_temp_bytes
 size_t scan_temp_bytes(int n);
 hipError_t launch_scan(const int* in, int* out, int n, void* temp, size_t temp_bytes, hipStream_t s);
-// max over n ints into *out (single block reduce + atomicMax)
-hipError_t launch_max_count(const int* counts, int n, int* out, hipStream_t s);
+// *dst = *src (the live count after a sort: start[n_cells] -> the species count)
+hipError_t launch_store_count(const int* src, int64_t* dst, hipStream_t s);
 
 struct DepositFinishArgs {
     int n_cells;
@@ -149,7 +151,10 @@ struct CoulombArgs {
     double dt;
     uint64_t seed;
     int64_t step;
-    int maxp;                         // LDS capacity per species per cell (power of two >= 64)
+    int maxp;                         // LDS index-list slots of the per-cell pass (power of two)
+    int maxp_big;                     // slots of the large-cell pass (power of two)
+    int* big_list;                    // cells left to the large-cell pass, [n_cells]
+    int* big_n;                       // their number (zero before the launch)
     int model;                        // 0 Nanbu, 1 Perez
     int pairing;                      // 0 TakizukaAbe, 1 NanbuYonemura
     int weight_correction;            // Nanbu weight correction active (unequal weights)
@@ -182,7 +187,10 @@ struct MccArgs {
     double dt;
     uint64_t seed;
     int64_t step;
-    int maxp;
+    int maxp;                     // LDS slots (electrons of a cell) of the per-cell pass
+    int maxp_big;                 // slots of the large-cell pass
+    int* big_list;                // cells left to the large-cell pass, [n_cells]
+    int* big_n;                   // their number (zero before the launch)
     uint32_t* flags;
     unsigned long long* counts;   // candidates, elastic, excitation, ionization
     // deposit of created parcels

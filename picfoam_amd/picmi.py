@@ -37,6 +37,7 @@ EXPORTS = (
     "pic_mcc_initialize", "pic_particles_set", "pic_particles_set_positions", "pic_particles_count",
     "pic_particles_get", "pic_particles_positions", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
     "pic_collide_coulomb", "pic_collide_mcc", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
+    "pic_run",
     "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
     "pic_profile_reset", "pic_gnn_init", "pic_gnn_n_params", "pic_gnn_params", "pic_gnn_record",
     "pic_gnn_snapshot_count", "pic_gnn_snapshot", "pic_gnn_grad", "pic_gnn_train", "pic_gnn_eval", "pic_gnn_use",
@@ -140,6 +141,7 @@ def load_library(path: str = LIB_PATH):
         "pic_poisson_solve": (C.c_int, [vp, C.POINTER(pic_poisson_stats)]),
         "pic_diag_reduce": (C.c_int, [vp, C.POINTER(pic_diag)]),
         "pic_step": (C.c_int, [vp, dbl, C.POINTER(pic_step_stats)]),
+        "pic_run": (C.c_int, [vp, dbl, i64, C.POINTER(pic_step_stats)]),
         "pic_sync": (C.c_int, [vp]),
         "pic_step_index": (i64, [vp]),
         "pic_set_step_index": (C.c_int, [vp, i64]),
@@ -396,6 +398,12 @@ class Cloud:
     def step(self, dt):
         st = pic_step_stats()
         self._check(self.lib.pic_step(self.h, dt, C.byref(st)))
+        return st
+
+    def run(self, dt, n_steps):
+        """n_steps steps with one synchronisation at the end; counters summed over them"""
+        st = pic_step_stats()
+        self._check(self.lib.pic_run(self.h, dt, int(n_steps), C.byref(st)))
         return st
 
     def diag(self):

@@ -226,6 +226,7 @@ def test_coulomb_fp64_parity(oracle, counts, variant):
 @pytest.mark.parametrize("model,xs_model", [("NanbuElectronNeutral", "List"), ("RelativisticElectronNeutral", "List"),
                                             ("NanbuElectronNeutral", "FixedValue")])
 def test_mcc_fp64_parity(oracle, model, xs_model):
+    # large-cell pass exercised by one cell of 700 electrons
     say("mcc parity", model, xs_model)
     rng = np.random.default_rng(105)
     from picfoam_amd import cases
@@ -255,6 +256,7 @@ def test_mcc_fp64_parity(oracle, model, xs_model):
             c.set_cross_section(k, e, s_, th)
     c.mcc_initialize(1000.0, 0.01)
     ne = rng.integers(50, 150, nc)
+    ne[2] = 700     # above 512 electrons: the large-cell pass
     (ye, te, Ue, ie), _ = _two_species_cell_setup(rng, m, rm, ne, np.zeros(nc, int), Te=30.0)
     c.set_particles(0, ye, te, Ue, ie)
     c.step_index = 3
@@ -372,3 +374,45 @@ def test_superluminal_is_an_error():
     with pytest.raises(picmi.PicError) as ei:
         c.move(1e-6)
     assert ei.value.code == -3
+
+
+@pytest.mark.parametrize("case", ["thermalisation", "mcc"])
+def test_run_equals_repeated_steps(case):
+    """pic_run (one synchronisation for K steps, device-side parcel counts) gives the same
+    state, bit for bit, as K synchronous pic_step calls."""
+    say("run vs steps", case)
+    from picfoam_amd import cases, picmi
+
+    def make():
+        if case == "thermalisation":
+            mesh, species, models, ex = cases.thermalisation(ncell=4, ppc=40)
+            c = picmi.Cloud(mesh, species, models)
+            c.init_maxwellian(0, 40, ex["Te_K"], id0=0, seed=11)
+            c.init_maxwellian(1, 40, ex["Ti_K"], id0=0, seed=12)
+            return c, ex["dt"]
+        mesh, species, models, ex = cases.argon_mcc(ncell=4, ppc=60, precision="fp64", L=1e-3, growth=20.0)
+        models["seed"] = 3
+        c = picmi.Cloud(mesh, species, models)
+        e, el, exs, iz = ex["xs"]
+        for k, (s_, th) in enumerate(((el, 0.0), (exs, 11.5), (iz, 15.76))):
+            c.set_cross_section(k, e, s_, th)
+        c.mcc_initialize(1000.0, 0.01)
+        c.set_field(picmi.FIELD_E, ex["E"])
+        c.init_maxwellian(0, 60, ex["T_K"], seed=3)
+        return c, ex["dt"]
+
+    K = 6
+    a, dt = make()
+    st_steps = [a.step(dt) for _ in range(K)]
+    b, _ = make()
+    st_run = b.run(dt, K)
+    for s in range(len(a.species)):
+        ya, ta, Ua, ia = a.get_particles(s)
+        yb, tb, Ub, ib = b.get_particles(s)
+        assert len(ia) == len(ib)
+        oa, ob = np.argsort(ia, kind="stable"), np.argsort(ib, kind="stable")
+        assert np.array_equal(ia[oa], ib[ob])
+        assert np.array_equal(Ua[oa], Ub[ob]) and np.array_equal(ya[oa], yb[ob]) and np.array_equal(ta[oa], tb[ob])
+    assert st_run.coulomb_collisions == sum(x.coulomb_collisions for x in st_steps)
+    assert st_run.mcc_candidates == sum(x.mcc_candidates for x in st_steps)
+    assert list(st_run.n_parcels) == list(st_steps[-1].n_parcels)
