@@ -62,10 +62,23 @@ struct StageK {
     const int* permQ;
     double mP, qP, mQ, qQ, wP, wQ;
     double nP, nQ, nPQ, lnL;
+    // Nanbu constants: cst = (qP qQ / eps0 mu)^2 (nP nQ / nPQ) dt / (4 pi), fP = mQ/(mP+mQ),
+    // fQ = mP/(mP+mQ), b0c = qP qQ / (2 pi eps0 mu), hbc = h / (2 mu)
+    double cst, fP, fQ, b0c, hbc;
     int kind, n, units, stage;
     int g1_units, g1_lmax, g1_stride, g1_xo, g1_yo, g1_ko;
     int g2_lmax, g2_stride, g2_xo, g2_yo, g2_ko;
 };
+
+__device__ __forceinline__ void finish_stage(StageK& d, double dt) {
+    const double mu = d.mP * d.mQ / (d.mP + d.mQ);
+    const double ad = d.qP * d.qQ / (kEps0 * mu);
+    d.cst = 1.0 / (4.0 * kPi) * (ad * ad) * (d.nQ * d.nP / d.nPQ) * dt;
+    d.fP = d.mQ / (d.mP + d.mQ);
+    d.fQ = d.mP / (d.mP + d.mQ);
+    d.b0c = d.qP * d.qQ / (2.0 * kPi * kEps0) / mu;
+    d.hbc = kH / (2.0 * mu);
+}
 
 // collision t of unit u; false once the unit is exhausted
 __device__ __forceinline__ bool decode(const StageK& d, int u, int t, int& iP, int& iQ, int& k) {
@@ -112,19 +125,17 @@ __device__ __forceinline__ bool decode(const StageK& d, int u, int t, int& iP, i
 // NanbuCollision::collide (NanbuCollision.C:60-134) on two velocities, arithmetic M
 template <class M>
 __device__ __forceinline__ void nanbu_scatter(vec3<M>& uP, vec3<M>& uQ, const StageK& d, int calc_bmin, double debye,
-                                              double dt, double u1, double u2) {
-    const double mP = d.mP, mQ = d.mQ;
-    const double reMass_d = mP * mQ / (mP + mQ);
-    const M reMass = (M)reMass_d;
+                                              double u1, double u2) {
     const vec3<M> g = uP - uQ;
     const M gm = mag(g);
     const M gp = sqrt(g.y * g.y + g.z * g.z);
     if (!(gm > (M)0) || !(gp > (M)0)) return;  // the reference divides by zero here
     M lnL;
     if (d.lnL == 0.0) {
-        M b0 = (M)(d.qP * d.qQ / (2.0 * kPi * kEps0)) / (reMass * gm * gm);
+        const M ig2 = (M)1 / (gm * gm);
+        M b0 = (M)d.b0c * ig2;
         if (calc_bmin) {
-            const M hb = (M)kH / ((M)2 * (gm * reMass));
+            const M hb = (M)d.hbc / gm;
             b0 = hb > fabs(b0) ? hb : fabs(b0);
         }
         const M l = (M)0.5 * log((M)1 + ((M)debye * (M)debye) / (b0 * b0));
@@ -132,10 +143,8 @@ __device__ __forceinline__ void nanbu_scatter(vec3<M>& uP, vec3<M>& uQ, const St
     } else {
         lnL = (M)d.lnL;
     }
-    const double ad = d.qP * d.qQ / (kEps0 * reMass_d);
-    // s = lnL/(4 pi) (qP qQ / eps0 mu)^2 (nP nQ / nPQ) g^-3 dt, the constant part in FP64
-    const M cst = (M)(1.0 / (4.0 * kPi) * (ad * ad) * (d.nQ * d.nP / d.nPQ) * dt);
-    const M s = lnL * cst * ((M)1 / (gm * gm * gm));
+    // s = lnL/(4 pi) (qP qQ / eps0 mu)^2 (nP nQ / nPQ) g^-3 dt
+    const M s = lnL * (M)d.cst * ((M)1 / (gm * gm * gm));
     const M cc = cos_chi<M>(s, (M)u1);
     const M sc = sqrt((M)1 - cc * cc);
     M se, ce;
@@ -143,8 +152,8 @@ __device__ __forceinline__ void nanbu_scatter(vec3<M>& uP, vec3<M>& uQ, const St
     const M igp = (M)1 / gp;
     const vec3<M> h = mk3<M>(gp * ce, -(g.y * g.x * ce + gm * g.z * se) * igp, -(g.z * g.x * ce - gm * g.y * se) * igp);
     const vec3<M> dv = ((M)1 - cc) * g + sc * h;
-    uP = uP - (M)(mQ / (mP + mQ)) * dv;
-    uQ = uQ + (M)(mP / (mP + mQ)) * dv;
+    uP = uP - (M)d.fP * dv;
+    uQ = uQ + (M)d.fQ * dv;
 }
 
 // PerezRelativisticCollision::collide (PerezRelativisticCollision.C:62-196), FP64
@@ -184,8 +193,9 @@ __device__ __forceinline__ void perez_scatter(vec3<double>& UP, vec3<double>& UQ
     com_leave(f, mP, mQ, cc, sc, se, ce, UP, UQ);
 }
 
-// one collision of parcel iP (species of d's P side) with parcel iQ; MODEL 0 Nanbu, 1 Perez
-template <class UT, int MODEL>
+// one collision of parcel iP (species of d's P side) with parcel iQ; MODEL 0 Nanbu, 1 Perez;
+// WC: Nanbu weight correction active
+template <class UT, int MODEL, bool WC>
 __device__ __forceinline__ void collide(const CoulombArgs& a, const StageK& d, int cell, int iP, int iQ, int k,
                                         double debye) {
     uint4 o = rng4(a.seed, (uint32_t)cell, (uint32_t)a.step, ((uint32_t)d.stage << 24) | (uint32_t)k,
@@ -202,9 +212,9 @@ __device__ __forceinline__ void collide(const CoulombArgs& a, const StageK& d, i
         nP = mk3<UT>((UT)dP.x, (UT)dP.y, (UT)dP.z);
         nQ = mk3<UT>((UT)dQ.x, (UT)dQ.y, (UT)dQ.z);
     } else {
-        nanbu_scatter<UT>(nP, nQ, d, a.calculate_bmin, debye, a.dt, u1, u2);
+        nanbu_scatter<UT>(nP, nQ, d, a.calculate_bmin, debye, u1, u2);
     }
-    if (a.weight_correction) {
+    if (WC) {
         // NanbuWeightCorrectionModel::correctVelocity
         o = rng4(a.seed, (uint32_t)cell, (uint32_t)a.step, ((uint32_t)d.stage << 24) | (uint32_t)k,
                  (kTagCoulomb << 24) | 1u);
@@ -252,17 +262,31 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
         for (int k = threadIdx.x; k < n8; k += blockDim.x)
             key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k : ~0ull;
         __syncthreads();
-        for (int k0 = 0; k0 < n; k0 += blockDim.x) {
-            const int k = k0 + (int)threadIdx.x;
-            const uint64_t kk = k < n ? key[k] : 0ull;
+        // rounds of up to 64 keys; a partial round of E keys gives each key G = 64 / E lanes
+        // (rounded down to a power of two) that count over G slices of the list and add up
+        for (int k0 = 0; k0 < n; k0 += 64) {
+            const int E = n - k0 < 64 ? n - k0 : 64;
+            int lg = 0;
+            while ((E << (lg + 1)) <= 64) ++lg;
+            const int lane = (int)threadIdx.x;
+            const int e = lane >> lg, g = lane & ((1 << lg) - 1);
+            const int k = k0 + e;
+            const bool act = e < E;
+            const uint64_t kk = act ? key[k] : 0ull;
+            // slice length: ceil(n8 / G) rounded up to whole 8-key reads, so the G slices
+            // cover the list (G * seg >= n8) also when n8 < G
+            const int seg = ((((n8 + (1 << lg) - 1) >> lg) + 7) & ~7);
+            const int j0 = g * seg;
+            const int j1 = j0 + seg < n8 ? j0 + seg : n8;
             int r = 0;
-            for (int j = 0; j < n8; j += 8) {
+            for (int j = j0; j < j1; j += 8) {
                 const uint64_t* q = key + j;
                 const uint64_t x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3], x4 = q[4], x5 = q[5], x6 = q[6], x7 = q[7];
                 r += (int)(x0 < kk) + (int)(x1 < kk) + (int)(x2 < kk) + (int)(x3 < kk) + (int)(x4 < kk) +
                      (int)(x5 < kk) + (int)(x6 < kk) + (int)(x7 < kk);
             }
-            if (k < n) val[r] = first + k;
+            for (int off = (1 << lg) >> 1; off > 0; off >>= 1) r += __shfl_xor(r, off);
+            if (act && g == 0) val[r] = first + k;
         }
         __syncthreads();
         return;
@@ -298,7 +322,7 @@ __device__ __forceinline__ int cell_max_count(const CoulombArgs& a, int cell) {
 }
 
 // every pairing stage of one cell by one wave; maxp = LDS slots per index list
-template <class UT, int MODEL>
+template <class UT, int MODEL, bool WC>
 __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t* lds, StageK* sk) {
     const int kstride = maxp + 8;
     uint64_t* keyA = lds;
@@ -402,6 +426,7 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
                             d.kind = kIntraOddNY; d.units = (cnt - 1) / 2;
                             d.nPQ = 2.0 * a.weight[spc] * ((cnt - 1) / 2 + 1) / Vc;
                         }
+                        finish_stage(d, a.dt);
                     }
                     ++nd;
                 }
@@ -464,6 +489,7 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
                                 d.units = d.g1_units + (nMaxG2 > 0 ? cyc : 0);
                             }
                         }
+                        finish_stage(d, a.dt);
                     }
                     nd = 1;
                 }
@@ -477,7 +503,7 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
                 const int uu = u < units0 ? u : u - units0;
                 int iP, iQ, k;
                 for (int t = 0; decode(d, uu, t, iP, iQ, k); ++t) {
-                    collide<UT, MODEL>(a, d, cell, iP, iQ, k, debye);
+                    collide<UT, MODEL, WC>(a, d, cell, iP, iQ, k, debye);
                     ++ncoll;
                 }
             }
@@ -490,8 +516,8 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
 
 // one wave per cell with small index lists in LDS; cells with more parcels of a species
 // than a.maxp go to the list of the large-cell pass
-template <class UT, int MODEL>
-__global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
+template <class UT, int MODEL, bool WC>
+__device__ __forceinline__ void coulomb_pass(const CoulombArgs& a) {
     extern __shared__ uint64_t lds[];
     __shared__ StageK sk[2];
     const int cell = blockIdx.x;
@@ -499,12 +525,23 @@ __global__ void __launch_bounds__(64) k_coulomb(CoulombArgs a) {
         if (threadIdx.x == 0) a.big_list[atomicAdd(a.big_n, 1)] = cell;
         return;
     }
-    coulomb_cell<UT, MODEL>(a, cell, a.maxp, lds, sk);
+    coulomb_cell<UT, MODEL, WC>(a, cell, a.maxp, lds, sk);
+}
+
+// Nanbu scattering fits three waves per SIMD without spilling; Perez keeps the default
+template <class UT, bool WC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_coulomb_nanbu(CoulombArgs a) {
+    coulomb_pass<UT, 0, WC>(a);
+}
+
+template <class UT, bool WC>
+__global__ void __launch_bounds__(64) k_coulomb_perez(CoulombArgs a) {
+    coulomb_pass<UT, 1, WC>(a);
 }
 
 // the listed large cells, up to a.maxp_big parcels of a species; larger cells raise the
 // overflow flag and are left alone
-template <class UT, int MODEL>
+template <class UT, int MODEL, bool WC>
 __global__ void __launch_bounds__(64) k_coulomb_big(CoulombArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ StageK sk[2];
@@ -515,7 +552,7 @@ __global__ void __launch_bounds__(64) k_coulomb_big(CoulombArgs a) {
             if (threadIdx.x == 0) atomicOr(a.flags, kFlagCellOverflow);
             continue;
         }
-        coulomb_cell<UT, MODEL>(a, cell, a.maxp_big, lds, sk);
+        coulomb_cell<UT, MODEL, WC>(a, cell, a.maxp_big, lds, sk);
         __syncthreads();
     }
 }
@@ -524,21 +561,30 @@ size_t coulomb_lds_bytes(int maxp) {
     return (size_t)(2 * (maxp + 8)) * sizeof(uint64_t) + (size_t)(2 * maxp) * sizeof(int);
 }
 
-template <class UT, int MODEL>
+template <class UT, int MODEL, bool WC>
 static void launch_coulomb_t(const CoulombArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((k_coulomb<UT, MODEL>), dim3(a.n_cells), dim3(64), coulomb_lds_bytes(a.maxp), s, a);
+    if (MODEL == 1)
+        hipLaunchKernelGGL((k_coulomb_perez<UT, WC>), dim3(a.n_cells), dim3(64), coulomb_lds_bytes(a.maxp), s, a);
+    else
+        hipLaunchKernelGGL((k_coulomb_nanbu<UT, WC>), dim3(a.n_cells), dim3(64), coulomb_lds_bytes(a.maxp), s, a);
     const int nb = a.n_cells < 512 ? a.n_cells : 512;
-    hipLaunchKernelGGL((k_coulomb_big<UT, MODEL>), dim3(nb), dim3(64), coulomb_lds_bytes(a.maxp_big), s, a);
+    hipLaunchKernelGGL((k_coulomb_big<UT, MODEL, WC>), dim3(nb), dim3(64), coulomb_lds_bytes(a.maxp_big), s, a);
+}
+
+template <class UT, int MODEL>
+static void launch_coulomb_m(const CoulombArgs& a, hipStream_t s) {
+    if (a.weight_correction) launch_coulomb_t<UT, MODEL, true>(a, s);
+    else launch_coulomb_t<UT, MODEL, false>(a, s);
 }
 
 hipError_t launch_coulomb(const CoulombArgs& a, bool fp32_velocity, hipStream_t s) {
     if (a.n_cells <= 0) return hipSuccess;
     if (a.model == 1) {
-        if (fp32_velocity) launch_coulomb_t<float, 1>(a, s);
-        else launch_coulomb_t<double, 1>(a, s);
+        if (fp32_velocity) launch_coulomb_m<float, 1>(a, s);
+        else launch_coulomb_m<double, 1>(a, s);
     } else {
-        if (fp32_velocity) launch_coulomb_t<float, 0>(a, s);
-        else launch_coulomb_t<double, 0>(a, s);
+        if (fp32_velocity) launch_coulomb_m<float, 0>(a, s);
+        else launch_coulomb_m<double, 0>(a, s);
     }
     return hipGetLastError();
 }

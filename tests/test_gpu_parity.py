@@ -416,3 +416,42 @@ def test_run_equals_repeated_steps(case):
     assert st_run.coulomb_collisions == sum(x.coulomb_collisions for x in st_steps)
     assert st_run.mcc_candidates == sum(x.mcc_candidates for x in st_steps)
     assert list(st_run.n_parcels) == list(st_steps[-1].n_parcels)
+
+
+def test_coulomb_shuffle_every_list_length(oracle):
+    """Regression for the ranked shuffle: lists of every length 1..20, around one and two
+    64-key rounds, and up to the 256-key limit (cells with few keys spread each key over
+    several lanes; a slicing error there left index slots unset)."""
+    say("coulomb shuffle lengths")
+    rng = np.random.default_rng(107)
+    m = small_hex(n=(5, 5, 3), L=(1e-4, 1e-4, 1e-4))
+    rm = oracle.RefMesh(m)
+    nc = rm.n_cells
+    lengths = list(range(1, 21)) + list(range(60, 70)) + list(range(124, 136)) + list(range(250, 257))
+    lengths = (lengths + lengths[::-1])[:nc]
+    ne = np.array(lengths[:nc])
+    ni = np.array(lengths[::-1][:nc])
+    w = 1e6
+    mi = 1836 * ME
+    species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": w, "capacity": 40000},
+               {"mass": mi, "charge": QE, "nEquivalentParticles": w, "capacity": 40000}]
+    c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu", "coulombLog": {(0, 0): 10.0, (0, 1): 10.0,
+                                                                                         (1, 1): 10.0}, "seed": 5},
+              species)
+    (ye, te, Ue, ie), (yi, ti, Ui, ii) = _two_species_cell_setup(rng, m, rm, ne, ni, mi=mi)
+    c.set_particles(0, ye, te, Ue, ie)
+    c.set_particles(1, yi, ti, Ui, ii)
+    c.step_index = 2
+    dt = 1e-9
+    c.collide_coulomb(dt)
+    _, _, Ueg, ieg = c.get_particles(0)
+    _, _, Uig, iig = c.get_particles(1)
+    cse = np.concatenate([[0], np.cumsum(ne)]).astype(np.int32)
+    csi = np.concatenate([[0], np.cumsum(ni)]).astype(np.int32)
+    clm = np.zeros((8, 8)); clm[:2, :2] = 10.0
+    Us, st = oracle.coulomb(rm, [(ME, -QE, w), (mi, QE, w)], [Ue, Ui], [ie, ii], [cse, csi], dt, seed=5, step=2,
+                            coulomb_log=clm)
+    (Ueg_s,) = sort_by_id(ieg, Ueg)
+    (Uig_s,) = sort_by_id(iig, Uig)
+    assert rel_err(Ueg_s, Us[0]) < TOL
+    assert rel_err(Uig_s, Us[1]) < TOL
