@@ -144,6 +144,52 @@ def run_steps(args, precision, steps, warmup, profile, sync_steps=0):
     return n_total, t1 - t0, stages, dt, sync_ms
 
 
+def poisson_benchmark(ncell, ppc, snaps, train_steps, evals, warm, loss_kind=0, guesses=("previous", "extrapolate")):
+    """Poisson ms per step (right-hand side, initial guess, PCG to ||r||/||b|| <= 1e-8, gradient)
+    on the field-solve workload, for four initial guesses: the previous phi (the reference's
+    warm start), the learned guess on top of it, the linear extrapolation 2 phi(n-1) -
+    phi(n-2), and the learned guess on top of that.  The network (3 message-passing layers,
+    width 64, BF16 MFMA) is trained on the snapshots the same run records."""
+    from picfoam_amd import cases, picmi
+    out = {}
+    for guess in guesses:
+        mesh, species, models, ex = cases.poisson_plasma(ncell=ncell, ppc=ppc, guess=guess)
+        c = picmi.Cloud(mesh, species, models, ex["bc"])
+        c.set_field(picmi.FIELD_SPACE_CHARGE, np.full(ex["n_cells"], ex["space_charge"]))
+        c.init_maxwellian(0, ppc, ex["T_K"], seed=21)
+        dt = ex["dt"]
+        for _ in range(warm):
+            c.step(dt)
+        c.gnn_init(layers=3, width=64, seed=5)
+        c.gnn_loss(loss_kind)
+        c.gnn_record(snaps)
+        base = [c.step(dt).poisson for _ in range(snaps)]
+        t0 = time.perf_counter()
+        first, last = c.gnn_train(train_steps, lr=2e-3, seed=2)
+        t_train = time.perf_counter() - t0
+        c.gnn_record(0)
+        c.gnn_use(True)
+        learned = [c.step(dt).poisson for _ in range(evals)]
+        c.gnn_use(False)
+        after = [c.step(dt).poisson for _ in range(evals)]
+        c.close()
+        ms_b = float(np.mean([p.ms for p in base] + [p.ms for p in after]))
+        it_b = float(np.mean([p.iterations for p in base] + [p.iterations for p in after]))
+        ms_l = float(np.mean([p.ms for p in learned]))
+        it_l = float(np.mean([p.iterations for p in learned]))
+        out[guess] = {"ms": round(ms_b, 3), "iterations": round(it_b, 2)}
+        out[guess + "+learned"] = {"ms": round(ms_l, 3), "iterations": round(it_l, 2),
+                                   "train_loss_first_last": [round(first, 4), round(last, 4)],
+                                   "train_s": round(t_train, 2)}
+    if "previous" in out:
+        out["learned_vs_previous_ms_ratio"] = round(out["previous+learned"]["ms"] / out["previous"]["ms"], 3)
+    out["training_loss"] = ("A-norm error", "residual norm", "A-norm + residual")[loss_kind]
+    out["config"] = (f"{ncell}^3 hex, {ncell ** 3 * ppc} electrons + neutralising background, phi = 0 on x walls, "
+                     f"cyclic y/z, PCG+DIC FP64 to 1e-8; {snaps} snapshots, {train_steps} Adam steps, "
+                     f"{evals} evaluated steps")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +201,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
+    ap.add_argument("--no-poisson", action="store_true", help="skip the Poisson initial-guess comparison")
+    ap.add_argument("--poisson-ncell", type=int, default=100)
+    ap.add_argument("--poisson-ppc", type=int, default=8)
+    ap.add_argument("--poisson-loss", type=int, default=0, choices=[0, 1, 2],
+                    help="training loss of the learned guess: 0 A-norm error, 1 residual norm, 2 both")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device three times (a plain copy of known size whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic figures)")
@@ -205,6 +256,10 @@ def main():
         n2, el2, _, _, _ = run_steps(args, "mixed", args.steps, args.warmup, False)
         out["mixed_precision"] = {"value": n2 * args.steps / el2, "ms_per_step": 1e3 * el2 / args.steps,
                                   "speedup_vs_fp64": (n2 * args.steps / el2) / value}
+    if not args.no_poisson:
+        print("bench: Poisson initial-guess comparison", file=sys.stderr, flush=True)
+        out["poisson"] = poisson_benchmark(args.poisson_ncell, args.poisson_ppc, snaps=48, train_steps=400, evals=24,
+                                           warm=4, loss_kind=args.poisson_loss)
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(target_s=args.cpu_seconds)

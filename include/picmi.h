@@ -118,6 +118,7 @@ typedef struct {
     int32_t poisson_max_iter;
     int32_t poisson_mode;             /* PIC_POISSON_* */
     int32_t sort_interval;            /* sort every k steps when no collision model needs it */
+    int32_t poisson_guess;            /* PCG start: 0 the previous phi, 1 2 phi(n-1) - phi(n-2) */
 } pic_models;
 
 typedef struct {

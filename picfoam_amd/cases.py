@@ -148,3 +148,20 @@ def argon_mcc(ncell=100, ppc=100, T_eV=2.0, Ex=5e4, nN=1e21, TN=300.0, L=None, p
     extra = {"E": np.tile([Ex, 0.0, 0.0], (ncell ** 3, 1)), "dt": dt, "xs": (e, el, ex, iz), "T_K": T_eV * EV_K,
              "ppc": ppc, "vmax": vmax}
     return mesh, species, models, extra
+
+
+def poisson_plasma(ncell=100, ppc=8, T_eV=2.0, density=1e16, L=0.01, wp_dt=0.1, guess="previous", tol=1e-8):
+    """Field-solve workload on the MCC mesh: ncell^3 hex box of side L, electrons (ppc per
+    cell, Maxwellian T_eV) over a neutralising ion background given as spaceCharge, phi = 0
+    on the two x walls (electrons leaving there are deleted), cyclic in y and z.
+    ElectroStatic PCG+DIC to ||r||/||b|| <= tol every step; dt = wp_dt / omega_p."""
+    mesh = meshgen.hex_box(ncell, ncell, ncell, L, L, L, cyclic=(False, True, True))
+    ncells = ncell ** 3
+    w = density * L ** 3 / (ncells * ppc)
+    species = [{"name": "e", "mass": ME, "charge": -QE, "nEquivalentParticles": w, "capacity": ncells * ppc}]
+    models = {"ParticlePusher": "Boris", "MaxwellSolver": "ElectroStatic", "poissonTolerance": tol,
+              "poissonGuess": guess}
+    bc = {"xmin": ("fixedValue", 0.0), "xmax": ("fixedValue", 0.0)}
+    extra = {"bc": bc, "space_charge": density * QE, "dt": wp_dt / plasma_frequency(density), "T_K": T_eV * EV_K,
+             "ppc": ppc, "n_cells": ncells}
+    return mesh, species, models, extra

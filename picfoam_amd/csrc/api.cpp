@@ -116,6 +116,9 @@ struct pic_ctx {
     std::unique_ptr<Gnn> gnn;
     bool gnn_use = false, gnn_rec = false;
     double* d_phi_prev = nullptr;
+    double* d_phi_old = nullptr;             // phi of the solve before the last (extrapolation)
+    int poisson_guess = 0;
+    int64_t n_solves = 0;
     // profiling
     bool profile = false;
     std::vector<StageEvent> pending;
@@ -506,6 +509,15 @@ int do_poisson(pic_ctx* c, pic_poisson_stats* st) {
         pic_ctx::Scope sc(c, PIC_STAGE_POISSON);
         hipError_t e = poisson_rhs(c->pd, c->d_rho, c->d_b, c->hp.singular, c->pw.red, c->stream);
         if (int r = check_hip(c, e, "poisson rhs")) return r;
+        if (c->poisson_guess == 1) {
+            // warm start 2 phi(n-1) - phi(n-2) once two solutions exist
+            if (c->n_solves >= 2)
+                poisson_extrapolate(c->d_phi, c->d_phi_old, c->hm.n_cells, c->stream);
+            else
+                hipMemcpyAsync(c->d_phi_old, c->d_phi, sizeof(double) * c->hm.n_cells, hipMemcpyDeviceToDevice,
+                               c->stream);
+        }
+        ++c->n_solves;
         const bool learned = c->gnn && (c->gnn_use || c->gnn_rec);
         if (learned) {
             // features of the warm start: r = b - A phi_prev, b, phi_prev
@@ -652,6 +664,8 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->pmaxit = models->poisson_max_iter > 0 ? models->poisson_max_iter : 10000;
     c->pmode = models->poisson_mode;
     c->sort_interval = models->sort_interval > 0 ? models->sort_interval : 1;
+    c->poisson_guess = models->poisson_guess;
+    if (c->poisson_guess < 0 || c->poisson_guess > 1) return c->fail(PIC_ERR_INVALID, "poisson_guess");
     if (c->mcc) {
         if (c->electron < 0 || c->electron >= n_species) return c->fail(PIC_ERR_INVALID, "electronTypeId not set");
         if (c->ion >= n_species) return c->fail(PIC_ERR_INVALID, "ion species out of range");
@@ -743,6 +757,7 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->d_E = c->alloc<double>(3 * (size_t)nc);
     c->d_B = c->alloc<double>(3 * (size_t)nc);
     c->d_phi = c->alloc<double>(nc);
+    if (c->electrostatic && c->poisson_guess == 1) c->d_phi_old = c->alloc<double>(nc);
     c->d_rho = c->alloc<double>(nc);
     c->d_space = c->alloc<double>(nc);
     c->d_Ep = c->alloc<double>(3 * (size_t)np);

@@ -293,6 +293,20 @@ hipError_t poisson_residual(const PoissonDev& P, PoissonWork& W, const double* x
     return hipGetLastError();
 }
 
+__global__ void k_extrapolate(double* __restrict__ phi, double* __restrict__ old, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double t = phi[i];
+    phi[i] = 2.0 * t - old[i];
+    old[i] = t;
+}
+
+hipError_t poisson_extrapolate(double* phi, double* phi_old, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_extrapolate, dim3((n + 255) / 256), dim3(256), 0, s, phi, phi_old, n);
+    return hipGetLastError();
+}
+
 int poisson_solve(const PoissonDev& P, PoissonWork& W, double* phi, const double* b, double tol, int max_iter,
                   int mode, bool singular, int* iters, int* refinements, double* rel, hipStream_t s) {
     const int n = P.n, nb = nblk(n);

@@ -231,6 +231,8 @@ struct PoissonWork {
     double* scal;         // device scalars
     int nblocks;
 };
+// phi <- 2 phi - phi_old, phi_old <- the previous phi (linear extrapolation in time)
+hipError_t poisson_extrapolate(double* phi, double* phi_old, int n, hipStream_t s);
 // r = b - A x (FP64)
 hipError_t poisson_residual(const PoissonDev& P, PoissonWork& W, const double* x, const double* b, double* r,
                             hipStream_t s);

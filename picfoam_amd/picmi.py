@@ -80,7 +80,7 @@ class pic_models(C.Structure):
                 ("bg_n_equivalent", C.c_double), ("bg_velocity", C.c_double * 3), ("create_ion", C.c_int32),
                 ("create_electron", C.c_int32), ("precision", C.c_int32), ("hc_reference_form", C.c_int32),
                 ("seed", C.c_uint64), ("poisson_tolerance", C.c_double), ("poisson_max_iter", C.c_int32),
-                ("poisson_mode", C.c_int32), ("sort_interval", C.c_int32)]
+                ("poisson_mode", C.c_int32), ("sort_interval", C.c_int32), ("poisson_guess", C.c_int32)]
 
 
 class pic_poisson_stats(C.Structure):
@@ -265,6 +265,7 @@ class Cloud:
         m.poisson_max_iter = int(models.get("poissonMaxIter", 20000))
         m.poisson_mode = {"fp64": POISSON_FP64, "mixed": POISSON_MIXED_IR}[models.get("poissonMode", "fp64")]
         m.sort_interval = int(models.get("sortInterval", 1))
+        m.poisson_guess = {"previous": 0, "extrapolate": 1}[models.get("poissonGuess", "previous")]
         self._models = m
         h = C.c_void_p()
         rc = self.lib.pic_create(C.byref(self.mesh.c), sp, len(species), C.byref(m), C.byref(h))
