@@ -807,12 +807,42 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         c->pd.col_start_h = P.col_start.data();
         c->pd.bsrc = c->upload(P.bsrc);
         c->pd.V = c->d_V;
+        {
+            // the system renumbered by colour, coefficients column-major [k * n + new]
+            const int n = P.n, K = P.K;
+            std::vector<int32_t> inv(n);
+            for (int t = 0; t < n; ++t) inv[P.corder[t]] = t;
+            std::vector<int32_t> nbT((size_t)n * K);
+            std::vector<double> aT((size_t)n * K), dp(n), rdp(n);
+            for (int t = 0; t < n; ++t) {
+                const int i = P.corder[t];
+                for (int k = 0; k < K; ++k) {
+                    const int j = P.nb[(size_t)i * K + k];
+                    nbT[(size_t)k * n + t] = j >= 0 ? inv[j] : -1;
+                    aT[(size_t)k * n + t] = P.a[(size_t)i * K + k];
+                }
+                dp[t] = P.diag[i];
+                rdp[t] = P.rD[i];
+            }
+            std::vector<float> aTf(aT.begin(), aT.end()), dpf(dp.begin(), dp.end()), rdpf(rdp.begin(), rdp.end());
+            c->pd.nbT = c->upload(nbT);
+            c->pd.aT = c->upload(aT);
+            c->pd.aT_f = c->upload(aTf);
+            c->pd.diag_p = c->upload(dp);
+            c->pd.diag_pf = c->upload(dpf);
+            c->pd.rD_p = c->upload(rdp);
+            c->pd.rD_pf = c->upload(rdpf);
+            c->pd.col_start_d = c->upload(P.col_start);
+        }
         c->pw.r = c->alloc<double>(nc); c->pw.z = c->alloc<double>(nc); c->pw.p = c->alloc<double>(nc);
         c->pw.q = c->alloc<double>(nc); c->pw.x = c->alloc<double>(nc); c->pw.b = c->alloc<double>(nc);
+        c->pw.p2 = c->alloc<double>(nc); c->pw.xp = c->alloc<double>(nc); c->pw.bp = c->alloc<double>(nc);
         c->pw.rf = c->alloc<float>(nc); c->pw.zf = c->alloc<float>(nc); c->pw.pf = c->alloc<float>(nc);
-        c->pw.qf = c->alloc<float>(nc); c->pw.xf = c->alloc<float>(nc);
-        c->pw.red = c->alloc<double>(1024 + 64);
+        c->pw.qf = c->alloc<float>(nc); c->pw.xf = c->alloc<float>(nc); c->pw.pf2 = c->alloc<float>(nc);
+        // reduction segments: p.q, r.r, spare, one r.z segment per colour (1024 blocks each)
+        c->pw.red = c->alloc<double>((size_t)(P.ncol + 3) * 1024 + 64);
         c->pw.scal = c->alloc<double>(64);
+        c->pw.cnt = c->alloc<unsigned>(8);
         c->d_b = c->alloc<double>(nc);
         c->d_gfo = c->upload(P.gfo);
         c->d_gkind = c->upload(P.g_kind);

@@ -1,11 +1,18 @@
 // Poisson solve of ElectroStaticSolver::solveFields (ElectroStaticSolver.C:52-71):
-// A phi = rho V / eps0 with the FV Laplacian in ELL form (K couplings per cell), solved
-// by PCG with a DIC (incomplete Cholesky, zero fill) preconditioner in the greedy
-// multicolour ordering, so each colour's forward and backward sweep is one parallel
-// kernel.  Mode 0 runs everything in FP64.  Mode 1 runs FP32 inner PCG sweeps (FP32
-// vectors and coefficients, FP64 dot products) inside FP64 iterative refinement; the
-// refinement residual b - A phi and the stopping test are FP64.
+// A phi = rho V / eps0 with the FV Laplacian, solved by PCG with a DIC (incomplete
+// Cholesky, zero fill) preconditioner in the greedy multicolour ordering.  The solve runs
+// on the system renumbered by colour (each colour a contiguous range, coefficients
+// column-major), so each colour's forward and backward sweep is one streaming kernel.
+// One PCG iteration is 2 + 2 (ncol - 1) streaming kernels plus three one-block kernels
+// that finish alpha, r.r (the convergence flag) and beta on the device: the
+// search-direction update is fused into the matrix product, the colour-0 sweep into the
+// vector update and the dot products into the sweeps; the host enqueues iterations in
+// batches and reads the convergence flag once per batch.  Mode 0 runs everything in FP64.  Mode 1 runs FP32
+// inner PCG (FP32 vectors and coefficients, FP64 dot products) inside FP64 iterative
+// refinement; the refinement residual b - A phi and the stopping test are FP64.
+#include <algorithm>
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "device.h"
@@ -14,7 +21,7 @@
 namespace picmi {
 
 namespace {
-constexpr int kRedBlocks = 512;
+constexpr int kRedBlocks = 1024;
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -26,26 +33,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
         for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
     __syncthreads();
     return r;
-}
-
-// y = A x, partial[block] = sum x.y
-template <class T>
-__global__ void __launch_bounds__(256) k_spmv_dot(int n, int K, const int32_t* __restrict__ nb,
-                                                  const T* __restrict__ a, const T* __restrict__ diag,
-                                                  const T* __restrict__ x, T* __restrict__ y, double* partial) {
-    __shared__ double sh[4];
-    double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        T s = diag[i] * x[i];
-        for (int k = 0; k < K; ++k) {
-            const int j = nb[(int64_t)i * K + k];
-            if (j >= 0) s -= a[(int64_t)i * K + k] * x[j];
-        }
-        y[i] = s;
-        acc += (double)x[i] * (double)s;
-    }
-    acc = block_sum(acc, sh);
-    if (threadIdx.x == 0 && partial) partial[blockIdx.x] = acc;
 }
 
 // partial[block] = sum a.b
@@ -66,77 +53,6 @@ __global__ void k_reduce(const double* partial, int nb, double* scal, int dst) {
     for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) scal[dst] = acc;
-}
-
-// scalar slots: 0 rz, 1 pq, 2 alpha, 3 rz_new, 4 beta, 5 rr, 6 bb
-__global__ void k_alpha(double* scal) {
-    if (threadIdx.x == 0) scal[2] = scal[0] / scal[1];
-}
-__global__ void k_beta(double* scal) {
-    if (threadIdx.x == 0) {
-        scal[4] = scal[3] / scal[0];
-        scal[0] = scal[3];
-    }
-}
-
-// x += alpha p, r -= alpha q, partial = r.r
-template <class T>
-__global__ void __launch_bounds__(256) k_update(int n, const double* scal, T* __restrict__ x, T* __restrict__ r,
-                                                const T* __restrict__ p, const T* __restrict__ q, double* partial) {
-    __shared__ double sh[4];
-    const T alpha = (T)scal[2];
-    double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        x[i] += alpha * p[i];
-        const T ri = r[i] - alpha * q[i];
-        r[i] = ri;
-        acc += (double)ri * (double)ri;
-    }
-    acc = block_sum(acc, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
-}
-
-template <class T>
-__global__ void k_xpby(int n, const double* scal, const T* __restrict__ z, T* __restrict__ p) {
-    const T beta = (T)scal[4];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
-}
-
-template <class T>
-__global__ void k_copy(int n, const T* __restrict__ s, T* __restrict__ d) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] = s[i];
-}
-
-// DIC forward sweep of one colour: z_i = (r_i + sum_{lower colours} a_ij z_j) / rD_i
-template <class T>
-__global__ void k_dic_fwd(int cnt, const int32_t* __restrict__ cells, int col, int K, const int32_t* __restrict__ nb,
-                          const T* __restrict__ a, const int32_t* __restrict__ colour, const T* __restrict__ rD,
-                          const T* __restrict__ r, T* __restrict__ z) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt) return;
-    const int i = cells[t];
-    T s = r[i];
-    for (int k = 0; k < K; ++k) {
-        const int j = nb[(int64_t)i * K + k];
-        if (j >= 0 && colour[j] < col) s += a[(int64_t)i * K + k] * z[j];
-    }
-    z[i] = s / rD[i];
-}
-
-// DIC backward sweep of one colour: z_i += (sum_{higher colours} a_ij z_j) / rD_i
-template <class T>
-__global__ void k_dic_bwd(int cnt, const int32_t* __restrict__ cells, int col, int K, const int32_t* __restrict__ nb,
-                          const T* __restrict__ a, const int32_t* __restrict__ colour, const T* __restrict__ rD,
-                          T* __restrict__ z) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt) return;
-    const int i = cells[t];
-    T s = 0;
-    for (int k = 0; k < K; ++k) {
-        const int j = nb[(int64_t)i * K + k];
-        if (j >= 0 && colour[j] > col) s += a[(int64_t)i * K + k] * z[j];
-    }
-    z[i] = z[i] + s / rD[i];
 }
 
 // r = b - A x (FP64), partial = r.r
@@ -200,75 +116,294 @@ int nblk(int n) {
     int b = (n + 255) / 256;
     return b < 1 ? 1 : (b > kRedBlocks ? kRedBlocks : b);
 }
+__device__ __forceinline__ int nblk_d(int n) {
+    int b = (n + 255) / 256;
+    return b < 1 ? 1 : (b > kRedBlocks ? kRedBlocks : b);
+}
+
+// scalar slots of W.scal
+enum : int { kRz = 0, kPq = 1, kAlpha = 2, kBeta = 4, kRR = 5, kBB = 6, kMean = 7, kDone = 8, kConvIt = 9, kTol2 = 10 };
+// reduction segments of W.red (kRedBlocks doubles each): 0 p.q, 1 r.r, 2 spare, 3 + c r.z of colour c
+constexpr int kSegPq = 0, kSegRr = 1, kSegRz0 = 3;
+
+// fixed-order sum of nb partials by one block (thread 0 gets the result)
+__device__ __forceinline__ double sum_partials(const double* partial, int nb, double* sh) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) t += partial[b];
+    return block_sum(t, sh);
+}
+
+// The scalars of an iteration are finished by one-block kernels after the grid kernels:
+// a grid-wide "last block" finish would need a device-scope fence per block, which
+// writes back the XCD's L2 and costs more than the extra launch.
+
+// alpha = rz / p.q
+__global__ void k_fin_alpha(double* scal, const double* red, int nb) {
+    __shared__ double sh[4];
+    if (scal[kDone] != 0.0) return;
+    const double pq = sum_partials(red + kSegPq * kRedBlocks, nb, sh);
+    if (threadIdx.x == 0) { scal[kPq] = pq; scal[kAlpha] = scal[kRz] / pq; }
+}
+
+// r.r and the convergence flag (it > 0); with one colour also r.z -> beta (zero at the
+// start of a solve, it = 0)
+__global__ void k_fin_rr(double* scal, const double* red, int nb, int rz_here, int it) {
+    __shared__ double sh[4];
+    if (scal[kDone] != 0.0) return;
+    double rr = 0.0, rz = 0.0;
+    if (it > 0) rr = sum_partials(red + kSegRr * kRedBlocks, nb, sh);
+    if (rz_here) rz = sum_partials(red + kSegRz0 * kRedBlocks, nb, sh);
+    if (threadIdx.x == 0) {
+        if (it > 0) {
+            scal[kRR] = rr;
+            if (rr <= scal[kTol2]) { scal[kDone] = 1.0; scal[kConvIt] = (double)it; }
+        }
+        if (rz_here) { scal[kBeta] = it > 0 ? rz / scal[kRz] : 0.0; scal[kRz] = rz; }
+    }
+}
+
+// r.z over every colour's partials -> beta (zero at the start of a solve)
+__global__ void k_fin_rz(double* scal, const double* red, int ncol, const int32_t* col_start, int init) {
+    __shared__ double sh[4];
+    if (scal[kDone] != 0.0) return;
+    double rz = 0.0;
+    for (int c = 0; c < ncol; ++c) {
+        const double t = sum_partials(red + (size_t)(kSegRz0 + c) * kRedBlocks, nblk_d(col_start[c + 1] - col_start[c]), sh);
+        if (threadIdx.x == 0) rz += t;
+    }
+    if (threadIdx.x == 0) {
+        scal[kBeta] = init ? 0.0 : rz / scal[kRz];
+        scal[kRz] = rz;
+    }
+}
+
+__global__ void k_pcg_reset(double* scal, double tol2) {
+    if (threadIdx.x == 0) {
+        scal[kDone] = 0.0; scal[kConvIt] = -1.0; scal[kTol2] = tol2; scal[kBeta] = 0.0;
+    }
+}
+
+// p_new = z + beta p_old, q = A p_new (neighbours' p formed on the fly), p.q partials
+template <class T>
+__global__ void __launch_bounds__(256) k_pcg_spmv(int n, int K, const int32_t* __restrict__ nbT,
+                                                  const T* __restrict__ aT, const T* __restrict__ diag,
+                                                  const T* __restrict__ z, const T* __restrict__ pold,
+                                                  T* __restrict__ pnew, T* __restrict__ q, double* scal, double* red) {
+    __shared__ double sh[4];
+    if (scal[kDone] != 0.0) return;
+    const T beta = (T)scal[kBeta];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const T pi = z[i] + beta * pold[i];
+        pnew[i] = pi;
+        T s = diag[i] * pi;
+        for (int k = 0; k < K; ++k) {
+            const int j = nbT[(size_t)k * n + i];
+            if (j >= 0) s -= aT[(size_t)k * n + i] * (z[j] + beta * pold[j]);
+        }
+        q[i] = s;
+        acc += (double)pi * (double)s;
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) red[kSegPq * kRedBlocks + blockIdx.x] = acc;
+}
+
+// x += alpha p, r -= alpha q, r.r partials; z = r / rD on the first colour (its whole
+// forward sweep); with one colour also the r.z partials
+template <class T>
+__global__ void __launch_bounds__(256) k_pcg_update(int n, int c0end, int rz_here, T* __restrict__ x, T* __restrict__ r,
+                                                    const T* __restrict__ p, const T* __restrict__ q,
+                                                    const T* __restrict__ rD, T* __restrict__ z, double* scal,
+                                                    double* red) {
+    __shared__ double sh[4];
+    if (scal[kDone] != 0.0) return;
+    const T alpha = (T)scal[kAlpha];
+    double acc = 0.0, acc2 = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        const T ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        acc += (double)ri * (double)ri;
+        if (i < c0end) {
+            const T zi = ri / rD[i];
+            z[i] = zi;
+            acc2 += (double)ri * (double)zi;
+        }
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) red[kSegRr * kRedBlocks + blockIdx.x] = acc;
+    if (rz_here) {
+        acc2 = block_sum(acc2, sh);
+        if (threadIdx.x == 0) red[kSegRz0 * kRedBlocks + blockIdx.x] = acc2;
+    }
+}
+
+// start of a solve: z = r / rD on the first colour, p_old = 0; with one colour r.z partials
+template <class T>
+__global__ void __launch_bounds__(256) k_pcg_zc0(int n, int c0end, int rz_here, const T* __restrict__ r,
+                                                 const T* __restrict__ rD, T* __restrict__ z, T* __restrict__ pold,
+                                                 double* red) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        pold[i] = (T)0;
+        if (i < c0end) {
+            const T zi = r[i] / rD[i];
+            z[i] = zi;
+            acc += (double)r[i] * (double)zi;
+        }
+    }
+    if (!rz_here) return;
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) red[kSegRz0 * kRedBlocks + blockIdx.x] = acc;
+}
+
+// DIC forward sweep of colour range [c0, c1): z_i = (r_i + sum_{j < c0} a_ij z_j) / rD_i;
+// the last colour's z is final there, so its r.z partials are stored (segment seg)
+template <class T>
+__global__ void __launch_bounds__(256) k_pcg_fwd(int n, int c0, int c1, int K, const int32_t* __restrict__ nbT,
+                                                 const T* __restrict__ aT, const T* __restrict__ rD,
+                                                 const T* __restrict__ r, T* __restrict__ z, double* scal, double* red,
+                                                 int seg) {
+    __shared__ double sh[4];
+    if (scal[kDone] != 0.0) return;
+    double acc = 0.0;
+    for (int i = c0 + blockIdx.x * blockDim.x + threadIdx.x; i < c1; i += gridDim.x * blockDim.x) {
+        T s = r[i];
+        for (int k = 0; k < K; ++k) {
+            const int j = nbT[(size_t)k * n + i];
+            if (j >= 0 && j < c0) s += aT[(size_t)k * n + i] * z[j];
+        }
+        const T zi = s / rD[i];
+        z[i] = zi;
+        acc += (double)r[i] * (double)zi;
+    }
+    if (seg < 0) return;
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) red[(size_t)(kSegRz0 + seg) * kRedBlocks + blockIdx.x] = acc;
+}
+
+// DIC backward sweep of colour range [c0, c1): z_i += (sum_{j >= c1} a_ij z_j) / rD_i and
+// its r.z partials (z of the colour is final there)
+template <class T>
+__global__ void __launch_bounds__(256) k_pcg_bwd(int n, int c0, int c1, int K, const int32_t* __restrict__ nbT,
+                                                 const T* __restrict__ aT, const T* __restrict__ rD,
+                                                 const T* __restrict__ r, T* __restrict__ z, double* scal, double* red,
+                                                 int seg) {
+    __shared__ double sh[4];
+    if (scal[kDone] != 0.0) return;
+    double acc = 0.0;
+    for (int i = c0 + blockIdx.x * blockDim.x + threadIdx.x; i < c1; i += gridDim.x * blockDim.x) {
+        T s = 0;
+        for (int k = 0; k < K; ++k) {
+            const int j = nbT[(size_t)k * n + i];
+            if (j >= c1) s += aT[(size_t)k * n + i] * z[j];
+        }
+        const T zi = z[i] + s / rD[i];
+        z[i] = zi;
+        acc += (double)r[i] * (double)zi;
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) red[(size_t)(kSegRz0 + seg) * kRedBlocks + blockIdx.x] = acc;
+}
+
+// residual of the colour-ordered FP64 system: r = b - A x, partial r.r
+__global__ void __launch_bounds__(256) k_residual_p(int n, int K, const int32_t* __restrict__ nbT,
+                                                    const double* __restrict__ aT, const double* __restrict__ diag,
+                                                    const double* __restrict__ x, const double* __restrict__ b,
+                                                    double* __restrict__ r, double* partial) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double s = diag[i] * x[i];
+        for (int k = 0; k < K; ++k) {
+            const int j = nbT[(size_t)k * n + i];
+            if (j >= 0) s -= aT[(size_t)k * n + i] * x[j];
+        }
+        const double ri = b[i] - s;
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ void k_gather(int n, const int32_t* __restrict__ perm, const double* __restrict__ src,
+                         double* __restrict__ dst) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[perm[i]];
+}
+__global__ void k_scatter_back(int n, const int32_t* __restrict__ perm, const double* __restrict__ src,
+                               double* __restrict__ dst) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[perm[i]] = src[i];
+}
+
+// fixed-order sum of partial into scal[slot]
+void reduce_to(const double* partial, int nb, double* scal, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, partial, nb, scal, slot);
+}
 
 template <class T>
-struct Ops {
-    const PoissonDev& P;
-    hipStream_t s;
-    double* red;
-    double* scal;
-    const T* a;
+struct Sys {
+    const T* aT;
     const T* diag;
     const T* rD;
-    void precond(const T* r, T* z) const {
-        const int K = P.K;
-        for (int c = 0; c < P.ncol; ++c) {
-            const int c0 = P.col_start_h[c], cnt = P.col_start_h[c + 1] - c0;
-            if (cnt > 0)
-                hipLaunchKernelGGL((k_dic_fwd<T>), dim3((cnt + 255) / 256), dim3(256), 0, s, cnt, P.corder + c0, c, K,
-                                   P.nb, a, P.colour, rD, r, z);
-        }
-        for (int c = P.ncol - 2; c >= 0; --c) {
-            const int c0 = P.col_start_h[c], cnt = P.col_start_h[c + 1] - c0;
-            if (cnt > 0)
-                hipLaunchKernelGGL((k_dic_bwd<T>), dim3((cnt + 255) / 256), dim3(256), 0, s, cnt, P.corder + c0, c, K,
-                                   P.nb, a, P.colour, rD, z);
-        }
-    }
-    void dot(const T* x, const T* y, int slot) const {
-        const int nb = nblk(P.n);
-        hipLaunchKernelGGL((k_dot<T>), dim3(nb), dim3(256), 0, s, P.n, x, y, red);
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, red, nb, scal, slot);
-    }
 };
 
-// Preconditioned CG on A x = r0 with x starting at the current content of x.
-// Stops when sqrt(r.r) <= tol_abs; checks the device residual every `check` iterations
-// (the per-iteration norms are recorded so the first converged iteration is reported).
+// Preconditioned CG on the colour-ordered system A x = r (x holds the start, r its
+// residual).  Iterations are enqueued in batches; the host reads the device convergence
+// flag after each batch.  Returns the iteration count of convergence or -1.
 template <class T>
-int pcg(const Ops<T>& O, T* x, T* r, T* z, T* p, T* q, double tol_abs, int max_iter, double* rr_hist,
-        int* converged_at, double* last_rr) {
-    const PoissonDev& P = O.P;
-    const int n = P.n, nb = nblk(n);
-    hipStream_t s = O.s;
-    O.precond(r, z);
-    hipLaunchKernelGGL((k_copy<T>), dim3(nb), dim3(256), 0, s, n, z, p);
-    O.dot(r, z, 0);
-    const int check = 8;
-    int it = 0;
-    *converged_at = -1;
-    std::vector<double> hist(check);
-    for (it = 1; it <= max_iter; ++it) {
-        hipLaunchKernelGGL((k_spmv_dot<T>), dim3(nb), dim3(256), 0, s, n, P.K, P.nb, O.a, O.diag, p, q, O.red);
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, O.red, nb, O.scal, 1);
-        hipLaunchKernelGGL(k_alpha, dim3(1), dim3(64), 0, s, O.scal);
-        hipLaunchKernelGGL((k_update<T>), dim3(nb), dim3(256), 0, s, n, O.scal, x, r, p, q, O.red);
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, O.red, nb, rr_hist, (it - 1) % check);
-        if (it % check == 0 || it == max_iter) {
-            hipMemcpyAsync(hist.data(), rr_hist, sizeof(double) * check, hipMemcpyDeviceToHost, s);
-            hipStreamSynchronize(s);
-            const int upto = ((it - 1) % check) + 1;
-            for (int k = 0; k < upto; ++k)
-                if (std::sqrt(hist[k]) <= tol_abs) { *converged_at = it - upto + k + 1; *last_rr = hist[k]; break; }
-            if (*converged_at < 0) *last_rr = hist[upto - 1];
-            if (*converged_at >= 0) break;
+int pcg_fused(const PoissonDev& P, const Sys<T>& S, PoissonWork& W, T* x, T* r, T* z, T* p, T* p2, T* q,
+              double tol_abs, int max_iter, hipStream_t s, int* iters_done, double* last_rr) {
+    const int n = P.n, K = P.K, ncol = P.ncol, nb = nblk(n);
+    const int* cs = P.col_start_h;
+    const int c0end = cs[1];
+    const int rz_here = ncol == 1 ? 1 : 0;
+    hipLaunchKernelGGL(k_pcg_reset, dim3(1), dim3(64), 0, s, W.scal, tol_abs * tol_abs);
+    // z = M^-1 r, r.z, beta = 0; p_old = 0
+    hipLaunchKernelGGL((k_pcg_zc0<T>), dim3(nb), dim3(256), 0, s, n, c0end, rz_here, r, S.rD, z, p2, W.red);
+    auto sweeps = [&](int init) {
+        for (int c = 1; c < ncol; ++c) {
+            const int a0 = cs[c], cnt = cs[c + 1] - a0;
+            hipLaunchKernelGGL((k_pcg_fwd<T>), dim3(nblk(cnt)), dim3(256), 0, s, n, a0, cs[c + 1], K, P.nbT, S.aT,
+                               S.rD, r, z, W.scal, W.red, c == ncol - 1 ? c : -1);
         }
-        O.precond(r, z);
-        O.dot(r, z, 3);
-        hipLaunchKernelGGL(k_beta, dim3(1), dim3(64), 0, s, O.scal);
-        hipLaunchKernelGGL((k_xpby<T>), dim3(nb), dim3(256), 0, s, n, O.scal, z, p);
+        for (int c = ncol - 2; c >= 0; --c) {
+            const int a0 = cs[c], cnt = cs[c + 1] - a0;
+            hipLaunchKernelGGL((k_pcg_bwd<T>), dim3(nblk(cnt)), dim3(256), 0, s, n, a0, cs[c + 1], K, P.nbT, S.aT,
+                               S.rD, r, z, W.scal, W.red, c);
+        }
+        if (ncol > 1)
+            hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(256), 0, s, W.scal, W.red, ncol, P.col_start_d, init);
+    };
+    if (rz_here) hipLaunchKernelGGL(k_fin_rr, dim3(1), dim3(256), 0, s, W.scal, W.red, nb, 1, 0);
+    sweeps(1);
+    T* pa = p;    // written this iteration
+    T* pb = p2;   // previous direction
+    int it = 0, batch = 8, conv = -1;
+    double h[3] = {0.0, -1.0, 0.0};
+    while (it < max_iter) {
+        const int B = std::min(batch, max_iter - it);
+        for (int k = 0; k < B; ++k) {
+            ++it;
+            hipLaunchKernelGGL((k_pcg_spmv<T>), dim3(nb), dim3(256), 0, s, n, K, P.nbT, S.aT, S.diag, z, pb, pa, q,
+                               W.scal, W.red);
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, s, W.scal, W.red, nb);
+            hipLaunchKernelGGL((k_pcg_update<T>), dim3(nb), dim3(256), 0, s, n, c0end, rz_here, x, r, pa, q, S.rD, z,
+                               W.scal, W.red);
+            hipLaunchKernelGGL(k_fin_rr, dim3(1), dim3(256), 0, s, W.scal, W.red, nb, rz_here, it);
+            sweeps(0);
+            std::swap(pa, pb);
+        }
+        hipMemcpyAsync(h, W.scal + kDone, sizeof(double) * 2, hipMemcpyDeviceToHost, s);
+        hipMemcpyAsync(h + 2, W.scal + kRR, sizeof(double), hipMemcpyDeviceToHost, s);
+        hipStreamSynchronize(s);
+        *last_rr = h[2];
+        if (h[0] != 0.0) { conv = (int)h[1]; break; }
+        batch = std::min(batch * 2, 32);
     }
-    return *converged_at >= 0 ? *converged_at : max_iter;
+    *iters_done = conv >= 0 ? conv : max_iter;
+    return conv;
 }
 
 }  // namespace
@@ -311,67 +446,64 @@ int poisson_solve(const PoissonDev& P, PoissonWork& W, double* phi, const double
                   int mode, bool singular, int* iters, int* refinements, double* rel, hipStream_t s) {
     const int n = P.n, nb = nblk(n);
     double* scal = W.scal;
-    double* hist = W.scal + 16;
-    // ||b||
-    {
-        Ops<double> O{P, s, W.red, scal, P.a, P.diag, P.rD};
-        O.dot(b, b, 6);
-    }
-    double bb = 0.0;
-    hipMemcpyAsync(&bb, scal + 6, sizeof(double), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
+    // colour order: x = phi[perm], b = b[perm]
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, n, P.corder, phi, W.xp);
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, n, P.corder, b, W.bp);
+    hipLaunchKernelGGL((k_dot<double>), dim3(nb), dim3(256), 0, s, n, W.bp, W.bp, W.red);
+    reduce_to(W.red, nb, scal, kBB, s);
+    auto residual = [&](double* rr_host) {
+        hipLaunchKernelGGL(k_residual_p, dim3(nb), dim3(256), 0, s, n, P.K, P.nbT, P.aT, P.diag_p, W.xp, W.bp, W.r,
+                           W.red);
+        reduce_to(W.red, nb, scal, kRR, s);
+        double h[2] = {0.0, 0.0};
+        hipMemcpyAsync(h, scal + kRR, sizeof(double) * 2, hipMemcpyDeviceToHost, s);
+        hipStreamSynchronize(s);
+        *rr_host = h[0];
+        return h[1];
+    };
+    double rr0 = 0.0;
+    const double bb = residual(&rr0);
     double bn = std::sqrt(bb);
     if (bn == 0.0) bn = 1.0;
     const double tol_abs = tol * bn;
     int total = 0, outer = 0;
-    double last_rr = 0.0;
+    double last_rr = rr0;
     int status = 0;
     if (mode == 0) {
-        hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, n, P.K, P.nb, P.a, P.diag, phi, b, W.r, W.red);
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, W.red, nb, scal, 5);
-        double rr0 = 0.0;
-        hipMemcpyAsync(&rr0, scal + 5, sizeof(double), hipMemcpyDeviceToHost, s);
-        hipStreamSynchronize(s);
-        last_rr = rr0;
         if (std::sqrt(rr0) > tol_abs) {
-            Ops<double> O{P, s, W.red, scal, P.a, P.diag, P.rD};
-            int conv = -1;
-            total = pcg<double>(O, phi, W.r, W.z, W.p, W.q, tol_abs, max_iter, hist, &conv, &last_rr);
+            Sys<double> S{P.aT, P.diag_p, P.rD_p};
+            int done = 0;
+            const int conv = pcg_fused<double>(P, S, W, W.xp, W.r, W.z, W.p, W.p2, W.q, tol_abs, max_iter, s, &done,
+                                               &last_rr);
+            total = done;
             if (conv < 0) status = -1;
         }
     } else {
         // FP64 iterative refinement around FP32 PCG
         const double inner_rel = 1e-4;
-        Ops<float> O{P, s, W.red, scal, P.a_f, P.diag_f, P.rD_f};
+        Sys<float> S{P.aT_f, P.diag_pf, P.rD_pf};
+        double rr = rr0;
         for (outer = 0; outer < 50; ++outer) {
-            hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, n, P.K, P.nb, P.a, P.diag, phi, b, W.r, W.red);
-            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, W.red, nb, scal, 5);
-            double rr = 0.0;
-            hipMemcpyAsync(&rr, scal + 5, sizeof(double), hipMemcpyDeviceToHost, s);
-            hipStreamSynchronize(s);
+            if (outer > 0) residual(&rr);
             last_rr = rr;
             if (std::sqrt(rr) <= tol_abs) break;
             if (total >= max_iter) { status = -1; break; }
             hipLaunchKernelGGL(k_d2f, dim3(nb), dim3(256), 0, s, n, W.r, W.rf);
             hipLaunchKernelGGL((k_zero<float>), dim3(nb), dim3(256), 0, s, n, W.xf);
-            int conv = -1;
-            double irr = 0.0;
             const double itol = std::max(inner_rel * std::sqrt(rr), 0.5 * tol_abs);
-            total += pcg<float>(O, W.xf, W.rf, W.zf, W.pf, W.qf, itol, max_iter - total, hist, &conv, &irr);
-            hipLaunchKernelGGL(k_add_f2d, dim3(nb), dim3(256), 0, s, n, W.xf, phi);
+            int done = 0;
+            double irr = 0.0;
+            pcg_fused<float>(P, S, W, W.xf, W.rf, W.zf, W.pf, W.pf2, W.qf, itol, max_iter - total, s, &done, &irr);
+            total += done;
+            hipLaunchKernelGGL(k_add_f2d, dim3(nb), dim3(256), 0, s, n, W.xf, W.xp);
         }
     }
+    if (mode == 0 && total > 0) residual(&last_rr);  // the true FP64 residual
+    hipLaunchKernelGGL(k_scatter_back, dim3(nb), dim3(256), 0, s, n, P.corder, W.xp, phi);
     if (singular) {
         hipLaunchKernelGGL(k_sum_partial, dim3(nb), dim3(256), 0, s, n, phi, W.red);
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, W.red, nb, scal, 7);
-        hipLaunchKernelGGL(k_sub_mean, dim3(nb), dim3(256), 0, s, n, phi, scal, 7);
-    }
-    if (mode == 0 && total > 0) {
-        // report the true FP64 residual
-        hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, s, n, P.K, P.nb, P.a, P.diag, phi, b, W.r, W.red);
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, W.red, nb, scal, 5);
-        hipMemcpyAsync(&last_rr, scal + 5, sizeof(double), hipMemcpyDeviceToHost, s);
-        hipStreamSynchronize(s);
+        reduce_to(W.red, nb, scal, kMean, s);
+        hipLaunchKernelGGL(k_sub_mean, dim3(nb), dim3(256), 0, s, n, phi, scal, kMean);
     }
     *iters = total;
     *refinements = outer;

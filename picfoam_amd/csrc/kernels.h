@@ -220,6 +220,16 @@ struct PoissonDev {
     const int32_t* col_start_h;  // host copy of colour starts
     const double* bsrc;
     const double* V;
+    // the same system renumbered by colour (cell new -> corder[new]), coefficients stored
+    // column-major [k * n + i]: colour c is the contiguous range col_start[c]..col_start[c+1]
+    const int32_t* nbT;
+    const double* aT;
+    const float* aT_f;
+    const double* diag_p;
+    const float* diag_pf;
+    const double* rD_p;
+    const float* rD_pf;
+    const int32_t* col_start_d;  // device copy of colour starts
 };
 hipError_t poisson_rhs(const PoissonDev& P, const double* rho, double* b, bool singular, double* scratch,
                        hipStream_t s);
@@ -227,8 +237,11 @@ hipError_t poisson_rhs(const PoissonDev& P, const double* rho, double* b, bool s
 struct PoissonWork {
     double *r, *z, *p, *q, *x, *b;
     float *rf, *zf, *pf, *qf, *xf;
-    double* red;          // reduction scratch (>= 4*nblocks + 16)
+    double *p2, *xp, *bp;  // second search-direction buffer, x and b in colour order
+    float* pf2;
+    double* red;          // reduction scratch: (ncol + 3) * 1024 + 64 doubles
     double* scal;         // device scalars
+    unsigned* cnt;        // block-completion counters of the fused reductions (zero between uses)
     int nblocks;
 };
 // phi <- 2 phi - phi_old, phi_old <- the previous phi (linear extrapolation in time)
