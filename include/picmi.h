@@ -142,6 +142,9 @@ typedef struct {
     double sum_w[8], sum_wU[8][3], sum_wU2[8], kinetic_energy[8];
     double field_energy;        /* FieldEnergyInfo.C:118-124: 0.5 eps0 sum |E|^2 V */
     int64_t n_parcels[8];
+    /* KineticEnergyInfo.C:73-77 relativistic branch: sum (gamma - 1) m c^2, without the
+     * parcel weight as in the reference */
+    double kinetic_energy_relativistic[8];
 } pic_diag;
 
 const char* pic_version(void);
@@ -191,6 +194,10 @@ int pic_collide_mcc(pic_ctx* ctx, double dt);          /* ElectronNeutralCollisi
 int pic_deposit(pic_ctx* ctx);                         /* calculateFields: charge density */
 int pic_poisson_solve(pic_ctx* ctx, pic_poisson_stats* stats); /* MaxwellSolver::solveFields */
 int pic_diag_reduce(pic_ctx* ctx, pic_diag* out);      /* DiagnosticsList::info sums */
+/* PICCloud::calculateFields (PICCloud.C:587-695) moments per cell of one species, or of all
+ * species (species = -1), out[n_cells][11]: rhoN, rhoM, picRhoN (parcel count), linearKE,
+ * momentum[3], j[3], rhoCharge; densities per m^3.  Sorts the parcels by cell first. */
+int pic_cell_moments(pic_ctx* ctx, int32_t species, double* out);
 int pic_step(pic_ctx* ctx, double dt, pic_step_stats* stats);  /* one PICCloud::evolve() */
 /* n_steps evolve() calls, the time loop of picFoam.C, with one host synchronisation at the
  * end: stats sum the counters over the steps (poisson: the last solve); an error raised on

@@ -93,6 +93,7 @@ struct pic_ctx {
     unsigned long long* d_stats = nullptr;  // 0 deleted, 1 guard, 2 coulomb, 3..6 mcc
     int64_t* d_n = nullptr;                  // device parcel counts per species (authoritative)
     int* d_big_list = nullptr;               // cells left to the large-cell collision passes
+    double* d_moments = nullptr;             // cell moments scratch [n_cells][11]
     int* d_big_n = nullptr;
     void* d_scan_tmp = nullptr;
     size_t scan_bytes = 0;
@@ -1253,6 +1254,7 @@ int pic_diag_reduce(pic_ctx* c, pic_diag* out) {
         for (int d = 0; d < 3; ++d) out->sum_wU[s][d] = h[1 + d] * w;
         out->sum_wU2[s] = h[4] * w;
         out->kinetic_energy[s] = 0.5 * S.mass * h[4] * w;
+        out->kinetic_energy_relativistic[s] = h[5] * S.mass * kC * kC;
     }
     {
         int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (c->hm.n_cells + 255) / 256));
@@ -1264,6 +1266,32 @@ int pic_diag_reduce(pic_ctx* c, pic_diag* out) {
         out->field_energy = 0.5 * kEps0 * fe;
     }
     return PIC_OK;
+}
+
+int pic_cell_moments(pic_ctx* c, int32_t species, double* out) {
+    if (!c || !out || species < -1 || species >= (int)c->sp.size()) return c ? c->fail(PIC_ERR_INVALID, "species") : PIC_ERR_INVALID;
+    const int nc = c->hm.n_cells;
+    if (!c->d_moments) c->d_moments = c->alloc<double>((size_t)nc * kMomentVals);
+    if (!c->d_moments) return c->fail(PIC_ERR_HIP, "out of device memory (moments)");
+    if (int r = do_sort(c)) return r;
+    hipMemsetAsync(c->d_moments, 0, sizeof(double) * (size_t)nc * kMomentVals, c->stream);
+    for (int s = 0; s < (int)c->sp.size(); ++s) {
+        if (species >= 0 && s != species) continue;
+        Species& S = c->sp[s];
+        hipError_t e;
+        if (c->mixed()) {
+            MomentArgs<float> a{nc, soa<float>(S, S.cur), S.start, c->d_V, S.mass, S.charge, S.weight, 4096, c->d_flags,
+                                c->d_moments};
+            e = launch_moments(a, c->stream);
+        } else {
+            MomentArgs<double> a{nc, soa<double>(S, S.cur), S.start, c->d_V, S.mass, S.charge, S.weight, 4096,
+                                 c->d_flags, c->d_moments};
+            e = launch_moments(a, c->stream);
+        }
+        if (int r = check_hip(c, e, "moments")) return r;
+    }
+    hipMemcpyAsync(out, c->d_moments, sizeof(double) * (size_t)nc * kMomentVals, hipMemcpyDeviceToHost, c->stream);
+    return sync_state(c);
 }
 
 int pic_sync(pic_ctx* c) {

@@ -245,3 +245,83 @@ hipError_t launch_field_energy(const double* E, const double* V, int n, double* 
 }
 
 }  // namespace picmi
+
+namespace picmi {
+
+// ---------------------------------------------------------------- cell moments
+// PICCloud::calculateFields (PICCloud.C:587-695) per cell for one species, from the
+// cell-sorted arrays: one wave per cell orders the cell's parcels by id in LDS (so the
+// sums do not depend on the order the sort left them in) and reduces
+//   [0] rhoN = sum w / V, [1] rhoM = sum w m / V, [2] picRhoN = parcel count,
+//   [3] linearKE = sum 0.5 w m U.U / V, [4..6] momentum = sum w m U / V,
+//   [7..9] j = sum w q U / V, [10] rhoCharge = sum w q / V
+template <class UT>
+__global__ void __launch_bounds__(64) k_moments(MomentArgs<UT> a) {
+    extern __shared__ uint64_t lds_m[];
+    uint64_t* key = lds_m;
+    int* val = (int*)(lds_m + a.maxp);
+    const int cell = blockIdx.x;
+    const int f0 = a.start[cell], n = a.start[cell + 1] - f0;
+    double* o = a.out + (size_t)cell * kMomentVals;
+    if (n > a.maxp) {
+        if (threadIdx.x == 0) atomicOr(a.flags, kFlagCellOverflow);
+        return;
+    }
+    int np2 = 1;
+    while (np2 < n) np2 <<= 1;
+    for (int k = threadIdx.x; k < np2; k += blockDim.x) {
+        key[k] = (k < n) ? (((uint64_t)a.p.id[f0 + k] << 32) | (uint32_t)k) : ~0ull;
+        val[k] = (k < n) ? f0 + k : -1;
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= np2; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & kk) == 0;
+                    const uint64_t x = key[i], y = key[l];
+                    if ((x > y) == up) {
+                        key[i] = y; key[l] = x;
+                        const int t = val[i]; val[i] = val[l]; val[l] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    double v[kMomentVals] = {};
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int i = val[k];
+        const double ux = a.p.ux[i], uy = a.p.uy[i], uz = a.p.uz[i];
+        v[0] += 1.0;
+        v[3] += ux * ux + uy * uy + uz * uz;
+        v[4] += ux; v[5] += uy; v[6] += uz;
+    }
+    for (int q = 0; q < kMomentVals; ++q)
+        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off);
+    if (threadIdx.x == 0) {
+        const double iV = 1.0 / a.V[cell];
+        const double w = a.weight, m = a.mass, qc = a.charge;
+        o[0] += v[0] * w * iV;
+        o[1] += v[0] * w * m * iV;
+        o[2] += v[0];
+        o[3] += 0.5 * w * m * v[3] * iV;
+        for (int d = 0; d < 3; ++d) {
+            o[4 + d] += w * m * v[4 + d] * iV;
+            o[7 + d] += w * qc * v[4 + d] * iV;
+        }
+        o[10] += v[0] * w * qc * iV;
+    }
+}
+
+template <class UT>
+static hipError_t launch_moments_t(const MomentArgs<UT>& a, hipStream_t s) {
+    if (a.n_cells <= 0) return hipSuccess;
+    const size_t lds = (size_t)a.maxp * (sizeof(uint64_t) + sizeof(int));
+    hipLaunchKernelGGL((k_moments<UT>), dim3(a.n_cells), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_moments(const MomentArgs<double>& a, hipStream_t s) { return launch_moments_t(a, s); }
+hipError_t launch_moments(const MomentArgs<float>& a, hipStream_t s) { return launch_moments_t(a, s); }
+
+}  // namespace picmi

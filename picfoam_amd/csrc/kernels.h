@@ -135,6 +135,22 @@ hipError_t launch_diag(const PartSoA<float>& p, int64_t n, double* partial, int 
 hipError_t launch_field_energy(const double* E, const double* V, int n, double* partial, int nblocks, double* out,
                                hipStream_t s);
 
+// cell moments of one species (added into out[n_cells][kMomentVals]); cells sorted
+constexpr int kMomentVals = 11;
+template <class UT>
+struct MomentArgs {
+    int n_cells;
+    PartSoA<UT> p;
+    const int* start;
+    const double* V;
+    double mass, charge, weight;
+    int maxp;                 // LDS slots: parcels of the species per cell (power of two)
+    uint32_t* flags;
+    double* out;
+};
+hipError_t launch_moments(const MomentArgs<double>& a, hipStream_t s);
+hipError_t launch_moments(const MomentArgs<float>& a, hipStream_t s);
+
 // ---------------------------------------------------------------- Coulomb (k_coulomb.hip)
 struct CoulombArgs {
     int n_cells;

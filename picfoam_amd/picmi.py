@@ -37,7 +37,7 @@ EXPORTS = (
     "pic_mcc_initialize", "pic_particles_set", "pic_particles_set_positions", "pic_particles_count",
     "pic_particles_get", "pic_particles_positions", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
     "pic_collide_coulomb", "pic_collide_mcc", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
-    "pic_run",
+    "pic_run", "pic_cell_moments",
     "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
     "pic_profile_reset", "pic_gnn_init", "pic_gnn_n_params", "pic_gnn_params", "pic_gnn_record",
     "pic_gnn_snapshot_count", "pic_gnn_snapshot", "pic_gnn_grad", "pic_gnn_train", "pic_gnn_eval", "pic_gnn_use",
@@ -96,7 +96,8 @@ class pic_step_stats(C.Structure):
 
 class pic_diag(C.Structure):
     _fields_ = [("sum_w", C.c_double * 8), ("sum_wU", (C.c_double * 3) * 8), ("sum_wU2", C.c_double * 8),
-                ("kinetic_energy", C.c_double * 8), ("field_energy", C.c_double), ("n_parcels", C.c_int64 * 8)]
+                ("kinetic_energy", C.c_double * 8), ("field_energy", C.c_double), ("n_parcels", C.c_int64 * 8),
+                ("kinetic_energy_relativistic", C.c_double * 8)]
 
 
 _lib = None
@@ -142,6 +143,7 @@ def load_library(path: str = LIB_PATH):
         "pic_diag_reduce": (C.c_int, [vp, C.POINTER(pic_diag)]),
         "pic_step": (C.c_int, [vp, dbl, C.POINTER(pic_step_stats)]),
         "pic_run": (C.c_int, [vp, dbl, i64, C.POINTER(pic_step_stats)]),
+        "pic_cell_moments": (C.c_int, [vp, i32, vp]),
         "pic_sync": (C.c_int, [vp]),
         "pic_step_index": (i64, [vp]),
         "pic_set_step_index": (C.c_int, [vp, i64]),
@@ -411,6 +413,15 @@ class Cloud:
         d = pic_diag()
         self._check(self.lib.pic_diag_reduce(self.h, C.byref(d)))
         return d
+
+    MOMENTS = ("rhoN", "rhoM", "picRhoN", "linearKE", "momentum_x", "momentum_y", "momentum_z", "j_x", "j_y", "j_z",
+               "rhoCharge")
+
+    def cell_moments(self, species=-1):
+        """calculateFields moments per cell [n_cells, 11] (columns as MOMENTS); species -1: all"""
+        out = np.zeros((self.n_cells, 11))
+        self._check(self.lib.pic_cell_moments(self.h, int(species), _ptr(out)))
+        return out
 
     def sync(self):
         self._check(self.lib.pic_sync(self.h))
