@@ -38,7 +38,7 @@ struct Species {
     int cur = 0;
     int* count = nullptr;   // [n_cells+1]
     int* start = nullptr;   // [n_cells+1]
-    int* cursor = nullptr;  // [n_cells]
+    int2* place = nullptr;  // [capacity] (cell, slot) from the counting pass
 };
 
 struct StageEvent {
@@ -75,6 +75,7 @@ struct pic_ctx {
     int64_t step = 0;
     bool rotate = false;
     bool sorted = false, counts_valid = false;
+    bool place_valid = false;                // Species::place matches the current layout
     // species
     std::vector<Species> sp;
     double Q0 = 1.0;
@@ -242,8 +243,8 @@ int alloc_species(pic_ctx* c, Species& s) {
     const int nc = c->hm.n_cells;
     s.count = c->alloc<int>(nc + 1);
     s.start = c->alloc<int>(nc + 1);
-    s.cursor = c->alloc<int>(nc + 1);
-    if (!s.count || !s.start || !s.cursor) return c->fail(PIC_ERR_HIP, "out of device memory (cell counts)");
+    s.place = c->alloc<int2>(cap);
+    if (!s.count || !s.start || !s.place) return c->fail(PIC_ERR_HIP, "out of device memory (cell counts)");
     return PIC_OK;
 }
 
@@ -278,6 +279,7 @@ MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
     a.rotate = c->rotate ? 1 : 0;
     a.hc_ref = c->hc_ref;
     a.count = S.count;
+    a.place = S.place;
     a.acc_cell = c->d_acc_cell;
     a.acc_point = c->d_acc_point;
     a.s42 = fixed_scale(c, s);
@@ -305,6 +307,7 @@ int do_move(pic_ctx* c, double dt) {
         if (int r = check_hip(c, e, "move kernel")) return r;
     }
     c->counts_valid = true;
+    c->place_valid = true;
     c->sorted = false;
     return PIC_OK;
 }
@@ -315,7 +318,7 @@ int do_move(pic_ctx* c, double dt) {
 int do_sort(pic_ctx* c) {
     pic_ctx::Scope sc(c, PIC_STAGE_SORT);
     const int nc = c->hm.n_cells;
-    if (!c->counts_valid) {
+    if (!c->counts_valid || !c->place_valid) {
         for (auto& s : c->sp) hipMemsetAsync(s.count, 0, sizeof(int) * (nc + 1), c->stream);
         for (int s = 0; s < (int)c->sp.size(); ++s) {
             hipError_t e = c->mixed() ? launch_deposit(move_args<float>(c, s, 0.0), kCellAverage, c->stream)
@@ -323,24 +326,23 @@ int do_sort(pic_ctx* c) {
             if (int r = check_hip(c, e, "count kernel")) return r;
         }
         c->counts_valid = true;
+        c->place_valid = true;
     }
     for (int s = 0; s < (int)c->sp.size(); ++s) {
         Species& S = c->sp[s];
         hipError_t e = launch_scan(S.count, S.start, nc + 1, c->d_scan_tmp, c->scan_bytes, c->stream);
         if (int r = check_hip(c, e, "scan")) return r;
-        hipMemsetAsync(S.cursor, 0, sizeof(int) * (nc + 1), c->stream);
         const int nb = 1 - S.cur;
         if (c->mixed())
-            e = launch_scatter(soa<float>(S, S.cur), soa<float>(S, nb), S.n, c->d_n + s, c->d_tets, S.start, S.cursor,
-                               c->stream);
+            e = launch_scatter(soa<float>(S, S.cur), soa<float>(S, nb), S.n, c->d_n + s, S.place, S.start, c->stream);
         else
-            e = launch_scatter(soa<double>(S, S.cur), soa<double>(S, nb), S.n, c->d_n + s, c->d_tets, S.start,
-                               S.cursor, c->stream);
+            e = launch_scatter(soa<double>(S, S.cur), soa<double>(S, nb), S.n, c->d_n + s, S.place, S.start, c->stream);
         if (int r = check_hip(c, e, "scatter")) return r;
         if (int r = check_hip(c, launch_store_count(S.start + nc, c->d_n + s, c->stream), "count")) return r;
         S.cur = nb;
     }
     c->sorted = true;
+    c->place_valid = false;   // the slots refer to the layout before this sort
     return PIC_OK;
 }
 
@@ -453,6 +455,7 @@ int do_mcc(pic_ctx* c, double dt) {
     pic_ctx::Scope sc(c, PIC_STAGE_MCC);
     if (c->xs_n[0] + c->xs_n[1] + c->xs_n[2] == 0) return c->fail(PIC_ERR_INVALID, "no cross sections set");
     const int r = c->mixed() ? do_mcc_t<float>(c, dt) : do_mcc_t<double>(c, dt);
+    c->place_valid = false;   // created parcels have no slot; the next sort recounts
     // created parcels are appended on the device: until the next synchronisation the
     // launch bound of the species that can grow is their capacity
     if (c->create_electron) c->sp[c->electron].n = c->sp[c->electron].capacity;
@@ -1173,6 +1176,7 @@ int pic_deposit(pic_ctx* c) {
         if (int r = check_hip(c, e, "deposit")) return r;
     }
     c->counts_valid = true;
+    c->place_valid = true;
     if (int r = do_deposit_finish(c)) return r;
     return sync_state(c);
 }

@@ -87,22 +87,27 @@ struct PartSoA {
 // take no part.  Must be called by all lanes of the wave.
 __device__ __forceinline__ int wave_aggregated_inc(int* counter, int key) {
     const int lane = __lane_id();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t active = __ballot(key >= 0);
-    int result = -1;
+    // group the lanes by key without touching memory: each lane learns its group's leader
+    // and its rank in the group, each leader the group size
+    int leader = lane, rank = 0, cnt = 0;
     while (active) {
-        int leader = __ffsll((unsigned long long)active) - 1;
-        int lk = __shfl(key, leader);
-        uint64_t mask = __ballot(key == lk) & active;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(counter + lk, __popcll(mask));
-        base = __shfl(base, leader);
+        const int l = __ffsll((unsigned long long)active) - 1;
+        const int lk = __shfl(key, l);
+        const uint64_t mask = __ballot(key == lk) & active;
         if ((mask >> lane) & 1ull) {
-            uint64_t lower = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-            result = base + __popcll(lower);
+            leader = l;
+            rank = __popcll(mask & below);
         }
+        if (lane == l) cnt = __popcll(mask);
         active &= ~mask;
     }
-    return result;
+    // then every leader adds its group size at once: one atomic round trip per wave
+    int base = 0;
+    if (key >= 0 && lane == leader) base = atomicAdd(counter + key, cnt);
+    base = __shfl(base, leader);
+    return key >= 0 ? base + rank : -1;
 }
 
 // Lorentz transformation of two parcels to their centre-of-momentum frame and back, the

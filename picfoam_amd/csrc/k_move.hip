@@ -211,8 +211,10 @@ __global__ void __launch_bounds__(256) k_move(MoveArgs<UT> a) {
             if (br) atomicOr(a.flags, kFlagTrackBroken);
         }
     }
-    // cell count (sort histogram; for CellAverage also the deposit)
-    wave_aggregated_inc(a.count, newcell);
+    // cell count (sort histogram; for CellAverage also the deposit) and the parcel's slot
+    // in its new cell, which the sort's scatter uses directly
+    const int slot = wave_aggregated_inc(a.count, newcell);
+    if (a.place && i < n) a.place[i] = make_int2(newcell, slot);
     if constexpr (DEP == kVolumeWeighting) {
         if (newcell >= 0) {
             const TetGeo& g = a.tets[t];
@@ -256,7 +258,8 @@ __global__ void __launch_bounds__(256) k_deposit(MoveArgs<UT> a) {
         t = a.p.tet[i];
         if (t >= 0) { cell = a.tets[t].cell; y1 = a.p.y1[i]; y2 = a.p.y2[i]; y3 = a.p.y3[i]; }
     }
-    wave_aggregated_inc(a.count, cell);
+    const int slot = wave_aggregated_inc(a.count, cell);
+    if (a.place && i < n) a.place[i] = make_int2(cell, slot);
     if constexpr (DEP == kVolumeWeighting) {
         if (cell >= 0) {
             const TetGeo& g = a.tets[t];
@@ -355,28 +358,24 @@ __global__ void k_init(InitArgs<UT> a) {
 }
 
 // Counting-sort scatter (buildCellOccupancy, PICCloud.C:327-354): live parcels go to
-// start[cell] + slot; the slot comes from a wave-aggregated atomic, so the order inside a
-// cell is not fixed; every consumer that depends on it orders the cell's parcels by id.
+// start[cell] + slot, with (cell, slot) handed over by the counting pass (the slot comes
+// from a wave-aggregated atomic there, so the order inside a cell is not fixed; every
+// consumer that depends on it orders the cell's parcels by id or by a random key).
 template <class UT>
 __global__ void __launch_bounds__(256) k_scatter(PartSoA<UT> src, PartSoA<UT> dst, int64_t n_bound,
-                                                  const int64_t* __restrict__ n_live, const TetGeo* __restrict__ tets,
-                                                  const int* __restrict__ start, int* __restrict__ cursor) {
+                                                  const int64_t* __restrict__ n_live, const int2* __restrict__ place,
+                                                  const int* __restrict__ start) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // the device count can exceed the arrays after a capacity overflow (flagged)
     const int64_t n = min(*n_live, n_bound);
-    int cell = -1, t = -1;
-    if (i < n) {
-        t = src.tet[i];
-        if (t >= 0) cell = tets[t].cell;
-    }
-    const int slot = wave_aggregated_inc(cursor, cell);
-    if (cell >= 0) {
-        const int64_t o = (int64_t)start[cell] + slot;
-        dst.y1[o] = src.y1[i]; dst.y2[o] = src.y2[i]; dst.y3[o] = src.y3[i];
-        dst.ux[o] = src.ux[i]; dst.uy[o] = src.uy[i]; dst.uz[o] = src.uz[i];
-        dst.tet[o] = t;
-        dst.id[o] = src.id[i];
-    }
+    if (i >= n) return;
+    const int2 pc = place[i];
+    if (pc.x < 0) return;
+    const int64_t o = (int64_t)start[pc.x] + pc.y;
+    dst.y1[o] = src.y1[i]; dst.y2[o] = src.y2[i]; dst.y3[o] = src.y3[i];
+    dst.ux[o] = src.ux[i]; dst.uy[o] = src.uy[i]; dst.uz[o] = src.uz[i];
+    dst.tet[o] = src.tet[i];
+    dst.id[o] = src.id[i];
 }
 
 // ---------------------------------------------------------------- instantiation / launch
@@ -483,19 +482,19 @@ hipError_t launch_init(const InitArgs<float>& a, hipStream_t s) { return launch_
 
 template <class UT>
 static hipError_t launch_scatter_t(const PartSoA<UT>& src, const PartSoA<UT>& dst, int64_t n, const int64_t* n_live,
-                                   const TetGeo* tets, const int* start, int* cursor, hipStream_t s) {
+                                   const int2* place, const int* start, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, n_live, tets,
-                       start, cursor);
+    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, n_live, place,
+                       start);
     return hipGetLastError();
 }
 hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int64_t* n_live,
-                          const TetGeo* tets, const int* start, int* cursor, hipStream_t s) {
-    return launch_scatter_t(src, dst, n, n_live, tets, start, cursor, s);
+                          const int2* place, const int* start, hipStream_t s) {
+    return launch_scatter_t(src, dst, n, n_live, place, start, s);
 }
 hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int64_t* n_live,
-                          const TetGeo* tets, const int* start, int* cursor, hipStream_t s) {
-    return launch_scatter_t(src, dst, n, n_live, tets, start, cursor, s);
+                          const int2* place, const int* start, hipStream_t s) {
+    return launch_scatter_t(src, dst, n, n_live, place, start, s);
 }
 
 }  // namespace picmi

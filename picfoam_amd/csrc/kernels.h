@@ -17,6 +17,7 @@ struct MoveArgs {
     double dt;
     int do_push, rotate, hc_ref;
     int* count;                      // per-cell parcel count of this species
+    int2* place;                     // per parcel (new cell, slot in the cell) for the sort, or null
     unsigned long long* acc_cell;    // VolumeWeighting fixed-point accumulators
     unsigned long long* acc_point;
     double s42;                      // q*w/Q0 * 2^42
@@ -65,11 +66,12 @@ hipError_t launch_positions(const PartSoA<float>& p, int64_t n, const TetGeo* te
                             const double* pts, double* x, hipStream_t s);
 hipError_t launch_init(const InitArgs<double>& a, hipStream_t s);
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s);
-// n: launch bound; n_live: live source count on the device
+// n: launch bound; n_live: live source count on the device; place: (cell, slot) per parcel
+// from the counting pass (cell < 0: dropped)
 hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int64_t* n_live,
-                          const TetGeo* tets, const int* start, int* cursor, hipStream_t s);
+                          const int2* place, const int* start, hipStream_t s);
 hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int64_t* n_live,
-                          const TetGeo* tets, const int* start, int* cursor, hipStream_t s);
+                          const int2* place, const int* start, hipStream_t s);
 
 // ---------------------------------------------------------------- fields (k_field.hip)
 // exclusive scan of n+1 ints (last = total); temp storage sized by scan_temp_bytes
