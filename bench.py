@@ -30,6 +30,22 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s + %HBM peak; Poisson ms/step learned-guess vs plain PCG"
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hot_kernels_thermalisation32_fp64_traffic.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (FP64, 32^3, 64+64 ppc), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
+                return int(f[2])
+    except OSError:
+        return None
+    return None
+
+
 def state_bytes(precision):
     u = 4 if precision == "mixed" else 8
     return 3 * 8 + 3 * u + 4 + 4
@@ -235,9 +251,15 @@ def main():
     dom = max(cand, key=lambda k: cand[k][0] * (move_n if k == "move" else coul_n))
     t_ms, b = cand[dom]
     achieved = b / (t_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": {"move": "k_move", "coulomb": "k_coulomb"}[dom], "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    kname = {"move": "k_move", "coulomb": "k_coulomb_nanbu"}[dom]
+    default_workload = args.ncell == 32 and args.ppc == 64 and args.precision == "fp64"
+    traffic = profiled_traffic(kname) if default_workload else None
+    roof = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_launch": b, "avg_launch_ms": round(t_ms, 4)}
+    if dom == "coulomb":
+        roof["limiter"] = ("VALU: FP64 scattering math (log, exp, sincos, sqrt, divisions) and the Philox streams "
+                           "per collision; the HBM fraction is low by construction")
     stage_ms = {k: round(v[0] / args.steps, 4) for k, v in stages.items() if v[1]}
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
