@@ -74,6 +74,7 @@ struct pic_ctx {
     int sort_interval = 1;
     int64_t step = 0;
     bool rotate = false;
+    bool e_zero = true;                      // E is zero in every cell (never set, no field solve)
     bool sorted = false, counts_valid = false;
     bool place_valid = false;                // Species::place matches the current layout
     // species
@@ -275,7 +276,11 @@ MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
     a.k = (UT)(S.charge * (0.5 * dt) / S.mass);
     a.kv = (UT)(S.charge * dt / S.mass);
     a.dt = dt;
-    a.do_push = (S.charge != 0.0) ? 1 : 0;
+    // with E and B zero everywhere the Boris push (rotation skipped, BorisPusher.C:42-55) is
+    // the identity: the kernel then leaves U untouched instead of rewriting it
+    const bool identity_push = c->e_zero && !c->rotate &&
+                               (c->pusher == kPushBoris || c->pusher == kPushBorisNR || c->pusher == kPushNone);
+    a.do_push = (S.charge != 0.0 && !identity_push) ? 1 : 0;
     a.rotate = c->rotate ? 1 : 0;
     a.hc_ref = c->hc_ref;
     a.count = S.count;
@@ -557,6 +562,7 @@ int do_poisson(pic_ctx* c, pic_poisson_stats* st) {
         g.E_f = c->d_Ef;
         if (int r = check_hip(c, launch_grad(g, c->stream), "grad")) return r;
     }
+    c->e_zero = false;
     if (int r = update_field_derived(c)) return r;
     hipEventRecord(e1, c->stream);
     hipEventSynchronize(e1);
@@ -889,10 +895,14 @@ int pic_set_cell_field(pic_ctx* c, int32_t field, const double* v) {
     if (!c || !v) return PIC_ERR_INVALID;
     const int nc = c->hm.n_cells;
     switch (field) {
-        case PIC_FIELD_E:
+        case PIC_FIELD_E: {
             hipMemcpyAsync(c->d_E, v, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, c->stream);
+            bool any = false;
+            for (size_t i = 0; i < 3 * (size_t)nc; ++i) if (v[i] != 0.0) { any = true; break; }
+            c->e_zero = !any;
             if (int r = update_field_derived(c)) return r;
             break;
+        }
         case PIC_FIELD_B: {
             hipMemcpyAsync(c->d_B, v, sizeof(double) * 3 * nc, hipMemcpyHostToDevice, c->stream);
             std::vector<float> f(3 * (size_t)nc);

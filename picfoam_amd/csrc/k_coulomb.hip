@@ -22,22 +22,28 @@ namespace picmi {
 namespace {
 
 // NanbuCollision.C:137-159 / PerezRelativisticCollision.C:198-219 in the arithmetic M.  The
-// three regimes are evaluated for every lane with one log, so lanes of different s do not
-// diverge: exp(-A) + 2 U sinh(A) = e^-A (1 - U) + U e^A.
+// three regimes are evaluated for every lane with one exp and one log, so lanes of
+// different s do not diverge:
+//   (1/A) log(exp(-A) + 2 U sinh(A)) = 1 + (1/A) log(U + (1 - U) exp(-2A)),
+// with 1/A = invA on the polynomial branch (s < 3) and A = 3 exp(-s) below s = 6.  exp(-s)
+// is evaluated only when some lane of the wave needs it.
 template <class M>
 __device__ __forceinline__ M cos_chi(M s, M U) {
     const bool small = s < (M)0.1, mid = s < (M)3, big = s < (M)6;
     const M invA = (M)0.00569578 +
                    ((M)0.95602 + ((M)-0.508139 + ((M)0.479139 + ((M)-0.12789 + (M)0.0238957 * s) * s) * s) * s) * s;
-    M A = mid ? (M)1 / invA : (M)3 * exp(-s);
-    const M coef = mid ? invA : (M)1 / A;
-    if (small || !big) A = (M)1;  // keeps the unused branch finite
-    const M eA = exp(A);
+    M Abig = (M)1;
+    if (__any(!mid && big)) Abig = (M)3 * exp(-s);
+    const M r = (M)1 / (mid ? invA : Abig);
+    M A = mid ? r : Abig;
+    const M coef = mid ? invA : r;
+    if (small || !big) A = (M)0;  // keeps the unused branch finite
+    const M e2 = exp((M)-2 * A);
     const M Us = U < (M)0.0001 ? (M)0.0001 : U;
-    const M arg = small ? Us : ((M)1 - U) / eA + U * eA;
+    const M arg = small ? Us : U + ((M)1 - U) * e2;
     const M L = log(arg);
-    const M cc = small ? (M)1 + s * L : (big ? coef * L : (M)2 * U - (M)1);
-    // rounding can put coef * L an ulp outside [-1, 1] (U near 0 or 1), which would make
+    const M cc = small ? (M)1 + s * L : (big ? (M)1 + coef * L : (M)2 * U - (M)1);
+    // rounding can put the value an ulp outside [-1, 1] (U near 0 or 1), which would make
     // sin(chi) = sqrt(1 - cos^2) a NaN; the reference formula has the same edge
     return cc > (M)1 ? (M)1 : (cc < (M)-1 ? (M)-1 : cc);
 }
