@@ -92,7 +92,8 @@ struct pic_ctx {
     bool has_space = false;
     unsigned long long *d_acc_cell = nullptr, *d_acc_point = nullptr;
     uint32_t* d_flags = nullptr;
-    unsigned long long* d_stats = nullptr;  // 0 deleted, 1 guard, 2 coulomb, 3..6 mcc
+    // 0 deleted, 1 guard, 3..6 mcc; Coulomb collisions in kNcollSlots counters from kStatsNcoll
+    unsigned long long* d_stats = nullptr;
     int64_t* d_n = nullptr;                  // device parcel counts per species (authoritative)
     int* d_big_list = nullptr;               // cells left to the large-cell collision passes
     double* d_moments = nullptr;             // cell moments scratch [n_cells][11]
@@ -395,7 +396,7 @@ int do_coulomb(pic_ctx* c, double dt) {
     a.pairing = c->pairing;
     a.weight_correction = c->weight_corr;
     a.flags = c->d_flags;
-    a.ncoll = c->d_stats + 2;
+    a.ncoll = c->d_stats + kStatsNcoll;
     return check_hip(c, launch_coulomb(a, c->mixed(), c->stream), "coulomb kernel");
 }
 
@@ -777,7 +778,7 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->d_acc_cell = c->alloc<unsigned long long>(nc);
     c->d_acc_point = c->alloc<unsigned long long>(np);
     c->d_flags = c->alloc<uint32_t>(1);
-    c->d_stats = c->alloc<unsigned long long>(16);
+    c->d_stats = c->alloc<unsigned long long>(kStatsWords);
     c->d_n = c->alloc<int64_t>(kMaxSpecies);
     c->d_big_list = c->alloc<int>(nc);
     c->d_big_n = c->alloc<int>(1);
@@ -1215,13 +1216,15 @@ int step_body(pic_ctx* c, double dt, pic_poisson_stats* ps) {
 }
 
 void read_step_stats(pic_ctx* c, const pic_poisson_stats& ps, pic_step_stats* st) {
-    unsigned long long h[16];
-    hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost);
+    std::vector<unsigned long long> h(kStatsWords);
+    hipMemcpy(h.data(), c->d_stats, sizeof(unsigned long long) * kStatsWords, hipMemcpyDeviceToHost);
+    unsigned long long ncoll = 0;
+    for (int k = 0; k < kNcollSlots; ++k) ncoll += h[kStatsNcoll + k * kNcollStride];
     std::memset(st, 0, sizeof(*st));
     for (size_t s = 0; s < c->sp.size(); ++s) st->n_parcels[s] = c->sp[s].n;
     st->n_deleted = (int64_t)h[0];
     st->n_track_guard = (int64_t)h[1];
-    st->coulomb_collisions = (int64_t)h[2];
+    st->coulomb_collisions = (int64_t)ncoll;
     st->mcc_candidates = (int64_t)h[3];
     st->mcc_elastic = (int64_t)h[4];
     st->mcc_excitation = (int64_t)h[5];
@@ -1231,7 +1234,7 @@ void read_step_stats(pic_ctx* c, const pic_poisson_stats& ps, pic_step_stats* st
 
 int pic_step(pic_ctx* c, double dt, pic_step_stats* st) {
     if (!c) return PIC_ERR_INVALID;
-    hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * 16, c->stream);
+    hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kStatsWords, c->stream);
     pic_poisson_stats ps{};
     if (int r = step_body(c, dt, &ps)) return r;
     if (int r = sync_state(c)) return r;
@@ -1241,7 +1244,7 @@ int pic_step(pic_ctx* c, double dt, pic_step_stats* st) {
 
 int pic_run(pic_ctx* c, double dt, int64_t n_steps, pic_step_stats* st) {
     if (!c || n_steps < 0) return PIC_ERR_INVALID;
-    hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * 16, c->stream);
+    hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kStatsWords, c->stream);
     pic_poisson_stats ps{};
     for (int64_t k = 0; k < n_steps; ++k)
         if (int r = step_body(c, dt, &ps)) return r;

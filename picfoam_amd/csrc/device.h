@@ -30,6 +30,85 @@ __device__ __forceinline__ vec3<T> cross(vec3<T> a, vec3<T> b) {
 template <class T>
 __device__ __forceinline__ T mag(vec3<T> a) { return sqrt(dot(a, a)); }
 
+// FP64 elementary functions for the collision kernels, accurate to a few ulp over the
+// arguments those kernels pass, in about a third of the instructions of the general
+// library routines (no special-value or denormal-argument handling).  The float overloads
+// call the library.
+//
+// ln 2 and pi/2 split into a head with 32 significant bits and a tail, so n ln2_hi is exact
+// for |n| < 2^20
+constexpr double kLn2Hi = 0x1.62e42fefp-1, kLn2Lo = 7.440617110012397e-11;
+constexpr double kPio2Hi = 0x1.921fb544p+0, kPio2Lo = 6.077100506506192e-11;
+
+// n / d from the hardware reciprocal estimate, two Newton steps and one correction of the
+// quotient (d normal, not near the exponent limits)
+__device__ __forceinline__ double quot(double n, double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    const double q = n * r;
+    return fma(r, fma(-d, q, n), q);
+}
+
+// log x, x positive and normal: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh z
+// with z = (m - 1)/(m + 1), |z| < 0.1716, by the series 2 sum z^(2k+1)/(2k+1) to k = 10
+// (next term below 1e-18 relative)
+__device__ __forceinline__ double log_pos(double x) {
+    double m = __builtin_amdgcn_frexp_mant(x);
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.70710678118654752) { m *= 2.0; --e; }
+    const double z = quot(m - 1.0, m + 1.0);
+    const double w = z * z;
+    double p = 1.0 / 21.0;
+    p = fma(p, w, 1.0 / 19.0); p = fma(p, w, 1.0 / 17.0); p = fma(p, w, 1.0 / 15.0);
+    p = fma(p, w, 1.0 / 13.0); p = fma(p, w, 1.0 / 11.0); p = fma(p, w, 1.0 / 9.0);
+    p = fma(p, w, 1.0 / 7.0); p = fma(p, w, 1.0 / 5.0); p = fma(p, w, 1.0 / 3.0);
+    const double lm = fma(2.0 * z * w, p, 2.0 * z);
+    const double de = (double)e;
+    return fma(de, kLn2Hi, fma(de, kLn2Lo, lm));
+}
+__device__ __forceinline__ float log_pos(float x) { return logf(x); }
+
+// exp x for x in [-700, 700]: x = n ln 2 + r, |r| <= ln2 / 2, Taylor series of e^r to r^13
+// (remainder below 5e-18 relative), scaled by 2^n
+__device__ __forceinline__ double exp_small(double x) {
+    const double n = rint(x * 1.4426950408889634);
+    const double r = fma(-n, kLn2Lo, fma(-n, kLn2Hi, x));
+    double p = 1.0 / 6227020800.0;                        // 1/13!
+    p = fma(p, r, 1.0 / 479001600.0); p = fma(p, r, 1.0 / 39916800.0); p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0); p = fma(p, r, 1.0 / 40320.0); p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0); p = fma(p, r, 1.0 / 120.0); p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0); p = fma(p, r, 0.5); p = fma(p, r, 1.0); p = fma(p, r, 1.0);
+    return __builtin_amdgcn_ldexp(p, (int)n);
+}
+__device__ __forceinline__ float exp_small(float x) { return expf(x); }
+
+// sin and cos of 2 pi u for u in [0, 1]: 4u = q + f with q the nearest integer (the
+// quadrant) and |f| <= 1/2 exact, theta = f pi/2 in [-pi/4, pi/4], Taylor series of sin to
+// theta^17 and of cos to theta^16 (remainders below 1e-17), rotated by q quarter turns
+__device__ __forceinline__ void sincos_turn(double u, double* s, double* c) {
+    const double t = 4.0 * u;
+    const double q = rint(t);
+    const double f = t - q;
+    const double th = fma(f, kPio2Hi, f * kPio2Lo);
+    const double w = th * th;
+    double ps = 1.0 / 355687428096000.0;                  // 1/17!
+    ps = fma(ps, -w, 1.0 / 1307674368000.0); ps = fma(ps, -w, 1.0 / 6227020800.0);
+    ps = fma(ps, -w, 1.0 / 39916800.0); ps = fma(ps, -w, 1.0 / 362880.0); ps = fma(ps, -w, 1.0 / 5040.0);
+    ps = fma(ps, -w, 1.0 / 120.0); ps = fma(ps, -w, 1.0 / 6.0);
+    const double sn = fma(th * w, -ps, th);
+    double pc = 1.0 / 20922789888000.0;                   // 1/16!
+    pc = fma(pc, -w, 1.0 / 87178291200.0); pc = fma(pc, -w, 1.0 / 479001600.0);
+    pc = fma(pc, -w, 1.0 / 3628800.0); pc = fma(pc, -w, 1.0 / 40320.0); pc = fma(pc, -w, 1.0 / 720.0);
+    pc = fma(pc, -w, 1.0 / 24.0); pc = fma(pc, -w, 0.5);
+    const double cs = fma(w, -pc, 1.0);
+    const int k = (int)q & 3;
+    const double a = (k & 1) ? cs : sn, b = (k & 1) ? sn : cs;
+    *s = (k & 2) ? -a : a;
+    *c = ((k + 1) & 2) ? -b : b;
+}
+__device__ __forceinline__ void sincos_turn(float u, float* s, float* c) { sincospif(2.f * u, s, c); }
+
 // Philox4x32-10 (Salmon et al., SC'11): counter (4 x u32), key (2 x u32).
 __device__ __forceinline__ uint4 philox4x32(uint4 c, uint2 k) {
 #pragma unroll

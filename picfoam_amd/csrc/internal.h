@@ -18,7 +18,7 @@ constexpr double kFixedScale = 4398046511104.0;  // 2^42 units per charge quantu
 constexpr int kMaxTrackSteps = 1000;
 constexpr int kMaxSpecies = 8;
 // per-cell index-list slots in LDS of the collision kernels (per-cell pass / large cells)
-constexpr int kCoulombSlots = 256, kCoulombSlotsBig = 2048;
+constexpr int kCoulombSlots = 128, kCoulombSlotsBig = 2048;
 constexpr int kMccSlots = 512, kMccSlotsBig = 4096;
 
 enum Pusher { kPushNone = 0, kPushBoris = 1, kPushBorisNR = 2, kPushVay = 3, kPushHC = 4 };

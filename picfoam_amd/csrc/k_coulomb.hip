@@ -33,23 +33,21 @@ __device__ __forceinline__ M cos_chi(M s, M U) {
     const M invA = (M)0.00569578 +
                    ((M)0.95602 + ((M)-0.508139 + ((M)0.479139 + ((M)-0.12789 + (M)0.0238957 * s) * s) * s) * s) * s;
     M Abig = (M)1;
-    if (__any(!mid && big)) Abig = (M)3 * exp(-s);
+    if (__any(!mid && big)) Abig = (M)3 * exp_small(-s);
     const M r = (M)1 / (mid ? invA : Abig);
     M A = mid ? r : Abig;
     const M coef = mid ? invA : r;
     if (small || !big) A = (M)0;  // keeps the unused branch finite
-    const M e2 = exp((M)-2 * A);
+    const M e2 = exp_small((M)-2 * A);
     const M Us = U < (M)0.0001 ? (M)0.0001 : U;
     const M arg = small ? Us : U + ((M)1 - U) * e2;
-    const M L = log(arg);
+    const M L = log_pos(arg);
     const M cc = small ? (M)1 + s * L : (big ? (M)1 + coef * L : (M)2 * U - (M)1);
     // rounding can put the value an ulp outside [-1, 1] (U near 0 or 1), which would make
     // sin(chi) = sqrt(1 - cos^2) a NaN; the reference formula has the same edge
     return cc > (M)1 ? (M)1 : (cc < (M)-1 ? (M)-1 : cc);
 }
 
-__device__ __forceinline__ void sincos_2pi(double u, double* s, double* c) { sincospi(2.0 * u, s, c); }
-__device__ __forceinline__ void sincos_2pi(float u, float* s, float* c) { sincospif(2.f * u, s, c); }
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -64,8 +62,8 @@ enum : int { kIntraEven = 0, kIntraOddTA = 1, kIntraOddNY = 2, kInterEqual = 3, 
 // g1_xo + l, l = m, m + g1_stride, ... < g1_lmax; the remaining units form group 2 alike.
 struct StageK {
     void* u[6];
-    const int* permP;
-    const int* permQ;
+    int cP, cQ;              // cached velocities: x component offsets of P and Q in the cell's LDS cache
+    int permP, permQ;        // offsets of the P and Q index lists from the first list in LDS
     double mP, qP, mQ, qQ, wP, wQ;
     double nP, nQ, nPQ, lnL;
     // Nanbu constants: cst = (qP qQ / eps0 mu)^2 (nP nQ / nPQ) dt / (4 pi), fP = mQ/(mP+mQ),
@@ -87,8 +85,9 @@ __device__ __forceinline__ void finish_stage(StageK& d, double dt) {
 }
 
 // collision t of unit u; false once the unit is exhausted
-__device__ __forceinline__ bool decode(const StageK& d, int u, int t, int& iP, int& iQ, int& k) {
-    const int* P = d.permP;
+__device__ __forceinline__ bool decode(const StageK& d, const int* lists, int u, int t, int& iP, int& iQ, int& k) {
+    const int* P = lists + d.permP;
+    const int* Q = lists + d.permQ;
     switch (d.kind) {
         case kIntraEven:
             if (t) return false;
@@ -109,7 +108,7 @@ __device__ __forceinline__ bool decode(const StageK& d, int u, int t, int& iP, i
             return false;
         case kInterEqual:
             if (t) return false;
-            iP = P[u]; iQ = d.permQ[d.n - 1 - u]; k = u;
+            iP = P[u]; iQ = Q[d.n - 1 - u]; k = u;
             return true;
         default: {
             int l, m, xo, yo, ko;
@@ -122,7 +121,7 @@ __device__ __forceinline__ bool decode(const StageK& d, int u, int t, int& iP, i
                 if (l >= d.g2_lmax) return false;
                 xo = d.g2_xo; yo = d.g2_yo; ko = d.g2_ko;
             }
-            iP = P[xo + l]; iQ = d.permQ[yo + m]; k = ko + l;
+            iP = P[xo + l]; iQ = Q[yo + m]; k = ko + l;
             return true;
         }
     }
@@ -133,29 +132,30 @@ template <class M>
 __device__ __forceinline__ void nanbu_scatter(vec3<M>& uP, vec3<M>& uQ, const StageK& d, int calc_bmin, double debye,
                                               double u1, double u2) {
     const vec3<M> g = uP - uQ;
-    const M gm = mag(g);
-    const M gp = sqrt(g.y * g.y + g.z * g.z);
-    if (!(gm > (M)0) || !(gp > (M)0)) return;  // the reference divides by zero here
+    const M g2 = dot(g, g), gp2 = g.y * g.y + g.z * g.z;
+    if (!(g2 > (M)0) || !(gp2 > (M)0)) return;  // the reference divides by zero here
+    const M ig = rsqrt(g2), igp = rsqrt(gp2);
+    const M gm = g2 * ig, gp = gp2 * igp;
     M lnL;
     if (d.lnL == 0.0) {
-        const M ig2 = (M)1 / (gm * gm);
+        const M ig2 = ig * ig;
         M b0 = (M)d.b0c * ig2;
         if (calc_bmin) {
             const M hb = (M)d.hbc / gm;
             b0 = hb > fabs(b0) ? hb : fabs(b0);
         }
-        const M l = (M)0.5 * log((M)1 + ((M)debye * (M)debye) / (b0 * b0));
+        const M l = (M)0.5 * log_pos((M)1 + ((M)debye * (M)debye) / (b0 * b0));
         lnL = l > (M)2 ? l : (M)2;
     } else {
         lnL = (M)d.lnL;
     }
     // s = lnL/(4 pi) (qP qQ / eps0 mu)^2 (nP nQ / nPQ) g^-3 dt
-    const M s = lnL * (M)d.cst * ((M)1 / (gm * gm * gm));
+    const M s = lnL * (M)d.cst * (ig * ig * ig);
     const M cc = cos_chi<M>(s, (M)u1);
-    const M sc = sqrt((M)1 - cc * cc);
+    const M sc2 = (M)1 - cc * cc;
+    const M sc = sc2 > (M)0 ? sc2 * rsqrt(sc2) : (M)0;
     M se, ce;
-    sincos_2pi((M)u2, &se, &ce);
-    const M igp = (M)1 / gp;
+    sincos_turn((M)u2, &se, &ce);
     const vec3<M> h = mk3<M>(gp * ce, -(g.y * g.x * ce + gm * g.z * se) * igp, -(g.z * g.x * ce - gm * g.y * se) * igp);
     const vec3<M> dv = ((M)1 - cc) * g + sc * h;
     uP = uP - (M)d.fP * dv;
@@ -179,7 +179,7 @@ __device__ __forceinline__ void perez_scatter(vec3<double>& UP, vec3<double>& UQ
             const double hb = kH / (2.0 * pm);
             b0 = hb > fabs(b0) ? hb : fabs(b0);
         }
-        const double l = 0.5 * log(1.0 + (debye * debye) / (b0 * b0));
+        const double l = 0.5 * log_pos(1.0 + (debye * debye) / (b0 * b0));
         lnL = l > 2.0 ? l : 2.0;
     } else {
         lnL = d.lnL;
@@ -195,20 +195,27 @@ __device__ __forceinline__ void perez_scatter(vec3<double>& UP, vec3<double>& UQ
     const double cc = cos_chi<double>(s, u1);
     const double sc = sqrt(1.0 - cc * cc);
     double se, ce;
-    sincos_2pi(u2, &se, &ce);
+    sincos_turn(u2, &se, &ce);
     com_leave(f, mP, mQ, cc, sc, se, ce, UP, UQ);
 }
 
 // one collision of parcel iP (species of d's P side) with parcel iQ; MODEL 0 Nanbu, 1 Perez;
 // WC: Nanbu weight correction active
-template <class UT, int MODEL, bool WC>
+// CU: velocities in the LDS cache (component stride cs), else in the species arrays
+template <class UT, int MODEL, bool WC, bool CU>
 __device__ __forceinline__ void collide(const CoulombArgs& a, const StageK& d, int cell, int iP, int iQ, int k,
-                                        double debye) {
+                                        double debye, UT* cache, int cs) {
     uint4 o = rng4(a.seed, (uint32_t)cell, (uint32_t)a.step, ((uint32_t)d.stage << 24) | (uint32_t)k,
                    kTagCoulomb << 24);
     const double u1 = u01(o.x, o.y), u2 = u01(o.z, o.w);
-    UT* pxP = (UT*)d.u[0]; UT* pyP = (UT*)d.u[1]; UT* pzP = (UT*)d.u[2];
-    UT* pxQ = (UT*)d.u[3]; UT* pyQ = (UT*)d.u[4]; UT* pzQ = (UT*)d.u[5];
+    UT *pxP, *pyP, *pzP, *pxQ, *pyQ, *pzQ;
+    if (CU) {
+        pxP = cache + d.cP; pyP = pxP + cs; pzP = pyP + cs;
+        pxQ = cache + d.cQ; pyQ = pxQ + cs; pzQ = pyQ + cs;
+    } else {
+        pxP = (UT*)d.u[0]; pyP = (UT*)d.u[1]; pzP = (UT*)d.u[2];
+        pxQ = (UT*)d.u[3]; pyQ = (UT*)d.u[4]; pzQ = (UT*)d.u[5];
+    }
     const vec3<UT> preP = mk3<UT>(pxP[iP], pyP[iP], pzP[iP]);
     const vec3<UT> preQ = mk3<UT>(pxQ[iQ], pyQ[iQ], pzQ[iQ]);
     vec3<UT> nP = preP, nQ = preQ;
@@ -257,12 +264,12 @@ __device__ __forceinline__ uint64_t shuffle_key(uint64_t seed, int64_t step, int
     return ((uint64_t)o.x << 24) | (o.y >> 8);
 }
 
-// val[0..n) = array indices of the cell's parcels of one species ordered by their 56-bit
+// val[0..n) = base + cell-local indices of the cell's parcels of one species ordered by their 56-bit
 // random key.  Up to 256 parcels the key carries the cell-local index in its low byte, so
 // keys are distinct and each parcel's place is the number of smaller keys (no barriers in
 // the count; eight keys per LDS round trip).  Larger cells use a bitonic sort.
 __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
-                           uint64_t* key, int* val) {
+                           uint64_t* key, int* val, int base) {
     if (n <= 256) {
         const int n8 = (n + 7) & ~7;
         for (int k = threadIdx.x; k < n8; k += blockDim.x)
@@ -292,7 +299,7 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
                      (int)(x5 < kk) + (int)(x6 < kk) + (int)(x7 < kk);
             }
             for (int off = (1 << lg) >> 1; off > 0; off >>= 1) r += __shfl_xor(r, off);
-            if (act && g == 0) val[r] = first + k;
+            if (act && g == 0) val[r] = base + k;
         }
         __syncthreads();
         return;
@@ -302,7 +309,7 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
     for (int k = threadIdx.x; k < np2; k += blockDim.x) {
         if (k < n) {
             key[k] = (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)(k & 255);
-            val[k] = first + k;
+            val[k] = base + k;
         } else {
             key[k] = ~0ull;
             val[k] = -1;
@@ -327,14 +334,19 @@ __device__ __forceinline__ int cell_max_count(const CoulombArgs& a, int cell) {
     return m;
 }
 
-// every pairing stage of one cell by one wave; maxp = LDS slots per index list
-template <class UT, int MODEL, bool WC>
-__device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t* lds, StageK* sk) {
+// every pairing stage of one cell by one wave; maxp = LDS slots per index list.  With CU the
+// velocities of the (at most two) species a stage needs are held in LDS for the whole cell:
+// read once and written back once, coalesced, instead of gathered per collision.
+template <class UT, int MODEL, bool WC, bool CU>
+__device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t* lds, StageK* sk) {
     const int kstride = maxp + 8;
     uint64_t* keyA = lds;
     uint64_t* keyB = lds + kstride;
     int* valA = (int*)(lds + 2 * kstride);
     int* valB = valA + maxp;
+    UT* cache = (UT*)(lds + 2 * kstride + maxp);   // CU: slot q component c at cache[(3q + c) maxp]
+    uint32_t* idc = (uint32_t*)(cache + 6 * maxp);  // CU: slot q parcel ids at idc[q maxp]
+    int hold0 = -1, hold1 = -1;                     // species held by the two cache slots
     const int ns = a.n_species;
     const double Vc = a.V[cell];
     // CoulombCollisionModel::debyeLength (CoulombCollisionModel.C:249-310), parcels summed
@@ -390,6 +402,66 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
     for (int s = 0; s < ns; ++s)
         if (a.charge[s] != 0.0) last = s;
     const bool ny = a.pairing == 1;
+    // cache slots (CU): write slot q back to the arrays of species sp
+    auto put_slot = [&](int q, int sp) {
+        const int f0 = a.start[sp][cell], n = a.start[sp][cell + 1] - f0;
+        const UT* cx = cache + (size_t)(3 * q) * maxp;
+        UT* gx = (UT*)a.ux[sp] + f0;
+        UT* gy = (UT*)a.uy[sp] + f0;
+        UT* gz = (UT*)a.uz[sp] + f0;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            gx[k] = cx[k]; gy[k] = cx[maxp + k]; gz[k] = cx[2 * maxp + k];
+        }
+    };
+    // fill slot qa with species sa and/or slot qb with species sb (a negative slot is left
+    // alone); the loads of both are in flight together
+    auto get_slots = [&](int qa, int sa, int qb, int sb) {
+        const int fa = qa >= 0 ? a.start[sa][cell] : 0, na = qa >= 0 ? a.start[sa][cell + 1] - fa : 0;
+        const int fb = qb >= 0 ? a.start[sb][cell] : 0, nb = qb >= 0 ? a.start[sb][cell + 1] - fb : 0;
+        const int n = na > nb ? na : nb;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            UT ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+            uint32_t ai = 0, bi = 0;
+            if (k < na) {
+                ax = ((const UT*)a.ux[sa])[fa + k]; ay = ((const UT*)a.uy[sa])[fa + k];
+                az = ((const UT*)a.uz[sa])[fa + k]; ai = a.id[sa][fa + k];
+            }
+            if (k < nb) {
+                bx = ((const UT*)a.ux[sb])[fb + k]; by = ((const UT*)a.uy[sb])[fb + k];
+                bz = ((const UT*)a.uz[sb])[fb + k]; bi = a.id[sb][fb + k];
+            }
+            if (k < na) {
+                UT* cx = cache + (size_t)(3 * qa) * maxp;
+                cx[k] = ax; cx[maxp + k] = ay; cx[2 * maxp + k] = az; idc[qa * maxp + k] = ai;
+            }
+            if (k < nb) {
+                UT* cx = cache + (size_t)(3 * qb) * maxp;
+                cx[k] = bx; cx[maxp + k] = by; cx[2 * maxp + k] = bz; idc[qb * maxp + k] = bi;
+            }
+        }
+    };
+    // bring species sa (and sb >= 0) into the two slots; returns their slots
+    auto hold = [&](int sa, int sb, int& qa, int& qb) {
+        qa = hold0 == sa ? 0 : (hold1 == sa ? 1 : -1);
+        qb = sb < 0 ? -1 : (hold0 == sb ? 0 : (hold1 == sb ? 1 : -1));
+        int fa = -1, fb = -1;
+        if (qa < 0) fa = qa = qb == 0 ? 1 : 0;
+        if (sb >= 0 && qb < 0) fb = qb = 1 - qa;
+        if (fa < 0 && fb < 0) return;
+        if (fa >= 0 && (fa == 0 ? hold0 : hold1) >= 0) put_slot(fa, fa == 0 ? hold0 : hold1);
+        if (fb >= 0 && (fb == 0 ? hold0 : hold1) >= 0) put_slot(fb, fb == 0 ? hold0 : hold1);
+        get_slots(fa, sa, fb, sb);
+        if (fa == 0 || fb == 0) hold0 = fa == 0 ? sa : sb;
+        if (fa == 1 || fb == 1) hold1 = fa == 1 ? sa : sb;
+        __syncthreads();
+    };
+    auto slot_ptrs = [&](StageK& d, int side, int q, int sp) {
+        if (CU) {
+            if (side == 0) d.cP = 3 * q * maxp; else d.cQ = 3 * q * maxp;
+        } else {
+            d.u[3 * side] = a.ux[sp]; d.u[3 * side + 1] = a.uy[sp]; d.u[3 * side + 2] = a.uz[sp];
+        }
+    };
     unsigned long long ncoll = 0;
     int st = 0;
     bool skip_last_intra = false;
@@ -406,6 +478,8 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
                     if (a.charge[s2] != 0.0) { nxt = s2; break; }
                 const bool pair2 = (nxt >= 0) && (nxt == last);
                 if (pair2) skip_last_intra = true;
+                int q0 = 0, q1 = 1;
+                if (CU && a.allow_intra) hold(i, pair2 ? last : -1, q0, q1);
                 for (int v = 0; v < (pair2 ? 2 : 1); ++v) {
                     const int spc = v == 0 ? i : last;
                     const int f0 = a.start[spc][cell];
@@ -413,11 +487,16 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
                     if (cnt <= 1 || !a.allow_intra) continue;
                     uint64_t* key = nd == 0 ? keyA : keyB;
                     int* val = nd == 0 ? valA : valB;
-                    keyed_perm(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, cnt, key, val);
+                    if (CU)
+                        keyed_perm(a.seed, a.step, stage_now + v, spc, idc + (v == 0 ? q0 : q1) * maxp, 0, cnt, key,
+                                   val, 0);
+                    else
+                        keyed_perm(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, cnt, key, val, f0);
                     if (threadIdx.x == 0) {
                         StageK& d = sk[nd];
-                        d.u[0] = d.u[3] = a.ux[spc]; d.u[1] = d.u[4] = a.uy[spc]; d.u[2] = d.u[5] = a.uz[spc];
-                        d.permP = d.permQ = val;
+                        slot_ptrs(d, 0, v == 0 ? q0 : q1, spc);
+                        slot_ptrs(d, 1, v == 0 ? q0 : q1, spc);
+                        d.permP = d.permQ = nd == 0 ? 0 : maxp;
                         d.mP = d.mQ = a.mass[spc]; d.qP = d.qQ = a.charge[spc]; d.wP = d.wQ = a.weight[spc];
                         d.lnL = a.coulomb_log[spc][spc];
                         d.n = cnt;
@@ -440,17 +519,24 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
                 const int a0 = a.start[i][cell], na = a.start[i][cell + 1] - a0;
                 const int b0 = a.start[j][cell], nb = a.start[j][cell + 1] - b0;
                 if (na > 0 && nb > 0) {
-                    keyed_perm(a.seed, a.step, stage_now, i, a.id[i], a0, na, keyA, valA);
-                    keyed_perm(a.seed, a.step, stage_now, j, a.id[j], b0, nb, keyB, valB);
+                    int qi = 0, qj = 1;
+                    if (CU) hold(i, j, qi, qj);
+                    if (CU) {
+                        keyed_perm(a.seed, a.step, stage_now, i, idc + qi * maxp, 0, na, keyA, valA, 0);
+                        keyed_perm(a.seed, a.step, stage_now, j, idc + qj * maxp, 0, nb, keyB, valB, 0);
+                    } else {
+                        keyed_perm(a.seed, a.step, stage_now, i, a.id[i], a0, na, keyA, valA, a0);
+                        keyed_perm(a.seed, a.step, stage_now, j, a.id[j], b0, nb, keyB, valB, b0);
+                    }
                     if (threadIdx.x == 0) {
                         StageK& d = sk[0];
                         d.stage = stage_now;
                         const double wi = a.weight[i], wj = a.weight[j];
                         if (na == nb) {
                             d.kind = kInterEqual; d.n = na; d.units = na;
-                            d.u[0] = a.ux[i]; d.u[1] = a.uy[i]; d.u[2] = a.uz[i];
-                            d.u[3] = a.ux[j]; d.u[4] = a.uy[j]; d.u[5] = a.uz[j];
-                            d.permP = valA; d.permQ = valB;
+                            slot_ptrs(d, 0, qi, i);
+                            slot_ptrs(d, 1, qj, j);
+                            d.permP = 0; d.permQ = maxp;
                             d.mP = a.mass[i]; d.qP = a.charge[i]; d.wP = wi;
                             d.mQ = a.mass[j]; d.qQ = a.charge[j]; d.wQ = wj;
                             d.lnL = a.coulomb_log[i][j];
@@ -461,10 +547,10 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
                             const int nMin = minIsA ? na : nb, nMax = minIsA ? nb : na;
                             const int smin = minIsA ? i : j, smax = minIsA ? j : i;
                             d.kind = kCyclic; d.n = nMax;
-                            d.u[0] = a.ux[smax]; d.u[1] = a.uy[smax]; d.u[2] = a.uz[smax];
-                            d.u[3] = a.ux[smin]; d.u[4] = a.uy[smin]; d.u[5] = a.uz[smin];
-                            d.permP = minIsA ? valB : valA;
-                            d.permQ = minIsA ? valA : valB;
+                            slot_ptrs(d, 0, minIsA ? qj : qi, smax);
+                            slot_ptrs(d, 1, minIsA ? qi : qj, smin);
+                            d.permP = minIsA ? maxp : 0;
+                            d.permQ = minIsA ? 0 : maxp;
                             d.mP = a.mass[smax]; d.qP = a.charge[smax]; d.wP = a.weight[smax];
                             d.mQ = a.mass[smin]; d.qQ = a.charge[smin]; d.wQ = a.weight[smin];
                             d.lnL = a.coulomb_log[smax][smin];
@@ -508,16 +594,26 @@ __device__ void coulomb_cell(const CoulombArgs& a, int cell, int maxp, uint64_t*
                 const StageK& d = sk[u < units0 ? 0 : 1];
                 const int uu = u < units0 ? u : u - units0;
                 int iP, iQ, k;
-                for (int t = 0; decode(d, uu, t, iP, iQ, k); ++t) {
-                    collide<UT, MODEL, WC>(a, d, cell, iP, iQ, k, debye);
+                for (int t = 0; decode(d, valA, uu, t, iP, iQ, k); ++t) {
+                    collide<UT, MODEL, WC, CU>(a, d, cell, iP, iQ, k, debye, cache, maxp);
                     ++ncoll;
                 }
             }
             __syncthreads();
         }
     }
+    if (CU) {
+        if (hold0 >= 0) put_slot(0, hold0);
+        if (hold1 >= 0) put_slot(1, hold1);
+    }
+    return ncoll;
+}
+
+// the wave's collision count into the step's total, spread over kNcollSlots counters a cache
+// line apart (one address taking an atomic from every block serialises the blocks)
+__device__ __forceinline__ void add_ncoll(const CoulombArgs& a, unsigned long long ncoll) {
     for (int off = 32; off > 0; off >>= 1) ncoll += __shfl_xor(ncoll, off);
-    if (threadIdx.x == 0 && ncoll) atomicAdd(a.ncoll, ncoll);
+    if (threadIdx.x == 0 && ncoll) atomicAdd(a.ncoll + kNcollStride * (blockIdx.x % kNcollSlots), ncoll);
 }
 
 // one wave per cell with small index lists in LDS; cells with more parcels of a species
@@ -531,7 +627,7 @@ __device__ __forceinline__ void coulomb_pass(const CoulombArgs& a) {
         if (threadIdx.x == 0) a.big_list[atomicAdd(a.big_n, 1)] = cell;
         return;
     }
-    coulomb_cell<UT, MODEL, WC>(a, cell, a.maxp, lds, sk);
+    add_ncoll(a, coulomb_cell<UT, MODEL, WC, true>(a, cell, a.maxp, lds, sk));
 }
 
 // Nanbu scattering fits three waves per SIMD without spilling; Perez keeps the default
@@ -552,29 +648,34 @@ __global__ void __launch_bounds__(64) k_coulomb_big(CoulombArgs a) {
     extern __shared__ uint64_t lds[];
     __shared__ StageK sk[2];
     const int nbig = *a.big_n;
+    unsigned long long ncoll = 0;
     for (int i = blockIdx.x; i < nbig; i += gridDim.x) {
         const int cell = a.big_list[i];
         if (cell_max_count(a, cell) > a.maxp_big) {
             if (threadIdx.x == 0) atomicOr(a.flags, kFlagCellOverflow);
             continue;
         }
-        coulomb_cell<UT, MODEL, WC>(a, cell, a.maxp_big, lds, sk);
+        ncoll += coulomb_cell<UT, MODEL, WC, false>(a, cell, a.maxp_big, lds, sk);
         __syncthreads();
     }
+    add_ncoll(a, ncoll);
 }
 
-size_t coulomb_lds_bytes(int maxp) {
-    return (size_t)(2 * (maxp + 8)) * sizeof(uint64_t) + (size_t)(2 * maxp) * sizeof(int);
+// key lists, index lists and (cached passes, cache_elem > 0) two slots of velocities and ids
+size_t coulomb_lds_bytes(int maxp, size_t cache_elem) {
+    return (size_t)(2 * (maxp + 8)) * sizeof(uint64_t) + (size_t)(2 * maxp) * sizeof(int) +
+           (cache_elem ? (size_t)(6 * maxp) * cache_elem + (size_t)(2 * maxp) * sizeof(uint32_t) : 0);
 }
 
 template <class UT, int MODEL, bool WC>
 static void launch_coulomb_t(const CoulombArgs& a, hipStream_t s) {
+    const size_t lds = coulomb_lds_bytes(a.maxp, sizeof(UT));
     if (MODEL == 1)
-        hipLaunchKernelGGL((k_coulomb_perez<UT, WC>), dim3(a.n_cells), dim3(64), coulomb_lds_bytes(a.maxp), s, a);
+        hipLaunchKernelGGL((k_coulomb_perez<UT, WC>), dim3(a.n_cells), dim3(64), lds, s, a);
     else
-        hipLaunchKernelGGL((k_coulomb_nanbu<UT, WC>), dim3(a.n_cells), dim3(64), coulomb_lds_bytes(a.maxp), s, a);
-    const int nb = a.n_cells < 512 ? a.n_cells : 512;
-    hipLaunchKernelGGL((k_coulomb_big<UT, MODEL, WC>), dim3(nb), dim3(64), coulomb_lds_bytes(a.maxp_big), s, a);
+        hipLaunchKernelGGL((k_coulomb_nanbu<UT, WC>), dim3(a.n_cells), dim3(64), lds, s, a);
+    const int nb = a.n_cells < 2048 ? a.n_cells : 2048;
+    hipLaunchKernelGGL((k_coulomb_big<UT, MODEL, WC>), dim3(nb), dim3(64), coulomb_lds_bytes(a.maxp_big, 0), s, a);
 }
 
 template <class UT, int MODEL>

@@ -177,9 +177,12 @@ struct CoulombArgs {
     int pairing;                      // 0 TakizukaAbe, 1 NanbuYonemura
     int weight_correction;            // Nanbu weight correction active (unequal weights)
     uint32_t* flags;
-    unsigned long long* ncoll;
+    unsigned long long* ncoll;        // kNcollSlots counters kNcollStride apart; the total is their sum
 };
-size_t coulomb_lds_bytes(int maxp);
+constexpr int kNcollSlots = 64, kNcollStride = 16;
+// step counters on the device: 16 words, then the Coulomb collision counters
+constexpr int kStatsNcoll = 16, kStatsWords = kStatsNcoll + kNcollSlots * kNcollStride;
+size_t coulomb_lds_bytes(int maxp, size_t cache_elem);
 hipError_t launch_coulomb(const CoulombArgs& a, bool fp32_velocity, hipStream_t s);
 
 // ---------------------------------------------------------------- MCC (k_mcc.hip)
