@@ -208,16 +208,14 @@ __device__ __forceinline__ void collide(const CoulombArgs& a, const StageK& d, i
     uint4 o = rng4(a.seed, (uint32_t)cell, (uint32_t)a.step, ((uint32_t)d.stage << 24) | (uint32_t)k,
                    kTagCoulomb << 24);
     const double u1 = u01(o.x, o.y), u2 = u01(o.z, o.w);
-    UT *pxP, *pyP, *pzP, *pxQ, *pyQ, *pzQ;
-    if (CU) {
-        pxP = cache + d.cP; pyP = pxP + cs; pzP = pyP + cs;
-        pxQ = cache + d.cQ; pyQ = pxQ + cs; pzQ = pyQ + cs;
-    } else {
-        pxP = (UT*)d.u[0]; pyP = (UT*)d.u[1]; pzP = (UT*)d.u[2];
-        pxQ = (UT*)d.u[3]; pyQ = (UT*)d.u[4]; pzQ = (UT*)d.u[5];
-    }
-    const vec3<UT> preP = mk3<UT>(pxP[iP], pyP[iP], pzP[iP]);
-    const vec3<UT> preQ = mk3<UT>(pxQ[iQ], pyQ[iQ], pzQ[iQ]);
+    // the table sits in LDS, so the arrays' address space is stated for global loads
+    typedef __attribute__((address_space(1))) UT GUT;
+    UT* cP = cache + d.cP;
+    UT* cQ = cache + d.cQ;
+    GUT* gP[3] = {(GUT*)d.u[0], (GUT*)d.u[1], (GUT*)d.u[2]};
+    GUT* gQ[3] = {(GUT*)d.u[3], (GUT*)d.u[4], (GUT*)d.u[5]};
+    const vec3<UT> preP = CU ? mk3<UT>(cP[iP], cP[cs + iP], cP[2 * cs + iP]) : mk3<UT>(gP[0][iP], gP[1][iP], gP[2][iP]);
+    const vec3<UT> preQ = CU ? mk3<UT>(cQ[iQ], cQ[cs + iQ], cQ[2 * cs + iQ]) : mk3<UT>(gQ[0][iQ], gQ[1][iQ], gQ[2][iQ]);
     vec3<UT> nP = preP, nQ = preQ;
     if (MODEL == 1) {
         vec3<double> dP = mk3<double>(preP.x, preP.y, preP.z), dQ = mk3<double>(preQ.x, preQ.y, preQ.z);
@@ -235,8 +233,13 @@ __device__ __forceinline__ void collide(const CoulombArgs& a, const StageK& d, i
         if (r > d.wP / d.wQ) nQ = preQ;
         if (r > d.wQ / d.wP) nP = preP;
     }
-    pxP[iP] = nP.x; pyP[iP] = nP.y; pzP[iP] = nP.z;
-    pxQ[iQ] = nQ.x; pyQ[iQ] = nQ.y; pzQ[iQ] = nQ.z;
+    if (CU) {
+        cP[iP] = nP.x; cP[cs + iP] = nP.y; cP[2 * cs + iP] = nP.z;
+        cQ[iQ] = nQ.x; cQ[cs + iQ] = nQ.y; cQ[2 * cs + iQ] = nQ.z;
+    } else {
+        gP[0][iP] = nP.x; gP[1][iP] = nP.y; gP[2][iP] = nP.z;
+        gQ[0][iQ] = nQ.x; gQ[1][iQ] = nQ.y; gQ[2][iQ] = nQ.z;
+    }
 }
 
 // bitonic sort of npow2 64-bit keys with int payload in LDS by one wave
@@ -332,6 +335,97 @@ __device__ __forceinline__ int cell_max_count(const CoulombArgs& a, int cell) {
         m = n > m ? n : m;
     }
     return m;
+}
+
+// next charged species after i, or -1
+__device__ __forceinline__ int next_charged(const CoulombArgs& a, int i) {
+    for (int s2 = i + 1; s2 < a.n_species; ++s2)
+        if (a.charge[s2] != 0.0) return s2;
+    return -1;
+}
+
+// table of the intra-species stage of species spc (cnt >= 2 parcels in a cell of volume Vc);
+// its shuffled parcel list sits at offset perm of the stage lists; velocities in the arrays
+__device__ __forceinline__ void stage_intra(StageK& d, const CoulombArgs& a, int spc, int cnt, int stage, double Vc,
+                                            bool ny, int perm) {
+    d.u[0] = d.u[3] = a.ux[spc]; d.u[1] = d.u[4] = a.uy[spc]; d.u[2] = d.u[5] = a.uz[spc];
+    d.permP = d.permQ = perm;
+    d.mP = d.mQ = a.mass[spc]; d.qP = d.qQ = a.charge[spc]; d.wP = d.wQ = a.weight[spc];
+    d.lnL = a.coulomb_log[spc][spc];
+    d.n = cnt;
+    d.stage = stage;
+    const double n = cnt * a.weight[spc] / Vc;
+    d.nP = d.nQ = n;
+    if (cnt % 2 == 0) {
+        d.kind = kIntraEven; d.units = cnt / 2; d.nPQ = n;
+    } else if (!ny) {
+        d.kind = kIntraOddTA; d.units = 1 + (cnt - 3) / 2; d.nPQ = (cnt + 3.0) * a.weight[spc] / Vc;
+    } else {
+        d.kind = kIntraOddNY; d.units = (cnt - 1) / 2;
+        d.nPQ = 2.0 * a.weight[spc] * ((cnt - 1) / 2 + 1) / Vc;
+    }
+    finish_stage(d, a.dt);
+}
+
+// table of the stage between species i (na parcels, list at offset permA) and j (nb, permB);
+// returns true when j is the P side (the Takizuka-Abe / Nanbu-Yonemura cyclic pairing puts
+// the species with more parcels there)
+__device__ __forceinline__ bool stage_inter(StageK& d, const CoulombArgs& a, int i, int j, int na, int nb, int stage,
+                                            double Vc, bool ny, int permA, int permB) {
+    d.stage = stage;
+    const double wi = a.weight[i], wj = a.weight[j];
+    bool jP = false;
+    if (na == nb) {
+        d.kind = kInterEqual; d.n = na; d.units = na;
+        d.permP = permA; d.permQ = permB;
+        d.mP = a.mass[i]; d.qP = a.charge[i]; d.wP = wi;
+        d.mQ = a.mass[j]; d.qQ = a.charge[j]; d.wQ = wj;
+        d.lnL = a.coulomb_log[i][j];
+        d.nP = na * wi / Vc; d.nQ = nb * wj / Vc;
+        d.nPQ = ny ? na * dmin(wi, wj) / Vc : d.nP;
+    } else {
+        const bool minIsA = na < nb;
+        jP = minIsA;
+        const int nMin = minIsA ? na : nb, nMax = minIsA ? nb : na;
+        const int smin = minIsA ? i : j, smax = minIsA ? j : i;
+        d.kind = kCyclic; d.n = nMax;
+        d.permP = minIsA ? permB : permA;
+        d.permQ = minIsA ? permA : permB;
+        d.mP = a.mass[smax]; d.qP = a.charge[smax]; d.wP = a.weight[smax];
+        d.mQ = a.mass[smin]; d.qQ = a.charge[smin]; d.wQ = a.weight[smin];
+        d.lnL = a.coulomb_log[smax][smin];
+        d.nP = nMax * a.weight[smax] / Vc;
+        d.nQ = nMin * a.weight[smin] / Vc;
+        if (ny) {
+            // NanbuYonemuraPairing.C:253-300
+            d.nPQ = nMax * dmin(wi, wj) / Vc;
+            d.g1_units = nMin; d.g1_lmax = nMax; d.g1_stride = nMin;
+            d.g1_xo = 0; d.g1_yo = 0; d.g1_ko = 0;
+            d.units = nMin;
+            d.g2_lmax = 0; d.g2_stride = 1; d.g2_xo = 0; d.g2_yo = 0; d.g2_ko = 0;
+        } else {
+            // TakizukaAbePairing.C:197-308: two groups
+            d.nPQ = d.nP;
+            const int delta = nMax / nMin;
+            const double delta_r = (double)(nMax % nMin) / nMin;
+            const int nMinG1 = (int)(delta_r * nMin);
+            const int nMaxG1 = (int)((delta + 1) * delta_r * nMin);
+            const int nMinG2 = (int)((1.0 - delta_r) * nMin);
+            const int nMaxG2 = (int)(delta * (1.0 - delta_r) * nMin);
+            const int cyc = nMinG2 > 0 ? nMinG2 : 1;
+            d.g1_units = nMinG1 > 0 ? nMinG1 : 0;
+            d.g1_lmax = nMaxG1; d.g1_stride = nMinG1 > 0 ? nMinG1 : 1;
+            d.g1_xo = 0; d.g1_yo = 0; d.g1_ko = 0;
+            d.g2_lmax = nMaxG2; d.g2_stride = cyc;
+            d.g2_xo = nMaxG1; d.g2_yo = nMinG1; d.g2_ko = nMaxG1;
+            d.units = d.g1_units + (nMaxG2 > 0 ? cyc : 0);
+        }
+    }
+    const int sP = jP ? j : i, sQ = jP ? i : j;
+    d.u[0] = a.ux[sP]; d.u[1] = a.uy[sP]; d.u[2] = a.uz[sP];
+    d.u[3] = a.ux[sQ]; d.u[4] = a.uy[sQ]; d.u[5] = a.uz[sQ];
+    finish_stage(d, a.dt);
+    return jP;
 }
 
 // every pairing stage of one cell by one wave; maxp = LDS slots per index list.  With CU the
@@ -458,8 +552,6 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
     auto slot_ptrs = [&](StageK& d, int side, int q, int sp) {
         if (CU) {
             if (side == 0) d.cP = 3 * q * maxp; else d.cQ = 3 * q * maxp;
-        } else {
-            d.u[3 * side] = a.ux[sp]; d.u[3 * side + 1] = a.uy[sp]; d.u[3 * side + 2] = a.uz[sp];
         }
     };
     unsigned long long ncoll = 0;
@@ -473,10 +565,7 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
             if (i == j && i == last && skip_last_intra) continue;
             int nd = 0;        // stage tables filled
             if (i == j) {
-                int nxt = -1;
-                for (int s2 = i + 1; s2 < ns; ++s2)
-                    if (a.charge[s2] != 0.0) { nxt = s2; break; }
-                const bool pair2 = (nxt >= 0) && (nxt == last);
+                const bool pair2 = next_charged(a, i) == last;
                 if (pair2) skip_last_intra = true;
                 int q0 = 0, q1 = 1;
                 if (CU && a.allow_intra) hold(i, pair2 ? last : -1, q0, q1);
@@ -494,24 +583,9 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
                         keyed_perm(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, cnt, key, val, f0);
                     if (threadIdx.x == 0) {
                         StageK& d = sk[nd];
+                        stage_intra(d, a, spc, cnt, stage_now + v, Vc, ny, nd == 0 ? 0 : maxp);
                         slot_ptrs(d, 0, v == 0 ? q0 : q1, spc);
                         slot_ptrs(d, 1, v == 0 ? q0 : q1, spc);
-                        d.permP = d.permQ = nd == 0 ? 0 : maxp;
-                        d.mP = d.mQ = a.mass[spc]; d.qP = d.qQ = a.charge[spc]; d.wP = d.wQ = a.weight[spc];
-                        d.lnL = a.coulomb_log[spc][spc];
-                        d.n = cnt;
-                        d.stage = stage_now + v;
-                        const double n = cnt * a.weight[spc] / Vc;
-                        d.nP = d.nQ = n;
-                        if (cnt % 2 == 0) {
-                            d.kind = kIntraEven; d.units = cnt / 2; d.nPQ = n;
-                        } else if (!ny) {
-                            d.kind = kIntraOddTA; d.units = 1 + (cnt - 3) / 2; d.nPQ = (cnt + 3.0) * a.weight[spc] / Vc;
-                        } else {
-                            d.kind = kIntraOddNY; d.units = (cnt - 1) / 2;
-                            d.nPQ = 2.0 * a.weight[spc] * ((cnt - 1) / 2 + 1) / Vc;
-                        }
-                        finish_stage(d, a.dt);
                     }
                     ++nd;
                 }
@@ -530,58 +604,9 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
                     }
                     if (threadIdx.x == 0) {
                         StageK& d = sk[0];
-                        d.stage = stage_now;
-                        const double wi = a.weight[i], wj = a.weight[j];
-                        if (na == nb) {
-                            d.kind = kInterEqual; d.n = na; d.units = na;
-                            slot_ptrs(d, 0, qi, i);
-                            slot_ptrs(d, 1, qj, j);
-                            d.permP = 0; d.permQ = maxp;
-                            d.mP = a.mass[i]; d.qP = a.charge[i]; d.wP = wi;
-                            d.mQ = a.mass[j]; d.qQ = a.charge[j]; d.wQ = wj;
-                            d.lnL = a.coulomb_log[i][j];
-                            d.nP = na * wi / Vc; d.nQ = nb * wj / Vc;
-                            d.nPQ = ny ? na * dmin(wi, wj) / Vc : d.nP;
-                        } else {
-                            const bool minIsA = na < nb;
-                            const int nMin = minIsA ? na : nb, nMax = minIsA ? nb : na;
-                            const int smin = minIsA ? i : j, smax = minIsA ? j : i;
-                            d.kind = kCyclic; d.n = nMax;
-                            slot_ptrs(d, 0, minIsA ? qj : qi, smax);
-                            slot_ptrs(d, 1, minIsA ? qi : qj, smin);
-                            d.permP = minIsA ? maxp : 0;
-                            d.permQ = minIsA ? 0 : maxp;
-                            d.mP = a.mass[smax]; d.qP = a.charge[smax]; d.wP = a.weight[smax];
-                            d.mQ = a.mass[smin]; d.qQ = a.charge[smin]; d.wQ = a.weight[smin];
-                            d.lnL = a.coulomb_log[smax][smin];
-                            d.nP = nMax * a.weight[smax] / Vc;
-                            d.nQ = nMin * a.weight[smin] / Vc;
-                            if (ny) {
-                                // NanbuYonemuraPairing.C:253-300
-                                d.nPQ = nMax * dmin(wi, wj) / Vc;
-                                d.g1_units = nMin; d.g1_lmax = nMax; d.g1_stride = nMin;
-                                d.g1_xo = 0; d.g1_yo = 0; d.g1_ko = 0;
-                                d.units = nMin;
-                                d.g2_lmax = 0; d.g2_stride = 1; d.g2_xo = 0; d.g2_yo = 0; d.g2_ko = 0;
-                            } else {
-                                // TakizukaAbePairing.C:197-308: two groups
-                                d.nPQ = d.nP;
-                                const int delta = nMax / nMin;
-                                const double delta_r = (double)(nMax % nMin) / nMin;
-                                const int nMinG1 = (int)(delta_r * nMin);
-                                const int nMaxG1 = (int)((delta + 1) * delta_r * nMin);
-                                const int nMinG2 = (int)((1.0 - delta_r) * nMin);
-                                const int nMaxG2 = (int)(delta * (1.0 - delta_r) * nMin);
-                                const int cyc = nMinG2 > 0 ? nMinG2 : 1;
-                                d.g1_units = nMinG1 > 0 ? nMinG1 : 0;
-                                d.g1_lmax = nMaxG1; d.g1_stride = nMinG1 > 0 ? nMinG1 : 1;
-                                d.g1_xo = 0; d.g1_yo = 0; d.g1_ko = 0;
-                                d.g2_lmax = nMaxG2; d.g2_stride = cyc;
-                                d.g2_xo = nMaxG1; d.g2_yo = nMinG1; d.g2_ko = nMaxG1;
-                                d.units = d.g1_units + (nMaxG2 > 0 ? cyc : 0);
-                            }
-                        }
-                        finish_stage(d, a.dt);
+                        const bool jP = stage_inter(d, a, i, j, na, nb, stage_now, Vc, ny, 0, maxp);
+                        slot_ptrs(d, 0, jP ? qj : qi, jP ? j : i);
+                        slot_ptrs(d, 1, jP ? qi : qj, jP ? i : j);
                     }
                     nd = 1;
                 }
@@ -661,16 +686,146 @@ __global__ void __launch_bounds__(64) k_coulomb_big(CoulombArgs a) {
     add_ncoll(a, ncoll);
 }
 
+// ---------------------------------------------------------------- several cells per wave
+// 64 / L cells per wave, L lanes per cell.  A stage's collision units are spread over the
+// cell's L lanes, so the short sequential runs that end a stage (a Takizuka-Abe odd triple,
+// parcels of the smaller species meeting two partners) share the wave's passes with the
+// other cells' collisions instead of holding all 64 lanes, and the stage tables of the
+// cells are filled together.  Per cell in LDS: one key list (maxp + 8) and two index lists
+// (maxp each, maxp <= 256).  Requires fixed Coulomb logarithms (no Debye length).
+
+// the cell's group of L lanes (lane r) ranks its n parcels of one species by their random
+// key as keyed_perm does; a group with n = 0 only takes part in the barriers
+template <int L>
+__device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
+                           uint64_t* key, int* val, int r) {
+    const int n8 = (n + 7) & ~7;
+    for (int k = r; k < n8; k += L)
+        key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k : ~0ull;
+    __syncthreads();
+    for (int k = r; k < n; k += L) {
+        const uint64_t kk = key[k];
+        int c = 0;
+        for (int j = 0; j < n8; j += 8) {
+            const uint64_t* q = key + j;
+            const uint64_t x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3], x4 = q[4], x5 = q[5], x6 = q[6], x7 = q[7];
+            c += (int)(x0 < kk) + (int)(x1 < kk) + (int)(x2 < kk) + (int)(x3 < kk) + (int)(x4 < kk) + (int)(x5 < kk) +
+                 (int)(x6 < kk) + (int)(x7 < kk);
+        }
+        val[c] = first + k;
+    }
+    __syncthreads();
+}
+
+template <class UT, int MODEL, bool WC, int L>
+__device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
+    constexpr int W = 64 / L;
+    extern __shared__ uint64_t lds[];
+    __shared__ StageK skw[W][2];
+    const int w = threadIdx.x / L, r = threadIdx.x % L;
+    const int maxp = a.maxp;
+    uint64_t* key = lds + (size_t)w * (2 * maxp + 8);
+    int* valA = (int*)(key + maxp + 8);
+    int* valB = valA + maxp;
+    StageK* sk = skw[w];
+    const int cell = blockIdx.x * W + w;
+    bool on = cell < a.n_cells;
+    if (on && cell_max_count(a, cell) > maxp) {
+        if (r == 0) a.big_list[atomicAdd(a.big_n, 1)] = cell;
+        on = false;
+    }
+    const double Vc = on ? a.V[cell] : 1.0;
+    const int ns = a.n_species;
+    int last = -1;
+    for (int s = 0; s < ns; ++s)
+        if (a.charge[s] != 0.0) last = s;
+    const bool ny = a.pairing == 1;
+    unsigned long long ncoll = 0;
+    int st = 0;
+    bool skip_last_intra = false;
+    // the stage order of coulomb_cell
+    for (int i = 0; i < ns; ++i) {
+        if (a.charge[i] == 0.0) continue;
+        for (int j = last; j >= i; --j) {
+            if (a.charge[j] == 0.0) continue;
+            const int stage_now = st++;
+            if (i == j && i == last && skip_last_intra) continue;
+            int nd = 0;
+            if (i == j) {
+                const bool pair2 = next_charged(a, i) == last;
+                if (pair2) skip_last_intra = true;
+                for (int v = 0; v < (pair2 ? 2 : 1); ++v) {
+                    const int spc = v == 0 ? i : last;
+                    const int f0 = on ? a.start[spc][cell] : 0;
+                    const int cnt = on ? a.start[spc][cell + 1] - f0 : 0;
+                    const bool go = cnt > 1 && a.allow_intra;
+                    group_perm<L>(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, go ? cnt : 0, key,
+                                  nd == 0 ? valA : valB, r);
+                    if (go && r == 0) stage_intra(sk[nd], a, spc, cnt, stage_now + v, Vc, ny, nd == 0 ? 0 : maxp);
+                    if (go) ++nd;
+                }
+            } else {
+                const int a0 = on ? a.start[i][cell] : 0, na = on ? a.start[i][cell + 1] - a0 : 0;
+                const int b0 = on ? a.start[j][cell] : 0, nb = on ? a.start[j][cell + 1] - b0 : 0;
+                const bool go = na > 0 && nb > 0;
+                group_perm<L>(a.seed, a.step, stage_now, i, a.id[i], a0, go ? na : 0, key, valA, r);
+                group_perm<L>(a.seed, a.step, stage_now, j, a.id[j], b0, go ? nb : 0, key, valB, r);
+                if (go && r == 0) stage_inter(sk[0], a, i, j, na, nb, stage_now, Vc, ny, 0, maxp);
+                nd = go ? 1 : 0;
+            }
+            __syncthreads();
+            const int units0 = nd > 0 ? sk[0].units : 0;
+            const int total = units0 + (nd > 1 ? sk[1].units : 0);
+            for (int u = r; u < total; u += L) {
+                const StageK& d = sk[u < units0 ? 0 : 1];
+                const int uu = u < units0 ? u : u - units0;
+                int iP, iQ, k;
+                for (int t = 0; decode(d, valA, uu, t, iP, iQ, k); ++t) {
+                    collide<UT, MODEL, WC, false>(a, d, cell, iP, iQ, k, 0.0, (UT*)nullptr, 0);
+                    ++ncoll;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    add_ncoll(a, ncoll);
+}
+
+template <class UT, bool WC, int L>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_coulomb_group_nanbu(CoulombArgs a) {
+    coulomb_group<UT, 0, WC, L>(a);
+}
+
+template <class UT, bool WC, int L>
+__global__ void __launch_bounds__(64) k_coulomb_group_perez(CoulombArgs a) {
+    coulomb_group<UT, 1, WC, L>(a);
+}
+
 // key lists, index lists and (cached passes, cache_elem > 0) two slots of velocities and ids
 size_t coulomb_lds_bytes(int maxp, size_t cache_elem) {
     return (size_t)(2 * (maxp + 8)) * sizeof(uint64_t) + (size_t)(2 * maxp) * sizeof(int) +
            (cache_elem ? (size_t)(6 * maxp) * cache_elem + (size_t)(2 * maxp) * sizeof(uint32_t) : 0);
 }
 
+template <class UT, int MODEL, bool WC, int L>
+static void launch_coulomb_group(const CoulombArgs& a, hipStream_t s) {
+    constexpr int W = 64 / L;
+    const size_t lds = (size_t)W * (2 * a.maxp + 8) * sizeof(uint64_t);
+    const int grid = (a.n_cells + W - 1) / W;
+    if (MODEL == 1)
+        hipLaunchKernelGGL((k_coulomb_group_perez<UT, WC, L>), dim3(grid), dim3(64), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_coulomb_group_nanbu<UT, WC, L>), dim3(grid), dim3(64), lds, s, a);
+}
+
 template <class UT, int MODEL, bool WC>
 static void launch_coulomb_t(const CoulombArgs& a, hipStream_t s) {
     const size_t lds = coulomb_lds_bytes(a.maxp, sizeof(UT));
-    if (MODEL == 1)
+    if (a.group == 16)
+        launch_coulomb_group<UT, MODEL, WC, 16>(a, s);
+    else if (a.group == 32)
+        launch_coulomb_group<UT, MODEL, WC, 32>(a, s);
+    else if (MODEL == 1)
         hipLaunchKernelGGL((k_coulomb_perez<UT, WC>), dim3(a.n_cells), dim3(64), lds, s, a);
     else
         hipLaunchKernelGGL((k_coulomb_nanbu<UT, WC>), dim3(a.n_cells), dim3(64), lds, s, a);

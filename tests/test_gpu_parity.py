@@ -174,8 +174,12 @@ COULOMB_VARIANTS = [
 
 
 @pytest.mark.parametrize("variant", COULOMB_VARIANTS, ids=lambda v: "-".join(str(x) for x in v))
-@pytest.mark.parametrize("counts", ["equal", "ragged"])
+@pytest.mark.parametrize("counts", ["equal", "ragged", "dense"])
 def test_coulomb_fp64_parity(oracle, counts, variant):
+    """GPU collisions equal the oracle's to 1e-12.  With fixed Coulomb logarithms the cells
+    of "equal" and "ragged" (mean below 80 parcels per species) run four to a wave, those of
+    "dense" two to a wave; the computed logarithm runs one cell per wave; the 300-parcel
+    cell of "ragged" goes to the large-cell pass."""
     model, pairing, wratio, wcorr = variant
     say("coulomb parity", counts, variant)
     rng = np.random.default_rng(104)
@@ -184,6 +188,9 @@ def test_coulomb_fp64_parity(oracle, counts, variant):
     nc = rm.n_cells
     if counts == "equal":
         ne = np.full(nc, 64); ni = np.full(nc, 64)
+    elif counts == "dense":
+        ne = rng.integers(90, 230, nc); ni = rng.integers(90, 230, nc)
+        ne[0], ni[0] = 251, 249
     else:
         ne = rng.integers(0, 90, nc); ni = rng.integers(0, 90, nc)
         ne[0], ni[0] = 7, 3
