@@ -160,17 +160,14 @@ struct PartSoA {
     uint32_t* id;
 };
 
-// Wave-aggregated atomic increment: lanes with equal keys (neighbouring lanes usually
-// share a cell because parcels are kept sorted by cell) issue one atomic per distinct key.
-// Returns this lane's slot (old value + rank among the equal-key lanes).  key < 0 lanes
-// take no part.  Must be called by all lanes of the wave.
-__device__ __forceinline__ int wave_aggregated_inc(int* counter, int key) {
+// Groups the wave's lanes by key (key < 0 lanes take no part) without touching memory:
+// each lane learns its group's leader lane and its rank in the group, each leader the group
+// size.  Must be called by all lanes of the wave.
+__device__ __forceinline__ void wave_group(int key, int& leader, int& rank, int& cnt) {
     const int lane = __lane_id();
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint64_t active = __ballot(key >= 0);
-    // group the lanes by key without touching memory: each lane learns its group's leader
-    // and its rank in the group, each leader the group size
-    int leader = lane, rank = 0, cnt = 0;
+    leader = lane; rank = 0; cnt = 0;
     while (active) {
         const int l = __ffsll((unsigned long long)active) - 1;
         const int lk = __shfl(key, l);
@@ -182,6 +179,16 @@ __device__ __forceinline__ int wave_aggregated_inc(int* counter, int key) {
         if (lane == l) cnt = __popcll(mask);
         active &= ~mask;
     }
+}
+
+// Wave-aggregated atomic increment: lanes with equal keys (neighbouring lanes usually
+// share a cell because parcels are kept sorted by cell) issue one atomic per distinct key.
+// Returns this lane's slot (old value + rank among the equal-key lanes).  key < 0 lanes
+// take no part.  Must be called by all lanes of the wave.
+__device__ __forceinline__ int wave_aggregated_inc(int* counter, int key) {
+    const int lane = __lane_id();
+    int leader, rank, cnt;
+    wave_group(key, leader, rank, cnt);
     // then every leader adds its group size at once: one atomic round trip per wave
     int base = 0;
     if (key >= 0 && lane == leader) base = atomicAdd(counter + key, cnt);

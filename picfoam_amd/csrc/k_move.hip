@@ -126,12 +126,16 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
         const double T3 = g.r[6] * dx + g.r[7] * dy + g.r[8] * dz;
         const double T0 = -(T1 + T2 + T3);
         const double det = g.det;
-        double muH = (det > 0.0 && __builtin_isnormal(det)) ? 1.0 / det : 1e300;
+        const bool dok = det > 0.0 && __builtin_isnormal(det);
+        // first face hit: the smallest mu_i = -Y_i / T_i >= 0 over T_i < 0 below the tet's
+        // end 1/det, found by comparing the fractions crosswise; one division for the winner
+        double bn = 1.0, bd = dok ? det : 1e-300;
         int iH = -1;
-        if (T0 < 0.0) { double mu = -Y0 / T0; if (mu >= 0.0 && mu < muH) { muH = mu; iH = 0; } }
-        if (T1 < 0.0) { double mu = -Y1 / T1; if (mu >= 0.0 && mu < muH) { muH = mu; iH = 1; } }
-        if (T2 < 0.0) { double mu = -Y2 / T2; if (mu >= 0.0 && mu < muH) { muH = mu; iH = 2; } }
-        if (T3 < 0.0) { double mu = -Y3 / T3; if (mu >= 0.0 && mu < muH) { muH = mu; iH = 3; } }
+        if (T0 < 0.0 && Y0 >= 0.0 && Y0 * bd < -T0 * bn) { bn = Y0; bd = -T0; iH = 0; }
+        if (T1 < 0.0 && Y1 >= 0.0 && Y1 * bd < -T1 * bn) { bn = Y1; bd = -T1; iH = 1; }
+        if (T2 < 0.0 && Y2 >= 0.0 && Y2 * bd < -T2 * bn) { bn = Y2; bd = -T2; iH = 2; }
+        if (T3 < 0.0 && Y3 >= 0.0 && Y3 * bd < -T3 * bn) { bn = Y3; bd = -T3; iH = 3; }
+        const double muH = (iH >= 0 || dok) ? bn / bd : 1e300;
         Y0 = Y0 + muH * T0;
         Y1 = Y1 + muH * T1;
         Y2 = Y2 + muH * T2;
