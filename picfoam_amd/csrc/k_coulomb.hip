@@ -691,28 +691,64 @@ __global__ void __launch_bounds__(64) k_coulomb_big(CoulombArgs a) {
 // cell's L lanes, so the short sequential runs that end a stage (a Takizuka-Abe odd triple,
 // parcels of the smaller species meeting two partners) share the wave's passes with the
 // other cells' collisions instead of holding all 64 lanes, and the stage tables of the
-// cells are filled together.  Per cell in LDS: one key list (maxp + 8) and two index lists
-// (maxp each, maxp <= 256).  Requires fixed Coulomb logarithms (no Debye length).
+// cells are filled together.  Per cell in LDS: one key list (maxp + 8), 4 L bins and two
+// index lists (maxp = 8 L each).  Requires fixed Coulomb logarithms (no Debye length).
 
-// the cell's group of L lanes (lane r) ranks its n parcels of one species by their random
-// key as keyed_perm does; a group with n = 0 only takes part in the barriers
+// The cell's group of L lanes (lane r) ranks its n <= 8 L parcels of one species by their
+// random key, the order keyed_perm produces.  The keys are bucketed by their top bits into
+// 4 L bins (an LDS histogram, its prefix sum over the group's lanes and a placement in
+// bucket order); a key's rank is its bucket's start plus the smaller keys of its own bucket,
+// about two comparisons instead of n.  A group with n = 0 only takes part in the barriers.
 template <int L>
 __device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
-                           uint64_t* key, int* val, int r) {
-    const int n8 = (n + 7) & ~7;
-    for (int k = r; k < n8; k += L)
-        key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k : ~0ull;
+                           uint64_t* key, int* bins, int* val, int r) {
+    constexpr int KM = 8, NB = 4 * L, PER = NB / L;
+    constexpr int SH = 64 - (L == 16 ? 6 : 7);   // top log2(NB) bits pick the bin
+    for (int b = r; b < NB; b += L) bins[b] = 0;
     __syncthreads();
-    for (int k = r; k < n; k += L) {
-        const uint64_t kk = key[k];
-        int c = 0;
-        for (int j = 0; j < n8; j += 8) {
-            const uint64_t* q = key + j;
-            const uint64_t x0 = q[0], x1 = q[1], x2 = q[2], x3 = q[3], x4 = q[4], x5 = q[5], x6 = q[6], x7 = q[7];
-            c += (int)(x0 < kk) + (int)(x1 < kk) + (int)(x2 < kk) + (int)(x3 < kk) + (int)(x4 < kk) + (int)(x5 < kk) +
-                 (int)(x6 < kk) + (int)(x7 < kk);
+    uint64_t kk[KM];
+#pragma unroll
+    for (int q = 0; q < KM; ++q) {
+        const int k = r + q * L;
+        kk[q] = 0;
+        if (k < n) {
+            kk[q] = (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k;
+            val[k] = atomicAdd(&bins[kk[q] >> SH], 1);   // place in the bucket, until the ranks
         }
-        val[c] = first + k;
+    }
+    __syncthreads();
+    // exclusive prefix of the bins: PER consecutive bins per lane, scanned over the group
+    int loc = 0;
+#pragma unroll
+    for (int b = 0; b < PER; ++b) loc += bins[r * PER + b];
+    int inc = loc;
+#pragma unroll
+    for (int off = 1; off < L; off <<= 1) {
+        const int y = __shfl_up(inc, off, L);
+        if (r >= off) inc += y;
+    }
+    int run = inc - loc;
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int c = bins[r * PER + b];
+        bins[r * PER + b] = run;
+        run += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KM; ++q)
+        if (r + q * L < n) key[bins[kk[q] >> SH] + val[r + q * L]] = kk[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KM; ++q) {
+        const int k = r + q * L;
+        if (k < n) {
+            const int b = (int)(kk[q] >> SH);
+            const int s0 = bins[b], s1 = b + 1 < NB ? bins[b + 1] : n;
+            int c = s0;
+            for (int j = s0; j < s1; ++j) c += (int)(key[j] < kk[q]);
+            val[c] = first + k;
+        }
     }
     __syncthreads();
 }
@@ -723,9 +759,10 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
     extern __shared__ uint64_t lds[];
     __shared__ StageK skw[W][2];
     const int w = threadIdx.x / L, r = threadIdx.x % L;
-    const int maxp = a.maxp;
-    uint64_t* key = lds + (size_t)w * (2 * maxp + 8);
-    int* valA = (int*)(key + maxp + 8);
+    const int maxp = a.maxp;                 // 8 L
+    uint64_t* key = lds + (size_t)w * (2 * maxp + 8 + 2 * L);
+    int* bins = (int*)(key + maxp + 8);      // 4 L
+    int* valA = bins + 4 * L;
     int* valB = valA + maxp;
     StageK* sk = skw[w];
     const int cell = blockIdx.x * W + w;
@@ -759,7 +796,7 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
                     const int f0 = on ? a.start[spc][cell] : 0;
                     const int cnt = on ? a.start[spc][cell + 1] - f0 : 0;
                     const bool go = cnt > 1 && a.allow_intra;
-                    group_perm<L>(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, go ? cnt : 0, key,
+                    group_perm<L>(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, go ? cnt : 0, key, bins,
                                   nd == 0 ? valA : valB, r);
                     if (go && r == 0) stage_intra(sk[nd], a, spc, cnt, stage_now + v, Vc, ny, nd == 0 ? 0 : maxp);
                     if (go) ++nd;
@@ -768,8 +805,8 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
                 const int a0 = on ? a.start[i][cell] : 0, na = on ? a.start[i][cell + 1] - a0 : 0;
                 const int b0 = on ? a.start[j][cell] : 0, nb = on ? a.start[j][cell + 1] - b0 : 0;
                 const bool go = na > 0 && nb > 0;
-                group_perm<L>(a.seed, a.step, stage_now, i, a.id[i], a0, go ? na : 0, key, valA, r);
-                group_perm<L>(a.seed, a.step, stage_now, j, a.id[j], b0, go ? nb : 0, key, valB, r);
+                group_perm<L>(a.seed, a.step, stage_now, i, a.id[i], a0, go ? na : 0, key, bins, valA, r);
+                group_perm<L>(a.seed, a.step, stage_now, j, a.id[j], b0, go ? nb : 0, key, bins, valB, r);
                 if (go && r == 0) stage_inter(sk[0], a, i, j, na, nb, stage_now, Vc, ny, 0, maxp);
                 nd = go ? 1 : 0;
             }
@@ -810,7 +847,7 @@ size_t coulomb_lds_bytes(int maxp, size_t cache_elem) {
 template <class UT, int MODEL, bool WC, int L>
 static void launch_coulomb_group(const CoulombArgs& a, hipStream_t s) {
     constexpr int W = 64 / L;
-    const size_t lds = (size_t)W * (2 * a.maxp + 8) * sizeof(uint64_t);
+    const size_t lds = (size_t)W * (2 * a.maxp + 8 + 2 * L) * sizeof(uint64_t);
     const int grid = (a.n_cells + W - 1) / W;
     if (MODEL == 1)
         hipLaunchKernelGGL((k_coulomb_group_perez<UT, WC, L>), dim3(grid), dim3(64), lds, s, a);
