@@ -334,17 +334,24 @@ int do_sort(pic_ctx* c) {
         c->counts_valid = true;
         c->place_valid = true;
     }
+    // every species' scan in the same launches; they also store the live counts (the
+    // scatter reads only the launch bound)
+    ScanSet ss{};
+    ss.n = nc + 1;
+    ss.tmp = (int*)c->d_scan_tmp;
+    for (int s = 0; s < (int)c->sp.size(); ++s) {
+        ss.in[s] = c->sp[s].count; ss.out[s] = c->sp[s].start; ss.total[s] = c->d_n + s;
+    }
+    if (int r = check_hip(c, launch_scan_cells(ss, (int)c->sp.size(), c->stream), "scan")) return r;
     for (int s = 0; s < (int)c->sp.size(); ++s) {
         Species& S = c->sp[s];
-        hipError_t e = launch_scan(S.count, S.start, nc + 1, c->d_scan_tmp, c->scan_bytes, c->stream);
-        if (int r = check_hip(c, e, "scan")) return r;
+        hipError_t e = hipSuccess;
         const int nb = 1 - S.cur;
         if (c->mixed())
-            e = launch_scatter(soa<float>(S, S.cur), soa<float>(S, nb), S.n, c->d_n + s, S.place, S.start, c->stream);
+            e = launch_scatter(soa<float>(S, S.cur), soa<float>(S, nb), S.n, S.place, S.start, c->stream);
         else
-            e = launch_scatter(soa<double>(S, S.cur), soa<double>(S, nb), S.n, c->d_n + s, S.place, S.start, c->stream);
+            e = launch_scatter(soa<double>(S, S.cur), soa<double>(S, nb), S.n, S.place, S.start, c->stream);
         if (int r = check_hip(c, e, "scatter")) return r;
-        if (int r = check_hip(c, launch_store_count(S.start + nc, c->d_n + s, c->stream), "count")) return r;
         S.cur = nb;
     }
     c->sorted = true;
@@ -794,7 +801,7 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->d_n = c->alloc<int64_t>(kMaxSpecies);
     c->d_big_list = c->alloc<int>(nc);
     c->d_big_n = c->alloc<int>(1);
-    c->scan_bytes = scan_temp_bytes(nc + 1);
+    c->scan_bytes = std::max(scan_temp_bytes(nc + 1), scan_cells_temp_bytes(nc + 1));
     c->d_scan_tmp = c->alloc<char>(c->scan_bytes);
     c->d_diag_part = c->alloc<double>(1024 * 8);
     c->d_diag_out = c->alloc<double>(16);

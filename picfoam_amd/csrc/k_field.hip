@@ -6,10 +6,10 @@
 namespace picmi {
 
 // ---------------------------------------------------------------- scan
-constexpr int kScanBlock = 1024;
 
-__global__ void __launch_bounds__(kScanBlock) k_scan_block(const int* __restrict__ in, int* __restrict__ out, int n,
-                                                           int* __restrict__ totals) {
+// block scan of 1024 counts (exclusive, into out) and the block's total into totals[block]
+__device__ __forceinline__ void k_scan_block_body(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                  int* __restrict__ totals) {
     __shared__ int s[kScanBlock];
     const int tid = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * kScanBlock + tid;
@@ -26,7 +26,8 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_block(const int* __restrict
     if (tid == kScanBlock - 1) totals[blockIdx.x] = s[tid];
 }
 
-__global__ void __launch_bounds__(kScanBlock) k_scan_totals(int* totals, int nb) {
+// exclusive scan of nb block totals in place by one block; returns the grand total
+__device__ __forceinline__ int k_scan_totals_body(int* totals, int nb) {
     __shared__ int s[kScanBlock];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
@@ -47,7 +48,15 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_totals(int* totals, int nb)
         if (tid == kScanBlock - 1) carry += s[tid];
         __syncthreads();
     }
+    return carry;
 }
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_block(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                           int* __restrict__ totals) {
+    k_scan_block_body(in, out, n, totals);
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_totals(int* totals, int nb) { k_scan_totals_body(totals, nb); }
 
 __global__ void k_scan_add(int* out, int n, const int* totals) {
     const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
@@ -64,6 +73,40 @@ hipError_t launch_scan(const int* in, int* out, int n, void* temp, size_t temp_b
     hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, in, out, n, totals);
     hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanBlock), 0, s, totals, nb);
     hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, s, out, n, totals);
+    return hipGetLastError();
+}
+
+// Exclusive scans of the cell counts of every species in three launches (block scans,
+// scan of the block totals, offsets added), blockIdx.y the species; the totals pass also
+// stores each species' grand total (its live parcel count after the sort).
+__global__ void __launch_bounds__(kScanBlock) k_scan_cells_block(ScanSet a) {
+    const int sp = blockIdx.y;
+    const int nb = (a.n + kScanBlock - 1) / kScanBlock;
+    k_scan_block_body(a.in[sp], a.out[sp], a.n, a.tmp + sp * (nb + 1));
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_cells_totals(ScanSet a) {
+    const int sp = blockIdx.x;
+    const int nb = (a.n + kScanBlock - 1) / kScanBlock;
+    const int total = k_scan_totals_body(a.tmp + sp * (nb + 1), nb);
+    if (threadIdx.x == 0) *a.total[sp] = total;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_cells_add(ScanSet a) {
+    const int sp = blockIdx.y;
+    const int nb = (a.n + kScanBlock - 1) / kScanBlock;
+    const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    if (i < a.n) a.out[sp][i] += a.tmp[sp * (nb + 1) + blockIdx.x];
+}
+
+size_t scan_cells_temp_bytes(int n) { return sizeof(int) * kMaxSpecies * (size_t)((n + kScanBlock - 1) / kScanBlock + 1); }
+
+hipError_t launch_scan_cells(const ScanSet& a, int n_species, hipStream_t s) {
+    if (a.n <= 0 || n_species <= 0) return hipSuccess;
+    const int nb = (a.n + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_scan_cells_block, dim3(nb, n_species), dim3(kScanBlock), 0, s, a);
+    hipLaunchKernelGGL(k_scan_cells_totals, dim3(n_species), dim3(kScanBlock), 0, s, a);
+    hipLaunchKernelGGL(k_scan_cells_add, dim3(nb, n_species), dim3(kScanBlock), 0, s, a);
     return hipGetLastError();
 }
 

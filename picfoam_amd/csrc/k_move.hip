@@ -218,7 +218,9 @@ __global__ void __launch_bounds__(256) k_move(MoveArgs<UT> a) {
     // cell count (sort histogram; for CellAverage also the deposit) and the parcel's slot
     // in its new cell, which the sort's scatter uses directly
     const int slot = wave_aggregated_inc(a.count, newcell);
-    if (a.place && i < n) a.place[i] = make_int2(newcell, slot);
+    // slots past the live count up to the launch bound are marked empty (cell -1), so the
+    // scatter needs only the bound
+    if (a.place && i < a.n) a.place[i] = make_int2(newcell, slot);
     if constexpr (DEP == kVolumeWeighting) {
         if (newcell >= 0) {
             const TetGeo& g = a.tets[t];
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__(256) k_deposit(MoveArgs<UT> a) {
         if (t >= 0) { cell = a.tets[t].cell; y1 = a.p.y1[i]; y2 = a.p.y2[i]; y3 = a.p.y3[i]; }
     }
     const int slot = wave_aggregated_inc(a.count, cell);
-    if (a.place && i < n) a.place[i] = make_int2(cell, slot);
+    if (a.place && i < a.n) a.place[i] = make_int2(cell, slot);
     if constexpr (DEP == kVolumeWeighting) {
         if (cell >= 0) {
             const TetGeo& g = a.tets[t];
@@ -364,15 +366,14 @@ __global__ void k_init(InitArgs<UT> a) {
 // Counting-sort scatter (buildCellOccupancy, PICCloud.C:327-354): live parcels go to
 // start[cell] + slot, with (cell, slot) handed over by the counting pass (the slot comes
 // from a wave-aggregated atomic there, so the order inside a cell is not fixed; every
-// consumer that depends on it orders the cell's parcels by id or by a random key).
+// consumer that depends on it orders the cell's parcels by id or by a random key).  The
+// counting pass marks deleted parcels and the slots between the live count and the launch
+// bound with cell -1, so the device live count (already the sorted count here) is not read.
 template <class UT>
 __global__ void __launch_bounds__(256) k_scatter(PartSoA<UT> src, PartSoA<UT> dst, int64_t n_bound,
-                                                  const int64_t* __restrict__ n_live, const int2* __restrict__ place,
-                                                  const int* __restrict__ start) {
+                                                  const int2* __restrict__ place, const int* __restrict__ start) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // the device count can exceed the arrays after a capacity overflow (flagged)
-    const int64_t n = min(*n_live, n_bound);
-    if (i >= n) return;
+    if (i >= n_bound) return;
     const int2 pc = place[i];
     if (pc.x < 0) return;
     const int64_t o = (int64_t)start[pc.x] + pc.y;
@@ -485,20 +486,19 @@ hipError_t launch_init(const InitArgs<double>& a, hipStream_t s) { return launch
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s) { return launch_init_t(a, s); }
 
 template <class UT>
-static hipError_t launch_scatter_t(const PartSoA<UT>& src, const PartSoA<UT>& dst, int64_t n, const int64_t* n_live,
-                                   const int2* place, const int* start, hipStream_t s) {
+static hipError_t launch_scatter_t(const PartSoA<UT>& src, const PartSoA<UT>& dst, int64_t n, const int2* place,
+                                   const int* start, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, n_live, place,
-                       start);
+    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, place, start);
     return hipGetLastError();
 }
-hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int64_t* n_live,
-                          const int2* place, const int* start, hipStream_t s) {
-    return launch_scatter_t(src, dst, n, n_live, place, start, s);
+hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int2* place,
+                          const int* start, hipStream_t s) {
+    return launch_scatter_t(src, dst, n, place, start, s);
 }
-hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int64_t* n_live,
-                          const int2* place, const int* start, hipStream_t s) {
-    return launch_scatter_t(src, dst, n, n_live, place, start, s);
+hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int2* place,
+                          const int* start, hipStream_t s) {
+    return launch_scatter_t(src, dst, n, place, start, s);
 }
 
 }  // namespace picmi

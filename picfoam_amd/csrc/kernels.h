@@ -66,12 +66,12 @@ hipError_t launch_positions(const PartSoA<float>& p, int64_t n, const TetGeo* te
                             const double* pts, double* x, hipStream_t s);
 hipError_t launch_init(const InitArgs<double>& a, hipStream_t s);
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s);
-// n: launch bound; n_live: live source count on the device; place: (cell, slot) per parcel
-// from the counting pass (cell < 0: dropped)
-hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int64_t* n_live,
-                          const int2* place, const int* start, hipStream_t s);
-hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int64_t* n_live,
-                          const int2* place, const int* start, hipStream_t s);
+// n: launch bound (>= the live source count); place: (cell, slot) per parcel from the
+// counting pass, cell < 0 for dropped parcels and the slots past the live count
+hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int2* place,
+                          const int* start, hipStream_t s);
+hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int2* place,
+                          const int* start, hipStream_t s);
 
 // ---------------------------------------------------------------- fields (k_field.hip)
 // exclusive scan of n+1 ints (last = total); temp storage sized by scan_temp_bytes
@@ -79,6 +79,19 @@ size_t scan_temp_bytes(int n);
 hipError_t launch_scan(const int* in, int* out, int n, void* temp, size_t temp_bytes, hipStream_t s);
 // *dst = *src (the live count after a sort: start[n_cells] -> the species count)
 hipError_t launch_store_count(const int* src, int64_t* dst, hipStream_t s);
+// exclusive scans of n ints (the last count zero, so out[n-1] is the total) for every
+// species in the same three launches, totals stored as int64; tmp sized by
+// scan_cells_temp_bytes
+constexpr int kScanBlock = 1024;
+struct ScanSet {
+    const int* in[kMaxSpecies];
+    int* out[kMaxSpecies];
+    int64_t* total[kMaxSpecies];
+    int* tmp;
+    int n;
+};
+size_t scan_cells_temp_bytes(int n);
+hipError_t launch_scan_cells(const ScanSet& a, int n_species, hipStream_t s);
 
 struct DepositFinishArgs {
     int n_cells;
