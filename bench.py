@@ -245,7 +245,7 @@ def main():
     coul_ms, coul_n = stages["coulomb"]
     per_launch_move = move_ms / max(move_n, 1)
     per_launch_coul = coul_ms / max(coul_n, 1)
-    move_bytes = (n_total / 2) * 2 * sb          # one species per launch
+    move_bytes = n_total * 2 * sb                # every species in one launch
     coul_bytes = n_total * (2 * 3 * (4 if args.precision == "mixed" else 8) + 4)
     cand = {"move": (per_launch_move, move_bytes), "coulomb": (per_launch_coul, coul_bytes)}
     dom = max(cand, key=lambda k: cand[k][0] * (move_n if k == "move" else coul_n))

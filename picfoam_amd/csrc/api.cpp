@@ -294,28 +294,46 @@ MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
     return a;
 }
 
-int do_move(pic_ctx* c, double dt) {
-    zero_deposit(c);
+template <class UT>
+int do_move_t(pic_ctx* c, double dt) {
+    pic_ctx::Scope sc(c, PIC_STAGE_MOVE);  // one timed launch for all moving species
+    MoveSet<UT> m{};
     for (int s = 0; s < (int)c->sp.size(); ++s) {
         Species& S = c->sp[s];
         if (S.n == 0) continue;
-        pic_ctx::Scope sc(c, PIC_STAGE_MOVE);  // one timed launch per species
-        hipError_t e;
         if (!S.solve_movement) {
             // stationary species are still counted / deposited (PICParcel.C:51)
-            if (c->mixed()) e = launch_deposit(move_args<float>(c, s, dt), c->cd, c->stream);
-            else e = launch_deposit(move_args<double>(c, s, dt), c->cd, c->stream);
-        } else if (c->mixed()) {
-            e = launch_move(move_args<float>(c, s, dt), c->pusher, c->fw, c->cd, c->stream);
+            if (int r = check_hip(c, launch_deposit(move_args<UT>(c, s, dt), c->cd, c->stream), "deposit")) return r;
         } else {
-            e = launch_move(move_args<double>(c, s, dt), c->pusher, c->fw, c->cd, c->stream);
+            m.a[m.ns++] = move_args<UT>(c, s, dt);
         }
-        if (int r = check_hip(c, e, "move kernel")) return r;
     }
+    if (m.ns == 0) return PIC_OK;
+    return check_hip(c, launch_move(m, c->pusher, c->fw, c->cd, c->stream), "move kernel");
+}
+
+int do_move(pic_ctx* c, double dt) {
+    zero_deposit(c);
+    if (int r = c->mixed() ? do_move_t<float>(c, dt) : do_move_t<double>(c, dt)) return r;
     c->counts_valid = true;
     c->place_valid = true;
     c->sorted = false;
     return PIC_OK;
+}
+
+template <class UT>
+ScatterSet<UT> scatter_set(pic_ctx* c) {
+    ScatterSet<UT> m{};
+    m.ns = (int)c->sp.size();
+    for (int s = 0; s < m.ns; ++s) {
+        Species& S = c->sp[s];
+        m.src[s] = soa<UT>(S, S.cur);
+        m.dst[s] = soa<UT>(S, 1 - S.cur);
+        m.n[s] = S.n;
+        m.place[s] = S.place;
+        m.start[s] = S.start;
+    }
+    return m;
 }
 
 // buildCellOccupancy (PICCloud.C:327-354): counting sort of every species by cell.  No
@@ -343,17 +361,10 @@ int do_sort(pic_ctx* c) {
         ss.in[s] = c->sp[s].count; ss.out[s] = c->sp[s].start; ss.total[s] = c->d_n + s;
     }
     if (int r = check_hip(c, launch_scan_cells(ss, (int)c->sp.size(), c->stream), "scan")) return r;
-    for (int s = 0; s < (int)c->sp.size(); ++s) {
-        Species& S = c->sp[s];
-        hipError_t e = hipSuccess;
-        const int nb = 1 - S.cur;
-        if (c->mixed())
-            e = launch_scatter(soa<float>(S, S.cur), soa<float>(S, nb), S.n, S.place, S.start, c->stream);
-        else
-            e = launch_scatter(soa<double>(S, S.cur), soa<double>(S, nb), S.n, S.place, S.start, c->stream);
-        if (int r = check_hip(c, e, "scatter")) return r;
-        S.cur = nb;
-    }
+    const hipError_t e = c->mixed() ? launch_scatter(scatter_set<float>(c), c->stream)
+                                    : launch_scatter(scatter_set<double>(c), c->stream);
+    if (int r = check_hip(c, e, "scatter")) return r;
+    for (auto& S : c->sp) S.cur = 1 - S.cur;
     c->sorted = true;
     c->place_valid = false;   // the slots refer to the layout before this sort
     return PIC_OK;

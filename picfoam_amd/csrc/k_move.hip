@@ -169,10 +169,11 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
     return t;
 }
 
-// One lane per parcel: push (if charged and moving) + track + deposit/count.
+// One lane per parcel: push (if charged and moving) + track + deposit/count; `block` is
+// the block's index within its species.
 template <class UT, int P, int W, int DEP>
-__global__ void __launch_bounds__(256) k_move(MoveArgs<UT> a) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block) {
+    const int64_t i = block * blockDim.x + threadIdx.x;
     const int64_t n = a.n_live ? min(*a.n_live, a.n) : a.n;
     int newcell = -1;
     double y1 = 0, y2 = 0, y3 = 0;
@@ -232,6 +233,15 @@ __global__ void __launch_bounds__(256) k_move(MoveArgs<UT> a) {
             atomicAdd(a.acc_point + g.vm[2], (unsigned long long)llrint(y3 * s));
         }
     }
+}
+
+// every moving species in one launch: blocks [block0[s], block0[s+1]) belong to species s,
+// so the short passes of slow species overlap the long tracks of fast ones
+template <class UT, int P, int W, int DEP>
+__global__ void __launch_bounds__(256) k_move(MoveSet<UT> m) {
+    int s = 0;
+    while (s + 1 < m.ns && (int64_t)blockIdx.x >= m.block0[s + 1]) ++s;
+    move_body<UT, P, W, DEP>(m.a[s], (int64_t)blockIdx.x - m.block0[s]);
 }
 
 template <class UT, int P, int W>
@@ -369,11 +379,17 @@ __global__ void k_init(InitArgs<UT> a) {
 // consumer that depends on it orders the cell's parcels by id or by a random key).  The
 // counting pass marks deleted parcels and the slots between the live count and the launch
 // bound with cell -1, so the device live count (already the sorted count here) is not read.
+// Every species in one launch, blocks [block0[s], block0[s+1]) for species s.
 template <class UT>
-__global__ void __launch_bounds__(256) k_scatter(PartSoA<UT> src, PartSoA<UT> dst, int64_t n_bound,
-                                                  const int2* __restrict__ place, const int* __restrict__ start) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_bound) return;
+__global__ void __launch_bounds__(256) k_scatter(ScatterSet<UT> m) {
+    int sp = 0;
+    while (sp + 1 < m.ns && (int64_t)blockIdx.x >= m.block0[sp + 1]) ++sp;
+    const PartSoA<UT>& src = m.src[sp];
+    const PartSoA<UT>& dst = m.dst[sp];
+    const int2* __restrict__ place = m.place[sp];
+    const int* __restrict__ start = m.start[sp];
+    const int64_t i = ((int64_t)blockIdx.x - m.block0[sp]) * blockDim.x + threadIdx.x;
+    if (i >= m.n[sp]) return;
     const int2 pc = place[i];
     if (pc.x < 0) return;
     const int64_t o = (int64_t)start[pc.x] + pc.y;
@@ -386,12 +402,18 @@ __global__ void __launch_bounds__(256) k_scatter(PartSoA<UT> src, PartSoA<UT> ds
 // ---------------------------------------------------------------- instantiation / launch
 
 template <class UT>
-static hipError_t launch_move_t(const MoveArgs<UT>& a, int pusher, int weighting, int dep, hipStream_t s) {
-    if (a.n <= 0) return hipSuccess;
-    const dim3 blk(256), grd((unsigned)((a.n + 255) / 256));
+static hipError_t launch_move_t(MoveSet<UT> m, int pusher, int weighting, int dep, hipStream_t s) {
+    int64_t blocks = 0;
+    for (int k = 0; k < m.ns; ++k) {
+        m.block0[k] = blocks;
+        blocks += (m.a[k].n + 255) / 256;
+    }
+    m.block0[m.ns] = blocks;
+    if (blocks <= 0) return hipSuccess;
+    const dim3 blk(256), grd((unsigned)blocks);
 #define PICMI_MOVE_CASE(P, W, D)                                                     \
     if (pusher == P && weighting == W && dep == D) {                                 \
-        hipLaunchKernelGGL((k_move<UT, P, W, D>), grd, blk, 0, s, a);                \
+        hipLaunchKernelGGL((k_move<UT, P, W, D>), grd, blk, 0, s, m);                \
         return hipGetLastError();                                                    \
     }
 #define PICMI_MOVE_P(P)                                                              \
@@ -427,11 +449,11 @@ static hipError_t launch_push_t(const MoveArgs<UT>& a, int pusher, int weighting
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_move(const MoveArgs<double>& a, int pusher, int weighting, int dep, hipStream_t s) {
-    return launch_move_t(a, pusher, weighting, dep, s);
+hipError_t launch_move(const MoveSet<double>& m, int pusher, int weighting, int dep, hipStream_t s) {
+    return launch_move_t(m, pusher, weighting, dep, s);
 }
-hipError_t launch_move(const MoveArgs<float>& a, int pusher, int weighting, int dep, hipStream_t s) {
-    return launch_move_t(a, pusher, weighting, dep, s);
+hipError_t launch_move(const MoveSet<float>& m, int pusher, int weighting, int dep, hipStream_t s) {
+    return launch_move_t(m, pusher, weighting, dep, s);
 }
 hipError_t launch_push(const MoveArgs<double>& a, int pusher, int weighting, hipStream_t s) {
     return launch_push_t(a, pusher, weighting, s);
@@ -486,19 +508,18 @@ hipError_t launch_init(const InitArgs<double>& a, hipStream_t s) { return launch
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s) { return launch_init_t(a, s); }
 
 template <class UT>
-static hipError_t launch_scatter_t(const PartSoA<UT>& src, const PartSoA<UT>& dst, int64_t n, const int2* place,
-                                   const int* start, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, place, start);
+static hipError_t launch_scatter_t(ScatterSet<UT> m, hipStream_t s) {
+    int64_t blocks = 0;
+    for (int k = 0; k < m.ns; ++k) {
+        m.block0[k] = blocks;
+        blocks += (m.n[k] + 255) / 256;
+    }
+    m.block0[m.ns] = blocks;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)blocks), dim3(256), 0, s, m);
     return hipGetLastError();
 }
-hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int2* place,
-                          const int* start, hipStream_t s) {
-    return launch_scatter_t(src, dst, n, place, start, s);
-}
-hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int2* place,
-                          const int* start, hipStream_t s) {
-    return launch_scatter_t(src, dst, n, place, start, s);
-}
+hipError_t launch_scatter(const ScatterSet<double>& m, hipStream_t s) { return launch_scatter_t(m, s); }
+hipError_t launch_scatter(const ScatterSet<float>& m, hipStream_t s) { return launch_scatter_t(m, s); }
 
 }  // namespace picmi

@@ -52,8 +52,16 @@ struct InitArgs {
     uint64_t seed;
 };
 
-hipError_t launch_move(const MoveArgs<double>& a, int pusher, int weighting, int dep, hipStream_t s);
-hipError_t launch_move(const MoveArgs<float>& a, int pusher, int weighting, int dep, hipStream_t s);
+// push + track + count/deposit of every moving species in one launch (block0 filled by
+// the launcher)
+template <class UT>
+struct MoveSet {
+    MoveArgs<UT> a[kMaxSpecies];
+    int64_t block0[kMaxSpecies + 1];
+    int ns;
+};
+hipError_t launch_move(const MoveSet<double>& m, int pusher, int weighting, int dep, hipStream_t s);
+hipError_t launch_move(const MoveSet<float>& m, int pusher, int weighting, int dep, hipStream_t s);
 hipError_t launch_push(const MoveArgs<double>& a, int pusher, int weighting, hipStream_t s);
 hipError_t launch_push(const MoveArgs<float>& a, int pusher, int weighting, hipStream_t s);
 hipError_t launch_deposit(const MoveArgs<double>& a, int dep, hipStream_t s);
@@ -66,12 +74,20 @@ hipError_t launch_positions(const PartSoA<float>& p, int64_t n, const TetGeo* te
                             const double* pts, double* x, hipStream_t s);
 hipError_t launch_init(const InitArgs<double>& a, hipStream_t s);
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s);
-// n: launch bound (>= the live source count); place: (cell, slot) per parcel from the
-// counting pass, cell < 0 for dropped parcels and the slots past the live count
-hipError_t launch_scatter(const PartSoA<double>& src, const PartSoA<double>& dst, int64_t n, const int2* place,
-                          const int* start, hipStream_t s);
-hipError_t launch_scatter(const PartSoA<float>& src, const PartSoA<float>& dst, int64_t n, const int2* place,
-                          const int* start, hipStream_t s);
+// counting-sort scatter of every species in one launch: n = launch bound (>= the live
+// source count); place = (cell, slot) per parcel from the counting pass, cell < 0 for
+// dropped parcels and the slots past the live count; block0 filled by the launcher
+template <class UT>
+struct ScatterSet {
+    PartSoA<UT> src[kMaxSpecies], dst[kMaxSpecies];
+    int64_t n[kMaxSpecies];
+    const int2* place[kMaxSpecies];
+    const int* start[kMaxSpecies];
+    int64_t block0[kMaxSpecies + 1];
+    int ns;
+};
+hipError_t launch_scatter(const ScatterSet<double>& m, hipStream_t s);
+hipError_t launch_scatter(const ScatterSet<float>& m, hipStream_t s);
 
 // ---------------------------------------------------------------- fields (k_field.hip)
 // exclusive scan of n+1 ints (last = total); temp storage sized by scan_temp_bytes
