@@ -77,6 +77,7 @@ struct pic_ctx {
     bool e_zero = true;                      // E is zero in every cell (never set, no field solve)
     bool sorted = false, counts_valid = false;
     bool place_valid = false;                // Species::place matches the current layout
+    bool big_n_zero = false;                 // d_big_n zeroed since its last use
     // species
     std::vector<Species> sp;
     double Q0 = 1.0;
@@ -252,13 +253,18 @@ int alloc_species(pic_ctx* c, Species& s) {
 
 double fixed_scale(const pic_ctx* c, int s) { return c->sp[s].charge * c->sp[s].weight / c->Q0 * kFixedScale; }
 
-// zero counts and accumulators before a deposit pass
+// zero counts and accumulators before a deposit pass, and the large-cell list count of the
+// Coulomb pass that follows, in one launch
 void zero_deposit(pic_ctx* c) {
-    for (auto& s : c->sp) hipMemsetAsync(s.count, 0, sizeof(int) * (c->hm.n_cells + 1), c->stream);
+    ZeroSet z{};
+    for (auto& s : c->sp) { z.p[z.k] = s.count; z.words[z.k++] = c->hm.n_cells + 1; }
     if (c->cd == kVolumeWeighting) {
-        hipMemsetAsync(c->d_acc_cell, 0, sizeof(unsigned long long) * c->hm.n_cells, c->stream);
-        hipMemsetAsync(c->d_acc_point, 0, sizeof(unsigned long long) * c->hm.n_points, c->stream);
+        z.p[z.k] = c->d_acc_cell; z.words[z.k++] = 2 * (int64_t)c->hm.n_cells;
+        z.p[z.k] = c->d_acc_point; z.words[z.k++] = 2 * (int64_t)c->hm.n_points;
     }
+    z.p[z.k] = c->d_big_n; z.words[z.k++] = 1;
+    launch_zero(z, c->stream);
+    c->big_n_zero = true;
 }
 
 template <class UT>
@@ -421,7 +427,8 @@ int do_coulomb(pic_ctx* c, double dt) {
     }
     a.big_list = c->d_big_list;
     a.big_n = c->d_big_n;
-    hipMemsetAsync(c->d_big_n, 0, sizeof(int), c->stream);
+    if (!c->big_n_zero) hipMemsetAsync(c->d_big_n, 0, sizeof(int), c->stream);
+    c->big_n_zero = false;
     a.model = c->coulomb_model;
     a.pairing = c->pairing;
     a.weight_correction = c->weight_corr;
@@ -476,6 +483,7 @@ int do_mcc_t(pic_ctx* c, double dt) {
     a.big_list = c->d_big_list;
     a.big_n = c->d_big_n;
     hipMemsetAsync(c->d_big_n, 0, sizeof(int), c->stream);
+    c->big_n_zero = false;
     a.flags = c->d_flags;
     a.counts = c->d_stats + 3;
     a.tets = c->d_tets;

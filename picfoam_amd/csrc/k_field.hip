@@ -110,6 +110,25 @@ hipError_t launch_scan_cells(const ScanSet& a, int n_species, hipStream_t s) {
     return hipGetLastError();
 }
 
+// zeroes the listed ranges of 4-byte words in one launch (the per-step counters)
+__global__ void __launch_bounds__(256) k_zero(ZeroSet z) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int r = 0; r < z.k; ++r) {
+        uint32_t* p = (uint32_t*)z.p[r];
+        for (int64_t i = t; i < z.words[r]; i += stride) p[i] = 0u;
+    }
+}
+
+hipError_t launch_zero(const ZeroSet& z, hipStream_t s) {
+    int64_t most = 0;
+    for (int r = 0; r < z.k; ++r) most = z.words[r] > most ? z.words[r] : most;
+    if (most <= 0) return hipSuccess;
+    const int64_t nb = (most + 255) / 256;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)(nb < 1024 ? nb : 1024)), dim3(256), 0, s, z);
+    return hipGetLastError();
+}
+
 __global__ void k_store_count(const int* __restrict__ src, int64_t* __restrict__ dst) {
     if (threadIdx.x == 0) *dst = *src;
 }

@@ -107,6 +107,13 @@ struct ScanSet {
     int n;
 };
 size_t scan_cells_temp_bytes(int n);
+// up to 12 ranges of 4-byte words zeroed by one launch
+struct ZeroSet {
+    void* p[12];
+    int64_t words[12];
+    int k;
+};
+hipError_t launch_zero(const ZeroSet& z, hipStream_t s);
 hipError_t launch_scan_cells(const ScanSet& a, int n_species, hipStream_t s);
 
 struct DepositFinishArgs {
