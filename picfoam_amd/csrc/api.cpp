@@ -99,6 +99,7 @@ struct pic_ctx {
     int* d_big_list = nullptr;               // cells left to the large-cell collision passes
     double* d_moments = nullptr;             // cell moments scratch [n_cells][11]
     int* d_big_n = nullptr;
+    PairConst* d_pair = nullptr;             // Coulomb constants per ordered species pair
     void* d_scan_tmp = nullptr;
     size_t scan_bytes = 0;
     double* d_diag_part = nullptr;
@@ -434,6 +435,7 @@ int do_coulomb(pic_ctx* c, double dt) {
     a.weight_correction = c->weight_corr;
     a.flags = c->d_flags;
     a.ncoll = c->d_stats + kStatsNcoll;
+    a.pair = c->d_pair;
     return check_hip(c, launch_coulomb(a, c->mixed(), c->stream), "coulomb kernel");
 }
 
@@ -785,6 +787,13 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         c->sp.push_back(S);
     }
     c->Q0 = qmax > 0 ? qmax : 1.0;
+    {
+        std::vector<double> ms(n_species), qs(n_species);
+        for (int s = 0; s < n_species; ++s) { ms[s] = c->sp[s].mass; qs[s] = c->sp[s].charge; }
+        std::vector<PairConst> pc(kMaxSpecies * kMaxSpecies);
+        pair_constants(ms.data(), qs.data(), n_species, pc.data());
+        c->d_pair = c->upload(pc);
+    }
     // device mesh
     const int nc = m.n_cells, np = m.n_points;
     c->d_tets = c->upload(m.tets);

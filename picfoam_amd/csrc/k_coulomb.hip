@@ -74,14 +74,13 @@ struct StageK {
     int g2_lmax, g2_stride, g2_xo, g2_yo, g2_ko;
 };
 
-__device__ __forceinline__ void finish_stage(StageK& d, double dt) {
-    const double mu = d.mP * d.mQ / (d.mP + d.mQ);
-    const double ad = d.qP * d.qQ / (kEps0 * mu);
-    d.cst = 1.0 / (4.0 * kPi) * (ad * ad) * (d.nQ * d.nP / d.nPQ) * dt;
-    d.fP = d.mQ / (d.mP + d.mQ);
-    d.fQ = d.mP / (d.mP + d.mQ);
-    d.b0c = d.qP * d.qQ / (2.0 * kPi * kEps0) / mu;
-    d.hbc = kH / (2.0 * mu);
+// the stage's Nanbu constants from the pair table (computed once on the host)
+__device__ __forceinline__ void finish_stage(StageK& d, double dt, const PairConst& pc) {
+    d.cst = pc.cst0 * (d.nQ * d.nP / d.nPQ) * dt;
+    d.fP = pc.fP;
+    d.fQ = pc.fQ;
+    d.b0c = pc.b0c;
+    d.hbc = pc.hbc;
 }
 
 // collision t of unit u; false once the unit is exhausted
@@ -364,7 +363,7 @@ __device__ __forceinline__ void stage_intra(StageK& d, const CoulombArgs& a, int
         d.kind = kIntraOddNY; d.units = (cnt - 1) / 2;
         d.nPQ = 2.0 * a.weight[spc] * ((cnt - 1) / 2 + 1) / Vc;
     }
-    finish_stage(d, a.dt);
+    finish_stage(d, a.dt, a.pair[spc * kMaxSpecies + spc]);
 }
 
 // table of the stage between species i (na parcels, list at offset permA) and j (nb, permB);
@@ -424,7 +423,7 @@ __device__ __forceinline__ bool stage_inter(StageK& d, const CoulombArgs& a, int
     const int sP = jP ? j : i, sQ = jP ? i : j;
     d.u[0] = a.ux[sP]; d.u[1] = a.uy[sP]; d.u[2] = a.uz[sP];
     d.u[3] = a.ux[sQ]; d.u[4] = a.uy[sQ]; d.u[5] = a.uz[sQ];
-    finish_stage(d, a.dt);
+    finish_stage(d, a.dt, a.pair[sP * kMaxSpecies + sQ]);
     return jP;
 }
 
@@ -836,6 +835,21 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
 template <class UT, bool WC, int L>
 __global__ void __launch_bounds__(64) k_coulomb_group_perez(CoulombArgs a) {
     coulomb_group<UT, 1, WC, L>(a);
+}
+
+void pair_constants(const double* mass, const double* charge, int n_species, PairConst* out) {
+    for (int p = 0; p < n_species; ++p)
+        for (int q = 0; q < n_species; ++q) {
+            const double mP = mass[p], mQ = mass[q], qP = charge[p], qQ = charge[q];
+            const double mu = mP * mQ / (mP + mQ);
+            const double ad = qP * qQ / (kEps0 * mu);
+            PairConst& c = out[p * kMaxSpecies + q];
+            c.cst0 = 1.0 / (4.0 * kPi) * (ad * ad);
+            c.fP = mQ / (mP + mQ);
+            c.fQ = mP / (mP + mQ);
+            c.b0c = qP * qQ / (2.0 * kPi * kEps0) / mu;
+            c.hbc = kH / (2.0 * mu);
+        }
 }
 
 // key lists, index lists and (cached passes, cache_elem > 0) two slots of velocities and ids

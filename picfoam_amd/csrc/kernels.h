@@ -216,7 +216,15 @@ struct CoulombArgs {
     int weight_correction;            // Nanbu weight correction active (unequal weights)
     uint32_t* flags;
     unsigned long long* ncoll;        // kNcollSlots counters kNcollStride apart; the total is their sum
+    const struct PairConst* pair;     // [P * kMaxSpecies + Q]
 };
+// Coulomb constants of an ordered species pair (P, Q), reduced mass mu:
+// cst0 = (qP qQ / eps0 mu)^2 / (4 pi), fP = mQ / (mP + mQ), fQ = mP / (mP + mQ),
+// b0c = qP qQ / (2 pi eps0 mu), hbc = h / (2 mu)
+struct PairConst {
+    double cst0, fP, fQ, b0c, hbc, pad[3];
+};
+void pair_constants(const double* mass, const double* charge, int n_species, PairConst* out);
 constexpr int kNcollSlots = 64, kNcollStride = 16;
 // step counters on the device: 16 words, then the Coulomb collision counters
 constexpr int kStatsNcoll = 16, kStatsWords = kStatsNcoll + kNcollSlots * kNcollStride;
