@@ -121,6 +121,10 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
         const double fdt = f * dt;
         const double dx = fdt * Ux, dy = fdt * Uy, dz = fdt * Uz;
         const TetGeo& g = tets[t];
+        // the whole record at once: neighbours and permutation travel with the geometry
+        // instead of costing two more dependent loads after the face is known
+        const int4 nb4 = *reinterpret_cast<const int4*>(g.nbr);
+        const uint32_t perm = g.perm;
         const double T1 = g.r[0] * dx + g.r[1] * dy + g.r[2] * dz;
         const double T2 = g.r[3] * dx + g.r[4] * dy + g.r[5] * dz;
         const double T3 = g.r[6] * dx + g.r[7] * dy + g.r[8] * dz;
@@ -148,13 +152,13 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
         double frac = muH * det;
         frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
         sf = sf + f * frac;
-        const int nb = g.nbr[iH];
+        const int nb = (iH & 2) ? ((iH & 1) ? nb4.w : nb4.z) : ((iH & 1) ? nb4.y : nb4.x);
         if (nb < 0) {
             if (nb <= -1000000) status = 2;
             t = -1;
             break;
         }
-        const uint32_t p = (g.perm >> (8 * iH)) & 0xffu;
+        const uint32_t p = (perm >> (8 * iH)) & 0xffu;
         const uint32_t p0 = p & 3u, p1 = (p >> 2) & 3u, p2 = (p >> 4) & 3u;
         // new slot p_k receives Y_k; written as selects so that nothing leaves registers
         double n0 = (p0 == 0) ? Y0 : (p1 == 0) ? Y1 : (p2 == 0) ? Y2 : Y3;
