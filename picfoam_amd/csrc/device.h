@@ -49,6 +49,7 @@ __device__ __forceinline__ double quot(double n, double d) {
     const double q = n * r;
     return fma(r, fma(-d, q, n), q);
 }
+__device__ __forceinline__ float quot(float n, float d) { return n / d; }
 
 // log x, x positive and normal: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh z
 // with z = (m - 1)/(m + 1), |z| < 0.1716, by the series 2 sum z^(2k+1)/(2k+1) to k = 10

@@ -34,7 +34,7 @@ __device__ __forceinline__ M cos_chi(M s, M U) {
                    ((M)0.95602 + ((M)-0.508139 + ((M)0.479139 + ((M)-0.12789 + (M)0.0238957 * s) * s) * s) * s) * s;
     M Abig = (M)1;
     if (__any(!mid && big)) Abig = (M)3 * exp_small(-s);
-    const M r = (M)1 / (mid ? invA : Abig);
+    const M r = quot((M)1, mid ? invA : Abig);
     M A = mid ? r : Abig;
     const M coef = mid ? invA : r;
     if (small || !big) A = (M)0;  // keeps the unused branch finite
