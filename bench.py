@@ -206,6 +206,62 @@ def poisson_benchmark(ncell, ppc, snaps, train_steps, evals, warm, loss_kind=0, 
     return out
 
 
+def timed(cloud, dt, steps, warmup):
+    """warmup + timed pic_run steps with stage events; (ms/step, stage ms/step, last stats)"""
+    cloud.run(dt, warmup)
+    cloud.sync()
+    cloud.profile(True)
+    t0 = time.perf_counter()
+    st = cloud.run(dt, steps)
+    cloud.sync()
+    el = time.perf_counter() - t0
+    stages = cloud.stage_times()
+    cloud.profile(False)
+    return 1e3 * el / steps, {k: round(v[0] / steps, 4) for k, v in stages.items() if v[1]}, st
+
+
+def extra_workloads(names, steps, warmup):
+    """The other full-size configurations, measured the same way (pic_run, device-resident
+    inputs): the magnetised electron push on the jittered 50^3x6 tet lattice (50M electrons,
+    Higuera-Cary, FP32 push / FP64 track) and the argon-like MCC on a 100^3 hex box (100M
+    electrons, Nanbu e-n with ionization, FP32 push)."""
+    from picfoam_amd import cases, picmi
+    out = {}
+    if "c3" in names:
+        print("bench: magnetised tets, 50^3 x 6 tet lattice, 50M electrons", file=sys.stderr, flush=True)
+        mesh, species, models, ex = cases.magnetised_tets(n_lattice=50, n_total=50_000_000, precision="mixed")
+        c = picmi.Cloud(mesh, species, models)
+        c.set_field(picmi.FIELD_E, ex["E"])
+        c.set_field(picmi.FIELD_B, ex["B"])
+        c.init_maxwellian(0, ex["ppc"], ex["T_K"], seed=5)
+        n = c.count(0)
+        ms, stages, _ = timed(c, ex["dt"], steps, warmup)
+        c.close()
+        out["magnetised_tets"] = {"particle_steps_per_s": n / (ms * 1e-3), "ms_per_step": round(ms, 3),
+                                  "stage_ms_per_step": stages, "parcels": n, "cells": mesh["n_cells"],
+                                  "precision": "FP32 push (Higuera-Cary), FP64 track, int64 deposit"}
+    if "c4" in names:
+        print("bench: argon MCC, 100^3 hex, 100M electrons", file=sys.stderr, flush=True)
+        mesh, species, models, ex = cases.argon_mcc(ncell=100, ppc=100, precision="mixed")
+        c = picmi.Cloud(mesh, species, models)
+        e, el, exs, iz = ex["xs"]
+        c.set_cross_section(0, e, el, 0.0)
+        c.set_cross_section(1, e, exs, 11.5)
+        c.set_cross_section(2, e, iz, 15.76)
+        c.mcc_initialize(1000.0, 0.01)
+        c.set_field(picmi.FIELD_E, ex["E"])
+        c.init_maxwellian(0, ex["ppc"], ex["T_K"], seed=3)
+        n = c.count(0)
+        ms, stages, st = timed(c, ex["dt"], steps, warmup)
+        n_end = c.count(0)
+        c.close()
+        out["argon_mcc"] = {"particle_steps_per_s": n / (ms * 1e-3), "ms_per_step": round(ms, 3),
+                            "stage_ms_per_step": stages, "electrons_start_end": [n, n_end],
+                            "mcc_candidates_last_run": st.mcc_candidates, "ionizations_last_run": st.mcc_ionization,
+                            "precision": "FP32 push (Boris), FP64 track, FP64 MCC"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +278,9 @@ def main():
     ap.add_argument("--poisson-ppc", type=int, default=8)
     ap.add_argument("--poisson-loss", type=int, default=0, choices=[0, 1, 2],
                     help="training loss of the learned guess: 0 A-norm error, 1 residual norm, 2 both")
+    ap.add_argument("--workloads", default="", metavar="LIST",
+                    help="also measure other full-size configurations: c3 (magnetised tets, 50M electrons), "
+                         "c4 (argon MCC, 100M electrons), comma-separated")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device three times (a plain copy of known size whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic figures)")
@@ -283,6 +342,8 @@ def main():
         print("bench: Poisson initial-guess comparison", file=sys.stderr, flush=True)
         out["poisson"] = poisson_benchmark(args.poisson_ncell, args.poisson_ppc, snaps=48, train_steps=400, evals=24,
                                            warm=4, loss_kind=args.poisson_loss)
+    if args.workloads:
+        out["workloads"] = extra_workloads(args.workloads.split(","), steps=10, warmup=2)
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(target_s=args.cpu_seconds)

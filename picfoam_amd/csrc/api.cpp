@@ -93,9 +93,11 @@ struct pic_ctx {
     bool has_space = false;
     unsigned long long *d_acc_cell = nullptr, *d_acc_point = nullptr;
     uint32_t* d_flags = nullptr;
-    // 0 deleted, 1 guard, 3..6 mcc; Coulomb collisions in kNcollSlots counters from kStatsNcoll
+    // 0 deleted, 1 guard; Coulomb collisions in kNcollSlots counters from kStatsNcoll, MCC
+    // counts in kNcollSlots groups from kStatsMcc
     unsigned long long* d_stats = nullptr;
     int64_t* d_n = nullptr;                  // device parcel counts per species (authoritative)
+    int64_t* d_n_next = nullptr;             // sorted counts from the scan, installed after the scatter
     int* d_big_list = nullptr;               // cells left to the large-cell collision passes
     double* d_moments = nullptr;             // cell moments scratch [n_cells][11]
     int* d_big_n = nullptr;
@@ -109,6 +111,8 @@ struct pic_ctx {
     double* d_xs_e[3] = {nullptr, nullptr, nullptr};
     double* d_xs_s[3] = {nullptr, nullptr, nullptr};
     int xs_n[3] = {0, 0, 0};
+    int* d_xs_guide[3] = {nullptr, nullptr, nullptr};
+    double xs_le0[3] = {0, 0, 0}, xs_idl[3] = {0, 0, 0};
     double xs_th[3] = {0, 0, 0};
     std::vector<double> xs_he[3], xs_hs[3];
     // Poisson
@@ -337,6 +341,7 @@ ScatterSet<UT> scatter_set(pic_ctx* c) {
         m.src[s] = soa<UT>(S, S.cur);
         m.dst[s] = soa<UT>(S, 1 - S.cur);
         m.n[s] = S.n;
+        m.n_live[s] = c->d_n + s;
         m.place[s] = S.place;
         m.start[s] = S.start;
     }
@@ -359,18 +364,19 @@ int do_sort(pic_ctx* c) {
         c->counts_valid = true;
         c->place_valid = true;
     }
-    // every species' scan in the same launches; they also store the live counts (the
-    // scatter reads only the launch bound)
+    // every species' scan in the same launches; the sorted counts wait in d_n_next while
+    // the scatter reads the live counts before the sort
     ScanSet ss{};
     ss.n = nc + 1;
     ss.tmp = (int*)c->d_scan_tmp;
     for (int s = 0; s < (int)c->sp.size(); ++s) {
-        ss.in[s] = c->sp[s].count; ss.out[s] = c->sp[s].start; ss.total[s] = c->d_n + s;
+        ss.in[s] = c->sp[s].count; ss.out[s] = c->sp[s].start; ss.total[s] = c->d_n_next + s;
     }
     if (int r = check_hip(c, launch_scan_cells(ss, (int)c->sp.size(), c->stream), "scan")) return r;
     const hipError_t e = c->mixed() ? launch_scatter(scatter_set<float>(c), c->stream)
                                     : launch_scatter(scatter_set<double>(c), c->stream);
     if (int r = check_hip(c, e, "scatter")) return r;
+    if (int r = check_hip(c, launch_copy_counts(c->d_n_next, c->d_n, (int)c->sp.size(), c->stream), "counts")) return r;
     for (auto& S : c->sp) S.cur = 1 - S.cur;
     c->sorted = true;
     c->place_valid = false;   // the slots refer to the layout before this sort
@@ -462,7 +468,10 @@ int do_mcc_t(pic_ctx* c, double dt) {
     }
     a.smax = c->d_smax;
     a.rem = c->d_rem;
-    for (int k = 0; k < 3; ++k) { a.xs_e[k] = c->d_xs_e[k]; a.xs_s[k] = c->d_xs_s[k]; a.xs_n[k] = c->xs_n[k]; }
+    for (int k = 0; k < 3; ++k) {
+        a.xs_e[k] = c->d_xs_e[k]; a.xs_s[k] = c->d_xs_s[k]; a.xs_n[k] = c->xs_n[k];
+        a.xs_guide[k] = c->d_xs_guide[k]; a.xs_le0[k] = c->xs_le0[k]; a.xs_idl[k] = c->xs_idl[k];
+    }
     a.th_ex = c->xs_th[1];
     a.th_iz = c->xs_th[2];
     a.relativistic = c->en_relativistic;
@@ -487,7 +496,7 @@ int do_mcc_t(pic_ctx* c, double dt) {
     hipMemsetAsync(c->d_big_n, 0, sizeof(int), c->stream);
     c->big_n_zero = false;
     a.flags = c->d_flags;
-    a.counts = c->d_stats + 3;
+    a.counts = c->d_stats + kStatsMcc;
     a.tets = c->d_tets;
     a.count_e = E.count;
     a.acc_cell = c->d_acc_cell;
@@ -827,6 +836,7 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->d_flags = c->alloc<uint32_t>(1);
     c->d_stats = c->alloc<unsigned long long>(kStatsWords);
     c->d_n = c->alloc<int64_t>(kMaxSpecies);
+    c->d_n_next = c->alloc<int64_t>(kMaxSpecies);
     c->d_big_list = c->alloc<int>(nc);
     c->d_big_n = c->alloc<int>(1);
     c->scan_bytes = std::max(scan_temp_bytes(nc + 1), scan_cells_temp_bytes(nc + 1));
@@ -1015,6 +1025,12 @@ int pic_set_cross_section(pic_ctx* c, int32_t process, int32_t n, const double* 
     c->xs_th[process] = th;
     c->d_xs_e[process] = c->upload(c->xs_he[process]);
     c->d_xs_s[process] = c->upload(c->xs_hs[process]);
+    c->d_xs_guide[process] = nullptr;
+    if (n >= 2 && e[0] > 0.0) {
+        std::vector<int> g(kXsGuide);
+        xs_guide(e, n, g.data(), &c->xs_le0[process], &c->xs_idl[process]);
+        c->d_xs_guide[process] = c->upload(g);
+    }
     return check_hip(c, hipStreamSynchronize(c->stream), "set cross section");
 }
 
@@ -1272,10 +1288,13 @@ void read_step_stats(pic_ctx* c, const pic_poisson_stats& ps, pic_step_stats* st
     st->n_deleted = (int64_t)h[0];
     st->n_track_guard = (int64_t)h[1];
     st->coulomb_collisions = (int64_t)ncoll;
-    st->mcc_candidates = (int64_t)h[3];
-    st->mcc_elastic = (int64_t)h[4];
-    st->mcc_excitation = (int64_t)h[5];
-    st->mcc_ionization = (int64_t)h[6];
+    unsigned long long mc[4] = {0, 0, 0, 0};
+    for (int k = 0; k < kNcollSlots; ++k)
+        for (int j = 0; j < 4; ++j) mc[j] += h[kStatsMcc + k * kNcollStride + j];
+    st->mcc_candidates = (int64_t)mc[0];
+    st->mcc_elastic = (int64_t)mc[1];
+    st->mcc_excitation = (int64_t)mc[2];
+    st->mcc_ionization = (int64_t)mc[3];
     st->poisson = ps;
 }
 

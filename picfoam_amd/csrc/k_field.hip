@@ -129,6 +129,15 @@ hipError_t launch_zero(const ZeroSet& z, hipStream_t s) {
     return hipGetLastError();
 }
 
+__global__ void k_copy_counts(const int64_t* __restrict__ src, int64_t* __restrict__ dst, int n) {
+    if ((int)threadIdx.x < n) dst[threadIdx.x] = src[threadIdx.x];
+}
+
+hipError_t launch_copy_counts(const int64_t* src, int64_t* dst, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_counts, dim3(1), dim3(64), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
 __global__ void k_store_count(const int* __restrict__ src, int64_t* __restrict__ dst) {
     if (threadIdx.x == 0) *dst = *src;
 }

@@ -9,6 +9,8 @@
 // Candidates are drawn with replacement as in the reference; lanes take one candidate
 // each and the lowest lane of a group of candidates that picked the same parcel replays
 // the group in candidate order, so every parcel sees its collisions in sequence.
+#include <cmath>
+
 #include "device.h"
 #include "kernels.h"
 
@@ -16,19 +18,32 @@ namespace picmi {
 
 namespace {
 
-// interpolateXY on an ascending table: clamped linear interpolation
-__device__ double xs_lookup(const double* __restrict__ e, const double* __restrict__ s, int n, double x) {
+// interpolateXY on an ascending table: clamped linear interpolation between e[lo] <= x <
+// e[lo + 1].  With a guide (tables starting above zero) the bracket comes from kXsGuide
+// bins equal in log(e), each holding the last entry at or below its lower edge, and a
+// step or two along the table; otherwise from a bisection.  Both give the same lo.
+__device__ double xs_lookup(const double* __restrict__ e, const double* __restrict__ s, int n,
+                            const int* __restrict__ guide, double le0, double idl, double x) {
     if (n <= 0) return 0.0;
-    if (!(x > e[0])) return (x == e[0] || x < e[0]) ? s[0] : s[0];
+    if (!(x > e[0])) return s[0];
     if (x >= e[n - 1]) return s[n - 1];
-    int lo = 0, hi = n - 1;  // e[lo] < x < e[hi] or equality found
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (e[mid] <= x) lo = mid;
-        else hi = mid;
+    int lo = 0;
+    if (guide) {
+        int b = (int)((log_pos(x) - le0) * idl);
+        b = b < 0 ? 0 : (b >= kXsGuide ? kXsGuide - 1 : b);
+        lo = guide[b];
+        while (lo > 0 && e[lo] > x) --lo;
+        while (lo + 1 < n && e[lo + 1] <= x) ++lo;
+    } else {
+        int hi = n - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (e[mid] <= x) lo = mid;
+            else hi = mid;
+        }
     }
     if (e[lo] == x) return s[lo];
-    return s[lo] + ((x - e[lo]) / (e[hi] - e[lo])) * (s[hi] - s[lo]);
+    return s[lo] + ((x - e[lo]) / (e[lo + 1] - e[lo])) * (s[lo + 1] - s[lo]);
 }
 
 __device__ __forceinline__ void mcc_rand(uint64_t seed, int64_t step, int cell, int k, int j, double& a, double& b) {
@@ -38,25 +53,30 @@ __device__ __forceinline__ void mcc_rand(uint64_t seed, int64_t step, int cell, 
 }
 
 __device__ __forceinline__ void box_muller(double a, double b, double& n1, double& n2) {
-    const double r = sqrt(-2.0 * log(1.0 - a));
-    n1 = r * cos(2.0 * kPi * b);
-    n2 = r * sin(2.0 * kPi * b);
+    const double r = sqrt(-2.0 * log_pos(1.0 - a));
+    double sn, cs;
+    sincos_turn(b, &sn, &cs);
+    n1 = r * cs;
+    n2 = r * sn;
 }
+
+// (1 + eV)^u for eV >= 0, u in [0, 1)
+__device__ __forceinline__ double pow1p(double eV, double u) { return exp_small(u * log_pos(1.0 + eV)); }
 
 // NanbuElectronNeutralCollision::updateVelocity with a sampled partner velocity
 __device__ void nanbu_scatter(double eV, vec3<double>& UE, vec3<double>& Un, double mE, double mN, double uphi,
                               double uchi) {
-    const double phi = 2.0 * kPi * uphi;
     double cc;
     if (eV < 1e-30) cc = 1.0;
-    else cc = 1.0 / eV * (eV + 2. - 2. * pow(1. + eV, uchi));
+    else cc = 1.0 / eV * (eV + 2. - 2. * pow1p(eV, uchi));
     double sc;
     if (cc * cc > 1.0) { cc = 1.0; sc = 0.0; }
     else sc = sqrt(1.0 - cc * cc);
     const vec3<double> g = UE - Un;
     const double gm = mag(g), gp = sqrt(g.y * g.y + g.z * g.z);
     if (!(gp > 0.0)) return;
-    const double cp = cos(phi), sp = sin(phi);
+    double sp, cp;
+    sincos_turn(uphi, &sp, &cp);
     const vec3<double> h = mk3<double>(gp * cp, -1.0 * (g.y * g.x * cp + gm * g.z * sp) / gp,
                                        -1.0 * (g.z * g.x * cp - gm * g.y * sp) / gp);
     const vec3<double> dv = (1.0 - cc) * g + sc * h;
@@ -76,12 +96,13 @@ __device__ void rel_scatter(double eV, vec3<double>& UE, vec3<double>& Un, doubl
     if (eV < 1e-30) {
         cc = 1.0; sc = 0.0;
     } else {
-        cc = 1.0 / eV * (eV + 2. - 2. * pow(1. + eV, uchi));
+        cc = 1.0 / eV * (eV + 2. - 2. * pow1p(eV, uchi));
         if (cc * cc > 1.0) { cc = 1.0; sc = 0.0; }
         else sc = sqrt(1.0 - cc * cc);
     }
-    const double phi = 2.0 * kPi * ueps;
-    com_leave(f, mE, mN, cc, sc, sin(phi), cos(phi), UE, Un);
+    double sp, cp;
+    sincos_turn(ueps, &sp, &cp);
+    com_leave(f, mE, mN, cc, sc, sp, cp, UE, Un);
 }
 
 template <class UT>
@@ -139,9 +160,9 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
     const double gamma = 1.0 / sqrt(1.0 - ((cR * cR) / (c * c)));
     double eV = (gamma - 1.0) * a.massE * c * c / kQe;
     const double sigmaT = smax_local / cR;
-    const double Qel = xs_lookup(a.xs_e[0], a.xs_s[0], a.xs_n[0], eV);
-    const double Qex = xs_lookup(a.xs_e[1], a.xs_s[1], a.xs_n[1], eV);
-    const double Qiz = xs_lookup(a.xs_e[2], a.xs_s[2], a.xs_n[2], eV);
+    const double Qel = xs_lookup(a.xs_e[0], a.xs_s[0], a.xs_n[0], a.xs_guide[0], a.xs_le0[0], a.xs_idl[0], eV);
+    const double Qex = xs_lookup(a.xs_e[1], a.xs_s[1], a.xs_n[1], a.xs_guide[1], a.xs_le0[1], a.xs_idl[1], eV);
+    const double Qiz = xs_lookup(a.xs_e[2], a.xs_s[2], a.xs_n[2], a.xs_guide[2], a.xs_le0[2], a.xs_idl[2], eV);
     const double Qt = Qel + Qex + Qiz;
     qtcr = fmax(qtcr, Qt * cR);
     double us, uc;
@@ -262,11 +283,14 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds
         v2 += __shfl_xor(v2, off);
         v3 += __shfl_xor(v3, off);
     }
+    // one of kNcollSlots counter groups a cache line apart: one address taking an atomic
+    // from every cell would serialise the cells
     if (threadIdx.x == 0) {
-        if (nc) atomicAdd(a.counts + 0, (unsigned long long)nc);
-        if (v1) atomicAdd(a.counts + 1, (unsigned long long)v1);
-        if (v2) atomicAdd(a.counts + 2, (unsigned long long)v2);
-        if (v3) atomicAdd(a.counts + 3, (unsigned long long)v3);
+        unsigned long long* cnt = a.counts + kNcollStride * (blockIdx.x % kNcollSlots);
+        if (nc) atomicAdd(cnt + 0, (unsigned long long)nc);
+        if (v1) atomicAdd(cnt + 1, (unsigned long long)v1);
+        if (v2) atomicAdd(cnt + 2, (unsigned long long)v2);
+        if (v3) atomicAdd(cnt + 3, (unsigned long long)v3);
     }
 }
 
@@ -311,5 +335,18 @@ static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
 }
 hipError_t launch_mcc(const MccArgs<double>& a, hipStream_t s) { return launch_mcc_t(a, s); }
 hipError_t launch_mcc(const MccArgs<float>& a, hipStream_t s) { return launch_mcc_t(a, s); }
+
+// the lookup guide of an ascending table with e[0] > 0 and n >= 2 (host)
+void xs_guide(const double* e, int n, int* guide, double* le0, double* idl) {
+    const double l0 = std::log(e[0]), l1 = std::log(e[n - 1]);
+    *le0 = l0;
+    *idl = kXsGuide / (l1 - l0);
+    int lo = 0;
+    for (int b = 0; b < kXsGuide; ++b) {
+        const double edge = std::exp(l0 + (l1 - l0) * b / kXsGuide);
+        while (lo + 1 < n - 1 && e[lo + 1] <= edge) ++lo;
+        guide[b] = lo;
+    }
+}
 
 }  // namespace picmi

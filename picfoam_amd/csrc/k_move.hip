@@ -223,9 +223,7 @@ __device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block) 
     // cell count (sort histogram; for CellAverage also the deposit) and the parcel's slot
     // in its new cell, which the sort's scatter uses directly
     const int slot = wave_aggregated_inc(a.count, newcell);
-    // slots past the live count up to the launch bound are marked empty (cell -1), so the
-    // scatter needs only the bound
-    if (a.place && i < a.n) a.place[i] = make_int2(newcell, slot);
+    if (a.place && i < n) a.place[i] = make_int2(newcell, slot);
     if constexpr (DEP == kVolumeWeighting) {
         if (newcell >= 0) {
             const TetGeo& g = a.tets[t];
@@ -279,7 +277,7 @@ __global__ void __launch_bounds__(256) k_deposit(MoveArgs<UT> a) {
         if (t >= 0) { cell = a.tets[t].cell; y1 = a.p.y1[i]; y2 = a.p.y2[i]; y3 = a.p.y3[i]; }
     }
     const int slot = wave_aggregated_inc(a.count, cell);
-    if (a.place && i < a.n) a.place[i] = make_int2(cell, slot);
+    if (a.place && i < n) a.place[i] = make_int2(cell, slot);
     if constexpr (DEP == kVolumeWeighting) {
         if (cell >= 0) {
             const TetGeo& g = a.tets[t];
@@ -380,9 +378,8 @@ __global__ void k_init(InitArgs<UT> a) {
 // Counting-sort scatter (buildCellOccupancy, PICCloud.C:327-354): live parcels go to
 // start[cell] + slot, with (cell, slot) handed over by the counting pass (the slot comes
 // from a wave-aggregated atomic there, so the order inside a cell is not fixed; every
-// consumer that depends on it orders the cell's parcels by id or by a random key).  The
-// counting pass marks deleted parcels and the slots between the live count and the launch
-// bound with cell -1, so the device live count (already the sorted count here) is not read.
+// consumer that depends on it orders the cell's parcels by id or by a random key).  Deleted
+// parcels carry cell -1.
 // Every species in one launch, blocks [block0[s], block0[s+1]) for species s.
 template <class UT>
 __global__ void __launch_bounds__(256) k_scatter(ScatterSet<UT> m) {
@@ -393,7 +390,9 @@ __global__ void __launch_bounds__(256) k_scatter(ScatterSet<UT> m) {
     const int2* __restrict__ place = m.place[sp];
     const int* __restrict__ start = m.start[sp];
     const int64_t i = ((int64_t)blockIdx.x - m.block0[sp]) * blockDim.x + threadIdx.x;
-    if (i >= m.n[sp]) return;
+    // the live count before the sort (the scan left the sorted count aside); it can exceed
+    // the arrays after a capacity overflow (flagged)
+    if (i >= min(*m.n_live[sp], m.n[sp])) return;
     const int2 pc = place[i];
     if (pc.x < 0) return;
     const int64_t o = (int64_t)start[pc.x] + pc.y;

@@ -75,12 +75,13 @@ hipError_t launch_positions(const PartSoA<float>& p, int64_t n, const TetGeo* te
 hipError_t launch_init(const InitArgs<double>& a, hipStream_t s);
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s);
 // counting-sort scatter of every species in one launch: n = launch bound (>= the live
-// source count); place = (cell, slot) per parcel from the counting pass, cell < 0 for
-// dropped parcels and the slots past the live count; block0 filled by the launcher
+// source count n_live); place = (cell, slot) per parcel from the counting pass, cell < 0
+// for dropped parcels; block0 filled by the launcher
 template <class UT>
 struct ScatterSet {
     PartSoA<UT> src[kMaxSpecies], dst[kMaxSpecies];
     int64_t n[kMaxSpecies];
+    const int64_t* n_live[kMaxSpecies];
     const int2* place[kMaxSpecies];
     const int* start[kMaxSpecies];
     int64_t block0[kMaxSpecies + 1];
@@ -96,8 +97,9 @@ hipError_t launch_scan(const int* in, int* out, int n, void* temp, size_t temp_b
 // *dst = *src (the live count after a sort: start[n_cells] -> the species count)
 hipError_t launch_store_count(const int* src, int64_t* dst, hipStream_t s);
 // exclusive scans of n ints (the last count zero, so out[n-1] is the total) for every
-// species in the same three launches, totals stored as int64; tmp sized by
-// scan_cells_temp_bytes
+// species in the same three launches, totals stored as int64 (kept aside until the scatter,
+// which reads the live counts before the sort, has run; launch_copy_counts then installs
+// them); tmp sized by scan_cells_temp_bytes
 constexpr int kScanBlock = 1024;
 struct ScanSet {
     const int* in[kMaxSpecies];
@@ -115,6 +117,7 @@ struct ZeroSet {
 };
 hipError_t launch_zero(const ZeroSet& z, hipStream_t s);
 hipError_t launch_scan_cells(const ScanSet& a, int n_species, hipStream_t s);
+hipError_t launch_copy_counts(const int64_t* src, int64_t* dst, int n, hipStream_t s);
 
 struct DepositFinishArgs {
     int n_cells;
@@ -226,12 +229,17 @@ struct PairConst {
 };
 void pair_constants(const double* mass, const double* charge, int n_species, PairConst* out);
 constexpr int kNcollSlots = 64, kNcollStride = 16;
-// step counters on the device: 16 words, then the Coulomb collision counters
-constexpr int kStatsNcoll = 16, kStatsWords = kStatsNcoll + kNcollSlots * kNcollStride;
+// step counters on the device: 16 words, then the Coulomb collision counters, then the MCC
+// counters (spread over slots for the same reason)
+constexpr int kStatsNcoll = 16, kStatsMcc = kStatsNcoll + kNcollSlots * kNcollStride,
+              kStatsWords = kStatsMcc + kNcollSlots * kNcollStride;
 size_t coulomb_lds_bytes(int maxp, size_t cache_elem);
 hipError_t launch_coulomb(const CoulombArgs& a, bool fp32_velocity, hipStream_t s);
 
 // ---------------------------------------------------------------- MCC (k_mcc.hip)
+constexpr int kXsGuide = 1024;
+void xs_guide(const double* e, int n, int* guide, double* le0, double* idl);
+
 template <class UT>
 struct MccArgs {
     int n_cells;
@@ -247,6 +255,8 @@ struct MccArgs {
     const double* xs_e[3];    // energies per process
     const double* xs_s[3];    // values
     int xs_n[3];
+    const int* xs_guide[3];   // kXsGuide bins equal in log(e) -> last entry at or below the bin, or null
+    double xs_le0[3], xs_idl[3];  // log(e[0]) and kXsGuide / (log(e[n-1]) - log(e[0]))
     double th_ex, th_iz;
     double massE, massN, nN, vthN, drift[3], wEwN, wNwE;
     int create_ion, create_electron;
@@ -259,7 +269,8 @@ struct MccArgs {
     int* big_list;                // cells left to the large-cell pass, [n_cells]
     int* big_n;                   // their number (zero before the launch)
     uint32_t* flags;
-    unsigned long long* counts;   // candidates, elastic, excitation, ionization
+    unsigned long long* counts;   // kNcollSlots slots kNcollStride apart, each: candidates,
+                                  // elastic, excitation, ionization; the totals are the sums
     // deposit of created parcels
     const TetGeo* tets;
     int* count_e;                 // per-cell counts (CellAverage / sort histogram)
