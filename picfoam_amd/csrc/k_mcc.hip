@@ -4,8 +4,8 @@
 // NanbuElectronNeutralCollision.C:98-130, ListCrossSection interpolateXY
 // ListCrossSection.C:32-92).
 //
-// One wave per cell.  The cell's electrons are ordered by id (bitonic sort in LDS) so that
-// a candidate index names the same parcel whatever order the sort scatter left them in.
+// One wave per cell.  The cell's electrons are ranked by id in LDS so that a candidate
+// index names the same parcel whatever order the sort scatter left them in.
 // Candidates are drawn with replacement as in the reference; lanes take one candidate
 // each and the lowest lane of a group of candidates that picked the same parcel replays
 // the group in candidate order, so every parcel sees its collisions in sequence.
@@ -110,24 +110,6 @@ __device__ __forceinline__ void en_scatter(const MccArgs<UT>& a, double eV, vec3
                                            double us, double uc) {
     if (a.relativistic) rel_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
     else nanbu_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
-}
-
-__device__ void bitonic_sort_u64(uint64_t* key, int* val, int npow2) {
-    for (int k = 2; k <= npow2; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < npow2; i += blockDim.x) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const bool up = (i & k) == 0;
-                    const uint64_t a = key[i], b = key[l];
-                    if ((a > b) == up) {
-                        key[i] = b; key[l] = a;
-                        const int t = val[i]; val[i] = val[l]; val[l] = t;
-                    }
-                }
-            }
-            __syncthreads();
-        }
 }
 
 template <class UT>
@@ -243,14 +225,23 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds
     unsigned c_el = 0, c_ex = 0, c_iz = 0;
     double qtcr = 0.0;
     if (eC > 0 && nc > 0) {
-        int np2 = 1;
-        while (np2 < eC) np2 <<= 1;
-        for (int k = threadIdx.x; k < np2; k += blockDim.x) {
+        // val[r] = the electron of id rank r: each key (id, cell-local index) is distinct and
+        // its rank is the number of smaller keys, counted eight keys per LDS read
+        const int n8 = (eC + 7) & ~7;
+        for (int k = threadIdx.x; k < n8; k += blockDim.x)
             key[k] = (k < eC) ? (((uint64_t)a.e.id[a0 + k] << 32) | (uint32_t)k) : ~0ull;
-            val[k] = (k < eC) ? a0 + k : -1;
+        __syncthreads();
+        for (int k = threadIdx.x; k < eC; k += blockDim.x) {
+            const uint64_t kk = key[k];
+            int r = 0;
+            for (int j = 0; j < n8; j += 8) {
+                const uint64_t* q = key + j;
+                r += (int)(q[0] < kk) + (int)(q[1] < kk) + (int)(q[2] < kk) + (int)(q[3] < kk) + (int)(q[4] < kk) +
+                     (int)(q[5] < kk) + (int)(q[6] < kk) + (int)(q[7] < kk);
+            }
+            val[r] = a0 + k;
         }
         __syncthreads();
-        bitonic_sort_u64(key, val, np2);
         for (int base = 0; base < nc; base += 64) {
             const int k = base + (int)threadIdx.x;
             int pick = -1;
