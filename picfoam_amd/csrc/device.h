@@ -29,6 +29,9 @@ __device__ __forceinline__ vec3<T> cross(vec3<T> a, vec3<T> b) {
 }
 template <class T>
 __device__ __forceinline__ T mag(vec3<T> a) { return sqrt(dot(a, a)); }
+// 1/sqrt(x) in the precision of x
+__device__ __forceinline__ float rsq(float x) { return rsqrtf(x); }
+__device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
 
 // FP64 elementary functions for the collision kernels, accurate to a few ulp over the
 // arguments those kernels pass, in about a third of the instructions of the general

@@ -10,74 +10,76 @@ namespace picmi {
 // ---------------------------------------------------------------- pushers
 // BorisPusher.C:69-112, BorisNRPusher.C:69-99, VayPusher.C:54-100, HigueraCary.C:69-115.
 // k = q*(dt/2)/m and kv = q*dt/m are precomputed on the host in FP64.
+// Divisions by a scalar are multiplications by its reciprocal (1/sqrt as rsqrt): a few ulp
+// from the reference's operation order, a fraction of the instructions.
 template <class T, int P>
 __device__ __forceinline__ vec3<T> push_u(vec3<T> U, vec3<T> E, vec3<T> B, T k, T kv, bool rotate, bool hc_ref) {
-    const T c = (T)kC;
+    const T ic = (T)(1.0 / kC);
     if constexpr (P == kPushBoris) {
-        T mu = mag(U) / c;
-        vec3<T> rU = U / sqrt((T)1 - mu * mu);
+        T mu = mag(U) * ic;
+        vec3<T> rU = rsq((T)1 - mu * mu) * U;
         vec3<T> oldrU = rU;
         vec3<T> rUsm = k * E;
         rU = rU + rUsm;
         if (rotate) {
-            T a = mag(oldrU) / c, b = mag(rU) / c;
-            T lF = (sqrt((T)1 + a * a) + sqrt((T)1 + b * b)) / (T)2;
-            vec3<T> t = (k * B) / lF;
+            T a = mag(oldrU) * ic, b = mag(rU) * ic;
+            T lF = (sqrt((T)1 + a * a) + sqrt((T)1 + b * b)) * (T)0.5;
+            vec3<T> t = (k / lF) * B;
             vec3<T> Ud = rU + cross(rU, t);
-            rU = rU + cross(Ud, ((T)2 * t) / ((T)1 + dot(t, t)));
+            rU = rU + cross(Ud, ((T)2 / ((T)1 + dot(t, t))) * t);
         }
         rU = rU + rUsm;
-        T g = mag(rU) / c;
-        return rU / sqrt((T)1 + g * g);
+        T g = mag(rU) * ic;
+        return rsq((T)1 + g * g) * rU;
     } else if constexpr (P == kPushBorisNR) {
         U = U + k * E;
         if (rotate) {
             vec3<T> t = k * B;
             vec3<T> Ud = U + cross(U, t);
-            U = U + cross(Ud, ((T)2 * t) / ((T)1 + dot(t, t)));
+            U = U + cross(Ud, ((T)2 / ((T)1 + dot(t, t))) * t);
         }
         return U + k * E;
     } else if constexpr (P == kPushVay) {
-        T mu = mag(U) / c;
-        T lF = (T)1 / sqrt((T)1 - mu * mu);
+        T mu = mag(U) * ic;
+        T lF = rsq((T)1 - mu * mu);
         vec3<T> rU = lF * U;
         vec3<T> Ud = rU + kv * E;
         vec3<T> Tv = k * B;
         Ud = Ud + cross(U, Tv);
-        T ud = mag(Ud) / c;
+        T ud = mag(Ud) * ic;
         T alpha = (T)1 + ud * ud;
         T T2 = dot(Tv, Tv);
         T s = alpha - T2;
-        T w = dot(Ud, Tv) / c;
+        T w = dot(Ud, Tv) * ic;
         T us2 = w * w;
-        alpha = (T)1 / sqrt((T)0.5 * (s + sqrt(s * s + (T)4 * (T2 + us2))));
+        alpha = rsq((T)0.5 * (s + sqrt(s * s + (T)4 * (T2 + us2))));
         Tv = alpha * Tv;
         s = (T)1 / ((T)1 + dot(Tv, Tv));
         alpha = dot(Ud, Tv);
         rU = s * ((Ud + alpha * Tv) + cross(Ud, Tv));
-        T g = mag(rU) / c;
-        return rU / sqrt((T)1 + g * g);
+        T g = mag(rU) * ic;
+        return rsq((T)1 + g * g) * rU;
     } else if constexpr (P == kPushHC) {
-        T mu = mag(U) / c;
-        vec3<T> rU = U / sqrt((T)1 - mu * mu);
+        T mu = mag(U) * ic;
+        vec3<T> rU = rsq((T)1 - mu * mu) * U;
         vec3<T> rUsm = k * E;
         rU = rU + rUsm;
         if (rotate) {
-            T g = mag(rU) / c;
+            T g = mag(rU) * ic;
             T gfm2 = (T)1 + g * g;
             vec3<T> t = k * B;
             T beta2 = dot(t, t);
             // published Higuera-Cary: (t.u/c)^2; the reference writes (t.u)^2
-            T tu = hc_ref ? dot(t, rU) : dot(t, rU) / c;
+            T tu = hc_ref ? dot(t, rU) : dot(t, rU) * ic;
             T sg = gfm2 - beta2;
-            T iG = (T)1 / sqrt((T)0.5 * (sg + sqrt(sg * sg + (T)4 * (beta2 + tu * tu))));
+            T iG = rsq((T)0.5 * (sg + sqrt(sg * sg + (T)4 * (beta2 + tu * tu))));
             t = iG * t;
             vec3<T> Ud = rU + cross(rU, t);
-            rU = rU + cross(Ud, ((T)2 * t) / ((T)1 + dot(t, t)));
+            rU = rU + cross(Ud, ((T)2 / ((T)1 + dot(t, t))) * t);
         }
         rU = rU + rUsm;
-        T g = mag(rU) / c;
-        return rU / sqrt((T)1 + g * g);
+        T g = mag(rU) * ic;
+        return rsq((T)1 + g * g) * rU;
     } else {
         return U;
     }
