@@ -70,6 +70,7 @@ struct StageK {
     // fQ = mP/(mP+mQ), b0c = qP qQ / (2 pi eps0 mu), hbc = h / (2 mu)
     double cst, fP, fQ, b0c, hbc;
     int kind, n, units, stage;
+    int heavy, hw, lw;       // the first `heavy` units hold hw collisions each, the others lw
     int g1_units, g1_lmax, g1_stride, g1_xo, g1_yo, g1_ko;
     int g2_lmax, g2_stride, g2_xo, g2_yo, g2_ko;
 };
@@ -355,13 +356,17 @@ __device__ __forceinline__ void stage_intra(StageK& d, const CoulombArgs& a, int
     d.stage = stage;
     const double n = cnt * a.weight[spc] / Vc;
     d.nP = d.nQ = n;
+    d.lw = 1;
     if (cnt % 2 == 0) {
         d.kind = kIntraEven; d.units = cnt / 2; d.nPQ = n;
+        d.heavy = 0; d.hw = 1;
     } else if (!ny) {
         d.kind = kIntraOddTA; d.units = 1 + (cnt - 3) / 2; d.nPQ = (cnt + 3.0) * a.weight[spc] / Vc;
+        d.heavy = 1; d.hw = 3;
     } else {
         d.kind = kIntraOddNY; d.units = (cnt - 1) / 2;
         d.nPQ = 2.0 * a.weight[spc] * ((cnt - 1) / 2 + 1) / Vc;
+        d.heavy = 1; d.hw = 2;
     }
     finish_stage(d, a.dt, a.pair[spc * kMaxSpecies + spc]);
 }
@@ -376,6 +381,7 @@ __device__ __forceinline__ bool stage_inter(StageK& d, const CoulombArgs& a, int
     bool jP = false;
     if (na == nb) {
         d.kind = kInterEqual; d.n = na; d.units = na;
+        d.heavy = 0; d.hw = d.lw = 1;
         d.permP = permA; d.permQ = permB;
         d.mP = a.mass[i]; d.qP = a.charge[i]; d.wP = wi;
         d.mQ = a.mass[j]; d.qQ = a.charge[j]; d.wQ = wj;
@@ -402,6 +408,7 @@ __device__ __forceinline__ bool stage_inter(StageK& d, const CoulombArgs& a, int
             d.g1_xo = 0; d.g1_yo = 0; d.g1_ko = 0;
             d.units = nMin;
             d.g2_lmax = 0; d.g2_stride = 1; d.g2_xo = 0; d.g2_yo = 0; d.g2_ko = 0;
+            d.heavy = nMax % nMin; d.lw = nMax / nMin; d.hw = d.lw + 1;
         } else {
             // TakizukaAbePairing.C:197-308: two groups
             d.nPQ = d.nP;
@@ -418,6 +425,7 @@ __device__ __forceinline__ bool stage_inter(StageK& d, const CoulombArgs& a, int
             d.g2_lmax = nMaxG2; d.g2_stride = cyc;
             d.g2_xo = nMaxG1; d.g2_yo = nMinG1; d.g2_ko = nMaxG1;
             d.units = d.g1_units + (nMaxG2 > 0 ? cyc : 0);
+            d.heavy = d.g1_units; d.lw = delta; d.hw = delta + 1;
         }
     }
     const int sP = jP ? j : i, sQ = jP ? i : j;
@@ -752,6 +760,25 @@ __device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, 
     __syncthreads();
 }
 
+// The unit run by a lane's slot s of a round of `total` units on L lanes (lane s mod L).
+// Units are independent, so any order gives the same result; the units with extra
+// collisions (the first `heavy` of a cyclic stage, the odd intra units 0) are moved onto
+// lanes that have no unit in the round's last, partial pass.
+__device__ __forceinline__ int balance_slot(int s, int total, int L, int nd, const StageK* sk, int units0) {
+    const int m = total % L;
+    if (m == 0) return s;
+    const StageK& d0 = sk[0];
+    if (d0.kind == kCyclic) {
+        const int H = d0.heavy;
+        const int span = m + H;
+        return (H > 0 && span <= total && s < span) ? (s + H) % span : s;
+    }
+    int u = s;
+    if (d0.heavy > 0 && L - 1 < total) u = u == 0 ? L - 1 : (u == L - 1 ? 0 : u);
+    if (nd > 1 && sk[1].heavy > 0 && L - 2 < total) u = u == units0 ? L - 2 : (u == L - 2 ? units0 : u);
+    return u;
+}
+
 template <class UT, int MODEL, bool WC, int L>
 __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
     constexpr int W = 64 / L;
@@ -810,16 +837,26 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
                 nd = go ? 1 : 0;
             }
             __syncthreads();
+            // one collision per iteration: a lane moves on to its next unit (slots r, r + L,
+            // ...) inside the same loop, so the wave runs as many iterations as its busiest
+            // lane has collisions
             const int units0 = nd > 0 ? sk[0].units : 0;
             const int total = units0 + (nd > 1 ? sk[1].units : 0);
-            for (int u = r; u < total; u += L) {
+            int slot = r, t = 0;
+            int u = slot < total ? balance_slot(slot, total, L, nd, sk, units0) : 0;
+            while (slot < total) {
                 const StageK& d = sk[u < units0 ? 0 : 1];
                 const int uu = u < units0 ? u : u - units0;
                 int iP, iQ, k;
-                for (int t = 0; decode(d, valA, uu, t, iP, iQ, k); ++t) {
+                if (decode(d, valA, uu, t, iP, iQ, k)) {
                     collide<UT, MODEL, WC, false>(a, d, cell, iP, iQ, k, 0.0, (UT*)nullptr, 0);
                     ++ncoll;
+                    ++t;
+                    if (decode(d, valA, uu, t, iP, iQ, k)) continue;
                 }
+                slot += L;
+                t = 0;
+                if (slot < total) u = balance_slot(slot, total, L, nd, sk, units0);
             }
             __syncthreads();
         }
