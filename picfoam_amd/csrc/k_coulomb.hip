@@ -435,6 +435,25 @@ __device__ __forceinline__ bool stage_inter(StageK& d, const CoulombArgs& a, int
     return jP;
 }
 
+// The unit run by a lane's slot s of a round of `total` units on L lanes (lane s mod L).
+// Units are independent, so any order gives the same result; the units with extra
+// collisions (the first `heavy` of a cyclic stage, the odd intra units 0) are moved onto
+// lanes that have no unit in the round's last, partial pass.
+__device__ __forceinline__ int balance_slot(int s, int total, int L, int nd, const StageK* sk, int units0) {
+    const int m = total % L;
+    if (m == 0) return s;
+    const StageK& d0 = sk[0];
+    if (d0.kind == kCyclic) {
+        const int H = d0.heavy;
+        const int span = m + H;
+        return (H > 0 && span <= total && s < span) ? (s + H) % span : s;
+    }
+    int u = s;
+    if (d0.heavy > 0 && L - 1 < total) u = u == 0 ? L - 1 : (u == L - 1 ? 0 : u);
+    if (nd > 1 && sk[1].heavy > 0 && L - 2 < total) u = u == units0 ? L - 2 : (u == L - 2 ? units0 : u);
+    return u;
+}
+
 // every pairing stage of one cell by one wave; maxp = LDS slots per index list.  With CU the
 // velocities of the (at most two) species a stage needs are held in LDS for the whole cell:
 // read once and written back once, coalesced, instead of gathered per collision.
@@ -620,16 +639,25 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
             }
             if (nd == 0) continue;
             __syncthreads();
+            // one collision per loop iteration, as in coulomb_group
             const int units0 = sk[0].units;
             const int total = units0 + (nd > 1 ? sk[1].units : 0);
-            for (int u = threadIdx.x; u < total; u += blockDim.x) {
+            const int L = blockDim.x;
+            int slot = threadIdx.x, t = 0;
+            int u = slot < total ? balance_slot(slot, total, L, nd, sk, units0) : 0;
+            while (slot < total) {
                 const StageK& d = sk[u < units0 ? 0 : 1];
                 const int uu = u < units0 ? u : u - units0;
                 int iP, iQ, k;
-                for (int t = 0; decode(d, valA, uu, t, iP, iQ, k); ++t) {
+                if (decode(d, valA, uu, t, iP, iQ, k)) {
                     collide<UT, MODEL, WC, CU>(a, d, cell, iP, iQ, k, debye, cache, maxp);
                     ++ncoll;
+                    ++t;
+                    if (decode(d, valA, uu, t, iP, iQ, k)) continue;
                 }
+                slot += L;
+                t = 0;
+                if (slot < total) u = balance_slot(slot, total, L, nd, sk, units0);
             }
             __syncthreads();
         }
@@ -760,24 +788,6 @@ __device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, 
     __syncthreads();
 }
 
-// The unit run by a lane's slot s of a round of `total` units on L lanes (lane s mod L).
-// Units are independent, so any order gives the same result; the units with extra
-// collisions (the first `heavy` of a cyclic stage, the odd intra units 0) are moved onto
-// lanes that have no unit in the round's last, partial pass.
-__device__ __forceinline__ int balance_slot(int s, int total, int L, int nd, const StageK* sk, int units0) {
-    const int m = total % L;
-    if (m == 0) return s;
-    const StageK& d0 = sk[0];
-    if (d0.kind == kCyclic) {
-        const int H = d0.heavy;
-        const int span = m + H;
-        return (H > 0 && span <= total && s < span) ? (s + H) % span : s;
-    }
-    int u = s;
-    if (d0.heavy > 0 && L - 1 < total) u = u == 0 ? L - 1 : (u == L - 1 ? 0 : u);
-    if (nd > 1 && sk[1].heavy > 0 && L - 2 < total) u = u == units0 ? L - 2 : (u == L - 2 ? units0 : u);
-    return u;
-}
 
 template <class UT, int MODEL, bool WC, int L>
 __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
