@@ -181,6 +181,9 @@ template <class UT, int P, int W, int DEP>
 __device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block) {
     const int64_t i = block * blockDim.x + threadIdx.x;
     const int64_t n = a.n_live ? min(*a.n_live, a.n) : a.n;
+    // blocks wholly past the live count (the launch bound can be a species' capacity while
+    // created parcels are possible) leave at once
+    if (block * blockDim.x >= n) return;
     int newcell = -1;
     double y1 = 0, y2 = 0, y3 = 0;
     int t = -1;
