@@ -498,3 +498,54 @@ def test_pcg_iterations_match_oracle(oracle, case):
     if bc is None:
         phi_g = phi_g - phi_g.mean(); phi_r = phi_r - phi_r.mean()
     assert rel_err(phi_g, phi_r) < 1e-7
+
+
+@pytest.mark.parametrize("lnl", [10.0, 0.0], ids=["fixed-lnL", "debye-lnL"])
+def test_coulomb_three_charged_species(oracle, lnl):
+    """Electrons, two ion species and a neutral species: six pairing stages per cell in the
+    reference order, the neutral species skipped.  With fixed logarithms the cells run four
+    to a wave; with the Debye logarithm one cell per wave, where the two velocity slots in
+    LDS must evict and reload species between stages.  Equal to the oracle to 1e-12."""
+    say("coulomb three species", lnl)
+    rng = np.random.default_rng(131)
+    m = small_hex(n=(3, 2, 2), L=(1e-4, 1e-4, 1e-4))
+    rm = oracle.RefMesh(m)
+    nc = rm.n_cells
+    tc, _, _ = rm.tets()
+    first_tet = np.array([np.nonzero(tc == c)[0][0] for c in range(rm.n_cells)], np.int32)
+    masses = [ME, 1836 * ME, 4 * 1836 * ME, 40 * 1836 * ME]
+    charges = [-QE, QE, 2 * QE, 0.0]
+    weights = [1e6, 1e6, 1e6, 1e6]              # Takizuka-Abe needs equal weights
+    counts = [rng.integers(0, 70, nc) for _ in range(4)]
+    counts[0][0], counts[1][0], counts[2][0] = 5, 0, 9
+    counts[0][1], counts[1][1], counts[2][1] = 64, 63, 1
+    species = [{"mass": masses[s], "charge": charges[s], "nEquivalentParticles": weights[s], "capacity": 5000}
+               for s in range(4)]
+    cl = {} if lnl == 0 else {(i, j): lnl for i in range(3) for j in range(i, 3)}
+    c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu", "PairingAlgorithm": "TakizukaAbe",
+                  "coulombLog": cl, "seed": 91}, species)
+    Us, ids, css = [], [], []
+    for s in range(4):
+        tet = np.concatenate([np.full(k, first_tet[ci], np.int32) for ci, k in enumerate(counts[s])])
+        n = len(tet)
+        y = random_bary(rng, n)
+        T = (10.0 if s == 0 else 1.0) * QE
+        U = rng.normal(size=(n, 3)) * np.sqrt(T / masses[s])
+        ident = (np.arange(n) + 1000 * s).astype(np.uint32)
+        c.set_particles(s, y, tet, U, ident)
+        Us.append(U); ids.append(ident)
+        css.append(np.concatenate([[0], np.cumsum(counts[s])]).astype(np.int32))
+    c.step_index = 3
+    dt = 1e-9
+    c.collide_coulomb(dt)
+    clm = np.zeros((8, 8))
+    if lnl:
+        clm[:3, :3] = lnl
+    ref, _ = oracle.coulomb(rm, [(masses[s], charges[s], weights[s]) for s in range(4)], Us, ids, css, dt, seed=91,
+                            step=3, coulomb_log=clm)
+    for s in range(4):
+        _, _, Ug, ig = c.get_particles(s)
+        (Ug_s,) = sort_by_id(ig, Ug)
+        assert rel_err(Ug_s, ref[s]) < TOL, s
+    assert np.array_equal(ref[3], Us[3])            # the neutral species is untouched
+    assert not np.array_equal(ref[2], Us[2])
