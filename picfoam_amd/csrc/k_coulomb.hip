@@ -127,6 +127,22 @@ __device__ __forceinline__ bool decode(const StageK& d, const int* lists, int u,
     }
 }
 
+// number of collisions of unit u (the t for which decode returns true)
+__device__ __forceinline__ int unit_count(const StageK& d, int u) {
+    switch (d.kind) {
+        case kIntraOddTA: return u == 0 ? 3 : 1;
+        case kIntraOddNY: return u == 0 ? 2 : 1;
+        case kIntraEven:
+        case kInterEqual: return 1;
+        default: {
+            const bool g1 = u < d.g1_units;
+            const int m = g1 ? u : u - d.g1_units;
+            const int lmax = g1 ? d.g1_lmax : d.g2_lmax, stride = g1 ? d.g1_stride : d.g2_stride;
+            return lmax > m ? (lmax - m + stride - 1) / stride : 0;
+        }
+    }
+}
+
 // NanbuCollision::collide (NanbuCollision.C:60-134) on two velocities, arithmetic M
 template <class M>
 __device__ __forceinline__ void nanbu_scatter(vec3<M>& uP, vec3<M>& uQ, const StageK& d, int calc_bmin, double debye,
@@ -643,21 +659,27 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
             const int units0 = sk[0].units;
             const int total = units0 + (nd > 1 ? sk[1].units : 0);
             const int L = blockDim.x;
-            int slot = threadIdx.x, t = 0;
-            int u = slot < total ? balance_slot(slot, total, L, nd, sk, units0) : 0;
+            int slot = threadIdx.x, t = 0, u = 0, cnt = 0;
+            if (slot < total) {
+                u = balance_slot(slot, total, L, nd, sk, units0);
+                cnt = unit_count(sk[u < units0 ? 0 : 1], u < units0 ? u : u - units0);
+            }
             while (slot < total) {
                 const StageK& d = sk[u < units0 ? 0 : 1];
                 const int uu = u < units0 ? u : u - units0;
-                int iP, iQ, k;
-                if (decode(d, valA, uu, t, iP, iQ, k)) {
+                if (t < cnt) {
+                    int iP, iQ, k;
+                    decode(d, valA, uu, t, iP, iQ, k);
                     collide<UT, MODEL, WC, CU>(a, d, cell, iP, iQ, k, debye, cache, maxp);
                     ++ncoll;
-                    ++t;
-                    if (decode(d, valA, uu, t, iP, iQ, k)) continue;
+                    if (++t < cnt) continue;
                 }
                 slot += L;
                 t = 0;
-                if (slot < total) u = balance_slot(slot, total, L, nd, sk, units0);
+                if (slot < total) {
+                    u = balance_slot(slot, total, L, nd, sk, units0);
+                    cnt = unit_count(sk[u < units0 ? 0 : 1], u < units0 ? u : u - units0);
+                }
             }
             __syncthreads();
         }
@@ -852,21 +874,27 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
             // lane has collisions
             const int units0 = nd > 0 ? sk[0].units : 0;
             const int total = units0 + (nd > 1 ? sk[1].units : 0);
-            int slot = r, t = 0;
-            int u = slot < total ? balance_slot(slot, total, L, nd, sk, units0) : 0;
+            int slot = r, t = 0, u = 0, cnt = 0;
+            if (slot < total) {
+                u = balance_slot(slot, total, L, nd, sk, units0);
+                cnt = unit_count(sk[u < units0 ? 0 : 1], u < units0 ? u : u - units0);
+            }
             while (slot < total) {
                 const StageK& d = sk[u < units0 ? 0 : 1];
                 const int uu = u < units0 ? u : u - units0;
-                int iP, iQ, k;
-                if (decode(d, valA, uu, t, iP, iQ, k)) {
+                if (t < cnt) {
+                    int iP, iQ, k;
+                    decode(d, valA, uu, t, iP, iQ, k);
                     collide<UT, MODEL, WC, false>(a, d, cell, iP, iQ, k, 0.0, (UT*)nullptr, 0);
                     ++ncoll;
-                    ++t;
-                    if (decode(d, valA, uu, t, iP, iQ, k)) continue;
+                    if (++t < cnt) continue;
                 }
                 slot += L;
                 t = 0;
-                if (slot < total) u = balance_slot(slot, total, L, nd, sk, units0);
+                if (slot < total) {
+                    u = balance_slot(slot, total, L, nd, sk, units0);
+                    cnt = unit_count(sk[u < units0 ? 0 : 1], u < units0 ? u : u - units0);
+                }
             }
             __syncthreads();
         }
