@@ -51,53 +51,69 @@ def state_bytes(precision):
     return 3 * 8 + 3 * u + 4 + 4
 
 
-def cpu_baseline(ncell_sample=8, ppc=64, target_s=12.0):
-    """The oracle (FP64 C restatement, one core) on a bounded sample of the same workload:
-    ncell_sample^3 cells of the same size, same ppc, dt and models."""
+def cpu_baseline(ncell_sample=8, ppc=64, target_s=10.0):
+    """The oracle (FP64 C restatement) on a bounded sample of the same workload:
+    ncell_sample^3 cells of the same size, same ppc, dt and models.  Timed twice: on every
+    host thread this process may use (OpenMP over the move's parcels and the Coulomb
+    cells; `value`) and on one thread (`value_1core`).  Its results do not depend on the
+    thread count."""
     from oracle import picref
     from picfoam_amd import cases
     mesh, species, models, ex = cases.thermalisation(ncell=ncell_sample, ppc=ppc)
     rm = picref.RefMesh(mesh)
-    rng = np.random.default_rng(1)
     nc = rm.n_cells
     tc, _, _ = rm.tets()
     starts = np.searchsorted(tc, np.arange(nc))
     dt = ex["dt"]
-    parts = []
-    for s, T in ((0, ex["Te_K"]), (1, ex["Ti_K"])):
-        n = nc * ppc
-        tet = (starts.repeat(ppc) + rng.integers(0, 12, n)).astype(np.int32)
-        e = -np.log(1 - rng.random((n, 4)))
-        y = np.ascontiguousarray((e / e.sum(1, keepdims=True))[:, 1:])
-        U = rng.normal(size=(n, 3)) * np.sqrt(cases.KB * T / species[s]["mass"])
-        parts.append([y, tet, U, np.arange(n, dtype=np.uint32)])
     sp = [(species[s]["mass"], species[s]["charge"], species[s]["nEquivalentParticles"]) for s in range(2)]
     cl = np.zeros((8, 8)); cl[:2, :2] = 10.0
     zeros = np.zeros((nc, 3))
     Q0 = abs(sp[0][1] * sp[0][2])
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        for s in range(2):
-            y, tet, U, ids = parts[s]
-            y, U, tet, _, _ = rm.move("Boris", "CellAverage", y, U, tet, sp[s][1], sp[s][0], dt, zeros, None, zeros)
-            order, cs = rm.sort_by_cell(tet)
-            parts[s] = [y[order], tet[order], U[order], ids[order], cs]
-        Us, _ = picref.coulomb(rm, sp, [parts[0][2], parts[1][2]], [parts[0][3], parts[1][3]],
-                               [parts[0][4], parts[1][4]], dt, seed=1, step=steps, coulomb_log=cl)
-        for s in range(2):
-            parts[s][2] = Us[s]
-        rm.deposit("CellAverage", [(parts[s][0], parts[s][1], sp[s][1] * sp[s][2]) for s in range(2)], Q0)
-        for s in range(2):
-            parts[s] = parts[s][:4]
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= target_s:
-            break
     n_total = 2 * nc * ppc
-    return {"value": n_total * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle, {ncell_sample}^3 cells x {ppc}+{ppc} ppc ({n_total} parcels), {steps} steps "
-                      f"in {el:.1f} s, same dt/models as the GPU workload"}
+
+    def run(threads):
+        picref.lib().ref_set_threads(threads)
+        rng = np.random.default_rng(1)
+        parts = []
+        for s, T in ((0, ex["Te_K"]), (1, ex["Ti_K"])):
+            n = nc * ppc
+            tet = (starts.repeat(ppc) + rng.integers(0, 12, n)).astype(np.int32)
+            e = -np.log(1 - rng.random((n, 4)))
+            y = np.ascontiguousarray((e / e.sum(1, keepdims=True))[:, 1:])
+            U = rng.normal(size=(n, 3)) * np.sqrt(cases.KB * T / species[s]["mass"])
+            parts.append([y, tet, U, np.arange(n, dtype=np.uint32)])
+        steps = 0
+        t0 = time.perf_counter()
+        while True:
+            for s in range(2):
+                y, tet, U, ids = parts[s]
+                y, U, tet, _, _ = rm.move("Boris", "CellAverage", y, U, tet, sp[s][1], sp[s][0], dt, zeros, None, zeros)
+                order, cs = rm.sort_by_cell(tet)
+                parts[s] = [y[order], tet[order], U[order], ids[order], cs]
+            Us, _ = picref.coulomb(rm, sp, [parts[0][2], parts[1][2]], [parts[0][3], parts[1][3]],
+                                   [parts[0][4], parts[1][4]], dt, seed=1, step=steps, coulomb_log=cl)
+            for s in range(2):
+                parts[s][2] = Us[s]
+            rm.deposit("CellAverage", [(parts[s][0], parts[s][1], sp[s][1] * sp[s][2]) for s in range(2)], Q0)
+            for s in range(2):
+                parts[s] = parts[s][:4]
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= target_s:
+                return steps, el
+
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS") or avail), avail))
+    steps_n, el_n = run(threads)
+    steps_1, el_1 = run(1)
+    return {"value": n_total * steps_n / el_n, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "value_1core": n_total * steps_1 / el_1,
+            "sample": f"FP64 oracle, {ncell_sample}^3 cells x {ppc}+{ppc} ppc ({n_total} parcels), same dt/models "
+                      f"as the GPU workload: {steps_n} steps in {el_n:.1f} s on {threads} threads (OpenMP over "
+                      f"move parcels and Coulomb cells), {steps_1} steps in {el_1:.1f} s on 1 thread"}
 
 
 def copy_calibration(mib):

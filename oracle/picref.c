@@ -23,6 +23,9 @@
  *   OpenFOAM-9 Random/shuffle streams, tracking tie-breaks, volPointInterpolation at
  *   non-coupled boundary points ................... parity unpinned (not reproducible offline)
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "picref.h"
 
 #include <math.h>
@@ -632,6 +635,8 @@ int ref_move(const ref_mesh* m, int pusher, int weighting, int n, double* y, dou
              double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell, int do_push,
              int hc_reference_form, int rotate, int* n_superluminal) {
     int guards = 0, sl = 0;
+    /* parcels are independent: spread over the host threads for the CPU baseline */
+#pragma omp parallel for schedule(static) reduction(+ : guards, sl)
     for (int i = 0; i < n; ++i) {
         if (tet[i] < 0) continue;
         if (do_push && q != 0.0) {
@@ -995,20 +1000,26 @@ void ref_coulomb(const ref_mesh* m, const ref_coulomb_params* p, double dt, doub
     int maxn = 1;
     for (int s = 0; s < ns; ++s)
         for (int c = 0; c < m->n_cells; ++c) if (cs[s][c + 1] - cs[s][c] > maxn) maxn = cs[s][c + 1] - cs[s][c];
-    int* pa = (int*)malloc(sizeof(int) * maxn);
-    int* pb = (int*)malloc(sizeof(int) * maxn);
     int calc_debye = 0;
     for (int i = 0; i < ns; ++i)
         for (int j = i; j < ns; ++j)
             if (p->charge[i] != 0.0 && p->charge[j] != 0.0 && p->coulomb_log[i][j] <= 0.0) calc_debye = 1;
     const int ny = p->pairing == 1;
+    /* cells are independent: spread over the host threads for the CPU baseline, each
+     * thread with its own lists and counters (the counters are summed at the end) */
+#pragma omp parallel
+    {
+    int* pa = (int*)malloc(sizeof(int) * maxn);
+    int* pb = (int*)malloc(sizeof(int) * maxn);
+    double lst[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     coll_ctx x;
-    x.p = p; x.U = U; x.dt = dt; x.stats = stats;
+    x.p = p; x.U = U; x.dt = dt; x.stats = stats ? lst : NULL;
+#pragma omp for schedule(dynamic, 16)
     for (int c = 0; c < m->n_cells; ++c) {
         double Vc = m->V[c];
         x.cell = c;
         x.debye = calc_debye ? debye_length(p, U, cs, c, Vc) : 0.0;
-        if (stats) stats[3] += x.debye;
+        if (stats) lst[3] += x.debye;
         int stage = 0;
         for (int i = 0; i < ns; ++i) {
             if (p->charge[i] == 0.0) continue;
@@ -1091,6 +1102,20 @@ void ref_coulomb(const ref_mesh* m, const ref_coulomb_params* p, double dt, doub
         }
     }
     free(pa); free(pb);
+    if (stats) {
+#pragma omp critical
+        for (int k = 0; k < 8; ++k) stats[k] += lst[k];
+    }
+    }
+}
+
+/* number of host threads the oracle's parallel loops use (CPU baseline timing) */
+void ref_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
 }
 
 /* ------------------------------------------------------------------ MCC */

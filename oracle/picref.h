@@ -100,6 +100,7 @@ typedef struct {
     int pairing;
     int weight_correction;
 } ref_coulomb_params;
+void ref_set_threads(int n);
 void ref_coulomb(const ref_mesh* m, const ref_coulomb_params* p, double dt, double* const* U,
                  const uint32_t* const* id, const int* const* cell_start, double* stats);
 
