@@ -146,6 +146,7 @@ def run_steps(args, precision, steps, warmup, profile, sync_steps=0):
     calls (synchronous, one round trip each) for comparison."""
     from picfoam_amd import cases, picmi
     mesh, species, models, ex = cases.thermalisation(ncell=args.ncell, ppc=args.ppc, precision=precision)
+    models["sortInterval"] = args.sort_interval
     cloud = picmi.Cloud(mesh, species, models)
     cloud.init_maxwellian(0, args.ppc, ex["Te_K"], id0=0, seed=11)
     cloud.init_maxwellian(1, args.ppc, ex["Ti_K"], id0=0, seed=12)
@@ -236,7 +237,7 @@ def timed(cloud, dt, steps, warmup):
     return 1e3 * el / steps, {k: round(v[0] / steps, 4) for k, v in stages.items() if v[1]}, st
 
 
-def extra_workloads(names, steps, warmup):
+def extra_workloads(names, steps, warmup, sort_interval=1):
     """The other full-size configurations, measured the same way (pic_run, device-resident
     inputs): the magnetised electron push on the jittered 50^3x6 tet lattice (50M electrons,
     Higuera-Cary, FP32 push / FP64 track) and the argon-like MCC on a 100^3 hex box (100M
@@ -259,6 +260,7 @@ def extra_workloads(names, steps, warmup):
     if "c4" in names:
         print("bench: argon MCC, 100^3 hex, 100M electrons", file=sys.stderr, flush=True)
         mesh, species, models, ex = cases.argon_mcc(ncell=100, ppc=100, precision="mixed")
+        models["sortInterval"] = sort_interval
         c = picmi.Cloud(mesh, species, models)
         e, el, exs, iz = ex["xs"]
         c.set_cross_section(0, e, el, 0.0)
@@ -286,6 +288,8 @@ def main():
     ap.add_argument("--ncell", type=int, default=32)
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--sort-interval", type=int, default=1,
+                    help="move the parcels into cell order every k steps (index sort in between)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
@@ -359,7 +363,8 @@ def main():
         out["poisson"] = poisson_benchmark(args.poisson_ncell, args.poisson_ppc, snaps=48, train_steps=400, evals=24,
                                            warm=4, loss_kind=args.poisson_loss)
     if args.workloads:
-        out["workloads"] = extra_workloads(args.workloads.split(","), steps=10, warmup=2)
+        out["workloads"] = extra_workloads(args.workloads.split(","), steps=10, warmup=2,
+                                           sort_interval=args.sort_interval)
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(target_s=args.cpu_seconds)

@@ -117,7 +117,8 @@ typedef struct {
     double poisson_tolerance;         /* ||r||2/||b||2 */
     int32_t poisson_max_iter;
     int32_t poisson_mode;             /* PIC_POISSON_* */
-    int32_t sort_interval;            /* sort every k steps when no collision model needs it */
+    int32_t sort_interval;            /* parcels move into cell order every k steps; a collision model
+                                         gets the cell order as an index on the steps between */
     int32_t poisson_guess;            /* PCG start: 0 the previous phi, 1 2 phi(n-1) - phi(n-2) */
 } pic_models;
 
@@ -176,6 +177,8 @@ int pic_particles_set(pic_ctx* ctx, int32_t species, int64_t n, const double* ba
 /* positions + containing cell -> (tet, bary) located on the device */
 int pic_particles_set_positions(pic_ctx* ctx, int32_t species, int64_t n, const double* x, const int32_t* cell,
                                 const double* U, const uint32_t* id);
+/* count and arrays as stored: between moving sorts (sort_interval > 1) they can hold
+ * parcels deleted since the last one, marked by tet -1 */
 int pic_particles_count(pic_ctx* ctx, int32_t species, int64_t* n);
 int pic_particles_get(pic_ctx* ctx, int32_t species, int64_t capacity, int64_t* n, double* bary, int32_t* tet,
                       double* U, uint32_t* id);

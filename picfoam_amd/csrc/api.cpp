@@ -39,6 +39,7 @@ struct Species {
     int* count = nullptr;   // [n_cells+1]
     int* start = nullptr;   // [n_cells+1]
     int2* place = nullptr;  // [capacity] (cell, slot) from the counting pass
+    int* perm = nullptr;    // [capacity] sorted slot -> array index after an index sort
 };
 
 struct StageEvent {
@@ -77,6 +78,7 @@ struct pic_ctx {
     bool e_zero = true;                      // E is zero in every cell (never set, no field solve)
     bool sorted = false, counts_valid = false;
     bool place_valid = false;                // Species::place matches the current layout
+    bool indexed = false;                    // the cell order is Species::perm, not the arrays' order
     bool big_n_zero = false;                 // d_big_n zeroed since its last use
     // species
     std::vector<Species> sp;
@@ -252,7 +254,8 @@ int alloc_species(pic_ctx* c, Species& s) {
     s.count = c->alloc<int>(nc + 1);
     s.start = c->alloc<int>(nc + 1);
     s.place = c->alloc<int2>(cap);
-    if (!s.count || !s.start || !s.place) return c->fail(PIC_ERR_HIP, "out of device memory (cell counts)");
+    s.perm = c->alloc<int>(cap);
+    if (!s.count || !s.start || !s.place || !s.perm) return c->fail(PIC_ERR_HIP, "out of device memory (cell counts)");
     return PIC_OK;
 }
 
@@ -329,6 +332,7 @@ int do_move(pic_ctx* c, double dt) {
     c->counts_valid = true;
     c->place_valid = true;
     c->sorted = false;
+    c->indexed = false;
     return PIC_OK;
 }
 
@@ -344,6 +348,7 @@ ScatterSet<UT> scatter_set(pic_ctx* c) {
         m.n_live[s] = c->d_n + s;
         m.place[s] = S.place;
         m.start[s] = S.start;
+        m.perm[s] = S.perm;
     }
     return m;
 }
@@ -351,7 +356,10 @@ ScatterSet<UT> scatter_set(pic_ctx* c) {
 // buildCellOccupancy (PICCloud.C:327-354): counting sort of every species by cell.  No
 // host round trip: the live count after the sort (start[n_cells]) stays on the device and
 // the host keeps S.n as an upper bound for launch sizes until the next synchronisation.
-int do_sort(pic_ctx* c) {
+// index_only: the parcels stay where they are and Species::perm lists them in cell order
+// (the reference's occupancy lists of parcel pointers, PICCloud.H:166); deleted parcels stay
+// in the arrays, marked by tet -1, until the next moving sort.
+int do_sort(pic_ctx* c, bool index_only = false) {
     pic_ctx::Scope sc(c, PIC_STAGE_SORT);
     const int nc = c->hm.n_cells;
     if (!c->counts_valid || !c->place_valid) {
@@ -373,12 +381,21 @@ int do_sort(pic_ctx* c) {
         ss.in[s] = c->sp[s].count; ss.out[s] = c->sp[s].start; ss.total[s] = c->d_n_next + s;
     }
     if (int r = check_hip(c, launch_scan_cells(ss, (int)c->sp.size(), c->stream), "scan")) return r;
+    if (index_only) {
+        const hipError_t e = c->mixed() ? launch_perm(scatter_set<float>(c), c->stream)
+                                        : launch_perm(scatter_set<double>(c), c->stream);
+        if (int r = check_hip(c, e, "index sort")) return r;
+        c->sorted = true;
+        c->indexed = true;
+        return PIC_OK;   // the layout is unchanged, so the slots stay valid
+    }
     const hipError_t e = c->mixed() ? launch_scatter(scatter_set<float>(c), c->stream)
                                     : launch_scatter(scatter_set<double>(c), c->stream);
     if (int r = check_hip(c, e, "scatter")) return r;
     if (int r = check_hip(c, launch_copy_counts(c->d_n_next, c->d_n, (int)c->sp.size(), c->stream), "counts")) return r;
     for (auto& S : c->sp) S.cur = 1 - S.cur;
     c->sorted = true;
+    c->indexed = false;
     c->place_valid = false;   // the slots refer to the layout before this sort
     return PIC_OK;
 }
@@ -401,6 +418,7 @@ int do_coulomb(pic_ctx* c, double dt) {
         a.charge[i] = c->sp[i].charge;
         a.weight[i] = c->sp[i].weight;
         a.start[i] = c->sp[i].start;
+        a.perm[i] = c->indexed ? c->sp[i].perm : nullptr;
         const int b = c->sp[i].cur;
         a.ux[i] = c->sp[i].u[b][0]; a.uy[i] = c->sp[i].u[b][1]; a.uz[i] = c->sp[i].u[b][2];
         a.id[i] = c->sp[i].id[b];
@@ -452,6 +470,7 @@ int do_mcc_t(pic_ctx* c, double dt) {
     a.n_cells = c->hm.n_cells;
     a.e = soa<UT>(E, E.cur);
     a.start = E.start;
+    a.perm = c->indexed ? E.perm : nullptr;
     a.n_e = c->d_n + c->electron;
     a.cap_e = E.capacity;
     if (c->ion >= 0) {
@@ -1087,6 +1106,7 @@ int pic_particles_set(pic_ctx* c, int32_t s, int64_t n, const double* bary, cons
     S.n = n;
     set_dev_n(c, s);
     c->sorted = false;
+    c->indexed = false;
     c->counts_valid = false;
     return check_hip(c, hipStreamSynchronize(c->stream), "particles set");
 }
@@ -1125,6 +1145,7 @@ int pic_particles_set_positions(pic_ctx* c, int32_t s, int64_t n, const double* 
     S.n = n;
     set_dev_n(c, s);
     c->sorted = false;
+    c->indexed = false;
     c->counts_valid = false;
     if (fails) return c->fail(PIC_ERR_INVALID, std::to_string(fails) + " positions lie outside their given cell");
     return check_hip(c, hipStreamSynchronize(c->stream), "particles set positions");
@@ -1200,6 +1221,7 @@ int pic_init_maxwellian(pic_ctx* c, int32_t s, int32_t nppc, double T, const dou
     S.n += add;
     set_dev_n(c, s);
     c->sorted = false;
+    c->indexed = false;
     c->counts_valid = false;
     return check_hip(c, hipStreamSynchronize(c->stream), "init");
 }
@@ -1264,9 +1286,12 @@ int pic_poisson_solve(pic_ctx* c, pic_poisson_stats* st) {
 // round trip, except inside the Poisson solve (convergence checks) and GNN recording
 int step_body(pic_ctx* c, double dt, pic_poisson_stats* ps) {
     if (int r = do_move(c, dt)) return r;
-    const bool need_sort = c->coulomb || c->mcc || ((c->step + 1) % c->sort_interval == 0);
-    if (need_sort)
-        if (int r = do_sort(c)) return r;
+    // the parcels move into cell order every sort_interval steps; collision models need the
+    // cell order every step, and in between get it as an index
+    const bool move_sort = (c->step + 1) % c->sort_interval == 0;
+    const bool collide = c->coulomb || c->mcc;
+    if (move_sort || collide)
+        if (int r = do_sort(c, !move_sort)) return r;
     if (c->coulomb)
         if (int r = do_coulomb(c, dt)) return r;
     if (c->mcc)

@@ -164,6 +164,10 @@ struct PartSoA {
     uint32_t* id;
 };
 
+// array index of the parcel in sorted slot i: the arrays are in cell order themselves when
+// perm is null, otherwise perm holds the order (an index sort between moving sorts)
+__device__ __forceinline__ int slot_index(const int* perm, int i) { return perm ? perm[i] : i; }
+
 // Groups the wave's lanes by key (key < 0 lanes take no part) without touching memory:
 // each lane learns its group's leader lane and its rank in the group, each leader the group
 // size.  Must be called by all lanes of the wave.

@@ -283,16 +283,18 @@ __device__ __forceinline__ uint64_t shuffle_key(uint64_t seed, int64_t step, int
     return ((uint64_t)o.x << 24) | (o.y >> 8);
 }
 
-// val[0..n) = base + cell-local indices of the cell's parcels of one species ordered by their 56-bit
-// random key.  Up to 256 parcels the key carries the cell-local index in its low byte, so
-// keys are distinct and each parcel's place is the number of smaller keys (no barriers in
-// the count; eight keys per LDS round trip).  Larger cells use a bitonic sort.
+// val[0..n) = the cell's parcels of one species ordered by their 56-bit random key: base +
+// cell-local index, or the array index through perm (perm non-null: base is unused).  Up to
+// 256 parcels the key carries the cell-local index in its low byte, so keys are distinct and
+// each parcel's place is the number of smaller keys (no barriers in the count; eight keys
+// per LDS round trip).  Larger cells use a bitonic sort.
 __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
-                           uint64_t* key, int* val, int base) {
+                           uint64_t* key, int* val, int base, const int* perm) {
     if (n <= 256) {
         const int n8 = (n + 7) & ~7;
         for (int k = threadIdx.x; k < n8; k += blockDim.x)
-            key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k : ~0ull;
+            key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[slot_index(perm, first + k)]) << 8) | (uint64_t)k
+                           : ~0ull;
         __syncthreads();
         // rounds of up to 64 keys; a partial round of E keys gives each key G = 64 / E lanes
         // (rounded down to a power of two) that count over G slices of the list and add up
@@ -318,7 +320,7 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
                      (int)(x5 < kk) + (int)(x6 < kk) + (int)(x7 < kk);
             }
             for (int off = (1 << lg) >> 1; off > 0; off >>= 1) r += __shfl_xor(r, off);
-            if (act && g == 0) val[r] = base + k;
+            if (act && g == 0) val[r] = perm ? perm[first + k] : base + k;
         }
         __syncthreads();
         return;
@@ -327,8 +329,9 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
     while (np2 < n) np2 <<= 1;
     for (int k = threadIdx.x; k < np2; k += blockDim.x) {
         if (k < n) {
-            key[k] = (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)(k & 255);
-            val[k] = base + k;
+            const int j = slot_index(perm, first + k);
+            key[k] = (shuffle_key(seed, step, stage, species, id[j]) << 8) | (uint64_t)(k & 255);
+            val[k] = perm ? j : base + k;
         } else {
             key[k] = ~0ull;
             val[k] = -1;
@@ -497,8 +500,9 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
             int np2 = 1;
             while (np2 < n) np2 <<= 1;
             for (int k = threadIdx.x; k < np2; k += blockDim.x) {
-                keyA[k] = (k < n) ? (((uint64_t)a.id[s][f0 + k] << 32) | (uint32_t)k) : ~0ull;
-                valA[k] = (k < n) ? f0 + k : -1;
+                const int j = k < n ? slot_index(a.perm[s], f0 + k) : -1;
+                keyA[k] = (k < n) ? (((uint64_t)a.id[s][j] << 32) | (uint32_t)k) : ~0ull;
+                valA[k] = j;
             }
             __syncthreads();
             bitonic_sort(keyA, valA, np2);
@@ -542,11 +546,12 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
     auto put_slot = [&](int q, int sp) {
         const int f0 = a.start[sp][cell], n = a.start[sp][cell + 1] - f0;
         const UT* cx = cache + (size_t)(3 * q) * maxp;
-        UT* gx = (UT*)a.ux[sp] + f0;
-        UT* gy = (UT*)a.uy[sp] + f0;
-        UT* gz = (UT*)a.uz[sp] + f0;
+        UT* gx = (UT*)a.ux[sp];
+        UT* gy = (UT*)a.uy[sp];
+        UT* gz = (UT*)a.uz[sp];
         for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            gx[k] = cx[k]; gy[k] = cx[maxp + k]; gz[k] = cx[2 * maxp + k];
+            const int j = slot_index(a.perm[sp], f0 + k);
+            gx[j] = cx[k]; gy[j] = cx[maxp + k]; gz[j] = cx[2 * maxp + k];
         }
     };
     // fill slot qa with species sa and/or slot qb with species sb (a negative slot is left
@@ -559,12 +564,14 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
             UT ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
             uint32_t ai = 0, bi = 0;
             if (k < na) {
-                ax = ((const UT*)a.ux[sa])[fa + k]; ay = ((const UT*)a.uy[sa])[fa + k];
-                az = ((const UT*)a.uz[sa])[fa + k]; ai = a.id[sa][fa + k];
+                const int j = slot_index(a.perm[sa], fa + k);
+                ax = ((const UT*)a.ux[sa])[j]; ay = ((const UT*)a.uy[sa])[j];
+                az = ((const UT*)a.uz[sa])[j]; ai = a.id[sa][j];
             }
             if (k < nb) {
-                bx = ((const UT*)a.ux[sb])[fb + k]; by = ((const UT*)a.uy[sb])[fb + k];
-                bz = ((const UT*)a.uz[sb])[fb + k]; bi = a.id[sb][fb + k];
+                const int j = slot_index(a.perm[sb], fb + k);
+                bx = ((const UT*)a.ux[sb])[j]; by = ((const UT*)a.uy[sb])[j];
+                bz = ((const UT*)a.uz[sb])[j]; bi = a.id[sb][j];
             }
             if (k < na) {
                 UT* cx = cache + (size_t)(3 * qa) * maxp;
@@ -620,9 +627,9 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
                     int* val = nd == 0 ? valA : valB;
                     if (CU)
                         keyed_perm(a.seed, a.step, stage_now + v, spc, idc + (v == 0 ? q0 : q1) * maxp, 0, cnt, key,
-                                   val, 0);
+                                   val, 0, nullptr);
                     else
-                        keyed_perm(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, cnt, key, val, f0);
+                        keyed_perm(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, cnt, key, val, f0, a.perm[spc]);
                     if (threadIdx.x == 0) {
                         StageK& d = sk[nd];
                         stage_intra(d, a, spc, cnt, stage_now + v, Vc, ny, nd == 0 ? 0 : maxp);
@@ -638,11 +645,11 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
                     int qi = 0, qj = 1;
                     if (CU) hold(i, j, qi, qj);
                     if (CU) {
-                        keyed_perm(a.seed, a.step, stage_now, i, idc + qi * maxp, 0, na, keyA, valA, 0);
-                        keyed_perm(a.seed, a.step, stage_now, j, idc + qj * maxp, 0, nb, keyB, valB, 0);
+                        keyed_perm(a.seed, a.step, stage_now, i, idc + qi * maxp, 0, na, keyA, valA, 0, nullptr);
+                        keyed_perm(a.seed, a.step, stage_now, j, idc + qj * maxp, 0, nb, keyB, valB, 0, nullptr);
                     } else {
-                        keyed_perm(a.seed, a.step, stage_now, i, a.id[i], a0, na, keyA, valA, a0);
-                        keyed_perm(a.seed, a.step, stage_now, j, a.id[j], b0, nb, keyB, valB, b0);
+                        keyed_perm(a.seed, a.step, stage_now, i, a.id[i], a0, na, keyA, valA, a0, a.perm[i]);
+                        keyed_perm(a.seed, a.step, stage_now, j, a.id[j], b0, nb, keyB, valB, b0, a.perm[j]);
                     }
                     if (threadIdx.x == 0) {
                         StageK& d = sk[0];
@@ -757,8 +764,8 @@ __global__ void __launch_bounds__(64) k_coulomb_big(CoulombArgs a) {
 // bucket order); a key's rank is its bucket's start plus the smaller keys of its own bucket,
 // about two comparisons instead of n.  A group with n = 0 only takes part in the barriers.
 template <int L>
-__device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
-                           uint64_t* key, int* bins, int* val, int r) {
+__device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, const int* perm,
+                           int first, int n, uint64_t* key, int* bins, int* val, int r) {
     constexpr int KM = 8, NB = 4 * L, PER = NB / L;
     constexpr int SH = 64 - (L == 16 ? 6 : 7);   // top log2(NB) bits pick the bin
     for (int b = r; b < NB; b += L) bins[b] = 0;
@@ -769,7 +776,7 @@ __device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, 
         const int k = r + q * L;
         kk[q] = 0;
         if (k < n) {
-            kk[q] = (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k;
+            kk[q] = (shuffle_key(seed, step, stage, species, id[slot_index(perm, first + k)]) << 8) | (uint64_t)k;
             val[k] = atomicAdd(&bins[kk[q] >> SH], 1);   // place in the bucket, until the ranks
         }
     }
@@ -804,7 +811,7 @@ __device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, 
             const int s0 = bins[b], s1 = b + 1 < NB ? bins[b + 1] : n;
             int c = s0;
             for (int j = s0; j < s1; ++j) c += (int)(key[j] < kk[q]);
-            val[c] = first + k;
+            val[c] = slot_index(perm, first + k);
         }
     }
     __syncthreads();
@@ -854,8 +861,8 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
                     const int f0 = on ? a.start[spc][cell] : 0;
                     const int cnt = on ? a.start[spc][cell + 1] - f0 : 0;
                     const bool go = cnt > 1 && a.allow_intra;
-                    group_perm<L>(a.seed, a.step, stage_now + v, spc, a.id[spc], f0, go ? cnt : 0, key, bins,
-                                  nd == 0 ? valA : valB, r);
+                    group_perm<L>(a.seed, a.step, stage_now + v, spc, a.id[spc], a.perm[spc], f0, go ? cnt : 0, key,
+                                  bins, nd == 0 ? valA : valB, r);
                     if (go && r == 0) stage_intra(sk[nd], a, spc, cnt, stage_now + v, Vc, ny, nd == 0 ? 0 : maxp);
                     if (go) ++nd;
                 }
@@ -863,8 +870,8 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
                 const int a0 = on ? a.start[i][cell] : 0, na = on ? a.start[i][cell + 1] - a0 : 0;
                 const int b0 = on ? a.start[j][cell] : 0, nb = on ? a.start[j][cell + 1] - b0 : 0;
                 const bool go = na > 0 && nb > 0;
-                group_perm<L>(a.seed, a.step, stage_now, i, a.id[i], a0, go ? na : 0, key, bins, valA, r);
-                group_perm<L>(a.seed, a.step, stage_now, j, a.id[j], b0, go ? nb : 0, key, bins, valB, r);
+                group_perm<L>(a.seed, a.step, stage_now, i, a.id[i], a.perm[i], a0, go ? na : 0, key, bins, valA, r);
+                group_perm<L>(a.seed, a.step, stage_now, j, a.id[j], a.perm[j], b0, go ? nb : 0, key, bins, valB, r);
                 if (go && r == 0) stage_inter(sk[0], a, i, j, na, nb, stage_now, Vc, ny, 0, maxp);
                 nd = go ? 1 : 0;
             }

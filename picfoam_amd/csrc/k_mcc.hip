@@ -229,7 +229,7 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds
         // its rank is the number of smaller keys, counted eight keys per LDS read
         const int n8 = (eC + 7) & ~7;
         for (int k = threadIdx.x; k < n8; k += blockDim.x)
-            key[k] = (k < eC) ? (((uint64_t)a.e.id[a0 + k] << 32) | (uint32_t)k) : ~0ull;
+            key[k] = (k < eC) ? (((uint64_t)a.e.id[slot_index(a.perm, a0 + k)] << 32) | (uint32_t)k) : ~0ull;
         __syncthreads();
         for (int k = threadIdx.x; k < eC; k += blockDim.x) {
             const uint64_t kk = key[k];
@@ -239,7 +239,7 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds
                 r += (int)(q[0] < kk) + (int)(q[1] < kk) + (int)(q[2] < kk) + (int)(q[3] < kk) + (int)(q[4] < kk) +
                      (int)(q[5] < kk) + (int)(q[6] < kk) + (int)(q[7] < kk);
             }
-            val[r] = a0 + k;
+            val[r] = slot_index(a.perm, a0 + k);
         }
         __syncthreads();
         for (int base = 0; base < nc; base += 64) {

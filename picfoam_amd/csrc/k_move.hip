@@ -407,6 +407,19 @@ __global__ void __launch_bounds__(256) k_scatter(ScatterSet<UT> m) {
     dst.id[o] = src.id[i];
 }
 
+// perm[start[cell] + slot] = i: the same placement as k_scatter, as an index.  The parcels
+// stay where the move left them; kernels that walk a cell's parcels read them through perm.
+template <class UT>
+__global__ void __launch_bounds__(256) k_perm(ScatterSet<UT> m) {
+    int sp = 0;
+    while (sp + 1 < m.ns && (int64_t)blockIdx.x >= m.block0[sp + 1]) ++sp;
+    const int64_t i = ((int64_t)blockIdx.x - m.block0[sp]) * blockDim.x + threadIdx.x;
+    if (i >= min(*m.n_live[sp], m.n[sp])) return;
+    const int2 pc = m.place[sp][i];
+    if (pc.x < 0) return;
+    m.perm[sp][(int64_t)m.start[sp][pc.x] + pc.y] = (int)i;
+}
+
 // ---------------------------------------------------------------- instantiation / launch
 
 template <class UT>
@@ -516,7 +529,7 @@ hipError_t launch_init(const InitArgs<double>& a, hipStream_t s) { return launch
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s) { return launch_init_t(a, s); }
 
 template <class UT>
-static hipError_t launch_scatter_t(ScatterSet<UT> m, hipStream_t s) {
+static hipError_t launch_scatter_t(ScatterSet<UT> m, bool index_only, hipStream_t s) {
     int64_t blocks = 0;
     for (int k = 0; k < m.ns; ++k) {
         m.block0[k] = blocks;
@@ -524,10 +537,15 @@ static hipError_t launch_scatter_t(ScatterSet<UT> m, hipStream_t s) {
     }
     m.block0[m.ns] = blocks;
     if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)blocks), dim3(256), 0, s, m);
+    if (index_only)
+        hipLaunchKernelGGL((k_perm<UT>), dim3((unsigned)blocks), dim3(256), 0, s, m);
+    else
+        hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)blocks), dim3(256), 0, s, m);
     return hipGetLastError();
 }
-hipError_t launch_scatter(const ScatterSet<double>& m, hipStream_t s) { return launch_scatter_t(m, s); }
-hipError_t launch_scatter(const ScatterSet<float>& m, hipStream_t s) { return launch_scatter_t(m, s); }
+hipError_t launch_scatter(const ScatterSet<double>& m, hipStream_t s) { return launch_scatter_t(m, false, s); }
+hipError_t launch_scatter(const ScatterSet<float>& m, hipStream_t s) { return launch_scatter_t(m, false, s); }
+hipError_t launch_perm(const ScatterSet<double>& m, hipStream_t s) { return launch_scatter_t(m, true, s); }
+hipError_t launch_perm(const ScatterSet<float>& m, hipStream_t s) { return launch_scatter_t(m, true, s); }
 
 }  // namespace picmi

@@ -84,11 +84,16 @@ struct ScatterSet {
     const int64_t* n_live[kMaxSpecies];
     const int2* place[kMaxSpecies];
     const int* start[kMaxSpecies];
+    int* perm[kMaxSpecies];            // launch_perm: sorted slot -> array index
     int64_t block0[kMaxSpecies + 1];
     int ns;
 };
 hipError_t launch_scatter(const ScatterSet<double>& m, hipStream_t s);
 hipError_t launch_scatter(const ScatterSet<float>& m, hipStream_t s);
+// the cell order as an index instead of moved parcels: perm[start[cell] + slot] = i for
+// every live parcel i (place[i] = (cell, slot)); src, dst and the live counts are left alone
+hipError_t launch_perm(const ScatterSet<double>& m, hipStream_t s);
+hipError_t launch_perm(const ScatterSet<float>& m, hipStream_t s);
 
 // ---------------------------------------------------------------- fields (k_field.hip)
 // exclusive scan of n+1 ints (last = total); temp storage sized by scan_temp_bytes
@@ -200,6 +205,7 @@ struct CoulombArgs {
     double coulomb_log[kMaxSpecies][kMaxSpecies];
     int calc_debye, calculate_bmin, allow_intra;
     const int* start[kMaxSpecies];    // cell_start per species [n_cells+1]
+    const int* perm[kMaxSpecies];     // sorted slot -> array index, or null (arrays sorted by cell)
     void* ux[kMaxSpecies];
     void* uy[kMaxSpecies];
     void* uz[kMaxSpecies];
@@ -243,8 +249,9 @@ void xs_guide(const double* e, int n, int* guide, double* le0, double* idl);
 template <class UT>
 struct MccArgs {
     int n_cells;
-    PartSoA<UT> e;            // electrons, sorted by cell
+    PartSoA<UT> e;            // electrons, in cell order directly or through perm
     const int* start;         // electron cell_start
+    const int* perm;          // sorted slot -> electron index, or null (electrons sorted by cell)
     int64_t* n_e;             // device count of electrons (appends)
     int64_t cap_e;
     PartSoA<UT> ion;          // created ions (may have null pointers when no ion species)

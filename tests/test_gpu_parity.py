@@ -425,6 +425,78 @@ def test_run_equals_repeated_steps(case):
     assert list(st_run.n_parcels) == list(st_steps[-1].n_parcels)
 
 
+@pytest.mark.parametrize("case", ["grouped", "debye", "large-cells", "open-x", "mcc"])
+def test_index_sort_equals_moving_sort(case):
+    """sortInterval 4 (the collision kernels read the cell order through an index on three
+    steps of four) gives the same parcels, bit for bit, and the same step statistics as
+    moving the parcels into cell order every step.  Cases: four cells per wave (fixed lnL),
+    one cell per wave with LDS velocity slots (Debye lnL), the large-cell pass (300 parcels
+    per cell and species), parcels deleted at x patches between moving sorts, and MCC with
+    ionization appending parcels."""
+    say("index sort vs moving sort", case)
+    from picfoam_amd import cases, meshgen, picmi
+
+    def make(interval):
+        if case == "mcc":
+            mesh, species, models, ex = cases.argon_mcc(ncell=4, ppc=60, precision="fp64", L=1e-3, growth=20.0)
+            models["seed"] = 3
+            models["sortInterval"] = interval
+            c = picmi.Cloud(mesh, species, models)
+            e, el, exs, iz = ex["xs"]
+            for k, (s_, th) in enumerate(((el, 0.0), (exs, 11.5), (iz, 15.76))):
+                c.set_cross_section(k, e, s_, th)
+            c.mcc_initialize(1000.0, 0.01)
+            c.set_field(picmi.FIELD_E, ex["E"])
+            c.init_maxwellian(0, 60, ex["T_K"], seed=3)
+            return c, ex["dt"]
+        ppc = 300 if case == "large-cells" else 40
+        ncell = 2 if case == "large-cells" else 4
+        dt = cases.thermalisation(ncell=ncell, ppc=ppc)[3]["dt"]   # lnL 0 has no Spitzer rate to set dt
+        lnl = 0.0 if case == "debye" else 10.0
+        mesh, species, models, ex = cases.thermalisation(ncell=ncell, ppc=ppc, coulomb_log=lnl, dt=dt)
+        if case == "open-x":
+            box = ex["box"]
+            mesh = meshgen.hex_box(ncell, ncell, ncell, box, box, box, cyclic=(False, True, True),
+                                   boundary_type="patch")
+        models["sortInterval"] = interval
+        c = picmi.Cloud(mesh, species, models)
+        c.init_maxwellian(0, ppc, ex["Te_K"], id0=0, seed=11)
+        c.init_maxwellian(1, ppc, ex["Ti_K"], id0=0, seed=12)
+        return c, ex["dt"]
+
+    K = 6
+    a, dt = make(1)
+    b, _ = make(4)
+    st_a = [a.step(dt) for _ in range(K)]
+    st_b = [b.step(dt) for _ in range(K)]
+    for k in range(K):
+        assert st_a[k].coulomb_collisions == st_b[k].coulomb_collisions, k
+        assert st_a[k].n_deleted == st_b[k].n_deleted, k
+        assert (st_a[k].mcc_candidates, st_a[k].mcc_elastic, st_a[k].mcc_excitation, st_a[k].mcc_ionization) == \
+               (st_b[k].mcc_candidates, st_b[k].mcc_elastic, st_b[k].mcc_excitation, st_b[k].mcc_ionization), k
+    if case == "open-x":
+        assert sum(s.n_deleted for s in st_a) > 0
+    if case == "mcc":
+        assert sum(s.mcc_ionization for s in st_a) > 0
+    if case != "mcc":
+        assert sum(s.coulomb_collisions for s in st_a) > 0
+    for s in range(len(a.species)):
+        ya, ta, Ua, ia = a.get_particles(s)
+        yb, tb, Ub, ib = b.get_particles(s)
+        # step 6 of interval 4 is an index step: parcels deleted since step 4 are still in
+        # the arrays, marked by tet -1
+        assert np.all(ta >= 0)
+        live = tb >= 0
+        yb, tb, Ub, ib = yb[live], tb[live], Ub[live], ib[live]
+        assert len(ia) == len(ib)
+        oa, ob = np.argsort(ia, kind="stable"), np.argsort(ib, kind="stable")
+        assert np.array_equal(ia[oa], ib[ob])
+        assert np.array_equal(Ua[oa], Ub[ob]) and np.array_equal(ya[oa], yb[ob]) and np.array_equal(ta[oa], tb[ob])
+    rho_a = a.get_field(picmi.FIELD_RHO)
+    rho_b = b.get_field(picmi.FIELD_RHO)
+    assert np.array_equal(rho_a, rho_b)
+
+
 def test_coulomb_shuffle_every_list_length(oracle):
     """Regression for the ranked shuffle: lists of every length 1..20, around one and two
     64-key rounds, and up to the 256-key limit (cells with few keys spread each key over
