@@ -99,7 +99,7 @@ struct pic_ctx {
     // counts in kNcollSlots groups from kStatsMcc
     unsigned long long* d_stats = nullptr;
     int64_t* d_n = nullptr;                  // device parcel counts per species (authoritative)
-    int64_t* d_n_next = nullptr;             // sorted counts from the scan, installed after the scatter
+    int64_t* d_n_next = nullptr;             // sorted counts from the scan; swapped with d_n after the scatter
     int* d_big_list = nullptr;               // cells left to the large-cell collision passes
     double* d_moments = nullptr;             // cell moments scratch [n_cells][11]
     int* d_big_n = nullptr;
@@ -392,7 +392,7 @@ int do_sort(pic_ctx* c, bool index_only = false) {
     const hipError_t e = c->mixed() ? launch_scatter(scatter_set<float>(c), c->stream)
                                     : launch_scatter(scatter_set<double>(c), c->stream);
     if (int r = check_hip(c, e, "scatter")) return r;
-    if (int r = check_hip(c, launch_copy_counts(c->d_n_next, c->d_n, (int)c->sp.size(), c->stream), "counts")) return r;
+    std::swap(c->d_n, c->d_n_next);   // the sorted counts are the live ones from here on
     for (auto& S : c->sp) S.cur = 1 - S.cur;
     c->sorted = true;
     c->indexed = false;
@@ -858,7 +858,7 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->d_n_next = c->alloc<int64_t>(kMaxSpecies);
     c->d_big_list = c->alloc<int>(nc);
     c->d_big_n = c->alloc<int>(1);
-    c->scan_bytes = std::max(scan_temp_bytes(nc + 1), scan_cells_temp_bytes(nc + 1));
+    c->scan_bytes = scan_cells_temp_bytes(nc + 1);
     c->d_scan_tmp = c->alloc<char>(c->scan_bytes);
     c->d_diag_part = c->alloc<double>(1024 * 8);
     c->d_diag_out = c->alloc<double>(16);

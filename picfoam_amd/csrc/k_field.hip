@@ -51,31 +51,6 @@ __device__ __forceinline__ int k_scan_totals_body(int* totals, int nb) {
     return carry;
 }
 
-__global__ void __launch_bounds__(kScanBlock) k_scan_block(const int* __restrict__ in, int* __restrict__ out, int n,
-                                                           int* __restrict__ totals) {
-    k_scan_block_body(in, out, n, totals);
-}
-
-__global__ void __launch_bounds__(kScanBlock) k_scan_totals(int* totals, int nb) { k_scan_totals_body(totals, nb); }
-
-__global__ void k_scan_add(int* out, int n, const int* totals) {
-    const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
-    if (i < n) out[i] += totals[blockIdx.x];
-}
-
-size_t scan_temp_bytes(int n) { return sizeof(int) * (size_t)((n + kScanBlock - 1) / kScanBlock + 1); }
-
-hipError_t launch_scan(const int* in, int* out, int n, void* temp, size_t temp_bytes, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    const int nb = (n + kScanBlock - 1) / kScanBlock;
-    if (temp_bytes < scan_temp_bytes(n)) return hipErrorInvalidValue;
-    int* totals = (int*)temp;
-    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kScanBlock), 0, s, in, out, n, totals);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanBlock), 0, s, totals, nb);
-    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, s, out, n, totals);
-    return hipGetLastError();
-}
-
 // Exclusive scans of the cell counts of every species in three launches (block scans,
 // scan of the block totals, offsets added), blockIdx.y the species; the totals pass also
 // stores each species' grand total (its live parcel count after the sort).
@@ -99,10 +74,82 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_cells_add(ScanSet a) {
     if (i < a.n) a.out[sp][i] += a.tmp[sp * (nb + 1) + blockIdx.x];
 }
 
+// Meshes of up to kScanOne cells: one block of 1024 threads per species scans the counts in
+// a single launch.  Thread t holds counts [32 t, 32 t + 32) in registers (all loads in flight
+// together), the thread sums are scanned over the wave with shuffles and over the 16 waves
+// through LDS.  A count at index kScanOne (n = kScanOne + 1: the zero past the last cell) is
+// taken by the last thread.
+constexpr int kScanPer = 32;
+constexpr int kScanOne = kScanPer * 1024;
+
+__global__ void __launch_bounds__(1024) k_scan_cells_one(ScanSet a) {
+    __shared__ int wsum[16];
+    const int sp = blockIdx.x;
+    const int* __restrict__ in = a.in[sp];
+    int* __restrict__ out = a.out[sp];
+    const int n = a.n;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int b = t * kScanPer;
+    int v[kScanPer];
+    if (b + kScanPer <= n) {
+        const int4* in4 = (const int4*)(in + b);
+#pragma unroll
+        for (int q = 0; q < kScanPer / 4; ++q) {
+            const int4 x = in4[q];
+            v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) v[k] = b + k < n ? in[b + k] : 0;
+    }
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) sum += v[k];
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int run = incl - sum;
+    for (int k = 0; k < w; ++k) run += wsum[k];
+    if (b + kScanPer <= n) {
+        int4* out4 = (int4*)(out + b);
+#pragma unroll
+        for (int q = 0; q < kScanPer / 4; ++q) {
+            int4 x;
+            x.x = run; run += v[4 * q];
+            x.y = run; run += v[4 * q + 1];
+            x.z = run; run += v[4 * q + 2];
+            x.w = run; run += v[4 * q + 3];
+            out4[q] = x;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            if (b + k < n) out[b + k] = run;
+            run += v[k];
+        }
+    }
+    if (t == 1023) {
+        if (n > kScanOne) {
+            out[kScanOne] = run;
+            run += in[kScanOne];
+        }
+        *a.total[sp] = run;
+    }
+}
+
 size_t scan_cells_temp_bytes(int n) { return sizeof(int) * kMaxSpecies * (size_t)((n + kScanBlock - 1) / kScanBlock + 1); }
 
 hipError_t launch_scan_cells(const ScanSet& a, int n_species, hipStream_t s) {
     if (a.n <= 0 || n_species <= 0) return hipSuccess;
+    if (a.n <= kScanOne + 1) {
+        hipLaunchKernelGGL(k_scan_cells_one, dim3(n_species), dim3(1024), 0, s, a);
+        return hipGetLastError();
+    }
     const int nb = (a.n + kScanBlock - 1) / kScanBlock;
     hipLaunchKernelGGL(k_scan_cells_block, dim3(nb, n_species), dim3(kScanBlock), 0, s, a);
     hipLaunchKernelGGL(k_scan_cells_totals, dim3(n_species), dim3(kScanBlock), 0, s, a);
@@ -126,24 +173,6 @@ hipError_t launch_zero(const ZeroSet& z, hipStream_t s) {
     if (most <= 0) return hipSuccess;
     const int64_t nb = (most + 255) / 256;
     hipLaunchKernelGGL(k_zero, dim3((unsigned)(nb < 1024 ? nb : 1024)), dim3(256), 0, s, z);
-    return hipGetLastError();
-}
-
-__global__ void k_copy_counts(const int64_t* __restrict__ src, int64_t* __restrict__ dst, int n) {
-    if ((int)threadIdx.x < n) dst[threadIdx.x] = src[threadIdx.x];
-}
-
-hipError_t launch_copy_counts(const int64_t* src, int64_t* dst, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_copy_counts, dim3(1), dim3(64), 0, s, src, dst, n);
-    return hipGetLastError();
-}
-
-__global__ void k_store_count(const int* __restrict__ src, int64_t* __restrict__ dst) {
-    if (threadIdx.x == 0) *dst = *src;
-}
-
-hipError_t launch_store_count(const int* src, int64_t* dst, hipStream_t s) {
-    hipLaunchKernelGGL(k_store_count, dim3(1), dim3(64), 0, s, src, dst);
     return hipGetLastError();
 }
 

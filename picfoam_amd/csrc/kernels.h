@@ -96,15 +96,11 @@ hipError_t launch_perm(const ScatterSet<double>& m, hipStream_t s);
 hipError_t launch_perm(const ScatterSet<float>& m, hipStream_t s);
 
 // ---------------------------------------------------------------- fields (k_field.hip)
-// exclusive scan of n+1 ints (last = total); temp storage sized by scan_temp_bytes
-size_t scan_temp_bytes(int n);
-hipError_t launch_scan(const int* in, int* out, int n, void* temp, size_t temp_bytes, hipStream_t s);
-// *dst = *src (the live count after a sort: start[n_cells] -> the species count)
-hipError_t launch_store_count(const int* src, int64_t* dst, hipStream_t s);
 // exclusive scans of n ints (the last count zero, so out[n-1] is the total) for every
-// species in the same three launches, totals stored as int64 (kept aside until the scatter,
-// which reads the live counts before the sort, has run; launch_copy_counts then installs
-// them); tmp sized by scan_cells_temp_bytes
+// species in the same launches (one for up to 32768 cells, else three), totals stored as
+// int64 into a second counter array: the scatter still reads the live counts before the
+// sort, and the host makes the second array the live one after it; tmp sized by
+// scan_cells_temp_bytes
 constexpr int kScanBlock = 1024;
 struct ScanSet {
     const int* in[kMaxSpecies];
@@ -122,7 +118,6 @@ struct ZeroSet {
 };
 hipError_t launch_zero(const ZeroSet& z, hipStream_t s);
 hipError_t launch_scan_cells(const ScanSet& a, int n_species, hipStream_t s);
-hipError_t launch_copy_counts(const int64_t* src, int64_t* dst, int n, hipStream_t s);
 
 struct DepositFinishArgs {
     int n_cells;

@@ -124,13 +124,19 @@ def test_deposit_bit_exact(oracle, kind, weighting):
     assert np.array_equal(rho_g, rho_r), rel_err(rho_g, rho_r)
 
 
-def test_sort_groups_by_cell(oracle):
-    say("sort")
+@pytest.mark.parametrize("mesh,n", [("tet", 9000), ("hex-32", 100000), ("hex-33", 100000)])
+def test_sort_groups_by_cell(oracle, mesh, n):
+    """Counting sort by cell, against the oracle's.  32^3 cells is the largest mesh the
+    one-launch scan takes (its count past the last cell at index 32768), 33^3 goes through
+    the three-launch scan."""
+    say("sort", mesh)
     rng = np.random.default_rng(103)
-    m = small_tet()
-    c = cloud(m, {"ParticlePusher": "Boris"})
+    m = {"tet": lambda: small_tet(), "hex-32": lambda: small_hex(n=(32, 32, 32), L=(1e-3, 1e-3, 1e-3)),
+         "hex-33": lambda: small_hex(n=(33, 33, 33), L=(1e-3, 1e-3, 1e-3))}[mesh]()
+    c = cloud(m, {"ParticlePusher": "Boris"},
+              [{"mass": ME, "charge": -QE, "nEquivalentParticles": 1e5, "capacity": n}])
     rm = oracle.RefMesh(m)
-    y, tet, U, ids = random_parcels(rng, rm.n_tets, 9000, 1.0)
+    y, tet, U, ids = random_parcels(rng, rm.n_tets, n, 1.0)
     tet[::7] = -1  # deleted parcels are compacted away
     c.set_particles(0, y, tet, U, ids)
     c.sort()
@@ -140,9 +146,11 @@ def test_sort_groups_by_cell(oracle):
     assert np.all(np.diff(cells) >= 0)
     order, cs = rm.sort_by_cell(tet)
     assert len(ig) == len(order)
-    for cidx in range(rm.n_cells):
-        a = np.sort(ig[cells == cidx]); b = np.sort(ids[order[cs[cidx]:cs[cidx + 1]]])
-        assert np.array_equal(a, b)
+    # the same parcels in every cell: (cell, id) pairs agree once ordered
+    cells_ref = np.repeat(np.arange(rm.n_cells), np.diff(cs))
+    ids_ref = ids[order]
+    oa = np.lexsort((ig, cells)); ob = np.lexsort((ids_ref, cells_ref))
+    assert np.array_equal(cells[oa], cells_ref[ob]) and np.array_equal(ig[oa], ids_ref[ob])
     # payload travelled with the id
     o = np.argsort(ig)
     live = np.nonzero(tet >= 0)[0]
