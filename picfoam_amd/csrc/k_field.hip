@@ -75,70 +75,95 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_cells_add(ScanSet a) {
 }
 
 // Meshes of up to kScanOne cells: one block of 1024 threads per species scans the counts in
-// a single launch.  Thread t holds counts [32 t, 32 t + 32) in registers (all loads in flight
-// together), the thread sums are scanned over the wave with shuffles and over the 16 waves
-// through LDS.  A count at index kScanOne (n = kScanOne + 1: the zero past the last cell) is
-// taken by the last thread.
-constexpr int kScanPer = 32;
-constexpr int kScanOne = kScanPer * 1024;
+// a single launch.  The counts are taken in kScanRounds rounds of 8192; in round r thread t
+// holds counts [8192 r + 8 t, 8192 r + 8 t + 8), so a wave's loads cover 2 KiB without gaps,
+// and every round's loads are in flight together.  The thread sums of all rounds are
+// scanned over the wave with shuffles and over the 16 waves through LDS.  A count at index
+// kScanOne (n = kScanOne + 1: the zero past the last cell) is taken by the last thread.
+constexpr int kScanChunk = 8;
+constexpr int kScanRounds = 4;
+constexpr int kScanRound = kScanChunk * 1024;
+constexpr int kScanOne = kScanRounds * kScanRound;
 
 __global__ void __launch_bounds__(1024) k_scan_cells_one(ScanSet a) {
-    __shared__ int wsum[16];
+    __shared__ int wsum[kScanRounds][16];
     const int sp = blockIdx.x;
     const int* __restrict__ in = a.in[sp];
     int* __restrict__ out = a.out[sp];
     const int n = a.n;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int b = t * kScanPer;
-    int v[kScanPer];
-    if (b + kScanPer <= n) {
-        const int4* in4 = (const int4*)(in + b);
+    int v[kScanRounds][kScanChunk];
 #pragma unroll
-        for (int q = 0; q < kScanPer / 4; ++q) {
-            const int4 x = in4[q];
-            v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+    for (int r = 0; r < kScanRounds; ++r) {
+        const int b = r * kScanRound + t * kScanChunk;
+        if (b + kScanChunk <= n) {
+            const int4 x = *(const int4*)(in + b), y = *(const int4*)(in + b + 4);
+            v[r][0] = x.x; v[r][1] = x.y; v[r][2] = x.z; v[r][3] = x.w;
+            v[r][4] = y.x; v[r][5] = y.y; v[r][6] = y.z; v[r][7] = y.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < kScanChunk; ++k) v[r][k] = b + k < n ? in[b + k] : 0;
         }
-    } else {
-#pragma unroll
-        for (int k = 0; k < kScanPer; ++k) v[k] = b + k < n ? in[b + k] : 0;
     }
-    int sum = 0;
+    int sum[kScanRounds], incl[kScanRounds];
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) sum += v[k];
-    int incl = sum;
+    for (int r = 0; r < kScanRounds; ++r) {
+        sum[r] = 0;
+#pragma unroll
+        for (int k = 0; k < kScanChunk; ++k) sum[r] += v[r][k];
+        incl[r] = sum[r];
+    }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(incl, off);
-        if (lane >= off) incl += y;
+#pragma unroll
+        for (int r = 0; r < kScanRounds; ++r) {
+            const int y = __shfl_up(incl[r], off);
+            if (lane >= off) incl[r] += y;
+        }
     }
-    if (lane == 63) wsum[w] = incl;
+    if (lane == 63) {
+#pragma unroll
+        for (int r = 0; r < kScanRounds; ++r) wsum[r][w] = incl[r];
+    }
     __syncthreads();
-    int run = incl - sum;
-    for (int k = 0; k < w; ++k) run += wsum[k];
-    if (b + kScanPer <= n) {
-        int4* out4 = (int4*)(out + b);
+    int carry = 0;
 #pragma unroll
-        for (int q = 0; q < kScanPer / 4; ++q) {
-            int4 x;
-            x.x = run; run += v[4 * q];
-            x.y = run; run += v[4 * q + 1];
-            x.z = run; run += v[4 * q + 2];
-            x.w = run; run += v[4 * q + 3];
-            out4[q] = x;
+    for (int r = 0; r < kScanRounds; ++r) {
+        int before = 0, total = 0;
+        for (int k = 0; k < 16; ++k) {
+            const int x = wsum[r][k];
+            before += k < w ? x : 0;
+            total += x;
         }
-    } else {
+        int run = carry + before + incl[r] - sum[r];
+        const int b = r * kScanRound + t * kScanChunk;
+        if (b + kScanChunk <= n) {
+            int4 x, y;
+            x.x = run; run += v[r][0];
+            x.y = run; run += v[r][1];
+            x.z = run; run += v[r][2];
+            x.w = run; run += v[r][3];
+            y.x = run; run += v[r][4];
+            y.y = run; run += v[r][5];
+            y.z = run; run += v[r][6];
+            y.w = run;
+            *(int4*)(out + b) = x;
+            *(int4*)(out + b + 4) = y;
+        } else {
 #pragma unroll
-        for (int k = 0; k < kScanPer; ++k) {
-            if (b + k < n) out[b + k] = run;
-            run += v[k];
+            for (int k = 0; k < kScanChunk; ++k) {
+                if (b + k < n) out[b + k] = run;
+                run += v[r][k];
+            }
         }
+        carry += total;
     }
     if (t == 1023) {
         if (n > kScanOne) {
-            out[kScanOne] = run;
-            run += in[kScanOne];
+            out[kScanOne] = carry;
+            carry += in[kScanOne];
         }
-        *a.total[sp] = run;
+        *a.total[sp] = carry;
     }
 }
 
