@@ -90,6 +90,9 @@ typedef struct {
     double n_equivalent;  /* nEquivalentParticles (weight) */
     int32_t solve_movement;
     int64_t capacity;     /* parcels held on the device for this species */
+    int32_t sort_interval; /* this species' sort_interval (see pic_models); 0 = the models' value.
+                              A heavy species that seldom changes cell can move into cell order
+                              less often than a light one */
 } pic_species;
 
 /* constant/picProperties keywords; names as the reference's selection tables */

@@ -40,6 +40,10 @@ struct Species {
     int* start = nullptr;   // [n_cells+1]
     int2* place = nullptr;  // [capacity] (cell, slot) from the counting pass
     int* perm = nullptr;    // [capacity] sorted slot -> array index after an index sort
+    uint32_t* id_cell = nullptr;  // [capacity] ids in cell order after an index sort
+    bool indexed = false;   // the cell order is perm (and id_cell), not the arrays' order
+    int sort_interval = 1;  // moving sorts every k steps (index sorts in between)
+    int ncur = 0;           // which of the context's two count rows holds the live count
 };
 
 struct StageEvent {
@@ -78,7 +82,6 @@ struct pic_ctx {
     bool e_zero = true;                      // E is zero in every cell (never set, no field solve)
     bool sorted = false, counts_valid = false;
     bool place_valid = false;                // Species::place matches the current layout
-    bool indexed = false;                    // the cell order is Species::perm, not the arrays' order
     bool big_n_zero = false;                 // d_big_n zeroed since its last use
     // species
     std::vector<Species> sp;
@@ -98,8 +101,10 @@ struct pic_ctx {
     // 0 deleted, 1 guard; Coulomb collisions in kNcollSlots counters from kStatsNcoll, MCC
     // counts in kNcollSlots groups from kStatsMcc
     unsigned long long* d_stats = nullptr;
-    int64_t* d_n = nullptr;                  // device parcel counts per species (authoritative)
-    int64_t* d_n_next = nullptr;             // sorted counts from the scan; swapped with d_n after the scatter
+    // device parcel counts, two rows of kMaxSpecies: row Species::ncur holds species s's live
+    // count (authoritative), the other row receives the sorted count from the scan and
+    // becomes the live one once the parcels have moved into cell order
+    int64_t* d_nrows = nullptr;
     int* d_big_list = nullptr;               // cells left to the large-cell collision passes
     double* d_moments = nullptr;             // cell moments scratch [n_cells][11]
     int* d_big_n = nullptr;
@@ -211,15 +216,20 @@ int parse_model(pic_ctx* c, const char* name, const char* keyword, const std::ve
     return c->fail(PIC_ERR_UNKNOWN_MODEL, m);
 }
 
+// the live parcel count of species s on the device, and the row the next sort's scan fills
+int64_t* live_n(pic_ctx* c, int s) { return c->d_nrows + c->sp[s].ncur * kMaxSpecies + s; }
+int64_t* next_n(pic_ctx* c, int s) { return c->d_nrows + (1 - c->sp[s].ncur) * kMaxSpecies + s; }
+
 // error flags + counts after a synchronisation point
 int sync_state(pic_ctx* c) {
     uint32_t flags = 0;
-    std::vector<int64_t> n(c->sp.size());
+    std::vector<int64_t> n(2 * kMaxSpecies);
     hipMemcpyAsync(&flags, c->d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
-    hipMemcpyAsync(n.data(), c->d_n, sizeof(int64_t) * c->sp.size(), hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(n.data(), c->d_nrows, sizeof(int64_t) * 2 * kMaxSpecies, hipMemcpyDeviceToHost, c->stream);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return check_hip(c, e, "stream synchronize");
-    for (size_t s = 0; s < c->sp.size(); ++s) c->sp[s].n = std::min<int64_t>(n[s], c->sp[s].capacity);
+    for (size_t s = 0; s < c->sp.size(); ++s)
+        c->sp[s].n = std::min<int64_t>(n[c->sp[s].ncur * kMaxSpecies + s], c->sp[s].capacity);
     if (flags) {
         hipMemsetAsync(c->d_flags, 0, sizeof(uint32_t), c->stream);
         if (flags & 1u) return c->fail(PIC_ERR_SUPERLUMINAL, "Particle velocity greater than speed of light (299792458)! Aborting...");
@@ -231,7 +241,7 @@ int sync_state(pic_ctx* c) {
 }
 
 void set_dev_n(pic_ctx* c, int s) {
-    hipMemcpyAsync(c->d_n + s, &c->sp[s].n, sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(live_n(c, s), &c->sp[s].n, sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
 }
 
 int alloc_species(pic_ctx* c, Species& s) {
@@ -255,7 +265,9 @@ int alloc_species(pic_ctx* c, Species& s) {
     s.start = c->alloc<int>(nc + 1);
     s.place = c->alloc<int2>(cap);
     s.perm = c->alloc<int>(cap);
-    if (!s.count || !s.start || !s.place || !s.perm) return c->fail(PIC_ERR_HIP, "out of device memory (cell counts)");
+    s.id_cell = c->alloc<uint32_t>(cap);
+    if (!s.count || !s.start || !s.place || !s.perm || !s.id_cell)
+        return c->fail(PIC_ERR_HIP, "out of device memory (cell counts)");
     return PIC_OK;
 }
 
@@ -281,7 +293,7 @@ MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
     MoveArgs<UT> a{};
     a.p = soa<UT>(S, S.cur);
     a.n = S.n;
-    a.n_live = c->d_n + s;
+    a.n_live = live_n(c, s);
     a.tets = c->d_tets;
     if constexpr (sizeof(UT) == 4) {
         a.Ecell = (const UT*)c->d_Ef; a.Epoint = (const UT*)c->d_Epf; a.Bcell = (const UT*)c->d_Bf;
@@ -332,7 +344,7 @@ int do_move(pic_ctx* c, double dt) {
     c->counts_valid = true;
     c->place_valid = true;
     c->sorted = false;
-    c->indexed = false;
+    for (auto& S : c->sp) S.indexed = false;
     return PIC_OK;
 }
 
@@ -345,10 +357,11 @@ ScatterSet<UT> scatter_set(pic_ctx* c) {
         m.src[s] = soa<UT>(S, S.cur);
         m.dst[s] = soa<UT>(S, 1 - S.cur);
         m.n[s] = S.n;
-        m.n_live[s] = c->d_n + s;
+        m.n_live[s] = live_n(c, s);
         m.place[s] = S.place;
         m.start[s] = S.start;
         m.perm[s] = S.perm;
+        m.id_cell[s] = S.id_cell;
     }
     return m;
 }
@@ -356,15 +369,18 @@ ScatterSet<UT> scatter_set(pic_ctx* c) {
 // buildCellOccupancy (PICCloud.C:327-354): counting sort of every species by cell.  No
 // host round trip: the live count after the sort (start[n_cells]) stays on the device and
 // the host keeps S.n as an upper bound for launch sizes until the next synchronisation.
-// index_only: the parcels stay where they are and Species::perm lists them in cell order
-// (the reference's occupancy lists of parcel pointers, PICCloud.H:166); deleted parcels stay
-// in the arrays, marked by tet -1, until the next moving sort.
-int do_sort(pic_ctx* c, bool index_only = false) {
+// in_step (the sort inside pic_step / pic_run): a species whose moving sort is not due this
+// step (Species::sort_interval) keeps its parcels where they are and gets the cell order as
+// an index, Species::perm and id_cell (the reference's occupancy lists hold parcel
+// pointers, PICCloud.H:166); its deleted parcels stay in the arrays, marked by tet -1,
+// until its next moving sort.
+int do_sort(pic_ctx* c, bool in_step = false) {
     pic_ctx::Scope sc(c, PIC_STAGE_SORT);
     const int nc = c->hm.n_cells;
+    const int ns = (int)c->sp.size();
     if (!c->counts_valid || !c->place_valid) {
         for (auto& s : c->sp) hipMemsetAsync(s.count, 0, sizeof(int) * (nc + 1), c->stream);
-        for (int s = 0; s < (int)c->sp.size(); ++s) {
+        for (int s = 0; s < ns; ++s) {
             hipError_t e = c->mixed() ? launch_deposit(move_args<float>(c, s, 0.0), kCellAverage, c->stream)
                                       : launch_deposit(move_args<double>(c, s, 0.0), kCellAverage, c->stream);
             if (int r = check_hip(c, e, "count kernel")) return r;
@@ -372,31 +388,41 @@ int do_sort(pic_ctx* c, bool index_only = false) {
         c->counts_valid = true;
         c->place_valid = true;
     }
-    // every species' scan in the same launches; the sorted counts wait in d_n_next while
-    // the scatter reads the live counts before the sort
+    // every species' scan in the same launches; the sorted counts go to the species' other
+    // count row while the scatter reads the live counts before the sort
     ScanSet ss{};
     ss.n = nc + 1;
     ss.tmp = (int*)c->d_scan_tmp;
-    for (int s = 0; s < (int)c->sp.size(); ++s) {
-        ss.in[s] = c->sp[s].count; ss.out[s] = c->sp[s].start; ss.total[s] = c->d_n_next + s;
+    for (int s = 0; s < ns; ++s) {
+        ss.in[s] = c->sp[s].count; ss.out[s] = c->sp[s].start; ss.total[s] = next_n(c, s);
     }
-    if (int r = check_hip(c, launch_scan_cells(ss, (int)c->sp.size(), c->stream), "scan")) return r;
-    if (index_only) {
-        const hipError_t e = c->mixed() ? launch_perm(scatter_set<float>(c), c->stream)
-                                        : launch_perm(scatter_set<double>(c), c->stream);
-        if (int r = check_hip(c, e, "index sort")) return r;
-        c->sorted = true;
-        c->indexed = true;
-        return PIC_OK;   // the layout is unchanged, so the slots stay valid
+    if (int r = check_hip(c, launch_scan_cells(ss, ns, c->stream), "scan")) return r;
+    int index[kMaxSpecies] = {};
+    bool moved = false;
+    for (int s = 0; s < ns; ++s) {
+        index[s] = in_step && (c->step + 1) % c->sp[s].sort_interval != 0;
+        moved = moved || !index[s];
     }
-    const hipError_t e = c->mixed() ? launch_scatter(scatter_set<float>(c), c->stream)
-                                    : launch_scatter(scatter_set<double>(c), c->stream);
+    hipError_t e;
+    if (c->mixed()) {
+        ScatterSet<float> m = scatter_set<float>(c);
+        for (int s = 0; s < ns; ++s) m.index[s] = index[s];
+        e = launch_scatter(m, c->stream);
+    } else {
+        ScatterSet<double> m = scatter_set<double>(c);
+        for (int s = 0; s < ns; ++s) m.index[s] = index[s];
+        e = launch_scatter(m, c->stream);
+    }
     if (int r = check_hip(c, e, "scatter")) return r;
-    std::swap(c->d_n, c->d_n_next);   // the sorted counts are the live ones from here on
-    for (auto& S : c->sp) S.cur = 1 - S.cur;
+    for (int s = 0; s < ns; ++s) {
+        Species& S = c->sp[s];
+        S.indexed = index[s] != 0;
+        if (S.indexed) continue;
+        S.cur = 1 - S.cur;
+        S.ncur = 1 - S.ncur;   // the sorted count is the live one from here on
+    }
     c->sorted = true;
-    c->indexed = false;
-    c->place_valid = false;   // the slots refer to the layout before this sort
+    if (moved) c->place_valid = false;   // the moved species' slots refer to the layout before this sort
     return PIC_OK;
 }
 
@@ -418,10 +444,10 @@ int do_coulomb(pic_ctx* c, double dt) {
         a.charge[i] = c->sp[i].charge;
         a.weight[i] = c->sp[i].weight;
         a.start[i] = c->sp[i].start;
-        a.perm[i] = c->indexed ? c->sp[i].perm : nullptr;
+        a.perm[i] = c->sp[i].indexed ? c->sp[i].perm : nullptr;
         const int b = c->sp[i].cur;
         a.ux[i] = c->sp[i].u[b][0]; a.uy[i] = c->sp[i].u[b][1]; a.uz[i] = c->sp[i].u[b][2];
-        a.id[i] = c->sp[i].id[b];
+        a.id[i] = c->sp[i].indexed ? c->sp[i].id_cell : c->sp[i].id[b];
         for (int j = 0; j < ns; ++j) {
             a.coulomb_log[i][j] = c->coulomb_log[i][j];
             if (j >= i && c->sp[i].charge != 0.0 && c->sp[j].charge != 0.0 && c->coulomb_log[i][j] <= 0.0) calc_debye = true;
@@ -470,18 +496,19 @@ int do_mcc_t(pic_ctx* c, double dt) {
     a.n_cells = c->hm.n_cells;
     a.e = soa<UT>(E, E.cur);
     a.start = E.start;
-    a.perm = c->indexed ? E.perm : nullptr;
-    a.n_e = c->d_n + c->electron;
+    a.perm = E.indexed ? E.perm : nullptr;
+    a.id_cell = E.indexed ? E.id_cell : E.id[E.cur];
+    a.n_e = live_n(c, c->electron);
     a.cap_e = E.capacity;
     if (c->ion >= 0) {
         Species& I = c->sp[c->ion];
         a.ion = soa<UT>(I, I.cur);
-        a.n_i = c->d_n + c->ion;
+        a.n_i = live_n(c, c->ion);
         a.cap_i = I.capacity;
         a.count_i = I.count;
         a.s42_i = fixed_scale(c, c->ion);
     } else {
-        a.n_i = c->d_n;
+        a.n_i = live_n(c, c->electron);   // unused: no ions are created
         a.cap_i = 0;
         a.count_i = E.count;
     }
@@ -809,6 +836,7 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         S.weight = species[s].n_equivalent;
         S.solve_movement = species[s].solve_movement;
         S.capacity = species[s].capacity;
+        S.sort_interval = species[s].sort_interval > 0 ? species[s].sort_interval : c->sort_interval;
         if (!(S.mass > 0) || !(S.weight > 0) || S.capacity < 0 || S.capacity > 2000000000LL)
             return c->fail(PIC_ERR_INVALID, "species " + std::to_string(s) + ": mass, weight and capacity must be positive (capacity < 2^31)");
         qmax = std::max(qmax, std::fabs(S.charge * S.weight));
@@ -854,8 +882,7 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->d_acc_point = c->alloc<unsigned long long>(np);
     c->d_flags = c->alloc<uint32_t>(1);
     c->d_stats = c->alloc<unsigned long long>(kStatsWords);
-    c->d_n = c->alloc<int64_t>(kMaxSpecies);
-    c->d_n_next = c->alloc<int64_t>(kMaxSpecies);
+    c->d_nrows = c->alloc<int64_t>(2 * kMaxSpecies);
     c->d_big_list = c->alloc<int>(nc);
     c->d_big_n = c->alloc<int>(1);
     c->scan_bytes = scan_cells_temp_bytes(nc + 1);
@@ -1106,7 +1133,7 @@ int pic_particles_set(pic_ctx* c, int32_t s, int64_t n, const double* bary, cons
     S.n = n;
     set_dev_n(c, s);
     c->sorted = false;
-    c->indexed = false;
+    for (auto& S : c->sp) S.indexed = false;
     c->counts_valid = false;
     return check_hip(c, hipStreamSynchronize(c->stream), "particles set");
 }
@@ -1145,7 +1172,7 @@ int pic_particles_set_positions(pic_ctx* c, int32_t s, int64_t n, const double* 
     S.n = n;
     set_dev_n(c, s);
     c->sorted = false;
-    c->indexed = false;
+    for (auto& S : c->sp) S.indexed = false;
     c->counts_valid = false;
     if (fails) return c->fail(PIC_ERR_INVALID, std::to_string(fails) + " positions lie outside their given cell");
     return check_hip(c, hipStreamSynchronize(c->stream), "particles set positions");
@@ -1221,7 +1248,7 @@ int pic_init_maxwellian(pic_ctx* c, int32_t s, int32_t nppc, double T, const dou
     S.n += add;
     set_dev_n(c, s);
     c->sorted = false;
-    c->indexed = false;
+    for (auto& S : c->sp) S.indexed = false;
     c->counts_valid = false;
     return check_hip(c, hipStreamSynchronize(c->stream), "init");
 }
@@ -1286,12 +1313,12 @@ int pic_poisson_solve(pic_ctx* c, pic_poisson_stats* st) {
 // round trip, except inside the Poisson solve (convergence checks) and GNN recording
 int step_body(pic_ctx* c, double dt, pic_poisson_stats* ps) {
     if (int r = do_move(c, dt)) return r;
-    // the parcels move into cell order every sort_interval steps; collision models need the
-    // cell order every step, and in between get it as an index
-    const bool move_sort = (c->step + 1) % c->sort_interval == 0;
-    const bool collide = c->coulomb || c->mcc;
-    if (move_sort || collide)
-        if (int r = do_sort(c, !move_sort)) return r;
+    // a species moves into cell order every Species::sort_interval steps; collision models
+    // need the cell order every step, and in between get it as an index
+    bool due = c->coulomb || c->mcc;
+    for (const auto& S : c->sp) due = due || (c->step + 1) % S.sort_interval == 0;
+    if (due)
+        if (int r = do_sort(c, true)) return r;
     if (c->coulomb)
         if (int r = do_coulomb(c, dt)) return r;
     if (c->mcc)

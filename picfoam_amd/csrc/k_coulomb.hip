@@ -284,7 +284,8 @@ __device__ __forceinline__ uint64_t shuffle_key(uint64_t seed, int64_t step, int
 }
 
 // val[0..n) = the cell's parcels of one species ordered by their 56-bit random key: base +
-// cell-local index, or the array index through perm (perm non-null: base is unused).  Up to
+// cell-local index, or the array index through perm (perm non-null: base is unused); id
+// holds the ids in cell order (id[first + k] for the cell's parcel k).  Up to
 // 256 parcels the key carries the cell-local index in its low byte, so keys are distinct and
 // each parcel's place is the number of smaller keys (no barriers in the count; eight keys
 // per LDS round trip).  Larger cells use a bitonic sort.
@@ -293,8 +294,7 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
     if (n <= 256) {
         const int n8 = (n + 7) & ~7;
         for (int k = threadIdx.x; k < n8; k += blockDim.x)
-            key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[slot_index(perm, first + k)]) << 8) | (uint64_t)k
-                           : ~0ull;
+            key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k : ~0ull;
         __syncthreads();
         // rounds of up to 64 keys; a partial round of E keys gives each key G = 64 / E lanes
         // (rounded down to a power of two) that count over G slices of the list and add up
@@ -329,9 +329,8 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
     while (np2 < n) np2 <<= 1;
     for (int k = threadIdx.x; k < np2; k += blockDim.x) {
         if (k < n) {
-            const int j = slot_index(perm, first + k);
-            key[k] = (shuffle_key(seed, step, stage, species, id[j]) << 8) | (uint64_t)(k & 255);
-            val[k] = perm ? j : base + k;
+            key[k] = (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)(k & 255);
+            val[k] = perm ? perm[first + k] : base + k;
         } else {
             key[k] = ~0ull;
             val[k] = -1;
@@ -500,9 +499,8 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
             int np2 = 1;
             while (np2 < n) np2 <<= 1;
             for (int k = threadIdx.x; k < np2; k += blockDim.x) {
-                const int j = k < n ? slot_index(a.perm[s], f0 + k) : -1;
-                keyA[k] = (k < n) ? (((uint64_t)a.id[s][j] << 32) | (uint32_t)k) : ~0ull;
-                valA[k] = j;
+                keyA[k] = (k < n) ? (((uint64_t)a.id[s][f0 + k] << 32) | (uint32_t)k) : ~0ull;
+                valA[k] = (k < n) ? slot_index(a.perm[s], f0 + k) : -1;
             }
             __syncthreads();
             bitonic_sort(keyA, valA, np2);
@@ -566,12 +564,12 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
             if (k < na) {
                 const int j = slot_index(a.perm[sa], fa + k);
                 ax = ((const UT*)a.ux[sa])[j]; ay = ((const UT*)a.uy[sa])[j];
-                az = ((const UT*)a.uz[sa])[j]; ai = a.id[sa][j];
+                az = ((const UT*)a.uz[sa])[j]; ai = a.id[sa][fa + k];
             }
             if (k < nb) {
                 const int j = slot_index(a.perm[sb], fb + k);
                 bx = ((const UT*)a.ux[sb])[j]; by = ((const UT*)a.uy[sb])[j];
-                bz = ((const UT*)a.uz[sb])[j]; bi = a.id[sb][j];
+                bz = ((const UT*)a.uz[sb])[j]; bi = a.id[sb][fb + k];
             }
             if (k < na) {
                 UT* cx = cache + (size_t)(3 * qa) * maxp;
@@ -776,7 +774,7 @@ __device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, 
         const int k = r + q * L;
         kk[q] = 0;
         if (k < n) {
-            kk[q] = (shuffle_key(seed, step, stage, species, id[slot_index(perm, first + k)]) << 8) | (uint64_t)k;
+            kk[q] = (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k;
             val[k] = atomicAdd(&bins[kk[q] >> SH], 1);   // place in the bucket, until the ranks
         }
     }

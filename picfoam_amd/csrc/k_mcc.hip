@@ -229,7 +229,7 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds
         // its rank is the number of smaller keys, counted eight keys per LDS read
         const int n8 = (eC + 7) & ~7;
         for (int k = threadIdx.x; k < n8; k += blockDim.x)
-            key[k] = (k < eC) ? (((uint64_t)a.e.id[slot_index(a.perm, a0 + k)] << 32) | (uint32_t)k) : ~0ull;
+            key[k] = (k < eC) ? (((uint64_t)a.id_cell[a0 + k] << 32) | (uint32_t)k) : ~0ull;
         __syncthreads();
         for (int k = threadIdx.x; k < eC; k += blockDim.x) {
             const uint64_t kk = key[k];

@@ -384,7 +384,8 @@ __global__ void k_init(InitArgs<UT> a) {
 // start[cell] + slot, with (cell, slot) handed over by the counting pass (the slot comes
 // from a wave-aggregated atomic there, so the order inside a cell is not fixed; every
 // consumer that depends on it orders the cell's parcels by id or by a random key).  Deleted
-// parcels carry cell -1.
+// parcels carry cell -1.  A species marked index[s] keeps its parcels in place and gets the
+// placement as an index (perm, plus its ids in cell order for the collision shuffle keys).
 // Every species in one launch, blocks [block0[s], block0[s+1]) for species s.
 template <class UT>
 __global__ void __launch_bounds__(256) k_scatter(ScatterSet<UT> m) {
@@ -401,23 +402,16 @@ __global__ void __launch_bounds__(256) k_scatter(ScatterSet<UT> m) {
     const int2 pc = place[i];
     if (pc.x < 0) return;
     const int64_t o = (int64_t)start[pc.x] + pc.y;
+    if (m.index[sp]) {
+        // the parcels stay; kernels that walk a cell's parcels read them through perm
+        m.perm[sp][o] = (int)i;
+        m.id_cell[sp][o] = src.id[i];
+        return;
+    }
     dst.y1[o] = src.y1[i]; dst.y2[o] = src.y2[i]; dst.y3[o] = src.y3[i];
     dst.ux[o] = src.ux[i]; dst.uy[o] = src.uy[i]; dst.uz[o] = src.uz[i];
     dst.tet[o] = src.tet[i];
     dst.id[o] = src.id[i];
-}
-
-// perm[start[cell] + slot] = i: the same placement as k_scatter, as an index.  The parcels
-// stay where the move left them; kernels that walk a cell's parcels read them through perm.
-template <class UT>
-__global__ void __launch_bounds__(256) k_perm(ScatterSet<UT> m) {
-    int sp = 0;
-    while (sp + 1 < m.ns && (int64_t)blockIdx.x >= m.block0[sp + 1]) ++sp;
-    const int64_t i = ((int64_t)blockIdx.x - m.block0[sp]) * blockDim.x + threadIdx.x;
-    if (i >= min(*m.n_live[sp], m.n[sp])) return;
-    const int2 pc = m.place[sp][i];
-    if (pc.x < 0) return;
-    m.perm[sp][(int64_t)m.start[sp][pc.x] + pc.y] = (int)i;
 }
 
 // ---------------------------------------------------------------- instantiation / launch
@@ -529,7 +523,7 @@ hipError_t launch_init(const InitArgs<double>& a, hipStream_t s) { return launch
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s) { return launch_init_t(a, s); }
 
 template <class UT>
-static hipError_t launch_scatter_t(ScatterSet<UT> m, bool index_only, hipStream_t s) {
+static hipError_t launch_scatter_t(ScatterSet<UT> m, hipStream_t s) {
     int64_t blocks = 0;
     for (int k = 0; k < m.ns; ++k) {
         m.block0[k] = blocks;
@@ -537,15 +531,10 @@ static hipError_t launch_scatter_t(ScatterSet<UT> m, bool index_only, hipStream_
     }
     m.block0[m.ns] = blocks;
     if (blocks <= 0) return hipSuccess;
-    if (index_only)
-        hipLaunchKernelGGL((k_perm<UT>), dim3((unsigned)blocks), dim3(256), 0, s, m);
-    else
-        hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)blocks), dim3(256), 0, s, m);
+    hipLaunchKernelGGL((k_scatter<UT>), dim3((unsigned)blocks), dim3(256), 0, s, m);
     return hipGetLastError();
 }
-hipError_t launch_scatter(const ScatterSet<double>& m, hipStream_t s) { return launch_scatter_t(m, false, s); }
-hipError_t launch_scatter(const ScatterSet<float>& m, hipStream_t s) { return launch_scatter_t(m, false, s); }
-hipError_t launch_perm(const ScatterSet<double>& m, hipStream_t s) { return launch_scatter_t(m, true, s); }
-hipError_t launch_perm(const ScatterSet<float>& m, hipStream_t s) { return launch_scatter_t(m, true, s); }
+hipError_t launch_scatter(const ScatterSet<double>& m, hipStream_t s) { return launch_scatter_t(m, s); }
+hipError_t launch_scatter(const ScatterSet<float>& m, hipStream_t s) { return launch_scatter_t(m, s); }
 
 }  // namespace picmi

@@ -84,16 +84,17 @@ struct ScatterSet {
     const int64_t* n_live[kMaxSpecies];
     const int2* place[kMaxSpecies];
     const int* start[kMaxSpecies];
-    int* perm[kMaxSpecies];            // launch_perm: sorted slot -> array index
+    // index[s] != 0: the cell order of species s as an index instead of moved parcels,
+    // perm[start[cell] + slot] = i and id_cell[start[cell] + slot] = id[i] for every live
+    // parcel i (place[i] = (cell, slot)); its src, dst and live count are left alone
+    int index[kMaxSpecies];
+    int* perm[kMaxSpecies];
+    uint32_t* id_cell[kMaxSpecies];
     int64_t block0[kMaxSpecies + 1];
     int ns;
 };
 hipError_t launch_scatter(const ScatterSet<double>& m, hipStream_t s);
 hipError_t launch_scatter(const ScatterSet<float>& m, hipStream_t s);
-// the cell order as an index instead of moved parcels: perm[start[cell] + slot] = i for
-// every live parcel i (place[i] = (cell, slot)); src, dst and the live counts are left alone
-hipError_t launch_perm(const ScatterSet<double>& m, hipStream_t s);
-hipError_t launch_perm(const ScatterSet<float>& m, hipStream_t s);
 
 // ---------------------------------------------------------------- fields (k_field.hip)
 // exclusive scans of n ints (the last count zero, so out[n-1] is the total) for every
@@ -204,7 +205,7 @@ struct CoulombArgs {
     void* ux[kMaxSpecies];
     void* uy[kMaxSpecies];
     void* uz[kMaxSpecies];
-    const uint32_t* id[kMaxSpecies];
+    const uint32_t* id[kMaxSpecies];  // parcel ids in cell order (slot start[cell] + k)
     const double* V;
     double dt;
     uint64_t seed;
@@ -247,6 +248,7 @@ struct MccArgs {
     PartSoA<UT> e;            // electrons, in cell order directly or through perm
     const int* start;         // electron cell_start
     const int* perm;          // sorted slot -> electron index, or null (electrons sorted by cell)
+    const uint32_t* id_cell;  // electron ids in cell order (slot start[cell] + k)
     int64_t* n_e;             // device count of electrons (appends)
     int64_t cap_e;
     PartSoA<UT> ion;          // created ions (may have null pointers when no ion species)

@@ -65,7 +65,7 @@ class pic_mesh(C.Structure):
 
 class pic_species(C.Structure):
     _fields_ = [("mass", C.c_double), ("charge", C.c_double), ("n_equivalent", C.c_double),
-                ("solve_movement", C.c_int32), ("capacity", C.c_int64)]
+                ("solve_movement", C.c_int32), ("capacity", C.c_int64), ("sort_interval", C.c_int32)]
 
 
 class pic_models(C.Structure):
@@ -226,6 +226,7 @@ class Cloud:
             sp[i].n_equivalent = s["nEquivalentParticles"]
             sp[i].solve_movement = 1 if s.get("solveMovement", True) else 0
             sp[i].capacity = int(s["capacity"])
+            sp[i].sort_interval = int(s.get("sortInterval", 0))
         m = pic_models()
         keep = []
         for key, field, default in (("ParticlePusher", "particle_pusher", "Boris"),

@@ -433,14 +433,15 @@ def test_run_equals_repeated_steps(case):
     assert list(st_run.n_parcels) == list(st_steps[-1].n_parcels)
 
 
-@pytest.mark.parametrize("case", ["grouped", "debye", "large-cells", "open-x", "mcc"])
+@pytest.mark.parametrize("case", ["grouped", "ions-only", "debye", "large-cells", "open-x", "mcc"])
 def test_index_sort_equals_moving_sort(case):
     """sortInterval 4 (the collision kernels read the cell order through an index on three
     steps of four) gives the same parcels, bit for bit, and the same step statistics as
     moving the parcels into cell order every step.  Cases: four cells per wave (fixed lnL),
-    one cell per wave with LDS velocity slots (Debye lnL), the large-cell pass (300 parcels
-    per cell and species), parcels deleted at x patches between moving sorts, and MCC with
-    ionization appending parcels."""
+    the same with only the ions on interval 4 (a species' own sortInterval; the electrons
+    move into cell order every step), one cell per wave with LDS velocity slots (Debye lnL),
+    the large-cell pass (300 parcels per cell and species), parcels deleted at x patches
+    between moving sorts, and MCC with ionization appending parcels."""
     say("index sort vs moving sort", case)
     from picfoam_amd import cases, meshgen, picmi
 
@@ -466,7 +467,11 @@ def test_index_sort_equals_moving_sort(case):
             box = ex["box"]
             mesh = meshgen.hex_box(ncell, ncell, ncell, box, box, box, cyclic=(False, True, True),
                                    boundary_type="patch")
-        models["sortInterval"] = interval
+        if case == "ions-only":
+            models["sortInterval"] = 1
+            species[1]["sortInterval"] = interval
+        else:
+            models["sortInterval"] = interval
         c = picmi.Cloud(mesh, species, models)
         c.init_maxwellian(0, ppc, ex["Te_K"], id0=0, seed=11)
         c.init_maxwellian(1, ppc, ex["Ti_K"], id0=0, seed=12)
