@@ -214,8 +214,10 @@ int pic_sync(pic_ctx* ctx);
 int64_t pic_step_index(const pic_ctx* ctx);
 int pic_set_step_index(pic_ctx* ctx, int64_t step);
 
-/* device timing of the stages run since the last reset (HIP events on the context stream) */
-int pic_profile_enable(pic_ctx* ctx, int32_t on);
+/* device timing of the stages run since the last reset (HIP events on the context stream).
+ * stages: bit (1 << PIC_STAGE_x) times stage x, -1 every stage, 0 none.  Each timed stage
+ * adds two event records to the stream (a few microseconds per step and stage) */
+int pic_profile_enable(pic_ctx* ctx, int32_t stages);
 int pic_stage_times(pic_ctx* ctx, double* ms /*[PIC_STAGE_COUNT]*/, int64_t* launches /*[PIC_STAGE_COUNT]*/);
 int pic_profile_reset(pic_ctx* ctx);
 

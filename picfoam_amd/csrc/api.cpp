@@ -137,7 +137,7 @@ struct pic_ctx {
     int poisson_guess = 0;
     int64_t n_solves = 0;
     // profiling
-    bool profile = false;
+    uint32_t profile = 0;                    // bit s: stage s is timed with HIP events
     std::vector<StageEvent> pending;
     std::vector<hipEvent_t> pool;
     double stage_ms[PIC_STAGE_COUNT] = {};
@@ -174,10 +174,10 @@ struct pic_ctx {
         int stage;
         hipEvent_t a = nullptr;
         Scope(pic_ctx* c_, int s) : c(c_), stage(s) {
-            if (c->profile) { a = c->ev(); hipEventRecord(a, c->stream); }
+            if ((c->profile >> s) & 1u) { a = c->ev(); hipEventRecord(a, c->stream); }
         }
         ~Scope() {
-            if (c->profile) {
+            if (a) {
                 hipEvent_t b = c->ev();
                 hipEventRecord(b, c->stream);
                 c->pending.push_back({stage, a, b});
@@ -1441,9 +1441,9 @@ int pic_set_step_index(pic_ctx* c, int64_t s) {
     return PIC_OK;
 }
 
-int pic_profile_enable(pic_ctx* c, int32_t on) {
+int pic_profile_enable(pic_ctx* c, int32_t stages) {
     if (!c) return PIC_ERR_INVALID;
-    c->profile = on != 0;
+    c->profile = (uint32_t)stages;
     return PIC_OK;
 }
 

@@ -435,8 +435,12 @@ class Cloud:
     def step_index(self, v):
         self._check(self.lib.pic_set_step_index(self.h, int(v)))
 
-    def profile(self, on=True):
-        self._check(self.lib.pic_profile_enable(self.h, 1 if on else 0))
+    def profile(self, on=True, stages=None):
+        """time the stages with HIP events (all, or the names in `stages`) from here on"""
+        mask = 0
+        if on:
+            mask = -1 if stages is None else sum(1 << STAGES.index(s) for s in stages)
+        self._check(self.lib.pic_profile_enable(self.h, mask))
         self._check(self.lib.pic_profile_reset(self.h))
 
     def stage_times(self):
