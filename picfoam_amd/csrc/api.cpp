@@ -30,6 +30,7 @@ struct Species {
     int solve_movement = 1;
     int64_t capacity = 0;
     int64_t n = 0;  // host mirror of the parcel count
+    int64_t n_sync = 0;  // the count at the last synchronisation (n can be a launch bound)
     // double-buffered SoA
     void* y[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     void* u[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
@@ -228,8 +229,10 @@ int sync_state(pic_ctx* c) {
     hipMemcpyAsync(n.data(), c->d_nrows, sizeof(int64_t) * 2 * kMaxSpecies, hipMemcpyDeviceToHost, c->stream);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return check_hip(c, e, "stream synchronize");
-    for (size_t s = 0; s < c->sp.size(); ++s)
+    for (size_t s = 0; s < c->sp.size(); ++s) {
         c->sp[s].n = std::min<int64_t>(n[c->sp[s].ncur * kMaxSpecies + s], c->sp[s].capacity);
+        c->sp[s].n_sync = c->sp[s].n;
+    }
     if (flags) {
         hipMemsetAsync(c->d_flags, 0, sizeof(uint32_t), c->stream);
         if (flags & 1u) return c->fail(PIC_ERR_SUPERLUMINAL, "Particle velocity greater than speed of light (299792458)! Aborting...");
@@ -242,6 +245,7 @@ int sync_state(pic_ctx* c) {
 
 void set_dev_n(pic_ctx* c, int s) {
     hipMemcpyAsync(live_n(c, s), &c->sp[s].n, sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+    c->sp[s].n_sync = c->sp[s].n;
 }
 
 int alloc_species(pic_ctx* c, Species& s) {
@@ -537,6 +541,13 @@ int do_mcc_t(pic_ctx* c, double dt) {
     // in the large-cell pass (48 KiB)
     a.maxp = kMccSlots;
     a.maxp_big = kMccSlotsBig;
+    // up to 150 electrons per cell on average (at the last synchronisation), four cells share
+    // a wave with LDS room for 1.5 times the mean plus 40; fuller cells take the large-cell pass
+    const double ppc = (double)E.n_sync / std::max(1, a.n_cells);
+    if (ppc <= 150.0) {
+        a.group = 16;
+        a.maxp_group = std::min(256, ((int)(1.5 * ppc) + 40 + 7) & ~7);
+    }
     a.big_list = c->d_big_list;
     a.big_n = c->d_big_n;
     hipMemsetAsync(c->d_big_n, 0, sizeof(int), c->stream);

@@ -300,6 +300,104 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     mcc_cell(a, cell, a.maxp, lds, picks);
 }
 
+// Several cells per wave: 64 / L cells, L lanes each, for cells of up to a.maxp_group
+// electrons (larger ones go to the large-cell pass).  A cell draws a few candidates per
+// step (about 5 of 100 electrons at the argon case's dt), so one cell per wave leaves most
+// lanes idle while the wave pays the loads, the id ranking and the barriers for that cell
+// alone.  Same candidate rule as mcc_cell, in rounds of L candidates per cell: a parcel
+// still sees its candidates in candidate order, so the results do not depend on L.
+template <class UT, int L>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_mcc_group(MccArgs<UT> a) {
+    constexpr int W = 64 / L;
+    extern __shared__ uint64_t lds[];
+    __shared__ int picks_all[64];
+    const int w = threadIdx.x / L, r = threadIdx.x % L;
+    const int maxp = a.maxp_group;
+    uint64_t* key = lds + (size_t)w * (maxp + maxp / 2);
+    int* val = (int*)(key + maxp);
+    int* picks = picks_all + w * L;
+    const int cell = blockIdx.x * W + w;
+    bool on = cell < a.n_cells;
+    const int a0 = on ? a.start[cell] : 0;
+    const int eC = on ? a.start[cell + 1] - a0 : 0;
+    if (on && eC > maxp) {
+        if (r == 0) a.big_list[atomicAdd(a.big_n, 1)] = cell;
+        on = false;
+    }
+    // candidates of this step: the group's first lane reads and rewrites the remainder
+    double smax = 0.0;
+    int nc = 0;
+    if (on && r == 0) {
+        smax = a.smax[cell];
+        const double coll = a.rem[cell] + eC * a.nN * smax * a.dt;
+        nc = (int)coll;
+        a.rem[cell] = coll - nc;
+    }
+    smax = __shfl(smax, w * L);
+    nc = __shfl(nc, w * L);
+    const int ncw = (eC > 0) ? nc : 0;
+    unsigned c_el = 0, c_ex = 0, c_iz = 0;
+    double qtcr = 0.0;
+    // val[r] = the electron of id rank r (keys (id, cell-local index) are distinct)
+    const int n8 = ncw > 0 ? (eC + 7) & ~7 : 0;
+    for (int k = r; k < n8; k += L)
+        key[k] = (k < eC) ? (((uint64_t)a.id_cell[a0 + k] << 32) | (uint32_t)k) : ~0ull;
+    __syncthreads();
+    for (int k = r; k < (ncw > 0 ? eC : 0); k += L) {
+        const uint64_t kk = key[k];
+        int rk = 0;
+        for (int j = 0; j < n8; j += 8) {
+            const uint64_t* q = key + j;
+            rk += (int)(q[0] < kk) + (int)(q[1] < kk) + (int)(q[2] < kk) + (int)(q[3] < kk) + (int)(q[4] < kk) +
+                  (int)(q[5] < kk) + (int)(q[6] < kk) + (int)(q[7] < kk);
+        }
+        val[rk] = slot_index(a.perm, a0 + k);
+    }
+    __syncthreads();
+    // rounds of L candidates; every group takes part in the barriers of the busiest one
+    int rounds = (ncw + L - 1) / L;
+    for (int off = 32; off > 0; off >>= 1) rounds = max(rounds, __shfl_xor(rounds, off));
+    for (int base = 0; base < rounds * L; base += L) {
+        const int k = base + r;
+        int pick = -1;
+        if (k < ncw) {
+            double upick, uacc;
+            mcc_rand(a.seed, a.step, cell, k, 0, upick, uacc);
+            pick = (int)(upick * eC);
+            if (pick >= eC) pick = eC - 1;
+        }
+        picks[r] = pick;
+        __syncthreads();
+        if (k < ncw) {
+            bool first = true;
+            for (int l = 0; l < r; ++l)
+                if (picks[l] == pick) { first = false; break; }
+            if (first) {
+                const int ei = val[pick];
+                for (int l = r; l < L && base + l < ncw; ++l)
+                    if (picks[l] == pick) candidate(a, cell, base + l, ei, smax, qtcr, c_el, c_ex, c_iz);
+            }
+        }
+        __syncthreads();
+    }
+    for (int off = L / 2; off > 0; off >>= 1) qtcr = fmax(qtcr, __shfl_xor(qtcr, off));
+    if (on && r == 0 && qtcr > smax) a.smax[cell] = qtcr;
+    unsigned v0 = (on && r == 0) ? (unsigned)nc : 0u, v1 = c_el, v2 = c_ex, v3 = c_iz;
+    for (int off = 32; off > 0; off >>= 1) {
+        v0 += __shfl_xor(v0, off);
+        v1 += __shfl_xor(v1, off);
+        v2 += __shfl_xor(v2, off);
+        v3 += __shfl_xor(v3, off);
+    }
+    if (threadIdx.x == 0) {
+        unsigned long long* cnt = a.counts + kNcollStride * (blockIdx.x % kNcollSlots);
+        if (v0) atomicAdd(cnt + 0, (unsigned long long)v0);
+        if (v1) atomicAdd(cnt + 1, (unsigned long long)v1);
+        if (v2) atomicAdd(cnt + 2, (unsigned long long)v2);
+        if (v3) atomicAdd(cnt + 3, (unsigned long long)v3);
+    }
+}
+
 template <class UT>
 __global__ void __launch_bounds__(64) k_mcc_big(MccArgs<UT> a) {
     extern __shared__ uint64_t lds[];
@@ -319,8 +417,13 @@ __global__ void __launch_bounds__(64) k_mcc_big(MccArgs<UT> a) {
 template <class UT>
 static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
     if (a.n_cells <= 0) return hipSuccess;
-    const size_t lds = (size_t)a.maxp * (sizeof(uint64_t) + sizeof(int));
-    hipLaunchKernelGGL((k_mcc<UT>), dim3(a.n_cells), dim3(64), lds, s, a);
+    if (a.group == 16) {
+        const size_t lds = (size_t)4 * (a.maxp_group + a.maxp_group / 2) * sizeof(uint64_t);
+        hipLaunchKernelGGL((k_mcc_group<UT, 16>), dim3((a.n_cells + 3) / 4), dim3(64), lds, s, a);
+    } else {
+        const size_t lds = (size_t)a.maxp * (sizeof(uint64_t) + sizeof(int));
+        hipLaunchKernelGGL((k_mcc<UT>), dim3(a.n_cells), dim3(64), lds, s, a);
+    }
     const size_t lds_big = (size_t)a.maxp_big * (sizeof(uint64_t) + sizeof(int));
     const int nb = a.n_cells < 512 ? a.n_cells : 512;
     hipLaunchKernelGGL((k_mcc_big<UT>), dim3(nb), dim3(64), lds_big, s, a);

@@ -269,6 +269,8 @@ struct MccArgs {
     uint64_t seed;
     int64_t step;
     int maxp;                     // LDS slots (electrons of a cell) of the per-cell pass
+    int group;                    // lanes per cell of the several-cells-per-wave pass, 0 = one cell per wave
+    int maxp_group;               // LDS slots per cell of that pass
     int maxp_big;                 // slots of the large-cell pass
     int* big_list;                // cells left to the large-cell pass, [n_cells]
     int* big_n;                   // their number (zero before the launch)
