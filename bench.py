@@ -150,7 +150,7 @@ def run_steps(args, precision, steps, warmup, profile, sync_steps=0):
     (ms, launches) in the timed pass, dt, synchronous ms per step)."""
     from picfoam_amd import cases, picmi
     mesh, species, models, ex = cases.thermalisation(ncell=args.ncell, ppc=args.ppc, precision=precision)
-    models["sortInterval"] = args.sort_interval
+    models["sortInterval"] = args.sort_interval or 1
     species[1]["sortInterval"] = args.ion_sort_interval
     cloud = picmi.Cloud(mesh, species, models)
     cloud.init_maxwellian(0, args.ppc, ex["Te_K"], id0=0, seed=11)
@@ -253,16 +253,20 @@ def timed(cloud, dt, steps, warmup):
     return 1e3 * el / steps, {k: round(v[0] / steps, 4) for k, v in stages.items() if v[1]}, st, n0
 
 
-def extra_workloads(names, steps, warmup, sort_interval=1):
+def extra_workloads(names, steps, warmup, sort_interval=None):
     """The other full-size configurations, measured the same way (pic_run, device-resident
     inputs): the magnetised electron push on the jittered 50^3x6 tet lattice (50M electrons,
     Higuera-Cary, FP32 push / FP64 track) and the argon-like MCC on a 100^3 hex box (100M
-    electrons, Nanbu e-n with ionization, FP32 push)."""
+    electrons, Nanbu e-n with ionization, FP32 push).  The parcels move into cell order every
+    8 steps on the tet lattice (no collision model needs the order there; it only keeps the
+    move's tet loads local) and every 2 steps on the argon case (index sort in between),
+    unless sort_interval is given."""
     from picfoam_amd import cases, picmi
     out = {}
     if "c3" in names:
         print("bench: magnetised tets, 50^3 x 6 tet lattice, 50M electrons", file=sys.stderr, flush=True)
         mesh, species, models, ex = cases.magnetised_tets(n_lattice=50, n_total=50_000_000, precision="mixed")
+        models["sortInterval"] = sort_interval or 8
         c = picmi.Cloud(mesh, species, models)
         c.set_field(picmi.FIELD_E, ex["E"])
         c.set_field(picmi.FIELD_B, ex["B"])
@@ -271,11 +275,12 @@ def extra_workloads(names, steps, warmup, sort_interval=1):
         c.close()
         out["magnetised_tets"] = {"particle_steps_per_s": n / (ms * 1e-3), "ms_per_step": round(ms, 3),
                                   "stage_ms_per_step": stages, "parcels": n, "cells": mesh["n_cells"],
-                                  "precision": "FP32 push (Higuera-Cary), FP64 track, int64 deposit"}
+                                  "precision": "FP32 push (Higuera-Cary), FP64 track, int64 deposit",
+                                  "sort_interval": models["sortInterval"]}
     if "c4" in names:
         print("bench: argon MCC, 100^3 hex, 100M electrons", file=sys.stderr, flush=True)
         mesh, species, models, ex = cases.argon_mcc(ncell=100, ppc=100, precision="mixed")
-        models["sortInterval"] = sort_interval
+        models["sortInterval"] = sort_interval or 2
         c = picmi.Cloud(mesh, species, models)
         e, el, exs, iz = ex["xs"]
         c.set_cross_section(0, e, el, 0.0)
@@ -290,7 +295,8 @@ def extra_workloads(names, steps, warmup, sort_interval=1):
         out["argon_mcc"] = {"particle_steps_per_s": n / (ms * 1e-3), "ms_per_step": round(ms, 3),
                             "stage_ms_per_step": stages, "electrons_timed_start_end": [n, n_end],
                             "mcc_candidates_last_run": st.mcc_candidates, "ionizations_last_run": st.mcc_ionization,
-                            "precision": "FP32 push (Boris), FP64 track, FP64 MCC"}
+                            "precision": "FP32 push (Boris), FP64 track, FP64 MCC",
+                            "sort_interval": models["sortInterval"]}
     return out
 
 
@@ -302,8 +308,9 @@ def main():
     ap.add_argument("--ncell", type=int, default=32)
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--sort-interval", type=int, default=1,
-                    help="move the parcels into cell order every k steps (index sort in between)")
+    ap.add_argument("--sort-interval", type=int, default=None,
+                    help="move the parcels into cell order every k steps (index sort in between); default 1 "
+                         "for the thermalisation electrons, 8 for the tet lattice, 2 for the argon case")
     ap.add_argument("--ion-sort-interval", type=int, default=8,
                     help="the same for the ions of the thermalisation workload alone (about 0.5 %% of "
                          "them change cell per step; the electrons, 30 %%, stay at --sort-interval)")
@@ -361,7 +368,7 @@ def main():
         "config": {"workload": f"two-species thermalisation, {args.ncell}^3 cyclic hex, {args.ppc}+{args.ppc} ppc "
                                f"({n_total} parcels), Boris + Nanbu/TakizukaAbe Coulomb (lnL 10), no field solve",
                    "parcels": n_total, "dt": dt, "precision": args.precision,
-                   "sort_interval": {"electrons": args.sort_interval,
+                   "sort_interval": {"electrons": args.sort_interval or 1,
                                      "ions": args.ion_sort_interval}},
         "roofline": roof,
         "stage_ms_per_step": stage_ms,
