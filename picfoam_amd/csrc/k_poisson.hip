@@ -122,7 +122,7 @@ __device__ __forceinline__ int nblk_d(int n) {
 }
 
 // scalar slots of W.scal
-enum : int { kRz = 0, kPq = 1, kAlpha = 2, kBeta = 4, kRR = 5, kBB = 6, kMean = 7, kDone = 8, kConvIt = 9, kTol2 = 10 };
+enum : int { kRR = 5, kBB = 6, kMean = 7, kDone = 8, kConvIt = 9, kTol2 = 10, kRzPar = 11 /* and 12 */ };
 // reduction segments of W.red (kRedBlocks doubles each): 0 p.q, 1 r.r, 2 spare, 3 + c r.z of colour c
 constexpr int kSegPq = 0, kSegRr = 1, kSegRz0 = 3;
 
@@ -133,65 +133,62 @@ __device__ __forceinline__ double sum_partials(const double* partial, int nb, do
     return block_sum(t, sh);
 }
 
-// The scalars of an iteration are finished by one-block kernels after the grid kernels:
-// a grid-wide "last block" finish would need a device-scope fence per block, which
-// writes back the XCD's L2 and costs more than the extra launch.
+// The scalars of an iteration (alpha, beta, the convergence test) are finished inside the
+// grid kernel that needs them: every block sums the previous kernel's partials itself, in
+// the same fixed order, so all blocks hold the same value bit for bit and no one-block
+// finishing launch (about 4 us plus its gap, three per iteration) sits between the grid
+// kernels.  A "last block" finish inside the producing kernel would need a device-scope
+// fence per block, which writes back the XCD's L2.  Block 0 stores what the host or a later
+// kernel reads: r.z by iteration parity (the kernel that stores it still reads the previous
+// one), r.r and the convergence flag.
 
-// alpha = rz / p.q
-__global__ void k_fin_alpha(double* scal, const double* red, int nb) {
-    __shared__ double sh[4];
-    if (scal[kDone] != 0.0) return;
-    const double pq = sum_partials(red + kSegPq * kRedBlocks, nb, sh);
-    if (threadIdx.x == 0) { scal[kPq] = pq; scal[kAlpha] = scal[kRz] / pq; }
+// sum_partials given to every thread of the block
+__device__ __forceinline__ double all_sum(const double* partial, int nb, double* sh) {
+    const double t = sum_partials(partial, nb, sh);
+    if (threadIdx.x == 0) sh[4] = t;
+    __syncthreads();
+    const double v = sh[4];
+    __syncthreads();
+    return v;
 }
 
-// r.r and the convergence flag (it > 0); with one colour also r.z -> beta (zero at the
-// start of a solve, it = 0)
-__global__ void k_fin_rr(double* scal, const double* red, int nb, int rz_here, int it) {
-    __shared__ double sh[4];
-    if (scal[kDone] != 0.0) return;
-    double rr = 0.0, rz = 0.0;
-    if (it > 0) rr = sum_partials(red + kSegRr * kRedBlocks, nb, sh);
-    if (rz_here) rz = sum_partials(red + kSegRz0 * kRedBlocks, nb, sh);
-    if (threadIdx.x == 0) {
-        if (it > 0) {
-            scal[kRR] = rr;
-            if (rr <= scal[kTol2]) { scal[kDone] = 1.0; scal[kConvIt] = (double)it; }
-        }
-        if (rz_here) { scal[kBeta] = it > 0 ? rz / scal[kRz] : 0.0; scal[kRz] = rz; }
+// r.r of the last update against the tolerance: true once converged (block 0 then records
+// the iteration for the host); r.r is kept for the host either way
+__device__ __forceinline__ bool converged(double* scal, const double* red, int nb_rr, int it, double* sh) {
+    const double rr = all_sum(red + kSegRr * kRedBlocks, nb_rr, sh);
+    const bool done = rr <= scal[kTol2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[kRR] = rr;
+        if (done) { scal[kDone] = 1.0; scal[kConvIt] = (double)it; }
     }
-}
-
-// r.z over every colour's partials -> beta (zero at the start of a solve)
-__global__ void k_fin_rz(double* scal, const double* red, int ncol, const int32_t* col_start, int init) {
-    __shared__ double sh[4];
-    if (scal[kDone] != 0.0) return;
-    double rz = 0.0;
-    for (int c = 0; c < ncol; ++c) {
-        const double t = sum_partials(red + (size_t)(kSegRz0 + c) * kRedBlocks, nblk_d(col_start[c + 1] - col_start[c]), sh);
-        if (threadIdx.x == 0) rz += t;
-    }
-    if (threadIdx.x == 0) {
-        scal[kBeta] = init ? 0.0 : rz / scal[kRz];
-        scal[kRz] = rz;
-    }
+    return done;
 }
 
 __global__ void k_pcg_reset(double* scal, double tol2) {
     if (threadIdx.x == 0) {
-        scal[kDone] = 0.0; scal[kConvIt] = -1.0; scal[kTol2] = tol2; scal[kBeta] = 0.0;
+        scal[kDone] = 0.0; scal[kConvIt] = -1.0; scal[kTol2] = tol2;
     }
 }
 
-// p_new = z + beta p_old, q = A p_new (neighbours' p formed on the fly), p.q partials
+// iteration it (from 1): r.z of the current z (every colour's partials), beta = r.z /
+// r.z of the previous iteration (0 at it = 1); p_new = z + beta p_old, q = A p_new
+// (neighbours' p formed on the fly), p.q partials.  rr_test (one colour, so no sweep kernel
+// follows the update): the convergence test of iteration it - 1 comes first.
 template <class T>
 __global__ void __launch_bounds__(256) k_pcg_spmv(int n, int K, const int32_t* __restrict__ nbT,
                                                   const T* __restrict__ aT, const T* __restrict__ diag,
                                                   const T* __restrict__ z, const T* __restrict__ pold,
-                                                  T* __restrict__ pnew, T* __restrict__ q, double* scal, double* red) {
-    __shared__ double sh[4];
+                                                  T* __restrict__ pnew, T* __restrict__ q, double* scal, double* red,
+                                                  int it, int ncol, const int32_t* __restrict__ col_start,
+                                                  int rr_test) {
+    __shared__ double sh[8];
     if (scal[kDone] != 0.0) return;
-    const T beta = (T)scal[kBeta];
+    if (rr_test && it > 1 && converged(scal, red, gridDim.x, it - 1, sh)) return;
+    double rz = 0.0;
+    for (int c = 0; c < ncol; ++c)
+        rz += all_sum(red + (size_t)(kSegRz0 + c) * kRedBlocks, nblk_d(col_start[c + 1] - col_start[c]), sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[kRzPar + (it & 1)] = rz;
+    const T beta = (T)(it == 1 ? 0.0 : rz / scal[kRzPar + ((it - 1) & 1)]);
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const T pi = z[i] + beta * pold[i];
@@ -214,10 +211,11 @@ template <class T>
 __global__ void __launch_bounds__(256) k_pcg_update(int n, int c0end, int rz_here, T* __restrict__ x, T* __restrict__ r,
                                                     const T* __restrict__ p, const T* __restrict__ q,
                                                     const T* __restrict__ rD, T* __restrict__ z, double* scal,
-                                                    double* red) {
-    __shared__ double sh[4];
+                                                    double* red, int it) {
+    __shared__ double sh[8];
     if (scal[kDone] != 0.0) return;
-    const T alpha = (T)scal[kAlpha];
+    // alpha = r.z / p.q of this iteration (p.q from the spmv's partials, same grid)
+    const T alpha = (T)(scal[kRzPar + (it & 1)] / all_sum(red + kSegPq * kRedBlocks, gridDim.x, sh));
     double acc = 0.0, acc2 = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
@@ -259,14 +257,17 @@ __global__ void __launch_bounds__(256) k_pcg_zc0(int n, int c0end, int rz_here, 
 }
 
 // DIC forward sweep of colour range [c0, c1): z_i = (r_i + sum_{j < c0} a_ij z_j) / rD_i;
-// the last colour's z is final there, so its r.z partials are stored (segment seg)
+// the last colour's z is final there, so its r.z partials are stored (segment seg).  The
+// first sweep after an update (it > 0) makes the convergence test of iteration it on the
+// update's nb_rr partials.
 template <class T>
 __global__ void __launch_bounds__(256) k_pcg_fwd(int n, int c0, int c1, int K, const int32_t* __restrict__ nbT,
                                                  const T* __restrict__ aT, const T* __restrict__ rD,
                                                  const T* __restrict__ r, T* __restrict__ z, double* scal, double* red,
-                                                 int seg) {
-    __shared__ double sh[4];
+                                                 int seg, int it, int nb_rr) {
+    __shared__ double sh[8];
     if (scal[kDone] != 0.0) return;
+    if (it > 0 && converged(scal, red, nb_rr, it, sh)) return;
     double acc = 0.0;
     for (int i = c0 + blockIdx.x * blockDim.x + threadIdx.x; i < c1; i += gridDim.x * blockDim.x) {
         T s = r[i];
@@ -290,7 +291,7 @@ __global__ void __launch_bounds__(256) k_pcg_bwd(int n, int c0, int c1, int K, c
                                                  const T* __restrict__ aT, const T* __restrict__ rD,
                                                  const T* __restrict__ r, T* __restrict__ z, double* scal, double* red,
                                                  int seg) {
-    __shared__ double sh[4];
+    __shared__ double sh[8];
     if (scal[kDone] != 0.0) return;
     double acc = 0.0;
     for (int i = c0 + blockIdx.x * blockDim.x + threadIdx.x; i < c1; i += gridDim.x * blockDim.x) {
@@ -360,24 +361,21 @@ int pcg_fused(const PoissonDev& P, const Sys<T>& S, PoissonWork& W, T* x, T* r, 
     const int c0end = cs[1];
     const int rz_here = ncol == 1 ? 1 : 0;
     hipLaunchKernelGGL(k_pcg_reset, dim3(1), dim3(64), 0, s, W.scal, tol_abs * tol_abs);
-    // z = M^-1 r, r.z, beta = 0; p_old = 0
+    // z = M^-1 r and its r.z partials (the first spmv forms beta = 0); p_old = 0
     hipLaunchKernelGGL((k_pcg_zc0<T>), dim3(nb), dim3(256), 0, s, n, c0end, rz_here, r, S.rD, z, p2, W.red);
-    auto sweeps = [&](int init) {
+    auto sweeps = [&](int it) {
         for (int c = 1; c < ncol; ++c) {
             const int a0 = cs[c], cnt = cs[c + 1] - a0;
             hipLaunchKernelGGL((k_pcg_fwd<T>), dim3(nblk(cnt)), dim3(256), 0, s, n, a0, cs[c + 1], K, P.nbT, S.aT,
-                               S.rD, r, z, W.scal, W.red, c == ncol - 1 ? c : -1);
+                               S.rD, r, z, W.scal, W.red, c == ncol - 1 ? c : -1, c == 1 ? it : 0, nb);
         }
         for (int c = ncol - 2; c >= 0; --c) {
             const int a0 = cs[c], cnt = cs[c + 1] - a0;
             hipLaunchKernelGGL((k_pcg_bwd<T>), dim3(nblk(cnt)), dim3(256), 0, s, n, a0, cs[c + 1], K, P.nbT, S.aT,
                                S.rD, r, z, W.scal, W.red, c);
         }
-        if (ncol > 1)
-            hipLaunchKernelGGL(k_fin_rz, dim3(1), dim3(256), 0, s, W.scal, W.red, ncol, P.col_start_d, init);
     };
-    if (rz_here) hipLaunchKernelGGL(k_fin_rr, dim3(1), dim3(256), 0, s, W.scal, W.red, nb, 1, 0);
-    sweeps(1);
+    sweeps(0);
     T* pa = p;    // written this iteration
     T* pb = p2;   // previous direction
     int it = 0, batch = 8, conv = -1;
@@ -387,12 +385,10 @@ int pcg_fused(const PoissonDev& P, const Sys<T>& S, PoissonWork& W, T* x, T* r, 
         for (int k = 0; k < B; ++k) {
             ++it;
             hipLaunchKernelGGL((k_pcg_spmv<T>), dim3(nb), dim3(256), 0, s, n, K, P.nbT, S.aT, S.diag, z, pb, pa, q,
-                               W.scal, W.red);
-            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, s, W.scal, W.red, nb);
+                               W.scal, W.red, it, ncol, P.col_start_d, rz_here);
             hipLaunchKernelGGL((k_pcg_update<T>), dim3(nb), dim3(256), 0, s, n, c0end, rz_here, x, r, pa, q, S.rD, z,
-                               W.scal, W.red);
-            hipLaunchKernelGGL(k_fin_rr, dim3(1), dim3(256), 0, s, W.scal, W.red, nb, rz_here, it);
-            sweeps(0);
+                               W.scal, W.red, it);
+            sweeps(it);
             std::swap(pa, pb);
         }
         hipMemcpyAsync(h, W.scal + kDone, sizeof(double) * 2, hipMemcpyDeviceToHost, s);
