@@ -81,7 +81,7 @@ struct pic_ctx {
     int64_t step = 0;
     bool rotate = false;
     bool e_zero = true;                      // E is zero in every cell (never set, no field solve)
-    bool sorted = false, counts_valid = false;
+    bool counts_valid = false;
     bool place_valid = false;                // Species::place matches the current layout
     bool big_n_zero = false;                 // d_big_n zeroed since its last use
     // species
@@ -347,7 +347,6 @@ int do_move(pic_ctx* c, double dt) {
     if (int r = c->mixed() ? do_move_t<float>(c, dt) : do_move_t<double>(c, dt)) return r;
     c->counts_valid = true;
     c->place_valid = true;
-    c->sorted = false;
     for (auto& S : c->sp) S.indexed = false;
     return PIC_OK;
 }
@@ -425,15 +424,8 @@ int do_sort(pic_ctx* c, bool in_step = false) {
         S.cur = 1 - S.cur;
         S.ncur = 1 - S.ncur;   // the sorted count is the live one from here on
     }
-    c->sorted = true;
     if (moved) c->place_valid = false;   // the moved species' slots refer to the layout before this sort
     return PIC_OK;
-}
-
-int pow2_at_least(int v) {
-    int p = 64;
-    while (p < v) p <<= 1;
-    return p;
 }
 
 int do_coulomb(pic_ctx* c, double dt) {
@@ -1143,7 +1135,6 @@ int pic_particles_set(pic_ctx* c, int32_t s, int64_t n, const double* bary, cons
     hipMemcpy(S.id[S.cur], id, sizeof(uint32_t) * n, hipMemcpyHostToDevice);
     S.n = n;
     set_dev_n(c, s);
-    c->sorted = false;
     for (auto& S : c->sp) S.indexed = false;
     c->counts_valid = false;
     return check_hip(c, hipStreamSynchronize(c->stream), "particles set");
@@ -1182,7 +1173,6 @@ int pic_particles_set_positions(pic_ctx* c, int32_t s, int64_t n, const double* 
     if (int r = check_hip(c, e, "locate")) return r;
     S.n = n;
     set_dev_n(c, s);
-    c->sorted = false;
     for (auto& S : c->sp) S.indexed = false;
     c->counts_valid = false;
     if (fails) return c->fail(PIC_ERR_INVALID, std::to_string(fails) + " positions lie outside their given cell");
@@ -1258,7 +1248,6 @@ int pic_init_maxwellian(pic_ctx* c, int32_t s, int32_t nppc, double T, const dou
     if (int r = check_hip(c, e, "init")) return r;
     S.n += add;
     set_dev_n(c, s);
-    c->sorted = false;
     for (auto& S : c->sp) S.indexed = false;
     c->counts_valid = false;
     return check_hip(c, hipStreamSynchronize(c->stream), "init");
