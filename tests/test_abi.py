@@ -1,11 +1,9 @@
 """The C-ABI library loads, exports every entry point include/picmi.h declares, and
 rejects invalid model selections the way the reference's selectors do (CPU only: no
 compute call is made)."""
-import ctypes as C
 import os
 import re
 
-import numpy as np
 import pytest
 
 from helpers import small_hex

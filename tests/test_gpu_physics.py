@@ -250,7 +250,6 @@ def test_c4_mcc_drift_and_ionization(oracle):
     assert v64[-1] < 0  # electrons drift against E
 
     # GPU FP64 against the CPU oracle over a few full steps of the same initial state
-    from oracle.picref import RefMesh
     c, ex = _c4_setup("fp64")
     y0, t0, U0, i0 = c.get_particles(0)
     rm = oracle.RefMesh(c.mesh.d)

@@ -7,7 +7,7 @@ mistake (dropped term, sign, transposed index) fails.
 import numpy as np
 import pytest
 
-from helpers import C_LIGHT, EPS0, KB, ME, MP, QE, periodic_diff, random_bary, random_parcels, small_hex, small_tet
+from helpers import C_LIGHT, EPS0, ME, MP, QE, periodic_diff, random_bary, random_parcels, small_hex, small_tet
 
 
 # ------------------------------------------------------------------ Philox
