@@ -446,13 +446,14 @@ def test_run_equals_repeated_steps(case):
     assert list(st_run.n_parcels) == list(st_steps[-1].n_parcels)
 
 
-@pytest.mark.parametrize("case", ["grouped", "ions-only", "debye", "large-cells", "open-x", "mcc"])
+@pytest.mark.parametrize("case", ["grouped", "ions-only", "ions-only-mixed", "debye", "large-cells", "open-x", "mcc"])
 def test_index_sort_equals_moving_sort(case):
     """sortInterval 4 (the collision kernels read the cell order through an index on three
     steps of four) gives the same parcels, bit for bit, and the same step statistics as
     moving the parcels into cell order every step.  Cases: four cells per wave (fixed lnL),
     the same with only the ions on interval 4 (a species' own sortInterval; the electrons
-    move into cell order every step), one cell per wave with LDS velocity slots (Debye lnL),
+    move into cell order every step), in FP64 and in mixed precision (FP32 velocities), one
+    cell per wave with LDS velocity slots (Debye lnL),
     the large-cell pass (300 parcels per cell and species), parcels deleted at x patches
     between moving sorts, and MCC with ionization appending parcels."""
     say("index sort vs moving sort", case)
@@ -475,12 +476,13 @@ def test_index_sort_equals_moving_sort(case):
         ncell = 2 if case == "large-cells" else 4
         dt = cases.thermalisation(ncell=ncell, ppc=ppc)[3]["dt"]   # lnL 0 has no Spitzer rate to set dt
         lnl = 0.0 if case == "debye" else 10.0
-        mesh, species, models, ex = cases.thermalisation(ncell=ncell, ppc=ppc, coulomb_log=lnl, dt=dt)
+        prec = "mixed" if case == "ions-only-mixed" else "fp64"
+        mesh, species, models, ex = cases.thermalisation(ncell=ncell, ppc=ppc, coulomb_log=lnl, dt=dt, precision=prec)
         if case == "open-x":
             box = ex["box"]
             mesh = meshgen.hex_box(ncell, ncell, ncell, box, box, box, cyclic=(False, True, True),
                                    boundary_type="patch")
-        if case == "ions-only":
+        if case.startswith("ions-only"):
             models["sortInterval"] = 1
             species[1]["sortInterval"] = interval
         else:
