@@ -122,40 +122,80 @@ __global__ void k_mean(int n, int K, const int32_t* __restrict__ nb, const uint1
     *(uint64_t*)(m + (int64_t)i * 64 + c0) = o;
 }
 
-// z = [h | m] W + b ; h_out = h + relu(z).  One wave per 16 nodes; MFMA 16x16x16 BF16.
+// z = [h | m] W + b ; h_out = h + relu(z).  MFMA 16x16x16 BF16 on tiles of 16 nodes.
 // Fragment layout (v_mfma_f32_16x16x16_bf16): lane l holds A[l%16][4(l/16)..+3],
 // B[4(l/16)..+3][l%16] and D[4(l/16)+r][l%16], r = 0..3.
+// Persistent waves: a wave loads its 32 B fragments of W (8 k-blocks x 4 column tiles) once
+// and keeps them in registers for every tile it takes, instead of reading W (16 KiB) again
+// per tile.  The tile's pre-activations z (FP32) go through LDS, so that h, z and h_out
+// move as 16-byte pieces, 16 consecutive channels of one node per lane.  Every block runs
+// the same number of tile rounds, so its barriers are reached by all of its waves.
+constexpr int kLayerBlocks = 1024;
 __global__ void __launch_bounds__(256) k_layer_fwd(int n, const uint16_t* __restrict__ h, const uint16_t* __restrict__ m,
                                                    const uint16_t* __restrict__ WT /*[64][128]*/,
                                                    const float* __restrict__ b, uint16_t* __restrict__ z,
                                                    uint16_t* __restrict__ hout) {
+    __shared__ float tile[4][16][65];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int n0 = (blockIdx.x * 4 + wave) * 16;
-    if (n0 >= n) return;
     const int row = lane & 15, kq = lane >> 4;
-    const int node = n0 + row;
-    f4 acc[4];
-    for (int ct = 0; ct < 4; ++ct) acc[ct] = (f4){0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < 8; ++kb) {
-        const uint16_t* src = kb < 4 ? h : m;
-        const int k0 = (kb & 3) * 16 + kq * 4;
-        s4 a = (s4){0, 0, 0, 0};
-        if (node < n) a = *(const s4*)(src + (int64_t)node * 64 + k0);
-        for (int ct = 0; ct < 4; ++ct) {
-            const int col = ct * 16 + row;
-            const s4 bv = *(const s4*)(WT + col * 128 + kb * 16 + kq * 4);
-            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[ct], 0, 0, 0);
+    s4 bv[8][4];
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) bv[kb][ct] = *(const s4*)(WT + (ct * 16 + row) * 128 + kb * 16 + kq * 4);
+    float bias[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) bias[ct] = b[ct * 16 + row];
+    float(*tw)[65] = tile[wave];
+    const int tiles = (n + 15) / 16;
+    const int rounds = (tiles + 4 * (int)gridDim.x - 1) / (4 * (int)gridDim.x);
+    const int ln = lane >> 2, q = lane & 3;   // epilogue: node ln of the tile, channels 16 q ..
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int t = (rd * (int)gridDim.x + (int)blockIdx.x) * 4 + wave;
+        const int n0 = t * 16;
+        if (t < tiles) {
+            const int node = n0 + row;
+            f4 acc[4];
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb) {
+                const uint16_t* src = kb < 4 ? h : m;
+                const int k0 = (kb & 3) * 16 + kq * 4;
+                s4 a = (s4){0, 0, 0, 0};
+                if (node < n) a = *(const s4*)(src + (int64_t)node * 64 + k0);
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[kb][ct], acc[ct], 0, 0, 0);
+            }
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tw[kq * 4 + r][ct * 16 + row] = acc[ct][r] + bias[ct];
         }
+        __syncthreads();
+        const int nd = n0 + ln;
+        if (t < tiles && nd < n) {
+            const int64_t o = (int64_t)nd * 64 + q * 16;
+            const uint4 h0 = *(const uint4*)(h + o), h1 = *(const uint4*)(h + o + 8);
+            const uint32_t hw[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+            uint32_t zo[8], ho[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float z0 = tw[ln][q * 16 + 2 * c], z1 = tw[ln][q * 16 + 2 * c + 1];
+                zo[c] = (uint32_t)f2bf(z0) | ((uint32_t)f2bf(z1) << 16);
+                const float o0 = bf2f((uint16_t)(hw[c] & 0xffffu)) + fmaxf(z0, 0.f);
+                const float o1 = bf2f((uint16_t)(hw[c] >> 16)) + fmaxf(z1, 0.f);
+                ho[c] = (uint32_t)f2bf(o0) | ((uint32_t)f2bf(o1) << 16);
+            }
+            *(uint4*)(hout + o) = make_uint4(ho[0], ho[1], ho[2], ho[3]);
+            *(uint4*)(hout + o + 8) = make_uint4(ho[4], ho[5], ho[6], ho[7]);
+            if (z) {
+                *(uint4*)(z + o) = make_uint4(zo[0], zo[1], zo[2], zo[3]);
+                *(uint4*)(z + o + 8) = make_uint4(zo[4], zo[5], zo[6], zo[7]);
+            }
+        }
+        __syncthreads();
     }
-    for (int ct = 0; ct < 4; ++ct)
-        for (int r = 0; r < 4; ++r) {
-            const int nd = n0 + kq * 4 + r, col = ct * 16 + row;
-            if (nd >= n) continue;
-            const float zz = acc[ct][r] + b[col];
-            const int64_t o = (int64_t)nd * 64 + col;
-            if (z) z[o] = f2bf(zz);
-            hout[o] = f2bf(bf2f(h[o]) + fmaxf(zz, 0.f));
-        }
 }
 
 __global__ void k_decode(int n, const uint16_t* __restrict__ h, const float* __restrict__ w, const float* bdec,
@@ -499,7 +539,7 @@ hipError_t Gnn::forward(const float* xin, bool train) {
         uint16_t* ml = mm + (train ? (size_t)l * NH : 0);
         hipLaunchKernelGGL(k_mean, dim3((unsigned)((n * 16 + 255) / 256)), dim3(256), 0, stream, n, K, nb, hl, ml);
         const float* bl = lay + l * (8192 + 64) + 8192;
-        hipLaunchKernelGGL(k_layer_fwd, dim3((tiles + 3) / 4), dim3(256), 0, stream, n, hl, ml,
+        hipLaunchKernelGGL(k_layer_fwd, dim3(std::min((tiles + 3) / 4, kLayerBlocks)), dim3(256), 0, stream, n, hl, ml,
                            wbf + (size_t)l * 2 * 8192 + 8192, bl, train ? z + (size_t)(l + 1) * NH : nullptr, hn);
     }
     const uint16_t* hL = h + (train ? (size_t)L * NH : (L % 2) * NH);
