@@ -456,21 +456,24 @@ int do_coulomb(pic_ctx* c, double dt) {
     a.dt = dt;
     a.seed = c->seed;
     a.step = c->step;
-    // cells of up to kCoulombSlots parcels of a species in the one-cell-per-wave pass (index
-    // lists and velocities in LDS), kCoulombSlotsBig in the large-cell pass.  With fixed
-    // Coulomb logarithms and few parcels per cell, 4 (or 2) cells share a wave, up to 128
-    // (256) parcels of a species each; the mean population of the fullest species decides.
+    // cells of up to kCoulombSlots parcels of a species in the one-cell-per-wave pass with
+    // the stage velocities in LDS (group 0), kCoulombSlotsBig in the large-cell pass.  With
+    // fixed Coulomb logarithms, 64 / L cells share a wave, L lanes and up to 8 L parcels of a
+    // species each (L = 16, 32, 64); the mean population of the fullest species at the last
+    // synchronisation decides.
     a.maxp = kCoulombSlots;
     a.maxp_big = kCoulombSlotsBig;
-    a.group = 64;
+    a.group = 0;
     int64_t most = 0;
     for (int i = 0; i < ns; ++i)
-        if (c->sp[i].charge != 0.0) most = std::max<int64_t>(most, c->sp[i].n);
+        if (c->sp[i].charge != 0.0) most = std::max<int64_t>(most, c->sp[i].n_sync);
     const double ppc = (double)most / std::max(1, a.n_cells);
     if (!calc_debye && ppc <= 80.0) {
         a.group = 16; a.maxp = 128;
     } else if (!calc_debye && ppc <= 180.0) {
         a.group = 32; a.maxp = 256;
+    } else if (!calc_debye && ppc <= 360.0) {
+        a.group = 64; a.maxp = 512;
     }
     a.big_list = c->d_big_list;
     a.big_n = c->d_big_n;

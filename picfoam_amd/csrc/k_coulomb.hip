@@ -749,7 +749,7 @@ __global__ void __launch_bounds__(64) k_coulomb_big(CoulombArgs a) {
 }
 
 // ---------------------------------------------------------------- several cells per wave
-// 64 / L cells per wave, L lanes per cell.  A stage's collision units are spread over the
+// 64 / L cells per wave, L lanes per cell (L = 16, 32 or 64).  A stage's collision units are spread over the
 // cell's L lanes, so the short sequential runs that end a stage (a Takizuka-Abe odd triple,
 // parcels of the smaller species meeting two partners) share the wave's passes with the
 // other cells' collisions instead of holding all 64 lanes, and the stage tables of the
@@ -765,7 +765,7 @@ template <int L>
 __device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, const int* perm,
                            int first, int n, uint64_t* key, int* bins, int* val, int r) {
     constexpr int KM = 8, NB = 4 * L, PER = NB / L;
-    constexpr int SH = 64 - (L == 16 ? 6 : 7);   // top log2(NB) bits pick the bin
+    constexpr int SH = 64 - (L == 16 ? 6 : (L == 32 ? 7 : 8));   // top log2(NB) bits pick the bin
     for (int b = r; b < NB; b += L) bins[b] = 0;
     __syncthreads();
     uint64_t kk[KM];
@@ -956,6 +956,8 @@ static void launch_coulomb_t(const CoulombArgs& a, hipStream_t s) {
         launch_coulomb_group<UT, MODEL, WC, 16>(a, s);
     else if (a.group == 32)
         launch_coulomb_group<UT, MODEL, WC, 32>(a, s);
+    else if (a.group == 64)
+        launch_coulomb_group<UT, MODEL, WC, 64>(a, s);
     else if (MODEL == 1)
         hipLaunchKernelGGL((k_coulomb_perez<UT, WC>), dim3(a.n_cells), dim3(64), lds, s, a);
     else

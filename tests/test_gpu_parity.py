@@ -182,12 +182,14 @@ COULOMB_VARIANTS = [
 
 
 @pytest.mark.parametrize("variant", COULOMB_VARIANTS, ids=lambda v: "-".join(str(x) for x in v))
-@pytest.mark.parametrize("counts", ["equal", "ragged", "dense"])
+@pytest.mark.parametrize("counts", ["equal", "ragged", "dense", "denser"])
 def test_coulomb_fp64_parity(oracle, counts, variant):
     """GPU collisions equal the oracle's to 1e-12.  With fixed Coulomb logarithms the cells
     of "equal" and "ragged" (mean below 80 parcels per species) run four to a wave, those of
-    "dense" two to a wave; the computed logarithm runs one cell per wave; the 300-parcel
-    cell of "ragged" goes to the large-cell pass."""
+    "dense" two to a wave, those of "denser" (mean about 330) one to a wave with 64 lanes
+    (up to 512 parcels of a species); the computed logarithm runs one cell per wave; the
+    300-parcel cell of "ragged" and the 600-parcel cell of "denser" go to the large-cell
+    pass."""
     model, pairing, wratio, wcorr = variant
     say("coulomb parity", counts, variant)
     rng = np.random.default_rng(104)
@@ -199,6 +201,10 @@ def test_coulomb_fp64_parity(oracle, counts, variant):
     elif counts == "dense":
         ne = rng.integers(90, 230, nc); ni = rng.integers(90, 230, nc)
         ne[0], ni[0] = 251, 249
+    elif counts == "denser":
+        ne = rng.integers(150, 420, nc); ni = rng.integers(150, 420, nc)
+        ne[0], ni[0] = 512, 509
+        ne[1], ni[1] = 600, 200
     else:
         ne = rng.integers(0, 90, nc); ni = rng.integers(0, 90, nc)
         ne[0], ni[0] = 7, 3
