@@ -349,9 +349,9 @@ def main():
     b = {"move": move_bytes, "coulomb": coul_bytes}[dom]
     t_ms = dom_times[0] / max(dom_times[1], 1)   # the dominant stage's launches in the timed pass
     achieved = b / (t_ms * 1e-3) / 1e9
-    # with a fixed Coulomb logarithm, cells of up to 360 parcels per species on average run in
+    # with a fixed Coulomb logarithm, cells of up to 420 parcels per species on average run in
     # the grouped kernel (four cells per wave at 64 + 64), denser ones one per wave
-    coul_kernel = "k_coulomb_group_nanbu" if args.ppc <= 360 else "k_coulomb_nanbu"
+    coul_kernel = "k_coulomb_group_nanbu" if args.ppc <= 420 else "k_coulomb_nanbu"
     kname = {"move": "k_move", "coulomb": coul_kernel}[dom]
     default_workload = args.ncell == 32 and args.ppc == 64 and args.precision == "fp64"
     traffic = profiled_traffic(kname) if default_workload else None

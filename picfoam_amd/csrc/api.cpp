@@ -456,8 +456,9 @@ int do_coulomb(pic_ctx* c, double dt) {
     a.dt = dt;
     a.seed = c->seed;
     a.step = c->step;
-    // cells of up to kCoulombSlots parcels of a species in the one-cell-per-wave pass with
-    // the stage velocities in LDS (group 0), kCoulombSlotsBig in the large-cell pass.  With
+    // cells of up to kCoulombSlots (256, 512 in denser plasmas) parcels of a species in the
+    // one-cell-per-wave pass with the stage velocities in LDS (group 0), kCoulombSlotsBig in
+    // the large-cell pass.  With
     // fixed Coulomb logarithms, 64 / L cells share a wave, L lanes and up to 8 L parcels of a
     // species each (L = 16, 32, 64); the mean population of the fullest species at the last
     // synchronisation decides.
@@ -472,8 +473,13 @@ int do_coulomb(pic_ctx* c, double dt) {
         a.group = 16; a.maxp = 128;
     } else if (!calc_debye && ppc <= 180.0) {
         a.group = 32; a.maxp = 256;
-    } else if (!calc_debye && ppc <= 360.0) {
+    } else if (!calc_debye && ppc <= 420.0) {
+        // 512 slots stay 5 standard deviations above a mean of 420 (Poisson counts)
         a.group = 64; a.maxp = 512;
+    } else if (ppc > 80.0) {
+        // the one-cell-per-wave pass sized for denser cells (LDS lists and velocity slots of
+        // 256 or 512 parcels) rather than sending every cell to the large-cell pass
+        a.maxp = ppc <= 180.0 ? 256 : 512;
     }
     a.big_list = c->d_big_list;
     a.big_n = c->d_big_n;
