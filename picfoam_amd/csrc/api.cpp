@@ -542,12 +542,16 @@ int do_mcc_t(pic_ctx* c, double dt) {
     // in the large-cell pass (48 KiB)
     a.maxp = kMccSlots;
     a.maxp_big = kMccSlotsBig;
-    // up to 150 electrons per cell on average (at the last synchronisation), four cells share
-    // a wave with LDS room for 1.5 times the mean plus 40; fuller cells take the large-cell pass
+    // up to 150 (300) electrons per cell on average at the last synchronisation, four (two)
+    // cells share a wave with LDS room for 1.5 times the mean plus 40; fuller cells take the
+    // large-cell pass
     const double ppc = (double)E.n_sync / std::max(1, a.n_cells);
     if (ppc <= 150.0) {
         a.group = 16;
         a.maxp_group = std::min(256, ((int)(1.5 * ppc) + 40 + 7) & ~7);
+    } else if (ppc <= 300.0) {
+        a.group = 32;
+        a.maxp_group = std::min(512, ((int)(1.5 * ppc) + 40 + 7) & ~7);
     }
     a.big_list = c->d_big_list;
     a.big_n = c->d_big_n;

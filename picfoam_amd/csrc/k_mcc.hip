@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     mcc_cell(a, cell, a.maxp, lds, picks);
 }
 
-// Several cells per wave: 64 / L cells, L lanes each, for cells of up to a.maxp_group
+// Several cells per wave: 64 / L cells, L lanes each (L = 16 or 32), for cells of up to a.maxp_group
 // electrons (larger ones go to the large-cell pass).  A cell draws a few candidates per
 // step (about 5 of 100 electrons at the argon case's dt), so one cell per wave leaves most
 // lanes idle while the wave pays the loads, the id ranking and the barriers for that cell
@@ -420,6 +420,9 @@ static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
     if (a.group == 16) {
         const size_t lds = (size_t)4 * (a.maxp_group + a.maxp_group / 2) * sizeof(uint64_t);
         hipLaunchKernelGGL((k_mcc_group<UT, 16>), dim3((a.n_cells + 3) / 4), dim3(64), lds, s, a);
+    } else if (a.group == 32) {
+        const size_t lds = (size_t)2 * (a.maxp_group + a.maxp_group / 2) * sizeof(uint64_t);
+        hipLaunchKernelGGL((k_mcc_group<UT, 32>), dim3((a.n_cells + 1) / 2), dim3(64), lds, s, a);
     } else {
         const size_t lds = (size_t)a.maxp * (sizeof(uint64_t) + sizeof(int));
         hipLaunchKernelGGL((k_mcc<UT>), dim3(a.n_cells), dim3(64), lds, s, a);

@@ -249,12 +249,14 @@ def test_coulomb_fp64_parity(oracle, counts, variant):
                                                  ("NanbuElectronNeutral", "FixedValue", "few"),
                                                  ("NanbuElectronNeutral", "List", "many"),
                                                  ("RelativisticElectronNeutral", "List", "many"),
+                                                 ("NanbuElectronNeutral", "List", "two-per-wave"),
                                                  ("NanbuElectronNeutral", "List", "per-cell"),
                                                  ("NanbuElectronNeutral", "List", "many-mixed")])
 def test_mcc_fp64_parity(oracle, model, xs_model, case):
     """MCC against the oracle.  few: 50-150 electrons per cell (four cells per wave) and
     a few candidates per cell; many: the same cells at 25 times the time step, about 100
-    candidates per cell (several rounds per cell, parcels picked more than once); per-cell: 200-400
+    candidates per cell (several rounds per cell, parcels picked more than once);
+    two-per-wave: 130-260 electrons per cell (two cells per wave); per-cell: 300-500
     electrons per cell (one cell per wave); many-mixed: the many case with FP32 velocities
     (inputs FP32-representable, the MCC arithmetic FP64 as in the FP64 build, so the
     results differ from the oracle by the final rounding to FP32).  One cell of 700
@@ -290,7 +292,8 @@ def test_mcc_fp64_parity(oracle, model, xs_model, case):
         for k, (s_, th) in enumerate(((el, 0.0), (ex, 11.5), (iz, 15.76))):
             c.set_cross_section(k, e, s_, th)
     c.mcc_initialize(1000.0, 0.01)
-    ne = rng.integers(200, 400, nc) if case == "per-cell" else rng.integers(50, 150, nc)
+    ne = {"per-cell": (300, 500), "two-per-wave": (130, 260)}.get(case, (50, 150))
+    ne = rng.integers(ne[0], ne[1], nc)
     ne[2] = 700     # above 512 electrons: the large-cell pass
     (ye, te, Ue, ie), _ = _two_species_cell_setup(rng, m, rm, ne, np.zeros(nc, int), Te=30.0)
     if mixed:
