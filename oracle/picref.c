@@ -17,11 +17,20 @@
  *   Poisson PCG/DIC + grad ....................... pinned: dense solve, manufactured sin(kx)
  *   Coulomb (Nanbu + TA pairing) ................. pinned: momentum/energy conservation per pair,
  *                                                    <cos chi> = exp(-s)
+ *   Perez Coulomb ................................ pinned: relativistic 4-momentum conservation,
+ *                                                    <cos chi> = exp(-s) and Nanbu's angle at v<<c
+ *   Nanbu-Yonemura pairing, Nanbu weight
+ *   correction ................................... pinned: the reference's collision counts;
+ *                                                    keep probability w_P/w_Q (binomial test)
  *   MCC (null collision) ......................... pinned: collision frequency n*sigma*v,
  *                                                    candidate count, elastic energy loss 2m/M
+ *   relativistic e-n scattering .................. pinned: 4-momentum conservation, Nanbu's
+ *                                                    cos chi at v<<c
  *   Philox4x32-10 ................................ pinned: Random123 known-answer vectors
  *   OpenFOAM-9 Random/shuffle streams, tracking tie-breaks, volPointInterpolation at
- *   non-coupled boundary points ................... parity unpinned (not reproducible offline)
+ *   non-coupled boundary points, the non-orthogonal Laplacian correction (the Laplacian
+ *   here is the orthogonal part; every field-solve mesh is orthogonal)
+ *                                                  parity unpinned (not reproducible offline)
  */
 #ifdef _OPENMP
 #include <omp.h>
