@@ -456,12 +456,11 @@ int do_coulomb(pic_ctx* c, double dt) {
     a.dt = dt;
     a.seed = c->seed;
     a.step = c->step;
-    // cells of up to kCoulombSlots (256, 512 in denser plasmas) parcels of a species in the
-    // one-cell-per-wave pass with the stage velocities in LDS (group 0), kCoulombSlotsBig in
-    // the large-cell pass.  With
-    // fixed Coulomb logarithms, 64 / L cells share a wave, L lanes and up to 8 L parcels of a
-    // species each (L = 16, 32, 64); the mean population of the fullest species at the last
-    // synchronisation decides.
+    // 64 / L cells share a wave in the grouped pass, L lanes and up to 8 L parcels of a
+    // species each (L = 16, 32, 64), chosen by the mean population of the fullest species at
+    // the last synchronisation; denser plasmas run one cell per wave with the stage
+    // velocities in LDS (group 0, 512 parcels of a species).  Larger cells go to the
+    // large-cell pass (kCoulombSlotsBig).
     a.maxp = kCoulombSlots;
     a.maxp_big = kCoulombSlotsBig;
     a.group = 0;
@@ -469,17 +468,15 @@ int do_coulomb(pic_ctx* c, double dt) {
     for (int i = 0; i < ns; ++i)
         if (c->sp[i].charge != 0.0) most = std::max<int64_t>(most, c->sp[i].n_sync);
     const double ppc = (double)most / std::max(1, a.n_cells);
-    if (!calc_debye && ppc <= 80.0) {
+    if (ppc <= 80.0) {
         a.group = 16; a.maxp = 128;
-    } else if (!calc_debye && ppc <= 180.0) {
+    } else if (ppc <= 180.0) {
         a.group = 32; a.maxp = 256;
-    } else if (!calc_debye && ppc <= 420.0) {
+    } else if (ppc <= 420.0) {
         // 512 slots stay 5 standard deviations above a mean of 420 (Poisson counts)
         a.group = 64; a.maxp = 512;
-    } else if (ppc > 80.0) {
-        // the one-cell-per-wave pass sized for denser cells (LDS lists and velocity slots of
-        // 256 or 512 parcels) rather than sending every cell to the large-cell pass
-        a.maxp = ppc <= 180.0 ? 256 : 512;
+    } else {
+        a.maxp = 512;   // LDS lists and velocity slots for the denser cells
     }
     a.big_list = c->d_big_list;
     a.big_n = c->d_big_n;

@@ -754,7 +754,8 @@ __global__ void __launch_bounds__(64) k_coulomb_big(CoulombArgs a) {
 // parcels of the smaller species meeting two partners) share the wave's passes with the
 // other cells' collisions instead of holding all 64 lanes, and the stage tables of the
 // cells are filled together.  Per cell in LDS: one key list (maxp + 8), 4 L bins and two
-// index lists (maxp = 8 L each).  Requires fixed Coulomb logarithms (no Debye length).
+// index lists (maxp = 8 L each).  A Coulomb logarithm from the Debye length takes a pre-pass
+// over the cell's charged species (ranked by id, summed by a fixed tree).
 
 // The cell's group of L lanes (lane r) ranks its n <= 8 L parcels of one species by their
 // random key, the order keyed_perm produces.  The keys are bucketed by their top bits into
@@ -816,7 +817,7 @@ __device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, 
 }
 
 
-template <class UT, int MODEL, bool WC, int L>
+template <class UT, int MODEL, bool WC, int L, bool DEB>
 __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
     constexpr int W = 64 / L;
     extern __shared__ uint64_t lds[];
@@ -836,6 +837,63 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
     }
     const double Vc = on ? a.V[cell] : 1.0;
     const int ns = a.n_species;
+    // CoulombCollisionModel::debyeLength (CoulombCollisionModel.C:249-310) as coulomb_cell
+    // computes it: each charged species' parcels ranked by id (distinct keys (id, cell-local
+    // index), the rank is the count of smaller keys), lane r summing ranks r, r + L, ...,
+    // then a fixed shuffle tree over the group, so the sums do not depend on the order the
+    // sort left the parcels in
+    double debye = 0.0;
+    if constexpr (DEB) {
+        double deb = 0.0, nmax = 0.0;
+        for (int s = 0; s < ns; ++s) {
+            if (a.charge[s] == 0.0) continue;
+            const int f0 = on ? a.start[s][cell] : 0;
+            const int n = on ? a.start[s][cell + 1] - f0 : 0;
+            const int n8 = (n + 7) & ~7;
+            for (int k = r; k < n8; k += L)
+                key[k] = k < n ? (((uint64_t)a.id[s][f0 + k] << 32) | (uint32_t)k) : ~0ull;
+            __syncthreads();
+            for (int k = r; k < n; k += L) {
+                const uint64_t kk = key[k];
+                int rk = 0;
+                for (int j = 0; j < n8; j += 8) {
+                    const uint64_t* q = key + j;
+                    rk += (int)(q[0] < kk) + (int)(q[1] < kk) + (int)(q[2] < kk) + (int)(q[3] < kk) +
+                          (int)(q[4] < kk) + (int)(q[5] < kk) + (int)(q[6] < kk) + (int)(q[7] < kk);
+                }
+                valA[rk] = slot_index(a.perm[s], f0 + k);
+            }
+            __syncthreads();
+            const UT* ux = (const UT*)a.ux[s];
+            const UT* uy = (const UT*)a.uy[s];
+            const UT* uz = (const UT*)a.uz[s];
+            const double wgt = a.weight[s];
+            double vs = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+            for (int k = r; k < n; k += L) {
+                const int i = valA[k];
+                const double x = ux[i], y = uy[i], z = uz[i];
+                vs += (x * x + y * y + z * z) * wgt;
+                vx += wgt * x; vy += wgt * y; vz += wgt * z;
+            }
+            for (int off = L / 2; off > 0; off >>= 1) {
+                vs += __shfl_xor(vs, off); vx += __shfl_xor(vx, off);
+                vy += __shfl_xor(vy, off); vz += __shfl_xor(vz, off);
+            }
+            __syncthreads();   // valA and key are reused by the next species
+            if (n <= 0) continue;
+            double nn = n * wgt;
+            const double mvx = vx / nn, mvy = vy / nn, mvz = vz / nn, mvs = vs / nn;
+            nn /= Vc;
+            if (nn > nmax) nmax = nn;
+            const double T = a.mass[s] / (3.0 * kKB) * (mvs - (mvx * mvx + mvy * mvy + mvz * mvz));
+            if (T > 0.0) deb += nn * a.charge[s] * a.charge[s] / kEps0 * 1.0 / (T * kKB);
+        }
+        if (deb > 0.0) {
+            const double rmin = pow(4.0 * kPi * nmax / 3.0, -1.0 / 3.0);
+            const double dl = sqrt(1.0 / deb);
+            debye = dl > rmin ? dl : rmin;
+        }
+    }
     int last = -1;
     for (int s = 0; s < ns; ++s)
         if (a.charge[s] != 0.0) last = s;
@@ -890,7 +948,7 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
                 if (t < cnt) {
                     int iP, iQ, k;
                     decode(d, valA, uu, t, iP, iQ, k);
-                    collide<UT, MODEL, WC, false>(a, d, cell, iP, iQ, k, 0.0, (UT*)nullptr, 0);
+                    collide<UT, MODEL, WC, false>(a, d, cell, iP, iQ, k, debye, (UT*)nullptr, 0);
                     ++ncoll;
                     if (++t < cnt) continue;
                 }
@@ -907,14 +965,14 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
     add_ncoll(a, ncoll);
 }
 
-template <class UT, bool WC, int L>
+template <class UT, bool WC, int L, bool DEB>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_coulomb_group_nanbu(CoulombArgs a) {
-    coulomb_group<UT, 0, WC, L>(a);
+    coulomb_group<UT, 0, WC, L, DEB>(a);
 }
 
-template <class UT, bool WC, int L>
+template <class UT, bool WC, int L, bool DEB>
 __global__ void __launch_bounds__(64) k_coulomb_group_perez(CoulombArgs a) {
-    coulomb_group<UT, 1, WC, L>(a);
+    coulomb_group<UT, 1, WC, L, DEB>(a);
 }
 
 void pair_constants(const double* mass, const double* charge, int n_species, PairConst* out) {
@@ -943,10 +1001,14 @@ static void launch_coulomb_group(const CoulombArgs& a, hipStream_t s) {
     constexpr int W = 64 / L;
     const size_t lds = (size_t)W * (2 * a.maxp + 8 + 2 * L) * sizeof(uint64_t);
     const int grid = (a.n_cells + W - 1) / W;
-    if (MODEL == 1)
-        hipLaunchKernelGGL((k_coulomb_group_perez<UT, WC, L>), dim3(grid), dim3(64), lds, s, a);
-    else
-        hipLaunchKernelGGL((k_coulomb_group_nanbu<UT, WC, L>), dim3(grid), dim3(64), lds, s, a);
+    // the Debye pre-pass is its own instantiation, so fixed logarithms keep their registers
+    if (MODEL == 1) {
+        if (a.calc_debye) hipLaunchKernelGGL((k_coulomb_group_perez<UT, WC, L, true>), dim3(grid), dim3(64), lds, s, a);
+        else hipLaunchKernelGGL((k_coulomb_group_perez<UT, WC, L, false>), dim3(grid), dim3(64), lds, s, a);
+    } else {
+        if (a.calc_debye) hipLaunchKernelGGL((k_coulomb_group_nanbu<UT, WC, L, true>), dim3(grid), dim3(64), lds, s, a);
+        else hipLaunchKernelGGL((k_coulomb_group_nanbu<UT, WC, L, false>), dim3(grid), dim3(64), lds, s, a);
+    }
 }
 
 template <class UT, int MODEL, bool WC>

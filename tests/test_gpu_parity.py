@@ -618,15 +618,15 @@ def test_pcg_iterations_match_oracle(oracle, case):
     assert rel_err(phi_g, phi_r) < 1e-7
 
 
-@pytest.mark.parametrize("dense", [False, True], ids=["sparse", "dense"])
+@pytest.mark.parametrize("density", ["sparse", "dense", "densest"])
 @pytest.mark.parametrize("lnl", [10.0, 0.0], ids=["fixed-lnL", "debye-lnL"])
-def test_coulomb_three_charged_species(oracle, lnl, dense):
+def test_coulomb_three_charged_species(oracle, lnl, density):
     """Electrons, two ion species and a neutral species: six pairing stages per cell in the
     reference order, the neutral species skipped.  With fixed logarithms the cells run four
-    to a wave; with the Debye logarithm one cell per wave, where the two velocity slots in
-    LDS must evict and reload species between stages.  dense: 150-300 electrons and ions
-    per cell (the one-cell-per-wave pass sized for 512 parcels with the Debye logarithm,
-    one cell per wave on 64 lanes with fixed ones).  Equal to the oracle to 1e-12."""
+    to a wave, the Debye logarithm computed in a pre-pass per cell.  dense: 150-300
+    electrons and ions per cell (one cell per wave on 64 lanes); densest: 440-512 (the
+    one-cell-per-wave pass with velocities in LDS, where the two velocity slots must evict
+    and reload species between stages).  Equal to the oracle to 1e-12."""
     say("coulomb three species", lnl)
     rng = np.random.default_rng(131)
     m = small_hex(n=(3, 2, 2), L=(1e-4, 1e-4, 1e-4))
@@ -638,11 +638,14 @@ def test_coulomb_three_charged_species(oracle, lnl, dense):
     charges = [-QE, QE, 2 * QE, 0.0]
     weights = [1e6, 1e6, 1e6, 1e6]              # Takizuka-Abe needs equal weights
     counts = [rng.integers(0, 70, nc) for _ in range(4)]
-    if dense:
+    if density == "dense":
         counts[0] = rng.integers(150, 300, nc); counts[1] = rng.integers(150, 300, nc)
-    counts[0][0], counts[1][0], counts[2][0] = 5, 0, 9
-    counts[0][1], counts[1][1], counts[2][1] = 64, 63, 1
-    species = [{"mass": masses[s], "charge": charges[s], "nEquivalentParticles": weights[s], "capacity": 5000}
+    if density != "densest":
+        counts[0][0], counts[1][0], counts[2][0] = 5, 0, 9
+        counts[0][1], counts[1][1], counts[2][1] = 64, 63, 1
+    else:
+        counts[0] = rng.integers(440, 513, nc); counts[1] = rng.integers(440, 513, nc)
+    species = [{"mass": masses[s], "charge": charges[s], "nEquivalentParticles": weights[s], "capacity": 8000}
                for s in range(4)]
     cl = {} if lnl == 0 else {(i, j): lnl for i in range(3) for j in range(i, 3)}
     c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu", "PairingAlgorithm": "TakizukaAbe",
