@@ -212,9 +212,9 @@ struct CoulombArgs {
     int64_t step;
     int maxp;                         // LDS index-list slots of the per-cell pass (power of two)
     int maxp_big;                     // slots of the large-cell pass (power of two)
-    int group;                        // lanes per cell of the grouped pass (fixed Coulomb logarithms
-                                      // only): 16, 32 or 64 (64 / group cells per wave); 0 = the
-                                      // one-cell-per-wave pass with the velocities in LDS
+    int group;                        // lanes per cell of the grouped pass: 16, 32 or 64 (64 / group
+                                      // cells per wave); 0 = the one-cell-per-wave pass with the
+                                      // velocities in LDS
     int* big_list;                    // cells left to the large-cell pass, [n_cells]
     int* big_n;                       // their number (zero before the launch)
     int model;                        // 0 Nanbu, 1 Perez
