@@ -44,10 +44,12 @@ def sort_by_id(ids, *arrays):
 
 def rel_err(a, b, scale=None):
     a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+    if not a.size and not b.size:
+        return 0.0
     s = np.max(np.abs(b)) if scale is None else scale
     if s == 0:
         s = 1.0
-    return float(np.max(np.abs(a - b)) / s) if a.size else 0.0
+    return float(np.max(np.abs(a - b)) / s)
 
 
 def periodic_diff(a, b, L):
