@@ -263,7 +263,8 @@ def test_mcc_fp64_parity(oracle, model, xs_model, case):
     electrons per cell (one cell per wave); many-mixed: the many case with FP32 velocities
     (inputs FP32-representable, the MCC arithmetic FP64 as in the FP64 build, so the
     results differ from the oracle by the final rounding to FP32).  One cell of 700
-    electrons takes the large-cell pass in every case."""
+    electrons takes the large-cell pass in every case; one cell has no electrons and one a
+    single electron."""
     say("mcc parity", model, xs_model, case)
     rng = np.random.default_rng(105)
     from picfoam_amd import cases
@@ -298,6 +299,7 @@ def test_mcc_fp64_parity(oracle, model, xs_model, case):
     ne = {"per-cell": (300, 500), "two-per-wave": (130, 260)}.get(case, (50, 150))
     ne = rng.integers(ne[0], ne[1], nc)
     ne[2] = 700     # above 512 electrons: the large-cell pass
+    ne[5], ne[6] = 0, 1     # a cell without electrons (its remainder unchanged) and a lone electron
     (ye, te, Ue, ie), _ = _two_species_cell_setup(rng, m, rm, ne, np.zeros(nc, int), Te=30.0)
     if mixed:
         Ue = Ue.astype(np.float32).astype(np.float64)
