@@ -477,6 +477,38 @@ def test_cloud_that_empties(oracle, mode):
     assert not np.any(c.deposit())
 
 
+@pytest.mark.parametrize("mode", ["step", "run"])
+def test_created_ions_beyond_capacity_are_an_error(mode):
+    """Ionization that would append more ions than the species capacity returns
+    PIC_ERR_CAPACITY (the reference's lists grow without bound; a fixed-capacity device
+    store reports it instead).  The ions written are the capacity's worth, the count is
+    clamped to it, and the electrons stay consistent."""
+    say("ion capacity", mode)
+    from picfoam_amd import cases, picmi
+    mesh, species, models, ex = cases.argon_mcc(ncell=4, ppc=60, precision="fp64", L=1e-3, growth=20.0)
+    models["seed"] = 3
+    species[1]["capacity"] = 2
+    c = picmi.Cloud(mesh, species, models)
+    e, el, exs, iz = ex["xs"]
+    for k, (s_, th) in enumerate(((el, 0.0), (exs, 11.5), (iz, 15.76))):
+        c.set_cross_section(k, e, s_, th)
+    c.mcc_initialize(1000.0, 0.01)
+    c.set_field(picmi.FIELD_E, ex["E"])
+    c.init_maxwellian(0, 60, ex["T_K"], seed=3)
+    with pytest.raises(picmi.PicError) as ei:
+        for _ in range(40):
+            if mode == "step":
+                c.step(ex["dt"])
+            else:
+                c.run(ex["dt"], 4)
+    assert ei.value.code == -5
+    assert c.count(1) == 2
+    _, ti, _, ii = c.get_particles(1)
+    assert np.all(ti >= 0) and len(set(ii.tolist())) == 2
+    _, te, _, _ = c.get_particles(0)
+    assert len(te) == c.count(0) and np.all(te >= -1)
+
+
 @pytest.mark.parametrize("case", ["thermalisation", "mcc"])
 def test_run_equals_repeated_steps(case):
     """pic_run (one synchronisation for K steps, device-side parcel counts) gives the same
