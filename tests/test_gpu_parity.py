@@ -509,6 +509,30 @@ def test_created_ions_beyond_capacity_are_an_error(mode):
     assert len(te) == c.count(0) and np.all(te >= -1)
 
 
+def test_coulomb_cell_beyond_large_pass_is_an_error(oracle):
+    """A cell with more parcels of a species (2100) than the large-cell Coulomb pass holds
+    (2048) returns PIC_ERR_CELL_OVERFLOW; the device leaves that cell alone."""
+    say("coulomb cell overflow")
+    from picfoam_amd import picmi
+    rng = np.random.default_rng(107)
+    m = small_hex(n=(2, 2, 1), L=(1e-4, 1e-4, 1e-4))
+    rm = oracle.RefMesh(m)
+    species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": 1e6, "capacity": 4000},
+               {"mass": 1836 * ME, "charge": QE, "nEquivalentParticles": 1e6, "capacity": 4000}]
+    c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu",
+                  "coulombLog": {(0, 0): 10.0, (0, 1): 10.0, (1, 1): 10.0}, "seed": 7}, species)
+    (ye, te, Ue, ie), (yi, ti, Ui, ii) = _two_species_cell_setup(rng, m, rm, np.array([2100, 10, 10, 10]),
+                                                                np.array([10, 10, 10, 10]))
+    c.set_particles(0, ye, te, Ue, ie)
+    c.set_particles(1, yi, ti, Ui, ii)
+    with pytest.raises(picmi.PicError) as ei:
+        c.collide_coulomb(1e-9)
+    assert ei.value.code == -8
+    _, _, Ug, ig = c.get_particles(0)
+    (Ug_s,) = sort_by_id(ig, Ug)
+    assert np.array_equal(Ug_s[:2100], Ue[:2100])
+
+
 @pytest.mark.parametrize("case", ["thermalisation", "mcc"])
 def test_run_equals_repeated_steps(case):
     """pic_run (one synchronisation for K steps, device-side parcel counts) gives the same
