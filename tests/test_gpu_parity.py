@@ -533,6 +533,39 @@ def test_coulomb_cell_beyond_large_pass_is_an_error(oracle):
     assert np.array_equal(Ug_s[:2100], Ue[:2100])
 
 
+def test_mcc_cell_beyond_large_pass_is_an_error(oracle):
+    """A cell with more electrons (4200) than the large-cell MCC pass holds (4096) returns
+    PIC_ERR_CELL_OVERFLOW; the device leaves that cell's electrons alone."""
+    say("mcc cell overflow")
+    from picfoam_amd import cases, picmi
+    rng = np.random.default_rng(108)
+    m = small_hex(n=(2, 2, 1), L=(1e-3, 1e-3, 1e-3))
+    rm = oracle.RefMesh(m)
+    mAr = 39.948 * cases.AMU
+    species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": 1.0, "capacity": 8000},
+               {"mass": mAr, "charge": QE, "nEquivalentParticles": 1.0, "capacity": 8000, "solveMovement": False}]
+    models = {"ParticlePusher": "Boris", "ElectronNeutralCollisionModel": "NanbuElectronNeutral",
+              "BackgroundGasModel": "uniformBackground", "electronTypeId": 0, "ionTypeId": 1,
+              "uniformBackground": {"numberDensity": 1e21, "temperature": 300.0, "mass": mAr,
+                                    "nEquivalentParticles": 1.0}, "seed": 99}
+    c = cloud(m, models, species)
+    e, el, ex, iz = cases.argon_like_cross_sections()
+    for k, (s_, th) in enumerate(((el, 0.0), (ex, 11.5), (iz, 15.76))):
+        c.set_cross_section(k, e, s_, th)
+    c.mcc_initialize(1000.0, 0.01)
+    (ye, te, Ue, ie), _ = _two_species_cell_setup(rng, m, rm, np.array([4200, 20, 20, 20]), np.zeros(4, int),
+                                                  Te=30.0)
+    c.set_particles(0, ye, te, Ue, ie)
+    with pytest.raises(picmi.PicError) as ei:
+        c.collide_mcc(5e-10)
+    assert ei.value.code == -8
+    _, _, Ug, ig = c.get_particles(0)
+    sel = np.isin(ig, ie[:4200])
+    assert sel.sum() == 4200
+    (Ug_s,) = sort_by_id(ig[sel], Ug[sel])
+    assert np.array_equal(Ug_s, Ue[:4200])
+
+
 @pytest.mark.parametrize("case", ["thermalisation", "mcc"])
 def test_run_equals_repeated_steps(case):
     """pic_run (one synchronisation for K steps, device-side parcel counts) gives the same
