@@ -123,6 +123,12 @@ typedef struct {
     int32_t sort_interval;            /* parcels move into cell order every k steps; a collision model
                                          gets the cell order as an index on the steps between */
     int32_t poisson_guess;            /* PCG start: 0 the previous phi, 1 2 phi(n-1) - phi(n-2) */
+    /* <model>Coeffs.DebyeLength (CoulombCollisionModel.C:98-141): the species whose parcels
+     * enter the Debye length, as a bit mask of species ids (0 = every charged species, the
+     * reference's default; a listed species without charge is PIC_ERR_INVALID), and
+     * accountForDrift false (1 here) to keep the drift in the species temperature */
+    uint32_t debye_species;
+    int32_t debye_ignore_drift;
 } pic_models;
 
 typedef struct {

@@ -763,25 +763,24 @@ int ref_sort_by_cell(const ref_mesh* m, int n, const int* tet, int* order, int* 
 
 /* ------------------------------------------------------------------ Coulomb */
 
-typedef struct { uint64_t key; uint32_t id; int idx; } keyrec;
+typedef struct { uint64_t key; int idx; } keyrec;
 static int keyrec_cmp(const void* a, const void* b) {
     const keyrec* x = (const keyrec*)a; const keyrec* y = (const keyrec*)b;
     if (x->key != y->key) return x->key < y->key ? -1 : 1;
-    if (x->id != y->id) return x->id < y->id ? -1 : 1;
     return x->idx < y->idx ? -1 : (x->idx > y->idx);
 }
 
 /* shuffle (OpenFOAM Random::shuffle, TakizukaAbePairing.C:110, NanbuYonemuraPairing.C:124)
- * restated as a sort by a counter-based 56-bit random key of (particle id, step, stage,
- * species); ties (equal ids) fall back to the id, then the array position */
+ * restated as a sort by a counter-based 53-bit random key of (particle id, step, stage,
+ * species); equal keys (parcels sharing an id) keep their order in the cell's list */
 static void keyed_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
                        int* perm) {
     keyrec* r = (keyrec*)malloc(sizeof(keyrec) * (n > 0 ? n : 1));
     for (int k = 0; k < n; ++k) {
         uint32_t o[4];
         rng4(seed, id[first + k], (uint32_t)step, (uint32_t)((stage << 8) | species), TAG_SHUFFLE << 24, o);
-        r[k].key = ((uint64_t)o[0] << 24) | (o[1] >> 8);
-        r[k].id = id[first + k]; r[k].idx = first + k;
+        r[k].key = ((uint64_t)o[0] << 21) | (o[1] >> 11);
+        r[k].idx = first + k;
     }
     qsort(r, n, sizeof(keyrec), keyrec_cmp);
     for (int k = 0; k < n; ++k) perm[k] = r[k].idx;
@@ -925,12 +924,13 @@ static void perez_collide(v3* UP, v3* UQ, double mP, double qP, double mQ, doubl
     if (stats) { stats[1] += lnL; stats[2] += s; }
 }
 
-/* CoulombCollisionModel::debyeLength, CoulombCollisionModel.C:249-310 (all charged species,
- * drift removed) */
+/* CoulombCollisionModel::debyeLength, CoulombCollisionModel.C:249-310: the charged species of
+ * debyeLength_Species_ (ctor :98-122), the drift removed from the temperature when
+ * debyeLength_accountForDrift_ (:136-141, :288-292) */
 static double debye_length(const ref_coulomb_params* p, double* const* U, const int* const* cs, int c, double Vc) {
     double debye = 0.0, nmax = 0.0;
     for (int s = 0; s < p->n_species; ++s) {
-        if (p->charge[s] == 0.0) continue;
+        if (p->charge[s] == 0.0 || !((p->debye_mask >> s) & 1u)) continue;
         int a = cs[s][c], b = cs[s][c + 1];
         double n = 0.0, ch = 0.0, vs = 0.0;
         v3 v = V(0, 0, 0);
@@ -944,7 +944,7 @@ static double debye_length(const ref_coulomb_params* p, double* const* U, const 
         if (n <= 0.0) continue;
         v = vdiv(v, n); ch /= n; vs /= n; n /= Vc;
         if (n > nmax) nmax = n;
-        double T = p->mass[s] / (3.0 * KB) * (vs - vdot(v, v));
+        double T = p->mass[s] / (3.0 * KB) * (vs - (p->debye_drift ? vdot(v, v) : 0.0));
         if (T <= 0.0) continue;
         debye += n * ch * ch / EPS0 * 1.0 / (T * KB);
     }
@@ -990,6 +990,7 @@ static void collide(const coll_ctx* x, int sP, int iP, int sQ, int iQ, double ln
     st3(x->U[sP] + 3 * iP, uP);
     st3(x->U[sQ] + 3 * iQ, uQ);
     if (x->stats) x->stats[0] += 1.0;
+    if (p->hits) { p->hits[sP][iP] += 1; p->hits[sQ][iQ] += 1; }
 }
 
 static double dmin(double a, double b) { return a < b ? a : b; }
@@ -1289,34 +1290,34 @@ int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, d
                 eV -= (p->th_iz + Ep);
                 en_scatter(p, eV, &Ue, &Un, us, uc);
                 int created = 0;
-                if (uR <= wEwN && p->create_ion) {
-                    if (*n_i >= cap_i) { err = -1; }
-                    else {
-                        int ni = (*n_i)++;
-                        memcpy(yI + 3 * ni, yE + 3 * ei, sizeof(double) * 3);
-                        tetI[ni] = tetE[ei];
-                        st3(UI + 3 * ni, Un);
-                        uint32_t o[4];
-                        rng4(p->seed, idE[ei], (uint32_t)p->step, (uint32_t)k, (TAG_NEWID << 24) | 1u, o);
-                        idI[ni] = o[0];
-                        created = 1;
-                    }
+                /* the ion and the electron are created together or not at all (a full
+                 * species is an error; the reference's lists have no capacity) */
+                int make = uR <= wEwN;
+                int mk_i = make && p->create_ion, mk_e = make && p->create_electron;
+                int fit = (!mk_i || *n_i < cap_i) && (!mk_e || *n_e < cap_e);
+                if (make && !fit) err = -1;
+                if (mk_i && fit) {
+                    int ni = (*n_i)++;
+                    memcpy(yI + 3 * ni, yE + 3 * ei, sizeof(double) * 3);
+                    tetI[ni] = tetE[ei];
+                    st3(UI + 3 * ni, Un);
+                    uint32_t o[4];
+                    rng4(p->seed, idE[ei], (uint32_t)p->step, (uint32_t)k, (TAG_NEWID << 24) | 1u, o);
+                    idI[ni] = o[0];
+                    created = 1;
                 }
                 if (uR > wNwE) Ue = preUe;
-                if (uR <= wEwN && p->create_electron) {
-                    if (*n_e >= cap_e) { err = -1; }
-                    else {
-                        int ne = (*n_e)++;
-                        memcpy(yE + 3 * ne, yE + 3 * ei, sizeof(double) * 3);
-                        tetE[ne] = tetE[ei];
-                        v3 Unew = vscale(sqrt(2.0 * Ep * QE / p->massE), vdiv(preUe, vmag(preUe)));
-                        v3 Un2 = preUn;
-                        en_scatter(p, Ep, &Unew, &Un2, up2, uc2);
-                        st3(UE + 3 * ne, Unew);
-                        uint32_t o[4];
-                        rng4(p->seed, idE[ei], (uint32_t)p->step, (uint32_t)k, (TAG_NEWID << 24) | 2u, o);
-                        idE[ne] = o[0];
-                    }
+                if (mk_e && fit) {
+                    int ne = (*n_e)++;
+                    memcpy(yE + 3 * ne, yE + 3 * ei, sizeof(double) * 3);
+                    tetE[ne] = tetE[ei];
+                    v3 Unew = vscale(sqrt(2.0 * Ep * QE / p->massE), vdiv(preUe, vmag(preUe)));
+                    v3 Un2 = preUn;
+                    en_scatter(p, Ep, &Unew, &Un2, up2, uc2);
+                    st3(UE + 3 * ne, Unew);
+                    uint32_t o[4];
+                    rng4(p->seed, idE[ei], (uint32_t)p->step, (uint32_t)k, (TAG_NEWID << 24) | 2u, o);
+                    idE[ne] = o[0];
                 }
                 if (counts && created) counts[3]++;
             }

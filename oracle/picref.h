@@ -99,6 +99,9 @@ typedef struct {
     int model;
     int pairing;
     int weight_correction;
+    unsigned debye_mask;   /* species in the Debye length (charged ones among them) */
+    int debye_drift;       /* 1: drift removed from the species temperature */
+    int* const* hits;      /* optional per species, per parcel: collisions taken part in */
 } ref_coulomb_params;
 void ref_set_threads(int n);
 void ref_coulomb(const ref_mesh* m, const ref_coulomb_params* p, double dt, double* const* U,

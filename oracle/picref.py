@@ -213,7 +213,8 @@ class CoulombParams(C.Structure):
     _fields_ = [("n_species", C.c_int), ("mass", C.c_double * 8), ("charge", C.c_double * 8),
                 ("weight", C.c_double * 8), ("coulomb_log", (C.c_double * 8) * 8), ("calculate_bmin", C.c_int),
                 ("allow_intra", C.c_int), ("seed", C.c_uint64), ("step", C.c_int64), ("model", C.c_int),
-                ("pairing", C.c_int), ("weight_correction", C.c_int)]
+                ("pairing", C.c_int), ("weight_correction", C.c_int), ("debye_mask", C.c_uint),
+                ("debye_drift", C.c_int), ("hits", C.c_void_p)]
 
 
 COULOMB_MODELS = {"Nanbu": 0, "Perez": 1}
@@ -221,9 +222,13 @@ PAIRINGS = {"TakizukaAbe": 0, "NanbuYonemura": 1}
 
 
 def coulomb(rm: RefMesh, species, U_list, id_list, cs_list, dt, seed, step, coulomb_log=None, calculate_bmin=True,
-            allow_intra=True, model="Nanbu", pairing="TakizukaAbe", weight_correction=False):
+            allow_intra=True, model="Nanbu", pairing="TakizukaAbe", weight_correction=False, debye_species=None,
+            debye_drift=True, hits=None):
     """species: [(mass, charge, weight)]; U_list: sorted-by-cell velocity arrays (modified copies
-    returned).  stats: [0] collisions, [1] sum lnL, [2] sum s, [3] sum of Debye lengths."""
+    returned).  debye_species: the species ids in the Debye length (None: every charged one);
+    debye_drift: remove the mean drift from the species temperature there.  hits: optional
+    list of int32 arrays (one per species, parcel order of U_list) incremented per collision.
+    stats: [0] collisions, [1] sum lnL, [2] sum s, [3] sum of Debye lengths."""
     p = CoulombParams()
     p.n_species = len(species)
     for i, (m, q, w) in enumerate(species):
@@ -235,6 +240,8 @@ def coulomb(rm: RefMesh, species, U_list, id_list, cs_list, dt, seed, step, coul
     p.calculate_bmin = int(calculate_bmin); p.allow_intra = int(allow_intra)
     p.seed = seed; p.step = step
     p.model = COULOMB_MODELS[model]; p.pairing = PAIRINGS[pairing]; p.weight_correction = int(weight_correction)
+    p.debye_mask = 0xFF if debye_species is None else sum(1 << int(k) for k in debye_species)
+    p.debye_drift = int(debye_drift)
     Us = [np.ascontiguousarray(u, np.float64).copy() for u in U_list]
     ids = [np.ascontiguousarray(i, np.uint32) for i in id_list]
     css = [np.ascontiguousarray(c, np.int32) for c in cs_list]
@@ -243,6 +250,10 @@ def coulomb(rm: RefMesh, species, U_list, id_list, cs_list, dt, seed, step, coul
     Ip = (C.c_void_p * n)(*[i.ctypes.data for i in ids])
     Cp = (C.c_void_p * n)(*[c.ctypes.data for c in css])
     stats = np.zeros(8)
+    if hits is not None:
+        assert all(h.dtype == np.int32 and h.flags.c_contiguous for h in hits)
+        Hp = (C.c_void_p * n)(*[h.ctypes.data for h in hits])
+        p.hits = C.cast(Hp, C.c_void_p)
     lib().ref_coulomb(rm.h, C.byref(p), dt, Up, Ip, Cp, _p(stats))
     return Us, stats
 

@@ -70,6 +70,8 @@ struct pic_ctx {
     int precision = PIC_PREC_FP64;
     int hc_ref = 0;
     int allow_intra = 1, calc_bmin = 1;
+    uint32_t debye_mask = 0xFFu;             // species in the Debye length (the charged ones of them)
+    int debye_drift = 1;                     // DebyeLength.accountForDrift
     double coulomb_log[kMaxSpecies][kMaxSpecies] = {};
     int electron = -1, ion = -1;
     double bg_n = 0, bg_T = 0, bg_m = 0, bg_w = 0, bg_v[3] = {0, 0, 0};
@@ -221,6 +223,8 @@ int parse_model(pic_ctx* c, const char* name, const char* keyword, const std::ve
 int64_t* live_n(pic_ctx* c, int s) { return c->d_nrows + c->sp[s].ncur * kMaxSpecies + s; }
 int64_t* next_n(pic_ctx* c, int s) { return c->d_nrows + (1 - c->sp[s].ncur) * kMaxSpecies + s; }
 
+void set_dev_n(pic_ctx* c, int s);
+
 // error flags + counts after a synchronisation point
 int sync_state(pic_ctx* c) {
     uint32_t flags = 0;
@@ -230,8 +234,12 @@ int sync_state(pic_ctx* c) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return check_hip(c, e, "stream synchronize");
     for (size_t s = 0; s < c->sp.size(); ++s) {
-        c->sp[s].n = std::min<int64_t>(n[c->sp[s].ncur * kMaxSpecies + s], c->sp[s].capacity);
+        const int64_t dev = n[c->sp[s].ncur * kMaxSpecies + s];
+        c->sp[s].n = std::min<int64_t>(dev, c->sp[s].capacity);
         c->sp[s].n_sync = c->sp[s].n;
+        // creations past the capacity (flagged below) left the device count above it: the
+        // live row is clamped too, so later steps see the parcels that exist
+        if (dev > c->sp[s].capacity) set_dev_n(c, (int)s);
     }
     if (flags) {
         hipMemsetAsync(c->d_flags, 0, sizeof(uint32_t), c->stream);
@@ -450,6 +458,8 @@ int do_coulomb(pic_ctx* c, double dt) {
         }
     }
     a.calc_debye = calc_debye ? 1 : 0;
+    a.debye_mask = c->debye_mask;
+    a.debye_drift = c->debye_drift;
     a.calculate_bmin = c->calc_bmin;
     a.allow_intra = c->allow_intra;
     a.V = c->d_V;
@@ -768,6 +778,18 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->hc_ref = models->hc_reference_form;
     c->allow_intra = models->allow_intra_collision;
     c->calc_bmin = models->calculate_bmin;
+    if (models->debye_species) {
+        // CoulombCollisionModel.C:106-121: every listed species must exist and carry charge
+        for (int k = 0; k < 32; ++k) {
+            if (!((models->debye_species >> k) & 1u)) continue;
+            if (k >= n_species) return c->fail(PIC_ERR_INVALID, "DebyeLength species " + std::to_string(k) + " does not exist");
+            if (species[k].charge == 0.0)
+                return c->fail(PIC_ERR_INVALID, "Species " + std::to_string(k) +
+                                                    " considered for calculation of the Debye length is not a particle with charge");
+        }
+        c->debye_mask = models->debye_species;
+    }
+    c->debye_drift = models->debye_ignore_drift ? 0 : 1;
     std::memcpy(c->coulomb_log, models->coulomb_log, sizeof(c->coulomb_log));
     c->electron = models->electron_species;
     c->ion = models->ion_species;

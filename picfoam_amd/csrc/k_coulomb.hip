@@ -278,23 +278,30 @@ __device__ void bitonic_sort(uint64_t* key, int* val, int npow2) {
     }
 }
 
+// The shuffle key of a parcel is 53 random bits; the sort keys carry the parcel's cell-local
+// index in the 11 bits below it (cells of up to kCoulombSlotsBig = 2048 parcels of a
+// species), so sort keys are distinct even when ids repeat, and equal random keys order by
+// the cell-local index.
+constexpr int kKeyIndexBits = 11;
+static_assert(kCoulombSlotsBig <= (1 << kKeyIndexBits), "cell-local index must fit below the shuffle key");
+
 __device__ __forceinline__ uint64_t shuffle_key(uint64_t seed, int64_t step, int stage, int species, uint32_t pid) {
     const uint4 o = rng4(seed, pid, (uint32_t)step, ((uint32_t)stage << 8) | (uint32_t)species, kTagShuffle << 24);
-    return ((uint64_t)o.x << 24) | (o.y >> 8);
+    return ((uint64_t)o.x << 21) | (o.y >> 11);
 }
 
-// val[0..n) = the cell's parcels of one species ordered by their 56-bit random key: base +
+// val[0..n) = the cell's parcels of one species ordered by their 53-bit random key: base +
 // cell-local index, or the array index through perm (perm non-null: base is unused); id
-// holds the ids in cell order (id[first + k] for the cell's parcel k).  Up to
-// 256 parcels the key carries the cell-local index in its low byte, so keys are distinct and
-// each parcel's place is the number of smaller keys (no barriers in the count; eight keys
-// per LDS round trip).  Larger cells use a bitonic sort.
+// holds the ids in cell order (id[first + k] for the cell's parcel k).  Sort keys are
+// distinct (the cell-local index sits below the random key), so up to 256 parcels each
+// parcel's place is the number of smaller keys (no barriers in the count; eight keys per
+// LDS round trip).  Larger cells use a bitonic sort.
 __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, const uint32_t* id, int first, int n,
                            uint64_t* key, int* val, int base, const int* perm) {
     if (n <= 256) {
         const int n8 = (n + 7) & ~7;
         for (int k = threadIdx.x; k < n8; k += blockDim.x)
-            key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k : ~0ull;
+            key[k] = k < n ? (shuffle_key(seed, step, stage, species, id[first + k]) << kKeyIndexBits) | (uint64_t)k : ~0ull;
         __syncthreads();
         // rounds of up to 64 keys; a partial round of E keys gives each key G = 64 / E lanes
         // (rounded down to a power of two) that count over G slices of the list and add up
@@ -329,7 +336,7 @@ __device__ void keyed_perm(uint64_t seed, int64_t step, int stage, int species, 
     while (np2 < n) np2 <<= 1;
     for (int k = threadIdx.x; k < np2; k += blockDim.x) {
         if (k < n) {
-            key[k] = (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)(k & 255);
+            key[k] = (shuffle_key(seed, step, stage, species, id[first + k]) << kKeyIndexBits) | (uint64_t)k;
             val[k] = perm ? perm[first + k] : base + k;
         } else {
             key[k] = ~0ull;
@@ -493,7 +500,7 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
     if (a.calc_debye) {
         double deb = 0.0, nmax = 0.0;
         for (int s = 0; s < ns; ++s) {
-            if (a.charge[s] == 0.0) continue;
+            if (a.charge[s] == 0.0 || !((a.debye_mask >> s) & 1u)) continue;
             const int f0 = a.start[s][cell], n = a.start[s][cell + 1] - f0;
             if (n <= 0) continue;
             int np2 = 1;
@@ -522,7 +529,8 @@ __device__ unsigned long long coulomb_cell(const CoulombArgs& a, int cell, int m
             const double mvs = vs / nn;
             nn /= Vc;
             if (nn > nmax) nmax = nn;
-            const double T = a.mass[s] / (3.0 * kKB) * (mvs - (mvx * mvx + mvy * mvy + mvz * mvz));
+            const double vd2 = a.debye_drift ? (mvx * mvx + mvy * mvy + mvz * mvz) : 0.0;
+            const double T = a.mass[s] / (3.0 * kKB) * (mvs - vd2);
             if (T > 0.0) deb += nn * ch * ch / kEps0 * 1.0 / (T * kKB);
             __syncthreads();
         }
@@ -775,7 +783,7 @@ __device__ void group_perm(uint64_t seed, int64_t step, int stage, int species, 
         const int k = r + q * L;
         kk[q] = 0;
         if (k < n) {
-            kk[q] = (shuffle_key(seed, step, stage, species, id[first + k]) << 8) | (uint64_t)k;
+            kk[q] = (shuffle_key(seed, step, stage, species, id[first + k]) << kKeyIndexBits) | (uint64_t)k;
             val[k] = atomicAdd(&bins[kk[q] >> SH], 1);   // place in the bucket, until the ranks
         }
     }
@@ -846,7 +854,7 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
     if constexpr (DEB) {
         double deb = 0.0, nmax = 0.0;
         for (int s = 0; s < ns; ++s) {
-            if (a.charge[s] == 0.0) continue;
+            if (a.charge[s] == 0.0 || !((a.debye_mask >> s) & 1u)) continue;
             const int f0 = on ? a.start[s][cell] : 0;
             const int n = on ? a.start[s][cell + 1] - f0 : 0;
             const int n8 = (n + 7) & ~7;
@@ -885,7 +893,8 @@ __device__ __forceinline__ void coulomb_group(const CoulombArgs& a) {
             const double mvx = vx / nn, mvy = vy / nn, mvz = vz / nn, mvs = vs / nn;
             nn /= Vc;
             if (nn > nmax) nmax = nn;
-            const double T = a.mass[s] / (3.0 * kKB) * (mvs - (mvx * mvx + mvy * mvy + mvz * mvz));
+            const double vd2 = a.debye_drift ? (mvx * mvx + mvy * mvy + mvz * mvz) : 0.0;
+            const double T = a.mass[s] / (3.0 * kKB) * (mvs - vd2);
             if (T > 0.0) deb += nn * a.charge[s] * a.charge[s] / kEps0 * 1.0 / (T * kKB);
         }
         if (deb > 0.0) {

@@ -172,37 +172,41 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
         const int t = a.e.tet[ei];
         const double y1 = a.e.y1[ei], y2 = a.e.y2[ei], y3 = a.e.y3[ei];
         bool created = false;
-        if (uR <= a.wEwN && a.create_ion) {
-            const int64_t slot = atomicAdd((unsigned long long*)a.n_i, 1ull);
-            if (slot >= a.cap_i) {
-                atomicOr(a.flags, kFlagCapacity);
-            } else {
-                a.ion.y1[slot] = y1; a.ion.y2[slot] = y2; a.ion.y3[slot] = y3;
-                a.ion.tet[slot] = t;
-                a.ion.ux[slot] = (UT)Un.x; a.ion.uy[slot] = (UT)Un.y; a.ion.uz[slot] = (UT)Un.z;
-                uint4 o = rng4(a.seed, a.e.id[ei], (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 1u);
-                a.ion.id[slot] = o.x;
-                deposit_new(a, t, y1, y2, y3, a.count_i, a.s42_i);
-                created = true;
-            }
+        // the ion and the electron of one ionization (ElectronNeutralCollisionModel.C:680-698)
+        // are created together or not at all: both slots are reserved first, and when either
+        // species is full the event creates neither, raises the capacity flag and leaves any
+        // slot it did get as a deleted parcel (tet -1), so the charge stays balanced
+        const bool make = uR <= a.wEwN;
+        const bool mk_i = make && a.create_ion, mk_e = make && a.create_electron;
+        const int64_t slot_i = mk_i ? (int64_t)atomicAdd((unsigned long long*)a.n_i, 1ull) : -1;
+        const int64_t slot_e = mk_e ? (int64_t)atomicAdd((unsigned long long*)a.n_e, 1ull) : -1;
+        const bool fit = (!mk_i || slot_i < a.cap_i) && (!mk_e || slot_e < a.cap_e);
+        if (make && !fit) {
+            atomicOr(a.flags, kFlagCapacity);
+            if (mk_i && slot_i < a.cap_i) { a.ion.tet[slot_i] = -1; a.ion.id[slot_i] = 0u; }
+            if (mk_e && slot_e < a.cap_e) { a.e.tet[slot_e] = -1; a.e.id[slot_e] = 0u; }
+        }
+        if (mk_i && fit) {
+            a.ion.y1[slot_i] = y1; a.ion.y2[slot_i] = y2; a.ion.y3[slot_i] = y3;
+            a.ion.tet[slot_i] = t;
+            a.ion.ux[slot_i] = (UT)Un.x; a.ion.uy[slot_i] = (UT)Un.y; a.ion.uz[slot_i] = (UT)Un.z;
+            uint4 o = rng4(a.seed, a.e.id[ei], (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 1u);
+            a.ion.id[slot_i] = o.x;
+            deposit_new(a, t, y1, y2, y3, a.count_i, a.s42_i);
+            created = true;
         }
         if (uR > a.wNwE) Ue = preUe;
-        if (uR <= a.wEwN && a.create_electron) {
-            const int64_t slot = atomicAdd((unsigned long long*)a.n_e, 1ull);
-            if (slot >= a.cap_e) {
-                atomicOr(a.flags, kFlagCapacity);
-            } else {
-                const double sp = sqrt(2.0 * Ep * kQe / a.massE);
-                vec3<double> Unew = sp * (preUe / mag(preUe));
-                vec3<double> Un2 = preUn;
-                en_scatter(a, Ep, Unew, Un2, up2, uc2);
-                a.e.y1[slot] = y1; a.e.y2[slot] = y2; a.e.y3[slot] = y3;
-                a.e.tet[slot] = t;
-                a.e.ux[slot] = (UT)Unew.x; a.e.uy[slot] = (UT)Unew.y; a.e.uz[slot] = (UT)Unew.z;
-                uint4 o = rng4(a.seed, a.e.id[ei], (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 2u);
-                a.e.id[slot] = o.x;
-                deposit_new(a, t, y1, y2, y3, a.count_e, a.s42_e);
-            }
+        if (mk_e && fit) {
+            const double sp = sqrt(2.0 * Ep * kQe / a.massE);
+            vec3<double> Unew = sp * (preUe / mag(preUe));
+            vec3<double> Un2 = preUn;
+            en_scatter(a, Ep, Unew, Un2, up2, uc2);
+            a.e.y1[slot_e] = y1; a.e.y2[slot_e] = y2; a.e.y3[slot_e] = y3;
+            a.e.tet[slot_e] = t;
+            a.e.ux[slot_e] = (UT)Unew.x; a.e.uy[slot_e] = (UT)Unew.y; a.e.uz[slot_e] = (UT)Unew.z;
+            uint4 o = rng4(a.seed, a.e.id[ei], (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 2u);
+            a.e.id[slot_e] = o.x;
+            deposit_new(a, t, y1, y2, y3, a.count_e, a.s42_e);
         }
         if (created) ++c_iz;
     }

@@ -200,6 +200,8 @@ struct CoulombArgs {
     double mass[kMaxSpecies], charge[kMaxSpecies], weight[kMaxSpecies];
     double coulomb_log[kMaxSpecies][kMaxSpecies];
     int calc_debye, calculate_bmin, allow_intra;
+    uint32_t debye_mask;              // species summed into the Debye length (charged ones only)
+    int debye_drift;                  // 1: the species temperature has the mean drift removed
     const int* start[kMaxSpecies];    // cell_start per species [n_cells+1]
     const int* perm[kMaxSpecies];     // sorted slot -> array index, or null (arrays sorted by cell)
     void* ux[kMaxSpecies];

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libpicmi.so")
 
 # status codes (picmi.h)
 PIC_OK = 0
+PIC_ERR_INVALID = -1
 ERRORS = {
     -1: "PIC_ERR_INVALID", -2: "PIC_ERR_UNKNOWN_MODEL", -3: "PIC_ERR_SUPERLUMINAL", -4: "PIC_ERR_HIP",
     -5: "PIC_ERR_CAPACITY", -6: "PIC_ERR_MESH", -7: "PIC_ERR_UNSUPPORTED", -8: "PIC_ERR_CELL_OVERFLOW",
@@ -80,7 +81,18 @@ class pic_models(C.Structure):
                 ("bg_n_equivalent", C.c_double), ("bg_velocity", C.c_double * 3), ("create_ion", C.c_int32),
                 ("create_electron", C.c_int32), ("precision", C.c_int32), ("hc_reference_form", C.c_int32),
                 ("seed", C.c_uint64), ("poisson_tolerance", C.c_double), ("poisson_max_iter", C.c_int32),
-                ("poisson_mode", C.c_int32), ("sort_interval", C.c_int32), ("poisson_guess", C.c_int32)]
+                ("poisson_mode", C.c_int32), ("sort_interval", C.c_int32), ("poisson_guess", C.c_int32),
+                ("debye_species", C.c_uint32), ("debye_ignore_drift", C.c_int32)]
+
+
+# the keywords Cloud reads from its models dictionary; any other key is an error (the
+# reference's dictionaries are read by lookup, so a misspelt keyword aborts there too)
+MODEL_KEYS = frozenset((
+    "ParticlePusher", "FieldWeigthing", "ChargeDistribution", "MaxwellSolver", "CoulombCollisionModel",
+    "PairingAlgorithm", "WeightCorrectionModel", "ElectronNeutralCollisionModel", "BackgroundGasModel",
+    "allowIntraCollision", "calculate_bmin", "DebyeLength", "coulombLog", "electronTypeId", "ionTypeId",
+    "uniformBackground", "createIon", "createElectron", "precision", "hcReferenceForm", "seed",
+    "poissonTolerance", "poissonMaxIter", "poissonMode", "sortInterval", "poissonGuess"))
 
 
 class pic_poisson_stats(C.Structure):
@@ -227,6 +239,9 @@ class Cloud:
             sp[i].solve_movement = 1 if s.get("solveMovement", True) else 0
             sp[i].capacity = int(s["capacity"])
             sp[i].sort_interval = int(s.get("sortInterval", 0))
+        unknown = sorted(set(models) - MODEL_KEYS)
+        if unknown:
+            raise PicError(PIC_ERR_INVALID, f"unknown model keyword(s) {unknown}; valid: {sorted(MODEL_KEYS)}")
         m = pic_models()
         keep = []
         for key, field, default in (("ParticlePusher", "particle_pusher", "Boris"),
@@ -243,7 +258,15 @@ class Cloud:
             setattr(m, field, v)
         self._keep = keep
         m.allow_intra_collision = 1 if models.get("allowIntraCollision", True) else 0
-        m.calculate_bmin = 1 if models.get("calculate_bmin", True) else 0
+        # <model>Coeffs.DebyeLength (CoulombCollisionModel.C:98-141): species (ids), accountForDrift,
+        # calculate_bmin (also accepted at the top level)
+        dl = dict(models.get("DebyeLength", {}))
+        bad = sorted(set(dl) - {"species", "accountForDrift", "calculate_bmin"})
+        if bad:
+            raise PicError(PIC_ERR_INVALID, f"unknown DebyeLength keyword(s) {bad}")
+        m.calculate_bmin = 1 if dl.get("calculate_bmin", models.get("calculate_bmin", True)) else 0
+        m.debye_species = sum(1 << int(k) for k in dl.get("species", ()))
+        m.debye_ignore_drift = 0 if dl.get("accountForDrift", True) else 1
         cl = np.zeros((8, 8))
         for (i, j), v in models.get("coulombLog", {}).items():
             cl[i, j] = cl[j, i] = v

@@ -42,14 +42,25 @@ def sort_by_id(ids, *arrays):
     return [a[o] for a in arrays]
 
 
-def rel_err(a, b, scale=None):
+def rel_err(a, b, floor=None, floor_frac=1e-3):
+    """Largest error relative to each element's own size: per row (the vector's norm) for
+    2-d arrays, per entry for 1-d ones.  Entries far below the largest are measured against
+    an absolute floor instead (default floor_frac times the largest size), since their
+    rounding comes from operands of the larger sizes."""
     a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
     if not a.size and not b.size:
         return 0.0
-    s = np.max(np.abs(b)) if scale is None else scale
-    if s == 0:
-        s = 1.0
-    return float(np.max(np.abs(a - b)) / s)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    if b.ndim == 2:
+        size = np.linalg.norm(b, axis=1)
+        err = np.linalg.norm(a - b, axis=1)
+    else:
+        size = np.abs(b).ravel()
+        err = np.abs(a - b).ravel()
+    fl = floor if floor is not None else floor_frac * float(np.max(size))
+    if fl == 0.0:
+        fl = 1.0
+    return float(np.max(err / np.maximum(size, fl)))
 
 
 def periodic_diff(a, b, L):

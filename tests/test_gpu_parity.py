@@ -45,13 +45,17 @@ def test_mesh_tables_match_oracle(oracle, kind):
 
 
 @pytest.mark.parametrize("kind", ["hex", "tet"])
-@pytest.mark.parametrize("pusher", ["Boris", "BorisNR", "Vay", "HigueraCary"])
+@pytest.mark.parametrize("pusher", ["Boris", "BorisNR", "Vay", "HigueraCary", "HigueraCary-ref"])
 @pytest.mark.parametrize("weighting", ["CellAverage", "VolumeWeighting"])
 def test_push_fp64_parity(oracle, kind, pusher, weighting):
+    """HigueraCary-ref: the reference's own (t.u)^2 term (HigueraCary.C:104, hcReferenceForm)"""
     say("push parity", kind, pusher, weighting)
     rng = np.random.default_rng(100)
     m = MESHES[kind]()
-    c = cloud(m, {"ParticlePusher": pusher, "FieldWeigthing": weighting, "ChargeDistribution": weighting})
+    hc_ref = pusher.endswith("-ref")
+    pusher = pusher.replace("-ref", "")
+    c = cloud(m, {"ParticlePusher": pusher, "FieldWeigthing": weighting, "ChargeDistribution": weighting,
+                  "hcReferenceForm": hc_ref})
     rm = oracle.RefMesh(m)
     n = 5000
     y, tet, U, ids = random_parcels(rng, rm.n_tets, n, 0.5 * C_LIGHT)
@@ -63,7 +67,7 @@ def test_push_fp64_parity(oracle, kind, pusher, weighting):
     c.push(dt)
     _, _, Ug, _ = c.get_particles(0)
     Ep = rm.cell_to_point(E)
-    Ur = rm.push(pusher, weighting, y, U, tet, -QE, ME, dt, E, Ep, B)
+    Ur = rm.push(pusher, weighting, y, U, tet, -QE, ME, dt, E, Ep, B, hc_ref=hc_ref)
     assert rel_err(Ug, Ur) < TOL
     if weighting == "VolumeWeighting":
         assert rel_err(c.get_field(5), Ep) < TOL
@@ -97,9 +101,16 @@ def test_move_fp64_parity(oracle, kind, weighting):
     xr = rm.positions(yr, tr)
     d = periodic_diff(xg, xr, L)
     assert np.max(np.abs(d)) / L.max() < TOL
-    # same cell almost everywhere (a parcel exactly on a face may sit on either side)
-    tc, _, _ = rm.tets()
-    assert np.mean(tc[tg] == tc[tr]) > 0.999
+    # tet ids are index work: equal for every parcel, except a parcel that ends on a tet face
+    # (the oracle's smallest barycentric coordinate within 1e-12), which either side of the
+    # face may hold; each such tie is listed and their number reported
+    mism = np.nonzero(tg != tr)[0]
+    y0 = 1.0 - yr.sum(1)
+    ymin = np.minimum(y0, yr.min(1))
+    ties = [i for i in mism if ymin[i] < 1e-12]
+    say(f"move parity {kind} {weighting}: {len(mism)} tet ids differ of {n}, {len(ties)} at a face")
+    assert len(ties) == len(mism), [(int(i), float(ymin[i])) for i in mism if ymin[i] >= 1e-12][:10]
+    assert len(mism) <= 2
 
 
 @pytest.mark.parametrize("kind", ["hex", "tet"])
@@ -247,6 +258,116 @@ def test_coulomb_fp64_parity(oracle, counts, variant):
         assert np.any(Us[0] != Ue)
 
 
+@pytest.mark.parametrize("model", ["Nanbu", "Perez"])
+@pytest.mark.parametrize("counts", ["equal", "ragged", "denser"])
+def test_coulomb_mixed_precision_error_bound(oracle, counts, model):
+    """Mixed precision: the velocities are stored in FP32 and the Nanbu scattering runs in
+    FP32 (Perez in FP64 on the FP32 velocities).  On FP32-representable inputs the GPU
+    equals the FP64 oracle to a stated bound, per parcel relative to its speed: 2e-5 for
+    Nanbu (FP32 log/exp/sqrt of cos chi and the rotation, a few hundred ulp at 6e-8 each),
+    2e-7 for Perez (the final rounding to FP32)."""
+    say("coulomb mixed bound", counts, model)
+    rng = np.random.default_rng(144)
+    m = small_hex(n=(3, 2, 2), L=(1e-4, 1e-4, 1e-4))
+    rm = oracle.RefMesh(m)
+    nc = rm.n_cells
+    if counts == "equal":
+        ne = np.full(nc, 64); ni = np.full(nc, 64)
+    elif counts == "denser":
+        ne = rng.integers(150, 420, nc); ni = rng.integers(150, 420, nc)
+    else:
+        ne = rng.integers(0, 90, nc); ni = rng.integers(0, 90, nc)
+        ne[1], ni[1] = 1, 0
+        ne[3], ni[3] = 300, 5
+    mi = 1836 * ME
+    w = 1e6
+    species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": w, "capacity": 20000},
+               {"mass": mi, "charge": QE, "nEquivalentParticles": w, "capacity": 20000}]
+    c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": model, "PairingAlgorithm": "TakizukaAbe",
+                  "coulombLog": {(0, 0): 10.0, (0, 1): 10.0, (1, 1): 10.0}, "seed": 78, "precision": "mixed"},
+              species)
+    (ye, te, Ue, ie), (yi, ti, Ui, ii) = _two_species_cell_setup(rng, m, rm, ne, ni, mi=mi)
+    Ue = Ue.astype(np.float32).astype(np.float64)
+    Ui = Ui.astype(np.float32).astype(np.float64)
+    c.set_particles(0, ye, te, Ue, ie)
+    c.set_particles(1, yi, ti, Ui, ii)
+    c.step_index = 5
+    dt = 1e-9
+    c.collide_coulomb(dt)
+    _, _, Ueg, ieg = c.get_particles(0)
+    _, _, Uig, iig = c.get_particles(1)
+    c.close()
+    cse = np.concatenate([[0], np.cumsum(ne)]).astype(np.int32)
+    csi = np.concatenate([[0], np.cumsum(ni)]).astype(np.int32)
+    clm = np.zeros((8, 8)); clm[:2, :2] = 10.0
+    Us, _ = oracle.coulomb(rm, [(ME, -QE, w), (mi, QE, w)], [Ue, Ui], [ie, ii], [cse, csi], dt, seed=78, step=5,
+                           coulomb_log=clm, model=model)
+    (Ueg_s,) = sort_by_id(ieg, Ueg)
+    (Uig_s,) = sort_by_id(iig, Uig)
+    ee, ei = rel_err(Ueg_s, Us[0]), rel_err(Uig_s, Us[1])
+    say(f"coulomb mixed {model} {counts}: electrons {ee:.2e} ions {ei:.2e}")
+    bound = 2e-5 if model == "Nanbu" else 2e-7
+    assert ee < bound and ei < bound
+
+
+def test_coulomb_duplicate_ids_in_a_dense_cell(oracle):
+    """Parcel ids need not be unique (pic_particles_set takes any ids; MCC-created parcels
+    draw random ones).  Cells of 300-330 electrons that all share one id, position and
+    velocity, against 290-320 ions with distinct ids, on the 64-lane grouped pass (up to
+    512 parcels of a species): every electron's shuffle key is the same, so the sort keys
+    differ only in the cell-local index, which runs past 255.  With electron-ion stages only
+    (no intra-species collisions) the electrons are interchangeable, so the GPU's electron
+    velocities equal the oracle's as a set and the ions' one by one, to 1e-12."""
+    say("coulomb duplicate ids")
+    rng = np.random.default_rng(321)
+    m = small_hex(n=(2, 1, 1), L=(1e-4, 1e-4, 1e-4))
+    rm = oracle.RefMesh(m)
+    tc, _, _ = rm.tets()
+    first_tet = np.array([np.nonzero(tc == c)[0][0] for c in range(rm.n_cells)], np.int32)
+    ne = np.array([330, 300]); ni = np.array([290, 320])
+    mi = 1836 * ME
+    w = 1e6
+    ye = np.concatenate([np.repeat(random_bary(rng, 1), k, axis=0) for k in ne])
+    te = np.concatenate([np.full(k, first_tet[c], np.int32) for c, k in enumerate(ne)])
+    Ue = np.concatenate([np.repeat(rng.normal(size=(1, 3)) * 1.3e6, k, axis=0) for k in ne])
+    ie = np.full(ne.sum(), 7, np.uint32)
+    yi = random_bary(rng, ni.sum())
+    ti = np.concatenate([np.full(k, first_tet[c], np.int32) for c, k in enumerate(ni)])
+    Ui = rng.normal(size=(ni.sum(), 3)) * 1e4
+    ii = np.arange(ni.sum(), dtype=np.uint32) + 1000
+    species = [{"mass": ME, "charge": -QE, "nEquivalentParticles": w, "capacity": 4096},
+               {"mass": mi, "charge": QE, "nEquivalentParticles": w, "capacity": 4096}]
+    c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu", "PairingAlgorithm": "TakizukaAbe",
+                  "allowIntraCollision": False, "coulombLog": {(0, 0): 10.0, (0, 1): 10.0, (1, 1): 10.0},
+                  "seed": 9}, species)
+    c.set_particles(0, ye, te, Ue, ie)
+    c.set_particles(1, yi, ti, Ui, ii)
+    c.step_index = 3
+    dt = 2e-9
+    c.collide_coulomb(dt)
+    _, teg, Ueg, ieg = c.get_particles(0)
+    _, _, Uig, iig = c.get_particles(1)
+    c.close()
+    cse = np.concatenate([[0], np.cumsum(ne)]).astype(np.int32)
+    csi = np.concatenate([[0], np.cumsum(ni)]).astype(np.int32)
+    clm = np.zeros((8, 8)); clm[:2, :2] = 10.0
+    Us, st = oracle.coulomb(rm, [(ME, -QE, w), (mi, QE, w)], [Ue, Ui], [ie, ii], [cse, csi], dt, seed=9, step=3,
+                            coulomb_log=clm, allow_intra=False)
+    assert np.all(ieg == 7) and len(Ueg) == ne.sum()
+    # electrons: the same set per cell (cell of a GPU parcel from its tet)
+    cells_g = tc[teg]
+    for cc in range(rm.n_cells):
+        g = Ueg[cells_g == cc]
+        r = Us[0][cse[cc]:cse[cc + 1]]
+        g = g[np.lexsort(g.T[::-1])]
+        r = r[np.lexsort(r.T[::-1])]
+        assert len(g) == len(r)
+        assert rel_err(g, r) < TOL
+    (Uig_s,) = sort_by_id(iig, Uig)
+    assert rel_err(Uig_s, Us[1]) < TOL
+    assert np.all(np.isfinite(Ueg)) and np.any(Us[0] != Ue)
+
+
 @pytest.mark.parametrize("model,xs_model,case", [("NanbuElectronNeutral", "List", "few"),
                                                  ("RelativisticElectronNeutral", "List", "few"),
                                                  ("NanbuElectronNeutral", "FixedValue", "few"),
@@ -356,10 +477,11 @@ def test_poisson_parity(oracle, mode):
     phi_g = c.get_field(2)
     phi_r, it, rel = P.solve(rho, tol=1e-12)
     assert st["rel_residual"] <= 1e-10
-    assert rel_err(phi_g, phi_r) < 1e-8
+    # iterative solutions agree norm-wise, to the solve tolerance (relative to the largest value)
+    assert rel_err(phi_g, phi_r, floor_frac=1.0) < 1e-8
     Eg = c.get_field(0)
     Er = -P.grad(phi_r)
-    assert rel_err(Eg, Er) < 1e-7
+    assert rel_err(Eg, Er, floor_frac=1.0) < 1e-7
     # gradient parity on identical phi
     c.set_field(2, phi_r)
 
@@ -379,8 +501,10 @@ def test_grad_and_point_interp_parity(oracle):
     c.poisson_solve()
     phi = c.get_field(2)
     Er = -P.grad(phi)
-    assert rel_err(c.get_field(0), Er) < 1e-13
-    assert rel_err(c.get_field(5), rm.cell_to_point(Er)) < 1e-13
+    # per cell / point; vectors below 1 % of the largest are held to 1e-15 of it (their
+    # rounding comes from face sums of the larger values)
+    assert rel_err(c.get_field(0), Er, floor_frac=1e-2) < 1e-13
+    assert rel_err(c.get_field(5), rm.cell_to_point(Er), floor_frac=1e-2) < 1e-13
 
 
 def test_step_equals_stage_sequence(oracle):
@@ -481,8 +605,10 @@ def test_cloud_that_empties(oracle, mode):
 def test_created_ions_beyond_capacity_are_an_error(mode):
     """Ionization that would append more ions than the species capacity returns
     PIC_ERR_CAPACITY (the reference's lists grow without bound; a fixed-capacity device
-    store reports it instead).  The ions written are the capacity's worth, the count is
-    clamped to it, and the electrons stay consistent."""
+    store reports it instead).  The ions written are the capacity's worth and the count is
+    clamped to it, on the host and on the device; an ionization whose ion does not fit
+    creates no electron either (the pair is created together or not at all), and the
+    electrons stay consistent."""
     say("ion capacity", mode)
     from picfoam_amd import cases, picmi
     mesh, species, models, ex = cases.argon_mcc(ncell=4, ppc=60, precision="fp64", L=1e-3, growth=20.0)
@@ -507,6 +633,10 @@ def test_created_ions_beyond_capacity_are_an_error(mode):
     assert np.all(ti >= 0) and len(set(ii.tolist())) == 2
     _, te, _, _ = c.get_particles(0)
     assert len(te) == c.count(0) and np.all(te >= -1)
+    # the device count was clamped as well: a further step reports no new overflow until an
+    # ionization tries to create an ion again, and the ion count stays at the capacity
+    c.run(ex["dt"], 0)
+    assert c.count(1) == 2
 
 
 def test_coulomb_cell_beyond_large_pass_is_an_error(oracle):
@@ -759,19 +889,24 @@ def test_pcg_iterations_match_oracle(oracle, case):
     phi_g = c.get_field(2)
     if bc is None:
         phi_g = phi_g - phi_g.mean(); phi_r = phi_r - phi_r.mean()
-    assert rel_err(phi_g, phi_r) < 1e-7
+    assert rel_err(phi_g, phi_r, floor_frac=1.0) < 1e-7   # norm-wise, to the solve tolerance
 
 
 @pytest.mark.parametrize("density", ["sparse", "dense", "densest"])
-@pytest.mark.parametrize("lnl", [10.0, 0.0], ids=["fixed-lnL", "debye-lnL"])
+@pytest.mark.parametrize("lnl", [10.0, 0.0, "opts"], ids=["fixed-lnL", "debye-lnL", "debye-options"])
 def test_coulomb_three_charged_species(oracle, lnl, density):
     """Electrons, two ion species and a neutral species: six pairing stages per cell in the
     reference order, the neutral species skipped.  With fixed logarithms the cells run four
     to a wave, the Debye logarithm computed in a pre-pass per cell.  dense: 150-300
     electrons and ions per cell (one cell per wave on 64 lanes); densest: 440-512 (the
     one-cell-per-wave pass with velocities in LDS, where the two velocity slots must evict
-    and reload species between stages).  Equal to the oracle to 1e-12."""
-    say("coulomb three species", lnl)
+    and reload species between stages).  debye-options: DebyeLength with a species list
+    (electrons and the doubly charged ions) and accountForDrift false, every species
+    drifting.  Equal to the oracle to 1e-12."""
+    say("coulomb three species", lnl, density)
+    opts = lnl == "opts"
+    if opts:
+        lnl = 0.0
     rng = np.random.default_rng(131)
     m = small_hex(n=(3, 2, 2), L=(1e-4, 1e-4, 1e-4))
     rm = oracle.RefMesh(m)
@@ -792,8 +927,11 @@ def test_coulomb_three_charged_species(oracle, lnl, density):
     species = [{"mass": masses[s], "charge": charges[s], "nEquivalentParticles": weights[s], "capacity": 8000}
                for s in range(4)]
     cl = {} if lnl == 0 else {(i, j): lnl for i in range(3) for j in range(i, 3)}
-    c = cloud(m, {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu", "PairingAlgorithm": "TakizukaAbe",
-                  "coulombLog": cl, "seed": 91}, species)
+    models = {"ParticlePusher": "Boris", "CoulombCollisionModel": "Nanbu", "PairingAlgorithm": "TakizukaAbe",
+              "coulombLog": cl, "seed": 91}
+    if opts:
+        models["DebyeLength"] = {"species": [0, 2], "accountForDrift": False}
+    c = cloud(m, models, species)
     Us, ids, css = [], [], []
     for s in range(4):
         tet = np.concatenate([np.full(k, first_tet[ci], np.int32) for ci, k in enumerate(counts[s])])
@@ -801,6 +939,8 @@ def test_coulomb_three_charged_species(oracle, lnl, density):
         y = random_bary(rng, n)
         T = (10.0 if s == 0 else 1.0) * QE
         U = rng.normal(size=(n, 3)) * np.sqrt(T / masses[s])
+        if opts:
+            U += np.array([0.7, -0.3, 0.2]) * np.sqrt(T / masses[s])
         ident = (np.arange(n) + 1000 * s).astype(np.uint32)
         c.set_particles(s, y, tet, U, ident)
         Us.append(U); ids.append(ident)
@@ -811,8 +951,9 @@ def test_coulomb_three_charged_species(oracle, lnl, density):
     clm = np.zeros((8, 8))
     if lnl:
         clm[:3, :3] = lnl
+    kw = {"debye_species": [0, 2], "debye_drift": False} if opts else {}
     ref, _ = oracle.coulomb(rm, [(masses[s], charges[s], weights[s]) for s in range(4)], Us, ids, css, dt, seed=91,
-                            step=3, coulomb_log=clm)
+                            step=3, coulomb_log=clm, **kw)
     for s in range(4):
         _, _, Ug, ig = c.get_particles(s)
         (Ug_s,) = sort_by_id(ig, Ug)
