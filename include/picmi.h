@@ -192,6 +192,13 @@ int pic_particles_count(pic_ctx* ctx, int32_t species, int64_t* n);
 int pic_particles_get(pic_ctx* ctx, int32_t species, int64_t capacity, int64_t* n, double* bary, int32_t* tet,
                       double* U, uint32_t* id);
 int pic_particles_positions(pic_ctx* ctx, int32_t species, int64_t capacity, double* x);
+/* pic_particles_positions without any device allocation (no counterpart in the reference):
+ * for callers that cannot call hipMalloc (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it).  The positions are staged in a 32 MiB device array
+ * that every GPU the library runs on holds until the process ends, so one call takes at
+ * most 1398101 parcels (PIC_ERR_CAPACITY beyond).  Calls on one GPU share that array:
+ * calls that overlap on the same GPU race. */
+int pic_particles_positions_noalloc(pic_ctx* ctx, int32_t species, int64_t capacity, double* x);
 /* picInitialise Uniform/Maxwellian: nppc parcels per cell, uniform in volume, Maxwellian
  * at temperature [K] plus drift; ids start at id0 */
 int pic_init_maxwellian(pic_ctx* ctx, int32_t species, int32_t nppc, double temperature, const double* drift,

@@ -23,6 +23,7 @@ SOURCES = [
     ("k_mcc.hip", []),
     ("k_poisson.hip", []),
     ("k_gnn.hip", []),
+    ("noalloc.hip", []),
 ]
 
 

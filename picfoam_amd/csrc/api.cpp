@@ -833,6 +833,11 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         return PIC_ERR_NO_DEVICE;
     }
     hipGetDevice(&c->device);
+    // the allocation-free entry points' scratch address on this device, once per device
+    if (hipError_t e = noalloc_init(c->device)) {
+        g_last_error = std::string("device scratch: ") + hipGetErrorString(e);
+        return PIC_ERR_HIP;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_last_error = "hipStreamCreate failed";
         return PIC_ERR_HIP;
@@ -1259,6 +1264,25 @@ int pic_particles_positions(pic_ctx* c, int32_t s, int64_t cap, double* x) {
     hipMemcpy(x, dx, S.n * 3 * sizeof(double), hipMemcpyDeviceToHost);
     hipFree(dx);
     return check_hip(c, e, "positions");
+}
+
+int pic_particles_positions_noalloc(pic_ctx* c, int32_t s, int64_t cap, double* x) {
+    if (!c || s < 0 || s >= (int)c->sp.size() || !x) return PIC_ERR_INVALID;
+    Species& S = c->sp[s];
+    if (S.n > cap) return c->fail(PIC_ERR_CAPACITY, "output buffer too small");
+    const size_t bytes = (size_t)S.n * 3 * sizeof(double);
+    const size_t most = noalloc_scratch_bytes() / (3 * sizeof(double));
+    if (bytes > noalloc_scratch_bytes())
+        return c->fail(PIC_ERR_CAPACITY, std::to_string(S.n) + " parcels exceed the " + std::to_string(most) +
+                                             " the allocation-free positions call stages at once");
+    void* d = noalloc_scratch(c->device);
+    if (!d) return c->fail(PIC_ERR_HIP, "device scratch was not set up for this device");
+    if (S.n == 0) return PIC_OK;
+    hipError_t e = c->mixed() ? launch_positions(soa<float>(S, S.cur), S.n, c->d_tets, c->d_C, c->d_points, (double*)d, c->stream)
+                              : launch_positions(soa<double>(S, S.cur), S.n, c->d_tets, c->d_C, c->d_points, (double*)d, c->stream);
+    if (int r = check_hip(c, e, "positions")) return r;
+    if (int r = check_hip(c, hipMemcpyAsync(x, d, bytes, hipMemcpyDeviceToHost, c->stream), "positions copy")) return r;
+    return check_hip(c, hipStreamSynchronize(c->stream), "positions");
 }
 
 int pic_init_maxwellian(pic_ctx* c, int32_t s, int32_t nppc, double T, const double* drift, uint32_t id0,

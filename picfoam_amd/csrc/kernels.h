@@ -72,6 +72,11 @@ hipError_t launch_positions(const PartSoA<double>& p, int64_t n, const TetGeo* t
                             const double* pts, double* x, hipStream_t s);
 hipError_t launch_positions(const PartSoA<float>& p, int64_t n, const TetGeo* tets, const double* C,
                             const double* pts, double* x, hipStream_t s);
+// device scratch of the allocation-free entry points (noalloc.hip): noalloc_init takes the
+// array's address on `device` (current) once; noalloc_scratch returns it, or null before
+hipError_t noalloc_init(int device);
+void* noalloc_scratch(int device);
+size_t noalloc_scratch_bytes();
 hipError_t launch_init(const InitArgs<double>& a, hipStream_t s);
 hipError_t launch_init(const InitArgs<float>& a, hipStream_t s);
 // counting-sort scatter of every species in one launch: n = launch bound (>= the live

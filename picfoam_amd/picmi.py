@@ -36,7 +36,7 @@ EXPORTS = (
     "pic_version", "pic_device_count", "pic_create", "pic_destroy", "pic_last_error", "pic_mesh_sizes",
     "pic_mesh_tets", "pic_mesh_geometry", "pic_set_cell_field", "pic_get_cell_field", "pic_set_cross_section",
     "pic_mcc_initialize", "pic_particles_set", "pic_particles_set_positions", "pic_particles_count",
-    "pic_particles_get", "pic_particles_positions", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
+    "pic_particles_get", "pic_particles_positions", "pic_particles_positions_noalloc", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
     "pic_collide_coulomb", "pic_collide_mcc", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
     "pic_run", "pic_cell_moments",
     "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
@@ -144,6 +144,7 @@ def load_library(path: str = LIB_PATH):
         "pic_particles_count": (C.c_int, [vp, i32, C.POINTER(i64)]),
         "pic_particles_get": (C.c_int, [vp, i32, i64, C.POINTER(i64), vp, vp, vp, vp]),
         "pic_particles_positions": (C.c_int, [vp, i32, i64, vp]),
+        "pic_particles_positions_noalloc": (C.c_int, [vp, i32, i64, vp]),
         "pic_init_maxwellian": (C.c_int, [vp, i32, i32, dbl, vp, u32, C.c_uint64]),
         "pic_push": (C.c_int, [vp, dbl]),
         "pic_move": (C.c_int, [vp, dbl]),
