@@ -1216,16 +1216,26 @@ static inline void box_muller(double a, double b, double* n1, double* n2) {
     *n2 = r * sin(2.0 * PI_D * b);
 }
 
-typedef struct { uint32_t id; int idx; } idrec;
+/* order of a cell's electrons for the candidate draw: 52 bits of the splitmix64 finaliser
+ * of the id, then the position in the cell's list (a fixed, id-determined order that does
+ * not depend on where the sort left the parcels) */
+typedef struct { uint64_t key; int idx; } idrec;
+static uint64_t mcc_order_key(uint32_t id) {
+    uint64_t z = (uint64_t)id + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z >> 12;
+}
 static int idrec_cmp(const void* a, const void* b) {
     const idrec* x = (const idrec*)a; const idrec* y = (const idrec*)b;
-    if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
     return x->idx < y->idx ? -1 : (x->idx > y->idx);
 }
 
 /* ElectronNeutralCollisionModel::performBackgroundCollisions (ElectronNeutralCollisionModel.C:703-829)
  * with the elastic/excitation/ionization events (:504-700); candidates are drawn with
- * replacement from the cell's electrons taken in ascending id order. */
+ * replacement from the cell's electrons taken in mcc_order_key order. */
 int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, double* UE, int* tetE,
             uint32_t* idE, const int* cs, int* n_e, int cap_e, double* yI, double* UI, int* tetI, uint32_t* idI,
             int* n_i, int cap_i, double* scrmax, double* rem, int64_t* counts) {
@@ -1244,7 +1254,7 @@ int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, d
         rem[cell] = coll - nc;
         if (counts) counts[0] += nc;
         if (eC == 0 || nc == 0) continue;
-        for (int k = 0; k < eC; ++k) { L[k].id = idE[a0 + k]; L[k].idx = a0 + k; }
+        for (int k = 0; k < eC; ++k) { L[k].key = mcc_order_key(idE[a0 + k]); L[k].idx = a0 + k; }
         qsort(L, eC, sizeof(idrec), idrec_cmp);
         for (int k = 0; k < nc; ++k) {
             double upick, uacc, ua, ub, n1, n2, n3, n4;

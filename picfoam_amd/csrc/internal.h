@@ -20,6 +20,8 @@ constexpr int kMaxSpecies = 8;
 // per-cell index-list slots in LDS of the collision kernels (per-cell pass / large cells)
 constexpr int kCoulombSlots = 128, kCoulombSlotsBig = 2048;
 constexpr int kMccSlots = 512, kMccSlotsBig = 4096;
+// order-key buckets of the one-cell-per-wave MCC passes
+constexpr int kMccBins = 512, kMccBinsBig = 1024;
 
 enum Pusher { kPushNone = 0, kPushBoris = 1, kPushBorisNR = 2, kPushVay = 3, kPushHC = 4 };
 enum Weighting { kCellAverage = 0, kVolumeWeighting = 1 };
@@ -28,9 +30,9 @@ enum Weighting { kCellAverage = 0, kVolumeWeighting = 1 };
 // edge matrix (rows r1 = e2 x e3, r2 = e3 x e1, r3 = e1 x e2, det = e1 . r1), the owning
 // cell, the coupled (master) point ids of (base, A, B), the neighbouring tet across each
 // tet face (face i is opposite vertex i; <0 encodes -1-patch for a non-coupled patch) and,
-// per face, the permutation that carries barycentric slot k of this tet to slot perm(k) of
-// the neighbour (the dropped vertex goes to the neighbour's new vertex, whose coordinate is
-// zero on the face).
+// per face (byte i), for each slot j of the neighbour the slot of this tet whose barycentric
+// coordinate moves there (2 bits each; the dropped vertex's slot, zero on the face, goes to
+// the neighbour's new vertex).
 struct alignas(128) TetGeo {
     double r[9];
     double det;

@@ -4,11 +4,14 @@
 // NanbuElectronNeutralCollision.C:98-130, ListCrossSection interpolateXY
 // ListCrossSection.C:32-92).
 //
-// One wave per cell.  The cell's electrons are ranked by id in LDS so that a candidate
-// index names the same parcel whatever order the sort scatter left them in.
-// Candidates are drawn with replacement as in the reference; lanes take one candidate
-// each and the lowest lane of a group of candidates that picked the same parcel replays
-// the group in candidate order, so every parcel sees its collisions in sequence.
+// The cell's electrons are ordered by a 52-bit mix of their ids (the cell-local index below
+// it), so that a candidate index names the same parcel whatever order the sort scatter left
+// them in.  The order is not computed in full: the keys go into LDS buckets by their top
+// bits (a histogram, its prefix and one placement per key), and a candidate's rank is
+// resolved inside its bucket (about one key per bucket).  Candidates are drawn with
+// replacement as in the reference; lanes take one candidate each and the lowest lane of a
+// group of candidates that picked the same parcel replays the group in candidate order, so
+// every parcel sees its collisions in sequence.
 #include <cmath>
 
 #include "device.h"
@@ -105,10 +108,12 @@ __device__ void rel_scatter(double eV, vec3<double>& UE, vec3<double>& Un, doubl
     com_leave(f, mE, mN, cc, sc, sp, cp, UE, Un);
 }
 
-template <class UT>
+// REL: RelativisticElectronNeutral (a template parameter, so each kernel holds the
+// registers of one scattering model only)
+template <int REL, class UT>
 __device__ __forceinline__ void en_scatter(const MccArgs<UT>& a, double eV, vec3<double>& UE, vec3<double>& Un,
                                            double us, double uc) {
-    if (a.relativistic) rel_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
+    if constexpr (REL) rel_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
     else nanbu_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
 }
 
@@ -125,22 +130,53 @@ __device__ void deposit_new(const MccArgs<UT>& a, int t, double y1, double y2, d
     }
 }
 
-// one candidate (ElectronNeutralCollisionModel.C:747-800 and the event handlers)
+// An ionization whose pair (ion and secondary electron) is created after the round of
+// candidates it belongs to, from the primary's velocity before the event: everything else
+// the creation needs is recomputed from the candidate's random numbers.  Creating the pairs
+// apart keeps the candidate loop's registers free of the creation's.
+struct MccPair {
+    double pu[3];
+    int ei, k, cell, pad;
+};
+
+// the partner of candidate k of a cell, from the background Maxwellian
+// (uniformBackground::sampleVelocity, uniformBackground.C:121-127)
 template <class UT>
-__device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double smax_local, double& qtcr,
-                          unsigned& c_el, unsigned& c_ex, unsigned& c_iz) {
-    double ua, ub, n1, n2, n3, n4, upick, uacc;
-    mcc_rand(a.seed, a.step, cell, k, 0, upick, uacc);
+__device__ __forceinline__ vec3<double> bg_partner(const MccArgs<UT>& a, int cell, int k) {
+    double ua, ub, n1, n2, n3, n4;
     mcc_rand(a.seed, a.step, cell, k, 1, ua, ub);
     box_muller(ua, ub, n1, n2);
     mcc_rand(a.seed, a.step, cell, k, 2, ua, ub);
     box_muller(ua, ub, n3, n4);
-    vec3<double> Un = mk3<double>(a.vthN * n1 + a.drift[0], a.vthN * n2 + a.drift[1], a.vthN * n3 + a.drift[2]);
-    vec3<double> Ue = mk3<double>(a.e.ux[ei], a.e.uy[ei], a.e.uz[ei]);
+    return mk3<double>(a.vthN * n1 + a.drift[0], a.vthN * n2 + a.drift[1], a.vthN * n3 + a.drift[2]);
+}
+
+// relativistic kinetic energy [eV] of the relative motion (ElectronNeutralCollisionModel.C:755-759)
+__device__ __forceinline__ double rel_energy_ev(double cR, double massE) {
     const double c = kC;
-    const double cR = mag(Ue - Un);
     const double gamma = 1.0 / sqrt(1.0 - ((cR * cR) / (c * c)));
-    double eV = (gamma - 1.0) * a.massE * c * c / kQe;
+    return (gamma - 1.0) * massE * c * c / kQe;
+}
+
+// energy of the secondary electron, the Opal-like split (ElectronNeutralCollisionModel.C:656-659)
+__device__ __forceinline__ double opal_energy(double eV, double th, double uE) {
+    const double E0 = 2.0 - 100.0 / (eV + 10.0);
+    const double E1 = (eV - th) / 2.0 - E0;
+    const double alpha = 10.3;
+    return E0 + alpha * tan(uE * (atan(E1 / alpha) + atan(E0 / alpha)) - atan(E0 / alpha));
+}
+
+// one candidate (ElectronNeutralCollisionModel.C:747-800 and the event handlers); an
+// ionization that creates a pair appends it to pairs[*npairs] (LDS)
+template <int REL, class UT>
+__device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double smax_local, double& qtcr,
+                          unsigned& c_el, unsigned& c_ex, MccPair* pairs, int* npairs) {
+    double upick, uacc;
+    mcc_rand(a.seed, a.step, cell, k, 0, upick, uacc);
+    vec3<double> Un = bg_partner(a, cell, k);
+    vec3<double> Ue = mk3<double>(a.e.ux[ei], a.e.uy[ei], a.e.uz[ei]);
+    const double cR = mag(Ue - Un);
+    double eV = rel_energy_ev(cR, a.massE);
     const double sigmaT = smax_local / cR;
     const double Qel = xs_lookup(a.xs_e[0], a.xs_s[0], a.xs_n[0], a.xs_guide[0], a.xs_le0[0], a.xs_idl[0], eV);
     const double Qex = xs_lookup(a.xs_e[1], a.xs_s[1], a.xs_n[1], a.xs_guide[1], a.xs_le0[1], a.xs_idl[1], eV);
@@ -150,77 +186,168 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
     double us, uc;
     mcc_rand(a.seed, a.step, cell, k, 3, us, uc);
     if (uacc <= Qel / sigmaT) {
-        en_scatter(a, eV, Ue, Un, us, uc);
+        en_scatter<REL>(a, eV, Ue, Un, us, uc);
         ++c_el;
     } else if (eV >= a.th_ex && uacc <= (Qel + Qex) / sigmaT) {
         Ue = sqrt(1.0 - a.th_ex / eV) * Ue;
         eV -= a.th_ex;
-        en_scatter(a, eV, Ue, Un, us, uc);
+        en_scatter<REL>(a, eV, Ue, Un, us, uc);
         ++c_ex;
     } else if (eV >= a.th_iz && uacc <= Qt / sigmaT) {
-        double uE, uR, up2, uc2;
+        double uE, uR;
         mcc_rand(a.seed, a.step, cell, k, 4, uE, uR);
-        mcc_rand(a.seed, a.step, cell, k, 5, up2, uc2);
-        const double E0 = 2.0 - 100.0 / (eV + 10.0);
-        const double E1 = (eV - a.th_iz) / 2.0 - E0;
-        const double alpha = 10.3;
-        const double Ep = E0 + alpha * tan(uE * (atan(E1 / alpha) + atan(E0 / alpha)) - atan(E0 / alpha));
-        const vec3<double> preUe = Ue, preUn = Un;
+        const double Ep = opal_energy(eV, a.th_iz, uE);
+        const vec3<double> preUe = Ue;
         Ue = sqrt(1.0 - (a.th_iz + Ep) / eV) * Ue;
         eV -= (a.th_iz + Ep);
-        en_scatter(a, eV, Ue, Un, us, uc);
-        const int t = a.e.tet[ei];
-        const double y1 = a.e.y1[ei], y2 = a.e.y2[ei], y3 = a.e.y3[ei];
-        bool created = false;
-        // the ion and the electron of one ionization (ElectronNeutralCollisionModel.C:680-698)
-        // are created together or not at all: both slots are reserved first, and when either
-        // species is full the event creates neither, raises the capacity flag and leaves any
-        // slot it did get as a deleted parcel (tet -1), so the charge stays balanced
-        const bool make = uR <= a.wEwN;
-        const bool mk_i = make && a.create_ion, mk_e = make && a.create_electron;
-        const int64_t slot_i = mk_i ? (int64_t)atomicAdd((unsigned long long*)a.n_i, 1ull) : -1;
-        const int64_t slot_e = mk_e ? (int64_t)atomicAdd((unsigned long long*)a.n_e, 1ull) : -1;
-        const bool fit = (!mk_i || slot_i < a.cap_i) && (!mk_e || slot_e < a.cap_e);
-        if (make && !fit) {
-            atomicOr(a.flags, kFlagCapacity);
-            if (mk_i && slot_i < a.cap_i) { a.ion.tet[slot_i] = -1; a.ion.id[slot_i] = 0u; }
-            if (mk_e && slot_e < a.cap_e) { a.e.tet[slot_e] = -1; a.e.id[slot_e] = 0u; }
-        }
-        if (mk_i && fit) {
-            a.ion.y1[slot_i] = y1; a.ion.y2[slot_i] = y2; a.ion.y3[slot_i] = y3;
-            a.ion.tet[slot_i] = t;
-            a.ion.ux[slot_i] = (UT)Un.x; a.ion.uy[slot_i] = (UT)Un.y; a.ion.uz[slot_i] = (UT)Un.z;
-            uint4 o = rng4(a.seed, a.e.id[ei], (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 1u);
-            a.ion.id[slot_i] = o.x;
-            deposit_new(a, t, y1, y2, y3, a.count_i, a.s42_i);
-            created = true;
-        }
+        en_scatter<REL>(a, eV, Ue, Un, us, uc);
         if (uR > a.wNwE) Ue = preUe;
-        if (mk_e && fit) {
-            const double sp = sqrt(2.0 * Ep * kQe / a.massE);
-            vec3<double> Unew = sp * (preUe / mag(preUe));
-            vec3<double> Un2 = preUn;
-            en_scatter(a, Ep, Unew, Un2, up2, uc2);
-            a.e.y1[slot_e] = y1; a.e.y2[slot_e] = y2; a.e.y3[slot_e] = y3;
-            a.e.tet[slot_e] = t;
-            a.e.ux[slot_e] = (UT)Unew.x; a.e.uy[slot_e] = (UT)Unew.y; a.e.uz[slot_e] = (UT)Unew.z;
-            uint4 o = rng4(a.seed, a.e.id[ei], (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 2u);
-            a.e.id[slot_e] = o.x;
-            deposit_new(a, t, y1, y2, y3, a.count_e, a.s42_e);
+        if (uR <= a.wEwN && (a.create_ion || a.create_electron)) {
+            const int e = atomicAdd(npairs, 1);
+            MccPair& P = pairs[e];
+            P.pu[0] = preUe.x; P.pu[1] = preUe.y; P.pu[2] = preUe.z;
+            P.ei = ei; P.k = k; P.cell = cell;
         }
-        if (created) ++c_iz;
     }
     a.e.ux[ei] = (UT)Ue.x; a.e.uy[ei] = (UT)Ue.y; a.e.uz[ei] = (UT)Ue.z;
+}
+
+// The ion and the secondary electron of an ionization (ElectronNeutralCollisionModel.C:
+// 662-698): the ion carries the partner's velocity after the primary's scattering, the
+// electron Ep along the primary's old direction, scattered on the partner's old velocity.
+// They are created together or not at all: both slots are reserved first, and when either
+// species is full the event creates neither, raises the capacity flag and leaves any slot
+// it did get as a deleted parcel (tet -1), so the charge stays balanced.
+template <int REL, class UT>
+__device__ void create_pair(const MccArgs<UT>& a, const MccPair& P, unsigned& c_iz) {
+    const int cell = P.cell, k = P.k, ei = P.ei;
+    const vec3<double> preUe = mk3<double>(P.pu[0], P.pu[1], P.pu[2]);
+    vec3<double> Un = bg_partner(a, cell, k);
+    const vec3<double> preUn = Un;
+    double eV = rel_energy_ev(mag(preUe - Un), a.massE);
+    double us, uc, uE, uR, up2, uc2;
+    mcc_rand(a.seed, a.step, cell, k, 3, us, uc);
+    mcc_rand(a.seed, a.step, cell, k, 4, uE, uR);
+    mcc_rand(a.seed, a.step, cell, k, 5, up2, uc2);
+    const double Ep = opal_energy(eV, a.th_iz, uE);
+    vec3<double> Ue = sqrt(1.0 - (a.th_iz + Ep) / eV) * preUe;
+    eV -= (a.th_iz + Ep);
+    en_scatter<REL>(a, eV, Ue, Un, us, uc);
+    const int t = a.e.tet[ei];
+    const double y1 = a.e.y1[ei], y2 = a.e.y2[ei], y3 = a.e.y3[ei];
+    const uint32_t pid = a.e.id[ei];
+    const bool mk_i = a.create_ion, mk_e = a.create_electron;
+    const int64_t slot_i = mk_i ? (int64_t)atomicAdd((unsigned long long*)a.n_i, 1ull) : -1;
+    const int64_t slot_e = mk_e ? (int64_t)atomicAdd((unsigned long long*)a.n_e, 1ull) : -1;
+    const bool fit = (!mk_i || slot_i < a.cap_i) && (!mk_e || slot_e < a.cap_e);
+    if (!fit) {
+        atomicOr(a.flags, kFlagCapacity);
+        if (mk_i && slot_i < a.cap_i) { a.ion.tet[slot_i] = -1; a.ion.id[slot_i] = 0u; }
+        if (mk_e && slot_e < a.cap_e) { a.e.tet[slot_e] = -1; a.e.id[slot_e] = 0u; }
+        return;
+    }
+    if (mk_i) {
+        a.ion.y1[slot_i] = y1; a.ion.y2[slot_i] = y2; a.ion.y3[slot_i] = y3;
+        a.ion.tet[slot_i] = t;
+        a.ion.ux[slot_i] = (UT)Un.x; a.ion.uy[slot_i] = (UT)Un.y; a.ion.uz[slot_i] = (UT)Un.z;
+        a.ion.id[slot_i] = rng4(a.seed, pid, (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 1u).x;
+        deposit_new(a, t, y1, y2, y3, a.count_i, a.s42_i);
+        ++c_iz;
+    }
+    if (mk_e) {
+        const double sp = sqrt(2.0 * Ep * kQe / a.massE);
+        vec3<double> Unew = sp * (preUe / mag(preUe));
+        vec3<double> Un2 = preUn;
+        en_scatter<REL>(a, Ep, Unew, Un2, up2, uc2);
+        a.e.y1[slot_e] = y1; a.e.y2[slot_e] = y2; a.e.y3[slot_e] = y3;
+        a.e.tet[slot_e] = t;
+        a.e.ux[slot_e] = (UT)Unew.x; a.e.uy[slot_e] = (UT)Unew.y; a.e.uz[slot_e] = (UT)Unew.z;
+        a.e.id[slot_e] = rng4(a.seed, pid, (uint32_t)a.step, (uint32_t)k, (kTagNewId << 24) | 2u).x;
+        deposit_new(a, t, y1, y2, y3, a.count_e, a.s42_e);
+    }
+}
+
+// Order key of an electron in its cell: 52 bits of the splitmix64 finaliser of its id, the
+// cell-local index in the 12 bits below (cells of up to kMccSlotsBig = 4096 electrons), so
+// keys are distinct and parcels that share an id keep their list order.
+constexpr int kMccIndexBits = 12;
+static_assert(kMccSlotsBig <= (1 << kMccIndexBits), "cell-local index must fit below the order key");
+__device__ __forceinline__ uint64_t mcc_key(uint32_t id, int k) {
+    uint64_t z = (uint64_t)id + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((z >> kMccIndexBits) << kMccIndexBits) | (uint64_t)k;
+}
+
+// The n electrons of a cell (ids id[first + k]) bucketed by the top log2(NB) bits of their
+// keys, by the cell's L lanes (lane r): skey[0..n) holds the keys bucket after bucket (each
+// bucket unordered), bins[b] the start of bucket b.  place[] is scratch.  Ends with a barrier.
+template <int L, int NB>
+__device__ void mcc_bucket(const uint32_t* __restrict__ id, int first, int n, uint64_t* skey, int* place, int* bins,
+                           int r) {
+    constexpr int PER = NB / L;
+    constexpr int SH = 64 - __builtin_ctz(NB);
+    for (int b = r; b < NB; b += L) bins[b] = 0;
+    __syncthreads();
+    for (int k = r; k < n; k += L) place[k] = atomicAdd(&bins[(int)(mcc_key(id[first + k], k) >> SH)], 1);
+    __syncthreads();
+    // exclusive prefix of the bins: PER consecutive bins per lane, scanned over the lanes
+    int loc = 0;
+#pragma unroll
+    for (int b = 0; b < PER; ++b) loc += bins[r * PER + b];
+    int inc = loc;
+#pragma unroll
+    for (int off = 1; off < L; off <<= 1) {
+        const int y = __shfl_up(inc, off, L);
+        if (r >= off) inc += y;
+    }
+    int run = inc - loc;
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int c = bins[r * PER + b];
+        bins[r * PER + b] = run;
+        run += c;
+    }
+    __syncthreads();
+    for (int k = r; k < n; k += L) {
+        const uint64_t key = mcc_key(id[first + k], k);
+        skey[bins[(int)(key >> SH)] + place[k]] = key;
+    }
+    __syncthreads();
+}
+
+// cell-local index of the electron of rank `rank` (0 <= rank < n) in the key order
+template <int NB>
+__device__ __forceinline__ int mcc_select(const uint64_t* skey, const int* bins, int n, int rank) {
+    // the last bucket starting at or below rank (it holds rank: starts never decrease)
+    int lo = 0, hi = NB - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bins[mid] <= rank) lo = mid;
+        else hi = mid - 1;
+    }
+    const int s0 = bins[lo], s1 = lo + 1 < NB ? bins[lo + 1] : n;
+    const int want = rank - s0;
+    int pick = s0;
+    for (int j = s0; j < s1; ++j) {
+        const uint64_t kj = skey[j];
+        int c = 0;
+        for (int i = s0; i < s1; ++i) c += (int)(skey[i] < kj);
+        if (c == want) pick = j;
+    }
+    return (int)(skey[pick] & ((1u << kMccIndexBits) - 1));
 }
 
 }  // namespace
 
 // the candidates of one cell by one wave; maxp = LDS slots for the cell's electrons
-template <class UT>
-__device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds, int* picks) {
-    uint64_t* key = lds;
-    int* val = (int*)(lds + maxp);
+template <class UT, int REL, int NB>
+__device__ void mcc_cell(const MccArgs<UT>& a, int cell, uint64_t* lds, int* picks, MccPair* pairs, int* npairs) {
+    uint64_t* skey = lds;
     const int a0 = a.start[cell], eC = a.start[cell + 1] - a0;
+    int* place = (int*)(skey + eC);
+    int* bins = place + eC;
     const double smax = a.smax[cell];
     const double coll = a.rem[cell] + eC * a.nN * smax * a.dt;
     const int nc = (int)coll;
@@ -229,23 +356,7 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds
     unsigned c_el = 0, c_ex = 0, c_iz = 0;
     double qtcr = 0.0;
     if (eC > 0 && nc > 0) {
-        // val[r] = the electron of id rank r: each key (id, cell-local index) is distinct and
-        // its rank is the number of smaller keys, counted eight keys per LDS read
-        const int n8 = (eC + 7) & ~7;
-        for (int k = threadIdx.x; k < n8; k += blockDim.x)
-            key[k] = (k < eC) ? (((uint64_t)a.id_cell[a0 + k] << 32) | (uint32_t)k) : ~0ull;
-        __syncthreads();
-        for (int k = threadIdx.x; k < eC; k += blockDim.x) {
-            const uint64_t kk = key[k];
-            int r = 0;
-            for (int j = 0; j < n8; j += 8) {
-                const uint64_t* q = key + j;
-                r += (int)(q[0] < kk) + (int)(q[1] < kk) + (int)(q[2] < kk) + (int)(q[3] < kk) + (int)(q[4] < kk) +
-                     (int)(q[5] < kk) + (int)(q[6] < kk) + (int)(q[7] < kk);
-            }
-            val[r] = slot_index(a.perm, a0 + k);
-        }
-        __syncthreads();
+        mcc_bucket<64, NB>(a.id_cell, a0, eC, skey, place, bins, (int)threadIdx.x);
         for (int base = 0; base < nc; base += 64) {
             const int k = base + (int)threadIdx.x;
             int pick = -1;
@@ -256,17 +367,22 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds
                 if (pick >= eC) pick = eC - 1;
             }
             picks[threadIdx.x] = pick;
+            if (threadIdx.x == 0) *npairs = 0;
             __syncthreads();
             if (k < nc) {
                 bool first = true;
                 for (int l = 0; l < (int)threadIdx.x; ++l)
                     if (picks[l] == pick) { first = false; break; }
                 if (first) {
-                    const int ei = val[pick];
+                    const int ei = slot_index(a.perm, a0 + mcc_select<NB>(skey, bins, eC, pick));
                     for (int l = (int)threadIdx.x; l < 64 && base + l < nc; ++l)
-                        if (picks[l] == pick) candidate(a, cell, base + l, ei, smax, qtcr, c_el, c_ex, c_iz);
+                        if (picks[l] == pick)
+                            candidate<REL>(a, cell, base + l, ei, smax, qtcr, c_el, c_ex, pairs, npairs);
                 }
             }
+            __syncthreads();
+            const int np = *npairs;
+            if ((int)threadIdx.x < np) create_pair<REL>(a, pairs[threadIdx.x], c_iz);
             __syncthreads();
         }
     }
@@ -292,16 +408,18 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, int maxp, uint64_t* lds
 // one wave per cell; cells with more electrons than a.maxp go to the large-cell pass.  Three
 // waves per SIMD with a few spilled registers beat two without (measured on the 100^3 argon
 // case: 6.7 against 8.4 ms per step)
-template <class UT>
+template <class UT, int REL>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_mcc(MccArgs<UT> a) {
     extern __shared__ uint64_t lds[];
     __shared__ int picks[64];
+    __shared__ MccPair pairs[64];
+    __shared__ int npairs;
     const int cell = blockIdx.x;
     if (a.start[cell + 1] - a.start[cell] > a.maxp) {
         if (threadIdx.x == 0) a.big_list[atomicAdd(a.big_n, 1)] = cell;
         return;
     }
-    mcc_cell(a, cell, a.maxp, lds, picks);
+    mcc_cell<UT, REL, kMccBins>(a, cell, lds, picks, pairs, &npairs);
 }
 
 // Several cells per wave: 64 / L cells, L lanes each (L = 16 or 32), for cells of up to a.maxp_group
@@ -310,15 +428,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
 // lanes idle while the wave pays the loads, the id ranking and the barriers for that cell
 // alone.  Same candidate rule as mcc_cell, in rounds of L candidates per cell: a parcel
 // still sees its candidates in candidate order, so the results do not depend on L.
-template <class UT, int L>
+template <class UT, int REL, int L>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_mcc_group(MccArgs<UT> a) {
-    constexpr int W = 64 / L;
+    constexpr int W = 64 / L, NB = 8 * L;
     extern __shared__ uint64_t lds[];
     __shared__ int picks_all[64];
+    __shared__ MccPair pairs[64];
+    __shared__ int npairs;
     const int w = threadIdx.x / L, r = threadIdx.x % L;
     const int maxp = a.maxp_group;
-    uint64_t* key = lds + (size_t)w * (maxp + maxp / 2);
-    int* val = (int*)(key + maxp);
+    uint64_t* skey = lds + (size_t)w * (maxp + maxp / 2 + NB / 2);
+    int* place = (int*)(skey + maxp);
+    int* bins = place + maxp;
     int* picks = picks_all + w * L;
     const int cell = blockIdx.x * W + w;
     bool on = cell < a.n_cells;
@@ -339,25 +460,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     }
     smax = __shfl(smax, w * L);
     nc = __shfl(nc, w * L);
-    const int ncw = (eC > 0) ? nc : 0;
+    const int ncw = (on && eC > 0) ? nc : 0;
     unsigned c_el = 0, c_ex = 0, c_iz = 0;
     double qtcr = 0.0;
-    // val[r] = the electron of id rank r (keys (id, cell-local index) are distinct)
-    const int n8 = ncw > 0 ? (eC + 7) & ~7 : 0;
-    for (int k = r; k < n8; k += L)
-        key[k] = (k < eC) ? (((uint64_t)a.id_cell[a0 + k] << 32) | (uint32_t)k) : ~0ull;
-    __syncthreads();
-    for (int k = r; k < (ncw > 0 ? eC : 0); k += L) {
-        const uint64_t kk = key[k];
-        int rk = 0;
-        for (int j = 0; j < n8; j += 8) {
-            const uint64_t* q = key + j;
-            rk += (int)(q[0] < kk) + (int)(q[1] < kk) + (int)(q[2] < kk) + (int)(q[3] < kk) + (int)(q[4] < kk) +
-                  (int)(q[5] < kk) + (int)(q[6] < kk) + (int)(q[7] < kk);
-        }
-        val[rk] = slot_index(a.perm, a0 + k);
-    }
-    __syncthreads();
+    mcc_bucket<L, NB>(a.id_cell, a0, ncw > 0 ? eC : 0, skey, place, bins, r);
     // rounds of L candidates; every group takes part in the barriers of the busiest one
     int rounds = (ncw + L - 1) / L;
     for (int off = 32; off > 0; off >>= 1) rounds = max(rounds, __shfl_xor(rounds, off));
@@ -371,17 +477,22 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
             if (pick >= eC) pick = eC - 1;
         }
         picks[r] = pick;
+        if (threadIdx.x == 0) npairs = 0;
         __syncthreads();
         if (k < ncw) {
             bool first = true;
             for (int l = 0; l < r; ++l)
                 if (picks[l] == pick) { first = false; break; }
             if (first) {
-                const int ei = val[pick];
+                const int ei = slot_index(a.perm, a0 + mcc_select<NB>(skey, bins, eC, pick));
                 for (int l = r; l < L && base + l < ncw; ++l)
-                    if (picks[l] == pick) candidate(a, cell, base + l, ei, smax, qtcr, c_el, c_ex, c_iz);
+                    if (picks[l] == pick)
+                        candidate<REL>(a, cell, base + l, ei, smax, qtcr, c_el, c_ex, pairs, &npairs);
             }
         }
+        __syncthreads();
+        const int np = npairs;
+        if ((int)threadIdx.x < np) create_pair<REL>(a, pairs[threadIdx.x], c_iz);
         __syncthreads();
     }
     for (int off = L / 2; off > 0; off >>= 1) qtcr = fmax(qtcr, __shfl_xor(qtcr, off));
@@ -402,10 +513,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     }
 }
 
-template <class UT>
+template <class UT, int REL>
 __global__ void __launch_bounds__(64) k_mcc_big(MccArgs<UT> a) {
     extern __shared__ uint64_t lds[];
     __shared__ int picks[64];
+    __shared__ MccPair pairs[64];
+    __shared__ int npairs;
     const int nbig = *a.big_n;
     for (int i = blockIdx.x; i < nbig; i += gridDim.x) {
         const int cell = a.big_list[i];
@@ -413,28 +526,33 @@ __global__ void __launch_bounds__(64) k_mcc_big(MccArgs<UT> a) {
             if (threadIdx.x == 0) atomicOr(a.flags, kFlagCellOverflow);
             continue;
         }
-        mcc_cell(a, cell, a.maxp_big, lds, picks);
+        mcc_cell<UT, REL, kMccBinsBig>(a, cell, lds, picks, pairs, &npairs);
         __syncthreads();
     }
 }
 
-template <class UT>
+template <class UT, int REL>
 static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
     if (a.n_cells <= 0) return hipSuccess;
+    // per cell: maxp keys (8 B) and placements (4 B), and the bucket starts (4 B each)
     if (a.group == 16) {
-        const size_t lds = (size_t)4 * (a.maxp_group + a.maxp_group / 2) * sizeof(uint64_t);
-        hipLaunchKernelGGL((k_mcc_group<UT, 16>), dim3((a.n_cells + 3) / 4), dim3(64), lds, s, a);
+        const size_t lds = (size_t)4 * (a.maxp_group + a.maxp_group / 2 + 8 * 16 / 2) * sizeof(uint64_t);
+        hipLaunchKernelGGL((k_mcc_group<UT, REL, 16>), dim3((a.n_cells + 3) / 4), dim3(64), lds, s, a);
     } else if (a.group == 32) {
-        const size_t lds = (size_t)2 * (a.maxp_group + a.maxp_group / 2) * sizeof(uint64_t);
-        hipLaunchKernelGGL((k_mcc_group<UT, 32>), dim3((a.n_cells + 1) / 2), dim3(64), lds, s, a);
+        const size_t lds = (size_t)2 * (a.maxp_group + a.maxp_group / 2 + 8 * 32 / 2) * sizeof(uint64_t);
+        hipLaunchKernelGGL((k_mcc_group<UT, REL, 32>), dim3((a.n_cells + 1) / 2), dim3(64), lds, s, a);
     } else {
-        const size_t lds = (size_t)a.maxp * (sizeof(uint64_t) + sizeof(int));
-        hipLaunchKernelGGL((k_mcc<UT>), dim3(a.n_cells), dim3(64), lds, s, a);
+        const size_t lds = (size_t)a.maxp * (sizeof(uint64_t) + sizeof(int)) + kMccBins * sizeof(int);
+        hipLaunchKernelGGL((k_mcc<UT, REL>), dim3(a.n_cells), dim3(64), lds, s, a);
     }
-    const size_t lds_big = (size_t)a.maxp_big * (sizeof(uint64_t) + sizeof(int));
+    const size_t lds_big = (size_t)a.maxp_big * (sizeof(uint64_t) + sizeof(int)) + kMccBinsBig * sizeof(int);
     const int nb = a.n_cells < 512 ? a.n_cells : 512;
-    hipLaunchKernelGGL((k_mcc_big<UT>), dim3(nb), dim3(64), lds_big, s, a);
+    hipLaunchKernelGGL((k_mcc_big<UT, REL>), dim3(nb), dim3(64), lds_big, s, a);
     return hipGetLastError();
+}
+template <class UT>
+static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
+    return a.relativistic ? launch_mcc_t<UT, 1>(a, s) : launch_mcc_t<UT, 0>(a, s);
 }
 hipError_t launch_mcc(const MccArgs<double>& a, hipStream_t s) { return launch_mcc_t(a, s); }
 hipError_t launch_mcc(const MccArgs<float>& a, hipStream_t s) { return launch_mcc_t(a, s); }

@@ -104,6 +104,13 @@ __device__ __forceinline__ vec3<T> interp_E(const TetGeo& g, double y1, double y
     }
 }
 
+// the coordinate held in slot k (2 bits) of (Y0, Y1, Y2, Y3), as selects
+__device__ __forceinline__ double pick_slot(uint32_t k, double y0, double y1, double y2, double y3) {
+    const double lo = (k & 1u) ? y1 : y0;
+    const double hi = (k & 1u) ? y3 : y2;
+    return (k & 2u) ? hi : lo;
+}
+
 // Barycentric tracking through the tet decomposition (PICParcel::moveForward,
 // PICParcel.C:81-144, with OpenFOAM-9 particle::trackToFace re-derived): per tet the
 // remaining displacement (1-sf)*dt*U becomes Tx_i = r_i.d (det-scaled barycentric);
@@ -134,23 +141,24 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
         const double det = g.det;
         const bool dok = det > 0.0 && __builtin_isnormal(det);
         // first face hit: the smallest mu_i = -Y_i / T_i >= 0 over T_i < 0 below the tet's
-        // end 1/det, found by comparing the fractions crosswise; one division for the winner
+        // end 1/det, found by comparing the fractions crosswise (as selects, in face order);
+        // one division for the winner
         double bn = 1.0, bd = dok ? det : 1e-300;
         int iH = -1;
-        if (T0 < 0.0 && Y0 >= 0.0 && Y0 * bd < -T0 * bn) { bn = Y0; bd = -T0; iH = 0; }
-        if (T1 < 0.0 && Y1 >= 0.0 && Y1 * bd < -T1 * bn) { bn = Y1; bd = -T1; iH = 1; }
-        if (T2 < 0.0 && Y2 >= 0.0 && Y2 * bd < -T2 * bn) { bn = Y2; bd = -T2; iH = 2; }
-        if (T3 < 0.0 && Y3 >= 0.0 && Y3 * bd < -T3 * bn) { bn = Y3; bd = -T3; iH = 3; }
+        bool h = T0 < 0.0 && Y0 >= 0.0 && Y0 * bd < -T0 * bn;
+        bn = h ? Y0 : bn; bd = h ? -T0 : bd; iH = h ? 0 : iH;
+        h = T1 < 0.0 && Y1 >= 0.0 && Y1 * bd < -T1 * bn;
+        bn = h ? Y1 : bn; bd = h ? -T1 : bd; iH = h ? 1 : iH;
+        h = T2 < 0.0 && Y2 >= 0.0 && Y2 * bd < -T2 * bn;
+        bn = h ? Y2 : bn; bd = h ? -T2 : bd; iH = h ? 2 : iH;
+        h = T3 < 0.0 && Y3 >= 0.0 && Y3 * bd < -T3 * bn;
+        bn = h ? Y3 : bn; bd = h ? -T3 : bd; iH = h ? 3 : iH;
         const double muH = (iH >= 0 || dok) ? bn / bd : 1e300;
-        Y0 = Y0 + muH * T0;
-        Y1 = Y1 + muH * T1;
-        Y2 = Y2 + muH * T2;
-        Y3 = Y3 + muH * T3;
+        Y0 = (iH == 0) ? 0.0 : Y0 + muH * T0;
+        Y1 = (iH == 1) ? 0.0 : Y1 + muH * T1;
+        Y2 = (iH == 2) ? 0.0 : Y2 + muH * T2;
+        Y3 = (iH == 3) ? 0.0 : Y3 + muH * T3;
         if (iH < 0) break;
-        if (iH == 0) Y0 = 0.0;
-        if (iH == 1) Y1 = 0.0;
-        if (iH == 2) Y2 = 0.0;
-        if (iH == 3) Y3 = 0.0;
         double frac = muH * det;
         frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
         sf = sf + f * frac;
@@ -160,13 +168,12 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
             t = -1;
             break;
         }
+        // slot j of the neighbour takes the coordinate of this tet's slot src_j
         const uint32_t p = (perm >> (8 * iH)) & 0xffu;
-        const uint32_t p0 = p & 3u, p1 = (p >> 2) & 3u, p2 = (p >> 4) & 3u;
-        // new slot p_k receives Y_k; written as selects so that nothing leaves registers
-        double n0 = (p0 == 0) ? Y0 : (p1 == 0) ? Y1 : (p2 == 0) ? Y2 : Y3;
-        double n1 = (p0 == 1) ? Y0 : (p1 == 1) ? Y1 : (p2 == 1) ? Y2 : Y3;
-        double n2 = (p0 == 2) ? Y0 : (p1 == 2) ? Y1 : (p2 == 2) ? Y2 : Y3;
-        double n3 = (p0 == 3) ? Y0 : (p1 == 3) ? Y1 : (p2 == 3) ? Y2 : Y3;
+        const double n0 = pick_slot(p & 3u, Y0, Y1, Y2, Y3);
+        const double n1 = pick_slot((p >> 2) & 3u, Y0, Y1, Y2, Y3);
+        const double n2 = pick_slot((p >> 4) & 3u, Y0, Y1, Y2, Y3);
+        const double n3 = pick_slot(p >> 6, Y0, Y1, Y2, Y3);
         Y0 = n0; Y1 = n1; Y2 = n2; Y3 = n3;
         t = nb;
         if (++guard > kMaxTrackSteps) { status = 1; break; }
@@ -246,9 +253,10 @@ __device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block) 
 // so the short passes of slow species overlap the long tracks of fast ones
 template <class UT, int P, int W, int DEP>
 __global__ void __launch_bounds__(256) k_move(MoveSet<UT> m) {
+    const int64_t b = blockIdx.x;
     int s = 0;
-    while (s + 1 < m.ns && (int64_t)blockIdx.x >= m.block0[s + 1]) ++s;
-    move_body<UT, P, W, DEP>(m.a[s], (int64_t)blockIdx.x - m.block0[s]);
+    while (s + 1 < m.ns && b >= m.block0[s + 1]) ++s;
+    move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s]);
 }
 
 template <class UT, int P, int W>
@@ -389,13 +397,14 @@ __global__ void k_init(InitArgs<UT> a) {
 // Every species in one launch, blocks [block0[s], block0[s+1]) for species s.
 template <class UT>
 __global__ void __launch_bounds__(256) k_scatter(ScatterSet<UT> m) {
+    const int64_t b = blockIdx.x;
     int sp = 0;
-    while (sp + 1 < m.ns && (int64_t)blockIdx.x >= m.block0[sp + 1]) ++sp;
+    while (sp + 1 < m.ns && b >= m.block0[sp + 1]) ++sp;
     const PartSoA<UT>& src = m.src[sp];
     const PartSoA<UT>& dst = m.dst[sp];
     const int2* __restrict__ place = m.place[sp];
     const int* __restrict__ start = m.start[sp];
-    const int64_t i = ((int64_t)blockIdx.x - m.block0[sp]) * blockDim.x + threadIdx.x;
+    const int64_t i = (b - m.block0[sp]) * blockDim.x + threadIdx.x;
     // the live count before the sort (the scan left the sorted count aside); it can exceed
     // the arrays after a capacity overflow (flagged)
     if (i >= min(*m.n_live[sp], m.n[sp])) return;

@@ -205,6 +205,14 @@ std::string build_host_mesh(HostMesh& m) {
         for (int i = 0; i < 4; ++i) g.nbr[i] = -1000000;
         g.perm = 0;
     }
+    // packed per tet face: bits 2j..2j+1 hold the slot of this tet whose coordinate moves to
+    // slot j of the neighbour (map[k] is the neighbour slot of this tet's slot k; the hit
+    // face's own slot, whose coordinate is zero on the face, goes to the new vertex)
+    auto source_code = [](const int map[4]) {
+        uint32_t code = 0;
+        for (int k = 0; k < 4; ++k) code |= (uint32_t)k << (2 * map[k]);
+        return code;
+    };
     for (int c = 0; c < nc; ++c) {
         int t0 = m.cell_tet0[c], t1 = m.cell_tet0[c + 1];
         for (int t = t0; t < t1; ++t) {
@@ -232,10 +240,8 @@ std::string build_host_mesh(HostMesh& m) {
                     if (sv[j] == Q) { map[slots[1]] = j + 1; used[j + 1] = true; }
                 }
                 for (int j = 1; j <= 3; ++j) if (!used[j]) map[i] = j;
-                uint32_t code = 0;
-                for (int kk = 0; kk < 4; ++kk) code |= (uint32_t)map[kk] << (2 * kk);
                 g.nbr[i] = found;
-                g.perm |= code << (8 * i);
+                g.perm |= source_code(map) << (8 * i);
             }
             // face triangle i = 0
             int f = m.tet_face[t], pt = m.tet_pt[t];
@@ -250,10 +256,8 @@ std::string build_host_mesh(HostMesh& m) {
                 int map[4] = {0, -1, -1, -1};
                 for (int a = 0; a < 3; ++a)
                     for (int b = 0; b < 3; ++b) if (w[b] == vid[a]) map[a + 1] = b + 1;
-                uint32_t code = 0;
-                for (int kk = 0; kk < 4; ++kk) code |= (uint32_t)map[kk] << (2 * kk);
                 g.nbr[0] = found;
-                g.perm |= code;
+                g.perm |= source_code(map);
             } else {
                 int pi = m.face_patch[f];
                 if (m.ptype[pi] != 2) { g.nbr[0] = -1 - pi; continue; }
@@ -279,10 +283,8 @@ std::string build_host_mesh(HostMesh& m) {
                     if (ok) { found = s; std::memcpy(map, mp, sizeof map); }
                 }
                 if (found < 0) return "cyclic tet partner missing (face triangulations differ)";
-                uint32_t code = 0;
-                for (int kk = 0; kk < 4; ++kk) code |= (uint32_t)map[kk] << (2 * kk);
                 g.nbr[0] = found;
-                g.perm |= code;
+                g.perm |= source_code(map);
             }
         }
     }

@@ -387,8 +387,9 @@ def test_mcc_fp64_parity(oracle, model, xs_model, case):
     candidates per cell (several rounds per cell, parcels picked more than once);
     two-per-wave: 130-260 electrons per cell (two cells per wave); per-cell: 300-500
     electrons per cell (one cell per wave); many-mixed: the many case with FP32 velocities
-    (inputs FP32-representable, the MCC arithmetic FP64 as in the FP64 build, so the
-    results differ from the oracle by the final rounding to FP32).  One cell of 700
+    (inputs FP32-representable, the MCC arithmetic FP64 as in the FP64 build; a velocity is
+    rounded to FP32 after each candidate that picks it, up to a few per step here, so the
+    results differ from the oracle by those roundings: 3e-7 per parcel).  One cell of 700
     electrons takes the large-cell pass in every case; one cell has no electrons and one a
     single electron."""
     say("mcc parity", model, xs_model, case)
@@ -449,7 +450,7 @@ def test_mcc_fp64_parity(oracle, model, xs_model, case):
     (Ueg_s, ieg_s) = sort_by_id(ieg, Ueg, ieg)
     (Uer_s, ier_s) = sort_by_id(out["idE"], out["UE"], out["idE"])
     assert np.array_equal(ieg_s, ier_s)
-    utol = 1e-7 if mixed else 1e-11
+    utol = 3e-7 if mixed else 1e-11
     assert rel_err(Ueg_s, Uer_s) < utol
     (Uig_s,) = sort_by_id(iig, Uig)
     (Uir_s,) = sort_by_id(out["idI"], out["UI"])
@@ -758,7 +759,9 @@ def test_index_sort_equals_moving_sort(case):
 
     def make(interval):
         if case == "mcc":
-            mesh, species, models, ex = cases.argon_mcc(ncell=4, ppc=60, precision="fp64", L=1e-3, growth=20.0)
+            # 10 eV electrons: a few ionizations per step append parcels between moving sorts
+            mesh, species, models, ex = cases.argon_mcc(ncell=4, ppc=60, precision="fp64", L=1e-3, growth=20.0,
+                                                        T_eV=10.0)
             models["seed"] = 3
             models["sortInterval"] = interval
             c = picmi.Cloud(mesh, species, models)
