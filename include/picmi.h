@@ -206,7 +206,8 @@ int pic_init_maxwellian(pic_ctx* ctx, int32_t species, int32_t nppc, double temp
 
 /* stages of PICCloud::evolve (PICCloud.C:1426-1471) */
 int pic_push(pic_ctx* ctx, double dt);                 /* ParticlePusher::updateVelocity only */
-int pic_move(pic_ctx* ctx, double dt);                 /* Cloud::move: push + interpolate + track */
+int pic_move(pic_ctx* ctx, double dt);                 /* Cloud::move: push + interpolate + track (the
+                                                          tracking stage; PICParcel::move pushes first) */
 int pic_sort(pic_ctx* ctx);                            /* buildCellOccupancy */
 int pic_collide_coulomb(pic_ctx* ctx, double dt);      /* CoulombCollisionModel::handleCollisions */
 int pic_collide_mcc(pic_ctx* ctx, double dt);          /* ElectronNeutralCollisionModel::handleCollisions */
@@ -246,7 +247,8 @@ int pic_gnn_snapshot(pic_ctx* ctx, int32_t index, float* features, float* target
 int pic_gnn_grad(pic_ctx* ctx, int32_t index, float* grad, double* loss);
 int pic_gnn_train(pic_ctx* ctx, int32_t steps, double lr, uint64_t seed, double* loss_first, double* loss_last);
 int pic_gnn_eval(pic_ctx* ctx, int32_t index, double* loss);
-int pic_gnn_use(pic_ctx* ctx, int32_t on);            /* start the PCG from the network's guess */
+int pic_gnn_use(pic_ctx* ctx, int32_t on);            /* start the PCG from the network's guess (its
+                                                          inference runs inside pic_poisson_solve) */
 /* training loss: 0 = A-norm error e^T A e / d^T A d, 1 = residual |A e|^2 / |A d|^2, 2 = their sum */
 int pic_gnn_loss(pic_ctx* ctx, int32_t kind);
 

@@ -70,6 +70,7 @@ struct ref_mesh {
     int *cfo, *cf;               /* cell -> faces, OpenFOAM primitiveMesh::calcCells order */
     int n_tets;
     int *tet_cell, *tet_face, *tet_pt, *cell_tet0;
+    int* fbase;                  /* face base point, an index into the face's point list */
     int* pmaster;                /* cyclic-coupled point groups -> smallest point id */
     int* face_patch;             /* boundary face -> patch index (-1 for internal) */
     double* psep;                /* per patch: x_partner = x + sep */
@@ -158,6 +159,38 @@ static void cell_geometry(ref_mesh* m) {
         m->V[c] = vol[c] / 3.0;
     }
     free(est); free(nf); free(ctr); free(vol);
+}
+
+/* tetrahedron::quality (OpenFOAM-9): signed volume over (8/(9 sqrt 3)) R^3, R the
+ * circumradius from the circumcentre offset (|u|^2 v x w + |v|^2 w x u + |w|^2 u x v) /
+ * (2 u . v x w) of the edges u, v, w from a */
+static double tet_quality(v3 a, v3 b, v3 c, v3 d) {
+    v3 u = vsub(b, a), v = vsub(c, a), w = vsub(d, a);
+    double t6 = vdot(u, vcross(v, w));
+    if (t6 == 0.0) return 0.0;
+    v3 o = vadd(vadd(vscale(vdot(u, u), vcross(v, w)), vscale(vdot(v, v), vcross(w, u))), vscale(vdot(w, w), vcross(u, v)));
+    double R = vmag(vscale(1.0 / (2.0 * t6), o));
+    return (t6 / 6.0) / (8.0 / (9.0 * sqrt(3.0)) * R * R * R + 1e-150);
+}
+
+/* polyMeshTetDecomposition::findSharedBasePoint / findBasePoint (OpenFOAM-9): the first
+ * face point from which the minimum fan-tet quality with the owner centre (and the
+ * neighbour centre nc, A and B swapped, when given) exceeds minTetQuality = 1e-15 */
+static int face_base_point(const ref_mesh* m, int f, v3 oc, const v3* nc) {
+    const int* fp = m->fp + m->fo[f];
+    int n = m->fo[f + 1] - m->fo[f];
+    for (int b = 0; b < n; ++b) {
+        v3 pb = ld3(m->points + 3 * fp[b]);
+        double q = 1e300;
+        for (int i = 1; i <= n - 2; ++i) {
+            v3 pA = ld3(m->points + 3 * fp[(b + i) % n]), pB = ld3(m->points + 3 * fp[(b + i + 1) % n]);
+            double qo = tet_quality(oc, pb, pA, pB);
+            if (qo < q) q = qo;
+            if (nc) { double qn = tet_quality(*nc, pb, pB, pA); if (qn < q) q = qn; }
+        }
+        if (q > 1e-15) return b;
+    }
+    return -1;
 }
 
 static int uf_find(int* p, int a) {
@@ -261,6 +294,40 @@ ref_mesh* ref_mesh_create(int n_points, const double* points, int n_faces, const
     }
     for (int p = 0; p < n_points; ++p) m->pmaster[p] = uf_find(m->pmaster, p);
 
+    /* face base points (polyMeshTetDecomposition::findFaceBasePts, OpenFOAM-9): internal
+     * faces with both cell centres; on a cyclic pair the lower-numbered patch with the
+     * partner cell's centre carried across by the separation, and the partner face with the
+     * matching point (point j of one face is point n - j of the other); other boundary faces
+     * with the owner alone.  No qualifying point: point 0 */
+    m->fbase = (int*)malloc(sizeof(int) * n_faces);
+    for (int f = 0; f < n_faces; ++f) {
+        v3 oc = ld3(m->C + 3 * owner[f]);
+        m->fbase[f] = -1;
+        if (f < n_internal) {
+            v3 nc = ld3(m->C + 3 * neighbour[f]);
+            m->fbase[f] = face_base_point(m, f, oc, &nc);
+        } else {
+            int pi = m->face_patch[f];
+            if (m->ptype[pi] == REF_PATCH_CYCLIC) {
+                int q = m->pnbr[pi];
+                if (pi < q) {
+                    int g = m->pstart[q] + (f - m->pstart[pi]);
+                    v3 nc = vsub(ld3(m->C + 3 * owner[g]), ld3(m->psep + 3 * pi));
+                    m->fbase[f] = face_base_point(m, f, oc, &nc);
+                }
+            } else {
+                m->fbase[f] = face_base_point(m, f, oc, NULL);
+            }
+        }
+    }
+    for (int f = n_internal; f < n_faces; ++f) {
+        int pi = m->face_patch[f];
+        if (m->ptype[pi] != REF_PATCH_CYCLIC || pi < m->pnbr[pi]) continue;
+        int q = m->pnbr[pi], g = m->pstart[q] + (f - m->pstart[pi]), n = m->fo[f + 1] - m->fo[f];
+        m->fbase[f] = m->fbase[g] < 0 ? -1 : (n - m->fbase[g]) % n;
+    }
+    for (int f = 0; f < n_faces; ++f) if (m->fbase[f] < 0) m->fbase[f] = 0;
+
     /* cellPoints (ascending point id) + weights */
     m->cpo = (int*)malloc(sizeof(int) * (n_cells + 1));
     int cap = 0;
@@ -315,7 +382,7 @@ void ref_mesh_destroy(ref_mesh* m) {
     free(m->ptype); free(m->pstart); free(m->psize); free(m->pnbr);
     free(m->Cf); free(m->Sf); free(m->C); free(m->V); free(m->cfo); free(m->cf);
     free(m->tet_cell); free(m->tet_face); free(m->tet_pt); free(m->cell_tet0);
-    free(m->pmaster); free(m->face_patch); free(m->psep);
+    free(m->pmaster); free(m->face_patch); free(m->psep); free(m->fbase);
     free(m->cpo); free(m->cp); free(m->vw_w); free(m->ip_w);
     free(m);
 }
@@ -338,14 +405,17 @@ void ref_mesh_tets(const ref_mesh* m, int* tc, int* tf, int* tp) {
 
 void ref_mesh_point_master(const ref_mesh* m, int* pm) { memcpy(pm, m->pmaster, sizeof(int) * m->n_points); }
 
-/* tetIndices::faceTriIs (OpenFOAM-9): (base, A, B) = (f[0], f[pt], f[pt+1]) for the owner,
- * A and B swapped when the cell is the face's neighbour. */
+void ref_mesh_face_base(const ref_mesh* m, int* base) { memcpy(base, m->fbase, sizeof(int) * m->n_faces); }
+
+/* tetIndices::faceTriIs (OpenFOAM-9): (base, A, B) = (f[b], f[b+pt], f[b+pt+1]) around the
+ * face from its base point b, A and B swapped when the cell is the face's neighbour. */
 static void tet_points(const ref_mesh* m, int t, int vid[3]) {
     int f = m->tet_face[t], pt = m->tet_pt[t], c = m->tet_cell[t];
     const int* fp = m->fp + m->fo[f];
-    vid[0] = fp[0];
-    if (m->owner[f] == c) { vid[1] = fp[pt]; vid[2] = fp[pt + 1]; }
-    else { vid[1] = fp[pt + 1]; vid[2] = fp[pt]; }
+    int n = m->fo[f + 1] - m->fo[f], b = m->fbase[f];
+    vid[0] = fp[b];
+    if (m->owner[f] == c) { vid[1] = fp[(b + pt) % n]; vid[2] = fp[(b + pt + 1) % n]; }
+    else { vid[1] = fp[(b + pt + 1) % n]; vid[2] = fp[(b + pt) % n]; }
 }
 
 static void tet_vertices(const ref_mesh* m, int t, v3 x[4], int vid[3]) {

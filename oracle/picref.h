@@ -42,6 +42,8 @@ void ref_mesh_geometry(const ref_mesh* m, double* cell_centres, double* cell_vol
                        double* face_areas);
 void ref_mesh_tets(const ref_mesh* m, int* tet_cell, int* tet_face, int* tet_pt);
 void ref_mesh_point_master(const ref_mesh* m, int* pmaster);
+/* face base point per face (an index into its point list) */
+void ref_mesh_face_base(const ref_mesh* m, int* base);
 
 /* positions <-> (tet, barycentric y1..y3) */
 void ref_positions(const ref_mesh* m, int n, const double* y, const int* tet, double* x);

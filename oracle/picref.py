@@ -44,6 +44,7 @@ def lib():
         _lib.ref_mesh_geometry.argtypes = [vp, vp, vp, vp, vp]
         _lib.ref_mesh_tets.argtypes = [vp, vp, vp, vp]
         _lib.ref_mesh_point_master.argtypes = [vp, vp]
+        _lib.ref_mesh_face_base.argtypes = [vp, vp]
         _lib.ref_positions.argtypes = [vp, C.c_int, vp, vp, vp]
         _lib.ref_locate.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         _lib.ref_locate.restype = C.c_int
@@ -123,6 +124,11 @@ class RefMesh:
         tc = np.empty(self.n_tets, np.int32); tf = np.empty(self.n_tets, np.int32); tp = np.empty(self.n_tets, np.int32)
         lib().ref_mesh_tets(self.h, _p(tc), _p(tf), _p(tp))
         return tc, tf, tp
+
+    def face_base(self):
+        b = np.empty(len(self.mesh["owner"]), np.int32)
+        lib().ref_mesh_face_base(self.h, _p(b))
+        return b
 
     def point_master(self):
         pm = np.empty(self.n_points, np.int32)

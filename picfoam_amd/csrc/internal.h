@@ -55,6 +55,7 @@ struct HostMesh {
     std::vector<int32_t> cfo, cf;              // cell -> faces (OpenFOAM cells() order)
     std::vector<int32_t> cell_tet0;            // [n_cells+1]
     std::vector<int32_t> tet_cell, tet_face, tet_pt;
+    std::vector<int32_t> fbase;                // face base point (index into the face's points)
     std::vector<int32_t> pmaster;              // coupled point groups
     std::vector<int32_t> face_patch;
     std::vector<double> psep;                  // per patch separation

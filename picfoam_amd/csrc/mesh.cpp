@@ -31,13 +31,47 @@ int uf_find(std::vector<int32_t>& p, int a) {
     return a;
 }
 
-// tetIndices::faceTriIs: (f[0], f[pt], f[pt+1]) for the owner, A/B swapped for the neighbour
+// tetIndices::faceTriIs: (f[b], f[b+pt], f[b+pt+1]) around the face from its base point b,
+// A/B swapped for the neighbour
 void tet_points(const HostMesh& m, int t, int vid[3]) {
     int f = m.tet_face[t], pt = m.tet_pt[t], c = m.tet_cell[t];
     const int32_t* fp = m.fp.data() + m.fo[f];
-    vid[0] = fp[0];
-    if (m.owner[f] == c) { vid[1] = fp[pt]; vid[2] = fp[pt + 1]; }
-    else { vid[1] = fp[pt + 1]; vid[2] = fp[pt]; }
+    const int n = m.fo[f + 1] - m.fo[f], b = m.fbase.empty() ? 0 : m.fbase[f];
+    vid[0] = fp[b];
+    const int pa = fp[(b + pt) % n], pb = fp[(b + pt + 1) % n];
+    if (m.owner[f] == c) { vid[1] = pa; vid[2] = pb; }
+    else { vid[1] = pb; vid[2] = pa; }
+}
+
+// quality of the tet (a, b, c, d): its signed volume over the volume of the regular tet of
+// the same circumradius R, (8 / (9 sqrt 3)) R^3 (1 for a regular tet, <= 0 when inverted or
+// flat), the measure OpenFOAM's tetrahedron::quality takes
+double tet_quality(v3 a, v3 b, v3 c, v3 d) {
+    const v3 u = b - a, v = c - a, w = d - a;
+    const double t6 = dot(u, cross(v, w));
+    if (t6 == 0.0) return 0.0;
+    const v3 o = (1.0 / (2.0 * t6)) * ((dot(u, u) * cross(v, w) + dot(v, v) * cross(w, u)) + dot(w, w) * cross(u, v));
+    const double R = mag(o);
+    return (t6 / 6.0) / (8.0 / (9.0 * std::sqrt(3.0)) * R * R * R + 1e-150);
+}
+
+// the first point of face f from which every fan tet has quality above minTetQuality
+// (1e-15) with the owner cell centre oc and, if given, the neighbour cell centre nc (its
+// tets with A and B swapped); -1 when no point qualifies
+int face_base_point(const HostMesh& m, int f, v3 oc, const v3* nc) {
+    const int32_t* fp = m.fp.data() + m.fo[f];
+    const int n = m.fo[f + 1] - m.fo[f];
+    for (int b = 0; b < n; ++b) {
+        const v3 pb = ld(&m.points[3 * fp[b]]);
+        double q = 1e300;
+        for (int i = 1; i <= n - 2; ++i) {
+            const v3 pA = ld(&m.points[3 * fp[(b + i) % n]]), pB = ld(&m.points[3 * fp[(b + i + 1) % n]]);
+            q = std::min(q, tet_quality(oc, pb, pA, pB));
+            if (nc) q = std::min(q, tet_quality(*nc, pb, pB, pA));
+        }
+        if (q > 1e-15) return b;
+    }
+    return -1;
 }
 }  // namespace
 
@@ -184,6 +218,44 @@ std::string build_host_mesh(HostMesh& m) {
         }
     }
     for (int p = 0; p < np; ++p) m.pmaster[p] = uf_find(m.pmaster, p);
+
+    // face base points (polyMeshTetDecomposition::findFaceBasePts of OpenFOAM-9, restated):
+    // internal faces test both cell centres, a cyclic face on the lower-numbered patch of its
+    // pair tests its own cell centre and the partner cell's centre carried across, and the
+    // partner face then takes the matching point (the two faces list their points in
+    // opposite order from a common point 0, so point j there is point n - j here); other
+    // boundary faces test the owner alone.  A face without a qualifying point keeps point 0.
+    m.fbase.assign(nf, 0);
+    {
+        std::vector<int> base(nf, -1);
+        for (int f = 0; f < nf; ++f) {
+            const v3 oc = ld(&m.C[3 * m.owner[f]]);
+            if (f < ni) {
+                const v3 nc = ld(&m.C[3 * m.nbr[f]]);
+                base[f] = face_base_point(m, f, oc, &nc);
+                continue;
+            }
+            const int pi = m.face_patch[f];
+            if (m.ptype[pi] == 2) {
+                const int q = m.pnbr[pi];
+                if (pi > q) continue;   // the partner side follows below
+                const int g = m.pstart[q] + (f - m.pstart[pi]);
+                const v3 nc = ld(&m.C[3 * m.owner[g]]) - ld(&m.psep[3 * pi]);
+                base[f] = face_base_point(m, f, oc, &nc);
+            } else {
+                base[f] = face_base_point(m, f, oc, nullptr);
+            }
+        }
+        for (int f = ni; f < nf; ++f) {
+            const int pi = m.face_patch[f];
+            if (m.ptype[pi] != 2 || pi < m.pnbr[pi]) continue;
+            const int q = m.pnbr[pi];
+            const int g = m.pstart[q] + (f - m.pstart[pi]);
+            const int n = m.fo[f + 1] - m.fo[f];
+            base[f] = base[g] < 0 ? -1 : (n - base[g]) % n;
+        }
+        for (int f = 0; f < nf; ++f) m.fbase[f] = base[f] < 0 ? 0 : base[f];
+    }
 
     // tet geometry + adjacency
     const int64_t nt = m.tet_cell.size();

@@ -67,3 +67,38 @@ def periodic_diff(a, b, L):
     d = np.asarray(a) - np.asarray(b)
     L = np.asarray(L, dtype=np.float64)
     return d - L * np.round(d / L)
+
+
+def dart_hex(cyclic_x=False):
+    """Two hex cells stacked in z (1 x 1 x 2 box) whose shared face is made concave: its
+    second point (in face order) moves inside the triangle of the first, third and fourth,
+    so the fan from the face's point 0 folds over.  With cyclic_x, instead the x = 0 face of
+    the lower cell is made concave the same way, and the matching point of its cyclic
+    partner on x = 1 moves with it (the pair stays a translate)."""
+    m = meshgen.hex_box(1, 1, 2, 1.0, 1.0, 2.0, cyclic=(cyclic_x, False, False))
+    fo, fp = m["face_offsets"], m["face_points"]
+    P = m["points"].copy()
+
+    def fold(f):
+        pts = fp[fo[f]:fo[f + 1]]
+        ctr = P[pts].mean(0)
+        return pts[1], ctr + 0.3 * (P[pts[3]] - ctr)
+
+    if not cyclic_x:
+        b, new = fold(0)          # the only internal face
+        P[b] = new
+    else:
+        px = [p for p in m["patches"] if p["type"] == "cyclic"]
+        lo = min(px, key=lambda p: P[fp[fo[p["start"]]:fo[p["start"] + 1]]][:, 0].mean())
+        hi = px[1] if px[0] is lo else px[0]
+        f = lo["start"]
+        g = hi["start"]
+        b, new = fold(f)
+        sep = np.array([1.0, 0.0, 0.0])
+        old = P[b].copy()
+        gp = fp[fo[g]:fo[g + 1]]
+        bg = gp[np.argmin(np.linalg.norm(P[gp] - (old + sep), axis=1))]
+        P[b] = new
+        P[bg] = new + sep
+    m["points"] = P
+    return m

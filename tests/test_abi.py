@@ -89,3 +89,52 @@ def test_new_model_names_are_accepted(key, good):
     with pytest.raises(picmi.PicError) as ei:
         picmi.Cloud(m, sp, models)
     assert ei.value.code == -10, str(ei.value)
+
+
+# ------------------------------------------------------------------ struct layouts
+def _ctypes_structs():
+    from picfoam_amd import picmi
+    return {"pic_patch": picmi.pic_patch, "pic_mesh": picmi.pic_mesh, "pic_species": picmi.pic_species,
+            "pic_models": picmi.pic_models, "pic_poisson_stats": picmi.pic_poisson_stats,
+            "pic_step_stats": picmi.pic_step_stats, "pic_diag": picmi.pic_diag}
+
+
+def test_header_struct_layouts_match_the_python_mirror(tmp_path):
+    """A C program compiled against include/picmi.h prints sizeof and offsetof of every
+    field of every ABI struct; each equals the ctypes mirror in picfoam_amd/picmi.py (a
+    field the header lacks fails to compile, one the mirror lacks changes a size)."""
+    import json
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = _ctypes_structs()
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "picmi.h"', "int main(void) {", 'printf("{");']
+    for name, cls in structs.items():
+        lines.append(f'printf("\\"{name}\\": [%zu", sizeof({name}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf(", [%zu, %zu]", offsetof({name}, {fname}), sizeof((({name}*)0)->{fname}));')
+        lines.append('printf("], ");')
+    lines += ['printf("\\"end\\": 0}\\n");', "return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    got = json.loads(subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout)
+    import ctypes as C
+    for name, cls in structs.items():
+        size, *fields = got[name]
+        assert size == C.sizeof(cls), name
+        for (off, sz), (fname, _) in zip(fields, cls._fields_):
+            d = getattr(cls, fname)
+            assert (off, sz) == (d.offset, d.size), (name, fname, (off, sz), (d.offset, d.size))
+
+
+def test_cpp_program_against_the_header_builds():
+    """tests/abi/abi_smoke.cpp (run on the GPU by test_gpu_abi.py) compiles against the
+    header with -Wall and links libpicmi.so"""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "abi")], check=True, capture_output=True)
+    assert os.path.exists(os.path.join(ROOT, "tests", "abi", "bin", "abi_smoke"))

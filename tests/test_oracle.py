@@ -648,3 +648,75 @@ def test_relativistic_en_scatter_matches_nanbu_at_low_speed(oracle):
             c = g0 @ g1 / (np.linalg.norm(g0) * np.linalg.norm(g1))
             assert abs(c - cos_expect) < 1e-7, (rel, c, cos_expect)
             assert abs(np.linalg.norm(g1) / np.linalg.norm(g0) - 1) < 1e-7
+
+
+# ------------------------------------------------------------------ face base points
+def _tet_volumes(rm):
+    """signed volumes of every tet of the decomposition, from the positions of its vertices"""
+    vols = np.empty(rm.n_tets)
+    t = np.arange(rm.n_tets, dtype=np.int32)
+    X0 = rm.positions(np.zeros((rm.n_tets, 3)), t)
+    E = []
+    for k in range(3):
+        e = np.zeros((rm.n_tets, 3)); e[:, k] = 1.0
+        E.append(rm.positions(e, t) - X0)
+    vols[:] = np.einsum("ij,ij->i", E[0], np.cross(E[1], E[2])) / 6.0
+    return vols
+
+
+def _first_positive_base(m, f, centres):
+    """brute force: the first face point from which every fan triangle makes a positive
+    tet with each given cell centre (owner centre first, then the neighbour's with the
+    triangle reversed)"""
+    P = m["points"]
+    pts = m["face_points"][m["face_offsets"][f]:m["face_offsets"][f + 1]]
+    n = len(pts)
+    for b in range(n):
+        ok = True
+        for i in range(1, n - 1):
+            A, B = P[pts[(b + i) % n]], P[pts[(b + i + 1) % n]]
+            for k, c in enumerate(centres):
+                a_, b_ = (A, B) if k == 0 else (B, A)
+                if np.dot(P[pts[b]] - c, np.cross(a_ - c, b_ - c)) <= 0:
+                    ok = False
+        if ok:
+            return b
+    return -1
+
+
+def test_face_base_points_convex_meshes_use_point_zero(oracle):
+    """on meshes of convex cells with planar faces every face keeps its point 0"""
+    for m in (small_hex(), small_tet(), small_hex(cyclic=(False, True, False))):
+        assert np.all(oracle.RefMesh(m).face_base() == 0)
+
+
+@pytest.mark.parametrize("cyclic_x", [False, True])
+def test_face_base_points_of_a_concave_face(oracle, cyclic_x):
+    """polyMeshTetDecomposition's base point choice: on a face whose fan from point 0
+    folds over, the base moves to the first point from which every fan tet is positive
+    (brute force over the points, with both cell centres for an internal face); every tet
+    of the decomposition then has positive volume; the two faces of a cyclic pair take
+    periodic images of one point as their base."""
+    from helpers import dart_hex
+    m = dart_hex(cyclic_x=cyclic_x)
+    rm = oracle.RefMesh(m)
+    base = rm.face_base()
+    cc, _, _, _ = rm.geometry()
+    ni = len(m["neighbour"])
+    assert base[0] == _first_positive_base(m, 0, [cc[m["owner"][0]], cc[m["neighbour"][0]]])
+    if not cyclic_x:
+        assert base[0] == 1
+    for f in range(ni, len(m["owner"])):
+        p = next(p for p in m["patches"] if p["start"] <= f < p["start"] + p["size"])
+        if p["type"] != "cyclic":
+            assert base[f] == _first_positive_base(m, f, [cc[m["owner"][f]]]), f
+    assert np.all(_tet_volumes(rm) > 0)
+    if cyclic_x:
+        fo, fp, P = m["face_offsets"], m["face_points"], m["points"]
+        pc = [p for p in m["patches"] if p["type"] == "cyclic"]
+        f, g = pc[0]["start"], pc[1]["start"]
+        assert base[f] != 0 or base[g] != 0
+        bf = P[fp[fo[f] + base[f]]]
+        bg = P[fp[fo[g] + base[g]]]
+        d = bg - bf
+        assert np.allclose(np.abs(d), [1.0, 0.0, 0.0], atol=1e-12)
