@@ -195,6 +195,37 @@ def run_magnetised_tets(steps, warmup, sort_interval=8):
             "precision": "FP32 push (Higuera-Cary), FP64 track, int64 deposit", "sort_interval": sort_interval}
 
 
+def run_vw_field_solve(steps, warmup, ncell=100, ppc=8):
+    """The field-solve plasma of the Poisson block (100^3 hex, 8M electrons over a
+    neutralising background, phi = 0 on the x walls) with VolumeWeighting field weighting and
+    charge distribution: the move kernel then deposits each parcel's barycentric shares on
+    its cell and three points (LDS-privatised int64 sums), and the solve is followed by the
+    cell-to-point interpolation of E."""
+    from picfoam_amd import cases, picmi
+    mesh, species, models, ex = cases.poisson_plasma(ncell=ncell, ppc=ppc)
+    models["FieldWeigthing"] = "VolumeWeighting"
+    models["ChargeDistribution"] = "VolumeWeighting"
+    c = picmi.Cloud(mesh, species, models, ex["bc"])
+    c.set_field(picmi.FIELD_SPACE_CHARGE, np.full(ex["n_cells"], ex["space_charge"]))
+    c.init_maxwellian(0, ppc, ex["T_K"], seed=21)
+    dt = ex["dt"]
+    for _ in range(warmup):
+        c.step(dt)
+    n = c.count(0)
+    c.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c.step(dt)
+    el = time.perf_counter() - t0
+    stages = {k: round(v[0] / steps, 4) for k, v in c.stage_times().items() if v[1]}
+    c.profile(False)
+    c.close()
+    return {"ms_per_step": round(1e3 * el / steps, 3), "stage_ms_per_step": stages, "parcels": n,
+            "particle_steps_per_s": n * steps / el,
+            "note": "synchronous pic_step per step (the Poisson solve checks convergence on the host); move includes "
+                    "the VolumeWeighting deposit"}
+
+
 def poisson_benchmark(ncell, ppc, snaps, train_steps, evals, warm, loss_kind=0, guesses=("previous", "extrapolate")):
     """Poisson ms per step (right-hand side, initial guess, PCG to ||r||/||b|| <= 1e-8, gradient)
     on the field-solve workload, for four initial guesses: the previous phi (the reference's
@@ -345,6 +376,8 @@ def main():
     ap.add_argument("--no-poisson", action="store_true", help="skip the Poisson initial-guess comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the mixed headline run alone (for profiling)")
+    ap.add_argument("--only-vw-field-solve", action="store_true",
+                    help="the VolumeWeighting field-solve workload alone (for profiling the deposit)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
                     help="precision of the headline run (the other one runs beside it unless --no-fp64)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -364,6 +397,10 @@ def main():
         args.no_fp64 = args.no_thermalisation = args.no_tets = args.no_poisson = args.no_cpu_baseline = True
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+    if args.only_vw_field_solve:
+        print(json.dumps({"field_solve_volume_weighting": run_vw_field_solve(steps=args.steps, warmup=args.warmup)}),
+              flush=True)
+        return
 
     prec = args.precision
     N = args.ncell ** 3 * args.ppc
@@ -418,6 +455,9 @@ def main():
     if not args.no_tets:
         print("bench: magnetised tets, 50^3 x 6 tet lattice, 50M electrons", file=sys.stderr, flush=True)
         wl["magnetised_tets"] = run_magnetised_tets(steps=10, warmup=2)
+    if not args.no_poisson:
+        print("bench: field solve with VolumeWeighting, 100^3, 8M electrons", file=sys.stderr, flush=True)
+        wl["field_solve_volume_weighting"] = run_vw_field_solve(steps=10, warmup=2)
     if wl:
         out["workloads"] = wl
     if not args.no_poisson:

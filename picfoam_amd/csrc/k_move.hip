@@ -182,10 +182,69 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
     return t;
 }
 
+// VolumeWeighting deposit (VolumeWeighting.C:105-123) of one block's parcels, privatised in
+// LDS: the cell shares go to a window of kDepCells cells from the block's lowest cell, the
+// vertex shares to an open-addressing table of kDepPoints point ids, and each occupied LDS
+// slot is flushed with one global atomic (the parcels are in cell order, so a block's
+// 256 parcels share a few cells and their points: a few dozen global atomics instead of
+// four per parcel).  Shares that fit neither (a cell outside the window, a probe run past
+// kDepProbes) go to the global accumulators directly.  The sums are int64 fixed point, so
+// the result does not depend on the order.  Called by every thread of the block.
+constexpr int kDepCells = 256, kDepPoints = 512, kDepProbes = 16;
+struct DepLds {
+    unsigned long long cval[kDepCells];
+    unsigned long long pval[kDepPoints];
+    int pkey[kDepPoints];
+    int wmin[8];
+};
+
+__device__ void deposit_vw_block(const TetGeo* __restrict__ tets, int cell, int t, double y1, double y2, double y3,
+                                 double s42, unsigned long long* acc_cell, unsigned long long* acc_point, DepLds& L) {
+    int cm = cell >= 0 ? cell : 0x7fffffff;
+    for (int off = 32; off > 0; off >>= 1) cm = min(cm, __shfl_xor(cm, off));
+    if (__lane_id() == 0) L.wmin[threadIdx.x >> 6] = cm;
+    for (int k = threadIdx.x; k < kDepCells; k += blockDim.x) L.cval[k] = 0ull;
+    for (int k = threadIdx.x; k < kDepPoints; k += blockDim.x) { L.pkey[k] = -1; L.pval[k] = 0ull; }
+    __syncthreads();
+    cm = L.wmin[0];
+    for (int w = 1; w < (int)((blockDim.x + 63) >> 6); ++w) cm = min(cm, L.wmin[w]);
+    if (cell >= 0) {
+        const TetGeo& g = tets[t];
+        const double y0 = 1.0 - (y1 + y2 + y3);
+        const unsigned long long vc = (unsigned long long)llrint(y0 * s42);
+        const unsigned dc = (unsigned)(cell - cm);
+        if (dc < (unsigned)kDepCells) atomicAdd(&L.cval[dc], vc);
+        else atomicAdd(acc_cell + cell, vc);
+        const double yk[3] = {y1, y2, y3};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int p = g.vm[k];
+            const unsigned long long v = (unsigned long long)llrint(yk[k] * s42);
+            unsigned h = ((unsigned)p * 2654435761u) & (kDepPoints - 1);
+            bool done = false;
+            for (int probe = 0; probe < kDepProbes && !done; ++probe) {
+                const int old = atomicCAS(&L.pkey[h], -1, p);
+                if (old == -1 || old == p) {
+                    atomicAdd(&L.pval[h], v);
+                    done = true;
+                }
+                h = (h + 1) & (kDepPoints - 1);
+            }
+            if (!done) atomicAdd(acc_point + p, v);
+        }
+    }
+    __syncthreads();
+    if (cm != 0x7fffffff)
+        for (int k = threadIdx.x; k < kDepCells; k += blockDim.x)
+            if (L.cval[k]) atomicAdd(acc_cell + cm + k, L.cval[k]);
+    for (int k = threadIdx.x; k < kDepPoints; k += blockDim.x)
+        if (L.pkey[k] >= 0) atomicAdd(acc_point + L.pkey[k], L.pval[k]);
+}
+
 // One lane per parcel: push (if charged and moving) + track + deposit/count; `block` is
 // the block's index within its species.
 template <class UT, int P, int W, int DEP>
-__device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block) {
+__device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block, DepLds* dl) {
     const int64_t i = block * blockDim.x + threadIdx.x;
     const int64_t n = a.n_live ? min(*a.n_live, a.n) : a.n;
     // blocks wholly past the live count (the launch bound can be a species' capacity while
@@ -236,17 +295,8 @@ __device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block) 
     // in its new cell, which the sort's scatter uses directly
     const int slot = wave_aggregated_inc(a.count, newcell);
     if (a.place && i < n) a.place[i] = make_int2(newcell, slot);
-    if constexpr (DEP == kVolumeWeighting) {
-        if (newcell >= 0) {
-            const TetGeo& g = a.tets[t];
-            const double s = a.s42;
-            const double y0 = 1.0 - (y1 + y2 + y3);
-            atomicAdd(a.acc_cell + newcell, (unsigned long long)llrint(y0 * s));
-            atomicAdd(a.acc_point + g.vm[0], (unsigned long long)llrint(y1 * s));
-            atomicAdd(a.acc_point + g.vm[1], (unsigned long long)llrint(y2 * s));
-            atomicAdd(a.acc_point + g.vm[2], (unsigned long long)llrint(y3 * s));
-        }
-    }
+    if constexpr (DEP == kVolumeWeighting)
+        deposit_vw_block(a.tets, newcell, t, y1, y2, y3, a.s42, a.acc_cell, a.acc_point, *dl);
 }
 
 // every moving species in one launch: blocks [block0[s], block0[s+1]) belong to species s,
@@ -256,7 +306,12 @@ __global__ void __launch_bounds__(256) k_move(MoveSet<UT> m) {
     const int64_t b = blockIdx.x;
     int s = 0;
     while (s + 1 < m.ns && b >= m.block0[s + 1]) ++s;
-    move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s]);
+    if constexpr (DEP == kVolumeWeighting) {
+        __shared__ DepLds dl;
+        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], &dl);
+    } else {
+        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], nullptr);
+    }
 }
 
 template <class UT, int P, int W>
@@ -292,15 +347,8 @@ __global__ void __launch_bounds__(256) k_deposit(MoveArgs<UT> a) {
     const int slot = wave_aggregated_inc(a.count, cell);
     if (a.place && i < n) a.place[i] = make_int2(cell, slot);
     if constexpr (DEP == kVolumeWeighting) {
-        if (cell >= 0) {
-            const TetGeo& g = a.tets[t];
-            const double s = a.s42;
-            const double y0 = 1.0 - (y1 + y2 + y3);
-            atomicAdd(a.acc_cell + cell, (unsigned long long)llrint(y0 * s));
-            atomicAdd(a.acc_point + g.vm[0], (unsigned long long)llrint(y1 * s));
-            atomicAdd(a.acc_point + g.vm[1], (unsigned long long)llrint(y2 * s));
-            atomicAdd(a.acc_point + g.vm[2], (unsigned long long)llrint(y3 * s));
-        }
+        __shared__ DepLds dl;
+        deposit_vw_block(a.tets, cell, t, y1, y2, y3, a.s42, a.acc_cell, a.acc_point, dl);
     }
 }
 
