@@ -386,8 +386,12 @@ def main():
     ap.add_argument("--only-headline", action="store_true", help="the mixed headline run alone (for profiling)")
     ap.add_argument("--only-vw-field-solve", action="store_true",
                     help="the VolumeWeighting field-solve workload alone (for profiling the deposit)")
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"],
-                    help="precision of the headline run (the other one runs beside it unless --no-fp64)")
+    ap.add_argument("--only", choices=["thermalisation", "magnetised-tets", "poisson"],
+                    help="one sub-workload alone, printed as its JSON object (for profiling): the thermalisation "
+                         "in --precision (default fp64), the magnetised tets, or the Poisson initial-guess comparison")
+    ap.add_argument("--precision", default=None, choices=["mixed", "fp64"],
+                    help="precision of the headline run (default mixed; the other one runs beside it unless "
+                         "--no-fp64)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--poisson-ncell", type=int, default=100)
     ap.add_argument("--poisson-ppc", type=int, default=8)
@@ -405,12 +409,22 @@ def main():
         args.no_fp64 = args.no_thermalisation = args.no_tets = args.no_poisson = args.no_cpu_baseline = True
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+    if args.only:
+        if args.only == "thermalisation":
+            r = run_thermalisation(args.precision or "fp64", args.steps, args.warmup)
+        elif args.only == "magnetised-tets":
+            r = run_magnetised_tets(steps=args.steps, warmup=args.warmup)
+        else:
+            r = poisson_benchmark(args.poisson_ncell, args.poisson_ppc, snaps=48, train_steps=400, evals=24, warm=4,
+                                  loss_kind=args.poisson_loss)
+        print(json.dumps({args.only: r}), flush=True)
+        return
     if args.only_vw_field_solve:
         print(json.dumps({"field_solve_volume_weighting": run_vw_field_solve(steps=args.steps, warmup=args.warmup)}),
               flush=True)
         return
 
-    prec = args.precision
+    prec = args.precision or "mixed"
     N = args.ncell ** 3 * args.ppc
     print(f"bench: argon MCC {args.ncell}^3 cells, {N} electrons, {prec}", file=sys.stderr, flush=True)
     head = run_argon(prec, args.steps, args.warmup, args.ncell, args.ppc, args.sort_interval,
