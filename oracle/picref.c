@@ -4,32 +4,44 @@
  * TEST INFRASTRUCTURE ONLY (see picref.h).  Plain C, compiled with -ffp-contract=off so
  * that every product and sum is rounded as written.
  *
- * Parity status per function (the tests that pin each one are named in tests/):
+ * Parity status per function (the tests that pin each one are in tests/test_oracle.py and
+ * tests/test_oracle_pins.py):
  *   geometry, tets, locate, positions ............ pinned: closed-form volumes/centres,
  *                                                    barycentric round trips
+ *   face base points ............................. pinned: point 0 on convex meshes; a folded
+ *                                                    face against a brute-force positive-volume search
  *   pushers ...................................... pinned: |u| conservation, E-only kick,
  *                                                    gyro period/radius, ExB drift, Boris=HC=Vay (v<<c)
  *   tracking ..................................... pinned: straight-line endpoint x+U*dt with
  *                                                    periodic wrap, brute-force point location
  *   interpolation / deposit ...................... pinned: linear field reproduced exactly,
- *                                                    total charge, hand-computed single particles
+ *                                                    total charge, hand-computed single particles,
+ *                                                    VolumeWeighting distance weights by hand
  *   cell->point interpolation .................... pinned: constant/linear fields on interior points
- *   Poisson PCG/DIC + grad ....................... pinned: dense solve, manufactured sin(kx)
+ *   Poisson PCG/DIC + grad ....................... pinned: dense solve, manufactured sin(kx) at
+ *                                                    second order; the non-orthogonal correction by
+ *                                                    second-order convergence on a sheared lattice
+ *                                                    (the implicit part alone does not converge)
+ *   Debye length ................................. pinned: closed form sqrt(eps0 kT/(n q^2)) summed
+ *                                                    over species, with drift and the Wigner-Seitz floor
  *   Coulomb (Nanbu + TA pairing) ................. pinned: momentum/energy conservation per pair,
- *                                                    <cos chi> = exp(-s)
+ *                                                    <cos chi> = exp(-s) in every branch of the
+ *                                                    sampler, TA collisions per parcel and odd triples
  *   Perez Coulomb ................................ pinned: relativistic 4-momentum conservation,
  *                                                    <cos chi> = exp(-s) and Nanbu's angle at v<<c
  *   Nanbu-Yonemura pairing, Nanbu weight
  *   correction ................................... pinned: the reference's collision counts;
  *                                                    keep probability w_P/w_Q (binomial test)
  *   MCC (null collision) ......................... pinned: collision frequency n*sigma*v,
- *                                                    candidate count, elastic energy loss 2m/M
+ *                                                    candidate count, elastic energy loss 2m/M,
+ *                                                    excitation/ionization energy bookkeeping, the
+ *                                                    Opal split, pair probability w_E/w_N,
+ *                                                    background Maxwellian width kT/M
  *   relativistic e-n scattering .................. pinned: 4-momentum conservation, Nanbu's
  *                                                    cos chi at v<<c
  *   Philox4x32-10 ................................ pinned: Random123 known-answer vectors
  *   OpenFOAM-9 Random/shuffle streams, tracking tie-breaks, volPointInterpolation at
- *   non-coupled boundary points, the non-orthogonal Laplacian correction (the Laplacian
- *   here is the orthogonal part; every field-solve mesh is orthogonal)
+ *   non-coupled boundary points
  *                                                  parity unpinned (not reproducible offline)
  */
 #ifdef _OPENMP
@@ -1426,6 +1438,10 @@ struct ref_poisson {
     double* bcval;
     /* face interpolation weights for Gauss grad */
     double* w;              /* per face owner weight */
+    /* |Sf| x nonOrthCorrectionVector per face, oriented out of the face's owner (zero on
+     * non-coupled boundary faces); nonorth = 0 when every one is below 1e-10 |Sf| */
+    double* kc;
+    int nonorth;
 };
 
 static void add_coupling(struct ref_poisson* P, int i, int j, double a) {
@@ -1435,9 +1451,21 @@ static void add_coupling(struct ref_poisson* P, int i, int j, double a) {
     }
 }
 
-/* fvm::laplacian with "Gauss linear corrected" (orthogonal part: nonOrthDeltaCoeffs =
- * 1/max(nf.d, 0.05|d|)), cyclic coupling, fixedValue/zeroGradient patches; DIC is IC(0)
- * in a greedy multicolour ordering (cells coloured in index order). */
+/* OpenFOAM's surfaceInterpolation::makeNonOrthCorrectionVectors: k = nf - d*deltaCoeff on
+ * internal and coupled faces (boundary faces of other patches keep k = 0), stored times
+ * |Sf| */
+static void nonorth_vector(struct ref_poisson* P, int f, v3 Sf, v3 nf, v3 d, double delta_coeff) {
+    double mS = vmag(Sf);
+    P->kc[3 * f + 0] = mS * (nf.x - d.x * delta_coeff);
+    P->kc[3 * f + 1] = mS * (nf.y - d.y * delta_coeff);
+    P->kc[3 * f + 2] = mS * (nf.z - d.z * delta_coeff);
+}
+
+/* fvm::laplacian with "Gauss linear corrected": the implicit part with nonOrthDeltaCoeffs =
+ * 1/max(nf.d, 0.05|d|) (boundary faces: 1/(nf.(Cf - C))), cyclic coupling, fixedValue /
+ * zeroGradient patches, and the explicit non-orthogonal correction (ref_poisson_correction)
+ * from the potential the solve starts from; DIC is IC(0) in a greedy multicolour ordering
+ * (cells coloured in index order). */
 ref_poisson* ref_poisson_create(const ref_mesh* m, const int* bct, const double* bcv) {
     ref_poisson* P = (ref_poisson*)calloc(1, sizeof(ref_poisson));
     P->m = m;
@@ -1452,6 +1480,7 @@ ref_poisson* ref_poisson_create(const ref_mesh* m, const int* bct, const double*
     P->diag = (double*)calloc(n, sizeof(double));
     P->bsrc = (double*)calloc(n, sizeof(double));
     P->w = (double*)calloc(m->n_faces, sizeof(double));
+    P->kc = (double*)calloc(3 * (size_t)m->n_faces, sizeof(double));
     P->bctype = (int*)malloc(sizeof(int) * (m->n_patches + 1));
     P->bcval = (double*)malloc(sizeof(double) * (m->n_patches + 1));
     memcpy(P->bctype, bct, sizeof(int) * m->n_patches);
@@ -1474,6 +1503,7 @@ ref_poisson* ref_poisson_create(const ref_mesh* m, const int* bct, const double*
             P->diag[o] += a; P->diag[nb] += a;
             double so = fabs(vdot(Sf, vsub(Cf, Co))), sn = fabs(vdot(Sf, vsub(Cn, Cf)));
             P->w[f] = sn / (so + sn);
+            nonorth_vector(P, f, Sf, nf, d, a / mS);
             continue;
         }
         int pi = m->face_patch[f];
@@ -1485,10 +1515,11 @@ ref_poisson* ref_poisson_create(const ref_mesh* m, const int* bct, const double*
             v3 Cfn = Cf; /* this side's face centre */
             double so = fabs(vdot(Sf, vsub(Cfn, Co))), sn = fabs(vdot(Sf, vsub(Cn, Cfn)));
             P->w[f] = sn / (so + sn);
-            if (nb == o) continue; /* self-coupled: no net contribution */
             v3 d = vsub(Cn, Co);
             double nd = vdot(nf, d), md = 0.05 * vmag(d);
             double a = mS / (nd > md ? nd : md);
+            nonorth_vector(P, f, Sf, nf, d, a / mS);
+            if (nb == o) continue; /* self-coupled: no net contribution */
             add_coupling(P, o, nb, a);
             P->diag[o] += a;
         } else if (P->bctype[pi] == 1) {
@@ -1503,6 +1534,12 @@ ref_poisson* ref_poisson_create(const ref_mesh* m, const int* bct, const double*
         }
     }
     P->singular = !has_dirichlet;
+    P->nonorth = 0;
+    for (int f = 0; f < m->n_faces; ++f) {
+        double mS = vmag(ld3(m->Sf + 3 * f));
+        if (vmag(ld3(P->kc + 3 * f)) > 1e-10 * mS) P->nonorth = 1;
+    }
+    if (!P->nonorth) memset(P->kc, 0, 3 * (size_t)m->n_faces * sizeof(double));
     /* greedy colouring in cell order */
     P->colour = (int*)malloc(sizeof(int) * n);
     P->ncol = 0;
@@ -1537,7 +1574,7 @@ ref_poisson* ref_poisson_create(const ref_mesh* m, const int* bct, const double*
 void ref_poisson_destroy(ref_poisson* P) {
     if (!P) return;
     free(P->nb); free(P->a); free(P->diag); free(P->bsrc); free(P->colour); free(P->corder); free(P->rD);
-    free(P->bctype); free(P->bcval); free(P->w);
+    free(P->bctype); free(P->bcval); free(P->w); free(P->kc);
     free(P);
 }
 
@@ -1593,6 +1630,38 @@ static double dotp(int n, const double* a, const double* b) {
     return s;
 }
 
+/* The explicit part of fvm::laplacian(phiE) with "Gauss linear corrected"
+ * (gaussLaplacianScheme: the source takes -V div(|Sf| k . interpolate(grad phi)), with
+ * correctedSnGrad::correction = nonOrthCorrectionVectors & linear-interpolated Gauss-linear
+ * grad), evaluated on phi; ElectroStaticSolver.C:61-64 assembles it from the potential of
+ * the previous step, the field the solve starts from.  Returned as the term that adds to
+ * the right-hand side V rho/eps0 of the SPD form used here; zero on an orthogonal mesh. */
+int ref_poisson_nonorth(const ref_poisson* P) { return P->nonorth; }
+
+void ref_poisson_correction(const ref_poisson* P, const double* phi, double* corr) {
+    const ref_mesh* m = P->m;
+    memset(corr, 0, sizeof(double) * m->n_cells);
+    if (!P->nonorth) return;
+    double* g = (double*)malloc(sizeof(double) * 3 * m->n_cells);
+    ref_grad(P, phi, g);
+    for (int f = 0; f < m->n_faces; ++f) {
+        int o = m->owner[f], nb;
+        if (f < m->n_internal) {
+            nb = m->nbr[f];
+        } else {
+            int pi = m->face_patch[f];
+            if (m->ptype[pi] != REF_PATCH_CYCLIC) continue;
+            int q = m->pnbr[pi];
+            nb = m->owner[m->pstart[q] + (f - m->pstart[pi])];
+        }
+        double w = P->w[f], s = 0.0;
+        for (int d = 0; d < 3; ++d) s += P->kc[3 * f + d] * (w * g[3 * o + d] + (1.0 - w) * g[3 * nb + d]);
+        corr[o] += s;
+        if (f < m->n_internal) corr[nb] -= s;
+    }
+    free(g);
+}
+
 int ref_poisson_solve(const ref_poisson* P, const double* rho, double* phi, double tol, int max_iter, int precond,
                       double* final_rel) {
     int n = P->n;
@@ -1601,7 +1670,20 @@ int ref_poisson_solve(const ref_poisson* P, const double* rho, double* phi, doub
     double* z = (double*)malloc(sizeof(double) * n);
     double* pp = (double*)malloc(sizeof(double) * n);
     double* q = (double*)malloc(sizeof(double) * n);
-    ref_poisson_rhs(P, rho, b);
+    if (P->nonorth) {
+        /* right-hand side with the correction of the starting potential, then the zero
+         * mean of a singular system */
+        ref_poisson_correction(P, phi, q);
+        for (int i = 0; i < n; ++i) b[i] = rho[i] * P->m->V[i] / EPS0 + P->bsrc[i] + q[i];
+        if (P->singular) {
+            double s = 0.0;
+            for (int i = 0; i < n; ++i) s += b[i];
+            s /= n;
+            for (int i = 0; i < n; ++i) b[i] -= s;
+        }
+    } else {
+        ref_poisson_rhs(P, rho, b);
+    }
     ref_poisson_apply(P, phi, q);
     for (int i = 0; i < n; ++i) r[i] = b[i] - q[i];
     double bn = sqrt(dotp(n, b, b));

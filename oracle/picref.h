@@ -134,18 +134,24 @@ void ref_en_scatter(int relativistic, double eV, double* UE, double* Un, double 
 /* ListCrossSection interpolateXY (ListCrossSection.C:32-92) */
 double ref_interpolate_xy(double x, int n, const double* xs, const double* ys);
 
-/* Poisson: FV Laplacian (Gauss linear corrected, orthogonal part) + multicolour DIC PCG */
+/* Poisson: FV Laplacian (Gauss linear corrected: implicit orthogonal part, explicit
+ * non-orthogonal correction of the starting potential) + multicolour DIC PCG */
 typedef struct ref_poisson ref_poisson;
 ref_poisson* ref_poisson_create(const ref_mesh* m, const int* phi_bc_type, const double* phi_bc_value);
 void ref_poisson_destroy(ref_poisson* p);
 int ref_poisson_n_colours(const ref_poisson* p);
 void ref_poisson_colours(const ref_poisson* p, int* colour);
-/* solve A phi = rho V / eps0 (+BC) to ||r||/||b|| <= tol; phi warm start in/out; returns iters */
+/* solve A phi = rho V / eps0 (+BC) + correction(phi in) to ||r||/||b|| <= tol; phi warm start
+ * in/out; returns iters */
 int ref_poisson_solve(const ref_poisson* p, const double* rho, double* phi, double tol, int max_iter,
                       int precond, double* final_rel_res);
 void ref_poisson_apply(const ref_poisson* p, const double* x, double* y);
 void ref_poisson_rhs(const ref_poisson* p, const double* rho, double* b);
 void ref_grad(const ref_poisson* p, const double* phi, double* gradphi);
+/* 1 when some face's non-orthogonal correction vector exceeds 1e-10 |Sf| */
+int ref_poisson_nonorth(const ref_poisson* p);
+/* the explicit non-orthogonal correction of phi per cell (adds to the right-hand side) */
+void ref_poisson_correction(const ref_poisson* p, const double* phi, double* corr);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,9 @@ def lib():
         _lib.ref_poisson_solve.restype = C.c_int
         _lib.ref_poisson_apply.argtypes = [vp, vp, vp]
         _lib.ref_poisson_rhs.argtypes = [vp, vp, vp]
+        _lib.ref_poisson_nonorth.argtypes = [vp]
+        _lib.ref_poisson_nonorth.restype = C.c_int
+        _lib.ref_poisson_correction.argtypes = [vp, vp, vp]
         _lib.ref_grad.argtypes = [vp, vp, vp]
     return _lib
 
@@ -367,6 +370,16 @@ class RefPoisson:
         g = np.empty((self.rm.n_cells, 3))
         lib().ref_grad(self.h, _p(phi), _p(g))
         return g
+
+    @property
+    def nonorth(self):
+        return bool(lib().ref_poisson_nonorth(self.h))
+
+    def correction(self, phi):
+        phi = np.ascontiguousarray(phi, np.float64)
+        c = np.empty(self.rm.n_cells)
+        lib().ref_poisson_correction(self.h, _p(phi), _p(c))
+        return c
 
 
 def gnn_n_params(fin, H, L):

@@ -132,6 +132,9 @@ struct pic_ctx {
     // gradient tables
     int32_t *d_gfo = nullptr, *d_gkind = nullptr, *d_gother = nullptr;
     double *d_gw = nullptr, *d_gval = nullptr, *d_gS = nullptr;
+    // non-orthogonal correction: vectors per grad entry, -grad of the starting potential,
+    // and the correction per cell (null on an orthogonal mesh)
+    double *d_gk = nullptr, *d_mgrad = nullptr, *d_corr = nullptr;
     // learned initial guess
     std::unique_ptr<Gnn> gnn;
     bool gnn_use = false, gnn_rec = false;
@@ -635,7 +638,23 @@ int do_poisson(pic_ctx* c, pic_poisson_stats* st) {
     double rel = 0.0;
     {
         pic_ctx::Scope sc(c, PIC_STAGE_POISSON);
-        hipError_t e = poisson_rhs(c->pd, c->d_rho, c->d_b, c->hp.singular, c->pw.red, c->stream);
+        hipError_t e = hipSuccess;
+        if (c->d_corr) {
+            // the explicit non-orthogonal correction from the potential the solve starts
+            // from (the last solution), before any extrapolated or learned start replaces it
+            GradArgs g{};
+            g.n_cells = c->hm.n_cells;
+            g.gfo = c->d_gfo; g.g_kind = c->d_gkind; g.g_other = c->d_gother;
+            g.g_w = c->d_gw; g.g_val = c->d_gval; g.g_S = c->d_gS;
+            g.V = c->d_V;
+            g.phi = c->d_phi;
+            g.E = c->d_mgrad;
+            g.E_f = nullptr;
+            e = launch_grad(g, c->stream);
+            if (e == hipSuccess) e = launch_nonorth_corr(g, c->d_gk, c->d_corr, c->stream);
+            if (int r = check_hip(c, e, "non-orthogonal correction")) return r;
+        }
+        e = poisson_rhs(c->pd, c->d_rho, c->d_corr, c->d_b, c->hp.singular, c->pw.red, c->stream);
         if (int r = check_hip(c, e, "poisson rhs")) return r;
         if (c->poisson_guess == 1) {
             // warm start 2 phi(n-1) - phi(n-2) once two solutions exist
@@ -1004,6 +1023,12 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         c->d_gw = c->upload(P.g_w);
         c->d_gval = c->upload(P.g_val);
         c->d_gS = c->upload(P.g_S);
+        if (P.nonorth) {
+            c->d_gk = c->upload(P.g_k);
+            c->d_mgrad = c->alloc<double>(3 * (size_t)nc);
+            c->d_corr = c->alloc<double>(nc);
+            if (!c->d_gk || !c->d_mgrad || !c->d_corr) return c->fail(PIC_ERR_HIP, "out of device memory (poisson)");
+        }
         if (!c->pw.scal || !c->d_gS) return c->fail(PIC_ERR_HIP, "out of device memory (poisson)");
     }
     if ((r = check_hip(c.get(), hipStreamSynchronize(c->stream), "create"))) return r;

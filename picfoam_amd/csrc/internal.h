@@ -87,6 +87,10 @@ struct HostPoisson {
     std::vector<int32_t> g_kind;   // 0: interp between cells (w*self + (1-w)*other), 1: fixed value, 2: self
     std::vector<int32_t> g_other;
     std::vector<double> g_w, g_val, g_S;  // weight of self, value, signed area vector (3 each)
+    // |Sf| x nonOrthCorrectionVector per grad entry, pointing out of the cell (3 each; zero
+    // on non-coupled boundary faces); nonorth false when every one is below 1e-10 |Sf|
+    std::vector<double> g_k;
+    bool nonorth = false;
 };
 std::string build_host_poisson(const HostMesh& m, HostPoisson& p);
 

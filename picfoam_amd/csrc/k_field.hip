@@ -261,6 +261,32 @@ hipError_t launch_grad(const GradArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// The explicit part of fvm::laplacian with "Gauss linear corrected" (gaussLaplacianScheme,
+// correctedSnGrad::correction): per cell, the sum over its internal and coupled faces of
+// |Sf| k . (w grad_self + (1 - w) grad_other), k the face's nonOrthCorrectionVector out of the
+// cell.  a.E holds -grad of the potential the solve starts from, so the sum is negated.
+__global__ void k_nonorth_corr(GradArgs a, const double* __restrict__ gk, double* __restrict__ corr) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n_cells) return;
+    const double* mg = a.E;
+    const double sx = mg[3 * c], sy = mg[3 * c + 1], sz = mg[3 * c + 2];
+    double acc = 0.0;
+    for (int k = a.gfo[c]; k < a.gfo[c + 1]; ++k) {
+        if (a.g_kind[k] != 0) continue;
+        const int o = a.g_other[k];
+        const double w = a.g_w[k], v = a.g_val[k];
+        const double gx = w * sx + v * mg[3 * o], gy = w * sy + v * mg[3 * o + 1], gz = w * sz + v * mg[3 * o + 2];
+        acc = acc + (gk[3 * k] * gx + gk[3 * k + 1] * gy + gk[3 * k + 2] * gz);
+    }
+    corr[c] = -acc;
+}
+
+hipError_t launch_nonorth_corr(const GradArgs& a, const double* gk, double* corr, hipStream_t s) {
+    if (a.n_cells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nonorth_corr, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a, gk, corr);
+    return hipGetLastError();
+}
+
 // volPointInterpolation: inverse-distance weighted cell values summed over every cell of
 // the coupled point group (cyclic images share the master's value).
 __global__ void k_point_interp(PointInterpArgs a) {

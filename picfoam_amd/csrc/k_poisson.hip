@@ -76,11 +76,13 @@ __global__ void __launch_bounds__(256) k_residual(int n, int K, const int32_t* _
     if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
-__global__ void k_rhs(int n, const double* rho, const double* V, const double* bsrc, double* b, double* partial) {
+__global__ void k_rhs(int n, const double* rho, const double* V, const double* bsrc, const double* corr, double* b,
+                      double* partial) {
     __shared__ double sh[4];
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double v = rho[i] * V[i] / kEps0 + bsrc[i];
+        double v = rho[i] * V[i] / kEps0 + bsrc[i];
+        if (corr) v = v + corr[i];
         b[i] = v;
         acc += v;
     }
@@ -404,12 +406,12 @@ int pcg_fused(const PoissonDev& P, const Sys<T>& S, PoissonWork& W, T* x, T* r, 
 
 }  // namespace
 
-hipError_t poisson_rhs(const PoissonDev& P, const double* rho, double* b, bool singular, double* scratch,
-                       hipStream_t s) {
+hipError_t poisson_rhs(const PoissonDev& P, const double* rho, const double* corr, double* b, bool singular,
+                       double* scratch, hipStream_t s) {
     const int n = P.n, nb = nblk(n);
     double* partial = scratch;
     double* scal = scratch + kRedBlocks;
-    hipLaunchKernelGGL(k_rhs, dim3(nb), dim3(256), 0, s, n, rho, P.V, P.bsrc, b, partial);
+    hipLaunchKernelGGL(k_rhs, dim3(nb), dim3(256), 0, s, n, rho, P.V, P.bsrc, corr, b, partial);
     if (singular) {
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, partial, nb, scal, 0);
         hipLaunchKernelGGL(k_sub_mean, dim3(nb), dim3(256), 0, s, n, b, scal, 0);

@@ -159,6 +159,9 @@ struct GradArgs {
     float* E_f;
 };
 hipError_t launch_grad(const GradArgs& a, hipStream_t s);
+// corr[c] = sum over the cell's internal and coupled faces of |Sf| k . interpolated grad phi,
+// with a.E holding -grad phi (k_grad's output) and gk the correction vectors per entry
+hipError_t launch_nonorth_corr(const GradArgs& a, const double* gk, double* corr, hipStream_t s);
 
 struct PointInterpArgs {
     int n_points;
@@ -323,8 +326,9 @@ struct PoissonDev {
     const float* rD_pf;
     const int32_t* col_start_d;  // device copy of colour starts
 };
-hipError_t poisson_rhs(const PoissonDev& P, const double* rho, double* b, bool singular, double* scratch,
-                       hipStream_t s);
+// b = rho V / eps0 + Dirichlet sources (+ corr when not null), zero mean when singular
+hipError_t poisson_rhs(const PoissonDev& P, const double* rho, const double* corr, double* b, bool singular,
+                       double* scratch, hipStream_t s);
 // PCG with DIC in FP64 (mode 0) or FP32-inner CG inside FP64 iterative refinement (mode 1)
 struct PoissonWork {
     double *r, *z, *p, *q, *x, *b;

@@ -428,8 +428,10 @@ void add_coupling(HostPoisson& P, int i, int j, double a) {
 }
 }  // namespace
 
-// fvm::laplacian (Gauss linear corrected, orthogonal part), cyclic coupling, fixedValue /
-// zeroGradient patches; IC(0) "DIC" factor in the greedy multicolour ordering.
+// fvm::laplacian (Gauss linear corrected): the implicit part with nonOrthDeltaCoeffs
+// 1/max(nf.d, 0.05|d|) (boundary faces 1/(nf.(Cf - C))), cyclic coupling, fixedValue /
+// zeroGradient patches, and the vectors of the explicit non-orthogonal correction; IC(0)
+// "DIC" factor in the greedy multicolour ordering.
 std::string build_host_poisson(const HostMesh& m, HostPoisson& P) {
     const int n = m.n_cells;
     P.n = n;
@@ -441,6 +443,13 @@ std::string build_host_poisson(const HostMesh& m, HostPoisson& P) {
     P.diag.assign(n, 0.0);
     P.bsrc.assign(n, 0.0);
     P.w.assign(m.n_faces, 0.0);
+    // surfaceInterpolation's nonOrthCorrectionVectors nf - d*deltaCoeff, times |Sf|, out of
+    // the face's owner; internal and coupled faces only
+    std::vector<double> kf(3 * (size_t)m.n_faces, 0.0);
+    auto nonorth_vector = [&](int f, v3 Sf, v3 nf, v3 d, double delta_coeff) {
+        const double mS = mag(Sf);
+        st(&kf[3 * (size_t)f], mS * (nf - delta_coeff * d));
+    };
     bool dir = false;
     for (int f = 0; f < m.n_faces; ++f) {
         v3 Sf = ld(&m.Sf[3 * f]), Cf = ld(&m.Cf[3 * f]);
@@ -460,6 +469,7 @@ std::string build_host_poisson(const HostMesh& m, HostPoisson& P) {
             P.diag[nb] += a;
             double so = std::fabs(dot(Sf, Cf - Co)), sn = std::fabs(dot(Sf, Cn - Cf));
             P.w[f] = sn / (so + sn);
+            nonorth_vector(f, Sf, nf, d, a / mS);
             continue;
         }
         int pi = m.face_patch[f];
@@ -470,10 +480,11 @@ std::string build_host_poisson(const HostMesh& m, HostPoisson& P) {
             v3 Cn = ld(&m.C[3 * nb]) - ld(&m.psep[3 * pi]);
             double so = std::fabs(dot(Sf, Cf - Co)), sn = std::fabs(dot(Sf, Cn - Cf));
             P.w[f] = sn / (so + sn);
-            if (nb == o) continue;
             v3 d = Cn - Co;
             double nd = dot(nf, d), md = 0.05 * mag(d);
             double a = mS / (nd > md ? nd : md);
+            nonorth_vector(f, Sf, nf, d, a / mS);
+            if (nb == o) continue;
             add_coupling(P, o, nb, a);
             P.diag[o] += a;
         } else if (m.phi_bc[pi] == 1) {
@@ -488,6 +499,9 @@ std::string build_host_poisson(const HostMesh& m, HostPoisson& P) {
         }
     }
     P.singular = !dir;
+    P.nonorth = false;
+    for (int f = 0; f < m.n_faces; ++f)
+        if (mag(ld(&kf[3 * (size_t)f])) > 1e-10 * mag(ld(&m.Sf[3 * f]))) P.nonorth = true;
     P.colour.assign(n, 0);
     P.ncol = 0;
     for (int i = 0; i < n; ++i) {
@@ -530,7 +544,7 @@ std::string build_host_poisson(const HostMesh& m, HostPoisson& P) {
         std::sort(cf[c].begin(), cf[c].end());
     }
     P.gfo.assign(n + 1, 0);
-    P.g_kind.clear(); P.g_other.clear(); P.g_w.clear(); P.g_val.clear(); P.g_S.clear();
+    P.g_kind.clear(); P.g_other.clear(); P.g_w.clear(); P.g_val.clear(); P.g_S.clear(); P.g_k.clear();
     for (int c = 0; c < n; ++c) {
         P.gfo[c] = (int32_t)P.g_kind.size();
         for (int f : cf[c]) {
@@ -569,6 +583,9 @@ std::string build_host_poisson(const HostMesh& m, HostPoisson& P) {
             P.g_S.push_back(sgn * Sf.x);
             P.g_S.push_back(sgn * Sf.y);
             P.g_S.push_back(sgn * Sf.z);
+            // a coupled face's vector already points out of this side's owner
+            const double ks = P.nonorth ? sgn : 0.0;
+            for (int d = 0; d < 3; ++d) P.g_k.push_back(ks * kf[3 * (size_t)f + d]);
         }
     }
     P.gfo[n] = (int32_t)P.g_kind.size();

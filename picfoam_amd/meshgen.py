@@ -144,6 +144,20 @@ def hex_box(nx, ny, nz, lx, ly, lz, cyclic=(True, True, True), boundary_type="pa
     return mesh
 
 
+def sheared(mesh, matrix):
+    """The same mesh with every point mapped by x -> M x, for a linear map M with a positive
+    determinant.  Cyclic pairs stay translations of each other, so a sheared periodic hex
+    box is a lattice of identical parallelepipeds whose faces are not orthogonal to the
+    lines joining the cell centres (a non-orthogonal mesh)."""
+    M = np.asarray(matrix, dtype=np.float64)
+    if M.shape != (3, 3) or not np.linalg.det(M) > 0:
+        raise ValueError("sheared: need a 3 x 3 matrix with a positive determinant")
+    out = dict(mesh)
+    out["points"] = np.ascontiguousarray(mesh["points"] @ M.T)
+    out["kind"] = mesh.get("kind", "hex") + "-sheared"
+    return out
+
+
 # Kuhn (Freudenthal) split of the unit cube into 6 tets along the main diagonal: one tet
 # per permutation of the axes.  It is conforming across neighbouring cubes.
 _KUHN_PERMS = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))

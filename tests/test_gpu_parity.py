@@ -913,6 +913,49 @@ def test_pcg_iterations_match_oracle(oracle, case):
     assert rel_err(phi_g, phi_r, floor_frac=1.0) < 1e-7   # norm-wise, to the solve tolerance
 
 
+@pytest.mark.parametrize("case", ["tet-walls", "sheared-hex-cyclic"])
+def test_poisson_nonorthogonal_correction_parity(oracle, case):
+    """Gauss linear corrected on non-orthogonal meshes: a jittered tet lattice with
+    fixedValue x faces, and a periodic lattice of parallelepipeds (x -> x + y/2).  Three
+    solves in a row with different charges, each starting from the last potential, whose
+    explicit correction enters the right-hand side: the same potentials as the oracle to
+    the solve tolerance and the same iteration counts (within one for rounding)."""
+    from picfoam_amd import meshgen
+    say("non-orthogonal Poisson", case)
+    rng = np.random.default_rng(109)
+    if case == "tet-walls":
+        m = small_tet(n=5, L=1e-3, jitter=0.2, cyclic=(False, True, True))
+        bc = {"xmin": ("fixedValue", 0.0), "xmax": ("fixedValue", 2.0)}
+    else:
+        m = meshgen.sheared(small_hex(n=(8, 12, 3), L=(1.0, 2.0, 0.4)), [[1, 0.5, 0], [0, 1, 0], [0, 0, 1]])
+        bc = None
+    c = cloud(m, {"ParticlePusher": "Boris", "MaxwellSolver": "ElectroStatic", "poissonTolerance": 1e-11}, phi_bc=bc)
+    rm = oracle.RefMesh(m)
+    P = oracle.RefPoisson(rm, bc or {})
+    assert P.nonorth
+    phi_r = np.zeros(rm.n_cells)
+    for k in range(3):
+        rho = rng.normal(size=rm.n_cells) * EPS0 * 1e3
+        if bc is None:
+            rho -= rho.mean()
+        c.set_field(4, rho)
+        c.deposit()
+        st = c.poisson_solve()
+        phi_r, it_r, _ = P.solve(rho, phi_r, tol=1e-11)
+        phi_g = c.get_field(2)
+        say(f"solve {k}: iterations gpu {st['iterations']} oracle {it_r}")
+        assert abs(st["iterations"] - it_r) <= 1
+        if bc is None:
+            phi_g = phi_g - phi_g.mean(); phi_r = phi_r - phi_r.mean()
+        assert rel_err(phi_g, phi_r, floor_frac=1.0) < 1e-8
+        assert rel_err(c.get_field(0), -P.grad(phi_r), floor_frac=1.0) < 1e-7
+    # the correction is live: dropping it changes the last potential far beyond the tolerance
+    b0, _, _ = P.solve(rho, np.zeros(rm.n_cells), tol=1e-11)
+    if bc is None:
+        b0 = b0 - b0.mean()
+    assert rel_err(b0, phi_r, floor_frac=1.0) > 1e-4
+
+
 @pytest.mark.parametrize("density", ["sparse", "dense", "densest"])
 @pytest.mark.parametrize("lnl", [10.0, 0.0, "opts"], ids=["fixed-lnL", "debye-lnL", "debye-options"])
 def test_coulomb_three_charged_species(oracle, lnl, density):
