@@ -130,11 +130,11 @@ struct pic_ctx {
     PoissonWork pw{};
     double* d_b = nullptr;
     // gradient tables
-    int32_t *d_gfo = nullptr, *d_gkind = nullptr, *d_gother = nullptr;
-    double *d_gw = nullptr, *d_gval = nullptr, *d_gS = nullptr;
-    // non-orthogonal correction: vectors per grad entry, -grad of the starting potential,
-    // and the correction per cell (null on an orthogonal mesh)
-    double *d_gk = nullptr, *d_mgrad = nullptr, *d_corr = nullptr;
+    // Gauss-gradient face table (column-major, see GradArgs) with the non-orthogonal
+    // correction vectors (null on an orthogonal mesh); -grad of the potential a solve starts
+    // from and the correction per cell (non-orthogonal meshes only)
+    GradArgs gtab{};
+    double *d_mgrad = nullptr, *d_corr = nullptr;
     // learned initial guess
     std::unique_ptr<Gnn> gnn;
     bool gnn_use = false, gnn_rec = false;
@@ -642,16 +642,12 @@ int do_poisson(pic_ctx* c, pic_poisson_stats* st) {
         if (c->d_corr) {
             // the explicit non-orthogonal correction from the potential the solve starts
             // from (the last solution), before any extrapolated or learned start replaces it
-            GradArgs g{};
-            g.n_cells = c->hm.n_cells;
-            g.gfo = c->d_gfo; g.g_kind = c->d_gkind; g.g_other = c->d_gother;
-            g.g_w = c->d_gw; g.g_val = c->d_gval; g.g_S = c->d_gS;
-            g.V = c->d_V;
+            GradArgs g = c->gtab;
             g.phi = c->d_phi;
             g.E = c->d_mgrad;
             g.E_f = nullptr;
             e = launch_grad(g, c->stream);
-            if (e == hipSuccess) e = launch_nonorth_corr(g, c->d_gk, c->d_corr, c->stream);
+            if (e == hipSuccess) e = launch_nonorth_corr(g, c->d_corr, c->stream);
             if (int r = check_hip(c, e, "non-orthogonal correction")) return r;
         }
         e = poisson_rhs(c->pd, c->d_rho, c->d_corr, c->d_b, c->hp.singular, c->pw.red, c->stream);
@@ -690,11 +686,7 @@ int do_poisson(pic_ctx* c, pic_poisson_stats* st) {
             G.snap_next = (slot + 1) % G.snap_cap;
             G.snap_count = std::min(G.snap_count + 1, G.snap_cap);
         }
-        GradArgs g{};
-        g.n_cells = c->hm.n_cells;
-        g.gfo = c->d_gfo; g.g_kind = c->d_gkind; g.g_other = c->d_gother;
-        g.g_w = c->d_gw; g.g_val = c->d_gval; g.g_S = c->d_gS;
-        g.V = c->d_V;
+        GradArgs g = c->gtab;
         g.phi = c->d_phi;
         g.E = c->d_E;
         g.E_f = c->d_Ef;
@@ -1017,19 +1009,42 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         c->pw.scal = c->alloc<double>(64);
         c->pw.cnt = c->alloc<unsigned>(8);
         c->d_b = c->alloc<double>(nc);
-        c->d_gfo = c->upload(P.gfo);
-        c->d_gkind = c->upload(P.g_kind);
-        c->d_gother = c->upload(P.g_other);
-        c->d_gw = c->upload(P.g_w);
-        c->d_gval = c->upload(P.g_val);
-        c->d_gS = c->upload(P.g_S);
+        {
+            // the per-cell face lists of the gradient, column-major over K slots
+            int K = 0;
+            for (int i = 0; i < nc; ++i) K = std::max(K, P.gfo[i + 1] - P.gfo[i]);
+            const size_t kn = (size_t)K * nc;
+            std::vector<int32_t> src(kn, kGradUnused);
+            std::vector<double> w(kn, 0.0), val(kn, 0.0), S(3 * kn, 0.0), gk(P.nonorth ? 3 * kn : 0, 0.0);
+            for (int i = 0; i < nc; ++i)
+                for (int t = P.gfo[i]; t < P.gfo[i + 1]; ++t) {
+                    const size_t e = (size_t)(t - P.gfo[i]) * nc + i;
+                    src[e] = P.g_kind[t] == 0 ? P.g_other[t] : (P.g_kind[t] == 1 ? kGradFixed : kGradSelf);
+                    w[e] = P.g_w[t];
+                    val[e] = P.g_val[t];
+                    for (int d = 0; d < 3; ++d) {
+                        S[d * kn + e] = P.g_S[3 * (size_t)t + d];
+                        if (P.nonorth) gk[d * kn + e] = P.g_k[3 * (size_t)t + d];
+                    }
+                }
+            GradArgs& g = c->gtab;
+            g.n_cells = nc;
+            g.K = K;
+            g.src = c->upload(src);
+            g.w = c->upload(w);
+            g.val = c->upload(val);
+            g.S = c->upload(S);
+            g.gk = P.nonorth ? c->upload(gk) : nullptr;
+            g.V = c->d_V;
+            if (!g.src || !g.w || !g.val || !g.S || (P.nonorth && !g.gk))
+                return c->fail(PIC_ERR_HIP, "out of device memory (poisson)");
+        }
         if (P.nonorth) {
-            c->d_gk = c->upload(P.g_k);
             c->d_mgrad = c->alloc<double>(3 * (size_t)nc);
             c->d_corr = c->alloc<double>(nc);
-            if (!c->d_gk || !c->d_mgrad || !c->d_corr) return c->fail(PIC_ERR_HIP, "out of device memory (poisson)");
+            if (!c->d_mgrad || !c->d_corr) return c->fail(PIC_ERR_HIP, "out of device memory (poisson)");
         }
-        if (!c->pw.scal || !c->d_gS) return c->fail(PIC_ERR_HIP, "out of device memory (poisson)");
+        if (!c->pw.scal) return c->fail(PIC_ERR_HIP, "out of device memory (poisson)");
     }
     if ((r = check_hip(c.get(), hipStreamSynchronize(c->stream), "create"))) return r;
     *out = c.release();

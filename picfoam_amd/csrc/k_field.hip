@@ -233,21 +233,25 @@ hipError_t launch_deposit_finish(const DepositFinishArgs& a, hipStream_t s) {
 
 // ---------------------------------------------------------------- Gauss gradient
 // fvc::grad Gauss linear (ElectroStaticSolver.C:68): grad_c = (1/V) sum_f S_f phi_f, faces
-// in ascending order; E = -grad.
-__global__ void k_grad(GradArgs a) {
+// in ascending order; E = -grad.  The face table is column-major, so a wave's loads of one
+// slot are contiguous.
+__global__ void __launch_bounds__(256) k_grad(GradArgs a) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n_cells) return;
+    const size_t n = (size_t)a.n_cells, kn = (size_t)a.K * n;
     double gx = 0.0, gy = 0.0, gz = 0.0;
     const double ps = a.phi[c];
-    for (int k = a.gfo[c]; k < a.gfo[c + 1]; ++k) {
+    for (int k = 0; k < a.K; ++k) {
+        const size_t e = (size_t)k * n + c;
+        const int src = a.src[e];
+        if (src == kGradUnused) break;
         double pf;
-        const int kind = a.g_kind[k];
-        if (kind == 0) pf = a.g_w[k] * ps + a.g_val[k] * a.phi[a.g_other[k]];
-        else if (kind == 1) pf = a.g_val[k];
+        if (src >= 0) pf = a.w[e] * ps + a.val[e] * a.phi[src];
+        else if (src == kGradFixed) pf = a.val[e];
         else pf = ps;
-        gx = gx + a.g_S[3 * k] * pf;
-        gy = gy + a.g_S[3 * k + 1] * pf;
-        gz = gz + a.g_S[3 * k + 2] * pf;
+        gx = gx + a.S[e] * pf;
+        gy = gy + a.S[kn + e] * pf;
+        gz = gz + a.S[2 * kn + e] * pf;
     }
     const double V = a.V[c];
     const double ex = -(gx / V), ey = -(gy / V), ez = -(gz / V);
@@ -265,25 +269,28 @@ hipError_t launch_grad(const GradArgs& a, hipStream_t s) {
 // correctedSnGrad::correction): per cell, the sum over its internal and coupled faces of
 // |Sf| k . (w grad_self + (1 - w) grad_other), k the face's nonOrthCorrectionVector out of the
 // cell.  a.E holds -grad of the potential the solve starts from, so the sum is negated.
-__global__ void k_nonorth_corr(GradArgs a, const double* __restrict__ gk, double* __restrict__ corr) {
+__global__ void __launch_bounds__(256) k_nonorth_corr(GradArgs a, double* __restrict__ corr) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n_cells) return;
+    const size_t n = (size_t)a.n_cells, kn = (size_t)a.K * n;
     const double* mg = a.E;
     const double sx = mg[3 * c], sy = mg[3 * c + 1], sz = mg[3 * c + 2];
     double acc = 0.0;
-    for (int k = a.gfo[c]; k < a.gfo[c + 1]; ++k) {
-        if (a.g_kind[k] != 0) continue;
-        const int o = a.g_other[k];
-        const double w = a.g_w[k], v = a.g_val[k];
+    for (int k = 0; k < a.K; ++k) {
+        const size_t e = (size_t)k * n + c;
+        const int o = a.src[e];
+        if (o == kGradUnused) break;
+        if (o < 0) continue;
+        const double w = a.w[e], v = a.val[e];
         const double gx = w * sx + v * mg[3 * o], gy = w * sy + v * mg[3 * o + 1], gz = w * sz + v * mg[3 * o + 2];
-        acc = acc + (gk[3 * k] * gx + gk[3 * k + 1] * gy + gk[3 * k + 2] * gz);
+        acc = acc + (a.gk[e] * gx + a.gk[kn + e] * gy + a.gk[2 * kn + e] * gz);
     }
     corr[c] = -acc;
 }
 
-hipError_t launch_nonorth_corr(const GradArgs& a, const double* gk, double* corr, hipStream_t s) {
-    if (a.n_cells <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nonorth_corr, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a, gk, corr);
+hipError_t launch_nonorth_corr(const GradArgs& a, double* corr, hipStream_t s) {
+    if (a.n_cells <= 0 || !a.gk) return hipSuccess;
+    hipLaunchKernelGGL(k_nonorth_corr, dim3((a.n_cells + 255) / 256), dim3(256), 0, s, a, corr);
     return hipGetLastError();
 }
 

@@ -144,15 +144,19 @@ struct DepositFinishArgs {
 hipError_t launch_deposit_finish(const DepositFinishArgs& a, hipStream_t s);
 
 // E = -grad(phi) (Gauss linear) then point interpolation; Ecell_f / Epoint_f are the FP32
-// copies used by the mixed-precision push (may be null)
+// copies used by the mixed-precision push (may be null).  The face table is column-major,
+// entry e = k * n_cells + c for slot k < K of cell c (the cell's faces in ascending order):
+// src >= 0 is the other cell (phi_f = w phi_c + val phi_src), -1 a fixed value (phi_f =
+// val), -2 the cell's own value (zeroGradient), -3 an unused slot (only after the used ones).
+// S and gk are three planes of K * n_cells each (x, y, z).
+constexpr int32_t kGradFixed = -1, kGradSelf = -2, kGradUnused = -3;
 struct GradArgs {
-    int n_cells;
-    const int32_t* gfo;
-    const int32_t* g_kind;
-    const int32_t* g_other;
-    const double* g_w;
-    const double* g_val;
-    const double* g_S;
+    int n_cells, K;
+    const int32_t* src;
+    const double* w;
+    const double* val;
+    const double* S;     // outward area vector
+    const double* gk;    // |Sf| x nonOrthCorrectionVector out of the cell (null: orthogonal)
     const double* V;
     const double* phi;
     double* E;
@@ -160,8 +164,8 @@ struct GradArgs {
 };
 hipError_t launch_grad(const GradArgs& a, hipStream_t s);
 // corr[c] = sum over the cell's internal and coupled faces of |Sf| k . interpolated grad phi,
-// with a.E holding -grad phi (k_grad's output) and gk the correction vectors per entry
-hipError_t launch_nonorth_corr(const GradArgs& a, const double* gk, double* corr, hipStream_t s);
+// with a.E holding -grad phi (k_grad's output)
+hipError_t launch_nonorth_corr(const GradArgs& a, double* corr, hipStream_t s);
 
 struct PointInterpArgs {
     int n_points;
