@@ -36,8 +36,11 @@ enum {
     PIC_ERR_NO_DEVICE = -10      /* no HIP device */
 };
 
-/* constant/polyMesh/boundary types */
-enum { PIC_PATCH_PATCH = 0, PIC_PATCH_WALL = 1, PIC_PATCH_CYCLIC = 2 };
+/* constant/polyMesh/boundary types.  PIC_PATCH_EMPTY marks the front and back of a
+ * reduced-dimension (2-D or 1-D) case: the tracking velocity loses its components normal
+ * to them (PICParcel.C:118-123, meshTools::constrainDirection over polyMesh::solutionD), so
+ * parcels stay in their plane; no flux crosses them in the Poisson solve. */
+enum { PIC_PATCH_PATCH = 0, PIC_PATCH_WALL = 1, PIC_PATCH_CYCLIC = 2, PIC_PATCH_EMPTY = 3 };
 /* phi boundary conditions (0/phiE) */
 enum { PIC_BC_ZERO_GRADIENT = 0, PIC_BC_FIXED_VALUE = 1 };
 /* arithmetic of the particle step */

@@ -60,6 +60,7 @@
 #define HPLANCK 6.62607015e-34
 #define PI_D 3.14159265358979323846
 #define MAX_TRACK_STEPS 1000
+#define REF_EMPTY_FACE (-900000)
 
 typedef struct { double x, y, z; } v3;
 static inline v3 V(double x, double y, double z) { v3 r = {x, y, z}; return r; }
@@ -86,6 +87,7 @@ struct ref_mesh {
     int* pmaster;                /* cyclic-coupled point groups -> smallest point id */
     int* face_patch;             /* boundary face -> patch index (-1 for internal) */
     double* psep;                /* per patch: x_partner = x + sep */
+    int track_mask;              /* polyMesh::solutionD: bit d set for an empty direction */
     /* VolumeWeighting (ChargeDistribution) weights, per cell-point (ascending point id) */
     int *cpo, *cp;
     double* vw_w;
@@ -280,6 +282,22 @@ ref_mesh* ref_mesh_create(int n_points, const double* points, int n_faces, const
     m->psep = (double*)calloc(3 * (n_patches + 1), sizeof(double));
     m->pmaster = (int*)malloc(sizeof(int) * n_points);
     for (int p = 0; p < n_points; ++p) m->pmaster[p] = p;
+    /* polyMesh::calcDirections: summed |area components| of the empty patches, normalised;
+     * a component above 1e-6 marks an empty direction */
+    {
+        double ed[3] = {0.0, 0.0, 0.0};
+        int any = 0;
+        for (int pi = 0; pi < n_patches; ++pi) {
+            if (m->ptype[pi] != REF_PATCH_EMPTY) continue;
+            for (int f = m->pstart[pi]; f < m->pstart[pi] + m->psize[pi]; ++f)
+                for (int d = 0; d < 3; ++d) ed[d] += fabs(m->Sf[3 * f + d]);
+            if (m->psize[pi] > 0) any = 1;
+        }
+        double e = sqrt(ed[0] * ed[0] + ed[1] * ed[1] + ed[2] * ed[2]);
+        m->track_mask = 0;
+        if (any && e > 0.0)
+            for (int d = 0; d < 3; ++d) if (ed[d] / e > 1e-6) m->track_mask |= 1 << d;
+    }
     for (int pi = 0; pi < n_patches; ++pi) {
         for (int f = m->pstart[pi]; f < m->pstart[pi] + m->psize[pi]; ++f) m->face_patch[f] = pi;
         if (m->ptype[pi] == REF_PATCH_CYCLIC && m->psize[pi] > 0) {
@@ -646,6 +664,7 @@ static int cross_face(const ref_mesh* m, int t, int iH, double y[4]) {
         return -1000000;
     }
     int pi = m->face_patch[f];
+    if (m->ptype[pi] == REF_PATCH_EMPTY) return REF_EMPTY_FACE;
     if (m->ptype[pi] != REF_PATCH_CYCLIC) return -1 - pi;
     int q = m->pnbr[pi];
     int g = m->pstart[q] + (f - m->pstart[pi]);
@@ -684,6 +703,11 @@ static int track_one(const ref_mesh* m, double* yy, double* Up, int* tetp, doubl
     y[1] = yy[0]; y[2] = yy[1]; y[3] = yy[2];
     y[0] = 1.0 - (y[1] + y[2] + y[3]);
     v3 U = ld3(Up);
+    /* reduced-dimension cases track with the empty components removed (PICParcel.C:118-123:
+     * meshTools::constrainDirection over solutionD); the parcel keeps its full U */
+    if (m->track_mask & 1) U.x = 0.0;
+    if (m->track_mask & 2) U.y = 0.0;
+    if (m->track_mask & 4) U.z = 0.0;
     double sf = 0.0;
     int guard = 0, status = 0;
     while (sf < 1.0) {
@@ -712,6 +736,7 @@ static int track_one(const ref_mesh* m, double* yy, double* Up, int* tetp, doubl
         if (frac > 1.0) frac = 1.0;
         sf = sf + f * frac;
         int s = cross_face(m, t, iH, y);
+        if (s == REF_EMPTY_FACE) break; /* on an empty face (rounding only): stays */
         if (s <= -1000000) { status = 2; break; }
         if (s < 0) { t = -1; break; } /* left through a non-coupled patch: deleted (NoInflow) */
         t = s;
@@ -1733,6 +1758,7 @@ void ref_grad(const ref_poisson* P, const double* phi, double* gr) {
             gr[3 * nb + 0] -= Sf.x * pf; gr[3 * nb + 1] -= Sf.y * pf; gr[3 * nb + 2] -= Sf.z * pf;
         } else {
             int pi = m->face_patch[f];
+            if (m->ptype[pi] == REF_PATCH_EMPTY) continue; /* an empty fvPatch has no faces */
             if (m->ptype[pi] == REF_PATCH_CYCLIC) {
                 int q = m->pnbr[pi];
                 int g = m->pstart[q] + (f - m->pstart[pi]);

@@ -23,7 +23,7 @@ extern "C" {
 typedef struct ref_mesh ref_mesh;
 
 /* patch types (constant/polyMesh/boundary `type`) */
-enum { REF_PATCH_PATCH = 0, REF_PATCH_WALL = 1, REF_PATCH_CYCLIC = 2 };
+enum { REF_PATCH_PATCH = 0, REF_PATCH_WALL = 1, REF_PATCH_CYCLIC = 2, REF_PATCH_EMPTY = 3 };
 /* pushers (ParticlePusherNew.C:39) */
 enum { REF_PUSH_NONE = 0, REF_PUSH_BORIS = 1, REF_PUSH_BORISNR = 2, REF_PUSH_VAY = 3, REF_PUSH_HC = 4 };
 /* FieldWeigthing / ChargeDistribution (FieldWeigthing.C:77, ChargeDistributionNew.C:44) */

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libpicref.so")
 
 PUSHERS = {"none": 0, "Boris": 1, "BorisNR": 2, "Vay": 3, "HigueraCary": 4}
 WEIGHTINGS = {"CellAverage": 0, "VolumeWeighting": 1}
-PATCH = {"patch": 0, "wall": 1, "cyclic": 2}
+PATCH = {"patch": 0, "wall": 1, "cyclic": 2, "empty": 3}
 
 
 def build():

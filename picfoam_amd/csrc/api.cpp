@@ -325,6 +325,7 @@ MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
     a.do_push = (S.charge != 0.0 && !identity_push) ? 1 : 0;
     a.rotate = c->rotate ? 1 : 0;
     a.hc_ref = c->hc_ref;
+    a.track_mask = c->hm.track_mask;
     a.count = S.count;
     a.place = S.place;
     a.acc_cell = c->d_acc_cell;
@@ -868,6 +869,10 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     for (int p = 0; p < m.n_patches; ++p) {
         const pic_patch& pp = mesh->patches[p];
         if (pp.type == PIC_PATCH_WALL) return c->fail(PIC_ERR_UNSUPPORTED, "wall patches (WallReflectionModel) are not implemented");
+        if (pp.type < PIC_PATCH_PATCH || pp.type > PIC_PATCH_EMPTY)
+            return c->fail(PIC_ERR_INVALID, "patch " + std::to_string(p) + ": unknown type");
+        if (pp.type == PIC_PATCH_EMPTY && pp.phi_bc != PIC_BC_ZERO_GRADIENT)
+            return c->fail(PIC_ERR_INVALID, "patch " + std::to_string(p) + ": an empty patch takes no phi value");
         m.ptype.push_back(pp.type); m.pstart.push_back(pp.start); m.psize.push_back(pp.size);
         m.pnbr.push_back(pp.neighbour_patch); m.phi_bc.push_back(pp.phi_bc); m.phi_val.push_back(pp.phi_value);
     }

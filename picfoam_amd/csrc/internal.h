@@ -16,6 +16,9 @@ constexpr double kH = 6.62607015e-34;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kFixedScale = 4398046511104.0;  // 2^42 units per charge quantum
 constexpr int kMaxTrackSteps = 1000;
+// tet-face neighbour code of a face on an empty patch (a reduced-dimension case): a parcel
+// that reaches one stops there and keeps its place (other non-coupled patches: -1 - patch)
+constexpr int kTetEmptyFace = -900000;
 constexpr int kMaxSpecies = 8;
 // per-cell index-list slots in LDS of the collision kernels (per-cell pass / large cells)
 constexpr int kCoulombSlots = 128, kCoulombSlotsBig = 2048;
@@ -59,6 +62,9 @@ struct HostMesh {
     std::vector<int32_t> pmaster;              // coupled point groups
     std::vector<int32_t> face_patch;
     std::vector<double> psep;                  // per patch separation
+    // polyMesh::solutionD: bit d set when direction d is empty (reduced-dimension case);
+    // the tracking velocity drops those components
+    uint32_t track_mask = 0;
     std::vector<TetGeo> tets;
     // cell points (ascending id) with VolumeWeighting distance weights and
     // volPointInterpolation inverse-distance weights

@@ -163,6 +163,7 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
         frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
         sf = sf + f * frac;
         const int nb = (iH & 2) ? ((iH & 1) ? nb4.w : nb4.z) : ((iH & 1) ? nb4.y : nb4.x);
+        if (nb == kTetEmptyFace) break;   // only rounding brings a parcel onto an empty face
         if (nb < 0) {
             if (nb <= -1000000) status = 2;
             t = -1;
@@ -273,7 +274,11 @@ __device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block, 
             }
             const double Ux = (double)U.x, Uy = (double)U.y, Uz = (double)U.z;
             if (sqrt(Ux * Ux + Uy * Uy + Uz * Uz) >= kC) atomicOr(a.flags, kFlagSuperluminal);
-            t = track(a.tets, t, y1, y2, y3, Ux, Uy, Uz, a.dt, status);
+            // reduced-dimension cases track with the empty components of U removed
+            // (PICParcel.C:118-123); U itself keeps all three
+            const uint32_t m = a.track_mask;
+            t = track(a.tets, t, y1, y2, y3, (m & 1u) ? 0.0 : Ux, (m & 2u) ? 0.0 : Uy, (m & 4u) ? 0.0 : Uz, a.dt,
+                      status);
             a.p.y1[i] = y1; a.p.y2[i] = y2; a.p.y3[i] = y3;
             a.p.tet[i] = t;
             if (t >= 0) newcell = a.tets[t].cell;

@@ -16,6 +16,7 @@ struct MoveArgs {
     UT k, kv;          // q*(dt/2)/m, q*dt/m
     double dt;
     int do_push, rotate, hc_ref;
+    uint32_t track_mask;             // bit d: direction d is empty, its velocity component is not tracked
     int* count;                      // per-cell parcel count of this species
     int2* place;                     // per parcel (new cell, slot in the cell) for the sort, or null
     unsigned long long* acc_cell;    // VolumeWeighting fixed-point accumulators

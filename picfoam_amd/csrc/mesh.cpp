@@ -189,6 +189,22 @@ std::string build_host_mesh(HostMesh& m) {
         for (int f = m.pstart[pi]; f < m.pstart[pi] + m.psize[pi]; ++f) m.face_patch[f] = pi;
     }
     for (int f = ni; f < nf; ++f) if (m.face_patch[f] < 0) return "boundary face not in any patch";
+    // polyMesh::calcDirections: the summed |area components| of the empty patches, normalised;
+    // a component above 1e-6 is an empty direction
+    {
+        double ed[3] = {0.0, 0.0, 0.0};
+        bool any = false;
+        for (int pi = 0; pi < m.n_patches; ++pi) {
+            if (m.ptype[pi] != 3) continue;
+            for (int f = m.pstart[pi]; f < m.pstart[pi] + m.psize[pi]; ++f)
+                for (int d = 0; d < 3; ++d) ed[d] += std::fabs(m.Sf[3 * (size_t)f + d]);
+            any = any || m.psize[pi] > 0;
+        }
+        m.track_mask = 0;
+        const double e = std::sqrt(ed[0] * ed[0] + ed[1] * ed[1] + ed[2] * ed[2]);
+        if (any && e > 0.0)
+            for (int d = 0; d < 3; ++d) if (ed[d] / e > 1e-6) m.track_mask |= 1u << d;
+    }
     for (int pi = 0; pi < m.n_patches; ++pi) {
         if (m.ptype[pi] != 2 || m.psize[pi] == 0) continue;
         int q = m.pnbr[pi];
@@ -332,7 +348,7 @@ std::string build_host_mesh(HostMesh& m) {
                 g.perm |= source_code(map);
             } else {
                 int pi = m.face_patch[f];
-                if (m.ptype[pi] != 2) { g.nbr[0] = -1 - pi; continue; }
+                if (m.ptype[pi] != 2) { g.nbr[0] = m.ptype[pi] == 3 ? kTetEmptyFace : -1 - pi; continue; }
                 int q = m.pnbr[pi];
                 int gface = m.pstart[q] + (f - m.pstart[pi]);
                 int c2 = m.owner[gface];
@@ -548,6 +564,7 @@ std::string build_host_poisson(const HostMesh& m, HostPoisson& P) {
     for (int c = 0; c < n; ++c) {
         P.gfo[c] = (int32_t)P.g_kind.size();
         for (int f : cf[c]) {
+            if (f >= m.n_internal && m.ptype[m.face_patch[f]] == 3) continue;
             v3 Sf = ld(&m.Sf[3 * f]);
             bool own = m.owner[f] == c;
             // an internal face whose owner and neighbour coincide cannot occur; a cell can

@@ -9,7 +9,7 @@ A mesh is a dict with the arrays an OpenFOAM ``constant/polyMesh`` directory hol
 * ``neighbour``   int32 [n_internal_faces]; internal faces come first, in
   upper-triangular order (sorted by owner, then neighbour)
 * ``patches``     list of dicts ``{name, type, start, size, neighbour_patch}``
-  where ``type`` is ``"cyclic"``, ``"patch"`` or ``"wall"``. Face ``i`` of a cyclic
+  where ``type`` is ``"cyclic"``, ``"patch"``, ``"wall"`` or ``"empty"``. Face ``i`` of a cyclic
   patch matches face ``i`` of its neighbour patch; its point 0 is the image of the
   partner's point 0 and the point order is reversed, as OpenFOAM's cyclic patches
   require.
@@ -64,11 +64,14 @@ def _assemble(points, int_faces, int_owner, int_nbr, patch_defs):
     }
 
 
-def hex_box(nx, ny, nz, lx, ly, lz, cyclic=(True, True, True), boundary_type="patch", origin=(0.0, 0.0, 0.0)):
+def hex_box(nx, ny, nz, lx, ly, lz, cyclic=(True, True, True), boundary_type="patch", origin=(0.0, 0.0, 0.0),
+            empty=(False, False, False)):
     """Uniform hex box of nx*ny*nz cells, cell (i,j,k) = i + nx*(j + ny*k).
 
     ``cyclic[a]`` makes the two faces normal to axis ``a`` a cyclic pair; otherwise they
     are patches of ``boundary_type`` ("patch" deletes particles, "wall" reflects them).
+    ``empty[a]`` makes them empty patches instead (a reduced-dimension case, normally with
+    one cell along ``a``).
     """
     nx, ny, nz = int(nx), int(ny), int(nz)
     xs = origin[0] + np.linspace(0.0, lx, nx + 1)
@@ -129,7 +132,10 @@ def hex_box(nx, ny, nz, lx, ly, lz, cyclic=(True, True, True), boundary_type="pa
     defs_z = (fmin, omin, fmax, omax)
     for a, (fmin, omin, fmax, omax) in enumerate((defs_x, defs_y, defs_z)):
         lo, hi = names[a]
-        if cyclic[a]:
+        if empty[a]:
+            patch_defs.append((lo, "empty", fmin, omin, None))
+            patch_defs.append((hi, "empty", fmax, omax, None))
+        elif cyclic[a]:
             patch_defs.append((lo, "cyclic", fmin, omin, hi))
             patch_defs.append((hi, "cyclic", fmax, omax, lo))
         else:

@@ -24,7 +24,7 @@ ERRORS = {
     -5: "PIC_ERR_CAPACITY", -6: "PIC_ERR_MESH", -7: "PIC_ERR_UNSUPPORTED", -8: "PIC_ERR_CELL_OVERFLOW",
     -9: "PIC_ERR_NOT_CONVERGED", -10: "PIC_ERR_NO_DEVICE",
 }
-PATCH_TYPES = {"patch": 0, "wall": 1, "cyclic": 2}
+PATCH_TYPES = {"patch": 0, "wall": 1, "cyclic": 2, "empty": 3}
 PREC_FP64, PREC_MIXED = 0, 1
 POISSON_FP64, POISSON_MIXED_IR = 0, 1
 FIELD_E, FIELD_B, FIELD_PHI, FIELD_RHO, FIELD_SPACE_CHARGE, FIELD_E_POINT, FIELD_MCC_SMAX, FIELD_MCC_REM = range(8)

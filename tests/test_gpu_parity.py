@@ -10,6 +10,7 @@ import pytest
 
 from helpers import ME, MP, QE, C_LIGHT, EPS0, periodic_diff, random_bary, random_parcels, rel_err, small_hex, \
     small_tet, sort_by_id
+from picfoam_amd import meshgen
 
 pytestmark = pytest.mark.gpu
 
@@ -28,7 +29,9 @@ def cloud(mesh, models, species=None, phi_bc=None):
     return picmi.Cloud(mesh, species, models, phi_bc)
 
 
-MESHES = {"hex": lambda: small_hex(), "tet": lambda: small_tet(n=3, L=1e-3, jitter=0.2)}
+MESHES = {"hex": lambda: small_hex(), "tet": lambda: small_tet(n=3, L=1e-3, jitter=0.2),
+          # a reduced-dimension slab: one cell across z between empty patches
+          "hex-2d": lambda: meshgen.hex_box(4, 3, 1, 1e-3, 1.5e-3, 0.4e-3, empty=(False, False, True))}
 
 
 @pytest.mark.parametrize("kind", ["hex", "tet", "dart", "dart-cyclic"])
@@ -86,10 +89,11 @@ def test_push_fp64_parity(oracle, kind, pusher, weighting):
         assert rel_err(c.get_field(5), Ep) < TOL
 
 
-@pytest.mark.parametrize("kind", ["hex", "tet"])
+@pytest.mark.parametrize("kind", ["hex", "tet", "hex-2d"])
 @pytest.mark.parametrize("weighting", ["CellAverage", "VolumeWeighting"])
 def test_move_fp64_parity(oracle, kind, weighting):
-    """push + interpolation + tracking across several cells and periodic boundaries"""
+    """push + interpolation + tracking across several cells and periodic boundaries; on the
+    2-D slab the parcels track in the plane with their full 3-D velocities"""
     say("move parity", kind, weighting)
     rng = np.random.default_rng(101)
     m = MESHES[kind]()

@@ -188,6 +188,35 @@ def test_tracking_straight_line_periodic(oracle, kind):
     assert y2.min() > -1e-12 and (1 - y2.sum(axis=1)).min() > -1e-12
 
 
+def test_tracking_reduced_dimension_keeps_the_plane(oracle):
+    """A 2-D slab (one cell across z, empty front and back patches): the tracking velocity
+    loses its z component (polyMesh::solutionD), so every parcel ends at x + (Ux, Uy, 0) dt
+    with its z unchanged, across the cyclic x and y boundaries, while U keeps all three
+    components.  The empty direction is found from the patch areas alone."""
+    from picfoam_amd import meshgen
+    rng = np.random.default_rng(41)
+    m = meshgen.hex_box(6, 5, 1, 1.2e-3, 1.0e-3, 0.2e-3, empty=(False, False, True))
+    rm = oracle.RefMesh(m)
+    L = np.array([1.2e-3, 1.0e-3, 0.2e-3])
+    y, tet, U, ids = random_parcels(rng, rm.n_tets, 2000, 1.0)
+    U = U * 5e-4
+    U[:, 2] = rng.normal(size=len(U)) * 1e-3   # would cross the slab five times over
+    x0 = rm.positions(y, tet)
+    Z = np.zeros((rm.n_cells, 3))
+    y2, U2, tet2, guards, _ = rm.move("none", "CellAverage", y, U, tet, -QE, ME, 1.0, Z, None, Z, do_push=False)
+    assert guards == 0 and np.all(tet2 >= 0)
+    x1 = rm.positions(y2, tet2)
+    Ut = U.copy(); Ut[:, 2] = 0.0
+    d = periodic_diff(x1, x0 + Ut, L)
+    assert np.max(np.abs(d)) < 1e-14
+    assert np.array_equal(U2, U)
+    # a 3-D box with the same parcels moves them in z as well
+    m3 = meshgen.hex_box(6, 5, 1, 1.2e-3, 1.0e-3, 0.2e-3)
+    rm3 = oracle.RefMesh(m3)
+    y3, _, t3, _, _ = rm3.move("none", "CellAverage", y, U, tet, -QE, ME, 1.0, Z, None, Z, do_push=False)
+    assert np.max(np.abs(periodic_diff(rm3.positions(y3, t3), x0 + U, L))) < 1e-14
+
+
 def test_tracking_deletes_through_open_patch(oracle):
     m = small_hex(n=(3, 3, 3), L=(1.0, 1.0, 1.0), cyclic=(False, True, True))
     rm = oracle.RefMesh(m)
