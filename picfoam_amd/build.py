@@ -17,6 +17,8 @@ SOURCES = [
     # (file, extra flags)
     ("mesh.cpp", ["-ffp-contract=off"]),
     ("api.cpp", []),
+    ("foam_dict.cpp", []),
+    ("foam_case.cpp", []),
     ("k_move.hip", []),
     ("k_field.hip", []),
     ("k_coulomb.hip", []),

@@ -78,6 +78,9 @@ struct HostMesh {
 
 // Returns empty string on success, otherwise an error message.
 std::string build_host_mesh(HostMesh& m);
+// point ids (base, A, B) of tet t (after build_host_mesh); with the cell centre they are its
+// four vertices
+void tet_points(const HostMesh& m, int t, int vid[3]);
 
 // Poisson operator in ELL form (K slots per cell) with multicolour DIC.
 struct HostPoisson {

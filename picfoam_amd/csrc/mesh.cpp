@@ -31,6 +31,8 @@ int uf_find(std::vector<int32_t>& p, int a) {
     return a;
 }
 
+}  // namespace
+
 // tetIndices::faceTriIs: (f[b], f[b+pt], f[b+pt+1]) around the face from its base point b,
 // A/B swapped for the neighbour
 void tet_points(const HostMesh& m, int t, int vid[3]) {
@@ -42,6 +44,8 @@ void tet_points(const HostMesh& m, int t, int vid[3]) {
     if (m.owner[f] == c) { vid[1] = pa; vid[2] = pb; }
     else { vid[1] = pb; vid[2] = pa; }
 }
+
+namespace {
 
 // quality of the tet (a, b, c, d): its signed volume over the volume of the regular tet of
 // the same circumradius R, (8 / (9 sqrt 3)) R^3 (1 for a regular tet, <= 0 when inverted or

@@ -26,6 +26,7 @@ ERRORS = {
 }
 PATCH_TYPES = {"patch": 0, "wall": 1, "cyclic": 2, "empty": 3}
 PREC_FP64, PREC_MIXED = 0, 1
+PIC_BC_ZERO_GRADIENT, PIC_BC_FIXED_VALUE = 0, 1
 POISSON_FP64, POISSON_MIXED_IR = 0, 1
 FIELD_E, FIELD_B, FIELD_PHI, FIELD_RHO, FIELD_SPACE_CHARGE, FIELD_E_POINT, FIELD_MCC_SMAX, FIELD_MCC_REM = range(8)
 XS_ELASTIC, XS_EXCITATION, XS_IONIZATION = range(3)
@@ -302,6 +303,20 @@ class Cloud:
         nt, nc, npnt = C.c_int64(), C.c_int32(), C.c_int32()
         self.lib.pic_mesh_sizes(self.h, C.byref(nt), C.byref(nc), C.byref(npnt))
         self.n_tets, self.n_cells, self.n_points = nt.value, nc.value, npnt.value
+
+    @classmethod
+    def adopt(cls, handle, species: list[dict], mesh_arrays=None):
+        """A Cloud around a context created elsewhere (the case layer's pic_case_create);
+        the new object owns the handle and destroys it on close."""
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.mesh = mesh_arrays
+        self.species = species
+        self.h = handle
+        nt, nc, npnt = C.c_int64(), C.c_int32(), C.c_int32()
+        self._check(self.lib.pic_mesh_sizes(self.h, C.byref(nt), C.byref(nc), C.byref(npnt)))
+        self.n_tets, self.n_cells, self.n_points = nt.value, nc.value, npnt.value
+        return self
 
     def close(self):
         if getattr(self, "h", None):

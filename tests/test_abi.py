@@ -11,19 +11,27 @@ from helpers import small_hex
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_entry_points():
-    text = open(os.path.join(ROOT, "include", "picmi.h")).read()
+def declared_entry_points(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
     return sorted(set(re.findall(r"\b(pic_[a-z_0-9]+)\s*\(", text)))
 
 
+def test_every_header_is_checked():
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["picfoam_case.h", "picmi.h"]
+
+
 def test_library_exports_every_declared_symbol():
-    from picfoam_amd import picmi
+    from picfoam_amd import case, picmi
     lib = picmi.load_library()
-    names = declared_entry_points()
+    names = declared_entry_points("picmi.h")
     assert len(names) >= 30
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(picmi.EXPORTS)
+    cnames = declared_entry_points("picfoam_case.h")
+    for n in cnames:
+        assert hasattr(lib, n), n
+    assert set(cnames) - set(names) == set(case.EXPORTS)
     assert b"gfx950" in lib.pic_version()
 
 

@@ -1,0 +1,171 @@
+"""Case directories on the GPU through the case layer: picInitialise, the picFoam time loop
+and the files written back, against the FP64 oracle on the same initial state."""
+import os
+import shutil
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import rel_err, sort_by_id
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CASES = os.path.join(ROOT, "tests", "golden", "cases")
+EPS0 = 8.8541878128e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def copy_case(name, tmp_path):
+    dst = tmp_path / name
+    shutil.copytree(os.path.join(CASES, name), dst)
+    return dst
+
+
+def read_list(path):
+    """the entries of an OpenFOAM ASCII list file written by the case layer"""
+    text = open(path).read()
+    body = text[text.index("}\n", text.index("FoamFile")) + 2:]
+    body = body[body.index("\n(\n") + 3: body.rindex("\n)")]
+    return [ln for ln in body.split("\n") if ln.strip()]
+
+
+def test_plasma_oscillation_case_matches_oracle(oracle, tmp_path):
+    """The plasma-oscillation case: UniformDisplacement parcels uploaded where the host put
+    them; picInitialise's charge density, Poisson solve and half step back of the
+    velocities equal the oracle's; five time steps of the picFoam loop (VolumeWeighting
+    push and deposit, Poisson, E) equal the oracle's; the lagrangian files written at the
+    end read back to the device's parcels."""
+    from picfoam_amd.case import Case
+    d = copy_case("plasma_oscillation", tmp_path)
+    # both sides solve to 1e-13, so that the comparison is not limited by the solve tolerance
+    fv = d / "system" / "fvSolution"
+    fv.write_text(fv.read_text().replace("tolerance       1e-10;", "tolerance       1e-13;"))
+    say("case plasma_oscillation: initialise")
+    cs = Case(d)
+    cs.initialise(seed=1)
+    t0, t1, dt, wi = cs.times
+    cloud = cs.create("fp64", capacity_factor=1.0)
+    m = cs.mesh_dict()
+    rm = oracle.RefMesh(m)
+    P = oracle.RefPoisson(rm)
+    _, species, _ = cs.describe()
+    L = 6.0
+    for s in range(2):
+        xh, _, _ = cs.parcels(s)
+        xg = cloud.positions(s)
+        assert np.max(np.abs(xg - xh)) < 1e-15 * L
+    # picInitialise: rho of both species, phi, E, then U(-dt/2) of the cold electrons
+    y0, t0e, U0, i0 = cloud.get_particles(0)
+    yi, ti, _, _ = cloud.get_particles(1)
+    w = species[0]["nEquivalentParticles"]
+    qe, qi, me = species[0]["charge"], species[1]["charge"], species[0]["mass"]
+    parts = [(y0, t0e, qe * w), (yi, ti, qi * w)]
+    Q0 = abs(qe) * w
+    ac, ap = rm.deposit("VolumeWeighting", parts, Q0)
+    rho = rm.deposit_finish("VolumeWeighting", ac, ap, Q0)
+    assert np.array_equal(cloud.get_field(3), rho)
+    phi, _, _ = P.solve(rho, tol=1e-13)
+    E = -P.grad(phi)
+    assert rel_err(cloud.get_field(2) - cloud.get_field(2).mean(), phi - phi.mean(), floor_frac=1.0) < 1e-8
+    Ep = rm.cell_to_point(E)
+    Ur = rm.push("Boris", "VolumeWeighting", y0, np.zeros_like(U0), t0e, qe, me, -0.5 * dt, E, Ep, np.zeros_like(E))
+    assert rel_err(U0, Ur, floor_frac=1e-2) < 1e-7
+    # five steps of the picFoam loop on both sides from the device's state
+    say("case plasma_oscillation: 5 steps")
+    n = cs.run(cloud, until=5 * dt)
+    assert n == 5 and cs.time_name == "5e-10"
+    # the oracle steps from the same parcels with its own fields of the same charge
+    y, t, U = y0, t0e, U0
+    phi_r, E_r, Ep_r = phi, E, Ep
+    for _ in range(5):
+        y, U, t, _, _ = rm.move("Boris", "VolumeWeighting", y, U, t, qe, me, dt, E_r, Ep_r, np.zeros_like(E_r))
+        ac, ap = rm.deposit("VolumeWeighting", [(y, t, qe * w), (yi, ti, qi * w)], Q0)
+        rho_r = rm.deposit_finish("VolumeWeighting", ac, ap, Q0)
+        phi_r, _, _ = P.solve(rho_r, phi_r, tol=1e-13)
+        E_r = -P.grad(phi_r)
+        Ep_r = rm.cell_to_point(E_r)
+    yg, tg, Ug, ig = cloud.get_particles(0)
+    (Ug_s, xg_s) = sort_by_id(ig, Ug, rm.positions(yg, tg))
+    (Ur_s, xr_s) = sort_by_id(i0, U, rm.positions(y, t))
+    err_u = rel_err(Ug_s, Ur_s, floor_frac=1e-2)
+    err_x = np.max(np.abs(xg_s - xr_s)) / L
+    say(f"after 5 steps: U rel err {err_u:.2e}, x err / L {err_x:.2e}")
+    assert err_u < 1e-6 and err_x < 1e-12
+    # the files of the last step read back to the device's parcels
+    cd = d / "5e-10" / "lagrangian" / "pic"
+    for f in ("positions", "coordinates", "U", "typeId", "nParticle", "chargeModifier"):
+        assert (cd / f).is_file(), f
+    for f in ("phiE", "E", "rhoCharge"):
+        assert (d / "5e-10" / f).is_file(), f
+    assert len(read_list(cd / "typeId")) == cloud.count(0) + cloud.count(1)
+    back = Case(d)
+    back.read_parcels("5e-10")
+    xb, cb, Ub = back.parcels(0)
+    xg = cloud.positions(0)
+    assert np.max(np.abs(xb - xg)) < 1e-15 * L
+    assert np.array_equal(Ub, Ug)
+    cloud.close()
+
+
+def test_thermalization_case_matches_oracle_and_runs(oracle, tmp_path):
+    """The 2-D thermalisation case (Perez collisions, Nanbu-Yonemura pairing, lnL 15, no
+    field solve): two steps equal the oracle's move, sort and collisions parcel by parcel;
+    the picFoam loop then runs to endTime, writing at 2e-13, 4e-13 and the end, with every
+    parcel kept on the slab's mid-plane."""
+    from picfoam_amd.case import Case
+    d = copy_case("thermalization", tmp_path)
+    cs = Case(d)
+    cs.initialise(seed=4)
+    _, _, dt, _ = cs.times
+    _, species, models = cs.describe()
+    cloud = cs.create("fp64", capacity_factor=1.0)
+    rm = oracle.RefMesh(cs.mesh_dict())
+    st = [cloud.get_particles(s) for s in range(2)]
+    masses = [s["mass"] for s in species]
+    charges = [s["charge"] for s in species]
+    w = species[0]["nEquivalentParticles"]
+    ys, ts, Us, ids = [list(v) for v in zip(*st)]
+    Z = np.zeros((rm.n_cells, 3))
+    cl = np.zeros((8, 8)); cl[:2, :2] = 15.0
+    say("case thermalization: 2 steps against the oracle")
+    for step in range(2):
+        cloud.step(dt)
+        css = []
+        for s in range(2):
+            ys[s], Us[s], ts[s], _, _ = rm.move("Boris", "CellAverage", ys[s], Us[s], ts[s], charges[s], masses[s], dt,
+                                               Z, None, Z)
+            order, cs_ = rm.sort_by_cell(ts[s])
+            ys[s], ts[s], Us[s], ids[s] = ys[s][order], ts[s][order], Us[s][order], ids[s][order]
+            css.append(cs_)
+        Us, _ = oracle.coulomb(rm, [(masses[s], charges[s], w) for s in range(2)], Us, ids, css, dt,
+                               seed=models.seed, step=step, coulomb_log=cl, model="Perez", pairing="NanbuYonemura")
+    for s in range(2):
+        yg, tg, Ug, ig = cloud.get_particles(s)
+        (Ug_s, ig_s) = sort_by_id(ig, Ug, ig)
+        (Ur_s, ir_s) = sort_by_id(ids[s], Us[s], ids[s])
+        assert np.array_equal(ig_s, ir_s)
+        err = rel_err(Ug_s, Ur_s)
+        say(f"species {s}: U rel err after 2 steps {err:.2e}")
+        assert err < 1e-12
+    cloud.close()
+    # the time loop to endTime from a fresh cloud
+    cloud = cs.create("fp64", capacity_factor=1.0)
+    n = cs.run(cloud)
+    assert n == 5 and cs.time_name == "5e-13"
+    for tn in ("2e-13", "4e-13", "5e-13"):
+        assert (d / tn / "lagrangian" / "pic" / "coordinates").is_file(), tn
+    assert not (d / "3e-13").exists()
+    for s in range(2):
+        assert cloud.count(s) == 32000
+        x = cloud.positions(s)
+        assert np.max(np.abs(x[:, 2])) < 1e-18
+    back = Case(d)
+    back.read_parcels("5e-13")
+    assert [back.parcel_count(s) for s in range(2)] == [32000, 32000]
+    cloud.close()
