@@ -693,11 +693,82 @@ static int cross_face(const ref_mesh* m, int t, int iH, double y[4]) {
     return -1000000;
 }
 
+/* what a wall hit needs besides the mesh: the push of the velocity synchronisation and
+ * the reflection model */
+struct track_ctx {
+    int pusher, weighting, hc_reference_form, rotate, do_push;
+    double q, mass, dt;
+    const double *Ecell, *Epoint, *Bcell;
+    const ref_wall_params* walls;
+    uint32_t id;
+    int hits;      /* wall hits of this parcel in this step */
+    int resyncs;   /* velocity resyncs after a reflection (PICParcel.C:126-137) */
+    int64_t total_hits;
+};
+
+#define TAG_WALL 6u
+static inline double u01(uint32_t a, uint32_t b);
+static inline void rng4(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t out[4]);
+
+/* syncVelocityAtBoundary (PICParcel.C:201-210): a push of the velocity by fraction * dt
+ * with the field at the parcel's place on the face */
+static void sync_velocity(const ref_mesh* m, const struct track_ctx* x, int t, const double y[4], double* U,
+                          double fraction) {
+    if (!x->walls->sync || x->q == 0.0 || !x->do_push) return;
+    const double yy[3] = {y[1], y[2], y[3]};
+    v3 E = interp_one(m, x->weighting, t, yy, x->Ecell, x->Epoint);
+    double e[3] = {E.x, E.y, E.z};
+    ref_push_one(x->pusher, U, e, x->Bcell + 3 * m->tet_cell[t], x->q, x->mass, fraction * x->dt,
+                 x->hc_reference_form, x->rotate);
+}
+
+/* The wall's reflection (SpecularReflection.C:55-68, MaxwellianThermal.C:55-118,
+ * MixedDiffuseSpecular.C:55-131): n is the outward unit normal of the face triangle. */
+static void reflect(const ref_mesh* m, struct track_ctx* x, int patch, v3 n, double* Up) {
+    const ref_wall_params* w = x->walls;
+    v3 U = ld3(Up);
+    uint32_t o[4];
+    const uint32_t c = (x->walls->species << 16) | (uint32_t)(x->hits & 0xffff);
+    rng4(w->seed, x->id, (uint32_t)w->step, c, TAG_WALL << 24, o);
+    const double u_mix = u01(o[0], o[1]), u_n = u01(o[2], o[3]);
+    int diffuse = w->model == REF_WALL_MAXWELLIAN_THERMAL ||
+                  (w->model == REF_WALL_MIXED && w->diffuse_fraction > u_mix);
+    double un = vdot(U, n);
+    if (!diffuse) {
+        if (un > 0.0) U = vsub(U, vscale(2.0 * un, n));
+        st3(Up, U);
+        return;
+    }
+    v3 Ut = vsub(U, vscale(un, n));
+    for (uint32_t k = 2; vmag(Ut) < 1e-15 && k < 200; k += 2) {
+        /* incident along the normal: perturb the velocity until a tangent exists */
+        uint32_t a[4], b[4];
+        rng4(w->seed, x->id, (uint32_t)w->step, c, (TAG_WALL << 24) | k, a);
+        rng4(w->seed, x->id, (uint32_t)w->step, c, (TAG_WALL << 24) | (k + 1), b);
+        U = V(U.x * (0.8 + 0.2 * u01(a[0], a[1])), U.y * (0.8 + 0.2 * u01(a[2], a[3])),
+              U.z * (0.8 + 0.2 * u01(b[0], b[1])));
+        un = vdot(U, n);
+        Ut = vsub(U, vscale(un, n));
+    }
+    const v3 t1 = vdiv(Ut, vmag(Ut));
+    const v3 t2 = vcross(n, t1);
+    uint32_t g[4];
+    rng4(w->seed, x->id, (uint32_t)w->step, c, (TAG_WALL << 24) | 1u, g);
+    const double uc = u01(g[0], g[1]), ud = u01(g[2], g[3]);
+    const double r = sqrt(-2.0 * log(1.0 - uc));
+    const double n1 = r * cos(2.0 * PI_D * ud), n2 = r * sin(2.0 * PI_D * ud);
+    const double s = sqrt(KB * w->T[patch] / x->mass);
+    const double vn = sqrt(-2.0 * log(fmax(1.0 - u_n, 1e-300)));
+    U = vscale(s, vsub(vadd(vscale(n1, t1), vscale(n2, t2)), vscale(vn, n)));
+    U = vadd(U, ld3(w->U + 3 * patch));
+    st3(Up, U);
+}
+
 /* PICParcel::moveForward (PICParcel.C:81-144) with OpenFOAM-9 barycentric tracking
  * (particle::trackToFace / trackToStationaryTet re-derived): per tet, the displacement
  * (1-f)*dt*U is taken to barycentric form Tx_i = r_i.d (det-scaled); the first face with
  * Tx_i < 0 and 0 <= -y_i/Tx_i < 1/det is hit. */
-static int track_one(const ref_mesh* m, double* yy, double* Up, int* tetp, double dt) {
+static int track_one(const ref_mesh* m, double* yy, double* Up, int* tetp, double dt, struct track_ctx* x) {
     int t = *tetp;
     double y[4];
     y[1] = yy[0]; y[2] = yy[1]; y[3] = yy[2];
@@ -738,6 +809,25 @@ static int track_one(const ref_mesh* m, double* yy, double* Up, int* tetp, doubl
         int s = cross_face(m, t, iH, y);
         if (s == REF_EMPTY_FACE) break; /* on an empty face (rounding only): stays */
         if (s <= -1000000) { status = 2; break; }
+        if (s < 0 && x && x->walls && m->ptype[-1 - s] == REF_PATCH_WALL) {
+            /* PICParcel::hitPatch then the resync of moveForward: the parcel stays on the
+             * face in its tet and tracks the rest of the step with the new velocity */
+            const int patch = -1 - s;
+            v3 r1 = vcross(e2, e3), r2 = vcross(e3, e1), r3 = vcross(e1, e2);
+            v3 nn = vadd(vadd(r1, r2), r3);
+            nn = vdiv(nn, vmag(nn));
+            sync_velocity(m, x, t, y, Up, sf - 0.5);
+            reflect(m, x, patch, nn, Up);
+            if (x->resyncs++ == 0) sync_velocity(m, x, t, y, Up, 0.5 - sf);
+            ++x->hits;
+            ++x->total_hits;
+            U = ld3(Up);
+            if (m->track_mask & 1) U.x = 0.0;
+            if (m->track_mask & 2) U.y = 0.0;
+            if (m->track_mask & 4) U.z = 0.0;
+            if (++guard > MAX_TRACK_STEPS) { status = 1; break; }
+            continue;
+        }
         if (s < 0) { t = -1; break; } /* left through a non-coupled patch: deleted (NoInflow) */
         t = s;
         if (++guard > MAX_TRACK_STEPS) { status = 1; break; }
@@ -747,12 +837,14 @@ static int track_one(const ref_mesh* m, double* yy, double* Up, int* tetp, doubl
     return status;
 }
 
-int ref_move(const ref_mesh* m, int pusher, int weighting, int n, double* y, double* U, int* tet, double q,
-             double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell, int do_push,
-             int hc_reference_form, int rotate, int* n_superluminal) {
+int ref_move_walls(const ref_mesh* m, int pusher, int weighting, int n, double* y, double* U, int* tet, double q,
+                   double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell,
+                   int do_push, int hc_reference_form, int rotate, int* n_superluminal,
+                   const ref_wall_params* walls) {
     int guards = 0, sl = 0;
+    int64_t hits = 0;
     /* parcels are independent: spread over the host threads for the CPU baseline */
-#pragma omp parallel for schedule(static) reduction(+ : guards, sl)
+#pragma omp parallel for schedule(static) reduction(+ : guards, sl, hits)
     for (int i = 0; i < n; ++i) {
         if (tet[i] < 0) continue;
         if (do_push && q != 0.0) {
@@ -762,10 +854,21 @@ int ref_move(const ref_mesh* m, int pusher, int weighting, int n, double* y, dou
                          rotate);
         }
         if (vmag(ld3(U + 3 * i)) >= C_LIGHT) ++sl; /* PICParcel.C:61-66 */
-        if (track_one(m, y + 3 * i, U + 3 * i, tet + i, dt)) ++guards;
+        struct track_ctx x = {pusher, weighting, hc_reference_form, rotate, do_push, q, mass, dt,
+                              Ecell, Epoint, Bcell, walls, walls ? walls->id[i] : 0u, 0, 0, 0};
+        if (track_one(m, y + 3 * i, U + 3 * i, tet + i, dt, &x)) ++guards;
+        hits += x.total_hits;
     }
     if (n_superluminal) *n_superluminal = sl;
+    if (walls && walls->hits) *walls->hits += hits;
     return guards;
+}
+
+int ref_move(const ref_mesh* m, int pusher, int weighting, int n, double* y, double* U, int* tet, double q,
+             double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell, int do_push,
+             int hc_reference_form, int rotate, int* n_superluminal) {
+    return ref_move_walls(m, pusher, weighting, n, y, U, tet, q, mass, dt, Ecell, Epoint, Bcell, do_push,
+                          hc_reference_form, rotate, n_superluminal, NULL);
 }
 
 /* ------------------------------------------------------------------ deposit */

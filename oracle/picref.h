@@ -64,6 +64,28 @@ int ref_move(const ref_mesh* m, int pusher, int weighting, int n, double* y, dou
              double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell,
              int do_push, int hc_reference_form, int rotate, int* n_superluminal);
 
+/* Wall patches (PICParcel.C:153-184, :209-330; the WallReflectionModel models): a parcel that
+ * reaches a wall face has its velocity synchronised to the hit time, reflected by the
+ * model, and synchronised back (the second and later resyncs of a step skipped), then tracks
+ * on.  Without walls (NULL) a wall face deletes the parcel like any non-coupled patch. */
+enum { REF_WALL_SPECULAR = 0, REF_WALL_MAXWELLIAN_THERMAL = 1, REF_WALL_MIXED = 2 };
+typedef struct {
+    int model;                 /* REF_WALL_* */
+    double diffuse_fraction;   /* MixedDiffuseSpecularCoeffs.diffuseFraction */
+    const double* T;           /* [n_patches] wall temperature (boundaryT) */
+    const double* U;           /* [n_patches*3] wall velocity (boundaryU) */
+    int sync;                  /* SolverSettings.syncVelocityAtBoundary */
+    uint64_t seed;
+    int64_t step;
+    uint32_t species;
+    const uint32_t* id;        /* [n] parcel ids: the keys of the reflection draws */
+    int64_t* hits;             /* optional [1]: wall hits counted */
+} ref_wall_params;
+int ref_move_walls(const ref_mesh* m, int pusher, int weighting, int n, double* y, double* U, int* tet, double q,
+                   double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell,
+                   int do_push, int hc_reference_form, int rotate, int* n_superluminal,
+                   const ref_wall_params* walls);
+
 /* push only (leapfrog initialisation, picInitialise.C:228-237) */
 void ref_push(const ref_mesh* m, int pusher, int weighting, int n, const double* y, double* U, const int* tet,
               double q, double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell,

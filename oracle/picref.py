@@ -16,6 +16,13 @@ LIB = os.path.join(HERE, "libpicref.so")
 PUSHERS = {"none": 0, "Boris": 1, "BorisNR": 2, "Vay": 3, "HigueraCary": 4}
 WEIGHTINGS = {"CellAverage": 0, "VolumeWeighting": 1}
 PATCH = {"patch": 0, "wall": 1, "cyclic": 2, "empty": 3}
+WALL_MODELS = {"SpecularReflection": 0, "MaxwellianThermal": 1, "MixedDiffuseSpecular": 2}
+
+
+class WallParams(C.Structure):
+    _fields_ = [("model", C.c_int), ("diffuse_fraction", C.c_double), ("T", C.c_void_p), ("U", C.c_void_p),
+                ("sync", C.c_int), ("seed", C.c_uint64), ("step", C.c_int64), ("species", C.c_uint32),
+                ("id", C.c_void_p), ("hits", C.c_void_p)]
 
 
 def build():
@@ -53,6 +60,10 @@ def lib():
         _lib.ref_move.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                   vp, vp, vp, C.c_int, C.c_int, C.c_int, vp]
         _lib.ref_move.restype = C.c_int
+        _lib.ref_move_walls.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_double, C.c_double,
+                                        C.c_double, vp, vp, vp, C.c_int, C.c_int, C.c_int, vp,
+                                        C.POINTER(WallParams)]
+        _lib.ref_move_walls.restype = C.c_int
         _lib.ref_push.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                   vp, vp, vp, C.c_int, C.c_int]
         _lib.ref_deposit.argtypes = [vp, C.c_int, C.c_int, vp, vp, C.c_double, vp, vp]
@@ -151,7 +162,11 @@ class RefMesh:
         return y, tet, fails
 
     def move(self, pusher, weighting, y, U, tet, q, mass, dt, Ecell, Epoint, Bcell, do_push=True, hc_ref=False,
-             rotate=None):
+             rotate=None, walls=None):
+        """walls: None (a wall face deletes like any open patch) or a dict with model
+        ("SpecularReflection", "MaxwellianThermal", "MixedDiffuseSpecular"), diffuse_fraction,
+        T [n_patches], U [n_patches, 3], sync (bool), seed, step, species, ids [n]; its
+        "hits" entry is set to the number of wall hits"""
         y = np.ascontiguousarray(y, np.float64).copy(); U = np.ascontiguousarray(U, np.float64).copy()
         tet = np.ascontiguousarray(tet, np.int32).copy()
         Ecell = np.ascontiguousarray(Ecell, np.float64); Bcell = np.ascontiguousarray(Bcell, np.float64)
@@ -159,8 +174,22 @@ class RefMesh:
         if rotate is None:
             rotate = bool(np.any(Bcell != 0.0)) or pusher == "Vay"
         sl = C.c_int(0)
-        g = lib().ref_move(self.h, PUSHERS[pusher], WEIGHTINGS[weighting], len(tet), _p(y), _p(U), _p(tet), q, mass,
-                           dt, _p(Ecell), _p(Epoint), _p(Bcell), int(do_push), int(hc_ref), int(rotate), C.byref(sl))
+        if walls is None:
+            g = lib().ref_move(self.h, PUSHERS[pusher], WEIGHTINGS[weighting], len(tet), _p(y), _p(U), _p(tet), q,
+                               mass, dt, _p(Ecell), _p(Epoint), _p(Bcell), int(do_push), int(hc_ref), int(rotate),
+                               C.byref(sl))
+            return y, U, tet, g, sl.value
+        wT = np.ascontiguousarray(walls["T"], np.float64)
+        wU = np.ascontiguousarray(walls["U"], np.float64)
+        ids = np.ascontiguousarray(walls["ids"], np.uint32)
+        hits = np.zeros(1, np.int64)
+        w = WallParams(WALL_MODELS[walls["model"]], float(walls.get("diffuse_fraction", 0.0)), _p(wT), _p(wU),
+                       int(walls.get("sync", True)), int(walls["seed"]), int(walls["step"]), int(walls["species"]),
+                       _p(ids), _p(hits))
+        g = lib().ref_move_walls(self.h, PUSHERS[pusher], WEIGHTINGS[weighting], len(tet), _p(y), _p(U), _p(tet), q,
+                                 mass, dt, _p(Ecell), _p(Epoint), _p(Bcell), int(do_push), int(hc_ref), int(rotate),
+                                 C.byref(sl), C.byref(w))
+        walls["hits"] = int(hits[0])
         return y, U, tet, g, sl.value
 
     def push(self, pusher, weighting, y, U, tet, q, mass, dt, Ecell, Epoint, Bcell, hc_ref=False, rotate=None):
