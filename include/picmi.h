@@ -72,6 +72,8 @@ typedef struct {
     int32_t neighbour_patch; /* cyclic partner patch, -1 otherwise */
     int32_t phi_bc;          /* PIC_BC_* for the Poisson solve */
     double phi_value;        /* fixedValue */
+    double wall_temperature; /* wall patches: boundaryT (K), for diffuse reflection */
+    double wall_velocity[3]; /* wall patches: boundaryU (m/s) */
 } pic_patch;
 
 /* OpenFOAM polyMesh: internal faces first (upper-triangular order), face normals out of
@@ -132,6 +134,14 @@ typedef struct {
      * accountForDrift false (1 here) to keep the drift in the species temperature */
     uint32_t debye_species;
     int32_t debye_ignore_drift;
+    /* BoundaryModels.WallReflectionModel for wall patches: SpecularReflection (also when
+     * NULL), MaxwellianThermal, MixedDiffuseSpecular with MixedDiffuseSpecularCoeffs.
+     * diffuseFraction (PICParcel.C:209-330, WallReflectionModel/); a parcel reaching a wall
+     * has its velocity synchronised to the hit time, reflected and synchronised back
+     * (PICParcel.C:153-210) unless SolverSettings.syncVelocityAtBoundary is false (1 here) */
+    const char* wall_reflection;
+    double diffuse_fraction;
+    int32_t no_boundary_sync;
 } pic_models;
 
 typedef struct {

@@ -72,6 +72,15 @@ struct pic_ctx {
     int allow_intra = 1, calc_bmin = 1;
     uint32_t debye_mask = 0xFFu;             // species in the Debye length (the charged ones of them)
     int debye_drift = 1;                     // DebyeLength.accountForDrift
+    int wall_model = 0;                      // WallReflectionModel: Specular, MaxwellianThermal, Mixed
+    double diffuse_fraction = 0.0;
+    int wall_sync = 1;                       // syncVelocityAtBoundary
+    bool walls = false;                      // the mesh has wall patches
+    uint8_t* d_patch_kind = nullptr;         // per patch: 1 wall, 0 otherwise
+    double *d_wall_T = nullptr, *d_wall_U = nullptr;
+    WallHit* d_wall_q = nullptr;             // parcels on a wall after the move (all moving species)
+    unsigned* d_wall_n = nullptr;
+    unsigned wall_cap = 0;
     double coulomb_log[kMaxSpecies][kMaxSpecies] = {};
     int electron = -1, ion = -1;
     double bg_n = 0, bg_T = 0, bg_m = 0, bg_w = 0, bg_v[3] = {0, 0, 0};
@@ -298,6 +307,7 @@ void zero_deposit(pic_ctx* c) {
         z.p[z.k] = c->d_acc_point; z.words[z.k++] = 2 * (int64_t)c->hm.n_points;
     }
     z.p[z.k] = c->d_big_n; z.words[z.k++] = 1;
+    if (c->d_wall_n) { z.p[z.k] = c->d_wall_n; z.words[z.k++] = 1; }
     launch_zero(z, c->stream);
     c->big_n_zero = true;
 }
@@ -326,6 +336,21 @@ MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
     a.rotate = c->rotate ? 1 : 0;
     a.hc_ref = c->hc_ref;
     a.track_mask = c->hm.track_mask;
+    if (c->walls) {
+        a.patch_kind = c->d_patch_kind;
+        a.wall_T = c->d_wall_T;
+        a.wall_U = c->d_wall_U;
+        a.wall_model = c->wall_model;
+        a.wall_sync = c->wall_sync;
+        a.diffuse_fraction = c->diffuse_fraction;
+        a.wall_q = c->d_wall_q;
+        a.wall_n = c->d_wall_n;
+        a.wall_cap = c->wall_cap;
+    }
+    a.mass = S.mass;
+    a.seed = c->seed;
+    a.step = c->step;
+    a.species = (uint32_t)s;
     a.count = S.count;
     a.place = S.place;
     a.acc_cell = c->d_acc_cell;
@@ -351,7 +376,9 @@ int do_move_t(pic_ctx* c, double dt) {
         }
     }
     if (m.ns == 0) return PIC_OK;
-    return check_hip(c, launch_move(m, c->pusher, c->fw, c->cd, c->stream), "move kernel");
+    if (int r = check_hip(c, launch_move(m, c->pusher, c->fw, c->cd, c->stream), "move kernel")) return r;
+    if (c->walls) return check_hip(c, launch_wall(m, c->pusher, c->fw, c->cd, c->stream), "wall pass");
+    return PIC_OK;
 }
 
 int do_move(pic_ctx* c, double dt) {
@@ -785,6 +812,14 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         g_last_error = c->err = "ElectronNeutralCollisionModel needs the uniformBackground gas model (particle neutrals are not implemented)";
         return PIC_ERR_UNSUPPORTED;
     }
+    if ((r = parse_model(c.get(), models->wall_reflection ? models->wall_reflection : "SpecularReflection",
+                         "WallReflectionModel", {"SpecularReflection", "MaxwellianThermal", "MixedDiffuseSpecular"},
+                         &c->wall_model)))
+        return r;
+    c->diffuse_fraction = models->diffuse_fraction;
+    if (c->wall_model == 2 && !(c->diffuse_fraction >= 0.0 && c->diffuse_fraction <= 1.0))
+        return c->fail(PIC_ERR_INVALID, "MixedDiffuseSpecularCoeffs.diffuseFraction must lie in [0, 1]");
+    c->wall_sync = models->no_boundary_sync ? 0 : 1;
     c->precision = models->precision;
     if (c->precision != PIC_PREC_FP64 && c->precision != PIC_PREC_MIXED) return c->fail(PIC_ERR_INVALID, "precision");
     c->hc_ref = models->hc_reference_form;
@@ -868,7 +903,10 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     m.nbr.assign(mesh->neighbour, mesh->neighbour + m.n_internal);
     for (int p = 0; p < m.n_patches; ++p) {
         const pic_patch& pp = mesh->patches[p];
-        if (pp.type == PIC_PATCH_WALL) return c->fail(PIC_ERR_UNSUPPORTED, "wall patches (WallReflectionModel) are not implemented");
+        if (pp.type == PIC_PATCH_WALL) {
+            c->walls = true;
+            if (!(pp.wall_temperature >= 0.0)) return c->fail(PIC_ERR_INVALID, "patch " + std::to_string(p) + ": wall temperature");
+        }
         if (pp.type < PIC_PATCH_PATCH || pp.type > PIC_PATCH_EMPTY)
             return c->fail(PIC_ERR_INVALID, "patch " + std::to_string(p) + ": unknown type");
         if (pp.type == PIC_PATCH_EMPTY && pp.phi_bc != PIC_BC_ZERO_GRADIENT)
@@ -909,6 +947,21 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     // device mesh
     const int nc = m.n_cells, np = m.n_points;
     c->d_tets = c->upload(m.tets);
+    if (c->walls) {
+        std::vector<uint8_t> kind(m.n_patches, 0);
+        std::vector<double> wT(m.n_patches, 0.0), wU(3 * (size_t)m.n_patches, 0.0);
+        for (int p = 0; p < m.n_patches; ++p) {
+            const pic_patch& pp = mesh->patches[p];
+            if (pp.type != PIC_PATCH_WALL) continue;
+            kind[p] = 1;
+            wT[p] = pp.wall_temperature;
+            for (int d = 0; d < 3; ++d) wU[3 * p + d] = pp.wall_velocity[d];
+        }
+        c->d_patch_kind = c->upload(kind);
+        c->d_wall_T = c->upload(wT);
+        c->d_wall_U = c->upload(wU);
+        if (!c->d_patch_kind || !c->d_wall_T || !c->d_wall_U) return c->fail(PIC_ERR_HIP, "out of device memory (walls)");
+    }
     c->d_cell_tet0 = c->upload(m.cell_tet0);
     c->d_C = c->upload(m.C);
     c->d_V = c->upload(m.V);
@@ -960,6 +1013,16 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     }
     for (auto& S : c->sp)
         if ((r = alloc_species(c.get(), S))) return r;
+    if (c->walls) {
+        // every parcel of every species may end a move on a wall
+        size_t cap = 0;
+        for (auto& S : c->sp) cap += (size_t)S.capacity;
+        if (cap > 0xFFFFFFFFull) return c->fail(PIC_ERR_CAPACITY, "wall queue: total capacity exceeds 2^32 parcels");
+        c->wall_cap = (unsigned)cap;
+        c->d_wall_q = c->alloc<WallHit>(std::max<size_t>(cap, 1));
+        c->d_wall_n = c->alloc<unsigned>(1);
+        if (!c->d_wall_q || !c->d_wall_n) return c->fail(PIC_ERR_HIP, "out of device memory (wall queue)");
+    }
     // Poisson device tables
     if (c->has_poisson) {
         HostPoisson& P = c->hp;

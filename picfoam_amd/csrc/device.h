@@ -141,7 +141,7 @@ __device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
     return (double)x * (1.0 / 9007199254740992.0);
 }
 
-constexpr uint32_t kTagShuffle = 1u, kTagCoulomb = 2u, kTagMcc = 3u, kTagNewId = 4u, kTagInit = 5u;
+constexpr uint32_t kTagShuffle = 1u, kTagCoulomb = 2u, kTagMcc = 3u, kTagNewId = 4u, kTagInit = 5u, kTagWall = 6u;
 
 // error flags raised by kernels (bitmask in device memory)
 enum : uint32_t {

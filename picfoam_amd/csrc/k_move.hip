@@ -118,11 +118,22 @@ __device__ __forceinline__ double pick_slot(uint32_t k, double y0, double y1, do
 // continues in the neighbouring tet with its coordinates permuted by the adjacency table.
 // Returns the final tet (-1 = left through a non-coupled patch, deleted as NoInflow does);
 // *status = 1 when the iteration guard stopped the track, 2 when the table had no
-// neighbour.
-__device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, double& y1, double& y2, double& y3,
-                                     double Ux, double Uy, double Uz, double dt, int& status) {
-    double Y0 = 1.0 - (y1 + y2 + y3), Y1 = y1, Y2 = y2, Y3 = y3;
-    double sf = 0.0;
+// neighbour, 3 when the parcel reached a wall face (MoveArgs::patch_kind 1): it then stands
+// on the face in its tet, sf holds the step fraction done, and the wall pass (k_wall) goes on
+// from there.  The track starts at step fraction sf (0 for a new step); on_wall starts it on
+// the tet's face 0 with Y0 exactly 0, as the parcel stood when its wall was hit (recomputing
+// 1 - (Y1 + Y2 + Y3) there can give -1e-17, and a parcel still heading out would then miss
+// the wall).  U is tracked without its empty components (reduced-dimension cases,
+// PICParcel.C:118-123).
+template <class UT>
+__device__ __forceinline__ int track(const MoveArgs<UT>& a, int t, double& y1, double& y2, double& y3,
+                                     const vec3<UT>& U, double& sf, int& status, bool on_wall = false) {
+    const TetGeo* __restrict__ tets = a.tets;
+    const double dt = a.dt;
+    const uint32_t m = a.track_mask;
+    const double Ux = (m & 1u) ? 0.0 : (double)U.x, Uy = (m & 2u) ? 0.0 : (double)U.y,
+                 Uz = (m & 4u) ? 0.0 : (double)U.z;
+    double Y0 = on_wall ? 0.0 : 1.0 - (y1 + y2 + y3), Y1 = y1, Y2 = y2, Y3 = y3;
     int guard = 0;
     status = 0;
     while (sf < 1.0) {
@@ -165,6 +176,10 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
         const int nb = (iH & 2) ? ((iH & 1) ? nb4.w : nb4.z) : ((iH & 1) ? nb4.y : nb4.x);
         if (nb == kTetEmptyFace) break;   // only rounding brings a parcel onto an empty face
         if (nb < 0) {
+            if (nb > -1000000 && a.patch_kind && a.patch_kind[-1 - nb] == 1) {
+                status = 3;   // a wall: the parcel waits on the face for the wall pass
+                break;
+            }
             if (nb <= -1000000) status = 2;
             t = -1;
             break;
@@ -181,6 +196,69 @@ __device__ __forceinline__ int track(const TetGeo* __restrict__ tets, int t, dou
     }
     y1 = Y1; y2 = Y2; y3 = Y3;
     return t;
+}
+
+// A wall hit (PICParcel::hitPatch, PICParcel.C:153-184, with the resync of moveForward,
+// :126-137): the velocity is synchronised to the hit time by a push of (sf - 1/2) dt with the
+// field at the hit point, reflected by the wall model (SpecularReflection.C:55-68,
+// MaxwellianThermal.C:55-118, MixedDiffuseSpecular.C:55-131) about the face triangle's
+// outward normal (r1 + r2 + r3 of the tet record), and synchronised back by (1/2 - sf) dt,
+// that last push only on the first reflection of the step as in the reference.  The
+// diffuse draws are Philox keyed by (seed; id, step, species and hit, wall tag).
+template <class UT, int P, int W>
+__device__ __forceinline__ void wall_hit(const MoveArgs<UT>& a, const TetGeo& g, double Y1, double Y2, double Y3,
+                                         double sf, int patch, vec3<UT>& U, uint32_t id, int hits, int& resyncs) {
+    double nx = g.r[0] + g.r[3] + g.r[6], ny = g.r[1] + g.r[4] + g.r[7], nz = g.r[2] + g.r[5] + g.r[8];
+    const double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+    nx *= inv; ny *= inv; nz *= inv;
+    const bool sync = a.wall_sync && a.do_push;
+    vec3<UT> E = mk3<UT>(0, 0, 0), B = mk3<UT>(0, 0, 0);
+    if (sync) {
+        E = interp_E<UT, W>(g, Y1, Y2, Y3, a.Ecell, a.Epoint);
+        if (a.rotate) B = mk3<UT>(a.Bcell[3 * g.cell], a.Bcell[3 * g.cell + 1], a.Bcell[3 * g.cell + 2]);
+        const UT f = (UT)(sf - 0.5);
+        U = push_u<UT, P>(U, E, B, a.k * f, a.kv * f, a.rotate != 0, a.hc_ref != 0);
+    }
+    double ux = (double)U.x, uy = (double)U.y, uz = (double)U.z;
+    const uint32_t c = (a.species << 16) | ((uint32_t)hits & 0xffffu);
+    uint4 o = rng4(a.seed, id, (uint32_t)a.step, c, kTagWall << 24);
+    const double u_mix = u01(o.x, o.y), u_n = u01(o.z, o.w);
+    const bool diffuse = a.wall_model == 1 || (a.wall_model == 2 && a.diffuse_fraction > u_mix);
+    double un = ux * nx + uy * ny + uz * nz;
+    if (!diffuse) {
+        if (un > 0.0) { ux -= 2.0 * un * nx; uy -= 2.0 * un * ny; uz -= 2.0 * un * nz; }
+    } else {
+        double tx = ux - un * nx, ty = uy - un * ny, tz = uz - un * nz;
+        for (uint32_t k = 2; sqrt(tx * tx + ty * ty + tz * tz) < 1e-15 && k < 200; k += 2) {
+            // incident along the normal: perturb until a tangent exists
+            const uint4 p = rng4(a.seed, id, (uint32_t)a.step, c, (kTagWall << 24) | k);
+            const uint4 q = rng4(a.seed, id, (uint32_t)a.step, c, (kTagWall << 24) | (k + 1));
+            ux *= 0.8 + 0.2 * u01(p.x, p.y);
+            uy *= 0.8 + 0.2 * u01(p.z, p.w);
+            uz *= 0.8 + 0.2 * u01(q.x, q.y);
+            un = ux * nx + uy * ny + uz * nz;
+            tx = ux - un * nx; ty = uy - un * ny; tz = uz - un * nz;
+        }
+        const double it = 1.0 / sqrt(tx * tx + ty * ty + tz * tz);
+        tx *= it; ty *= it; tz *= it;
+        const double bx = ny * tz - nz * ty, by = nz * tx - nx * tz, bz = nx * ty - ny * tx;   // n x t1
+        o = rng4(a.seed, id, (uint32_t)a.step, c, (kTagWall << 24) | 1u);
+        const double uc = u01(o.x, o.y), ud = u01(o.z, o.w);
+        const double r = sqrt(-2.0 * log(1.0 - uc));
+        double sn, cs;
+        sincospi(2.0 * ud, &sn, &cs);
+        const double n1 = r * cs, n2 = r * sn;
+        const double s = sqrt(kKB * a.wall_T[patch] / a.mass);
+        const double vn = sqrt(-2.0 * log(fmax(1.0 - u_n, 1e-300)));
+        ux = s * (n1 * tx + n2 * bx - vn * nx) + a.wall_U[3 * patch];
+        uy = s * (n1 * ty + n2 * by - vn * ny) + a.wall_U[3 * patch + 1];
+        uz = s * (n1 * tz + n2 * bz - vn * nz) + a.wall_U[3 * patch + 2];
+    }
+    U = mk3<UT>((UT)ux, (UT)uy, (UT)uz);
+    if (sync && resyncs++ == 0) {
+        const UT f = (UT)(0.5 - sf);
+        U = push_u<UT, P>(U, E, B, a.k * f, a.kv * f, a.rotate != 0, a.hc_ref != 0);
+    }
 }
 
 // VolumeWeighting deposit (VolumeWeighting.C:105-123) of one block's parcels, privatised in
@@ -245,7 +323,7 @@ __device__ void deposit_vw_block(const TetGeo* __restrict__ tets, int cell, int 
 // One lane per parcel: push (if charged and moving) + track + deposit/count; `block` is
 // the block's index within its species.
 template <class UT, int P, int W, int DEP>
-__device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block, DepLds* dl) {
+__device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block, int set, DepLds* dl) {
     const int64_t i = block * blockDim.x + threadIdx.x;
     const int64_t n = a.n_live ? min(*a.n_live, a.n) : a.n;
     // blocks wholly past the live count (the launch bound can be a species' capacity while
@@ -270,19 +348,24 @@ __device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block, 
                     B = mk3<UT>(a.Bcell[3 * c], a.Bcell[3 * c + 1], a.Bcell[3 * c + 2]);
                 }
                 U = push_u<UT, P>(U, E, B, a.k, a.kv, a.rotate != 0, a.hc_ref != 0);
-                a.p.ux[i] = U.x; a.p.uy[i] = U.y; a.p.uz[i] = U.z;
             }
             const double Ux = (double)U.x, Uy = (double)U.y, Uz = (double)U.z;
             if (sqrt(Ux * Ux + Uy * Uy + Uz * Uz) >= kC) atomicOr(a.flags, kFlagSuperluminal);
-            // reduced-dimension cases track with the empty components of U removed
-            // (PICParcel.C:118-123); U itself keeps all three
-            const uint32_t m = a.track_mask;
-            t = track(a.tets, t, y1, y2, y3, (m & 1u) ? 0.0 : Ux, (m & 2u) ? 0.0 : Uy, (m & 4u) ? 0.0 : Uz, a.dt,
-                      status);
+            if (a.do_push) { a.p.ux[i] = U.x; a.p.uy[i] = U.y; a.p.uz[i] = U.z; }
+            double sf = 0.0;
+            t = track(a, t, y1, y2, y3, U, sf, status);
             a.p.y1[i] = y1; a.p.y2[i] = y2; a.p.y3[i] = y3;
             a.p.tet[i] = t;
-            if (t >= 0) newcell = a.tets[t].cell;
-            else deleted = (status != 2);
+            if (status == 3) {
+                // on a wall face: counted, placed and deposited by the wall pass
+                const unsigned k = atomicAdd(a.wall_n, 1u);
+                if (k < a.wall_cap) a.wall_q[k] = WallHit{i, sf, set, 0};
+                else atomicOr(a.flags, kFlagTrackBroken);
+            } else if (t >= 0) {
+                newcell = a.tets[t].cell;
+            } else {
+                deleted = (status != 2);
+            }
         }
     }
     // statistics: deleted parcels and guard hits (one atomic per wave)
@@ -313,9 +396,58 @@ __global__ void __launch_bounds__(256) k_move(MoveSet<UT> m) {
     while (s + 1 < m.ns && b >= m.block0[s + 1]) ++s;
     if constexpr (DEP == kVolumeWeighting) {
         __shared__ DepLds dl;
-        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], &dl);
+        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], s, &dl);
     } else {
-        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], nullptr);
+        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], s, nullptr);
+    }
+}
+
+// The wall pass: every queued parcel is reflected on its wall (wall_hit), tracks the rest of
+// its step (reaching further walls, reflected here in turn: the second and later ones
+// without the resync), and is then counted, placed and deposited as the move does for the
+// others.  A grid-stride loop over the queue length read on the device.
+template <class UT, int P, int W, int DEP>
+__global__ void __launch_bounds__(256) k_wall(MoveSet<UT> m) {
+    const MoveArgs<UT>& a0 = m.a[0];
+    const unsigned nq = min(*a0.wall_n, a0.wall_cap);
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < nq; k += gridDim.x * blockDim.x) {
+        const WallHit h = a0.wall_q[k];
+        const MoveArgs<UT>& a = m.a[h.set];
+        const int64_t i = h.i;
+        int t = a.p.tet[i];
+        double y1 = a.p.y1[i], y2 = a.p.y2[i], y3 = a.p.y3[i];
+        vec3<UT> U = mk3<UT>(a.p.ux[i], a.p.uy[i], a.p.uz[i]);
+        const uint32_t id = a.p.id[i];
+        double sf = h.sf;
+        int status = 3, hits = 0, resyncs = 0;
+        while (status == 3) {
+            const TetGeo& g = a.tets[t];
+            wall_hit<UT, P, W>(a, g, y1, y2, y3, sf, -1 - g.nbr[0], U, id, hits++, resyncs);
+            t = track(a, t, y1, y2, y3, U, sf, status, true);
+            if (status == 3 && hits > kMaxTrackSteps) { status = 1; break; }
+        }
+        a.p.ux[i] = U.x; a.p.uy[i] = U.y; a.p.uz[i] = U.z;
+        a.p.y1[i] = y1; a.p.y2[i] = y2; a.p.y3[i] = y3;
+        a.p.tet[i] = t;
+        if (status == 1) atomicAdd(a.stats + 1, 1ull);
+        if (status == 2) atomicOr(a.flags, kFlagTrackBroken);
+        if (t < 0) {
+            if (status != 2) atomicAdd(a.stats + 0, 1ull);
+            if (a.place) a.place[i] = make_int2(-1, -1);
+            continue;
+        }
+        const TetGeo& g = a.tets[t];
+        const int cell = g.cell;
+        const int slot = atomicAdd(a.count + cell, 1);
+        if (a.place) a.place[i] = make_int2(cell, slot);
+        if constexpr (DEP == kVolumeWeighting) {
+            // VolumeWeighting.C:105-123 with the move's fixed-point rounding
+            const double y0 = 1.0 - (y1 + y2 + y3);
+            atomicAdd(a.acc_cell + cell, (unsigned long long)llrint(y0 * a.s42));
+            atomicAdd(a.acc_point + g.vm[0], (unsigned long long)llrint(y1 * a.s42));
+            atomicAdd(a.acc_point + g.vm[1], (unsigned long long)llrint(y2 * a.s42));
+            atomicAdd(a.acc_point + g.vm[2], (unsigned long long)llrint(y3 * a.s42));
+        }
     }
 }
 
@@ -506,6 +638,38 @@ static hipError_t launch_move_t(MoveSet<UT> m, int pusher, int weighting, int de
 #undef PICMI_MOVE_P
 #undef PICMI_MOVE_CASE
     return hipErrorInvalidValue;
+}
+
+template <class UT>
+static hipError_t launch_wall_t(const MoveSet<UT>& m, int pusher, int weighting, int dep, hipStream_t s) {
+    if (m.ns <= 0 || !m.a[0].wall_q) return hipSuccess;
+    // the queue length stays on the device: a fixed grid strides over it
+    const dim3 blk(256), grd(256);
+#define PICMI_WALL_CASE(P, W, D)                                                     \
+    if (pusher == P && weighting == W && dep == D) {                                 \
+        hipLaunchKernelGGL((k_wall<UT, P, W, D>), grd, blk, 0, s, m);                \
+        return hipGetLastError();                                                    \
+    }
+#define PICMI_WALL_P(P)                                                              \
+    PICMI_WALL_CASE(P, kCellAverage, kCellAverage)                                   \
+    PICMI_WALL_CASE(P, kCellAverage, kVolumeWeighting)                               \
+    PICMI_WALL_CASE(P, kVolumeWeighting, kCellAverage)                               \
+    PICMI_WALL_CASE(P, kVolumeWeighting, kVolumeWeighting)
+    PICMI_WALL_P(kPushNone)
+    PICMI_WALL_P(kPushBoris)
+    PICMI_WALL_P(kPushBorisNR)
+    PICMI_WALL_P(kPushVay)
+    PICMI_WALL_P(kPushHC)
+#undef PICMI_WALL_P
+#undef PICMI_WALL_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_wall(const MoveSet<double>& m, int pusher, int weighting, int dep, hipStream_t s) {
+    return launch_wall_t(m, pusher, weighting, dep, s);
+}
+hipError_t launch_wall(const MoveSet<float>& m, int pusher, int weighting, int dep, hipStream_t s) {
+    return launch_wall_t(m, pusher, weighting, dep, s);
 }
 
 template <class UT>

@@ -4,6 +4,15 @@
 
 namespace picmi {
 
+// a parcel standing on a wall face after the move: its index, its species' slot in the
+// move set, and the step fraction done
+struct WallHit {
+    int64_t i;
+    double sf;
+    int32_t set;
+    int32_t pad;
+};
+
 template <class UT>
 struct MoveArgs {
     PartSoA<UT> p;
@@ -17,6 +26,21 @@ struct MoveArgs {
     double dt;
     int do_push, rotate, hc_ref;
     uint32_t track_mask;             // bit d: direction d is empty, its velocity component is not tracked
+    // wall patches (PICParcel.C:153-184, :209-330): kind 1 reflects, 0 deletes (null: no walls)
+    const uint8_t* patch_kind;       // [n_patches]
+    const double* wall_T;            // [n_patches] boundaryT
+    const double* wall_U;            // [n_patches * 3] boundaryU
+    int wall_model;                  // 0 SpecularReflection, 1 MaxwellianThermal, 2 MixedDiffuseSpecular
+    int wall_sync;                   // syncVelocityAtBoundary
+    double diffuse_fraction;
+    double mass;
+    uint64_t seed;
+    int64_t step;
+    uint32_t species;
+    // parcels that reached a wall in the move wait in this queue for the wall pass
+    struct WallHit* wall_q;
+    unsigned* wall_n;
+    unsigned wall_cap;
     int* count;                      // per-cell parcel count of this species
     int2* place;                     // per parcel (new cell, slot in the cell) for the sort, or null
     unsigned long long* acc_cell;    // VolumeWeighting fixed-point accumulators
@@ -63,6 +87,10 @@ struct MoveSet {
 };
 hipError_t launch_move(const MoveSet<double>& m, int pusher, int weighting, int dep, hipStream_t s);
 hipError_t launch_move(const MoveSet<float>& m, int pusher, int weighting, int dep, hipStream_t s);
+// the wall pass after a move with wall patches: reflection, resync and the rest of each
+// queued parcel's track, then its count, place and deposit
+hipError_t launch_wall(const MoveSet<double>& m, int pusher, int weighting, int dep, hipStream_t s);
+hipError_t launch_wall(const MoveSet<float>& m, int pusher, int weighting, int dep, hipStream_t s);
 hipError_t launch_push(const MoveArgs<double>& a, int pusher, int weighting, hipStream_t s);
 hipError_t launch_push(const MoveArgs<float>& a, int pusher, int weighting, hipStream_t s);
 hipError_t launch_deposit(const MoveArgs<double>& a, int dep, hipStream_t s);

@@ -55,7 +55,8 @@ class PicError(RuntimeError):
 
 class pic_patch(C.Structure):
     _fields_ = [("type", C.c_int32), ("start", C.c_int32), ("size", C.c_int32), ("neighbour_patch", C.c_int32),
-                ("phi_bc", C.c_int32), ("phi_value", C.c_double)]
+                ("phi_bc", C.c_int32), ("phi_value", C.c_double), ("wall_temperature", C.c_double),
+                ("wall_velocity", C.c_double * 3)]
 
 
 class pic_mesh(C.Structure):
@@ -83,7 +84,9 @@ class pic_models(C.Structure):
                 ("create_electron", C.c_int32), ("precision", C.c_int32), ("hc_reference_form", C.c_int32),
                 ("seed", C.c_uint64), ("poisson_tolerance", C.c_double), ("poisson_max_iter", C.c_int32),
                 ("poisson_mode", C.c_int32), ("sort_interval", C.c_int32), ("poisson_guess", C.c_int32),
-                ("debye_species", C.c_uint32), ("debye_ignore_drift", C.c_int32)]
+                ("debye_species", C.c_uint32), ("debye_ignore_drift", C.c_int32),
+                ("wall_reflection", C.c_char_p), ("diffuse_fraction", C.c_double),
+                ("no_boundary_sync", C.c_int32)]
 
 
 # the keywords Cloud reads from its models dictionary; any other key is an error (the
@@ -93,7 +96,8 @@ MODEL_KEYS = frozenset((
     "PairingAlgorithm", "WeightCorrectionModel", "ElectronNeutralCollisionModel", "BackgroundGasModel",
     "allowIntraCollision", "calculate_bmin", "DebyeLength", "coulombLog", "electronTypeId", "ionTypeId",
     "uniformBackground", "createIon", "createElectron", "precision", "hcReferenceForm", "seed",
-    "poissonTolerance", "poissonMaxIter", "poissonMode", "sortInterval", "poissonGuess"))
+    "poissonTolerance", "poissonMaxIter", "poissonMode", "sortInterval", "poissonGuess",
+    "WallReflectionModel", "diffuseFraction", "syncVelocityAtBoundary"))
 
 
 class pic_poisson_stats(C.Structure):
@@ -216,6 +220,10 @@ class Mesh:
             pts[i].neighbour_patch = p["neighbour_patch"]
             pts[i].phi_bc = 1 if bc[0] == "fixedValue" else 0
             pts[i].phi_value = float(bc[1])
+            # wall patches: boundaryT and boundaryU of the patch
+            pts[i].wall_temperature = float(p.get("wall_temperature", 0.0))
+            for d, v in enumerate(p.get("wall_velocity", (0.0, 0.0, 0.0))):
+                pts[i].wall_velocity[d] = float(v)
         self._patches = pts
         self.c = pic_mesh(len(self.points), len(self.owner), len(self.neighbour), self.n_cells,
                           len(mesh["patches"]), _ptr(self.points), _ptr(self.fo), _ptr(self.fp), _ptr(self.owner),
@@ -254,7 +262,8 @@ class Cloud:
                                     ("PairingAlgorithm", "pairing_algorithm", "TakizukaAbe"),
                                     ("WeightCorrectionModel", "weight_correction", "none"),
                                     ("ElectronNeutralCollisionModel", "electron_neutral", "none"),
-                                    ("BackgroundGasModel", "background_gas", "none")):
+                                    ("BackgroundGasModel", "background_gas", "none"),
+                                    ("WallReflectionModel", "wall_reflection", "SpecularReflection")):
             v = _enc(models.get(key, default))
             keep.append(v)
             setattr(m, field, v)
@@ -294,6 +303,8 @@ class Cloud:
         m.poisson_mode = {"fp64": POISSON_FP64, "mixed": POISSON_MIXED_IR}[models.get("poissonMode", "fp64")]
         m.sort_interval = int(models.get("sortInterval", 1))
         m.poisson_guess = {"previous": 0, "extrapolate": 1}[models.get("poissonGuess", "previous")]
+        m.diffuse_fraction = float(models.get("diffuseFraction", 0.0))
+        m.no_boundary_sync = 0 if models.get("syncVelocityAtBoundary", True) else 1
         self._models = m
         h = C.c_void_p()
         rc = self.lib.pic_create(C.byref(self.mesh.c), sp, len(species), C.byref(m), C.byref(h))
