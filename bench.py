@@ -234,6 +234,45 @@ def run_vw_field_solve(steps, warmup, ncell=100, ppc=8):
                     "the VolumeWeighting deposit"}
 
 
+def run_walls(steps, warmup, ncell=100, ppc=8):
+    """The field-solve plasma (100^3 hex, 8M electrons, phi = 0 on the x faces) twice: with
+    the x faces open (parcels reaching them are deleted) and as MaxwellianThermal walls at
+    300 K (reflected, tracked on within the step by the wall pass).  ms per step, the move
+    stage (the move kernel, and the wall pass when there are walls), and the wall hits per
+    step from the step statistics."""
+    from picfoam_amd import cases, picmi
+    out = {}
+    for kind in ("open", "walls"):
+        if kind == "open":
+            mesh, species, models, ex = cases.poisson_plasma(ncell=ncell, ppc=ppc)
+        else:
+            mesh, species, models, ex = cases.wall_plasma(ncell=ncell, ppc=ppc)
+        c = picmi.Cloud(mesh, species, models, ex["bc"])
+        c.set_field(picmi.FIELD_SPACE_CHARGE, np.full(ex["n_cells"], ex["space_charge"]))
+        c.init_maxwellian(0, ppc, ex["T_K"], seed=21)
+        dt = ex["dt"]
+        for _ in range(warmup):
+            c.step(dt)
+        n = c.count(0)
+        c.profile(True)
+        hits = deleted = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = c.step(dt)
+            hits += st.n_wall_hits
+            deleted += st.n_deleted
+        el = time.perf_counter() - t0
+        stages = {k: round(v[0] / steps, 4) for k, v in c.stage_times().items() if v[1]}
+        c.profile(False)
+        out[kind] = {"ms_per_step": round(1e3 * el / steps, 3), "stage_ms_per_step": stages, "parcels": n,
+                     "parcels_end": c.count(0), "wall_hits_per_step": hits / steps,
+                     "deleted_per_step": deleted / steps}
+        c.close()
+    out["note"] = ("synchronous pic_step per step; the move stage of the wall run holds the move kernel and the "
+                   "wall pass")
+    return out
+
+
 def poisson_benchmark(ncell, ppc, snaps, train_steps, evals, warm, loss_kind=0, guesses=("previous", "extrapolate")):
     """Poisson ms per step (right-hand side, initial guess, PCG to ||r||/||b|| <= 1e-8, gradient)
     on the field-solve workload, for four initial guesses: the previous phi (the reference's
@@ -382,11 +421,12 @@ def main():
     ap.add_argument("--no-thermalisation", action="store_true")
     ap.add_argument("--no-tets", action="store_true", help="skip the magnetised tet-lattice workload")
     ap.add_argument("--no-poisson", action="store_true", help="skip the Poisson initial-guess comparison")
+    ap.add_argument("--no-walls", action="store_true", help="skip the open-faces / walls comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the mixed headline run alone (for profiling)")
     ap.add_argument("--only-vw-field-solve", action="store_true",
                     help="the VolumeWeighting field-solve workload alone (for profiling the deposit)")
-    ap.add_argument("--only", choices=["thermalisation", "magnetised-tets", "poisson"],
+    ap.add_argument("--only", choices=["thermalisation", "magnetised-tets", "poisson", "walls"],
                     help="one sub-workload alone, printed as its JSON object (for profiling): the thermalisation "
                          "in --precision (default fp64), the magnetised tets, or the Poisson initial-guess comparison")
     ap.add_argument("--precision", default=None, choices=["mixed", "fp64"],
@@ -407,6 +447,7 @@ def main():
         sys.exit(2)
     if args.only_headline:
         args.no_fp64 = args.no_thermalisation = args.no_tets = args.no_poisson = args.no_cpu_baseline = True
+        args.no_walls = True
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
     if args.only:
@@ -414,6 +455,8 @@ def main():
             r = run_thermalisation(args.precision or "fp64", args.steps, args.warmup)
         elif args.only == "magnetised-tets":
             r = run_magnetised_tets(steps=args.steps, warmup=args.warmup)
+        elif args.only == "walls":
+            r = run_walls(args.steps, args.warmup)
         else:
             r = poisson_benchmark(args.poisson_ncell, args.poisson_ppc, snaps=48, train_steps=400, evals=24, warm=4,
                                   loss_kind=args.poisson_loss)
@@ -482,6 +525,10 @@ def main():
     if not args.no_poisson:
         print("bench: field solve with VolumeWeighting, 100^3, 8M electrons", file=sys.stderr, flush=True)
         wl["field_solve_volume_weighting"] = run_vw_field_solve(steps=10, warmup=2)
+    if not args.no_walls:
+        print("bench: field solve with open x faces and with MaxwellianThermal walls, 100^3, 8M electrons",
+              file=sys.stderr, flush=True)
+        wl["walls"] = run_walls(steps=10, warmup=2)
     if wl:
         out["workloads"] = wl
     if not args.no_poisson:

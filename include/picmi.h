@@ -36,7 +36,10 @@ enum {
     PIC_ERR_NO_DEVICE = -10      /* no HIP device */
 };
 
-/* constant/polyMesh/boundary types.  PIC_PATCH_EMPTY marks the front and back of a
+/* constant/polyMesh/boundary types.  A parcel reaching a PIC_PATCH_PATCH face leaves the
+ * domain (deleted); one reaching a PIC_PATCH_WALL face is reflected by
+ * pic_models.wall_reflection and tracks on within the same step (PICParcel.C:153-184).
+ * PIC_PATCH_EMPTY marks the front and back of a
  * reduced-dimension (2-D or 1-D) case: the tracking velocity loses its components normal
  * to them (PICParcel.C:118-123, meshTools::constrainDirection over polyMesh::solutionD), so
  * parcels stay in their plane; no flux crosses them in the Poisson solve. */
@@ -138,7 +141,10 @@ typedef struct {
      * NULL), MaxwellianThermal, MixedDiffuseSpecular with MixedDiffuseSpecularCoeffs.
      * diffuseFraction (PICParcel.C:209-330, WallReflectionModel/); a parcel reaching a wall
      * has its velocity synchronised to the hit time, reflected and synchronised back
-     * (PICParcel.C:153-210) unless SolverSettings.syncVelocityAtBoundary is false (1 here) */
+     * (PICParcel.C:153-210) unless SolverSettings.syncVelocityAtBoundary is false (1 here);
+     * as in the reference only the first reflection of a step is synchronised back
+     * (PICParcel.C:130-139).  The diffuse draws are keyed by (seed; parcel id, step,
+     * species and hit number) */
     const char* wall_reflection;
     double diffuse_fraction;
     int32_t no_boundary_sync;
@@ -158,6 +164,7 @@ typedef struct {
     int64_t coulomb_collisions;
     int64_t mcc_candidates, mcc_elastic, mcc_excitation, mcc_ionization;
     pic_poisson_stats poisson;
+    int64_t n_wall_hits;        /* wall reflections this step (a parcel may reflect more than once) */
 } pic_step_stats;
 
 typedef struct {

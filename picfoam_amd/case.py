@@ -136,7 +136,8 @@ class Case:
             patches.append({"name": self.patch_names[i], "type": {0: "patch", 1: "wall", 2: "cyclic", 3: "empty"}[p.type],
                             "start": p.start, "size": p.size, "neighbour_patch": p.neighbour_patch,
                             "phi_bc": ("fixedValue" if p.phi_bc == picmi.PIC_BC_FIXED_VALUE else "zeroGradient",
-                                       p.phi_value)})
+                                       p.phi_value),
+                            "wall_temperature": p.wall_temperature, "wall_velocity": tuple(p.wall_velocity)})
         return {"points": _view(m.points, C.c_double, m.n_points * 3).reshape(-1, 3),
                 "face_offsets": fo, "face_points": _view(m.face_points, C.c_int32, int(fo[-1])),
                 "owner": _view(m.owner, C.c_int32, nf),

@@ -165,3 +165,18 @@ def poisson_plasma(ncell=100, ppc=8, T_eV=2.0, density=1e16, L=0.01, wp_dt=0.1, 
     extra = {"bc": bc, "space_charge": density * QE, "dt": wp_dt / plasma_frequency(density), "T_K": T_eV * EV_K,
              "ppc": ppc, "n_cells": ncells}
     return mesh, species, models, extra
+
+
+def wall_plasma(ncell=100, ppc=8, T_eV=2.0, density=1e16, L=0.01, wp_dt=0.1, model="MaxwellianThermal",
+                wall_T=300.0, tol=1e-8):
+    """The field-solve plasma of poisson_plasma with the two x faces as walls at phi = 0:
+    electrons reaching them are reflected by the wall model (diffusely at wall_T for
+    MaxwellianThermal) and stay in the box."""
+    mesh, species, models, extra = poisson_plasma(ncell=ncell, ppc=ppc, T_eV=T_eV, density=density, L=L,
+                                                  wp_dt=wp_dt, tol=tol)
+    mesh = meshgen.hex_box(ncell, ncell, ncell, L, L, L, cyclic=(False, True, True), boundary_type="wall")
+    for p in mesh["patches"]:
+        if p["type"] == "wall":
+            p["wall_temperature"] = wall_T
+    models["WallReflectionModel"] = model
+    return mesh, species, models, extra

@@ -1503,6 +1503,7 @@ void read_step_stats(pic_ctx* c, const pic_poisson_stats& ps, pic_step_stats* st
     for (size_t s = 0; s < c->sp.size(); ++s) st->n_parcels[s] = c->sp[s].n;
     st->n_deleted = (int64_t)h[0];
     st->n_track_guard = (int64_t)h[1];
+    st->n_wall_hits = (int64_t)h[kStatsWallHits];
     st->coulomb_collisions = (int64_t)ncoll;
     unsigned long long mc[4] = {0, 0, 0, 0};
     for (int k = 0; k < kNcollSlots; ++k)

@@ -12,6 +12,7 @@
 #include <fstream>
 #include <functional>
 #include <memory>
+#include <regex>
 #include <set>
 #include <sstream>
 #include <string>
@@ -149,6 +150,8 @@ struct pic_case {
     std::string now_name;
     // fields of the start time
     std::vector<double> E, B;  // [n_cells*3], empty = zero
+    // boundaryT and boundaryU as read (rewritten into every time written, as AUTO_WRITE does)
+    std::vector<std::pair<std::string, std::string>> wall_files;
     // parcels
     std::vector<HostParcels> parcels;
     InitPost post;
@@ -479,6 +482,20 @@ void block_mesh(Case& c, const std::string& path) {
 // ---------------------------------------------------------------- fields of a time directory
 
 // internalField of a volVectorField: uniform (x y z) or nonuniform List<vector> N (...)
+// The boundaryField entry of a patch: its own name, else the first key that matches it as a
+// regular expression (OpenFOAM's quoted patch-name patterns)
+const Dict* patch_entry(const Dict& bf, const std::string& name) {
+    if (const Dict* pd = bf.sub(name)) return pd;
+    for (const picmi::foam::Entry& e : bf.entries) {
+        if (!e.dict) continue;
+        try {
+            if (std::regex_match(name, std::regex(e.key))) return e.dict.get();
+        } catch (const std::regex_error&) {
+        }
+    }
+    return nullptr;
+}
+
 std::vector<double> read_vector_field(const std::string& path, int n_cells) {
     auto d = picmi::foam::read_dict_file(path);
     Reader r = d->value("internalField");
@@ -505,7 +522,7 @@ void read_phi_bc(Case& c, const std::string& path) {
     auto d = picmi::foam::read_dict_file(path);
     const Dict& bf = d->sub_required("boundaryField");
     for (size_t p = 0; p < c.patches.size(); ++p) {
-        const Dict* pd = bf.sub(c.patch_names[p]);
+        const Dict* pd = patch_entry(bf, c.patch_names[p]);
         if (!pd) throw FoamError(path + ": boundaryField has no entry for patch " + c.patch_names[p]);
         const std::string t = pd->word("type");
         pic_patch& pp = c.patches[p];
@@ -526,6 +543,41 @@ void read_phi_bc(Case& c, const std::string& path) {
     }
 }
 
+// The wall temperatures and velocities of 0/boundaryT and 0/boundaryU (PICCloud.C:932-955,
+// read at the wall by MaxwellianThermal.C:105-117 and MixedDiffuseSpecular.C:107-119): one
+// uniform value per wall patch (the patch's value entry, else a uniform internalField)
+void read_wall_fields(Case& c, const std::string& t0) {
+    bool any = false;
+    for (const pic_patch& p : c.patches) any = any || p.type == PIC_PATCH_WALL;
+    c.wall_files.clear();
+    if (!any) return;
+    for (int f = 0; f < 2; ++f) {
+        const std::string name = f == 0 ? "boundaryT" : "boundaryU";
+        const std::string path = t0 + "/" + name;
+        if (!picmi::foam::file_exists(path)) throw FoamError(path + ": missing (the mesh has wall patches)");
+        c.wall_files.push_back({name, picmi::foam::read_text(path)});
+        auto d = picmi::foam::read_dict_file(path);
+        const Dict& bf = d->sub_required("boundaryField");
+        for (size_t p = 0; p < c.patches.size(); ++p) {
+            pic_patch& pp = c.patches[p];
+            if (pp.type != PIC_PATCH_WALL) continue;
+            const Dict* pd = patch_entry(bf, c.patch_names[p]);
+            if (!pd) throw FoamError(path + ": boundaryField has no entry for wall patch " + c.patch_names[p]);
+            Reader r = pd->has("value") ? pd->value("value") : d->value("internalField");
+            const std::string kind = r.word();
+            if (kind != "uniform")
+                r.fail("per-face values on wall patch " + c.patch_names[p] + " are not supported; give a uniform value");
+            if (f == 0) {
+                pp.wall_temperature = r.number();
+                if (!(pp.wall_temperature >= 0.0)) r.fail("negative wall temperature");
+            } else {
+                const auto v = r.vector3();
+                for (int k = 0; k < 3; ++k) pp.wall_velocity[k] = v[k];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- picProperties
 
 int species_index(const Case& c, const std::string& name, const std::string& where) {
@@ -536,6 +588,14 @@ int species_index(const Case& c, const std::string& name, const std::string& whe
 
 void require_none(const Dict& d, const std::string& key, const std::string& why) {
     if (d.has(key) && d.word(key) != "none") throw FoamError(d.where + ": " + key + " " + d.word(key) + ": " + why);
+}
+
+// a list entry with elements ("()" and "0()" are empty)
+bool list_has_elements(const picmi::foam::Entry* e) {
+    if (!e || e->dict) return false;
+    size_t k = 0;
+    if (k < e->value.size() && e->value[k].kind == picmi::foam::Token::Number) ++k;
+    return e->value.size() > k + 2;
 }
 
 void read_pic_properties(Case& c, const std::string& path) {
@@ -568,12 +628,23 @@ void read_pic_properties(Case& c, const std::string& path) {
     names.push_back({&m.charge_distribution, ss.word("ChargeDistribution")});
     names.push_back({&m.maxwell_solver, d.word("MaxwellSolver")});
     require_none(d, "ParticleMerging", "particle merging is not implemented");
+    // BoundaryModels (PICCloud.C:493-506): the wall model (WallReflectionModelNew.C:40, its
+    // coefficients in <model>Coeffs, WallReflectionModel.C:52); patch boundary and event
+    // models are not implemented
+    std::string wall = "SpecularReflection";
+    m.diffuse_fraction = 0.0;
     if (const Dict* bm = d.sub("BoundaryModels")) {
         for (const char* k : {"PatchBoundaryModels", "PatchEventModels"})
             if (const Dict* l = bm->sub(k))
-                if (l->has("modelList") && !l->words("modelList").empty())
+                if (list_has_elements(l->find("modelList")))
                     throw FoamError(l->where + ": boundary and patch-event models are not implemented");
+        wall = bm->word_or("WallReflectionModel", wall);
+        if (wall == "MixedDiffuseSpecular")
+            m.diffuse_fraction = bm->sub_required("MixedDiffuseSpecularCoeffs").scalar("diffuseFraction");
     }
+    names.push_back({&m.wall_reflection, wall});
+    // SolverSettings.syncVelocityAtBoundary, true by default (PICCloud.C:926)
+    m.no_boundary_sync = ss.boolean_or("syncVelocityAtBoundary", true) ? 0 : 1;
     const Dict& cm = d.sub_required("CollisionModels");
     require_none(cm, "BinaryCollisionModel", "neutral DSMC collisions are not implemented");
     require_none(cm, "IonNeutralCollisionModel", "ion-neutral collisions are not implemented");
@@ -768,6 +839,7 @@ void open_case(Case& c) {
         if (!picmi::foam::file_exists(t0 + "/phiE")) throw FoamError(t0 + "/phiE: missing (MaxwellSolver needs it)");
         read_phi_bc(c, t0 + "/phiE");
     }
+    read_wall_fields(c, t0);
     if (picmi::foam::file_exists(t0 + "/E")) c.E = read_vector_field(t0 + "/E", c.n_cells);
     if (!c.xs.empty() && picmi::foam::file_exists(t0 + "/picSigmaTcRMax")) {
         auto d = picmi::foam::read_dict_file(t0 + "/picSigmaTcRMax");
@@ -1425,6 +1497,10 @@ void write_time(Case& c, pic_ctx* ctx, const std::string& tn) {
     check(pic_get_cell_field(ctx, PIC_FIELD_RHO, rho.data()), ctx, "rhoCharge");
     check(pic_get_cell_field(ctx, PIC_FIELD_E, E.data()), ctx, "E");
     const std::string td = c.dir + "/" + tn;
+    for (const auto& wf : c.wall_files)
+        write_file(td + "/" + wf.first,
+                   std::regex_replace(wf.second, std::regex("location\\s+\"[^\"]*\";"), "location    \"" + tn + "\";",
+                                      std::regex_constants::format_first_only));
     write_file(td + "/E", vol_field(c, "volVectorField", tn, "E", "[1 1 -3 0 0 -1 0]", E, 3, false));
     write_file(td + "/rhoCharge", vol_field(c, "volScalarField", tn, "rhoCharge", "[0 -3 1 0 0 1 0]", rho, 1, false));
     if (std::string(c.models.maxwell_solver) != "none") {

@@ -405,11 +405,13 @@ __global__ void __launch_bounds__(256) k_move(MoveSet<UT> m) {
 // The wall pass: every queued parcel is reflected on its wall (wall_hit), tracks the rest of
 // its step (reaching further walls, reflected here in turn: the second and later ones
 // without the resync), and is then counted, placed and deposited as the move does for the
-// others.  A grid-stride loop over the queue length read on the device.
+// others.  A grid-stride loop over the queue length read on the device; the statistics
+// (deleted, guard, wall hits) are summed per wave and added once at the end.
 template <class UT, int P, int W, int DEP>
 __global__ void __launch_bounds__(256) k_wall(MoveSet<UT> m) {
     const MoveArgs<UT>& a0 = m.a[0];
     const unsigned nq = min(*a0.wall_n, a0.wall_cap);
+    unsigned long long n_del = 0, n_guard = 0, n_hits = 0;
     for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < nq; k += gridDim.x * blockDim.x) {
         const WallHit h = a0.wall_q[k];
         const MoveArgs<UT>& a = m.a[h.set];
@@ -429,10 +431,11 @@ __global__ void __launch_bounds__(256) k_wall(MoveSet<UT> m) {
         a.p.ux[i] = U.x; a.p.uy[i] = U.y; a.p.uz[i] = U.z;
         a.p.y1[i] = y1; a.p.y2[i] = y2; a.p.y3[i] = y3;
         a.p.tet[i] = t;
-        if (status == 1) atomicAdd(a.stats + 1, 1ull);
+        n_hits += (unsigned long long)hits;
+        n_guard += status == 1 ? 1ull : 0ull;
         if (status == 2) atomicOr(a.flags, kFlagTrackBroken);
         if (t < 0) {
-            if (status != 2) atomicAdd(a.stats + 0, 1ull);
+            n_del += status != 2 ? 1ull : 0ull;
             if (a.place) a.place[i] = make_int2(-1, -1);
             continue;
         }
@@ -448,6 +451,17 @@ __global__ void __launch_bounds__(256) k_wall(MoveSet<UT> m) {
             atomicAdd(a.acc_point + g.vm[1], (unsigned long long)llrint(y2 * a.s42));
             atomicAdd(a.acc_point + g.vm[2], (unsigned long long)llrint(y3 * a.s42));
         }
+    }
+    // every species' MoveArgs carries the same statistics words
+    for (int off = 32; off > 0; off >>= 1) {
+        n_del += __shfl_xor(n_del, off);
+        n_guard += __shfl_xor(n_guard, off);
+        n_hits += __shfl_xor(n_hits, off);
+    }
+    if (__lane_id() == 0) {
+        if (n_del) atomicAdd(a0.stats + 0, n_del);
+        if (n_guard) atomicAdd(a0.stats + 1, n_guard);
+        if (n_hits) atomicAdd(a0.stats + kStatsWallHits, n_hits);
     }
 }
 

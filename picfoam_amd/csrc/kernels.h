@@ -277,6 +277,8 @@ void pair_constants(const double* mass, const double* charge, int n_species, Pai
 constexpr int kNcollSlots = 64, kNcollStride = 16;
 // step counters on the device: 16 words, then the Coulomb collision counters, then the MCC
 // counters (spread over slots for the same reason)
+// statistics words: 0 deleted, 1 tracking guard, 2 wall hits, then the collision counters
+constexpr int kStatsWallHits = 2;
 constexpr int kStatsNcoll = 16, kStatsMcc = kStatsNcoll + kNcollSlots * kNcollStride,
               kStatsWords = kStatsMcc + kNcollSlots * kNcollStride;
 size_t coulomb_lds_bytes(int maxp, size_t cache_elem);

@@ -108,7 +108,8 @@ class pic_poisson_stats(C.Structure):
 class pic_step_stats(C.Structure):
     _fields_ = [("n_parcels", C.c_int64 * 8), ("n_deleted", C.c_int64), ("n_track_guard", C.c_int64),
                 ("coulomb_collisions", C.c_int64), ("mcc_candidates", C.c_int64), ("mcc_elastic", C.c_int64),
-                ("mcc_excitation", C.c_int64), ("mcc_ionization", C.c_int64), ("poisson", pic_poisson_stats)]
+                ("mcc_excitation", C.c_int64), ("mcc_ionization", C.c_int64), ("poisson", pic_poisson_stats),
+                ("n_wall_hits", C.c_int64)]
 
 
 class pic_diag(C.Structure):

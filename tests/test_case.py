@@ -214,6 +214,69 @@ blocks (hex (0 1 2 3 4 5 6 7) (2 2 2) simpleGrading (1 1 1));
     assert needle in str(e.value)
 
 
+def test_wall_models_and_fields():
+    """BoundaryModels.WallReflectionModel with its <model>Coeffs (WallReflectionModelNew.C:40,
+    WallReflectionModel.C:52), SolverSettings.syncVelocityAtBoundary (PICCloud.C:926) and
+    each wall patch's uniform value of 0/boundaryT and 0/boundaryU (PICCloud.C:932-955),
+    patch-name patterns in boundaryField matched as regular expressions."""
+    c = case(os.path.join(CASES, "wall_sheath"))
+    mesh, _, m = c.describe()
+    assert (m.wall_reflection, m.diffuse_fraction, m.no_boundary_sync) == (b"MixedDiffuseSpecular", 0.5, 0)
+    pm = c.mesh_dict()["patches"]
+    assert [p["type"] for p in pm] == ["wall", "wall", "cyclic", "cyclic", "cyclic", "cyclic"]
+    assert [p["wall_temperature"] for p in pm[:2]] == [300.0, 900.0]
+    assert [p["wall_velocity"] for p in pm[:2]] == [(0.0, 0.0, 0.0), (0.0, 2500.0, 0.0)]
+    assert [p["phi_bc"] for p in pm[:2]] == [("fixedValue", 0.0), ("fixedValue", 2.0)]
+    # the thermalisation case states SpecularReflection and no sync setting: the defaults
+    t = case(os.path.join(CASES, "thermalization"))
+    _, _, m = t.describe()
+    assert (m.wall_reflection, m.no_boundary_sync) == (b"SpecularReflection", 0)
+
+
+def _edit(path, old, new):
+    text = path.read_text()
+    assert old in text, old
+    path.write_text(text.replace(old, new))
+
+
+@pytest.mark.parametrize("what", ["no-sync", "nonuniform-T", "missing-U", "no-coeffs", "boundary-model", "bad-model"])
+def test_wall_case_settings_and_refusals(tmp_path, what):
+    from picfoam_amd.case import CaseError
+    d = copy_case("wall_sheath", tmp_path)
+    props = d / "constant" / "picProperties"
+    if what == "no-sync":
+        _edit(props, "syncVelocityAtBoundary  true;", "syncVelocityAtBoundary  false;")
+        c = case(d)
+        assert c.describe()[2].no_boundary_sync == 1
+        return
+    if what == "nonuniform-T":
+        _edit(d / "0" / "boundaryT", "value uniform 900;", "value nonuniform List<scalar> 4(900 900 900 900);")
+        needle = "per-face"
+    elif what == "missing-U":
+        os.remove(d / "0" / "boundaryU")
+        needle = "boundaryU: missing"
+    elif what == "no-coeffs":
+        _edit(props, "diffuseFraction 0.5;", "")
+        needle = "diffuseFraction"
+    elif what == "boundary-model":
+        _edit(props, "PatchBoundaryModels { modelList (); }",
+              "PatchBoundaryModels { modelList ( anode { type FloatingPotential; } ); }")
+        needle = "boundary and patch-event models"
+    else:
+        # an unknown wall model is refused when the device context is made from the case,
+        # with the valid names listed (WallReflectionModelNew.C:47-56), before the GPU is used
+        _edit(props, "WallReflectionModel MixedDiffuseSpecular;", "WallReflectionModel Sticky;")
+        c = case(d)
+        c.initialise(seed=2)
+        with pytest.raises(CaseError) as e:
+            c.create("fp64", capacity_factor=1.0)
+        assert "Unknown WallReflectionModel type Sticky" in str(e.value) and "MaxwellianThermal" in str(e.value)
+        return
+    with pytest.raises(CaseError) as e:
+        case(d)
+    assert needle in str(e.value), str(e.value)
+
+
 def test_dictionary_errors_name_the_file_and_line(tmp_path):
     from picfoam_amd.case import CaseError
     block = """vertices ((0 0 0) (1 0 0) (1 1 0) (0 1 0) (0 0 1) (1 0 1) (1 1 1) (0 1 1));

@@ -169,3 +169,73 @@ def test_thermalization_case_matches_oracle_and_runs(oracle, tmp_path):
     back.read_parcels("5e-13")
     assert [back.parcel_count(s) for s in range(2)] == [32000, 32000]
     cloud.close()
+
+
+def test_wall_case_matches_oracle_and_writes_wall_fields(oracle, tmp_path):
+    """The slab between two walls (MixedDiffuseSpecular 0.5, wall temperatures 300 and
+    900 K, the anode moving along y, phiE 0 and 2 V on the walls): five steps of the picFoam
+    loop equal the oracle's wall-aware move of both species, deposit and Poisson solve from
+    the device's initial state; boundaryT and boundaryU are written into the time
+    directories as read."""
+    from picfoam_amd.case import Case
+    d = copy_case("wall_sheath", tmp_path)
+    fv = d / "system" / "fvSolution"
+    fv.write_text(fv.read_text().replace("tolerance       1e-10;", "tolerance       1e-13;"))
+    say("case wall_sheath: initialise")
+    cs = Case(d)
+    cs.initialise(seed=2)
+    _, _, dt, _ = cs.times
+    _, species, models = cs.describe()
+    cloud = cs.create("fp64", capacity_factor=1.0)
+    m = cs.mesh_dict()
+    rm = oracle.RefMesh(m)
+    P = oracle.RefPoisson(rm, {p["name"]: p["phi_bc"] for p in m["patches"] if p["type"] == "wall"})
+    walls_T = np.array([p["wall_temperature"] for p in m["patches"]])
+    walls_U = np.array([p["wall_velocity"] for p in m["patches"]], np.float64)
+    st = [cloud.get_particles(s) for s in range(2)]
+    ys, ts, Us, ids = [list(v) for v in zip(*st)]
+    q = [s["charge"] for s in species]
+    ms = [s["mass"] for s in species]
+    w = species[0]["nEquivalentParticles"]
+    Q0 = max(abs(qq) * w for qq in q)
+    ac, ap = rm.deposit("VolumeWeighting", [(ys[s], ts[s], q[s] * w) for s in range(2)], Q0)
+    phi = P.solve(rm.deposit_finish("VolumeWeighting", ac, ap, Q0), tol=1e-13)[0]
+    hits = 0
+    say("case wall_sheath: 5 steps")
+    n = cs.run(cloud, until=5 * dt)
+    assert n == 5
+    for step in range(5):
+        E = -P.grad(phi)
+        Ep = rm.cell_to_point(E)
+        for s in range(2):
+            wd = {"model": "MixedDiffuseSpecular", "diffuse_fraction": 0.5, "T": walls_T, "U": walls_U, "sync": True,
+                  "seed": models.seed, "step": step, "species": s, "ids": ids[s]}
+            ys[s], Us[s], ts[s], g, _ = rm.move("Boris", "VolumeWeighting", ys[s], Us[s], ts[s], q[s], ms[s], dt, E, Ep,
+                                                np.zeros_like(E), walls=wd)
+            assert g == 0
+            hits += wd["hits"]
+        ac, ap = rm.deposit("VolumeWeighting", [(ys[s], ts[s], q[s] * w) for s in range(2)], Q0)
+        phi = P.solve(rm.deposit_finish("VolumeWeighting", ac, ap, Q0), phi, tol=1e-13)[0]
+    say(f"oracle: {hits} wall hits in 5 steps")
+    assert hits > 20
+    L = 8 * 6.25e-5
+    for s in range(2):
+        yg, tg, Ug, ig = cloud.get_particles(s)
+        assert cloud.count(s) == len(ids[s])
+        Ug_s, xg_s = sort_by_id(ig, Ug, rm.positions(yg, tg))
+        Ur_s, xr_s = sort_by_id(ids[s], Us[s], rm.positions(ys[s], ts[s]))
+        err_u = rel_err(Ug_s, Ur_s, floor_frac=1e-2)
+        err_x = np.max(np.abs(xg_s - xr_s)) / L
+        say(f"species {s}: U rel err {err_u:.2e}, x err / L {err_x:.2e}")
+        assert err_u < 1e-9 and err_x < 1e-12
+    cloud.close()
+    # the loop to endTime writes boundaryT and boundaryU beside the fields
+    cloud = cs.create("fp64", capacity_factor=1.0)
+    cs.run(cloud)
+    for tn in ("1e-10", "2e-10"):
+        for f in ("boundaryT", "boundaryU", "phiE", "E", "rhoCharge"):
+            assert (d / tn / f).is_file(), (tn, f)
+        assert f'location    "{tn}";' in (d / tn / "boundaryT").read_text()
+    cloud.close()
+    back = Case(d)
+    assert [p["wall_temperature"] for p in back.mesh_dict()["patches"][:2]] == [300.0, 900.0]

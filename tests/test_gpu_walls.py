@@ -207,9 +207,10 @@ def test_wall_step_sort_and_deposit(oracle, weighting):
         c.set_particles(s, y, tet, U, ids)
         init.append((y, tet, U, ids))
     c.step_index = STEP
-    c.step(dt)
+    st = c.step(dt)
     Ep = rm.cell_to_point(E)
     parts = []
+    hits = 0
     for s, (y, tet, U, ids) in enumerate(init):
         yg, tg, Ug, ig = c.get_particles(s)
         assert np.all(np.diff(rm.tets()[0][tg]) >= 0)   # grouped by cell
@@ -218,9 +219,12 @@ def test_wall_step_sort_and_deposit(oracle, weighting):
                                         E, Ep, np.zeros_like(E), walls=w)
         say(f"  species {s}: {w['hits']} wall hits")
         assert guards == 0 and w["hits"] > 100
+        hits += w["hits"]
         yg_s, tg_s, Ug_s, ig_s = sort_by_id(ig, yg, tg, Ug, ig)
         _compare(rm, L, yg_s, tg_s, Ug_s, ig_s, yr, tr, Ur, ids, TOL, TOL)
         parts.append((yg, tg, species[s]["charge"] * species[s]["nEquivalentParticles"]))
+    # the step statistics count every reflection, as the oracle does
+    assert st.n_wall_hits == hits and st.n_deleted == 0 and st.n_track_guard == 0
     Q0 = max(abs(p[2]) for p in parts)
     ac, ap = rm.deposit(weighting, parts, Q0)
     rho_r = rm.deposit_finish(weighting, ac, ap, Q0)
