@@ -59,10 +59,15 @@ enum {
     PIC_FIELD_SPACE_CHARGE = 4, /* [n_cells]   background space charge density */
     PIC_FIELD_E_POINT = 5,      /* [n_points*3] E interpolated to the mesh points (get only) */
     PIC_FIELD_MCC_SIGMA_CR_MAX = 6, /* [n_cells] background (sigma cR)_max (picSigmaTcRMax) */
-    PIC_FIELD_MCC_REMAINDER = 7     /* [n_cells] background candidate remainder */
+    PIC_FIELD_MCC_REMAINDER = 7,    /* [n_cells] background candidate remainder */
+    PIC_FIELD_ION_SIGMA_CR_MAX = 8, /* [n_cells] ion-background (sigma cR)_max (picSigmaTcRMax_IonBackground) */
+    PIC_FIELD_ION_REMAINDER = 9     /* [n_cells] ion-background candidate remainder */
 };
 /* cross-section processes (CrossSectionModel.H:52-59), tabulated "List" model */
 enum { PIC_XS_ELASTIC = 0, PIC_XS_EXCITATION = 1, PIC_XS_IONIZATION = 2 };
+/* ion-neutral processes (IonNeutralCollisionCoeffs.CrossSectionModels: ElasticCrossSection,
+ * ChargeExchangeCrossSection with its threshold) */
+enum { PIC_XS_ION_ELASTIC = 3, PIC_XS_ION_CHARGE_EXCHANGE = 4 };
 /* stage ids for pic_stage_times */
 enum {
     PIC_STAGE_MOVE = 0, PIC_STAGE_SORT = 1, PIC_STAGE_COULOMB = 2, PIC_STAGE_MCC = 3,
@@ -148,6 +153,11 @@ typedef struct {
     const char* wall_reflection;
     double diffuse_fraction;
     int32_t no_boundary_sync;
+    /* CollisionModels.IonNeutralCollisionModel: IsotropicScattering or none (NULL); the ions
+     * (ion_species) collide with the uniformBackground gas (IonNeutralCollisionModel.C:
+     * 447-560), elastically and by charge exchange; collisions with neutral parcels are not
+     * implemented (PIC_ERR_UNSUPPORTED when a neutral species holds parcels) */
+    const char* ion_neutral_collision;
 } pic_models;
 
 typedef struct {
@@ -165,6 +175,7 @@ typedef struct {
     int64_t mcc_candidates, mcc_elastic, mcc_excitation, mcc_ionization;
     pic_poisson_stats poisson;
     int64_t n_wall_hits;        /* wall reflections this step (a parcel may reflect more than once) */
+    int64_t ion_neutral_candidates, ion_neutral_elastic, ion_neutral_charge_exchange;
 } pic_step_stats;
 
 typedef struct {
@@ -199,6 +210,14 @@ int pic_set_cross_section(pic_ctx* ctx, int32_t process, int32_t n, const double
                           const double* sigma_m2, double threshold_ev);
 /* ElectronNeutralCollisionModel::initialize "FindMaximum" (ElectronNeutralCollisionModel.C:375-420) */
 int pic_mcc_initialize(pic_ctx* ctx, double maximum_energy_ev, double energy_step_ev);
+/* IonNeutralCollisionModel::initialize "FindMaximum" for the background (IonNeutralCollisionModel.C:
+ * 282-330): the largest sqrt(2 e E / m_ion) (Qel + Qcx)(E) on E = 0, step, ... below the maximum */
+int pic_ion_mcc_initialize(pic_ctx* ctx, double maximum_energy_ev, double energy_step_ev);
+/* the "Temperature" initialisation of either model (ElectronNeutralCollisionModel.C:432-482,
+ * IonNeutralCollisionModel.C:331-379): Qt(k T / e) times the most probable speed
+ * sqrt(2 k T / m) of the electron (model 0) or ion (model 1) species at temperature [K];
+ * a zero temperature stands for 1 eV as in the reference */
+int pic_mcc_initialize_temperature(pic_ctx* ctx, int32_t model, double temperature);
 
 /* particles: barycentric y1..y3 w.r.t. (cell centre, base, A, B) of tet, velocity U, id */
 int pic_particles_set(pic_ctx* ctx, int32_t species, int64_t n, const double* bary, const int32_t* tet,
@@ -231,6 +250,8 @@ int pic_move(pic_ctx* ctx, double dt);                 /* Cloud::move: push + in
 int pic_sort(pic_ctx* ctx);                            /* buildCellOccupancy */
 int pic_collide_coulomb(pic_ctx* ctx, double dt);      /* CoulombCollisionModel::handleCollisions */
 int pic_collide_mcc(pic_ctx* ctx, double dt);          /* ElectronNeutralCollisionModel::handleCollisions */
+int pic_collide_ion_neutral(pic_ctx* ctx, double dt);  /* IonNeutralCollisionModel::handleCollisions
+                                                          (the background part) */
 int pic_deposit(pic_ctx* ctx);                         /* calculateFields: charge density */
 int pic_poisson_solve(pic_ctx* ctx, pic_poisson_stats* stats); /* MaxwellSolver::solveFields */
 int pic_diag_reduce(pic_ctx* ctx, pic_diag* out);      /* DiagnosticsList::info sums */

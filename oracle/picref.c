@@ -1548,6 +1548,93 @@ int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, d
     return err;
 }
 
+/* ------------------------------------------------------------------ ion-neutral MCC */
+
+#define TAG_ION_MCC 7u
+
+static inline void ion_rand(uint64_t seed, int64_t step, int cell, int k, int j, double* a, double* b) {
+    uint32_t o[4];
+    rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)k, (TAG_ION_MCC << 24) | (uint32_t)j, o);
+    *a = u01(o[0], o[1]);
+    *b = u01(o[2], o[3]);
+}
+
+/* IonIsotropicScattering::updateVelocity with a sampled partner (IonIsotropicScattering.C:
+ * 108-140): the relative velocity turned to a uniformly random direction in the centre of
+ * mass, cos(theta) = 2 u1 - 1, phi = 2 pi u2 */
+void ref_ion_isotropic(double* UP, double* UQ, double mP, double mQ, double u1, double u2) {
+    v3 P = ld3(UP), Q = ld3(UQ);
+    v3 Ucm = vscale(1.0 / (mP + mQ), vadd(vscale(mP, P), vscale(mQ, Q)));
+    double cR = vmag(vsub(P, Q));
+    double ct = 2.0 * u1 - 1.0;
+    double st = sqrt(1.0 - ct * ct);
+    double phi = 2.0 * PI_D * u2;
+    v3 rel = vscale(cR, V(ct, st * cos(phi), st * sin(phi)));
+    st3(UP, vadd(Ucm, vscale(mQ / (mP + mQ), rel)));
+    st3(UQ, vsub(Ucm, vscale(mP / (mP + mQ), rel)));
+}
+
+/* IonNeutralCollisionModel::performBackgroundCollisions (IonNeutralCollisionModel.C:
+ * 447-560): per cell nI n sigma_cR_max dt candidates plus the remainder, each an ion drawn
+ * with replacement (the cell's ions in mcc_order_key order) against a partner sampled
+ * from the background Maxwellian; elastic (isotropic) with probability Qel cR / max, else
+ * charge exchange (the ion takes the partner's velocity) up to Qt cR / max above the
+ * threshold.  sigma_cR_max grows to the largest Qt cR met, while the step's acceptance uses
+ * the value it started with (as the reference's local copy does). */
+int ref_ion_mcc(const ref_mesh* m, const ref_ion_mcc_params* p, double dt, double* UI, const uint32_t* idI,
+                const int* cs, double* scrmax, double* rem, int64_t* counts) {
+    const double c = C_LIGHT;
+    int maxn = 1;
+    for (int cc = 0; cc < m->n_cells; ++cc) if (cs[cc + 1] - cs[cc] > maxn) maxn = cs[cc + 1] - cs[cc];
+    idrec* L = (idrec*)malloc(sizeof(idrec) * maxn);
+    double vth = sqrt(KB * p->TN / p->massN);
+    for (int cell = 0; cell < m->n_cells; ++cell) {
+        int a0 = cs[cell], nI = cs[cell + 1] - a0;
+        double smax = scrmax[cell];
+        double coll = rem[cell] + nI * p->nN * smax * dt;
+        int nc = (int)coll;
+        rem[cell] = coll - nc;
+        if (counts) counts[0] += nc;
+        if (nI == 0 || nc == 0) continue;
+        for (int k = 0; k < nI; ++k) { L[k].key = mcc_order_key(idI[a0 + k]); L[k].idx = a0 + k; }
+        qsort(L, nI, sizeof(idrec), idrec_cmp);
+        for (int k = 0; k < nc; ++k) {
+            double upick, uacc, ua, ub, n1, n2, n3, n4;
+            ion_rand(p->seed, p->step, cell, k, 0, &upick, &uacc);
+            int pick = (int)(upick * nI);
+            if (pick >= nI) pick = nI - 1;
+            int ii = L[pick].idx;
+            ion_rand(p->seed, p->step, cell, k, 1, &ua, &ub);
+            box_muller(ua, ub, &n1, &n2);
+            ion_rand(p->seed, p->step, cell, k, 2, &ua, &ub);
+            box_muller(ua, ub, &n3, &n4);
+            v3 Un = vadd(vscale(vth, V(n1, n2, n3)), V(p->driftN[0], p->driftN[1], p->driftN[2]));
+            v3 Ui = ld3(UI + 3 * ii);
+            double cR = vmag(vsub(Ui, Un));
+            double gamma = 1.0 / sqrt(1.0 - ((cR * cR) / (c * c)));
+            double eV = (gamma - 1.0) * p->massI * c * c / QE;
+            double sigmaT = smax / cR;
+            double Qel = sigma_of(p->n_el, p->e_el, p->s_el, eV);
+            double Qcx = sigma_of(p->n_cx, p->e_cx, p->s_cx, eV);
+            double Qt = Qel + Qcx;
+            if (Qt * cR > scrmax[cell]) scrmax[cell] = Qt * cR;
+            if (uacc <= Qel / sigmaT) {
+                double u1, u2, q[3] = {Un.x, Un.y, Un.z}, P[3] = {Ui.x, Ui.y, Ui.z};
+                ion_rand(p->seed, p->step, cell, k, 3, &u1, &u2);
+                ref_ion_isotropic(P, q, p->massI, p->massN, u1, u2);
+                Ui = ld3(P);
+                if (counts) counts[1]++;
+            } else if (eV >= p->th_cx && uacc <= Qt / sigmaT) {
+                Ui = Un;
+                if (counts) counts[2]++;
+            }
+            st3(UI + 3 * ii, Ui);
+        }
+    }
+    free(L);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ Poisson */
 
 struct ref_poisson {

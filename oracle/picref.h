@@ -147,6 +147,22 @@ int ref_mcc(const ref_mesh* m, const ref_mcc_params* p, double dt, double* yE, d
             uint32_t* idE, const int* cell_start, int* n_e, int cap_e, double* yI, double* UI, int* tetI,
             uint32_t* idI, int* n_i, int cap_i, double* sigma_cr_max, double* remainder, int64_t* counts);
 
+/* ion-neutral null-collision MCC with a uniform background (IsotropicScattering) */
+typedef struct {
+    double massI, massN, nN, TN, driftN[3];
+    int n_el, n_cx;
+    const double *e_el, *s_el, *e_cx, *s_cx;
+    double th_cx;
+    uint64_t seed;
+    int64_t step;
+} ref_ion_mcc_params;
+/* ions sorted by cell (cell_start); sigma_cr_max and remainder updated in place;
+ * counts[0..2] += candidates, elastic, charge exchange */
+int ref_ion_mcc(const ref_mesh* m, const ref_ion_mcc_params* p, double dt, double* UI, const uint32_t* idI,
+                const int* cell_start, double* sigma_cr_max, double* remainder, int64_t* counts);
+/* IonIsotropicScattering::updateVelocity on (UP, UQ) in place */
+void ref_ion_isotropic(double* UP, double* UQ, double mP, double mQ, double u1, double u2);
+
 /* electron-neutral scattering with a sampled partner velocity, in place:
  * NanbuElectronNeutralCollision.C:98-130 (relativistic 0) or
  * RelativisticElectronNeutralCollision.C:161-240 (relativistic 1) */

@@ -76,6 +76,10 @@ def lib():
         _lib.ref_mcc.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int,
                                  vp, vp, vp]
         _lib.ref_mcc.restype = C.c_int
+        _lib.ref_ion_mcc.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp, vp, vp]
+        _lib.ref_ion_mcc.restype = C.c_int
+        _lib.ref_ion_isotropic.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double, C.c_double]
+        _lib.ref_ion_isotropic.restype = None
         _lib.ref_interpolate_xy.argtypes = [C.c_double, C.c_int, vp, vp]
         _lib.ref_interpolate_xy.restype = C.c_double
         _lib.ref_en_scatter.argtypes = [C.c_int, C.c_double, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double]
@@ -349,6 +353,45 @@ def mcc(rm: RefMesh, params: dict, dt, yE, UE, tetE, idE, cs, cap_e, yI, UI, tet
     return {"err": err, "yE": yE2[:n_e.value], "UE": UE2[:n_e.value], "tetE": tE2[:n_e.value], "idE": iE2[:n_e.value],
             "yI": yI2[:n_i.value], "UI": UI2[:n_i.value], "tetI": tI2[:n_i.value], "idI": iI2[:n_i.value],
             "smax": smax, "rem": rem, "counts": counts}
+
+
+class IonMccParams(C.Structure):
+    _fields_ = [("massI", C.c_double), ("massN", C.c_double), ("nN", C.c_double), ("TN", C.c_double),
+                ("driftN", C.c_double * 3), ("n_el", C.c_int), ("n_cx", C.c_int), ("e_el", C.c_void_p),
+                ("s_el", C.c_void_p), ("e_cx", C.c_void_p), ("s_cx", C.c_void_p), ("th_cx", C.c_double),
+                ("seed", C.c_uint64), ("step", C.c_int64)]
+
+
+def ion_mcc(rm: RefMesh, params: dict, dt, UI, idI, cs, smax, rem):
+    """Ion-neutral collisions with the background; params: massI, massN, nN, TN, driftN,
+    xs = ((energies, values) elastic, (energies, values) charge exchange), th_cx, seed, step.
+    Returns (UI, smax, rem, counts[candidates, elastic, charge exchange])."""
+    p = IonMccParams()
+    keep = []
+    for k, (nm_e, nm_s, nm_n) in enumerate((("e_el", "s_el", "n_el"), ("e_cx", "s_cx", "n_cx"))):
+        e, s_ = params["xs"][k]
+        e = np.ascontiguousarray(e, np.float64); s_ = np.ascontiguousarray(s_, np.float64)
+        keep += [e, s_]
+        setattr(p, nm_e, e.ctypes.data); setattr(p, nm_s, s_.ctypes.data); setattr(p, nm_n, len(e))
+    for k in ("massI", "massN", "nN", "TN"):
+        setattr(p, k, float(params[k]))
+    p.th_cx = float(params.get("th_cx", 0.0))
+    for d in range(3):
+        p.driftN[d] = params.get("driftN", (0, 0, 0))[d]
+    p.seed = params["seed"]; p.step = params["step"]
+    U = np.ascontiguousarray(UI, np.float64).copy()
+    ids = np.ascontiguousarray(idI, np.uint32)
+    cs = np.ascontiguousarray(cs, np.int32)
+    smax = np.ascontiguousarray(smax, np.float64).copy(); rem = np.ascontiguousarray(rem, np.float64).copy()
+    counts = np.zeros(3, np.int64)
+    lib().ref_ion_mcc(rm.h, C.byref(p), dt, _p(U), _p(ids), _p(cs), _p(smax), _p(rem), _p(counts))
+    return U, smax, rem, counts
+
+
+def ion_isotropic(UP, UQ, mP, mQ, u1, u2):
+    P = np.array(UP, np.float64); Q = np.array(UQ, np.float64)
+    lib().ref_ion_isotropic(_p(P), _p(Q), float(mP), float(mQ), float(u1), float(u2))
+    return P, Q
 
 
 class RefPoisson:

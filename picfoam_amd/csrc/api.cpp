@@ -65,6 +65,7 @@ struct pic_ctx {
     // models
     int pusher = kPushBoris, fw = kCellAverage, cd = kCellAverage;
     bool electrostatic = false, coulomb = false, mcc = false, bg = false;
+    bool ion_mcc = false;                    // IonNeutralCollisionModel against the background
     int coulomb_model = 0, pairing = 0, weight_corr = 0;
     int en_relativistic = 0;
     int precision = PIC_PREC_FP64;
@@ -125,15 +126,19 @@ struct pic_ctx {
     size_t scan_bytes = 0;
     double* d_diag_part = nullptr;
     double* d_diag_out = nullptr;
-    // MCC
+    // MCC: (sigma cR)max and candidate remainders per cell, electrons and ions; cross-section
+    // tables by PIC_XS_* process (electron elastic, excitation, ionization; ion elastic,
+    // charge exchange)
     double *d_smax = nullptr, *d_rem = nullptr;
-    double* d_xs_e[3] = {nullptr, nullptr, nullptr};
-    double* d_xs_s[3] = {nullptr, nullptr, nullptr};
-    int xs_n[3] = {0, 0, 0};
-    int* d_xs_guide[3] = {nullptr, nullptr, nullptr};
-    double xs_le0[3] = {0, 0, 0}, xs_idl[3] = {0, 0, 0};
-    double xs_th[3] = {0, 0, 0};
-    std::vector<double> xs_he[3], xs_hs[3];
+    double *d_smax_i = nullptr, *d_rem_i = nullptr;
+    static constexpr int kXsProcs = 5;
+    double* d_xs_e[kXsProcs] = {};
+    double* d_xs_s[kXsProcs] = {};
+    int xs_n[kXsProcs] = {};
+    int* d_xs_guide[kXsProcs] = {};
+    double xs_le0[kXsProcs] = {}, xs_idl[kXsProcs] = {};
+    double xs_th[kXsProcs] = {};
+    std::vector<double> xs_he[kXsProcs], xs_hs[kXsProcs];
     // Poisson
     PoissonDev pd{};
     PoissonWork pw{};
@@ -606,6 +611,73 @@ int do_mcc_t(pic_ctx* c, double dt) {
     return check_hip(c, launch_mcc(a, c->stream), "mcc kernel");
 }
 
+// IonNeutralCollisionModel::performBackgroundCollisions (IonNeutralCollisionModel.C:447-560)
+// on the MCC kernels: the ion species in the electrons' place, no parcels created
+template <class UT>
+int do_ion_mcc_t(pic_ctx* c, double dt) {
+    Species& I = c->sp[c->ion];
+    MccArgs<UT> a{};
+    a.n_cells = c->hm.n_cells;
+    a.e = soa<UT>(I, I.cur);
+    a.start = I.start;
+    a.perm = I.indexed ? I.perm : nullptr;
+    a.id_cell = I.indexed ? I.id_cell : I.id[I.cur];
+    a.n_e = live_n(c, c->ion);
+    a.cap_e = I.capacity;
+    a.n_i = a.n_e;
+    a.count_i = I.count;
+    a.smax = c->d_smax_i;
+    a.rem = c->d_rem_i;
+    for (int k = 0; k < 2; ++k) {
+        const int p = PIC_XS_ION_ELASTIC + k;
+        a.xs_e[k] = c->d_xs_e[p]; a.xs_s[k] = c->d_xs_s[p]; a.xs_n[k] = c->xs_n[p];
+        a.xs_guide[k] = c->d_xs_guide[p]; a.xs_le0[k] = c->xs_le0[p]; a.xs_idl[k] = c->xs_idl[p];
+    }
+    a.th_ex = c->xs_th[PIC_XS_ION_CHARGE_EXCHANGE];
+    a.ion_neutral = 1;
+    a.massE = I.mass;
+    a.massN = c->bg_m;
+    a.nN = c->bg_n;
+    a.vthN = std::sqrt(kKB * c->bg_T / c->bg_m);
+    for (int d = 0; d < 3; ++d) a.drift[d] = c->bg_v[d];
+    a.dt = dt;
+    a.seed = c->seed;
+    a.step = c->step;
+    a.maxp = kMccSlots;
+    a.maxp_big = kMccSlotsBig;
+    const double ppc = (double)I.n_sync / std::max(1, a.n_cells);
+    if (ppc <= 150.0) {
+        a.group = 16;
+        a.maxp_group = std::min(256, ((int)(1.5 * ppc) + 40 + 7) & ~7);
+    } else if (ppc <= 300.0) {
+        a.group = 32;
+        a.maxp_group = std::min(512, ((int)(1.5 * ppc) + 40 + 7) & ~7);
+    }
+    a.big_list = c->d_big_list;
+    a.big_n = c->d_big_n;
+    hipMemsetAsync(c->d_big_n, 0, sizeof(int), c->stream);
+    c->big_n_zero = false;
+    a.flags = c->d_flags;
+    a.counts = c->d_stats + kStatsMcc;
+    a.tets = c->d_tets;
+    a.count_e = I.count;
+    return check_hip(c, launch_mcc(a, c->stream), "ion-neutral mcc kernel");
+}
+
+int do_ion_mcc(pic_ctx* c, double dt) {
+    pic_ctx::Scope sc(c, PIC_STAGE_MCC);
+    if (c->xs_n[PIC_XS_ION_ELASTIC] + c->xs_n[PIC_XS_ION_CHARGE_EXCHANGE] == 0)
+        return c->fail(PIC_ERR_INVALID, "no ion-neutral cross sections set");
+    // IonNeutralCollisionModel::performCollisions pairs ions with neutral parcels; only the
+    // background part is built
+    for (size_t s = 0; s < c->sp.size(); ++s)
+        if (c->sp[s].charge == 0.0 && c->sp[s].n > 0)
+            return c->fail(PIC_ERR_UNSUPPORTED, "ion-neutral collisions with neutral parcels are not implemented "
+                                                "(species " + std::to_string(s) + " holds parcels)");
+    if (c->sp[c->ion].n == 0) return PIC_OK;
+    return c->mixed() ? do_ion_mcc_t<float>(c, dt) : do_ion_mcc_t<double>(c, dt);
+}
+
 int do_mcc(pic_ctx* c, double dt) {
     pic_ctx::Scope sc(c, PIC_STAGE_MCC);
     if (c->xs_n[0] + c->xs_n[1] + c->xs_n[2] == 0) return c->fail(PIC_ERR_INVALID, "no cross sections set");
@@ -805,9 +877,19 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         return r;
     if ((r = parse_model(c.get(), models->background_gas, "BackgroundGasModel", {"none", "uniformBackground"}, &bgm)))
         return r;
+    int inm = 0;
+    if ((r = parse_model(c.get(), models->ion_neutral_collision ? models->ion_neutral_collision : "none",
+                         "IonNeutralCollisionModel", {"none", "IsotropicScattering"}, &inm)))
+        return r;
     c->mcc = en != 0;
     c->en_relativistic = en == 2 ? 1 : 0;
     c->bg = bgm == 1;
+    c->ion_mcc = inm != 0;
+    if (c->ion_mcc && !c->bg) {
+        g_last_error = c->err = "IonNeutralCollisionModel: only collisions with the uniformBackground gas are implemented "
+                                "(IonNeutralCollisionModel.C:447-560); neutral parcels are not";
+        return PIC_ERR_UNSUPPORTED;
+    }
     if (c->mcc && !c->bg) {
         g_last_error = c->err = "ElectronNeutralCollisionModel needs the uniformBackground gas model (particle neutrals are not implemented)";
         return PIC_ERR_UNSUPPORTED;
@@ -854,6 +936,12 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->sort_interval = models->sort_interval > 0 ? models->sort_interval : 1;
     c->poisson_guess = models->poisson_guess;
     if (c->poisson_guess < 0 || c->poisson_guess > 1) return c->fail(PIC_ERR_INVALID, "poisson_guess");
+    if (c->ion_mcc) {
+        // IonNeutralCollisionModel.C:170-181: the background neutral needs its ion species
+        if (c->ion < 0 || c->ion >= n_species)
+            return c->fail(PIC_ERR_INVALID, "IonNeutralCollisionModel: no ion type for the background gas (SpeciesRelations ionTypeId)");
+        if (!(c->bg_m > 0)) return c->fail(PIC_ERR_INVALID, "background gas mass");
+    }
     if (c->mcc) {
         if (c->electron < 0 || c->electron >= n_species) return c->fail(PIC_ERR_INVALID, "electronTypeId not set");
         if (c->ion >= n_species) return c->fail(PIC_ERR_INVALID, "ion species out of range");
@@ -1010,6 +1098,19 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
             rem[i] = (double)(x >> 11) * (1.0 / 9007199254740992.0);
         }
         hipMemcpyAsync(c->d_rem, rem.data(), sizeof(double) * nc, hipMemcpyHostToDevice, c->stream);
+    }
+    if (c->ion_mcc) {
+        c->d_smax_i = c->alloc<double>(nc);
+        c->d_rem_i = c->alloc<double>(nc);
+        if (!c->d_smax_i || !c->d_rem_i) return c->fail(PIC_ERR_HIP, "out of device memory (ion-neutral)");
+        // remainders start random in [0,1) (IonNeutralCollisionModel.C:190-194)
+        std::vector<double> rem(nc);
+        for (int i = 0; i < nc; ++i) {
+            uint64_t x = (uint64_t)i * 0xD1B54A32D192ED03ull ^ (c->seed + 0x8CB92BA72F3D8DD7ull);
+            x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
+            rem[i] = (double)(x >> 11) * (1.0 / 9007199254740992.0);
+        }
+        hipMemcpyAsync(c->d_rem_i, rem.data(), sizeof(double) * nc, hipMemcpyHostToDevice, c->stream);
     }
     for (auto& S : c->sp)
         if ((r = alloc_species(c.get(), S))) return r;
@@ -1179,6 +1280,12 @@ int pic_set_cell_field(pic_ctx* c, int32_t field, const double* v) {
             hipMemcpyAsync(field == PIC_FIELD_MCC_REMAINDER ? c->d_rem : c->d_smax, v, sizeof(double) * nc,
                            hipMemcpyHostToDevice, c->stream);
             break;
+        case PIC_FIELD_ION_SIGMA_CR_MAX:
+        case PIC_FIELD_ION_REMAINDER:
+            if (!c->ion_mcc) return c->fail(PIC_ERR_INVALID, "no ion-neutral collision model");
+            hipMemcpyAsync(field == PIC_FIELD_ION_REMAINDER ? c->d_rem_i : c->d_smax_i, v, sizeof(double) * nc,
+                           hipMemcpyHostToDevice, c->stream);
+            break;
         default:
             return c->fail(PIC_ERR_INVALID, "field cannot be set");
     }
@@ -1203,6 +1310,12 @@ int pic_get_cell_field(pic_ctx* c, int32_t field, double* v) {
             src = field == PIC_FIELD_MCC_REMAINDER ? c->d_rem : c->d_smax;
             n = nc;
             break;
+        case PIC_FIELD_ION_SIGMA_CR_MAX:
+        case PIC_FIELD_ION_REMAINDER:
+            if (!c->ion_mcc) return c->fail(PIC_ERR_INVALID, "no ion-neutral collision model");
+            src = field == PIC_FIELD_ION_REMAINDER ? c->d_rem_i : c->d_smax_i;
+            n = nc;
+            break;
         default: return c->fail(PIC_ERR_INVALID, "unknown field");
     }
     hipMemcpyAsync(v, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
@@ -1210,7 +1323,7 @@ int pic_get_cell_field(pic_ctx* c, int32_t field, double* v) {
 }
 
 int pic_set_cross_section(pic_ctx* c, int32_t process, int32_t n, const double* e, const double* s, double th) {
-    if (!c || process < 0 || process > 2 || n < 0 || (n > 0 && (!e || !s))) return PIC_ERR_INVALID;
+    if (!c || process < 0 || process >= pic_ctx::kXsProcs || n < 0 || (n > 0 && (!e || !s))) return PIC_ERR_INVALID;
     for (int i = 1; i < n; ++i)
         if (!(e[i] > e[i - 1])) return c->fail(PIC_ERR_INVALID, "cross-section energies must be strictly ascending");
     c->xs_he[process].assign(e, e + n);
@@ -1228,29 +1341,76 @@ int pic_set_cross_section(pic_ctx* c, int32_t process, int32_t n, const double* 
     return check_hip(c, hipStreamSynchronize(c->stream), "set cross section");
 }
 
+namespace {
+// ListCrossSection interpolateXY on the host tables (clamped at both ends)
+double xs_host(const pic_ctx* c, int k, double x) {
+    const int n = c->xs_n[k];
+    if (n == 0) return 0.0;
+    const double* xs = c->xs_he[k].data();
+    const double* ys = c->xs_hs[k].data();
+    if (x <= xs[0]) return ys[0];
+    if (x >= xs[n - 1]) return ys[n - 1];
+    int lo = (int)(std::upper_bound(xs, xs + n, x) - xs) - 1;
+    if (xs[lo] == x) return ys[lo];
+    return ys[lo] + ((x - xs[lo]) / (xs[lo + 1] - xs[lo])) * (ys[lo + 1] - ys[lo]);
+}
+
+// (sigma cR)max of a collision model's processes [p0, p1) for a parcel of mass m:
+// FindMaximum (the largest sqrt(2 e E / m) Qt(E) on E = 0, step, ... below emax) or
+// Temperature (Qt(kT/e) times the most probable speed sqrt(2 k T / m), 1 eV when T is zero)
+// (ElectronNeutralCollisionModel.C:370-482, IonNeutralCollisionModel.C:256-379,
+// PICCloudI.H:541-548)
+double smax_find(const pic_ctx* c, int p0, int p1, double m, double emax, double estep) {
+    double smax = 0.0;
+    for (double eV = 0.0; eV < emax; eV += estep) {
+        double q = 0.0;
+        for (int k = p0; k < p1; ++k) q += xs_host(c, k, eV);
+        smax = std::max(smax, std::sqrt(2.0 * kQe * eV / m) * q);
+    }
+    return smax;
+}
+double smax_temperature(const pic_ctx* c, int p0, int p1, double m, double T) {
+    if (T < 1e-300) T = 11604.52;
+    const double eV = T * kKB / kQe;
+    double q = 0.0;
+    for (int k = p0; k < p1; ++k) q += xs_host(c, k, eV);
+    return q * std::sqrt(2.0 * kKB * T / m);
+}
+int fill_cells(pic_ctx* c, double* d, double v, const char* what) {
+    std::vector<double> h(c->hm.n_cells, v);
+    hipMemcpyAsync(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream);
+    return check_hip(c, hipStreamSynchronize(c->stream), what);
+}
+}  // namespace
+
 int pic_mcc_initialize(pic_ctx* c, double emax, double estep) {
     if (!c || !c->mcc) return c ? c->fail(PIC_ERR_INVALID, "no electron-neutral model") : PIC_ERR_INVALID;
     if (!(estep > 0)) return c->fail(PIC_ERR_INVALID, "energy step");
-    auto lookup = [&](int k, double x) -> double {
-        const int n = c->xs_n[k];
-        if (n == 0) return 0.0;
-        const double* xs = c->xs_he[k].data();
-        const double* ys = c->xs_hs[k].data();
-        if (x <= xs[0]) return ys[0];
-        if (x >= xs[n - 1]) return ys[n - 1];
-        int lo = (int)(std::upper_bound(xs, xs + n, x) - xs) - 1;
-        if (xs[lo] == x) return ys[lo];
-        return ys[lo] + ((x - xs[lo]) / (xs[lo + 1] - xs[lo])) * (ys[lo + 1] - ys[lo]);
-    };
-    const double mE = c->sp[c->electron].mass;
-    double smax = 0.0;
-    for (double eV = 0.0; eV < emax; eV += estep) {
-        double q = lookup(0, eV) + lookup(1, eV) + lookup(2, eV);
-        smax = std::max(smax, std::sqrt(2.0 * kQe * eV / mE) * q);
+    const double smax = smax_find(c, PIC_XS_ELASTIC, PIC_XS_IONIZATION + 1, c->sp[c->electron].mass, emax, estep);
+    return fill_cells(c, c->d_smax, smax, "mcc initialize");
+}
+
+int pic_ion_mcc_initialize(pic_ctx* c, double emax, double estep) {
+    if (!c || !c->ion_mcc) return c ? c->fail(PIC_ERR_INVALID, "no ion-neutral model") : PIC_ERR_INVALID;
+    if (!(estep > 0)) return c->fail(PIC_ERR_INVALID, "energy step");
+    const double smax = smax_find(c, PIC_XS_ION_ELASTIC, PIC_XS_ION_CHARGE_EXCHANGE + 1, c->sp[c->ion].mass, emax,
+                                  estep);
+    return fill_cells(c, c->d_smax_i, smax, "ion-neutral initialize");
+}
+
+int pic_mcc_initialize_temperature(pic_ctx* c, int32_t model, double temperature) {
+    if (!c || (model != 0 && model != 1)) return PIC_ERR_INVALID;
+    if (!(temperature >= 0.0)) return c->fail(PIC_ERR_INVALID, "temperature");
+    if (model == 0) {
+        if (!c->mcc) return c->fail(PIC_ERR_INVALID, "no electron-neutral model");
+        const double smax = smax_temperature(c, PIC_XS_ELASTIC, PIC_XS_IONIZATION + 1, c->sp[c->electron].mass,
+                                             temperature);
+        return fill_cells(c, c->d_smax, smax, "mcc initialize");
     }
-    std::vector<double> v(c->hm.n_cells, smax);
-    hipMemcpyAsync(c->d_smax, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, c->stream);
-    return check_hip(c, hipStreamSynchronize(c->stream), "mcc initialize");
+    if (!c->ion_mcc) return c->fail(PIC_ERR_INVALID, "no ion-neutral model");
+    const double smax = smax_temperature(c, PIC_XS_ION_ELASTIC, PIC_XS_ION_CHARGE_EXCHANGE + 1, c->sp[c->ion].mass,
+                                         temperature);
+    return fill_cells(c, c->d_smax_i, smax, "ion-neutral initialize");
 }
 
 int pic_particles_set(pic_ctx* c, int32_t s, int64_t n, const double* bary, const int32_t* tet, const double* U,
@@ -1447,6 +1607,13 @@ int pic_collide_coulomb(pic_ctx* c, double dt) {
     return sync_state(c);
 }
 
+int pic_collide_ion_neutral(pic_ctx* c, double dt) {
+    if (!c || !c->ion_mcc) return c ? c->fail(PIC_ERR_INVALID, "IonNeutralCollisionModel is none") : PIC_ERR_INVALID;
+    if (int r = do_sort(c)) return r;
+    if (int r = do_ion_mcc(c, dt)) return r;
+    return sync_state(c);
+}
+
 int pic_collide_mcc(pic_ctx* c, double dt) {
     if (!c || !c->mcc) return c ? c->fail(PIC_ERR_INVALID, "ElectronNeutralCollisionModel is none") : PIC_ERR_INVALID;
     if (int r = do_sort(c)) return r;
@@ -1479,7 +1646,7 @@ int step_body(pic_ctx* c, double dt, pic_poisson_stats* ps) {
     if (int r = do_move(c, dt)) return r;
     // a species moves into cell order every Species::sort_interval steps; collision models
     // need the cell order every step, and in between get it as an index
-    bool due = c->coulomb || c->mcc;
+    bool due = c->coulomb || c->mcc || c->ion_mcc;
     for (const auto& S : c->sp) due = due || (c->step + 1) % S.sort_interval == 0;
     if (due)
         if (int r = do_sort(c, true)) return r;
@@ -1487,6 +1654,8 @@ int step_body(pic_ctx* c, double dt, pic_poisson_stats* ps) {
         if (int r = do_coulomb(c, dt)) return r;
     if (c->mcc)
         if (int r = do_mcc(c, dt)) return r;
+    if (c->ion_mcc)
+        if (int r = do_ion_mcc(c, dt)) return r;
     if (int r = do_deposit_finish(c)) return r;
     if (c->electrostatic)
         if (int r = do_poisson(c, ps)) return r;
@@ -1512,6 +1681,12 @@ void read_step_stats(pic_ctx* c, const pic_poisson_stats& ps, pic_step_stats* st
     st->mcc_elastic = (int64_t)mc[1];
     st->mcc_excitation = (int64_t)mc[2];
     st->mcc_ionization = (int64_t)mc[3];
+    unsigned long long ic[3] = {0, 0, 0};
+    for (int k = 0; k < kNcollSlots; ++k)
+        for (int j = 0; j < 3; ++j) ic[j] += h[kStatsMcc + k * kNcollStride + 4 + j];
+    st->ion_neutral_candidates = (int64_t)ic[0];
+    st->ion_neutral_elastic = (int64_t)ic[1];
+    st->ion_neutral_charge_exchange = (int64_t)ic[2];
     st->poisson = ps;
 }
 

@@ -141,7 +141,8 @@ __device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
     return (double)x * (1.0 / 9007199254740992.0);
 }
 
-constexpr uint32_t kTagShuffle = 1u, kTagCoulomb = 2u, kTagMcc = 3u, kTagNewId = 4u, kTagInit = 5u, kTagWall = 6u;
+constexpr uint32_t kTagShuffle = 1u, kTagCoulomb = 2u, kTagMcc = 3u, kTagNewId = 4u, kTagInit = 5u, kTagWall = 6u,
+                   kTagIonMcc = 7u;
 
 // error flags raised by kernels (bitmask in device memory)
 enum : uint32_t {

@@ -140,10 +140,17 @@ struct pic_case {
         double threshold;
     };
     std::vector<CrossSection> xs;
-    // (sigma cR)_max of the start time (picSigmaTcRMax), or the search range of
-    // CrossSectionInitialization.electronNeutral_searchParameter
-    std::vector<double> smax;
-    double mcc_max_energy = 0.0, mcc_energy_step = 0.0;
+    // (sigma cR)_max of the start time per collision model (0 electron-neutral:
+    // picSigmaTcRMax; 1 ion-neutral: picSigmaTcRMax_IonBackground), or how picInitialise's
+    // CrossSectionInitialization sets it: FindMaximum over [0, emax) in steps of estep (the
+    // default when nothing says otherwise: up to the largest tabulated energy, 1000 eV
+    // without one) or Temperature at T [K]
+    struct XsInit {
+        bool temperature = false;
+        double emax = 0.0, estep = 0.0, T = 0.0;
+    };
+    std::vector<double> smax[2];
+    XsInit xs_init[2];
     // time
     double start = 0.0, end = 0.0, dt = 0.0, write_interval = 0.0, now = 0.0;
     bool write_steps = false;  // writeControl timeStep
@@ -590,12 +597,23 @@ void require_none(const Dict& d, const std::string& key, const std::string& why)
     if (d.has(key) && d.word(key) != "none") throw FoamError(d.where + ": " + key + " " + d.word(key) + ": " + why);
 }
 
-// a list entry with elements ("()" and "0()" are empty)
-bool list_has_elements(const picmi::foam::Entry* e) {
+// a modelList naming a model other than none: "( patch { type <model>; ... } ... )"
+// (BoundaryModelList.C, BoundaryEventModelList.C: an entry of type none adds nothing)
+bool list_has_models(const picmi::foam::Entry* e) {
     if (!e || e->dict) return false;
+    const std::vector<picmi::foam::Token>& t = e->value;
     size_t k = 0;
-    if (k < e->value.size() && e->value[k].kind == picmi::foam::Token::Number) ++k;
-    return e->value.size() > k + 2;
+    if (k < t.size() && t[k].kind == picmi::foam::Token::Number) ++k;
+    if (t.size() <= k + 2) return false;   // "()" and "0()"
+    int depth = 0, entries = 0;
+    for (size_t i = k; i < t.size(); ++i) {
+        if (t[i].is('{')) { ++depth; if (depth == 1) ++entries; continue; }
+        if (t[i].is('}')) { --depth; continue; }
+        if (depth == 1 && t[i].kind == picmi::foam::Token::Word && t[i].text == "type" && i + 1 < t.size() &&
+            t[i + 1].text != "none")
+            return true;
+    }
+    return entries == 0;   // elements that are not dictionaries: not understood, refused
 }
 
 void read_pic_properties(Case& c, const std::string& path) {
@@ -636,7 +654,7 @@ void read_pic_properties(Case& c, const std::string& path) {
     if (const Dict* bm = d.sub("BoundaryModels")) {
         for (const char* k : {"PatchBoundaryModels", "PatchEventModels"})
             if (const Dict* l = bm->sub(k))
-                if (list_has_elements(l->find("modelList")))
+                if (list_has_models(l->find("modelList")))
                     throw FoamError(l->where + ": boundary and patch-event models are not implemented");
         wall = bm->word_or("WallReflectionModel", wall);
         if (wall == "MixedDiffuseSpecular")
@@ -647,7 +665,6 @@ void read_pic_properties(Case& c, const std::string& path) {
     m.no_boundary_sync = ss.boolean_or("syncVelocityAtBoundary", true) ? 0 : 1;
     const Dict& cm = d.sub_required("CollisionModels");
     require_none(cm, "BinaryCollisionModel", "neutral DSMC collisions are not implemented");
-    require_none(cm, "IonNeutralCollisionModel", "ion-neutral collisions are not implemented");
     // SpeciesRelations (PICCloud.C:96-130)
     m.electron_species = -1;
     m.ion_species = -1;
@@ -696,17 +713,18 @@ void read_pic_properties(Case& c, const std::string& path) {
     }
     names.push_back({&m.pairing_algorithm, pairing});
     names.push_back({&m.weight_correction, wcorr});
-    // electron-neutral MCC against the uniform background (ElectronNeutralCollisionModel.C:
-    // 120-230, uniformBackground.C:40-80, the cross-section models)
+    // the uniform background gas (uniformBackground.C:40-80), for electron-neutral
+    // (ElectronNeutralCollisionModel.C:120-230) and ion-neutral (IonNeutralCollisionModel.C:
+    // 93-197) collisions with their cross-section models
     const std::string en = cm.word_or("ElectronNeutralCollisionModel", "none");
+    const std::string in = cm.word_or("IonNeutralCollisionModel", "none");
     const std::string bg = cm.word_or("BackgroundGasModel", "none");
     names.push_back({&m.electron_neutral, en});
+    names.push_back({&m.ion_neutral_collision, in});
     names.push_back({&m.background_gas, bg});
     m.create_ion = m.create_electron = 1;
-    if (en != "none") {
-        if (bg != "uniformBackground")
-            throw FoamError(cm.where + ": electron-neutral collisions with neutral parcels are not implemented; "
-                            "use BackgroundGasModel uniformBackground");
+    int gas = -1;
+    if (bg == "uniformBackground") {
         const Dict& bc = cm.sub_required("BackgroundGasCoeffs");
         m.bg_temperature = bc.scalar("temperature");
         m.bg_number_density = bc.scalar_or("numberDensity", -1.0);
@@ -717,37 +735,73 @@ void read_pic_properties(Case& c, const std::string& path) {
         m.bg_n_equivalent = bc.scalar("nEquivalentParticles");
         const auto v = bc.vector3("velocity");
         for (int k = 0; k < 3; ++k) m.bg_velocity[k] = v[k];
-        const int gas = species_index(c, bc.word("species"), bc.where);
+        gas = species_index(c, bc.word("species"), bc.where);
         m.bg_mass = c.species[gas].mass;
-        const Dict& ec = cm.sub_required("ElectronNeutralCollisionCoeffs");
-        require_none(ec, "WeightCorrection", "weight correction of electron-neutral collisions is not implemented");
-        m.create_ion = ec.boolean_or("createIon", true) ? 1 : 0;
-        m.create_electron = ec.boolean_or("createElectron", true) ? 1 : 0;
-        const Dict& xm = ec.sub_required("CrossSectionModels");
+        // the background's ion: the SpeciesRelations entry naming the gas as its neutral
+        // (PICCloud::ionTypeId)
+        if (const Dict* rel = mp.sub("SpeciesRelations"))
+            for (const picmi::foam::Entry& e : rel->entries)
+                if (e.dict && e.dict->word_or("neutralTypeId", "none") == c.species_names[gas]) {
+                    const std::string ion = e.dict->word_or("ionTypeId", "none");
+                    if (ion != "none") m.ion_species = species_index(c, ion, e.dict->where);
+                    break;
+                }
+    } else if (bg != "none") {
+        throw FoamError(cm.where + ": BackgroundGasModel " + bg + " is not supported (uniformBackground, none)");
+    }
+    // a model's cross sections for the background gas: <Kind>CrossSection <model> with
+    // <Tag><model>Coeffs (CrossSectionModelList.C:48-88; FixedValueCrossSection.C, value and,
+    // but for elastic scattering, threshold; ListCrossSection.C:118-310, energies and values,
+    // the ion charge exchange's list under the keyword "value")
+    auto read_xs = [&](const Dict& coeffs, const char* what, int first, const char* const* kinds,
+                       const char* const* tags, int nproc) {
+        const Dict& xm = coeffs.sub_required("CrossSectionModels");
         const Dict& xg = xm.sub_required(c.species_names[gas]);
-        const char* kinds[3] = {"ElasticCrossSection", "ExcitationCrossSection", "IonizationCrossSection"};
-        const char* tags[3] = {"Elastic", "Excitation", "Ionization"};
-        for (int p = 0; p < 3; ++p) {
+        for (int p = 0; p < nproc; ++p) {
             const std::string model = xg.word_or(kinds[p], "none");
             if (model == "none") continue;
             Case::CrossSection x;
-            x.process = p;
+            x.process = first + p;
+            const bool elastic = std::string(tags[p]) == "Elastic";
             const Dict& co = xg.sub_required(std::string(tags[p]) + model + "Coeffs");
-            x.threshold = p == 0 ? 0.0 : co.scalar_or("threshold", 0.0);
+            x.threshold = elastic ? 0.0 : co.scalar("threshold");
             if (model == "FixedValue") {
                 const double v = co.scalar("value");
                 x.e = {0.0, 1e30};
                 x.s = {v, v};
             } else if (model == "List") {
                 x.e = co.scalars("energies");
-                x.s = co.scalars("crossSections");
+                x.s = co.scalars(first + p == PIC_XS_ION_CHARGE_EXCHANGE ? "value" : "values");
                 if (x.e.size() != x.s.size() || x.e.size() < 2)
-                    throw FoamError(co.where + ": energies and crossSections differ in length");
+                    throw FoamError(co.where + ": energies and values differ in length");
             } else {
-                throw FoamError(xg.where + ": cross-section model " + model + " is not supported (List, FixedValue)");
+                throw FoamError(xg.where + ": " + what + " cross-section model " + model +
+                                " is not supported (List, FixedValue)");
             }
             c.xs.push_back(x);
         }
+    };
+    if (en != "none") {
+        if (gas < 0)
+            throw FoamError(cm.where + ": electron-neutral collisions with neutral parcels are not implemented; "
+                            "use BackgroundGasModel uniformBackground");
+        const Dict& ec = cm.sub_required("ElectronNeutralCollisionCoeffs");
+        require_none(ec, "WeightCorrection", "weight correction of electron-neutral collisions is not implemented");
+        m.create_ion = ec.boolean_or("createIon", true) ? 1 : 0;
+        m.create_electron = ec.boolean_or("createElectron", true) ? 1 : 0;
+        const char* kinds[3] = {"ElasticCrossSection", "ExcitationCrossSection", "IonizationCrossSection"};
+        const char* tags[3] = {"Elastic", "Excitation", "Ionization"};
+        read_xs(ec, "electron-neutral", PIC_XS_ELASTIC, kinds, tags, 3);
+    }
+    if (in != "none") {
+        if (gas < 0)
+            throw FoamError(cm.where + ": ion-neutral collisions with neutral parcels are not implemented; "
+                            "use BackgroundGasModel uniformBackground");
+        const Dict& ic = cm.sub_required("IonNeutralCollisionCoeffs");
+        require_none(ic, "WeightCorrection", "weight correction of ion-neutral collisions (neutral parcels) is not implemented");
+        const char* kinds[2] = {"ElasticCrossSection", "ChargeExchangeCrossSection"};
+        const char* tags[2] = {"Elastic", "ChargeExchange"};
+        read_xs(ic, "ion-neutral", PIC_XS_ION_ELASTIC, kinds, tags, 2);
     }
     (void)neutral;
     m.precision = PIC_PREC_FP64;
@@ -841,26 +895,31 @@ void open_case(Case& c) {
     }
     read_wall_fields(c, t0);
     if (picmi::foam::file_exists(t0 + "/E")) c.E = read_vector_field(t0 + "/E", c.n_cells);
-    if (!c.xs.empty() && picmi::foam::file_exists(t0 + "/picSigmaTcRMax")) {
-        auto d = picmi::foam::read_dict_file(t0 + "/picSigmaTcRMax");
-        Reader r = d->value("internalField");
-        const std::string kind = r.word();
-        if (kind == "uniform") {
-            const double v = r.number();
-            if (v > 0.0) c.smax.assign(c.n_cells, v);
-        } else {
-            r.word();
-            r.list_open();
-            while (!r.accept(')')) c.smax.push_back(r.number());
-            if ((int)c.smax.size() != c.n_cells) r.fail("internalField size differs from the cell count");
+    for (int k = 0; k < 2; ++k) {
+        const bool on = std::string(k == 0 ? c.models.electron_neutral : c.models.ion_neutral_collision) != "none";
+        const std::string f = t0 + (k == 0 ? "/picSigmaTcRMax" : "/picSigmaTcRMax_IonBackground");
+        c.smax[k].clear();
+        if (!on) continue;
+        if (picmi::foam::file_exists(f)) {
+            auto d = picmi::foam::read_dict_file(f);
+            Reader r = d->value("internalField");
+            const std::string kind = r.word();
+            if (kind == "uniform") {
+                const double v = r.number();
+                if (v > 0.0) c.smax[k].assign(c.n_cells, v);
+            } else {
+                r.word();
+                r.list_open();
+                while (!r.accept(')')) c.smax[k].push_back(r.number());
+                if ((int)c.smax[k].size() != c.n_cells) r.fail("internalField size differs from the cell count");
+            }
         }
-    }
-    if (!c.xs.empty()) {
         double emax = 0.0;
         for (const Case::CrossSection& x : c.xs)
-            if (x.e.back() < 1e29) emax = std::max(emax, x.e.back());
-        c.mcc_max_energy = emax > 0.0 ? emax : 1000.0;
-        c.mcc_energy_step = c.mcc_max_energy * 1e-5;
+            if ((x.process >= PIC_XS_ION_ELASTIC) == (k == 1) && x.e.back() < 1e29) emax = std::max(emax, x.e.back());
+        c.xs_init[k] = Case::XsInit{};
+        c.xs_init[k].emax = emax > 0.0 ? emax : 1000.0;
+        c.xs_init[k].estep = c.xs_init[k].emax * 1e-5;
     }
     if (picmi::foam::file_exists(t0 + "/B")) c.B = read_vector_field(t0 + "/B", c.n_cells);
     // host geometry and tets (the same tables pic_create builds)
@@ -1255,11 +1314,49 @@ void initialise(Case& c, const std::string& name, uint64_t seed) {
                         " is not supported (UniformDisplacement, QuasineutralPlasma, Maxwellian, Equipartition, "
                         "ParticleList, NoParticle)");
     }
-    if (const Dict* xi = d.sub("CrossSectionInitialization"))
-        if (const Dict* sp = xi->sub("electronNeutral_searchParameter")) {
-            c.mcc_max_energy = sp->scalar("maximumEnergy");
-            c.mcc_energy_step = sp->scalar("energyStep");
+    // the collision models' (sigma cR)_max (picInitialise.C:204-213; the models' initialize,
+    // ElectronNeutralCollisionModel.C:370-482, IonNeutralCollisionModel.C:256-379) at the
+    // species temperatures: the dictionary's, replaced by the parcels' own for the models that
+    // compute them (Maxwellian, ParticleList, NoParticle; picInitialise.C:129-170)
+    if (d.boolean_or("initalizeCollisionModels", false)) {
+        if (model == "Maxwellian" || model == "ParticleList" || model == "NoParticle")
+            for (size_t s = 0; s < c.species.size(); ++s) {
+                const HostParcels& P = c.parcels[s];
+                const size_t np = P.size();
+                if (np == 0) continue;
+                double m1[3] = {0, 0, 0}, m2 = 0.0;
+                for (size_t i = 0; i < np; ++i) {
+                    for (int k = 0; k < 3; ++k) m1[k] += P.U[3 * i + k];
+                    m2 += P.U[3 * i] * P.U[3 * i] + P.U[3 * i + 1] * P.U[3 * i + 1] + P.U[3 * i + 2] * P.U[3 * i + 2];
+                }
+                for (int k = 0; k < 3; ++k) m1[k] /= (double)np;
+                T[s] = c.species[s].mass / (3.0 * kBoltzmann) *
+                       (m2 / (double)np - (m1[0] * m1[0] + m1[1] * m1[1] + m1[2] * m1[2]));
+            }
+        const char* key[2] = {"electronNeutral", "ionNeutral"};
+        const int sp[2] = {c.models.electron_species, c.models.ion_species};
+        for (int k = 0; k < 2; ++k) {
+            // an inactive model returns before it reads the dictionary
+            const bool on = std::string(k == 0 ? c.models.electron_neutral : c.models.ion_neutral_collision) != "none";
+            if (!on) continue;
+            const Dict& xi = d.sub_required("CrossSectionInitialization");
+            const std::string mode = xi.word(key[k]);
+            Case::XsInit& xs = c.xs_init[k];
+            if (mode == "FindMaximum") {
+                const Dict& sd = xi.sub_required(std::string(key[k]) + "_searchParameter");
+                xs.temperature = false;
+                xs.emax = sd.scalar("maximumEnergy");
+                xs.estep = sd.scalar("energyStep");
+                if (!(xs.estep > 0.0)) sd.fail("energyStep", "must be positive");
+            } else if (mode == "Temperature") {
+                xs.temperature = true;
+                xs.T = sp[k] >= 0 ? T[sp[k]] : 0.0;
+            } else {
+                xi.fail(key[k], "Invalid initialization mode, valid options are: (FindMaximum, Temperature)");
+            }
+            c.smax[k].clear();   // initialised here rather than read from the start time
         }
+    }
     c.post.active = true;
     c.post.solve_fields = d.boolean_or("solveMaxwellEquations", false);
     c.post.leap_frog = d.boolean_or("initalizeLeapFrog", false);
@@ -1696,11 +1793,18 @@ int pic_case_create(pic_case* c, int32_t precision, double factor, pic_ctx** out
                 r = pic_particles_set_positions(ctx, s, n, P.x.data(), P.cell.data(), P.U.data(), ids.data());
             if (r != PIC_OK) return fail(r, "parcels");
         }
-        if (!c->xs.empty()) {
-            if (!c->smax.empty()) r = pic_set_cell_field(ctx, PIC_FIELD_MCC_SIGMA_CR_MAX, c->smax.data());
-            else r = pic_mcc_initialize(ctx, c->mcc_max_energy, c->mcc_energy_step);
-            if (r != PIC_OK) return fail(r, "MCC (sigma cR)max");
+        for (int k = 0; k < 2 && r == PIC_OK; ++k) {
+            if (std::string(k == 0 ? c->models.electron_neutral : c->models.ion_neutral_collision) == "none") continue;
+            const pic_case::XsInit& xs = c->xs_init[k];
+            if (!c->smax[k].empty())
+                r = pic_set_cell_field(ctx, k == 0 ? PIC_FIELD_MCC_SIGMA_CR_MAX : PIC_FIELD_ION_SIGMA_CR_MAX,
+                                       c->smax[k].data());
+            else if (xs.temperature)
+                r = pic_mcc_initialize_temperature(ctx, k, xs.T);
+            else
+                r = k == 0 ? pic_mcc_initialize(ctx, xs.emax, xs.estep) : pic_ion_mcc_initialize(ctx, xs.emax, xs.estep);
         }
+        if (r != PIC_OK) return fail(r, "MCC (sigma cR)max");
         if ((r = pic_deposit(ctx)) != PIC_OK) return fail(r, "charge density");
         if (c->post.active) {
             // picInitialise.C:213-237

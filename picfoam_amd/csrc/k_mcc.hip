@@ -49,8 +49,13 @@ __device__ double xs_lookup(const double* __restrict__ e, const double* __restri
     return s[lo] + ((x - e[lo]) / (e[lo + 1] - e[lo])) * (s[lo + 1] - s[lo]);
 }
 
-__device__ __forceinline__ void mcc_rand(uint64_t seed, int64_t step, int cell, int k, int j, double& a, double& b) {
-    uint4 o = rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)k, (kTagMcc << 24) | (uint32_t)j);
+// the draws of candidate k of a cell, keyed by the collision model's tag (kTagMcc for
+// electrons, kTagIonMcc for ions: mcc_tag<REL>)
+template <int REL>
+constexpr uint32_t mcc_tag() { return REL == 2 ? kTagIonMcc : kTagMcc; }
+__device__ __forceinline__ void mcc_rand(uint32_t tag, uint64_t seed, int64_t step, int cell, int k, int j, double& a,
+                                         double& b) {
+    uint4 o = rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)k, (tag << 24) | (uint32_t)j);
     a = u01(o.x, o.y);
     b = u01(o.z, o.w);
 }
@@ -108,12 +113,13 @@ __device__ void rel_scatter(double eV, vec3<double>& UE, vec3<double>& Un, doubl
     com_leave(f, mE, mN, cc, sc, sp, cp, UE, Un);
 }
 
-// REL: RelativisticElectronNeutral (a template parameter, so each kernel holds the
-// registers of one scattering model only)
+// REL: the scattering model, a template parameter so that each kernel holds the registers
+// of one model only: kMccNanbu, kMccRelativistic (RelativisticElectronNeutral) or kMccIon
+// (ion-neutral IsotropicScattering with charge exchange)
 template <int REL, class UT>
 __device__ __forceinline__ void en_scatter(const MccArgs<UT>& a, double eV, vec3<double>& UE, vec3<double>& Un,
                                            double us, double uc) {
-    if constexpr (REL) rel_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
+    if constexpr (REL == kMccRelativistic) rel_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
     else nanbu_scatter(eV, UE, Un, a.massE, a.massN, us, uc);
 }
 
@@ -141,12 +147,12 @@ struct MccPair {
 
 // the partner of candidate k of a cell, from the background Maxwellian
 // (uniformBackground::sampleVelocity, uniformBackground.C:121-127)
-template <class UT>
+template <int REL, class UT>
 __device__ __forceinline__ vec3<double> bg_partner(const MccArgs<UT>& a, int cell, int k) {
     double ua, ub, n1, n2, n3, n4;
-    mcc_rand(a.seed, a.step, cell, k, 1, ua, ub);
+    mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 1, ua, ub);
     box_muller(ua, ub, n1, n2);
-    mcc_rand(a.seed, a.step, cell, k, 2, ua, ub);
+    mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 2, ua, ub);
     box_muller(ua, ub, n3, n4);
     return mk3<double>(a.vthN * n1 + a.drift[0], a.vthN * n2 + a.drift[1], a.vthN * n3 + a.drift[2]);
 }
@@ -166,14 +172,55 @@ __device__ __forceinline__ double opal_energy(double eV, double th, double uE) {
     return E0 + alpha * tan(uE * (atan(E1 / alpha) + atan(E0 / alpha)) - atan(E0 / alpha));
 }
 
+// one ion-neutral candidate against the background (IonNeutralCollisionModel.C:497-545):
+// elastic with probability Qel cR / max, scattered isotropically in the centre of mass
+// (IonIsotropicScattering.C:108-140: cos(theta) = 2 u1 - 1 along x, phi = 2 pi u2), else up
+// to Qt cR / max above the threshold a charge exchange, the ion taking the partner's
+// velocity.  The ion's slot is a.e (the ion species) and its mass a.massE.
+template <class UT>
+__device__ void ion_candidate(const MccArgs<UT>& a, int cell, int k, int ii, double smax_local, double& qtcr,
+                              unsigned& c_el, unsigned& c_cx) {
+    constexpr int REL = 2;
+    double upick, uacc;
+    mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 0, upick, uacc);
+    const vec3<double> Un = bg_partner<REL>(a, cell, k);
+    vec3<double> Ui = mk3<double>(a.e.ux[ii], a.e.uy[ii], a.e.uz[ii]);
+    const double cR = mag(Ui - Un);
+    const double eV = rel_energy_ev(cR, a.massE);
+    const double sigmaT = smax_local / cR;
+    const double Qel = xs_lookup(a.xs_e[0], a.xs_s[0], a.xs_n[0], a.xs_guide[0], a.xs_le0[0], a.xs_idl[0], eV);
+    const double Qcx = xs_lookup(a.xs_e[1], a.xs_s[1], a.xs_n[1], a.xs_guide[1], a.xs_le0[1], a.xs_idl[1], eV);
+    const double Qt = Qel + Qcx;
+    qtcr = fmax(qtcr, Qt * cR);
+    if (uacc <= Qel / sigmaT) {
+        double u1, u2;
+        mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 3, u1, u2);
+        const double mP = a.massE, mQ = a.massN, ims = 1.0 / (mP + mQ);
+        const vec3<double> cm = ims * (mP * Ui + mQ * Un);
+        const double ct = 2.0 * u1 - 1.0, st = sqrt(1.0 - ct * ct);
+        double sp, cp;
+        sincos_turn(u2, &sp, &cp);
+        Ui = cm + (mQ * ims * cR) * mk3<double>(ct, st * cp, st * sp);
+        ++c_el;
+    } else if (eV >= a.th_ex && uacc <= Qt / sigmaT) {
+        Ui = Un;
+        ++c_cx;
+    }
+    a.e.ux[ii] = (UT)Ui.x; a.e.uy[ii] = (UT)Ui.y; a.e.uz[ii] = (UT)Ui.z;
+}
+
 // one candidate (ElectronNeutralCollisionModel.C:747-800 and the event handlers); an
 // ionization that creates a pair appends it to pairs[*npairs] (LDS)
 template <int REL, class UT>
 __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double smax_local, double& qtcr,
                           unsigned& c_el, unsigned& c_ex, MccPair* pairs, int* npairs) {
+    if constexpr (REL == kMccIon) {
+        ion_candidate(a, cell, k, ei, smax_local, qtcr, c_el, c_ex);
+        return;
+    }
     double upick, uacc;
-    mcc_rand(a.seed, a.step, cell, k, 0, upick, uacc);
-    vec3<double> Un = bg_partner(a, cell, k);
+    mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 0, upick, uacc);
+    vec3<double> Un = bg_partner<REL>(a, cell, k);
     vec3<double> Ue = mk3<double>(a.e.ux[ei], a.e.uy[ei], a.e.uz[ei]);
     const double cR = mag(Ue - Un);
     double eV = rel_energy_ev(cR, a.massE);
@@ -184,7 +231,7 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
     const double Qt = Qel + Qex + Qiz;
     qtcr = fmax(qtcr, Qt * cR);
     double us, uc;
-    mcc_rand(a.seed, a.step, cell, k, 3, us, uc);
+    mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 3, us, uc);
     if (uacc <= Qel / sigmaT) {
         en_scatter<REL>(a, eV, Ue, Un, us, uc);
         ++c_el;
@@ -195,7 +242,7 @@ __device__ void candidate(const MccArgs<UT>& a, int cell, int k, int ei, double 
         ++c_ex;
     } else if (eV >= a.th_iz && uacc <= Qt / sigmaT) {
         double uE, uR;
-        mcc_rand(a.seed, a.step, cell, k, 4, uE, uR);
+        mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 4, uE, uR);
         const double Ep = opal_energy(eV, a.th_iz, uE);
         const vec3<double> preUe = Ue;
         Ue = sqrt(1.0 - (a.th_iz + Ep) / eV) * Ue;
@@ -222,13 +269,13 @@ template <int REL, class UT>
 __device__ void create_pair(const MccArgs<UT>& a, const MccPair& P, unsigned& c_iz) {
     const int cell = P.cell, k = P.k, ei = P.ei;
     const vec3<double> preUe = mk3<double>(P.pu[0], P.pu[1], P.pu[2]);
-    vec3<double> Un = bg_partner(a, cell, k);
+    vec3<double> Un = bg_partner<REL>(a, cell, k);
     const vec3<double> preUn = Un;
     double eV = rel_energy_ev(mag(preUe - Un), a.massE);
     double us, uc, uE, uR, up2, uc2;
-    mcc_rand(a.seed, a.step, cell, k, 3, us, uc);
-    mcc_rand(a.seed, a.step, cell, k, 4, uE, uR);
-    mcc_rand(a.seed, a.step, cell, k, 5, up2, uc2);
+    mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 3, us, uc);
+    mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 4, uE, uR);
+    mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 5, up2, uc2);
     const double Ep = opal_energy(eV, a.th_iz, uE);
     vec3<double> Ue = sqrt(1.0 - (a.th_iz + Ep) / eV) * preUe;
     eV -= (a.th_iz + Ep);
@@ -362,7 +409,7 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, uint64_t* lds, int* pic
             int pick = -1;
             if (k < nc) {
                 double upick, uacc;
-                mcc_rand(a.seed, a.step, cell, k, 0, upick, uacc);
+                mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 0, upick, uacc);
                 pick = (int)(upick * eC);
                 if (pick >= eC) pick = eC - 1;
             }
@@ -381,8 +428,10 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, uint64_t* lds, int* pic
                 }
             }
             __syncthreads();
-            const int np = *npairs;
-            if ((int)threadIdx.x < np) create_pair<REL>(a, pairs[threadIdx.x], c_iz);
+            if constexpr (REL != kMccIon) {
+                const int np = *npairs;
+                if ((int)threadIdx.x < np) create_pair<REL>(a, pairs[threadIdx.x], c_iz);
+            }
             __syncthreads();
         }
     }
@@ -397,7 +446,7 @@ __device__ void mcc_cell(const MccArgs<UT>& a, int cell, uint64_t* lds, int* pic
     // one of kNcollSlots counter groups a cache line apart: one address taking an atomic
     // from every cell would serialise the cells
     if (threadIdx.x == 0) {
-        unsigned long long* cnt = a.counts + kNcollStride * (blockIdx.x % kNcollSlots);
+        unsigned long long* cnt = a.counts + kNcollStride * (blockIdx.x % kNcollSlots) + (REL == kMccIon ? 4 : 0);
         if (nc) atomicAdd(cnt + 0, (unsigned long long)nc);
         if (v1) atomicAdd(cnt + 1, (unsigned long long)v1);
         if (v2) atomicAdd(cnt + 2, (unsigned long long)v2);
@@ -472,7 +521,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
         int pick = -1;
         if (k < ncw) {
             double upick, uacc;
-            mcc_rand(a.seed, a.step, cell, k, 0, upick, uacc);
+            mcc_rand(mcc_tag<REL>(), a.seed, a.step, cell, k, 0, upick, uacc);
             pick = (int)(upick * eC);
             if (pick >= eC) pick = eC - 1;
         }
@@ -491,8 +540,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
             }
         }
         __syncthreads();
-        const int np = npairs;
-        if ((int)threadIdx.x < np) create_pair<REL>(a, pairs[threadIdx.x], c_iz);
+        if constexpr (REL != kMccIon) {
+            const int np = npairs;
+            if ((int)threadIdx.x < np) create_pair<REL>(a, pairs[threadIdx.x], c_iz);
+        }
         __syncthreads();
     }
     for (int off = L / 2; off > 0; off >>= 1) qtcr = fmax(qtcr, __shfl_xor(qtcr, off));
@@ -505,7 +556,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
         v3 += __shfl_xor(v3, off);
     }
     if (threadIdx.x == 0) {
-        unsigned long long* cnt = a.counts + kNcollStride * (blockIdx.x % kNcollSlots);
+        unsigned long long* cnt = a.counts + kNcollStride * (blockIdx.x % kNcollSlots) + (REL == kMccIon ? 4 : 0);
         if (v0) atomicAdd(cnt + 0, (unsigned long long)v0);
         if (v1) atomicAdd(cnt + 1, (unsigned long long)v1);
         if (v2) atomicAdd(cnt + 2, (unsigned long long)v2);
@@ -552,7 +603,8 @@ static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
 }
 template <class UT>
 static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
-    return a.relativistic ? launch_mcc_t<UT, 1>(a, s) : launch_mcc_t<UT, 0>(a, s);
+    if (a.ion_neutral) return launch_mcc_t<UT, kMccIon>(a, s);
+    return a.relativistic ? launch_mcc_t<UT, kMccRelativistic>(a, s) : launch_mcc_t<UT, kMccNanbu>(a, s);
 }
 hipError_t launch_mcc(const MccArgs<double>& a, hipStream_t s) { return launch_mcc_t(a, s); }
 hipError_t launch_mcc(const MccArgs<float>& a, hipStream_t s) { return launch_mcc_t(a, s); }

@@ -331,7 +331,15 @@ struct MccArgs {
     unsigned long long* acc_point;
     double s42_e, s42_i;
     int dep_vw;
+    // the collision model: electron-neutral (draws tagged kTagMcc, counters from word 0 of
+    // each slot) or ion-neutral against the background (ion_neutral 1: a.e, start, perm and
+    // id_cell are the ion species', xs_*[0] elastic and [1] charge exchange with th_ex its
+    // threshold, massE the ion mass; draws tagged kTagIonMcc, counters from word 4:
+    // candidates, elastic, charge exchange)
+    int ion_neutral;
 };
+// scattering models of the MCC kernels
+constexpr int kMccNanbu = 0, kMccRelativistic = 1, kMccIon = 2;
 hipError_t launch_mcc(const MccArgs<double>& a, hipStream_t s);
 hipError_t launch_mcc(const MccArgs<float>& a, hipStream_t s);
 

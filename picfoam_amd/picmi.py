@@ -28,17 +28,18 @@ PATCH_TYPES = {"patch": 0, "wall": 1, "cyclic": 2, "empty": 3}
 PREC_FP64, PREC_MIXED = 0, 1
 PIC_BC_ZERO_GRADIENT, PIC_BC_FIXED_VALUE = 0, 1
 POISSON_FP64, POISSON_MIXED_IR = 0, 1
-FIELD_E, FIELD_B, FIELD_PHI, FIELD_RHO, FIELD_SPACE_CHARGE, FIELD_E_POINT, FIELD_MCC_SMAX, FIELD_MCC_REM = range(8)
-XS_ELASTIC, XS_EXCITATION, XS_IONIZATION = range(3)
+(FIELD_E, FIELD_B, FIELD_PHI, FIELD_RHO, FIELD_SPACE_CHARGE, FIELD_E_POINT, FIELD_MCC_SMAX, FIELD_MCC_REM,
+ FIELD_ION_SMAX, FIELD_ION_REM) = range(10)
+XS_ELASTIC, XS_EXCITATION, XS_IONIZATION, XS_ION_ELASTIC, XS_ION_CHARGE_EXCHANGE = range(5)
 STAGES = ("move", "sort", "coulomb", "mcc", "deposit", "poisson", "field")
 
 # exported entry points (every one declared in include/picmi.h)
 EXPORTS = (
     "pic_version", "pic_device_count", "pic_create", "pic_destroy", "pic_last_error", "pic_mesh_sizes",
     "pic_mesh_tets", "pic_mesh_geometry", "pic_set_cell_field", "pic_get_cell_field", "pic_set_cross_section",
-    "pic_mcc_initialize", "pic_particles_set", "pic_particles_set_positions", "pic_particles_count",
+    "pic_mcc_initialize", "pic_ion_mcc_initialize", "pic_mcc_initialize_temperature", "pic_particles_set", "pic_particles_set_positions", "pic_particles_count",
     "pic_particles_get", "pic_particles_positions", "pic_particles_positions_noalloc", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
-    "pic_collide_coulomb", "pic_collide_mcc", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
+    "pic_collide_coulomb", "pic_collide_mcc", "pic_collide_ion_neutral", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
     "pic_run", "pic_cell_moments",
     "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
     "pic_profile_reset", "pic_gnn_init", "pic_gnn_n_params", "pic_gnn_params", "pic_gnn_record",
@@ -86,7 +87,7 @@ class pic_models(C.Structure):
                 ("poisson_mode", C.c_int32), ("sort_interval", C.c_int32), ("poisson_guess", C.c_int32),
                 ("debye_species", C.c_uint32), ("debye_ignore_drift", C.c_int32),
                 ("wall_reflection", C.c_char_p), ("diffuse_fraction", C.c_double),
-                ("no_boundary_sync", C.c_int32)]
+                ("no_boundary_sync", C.c_int32), ("ion_neutral_collision", C.c_char_p)]
 
 
 # the keywords Cloud reads from its models dictionary; any other key is an error (the
@@ -97,7 +98,7 @@ MODEL_KEYS = frozenset((
     "allowIntraCollision", "calculate_bmin", "DebyeLength", "coulombLog", "electronTypeId", "ionTypeId",
     "uniformBackground", "createIon", "createElectron", "precision", "hcReferenceForm", "seed",
     "poissonTolerance", "poissonMaxIter", "poissonMode", "sortInterval", "poissonGuess",
-    "WallReflectionModel", "diffuseFraction", "syncVelocityAtBoundary"))
+    "WallReflectionModel", "diffuseFraction", "syncVelocityAtBoundary", "IonNeutralCollisionModel"))
 
 
 class pic_poisson_stats(C.Structure):
@@ -109,7 +110,8 @@ class pic_step_stats(C.Structure):
     _fields_ = [("n_parcels", C.c_int64 * 8), ("n_deleted", C.c_int64), ("n_track_guard", C.c_int64),
                 ("coulomb_collisions", C.c_int64), ("mcc_candidates", C.c_int64), ("mcc_elastic", C.c_int64),
                 ("mcc_excitation", C.c_int64), ("mcc_ionization", C.c_int64), ("poisson", pic_poisson_stats),
-                ("n_wall_hits", C.c_int64)]
+                ("n_wall_hits", C.c_int64), ("ion_neutral_candidates", C.c_int64),
+                ("ion_neutral_elastic", C.c_int64), ("ion_neutral_charge_exchange", C.c_int64)]
 
 
 class pic_diag(C.Structure):
@@ -145,6 +147,8 @@ def load_library(path: str = LIB_PATH):
         "pic_get_cell_field": (C.c_int, [vp, i32, vp]),
         "pic_set_cross_section": (C.c_int, [vp, i32, i32, vp, vp, dbl]),
         "pic_mcc_initialize": (C.c_int, [vp, dbl, dbl]),
+        "pic_ion_mcc_initialize": (C.c_int, [vp, dbl, dbl]),
+        "pic_mcc_initialize_temperature": (C.c_int, [vp, i32, dbl]),
         "pic_particles_set": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
         "pic_particles_set_positions": (C.c_int, [vp, i32, i64, vp, vp, vp, vp]),
         "pic_particles_count": (C.c_int, [vp, i32, C.POINTER(i64)]),
@@ -157,6 +161,7 @@ def load_library(path: str = LIB_PATH):
         "pic_sort": (C.c_int, [vp]),
         "pic_collide_coulomb": (C.c_int, [vp, dbl]),
         "pic_collide_mcc": (C.c_int, [vp, dbl]),
+        "pic_collide_ion_neutral": (C.c_int, [vp, dbl]),
         "pic_deposit": (C.c_int, [vp]),
         "pic_poisson_solve": (C.c_int, [vp, C.POINTER(pic_poisson_stats)]),
         "pic_diag_reduce": (C.c_int, [vp, C.POINTER(pic_diag)]),
@@ -264,7 +269,8 @@ class Cloud:
                                     ("WeightCorrectionModel", "weight_correction", "none"),
                                     ("ElectronNeutralCollisionModel", "electron_neutral", "none"),
                                     ("BackgroundGasModel", "background_gas", "none"),
-                                    ("WallReflectionModel", "wall_reflection", "SpecularReflection")):
+                                    ("WallReflectionModel", "wall_reflection", "SpecularReflection"),
+                                    ("IonNeutralCollisionModel", "ion_neutral_collision", "none")):
             v = _enc(models.get(key, default))
             keep.append(v)
             setattr(m, field, v)
@@ -364,7 +370,8 @@ class Cloud:
     def get_field(self, field):
         n = {FIELD_E: 3 * self.n_cells, FIELD_B: 3 * self.n_cells, FIELD_PHI: self.n_cells,
              FIELD_RHO: self.n_cells, FIELD_SPACE_CHARGE: self.n_cells, FIELD_E_POINT: 3 * self.n_points,
-             FIELD_MCC_SMAX: self.n_cells, FIELD_MCC_REM: self.n_cells}[field]
+             FIELD_MCC_SMAX: self.n_cells, FIELD_MCC_REM: self.n_cells, FIELD_ION_SMAX: self.n_cells,
+             FIELD_ION_REM: self.n_cells}[field]
         out = np.empty(n)
         self._check(self.lib.pic_get_cell_field(self.h, field, _ptr(out)))
         return out.reshape(-1, 3) if field in (FIELD_E, FIELD_B, FIELD_E_POINT) else out
@@ -374,7 +381,8 @@ class Cloud:
         self._check(self.lib.pic_set_cross_section(self.h, process, len(e), _ptr(e), _ptr(s), float(threshold)))
 
     def set_cross_section_model(self, process, model, **coeffs):
-        """CrossSectionModel selection per process (0 elastic, 1 excitation, 2 ionization):
+        """CrossSectionModel selection per process (0 elastic, 1 excitation, 2 ionization of
+        the electrons; 3 elastic, 4 charge exchange of the ions):
         "List" (energies, values, threshold; ListCrossSection.C), "FixedValue" (value,
         threshold; FixedValueCrossSection.C: the same sigma at every energy) or "none"."""
         if model == "List":
@@ -388,6 +396,13 @@ class Cloud:
 
     def mcc_initialize(self, maximum_energy, energy_step):
         self._check(self.lib.pic_mcc_initialize(self.h, maximum_energy, energy_step))
+
+    def ion_mcc_initialize(self, maximum_energy, energy_step):
+        self._check(self.lib.pic_ion_mcc_initialize(self.h, maximum_energy, energy_step))
+
+    def mcc_initialize_temperature(self, model, temperature):
+        """model 0 electron-neutral, 1 ion-neutral; temperature [K]"""
+        self._check(self.lib.pic_mcc_initialize_temperature(self.h, int(model), float(temperature)))
 
     # particles
     def set_particles(self, s, bary, tet, U, ids):
@@ -439,6 +454,9 @@ class Cloud:
 
     def collide_mcc(self, dt):
         self._check(self.lib.pic_collide_mcc(self.h, dt))
+
+    def collide_ion_neutral(self, dt):
+        self._check(self.lib.pic_collide_ion_neutral(self.h, dt))
 
     def deposit(self):
         self._check(self.lib.pic_deposit(self.h))

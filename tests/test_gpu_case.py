@@ -239,3 +239,73 @@ def test_wall_case_matches_oracle_and_writes_wall_fields(oracle, tmp_path):
     cloud.close()
     back = Case(d)
     assert [p["wall_temperature"] for p in back.mesh_dict()["patches"][:2]] == [300.0, 900.0]
+
+
+def test_ion_drift_case_matches_oracle_and_drifts(oracle, tmp_path):
+    """Argon ions in argon at E/N = 2000 Td (the reference's ionDriftVelocity settings on a
+    block mesh: 1000 cells along x, empty across, the uniform E of 0/E, IsotropicScattering
+    with equal elastic and charge-exchange cross sections against the uniform background,
+    (sigma cR)_max from the ion temperature): 40 steps of the loop (about 30 collisions at
+    the start's rate) equal the oracle's
+    move, cell order and ion-neutral collisions parcel by parcel; run to endTime, the
+    ions' mean velocity over the second half is the drift the case states, 2158 m/s along
+    -x (held to 3 %; the oracle's own run of the same physics gives 2144)."""
+    from picfoam_amd.case import Case
+    d = copy_case("ion_drift", tmp_path)
+    cs = Case(d)
+    cs.initialise(seed=1)
+    _, _, dt, _ = cs.times
+    _, species, models = cs.describe()
+    cloud = cs.create("fp64", capacity_factor=1.0)
+    m = cs.mesh_dict()
+    rm = oracle.RefMesh(m)
+    kB = 1.380649e-23
+    mI = species[1]["mass"]
+    # (sigma cR)_max by the Temperature mode at the ions' 300 K: Qt(kT/e) sqrt(2 k T / m)
+    smax0 = cloud.get_field(8).copy()
+    assert np.allclose(smax0, 2 * 4.300478403e-19 * np.sqrt(2 * kB * 300.0 / mI), rtol=1e-14)
+    rem = cloud.get_field(9).copy()
+    smax = smax0.copy()
+    y, t, U, ids = cloud.get_particles(1)
+    E = cloud.get_field(0)
+    Z = np.zeros_like(E)
+    fixed = (np.array([0.0, 1e30]), np.array([4.300478403e-19] * 2))
+    P = {"massI": mI, "massN": models.bg_mass, "nN": models.bg_number_density, "TN": 300.0, "driftN": (0, 0, 0),
+         "xs": (fixed, fixed), "th_cx": 0.0, "seed": models.seed}
+    say("case ion_drift: 40 steps against the oracle")
+    collisions = 0
+    for step in range(40):
+        st = cloud.step(dt)
+        y, U, t, g, _ = rm.move("Boris", "VolumeWeighting", y, U, t, species[1]["charge"], mI, dt, E,
+                                rm.cell_to_point(E), Z)
+        assert g == 0
+        order, cell_start = rm.sort_by_cell(t)
+        y, t, U, ids = y[order], t[order], U[order], ids[order]
+        P["step"] = step
+        U, smax, rem, cnt = oracle.ion_mcc(rm, P, dt, U, ids, cell_start, smax, rem)
+        assert (st.ion_neutral_candidates, st.ion_neutral_elastic, st.ion_neutral_charge_exchange) == tuple(cnt)
+        collisions += cnt[1] + cnt[2]
+    yg, tg, Ug, ig = cloud.get_particles(1)
+    (Ug_s, ig_s) = sort_by_id(ig, Ug, ig)
+    (Ur_s, ir_s) = sort_by_id(ids, U, ids)
+    assert np.array_equal(ig_s, ir_s)
+    err = rel_err(Ug_s, Ur_s)
+    say(f"  {collisions} collisions in 40 steps, U rel err {err:.2e}")
+    assert collisions > 10 and err < 1e-11
+    assert np.allclose(cloud.get_field(8), smax, rtol=1e-14)
+    cloud.close()
+    # the whole run, the mean ion velocity sampled every 100 steps
+    cloud = cs.create("fp64", capacity_factor=1.0)
+    say("case ion_drift: 11000 steps")
+    ux = []
+    for k in range(1, 111):
+        cs.run(cloud, until=k * 100 * dt)
+        _, _, Ug, _ = cloud.get_particles(1)
+        ux.append(Ug[:, 0].mean())
+    drift = -float(np.mean(ux[55:]))
+    say(f"  drift {drift:.1f} m/s over the second half, {cloud.count(1)} ions")
+    assert abs(drift - 2158.0) < 0.03 * 2158.0
+    assert cloud.count(1) > 4900
+    for tn in ("1e-07", "1.1e-07"):
+        assert (d / tn / "lagrangian" / "pic" / "U").is_file(), tn
+    cloud.close()
