@@ -61,7 +61,14 @@ enum {
     PIC_FIELD_MCC_SIGMA_CR_MAX = 6, /* [n_cells] background (sigma cR)_max (picSigmaTcRMax) */
     PIC_FIELD_MCC_REMAINDER = 7,    /* [n_cells] background candidate remainder */
     PIC_FIELD_ION_SIGMA_CR_MAX = 8, /* [n_cells] ion-background (sigma cR)_max (picSigmaTcRMax_IonBackground) */
-    PIC_FIELD_ION_REMAINDER = 9     /* [n_cells] ion-background candidate remainder */
+    PIC_FIELD_ION_REMAINDER = 9,    /* [n_cells] ion-background candidate remainder */
+    PIC_FIELD_BINARY_SIGMA_CR_MAX = 10,     /* [n_cells] neutral (sigma_T cR)_max (picSigmaTcRMax) */
+    PIC_FIELD_BINARY_REMAINDER = 11,        /* [n_cells] neutral pair candidate remainder
+                                               (pic:collisionSelectionRemainder) */
+    PIC_FIELD_BINARY_BG_REMAINDER = 12,     /* [n_cells] neutral-background candidate remainder */
+    PIC_FIELD_ION_PARCEL_SIGMA_CR_MAX = 13, /* [n_cells] ion-neutral parcel (sigma cR)_max
+                                               (picSigmaTcRMax_IonNeutral) */
+    PIC_FIELD_ION_PARCEL_REMAINDER = 14     /* [n_cells] ion-neutral parcel candidate remainder */
 };
 /* cross-section processes (CrossSectionModel.H:52-59), tabulated "List" model */
 enum { PIC_XS_ELASTIC = 0, PIC_XS_EXCITATION = 1, PIC_XS_IONIZATION = 2 };
@@ -106,6 +113,10 @@ typedef struct {
     int32_t sort_interval; /* this species' sort_interval (see pic_models); 0 = the models' value.
                               A heavy species that seldom changes cell can move into cell order
                               less often than a light one */
+    double diameter;      /* moleculeProperties.<species>.diameter (m), the hard-sphere size of the
+                             binary collision models (PICParcelI.H:45) */
+    double omega;         /* moleculeProperties.<species>.omega, the viscosity index of the
+                             VariableHardSphere model (PICParcelI.H:46) */
 } pic_species;
 
 /* constant/picProperties keywords; names as the reference's selection tables */
@@ -154,10 +165,28 @@ typedef struct {
     double diffuse_fraction;
     int32_t no_boundary_sync;
     /* CollisionModels.IonNeutralCollisionModel: IsotropicScattering or none (NULL); the ions
-     * (ion_species) collide with the uniformBackground gas (IonNeutralCollisionModel.C:
-     * 447-560), elastically and by charge exchange; collisions with neutral parcels are not
-     * implemented (PIC_ERR_UNSUPPORTED when a neutral species holds parcels) */
+     * collide with the parcels of the neutral species (IonNeutralCollisionModel.C:580-713)
+     * and with the uniformBackground gas (:447-560), elastically and by charge exchange.  As
+     * in the reference (:170-175) one neutral and one ion species at most */
     const char* ion_neutral_collision;
+    /* CollisionModels.BinaryCollisionModel: IsotropicScattering or none (NULL).  Neutral parcels
+     * collide with each other (no time counter, 8 sub-cells; BinaryCollisionModel.C:229-425)
+     * and, with a uniformBackground gas, with the background (:427-525).  The total cross
+     * section is IsotropicScatteringCoeffs.TotalCrossSectionModel: HardSphere (NULL) or
+     * VariableHardSphere with VariableHardSphereCoeffs.Tref (vhs_tref, K); the weight
+     * correction is IsotropicScatteringCoeffs.WeightCorrection: none (NULL) or Nanbu */
+    const char* binary_collision;
+    const char* total_cross_section;
+    double vhs_tref;
+    const char* binary_weight_correction;
+    /* IonNeutralCollisionCoeffs.WeightCorrection (none/NULL or Nanbu) and updateNeutralVelocity
+     * false (1 here) for the ion collisions with neutral parcels (IonNeutralCollisionModel.C:
+     * 401-449) */
+    const char* ion_neutral_weight_correction;
+    int32_t ion_neutral_keep_neutral;
+    /* BackgroundGasCoeffs.species: the background gas' species, whose diameter and omega the
+     * binary model's background collisions use (BinaryCollisionModel.C:486-492) */
+    int32_t bg_species;
 } pic_models;
 
 typedef struct {
@@ -176,6 +205,10 @@ typedef struct {
     pic_poisson_stats poisson;
     int64_t n_wall_hits;        /* wall reflections this step (a parcel may reflect more than once) */
     int64_t ion_neutral_candidates, ion_neutral_elastic, ion_neutral_charge_exchange;
+    /* ion collisions with neutral parcels (IonNeutralCollisionModel.C:580-713) */
+    int64_t ion_parcel_candidates, ion_parcel_elastic, ion_parcel_charge_exchange;
+    /* BinaryCollisionModel: neutral pairs, then the neutrals with the background gas */
+    int64_t binary_candidates, binary_collisions, binary_bg_candidates, binary_bg_collisions;
 } pic_step_stats;
 
 typedef struct {
@@ -218,6 +251,12 @@ int pic_ion_mcc_initialize(pic_ctx* ctx, double maximum_energy_ev, double energy
  * sqrt(2 k T / m) of the electron (model 0) or ion (model 1) species at temperature [K];
  * a zero temperature stands for 1 eV as in the reference */
 int pic_mcc_initialize_temperature(pic_ctx* ctx, int32_t model, double temperature);
+/* BinaryCollisionModel::initialize (BinaryCollisionModel.C:534-600): every cell's (sigma_T
+ * cR)_max set to pi d^2 sqrt(2 k T / m) of the neutral species with the largest number
+ * density (the background's density added to its species), at that species' temperature
+ * (the background's when larger; 293.15 K for a zero temperature).  temperatures [K] and
+ * number_densities [m^-3] are per species, [n_species] */
+int pic_binary_initialize(pic_ctx* ctx, const double* temperatures, const double* number_densities);
 
 /* particles: barycentric y1..y3 w.r.t. (cell centre, base, A, B) of tet, velocity U, id */
 int pic_particles_set(pic_ctx* ctx, int32_t species, int64_t n, const double* bary, const int32_t* tet,
@@ -250,8 +289,10 @@ int pic_move(pic_ctx* ctx, double dt);                 /* Cloud::move: push + in
 int pic_sort(pic_ctx* ctx);                            /* buildCellOccupancy */
 int pic_collide_coulomb(pic_ctx* ctx, double dt);      /* CoulombCollisionModel::handleCollisions */
 int pic_collide_mcc(pic_ctx* ctx, double dt);          /* ElectronNeutralCollisionModel::handleCollisions */
-int pic_collide_ion_neutral(pic_ctx* ctx, double dt);  /* IonNeutralCollisionModel::handleCollisions
-                                                          (the background part) */
+int pic_collide_ion_neutral(pic_ctx* ctx, double dt);  /* IonNeutralCollisionModel::handleCollisions:
+                                                          the ions with the neutral parcels, then with
+                                                          the background gas */
+int pic_collide_binary(pic_ctx* ctx, double dt);       /* BinaryCollisionModel::handleCollisions */
 int pic_deposit(pic_ctx* ctx);                         /* calculateFields: charge density */
 int pic_poisson_solve(pic_ctx* ctx, pic_poisson_stats* stats); /* MaxwellSolver::solveFields */
 int pic_diag_reduce(pic_ctx* ctx, pic_diag* out);      /* DiagnosticsList::info sums */

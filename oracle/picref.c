@@ -1635,6 +1635,288 @@ int ref_ion_mcc(const ref_mesh* m, const ref_ion_mcc_params* p, double dt, doubl
     return 0;
 }
 
+/* ------------------------------------------------------------------ parcel pairs */
+
+#define TAG_ION_PAIR 8u
+#define TAG_DSMC 9u
+#define TAG_DSMC_BG 10u
+#define VSMALL 1e-300
+
+static inline void pair_rand(uint32_t tag, uint64_t seed, int64_t step, int cell, int k, int j, double* a, double* b) {
+    uint32_t o[4];
+    rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)k, (tag << 24) | (uint32_t)j, o);
+    *a = u01(o[0], o[1]);
+    *b = u01(o[2], o[3]);
+}
+
+/* the order of a cell's parcels in the pair passes: species after species (list order),
+ * each by the top 49 bits of the splitmix64 finaliser of its id, then by its position in
+ * the cell's list */
+typedef struct { uint64_t key; int s, idx; } prec;
+static int prec_cmp(const void* a, const void* b) {
+    const prec* x = (const prec*)a; const prec* y = (const prec*)b;
+    return x->key < y->key ? -1 : (x->key > y->key);
+}
+static uint64_t pair_key(int s, uint32_t id, int k) {
+    uint64_t z = (uint64_t)id + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((uint64_t)s << 61) | ((z >> 15) << 12) | (uint64_t)k;
+}
+/* the cell's parcels of species [0, ns) in that order; returns their number */
+static int cell_list(const ref_pair_params* p, int cell, prec* L, int* nA, double* wA, double* wB) {
+    int n = 0;
+    *nA = 0; *wA = 0.0; *wB = 0.0;
+    for (int s = 0; s < p->ns; ++s) {
+        int a0 = p->cell_start[s][cell], c = p->cell_start[s][cell + 1] - a0;
+        for (int k = 0; k < c; ++k) {
+            L[n].key = pair_key(s, p->id[s][a0 + k], k);
+            L[n].s = s;
+            L[n].idx = a0 + k;
+            ++n;
+        }
+        if (s < p->na) {
+            *nA += c;
+            if (c > 0 && p->w[s] > *wA) *wA = p->w[s];
+        } else if (c > 0 && p->w[s] > *wB) {
+            *wB = p->w[s];
+        }
+    }
+    qsort(L, n, sizeof(prec), prec_cmp);
+    return n;
+}
+
+/* HardSphere::sigmaTcR (HardSphere.C:50-82) and VariableHardSphere::sigmaTcR
+ * (VariableHardSphere.C:51-100): d_PQ and omega_PQ the means of the two species' */
+double ref_sigma_tcr(int vhs, double tref, double dP, double dQ, double omP, double omQ, double mP, double mQ,
+                     const double* UP, const double* UQ) {
+    double dPQ = 0.5 * (dP + dQ);
+    double cR = vmag(vsub(ld3(UP), ld3(UQ)));
+    if (cR < VSMALL) return 0.0;
+    if (!vhs) {
+        double sigmaTPQ = PI_D * dPQ * dPQ;
+        return sigmaTPQ * cR;
+    }
+    double omegaPQ = 0.5 * (omP + omQ);
+    double mR = mP * mQ / (mP + mQ);
+    double sigmaTPQ = PI_D * dPQ * dPQ * pow(2.0 * KB * tref / (mR * cR * cR), omegaPQ - 0.5) / exp(lgamma(2.5 - omegaPQ));
+    return sigmaTPQ * cR;
+}
+
+/* NanbuWeightCorrectionModel::correctVelocity (NanbuCorrection.C:50-69) */
+static void nanbu_weight_correct(double uw, double wP, double wQ, double* UP, double* UQ, const double* preP,
+                                 const double* preQ) {
+    if (uw > wP / wQ) { UQ[0] = preQ[0]; UQ[1] = preQ[1]; UQ[2] = preQ[2]; }
+    if (uw > wQ / wP) { UP[0] = preP[0]; UP[1] = preP[1]; UP[2] = preP[2]; }
+}
+
+/* BinaryCollisionModel::performCollisions (BinaryCollisionModel.C:229-425) and, with a
+ * background, performBackgroundCollisions (:427-525), over the na neutral species.  Per cell:
+ * the no-time-counter candidates rem + 1/2 n (n - 1) w_max (sigma_T cR)_max dt / V; the first
+ * parcel drawn from the cell, the second from the first one's octant about the cell centre
+ * when that holds another parcel (else from the cell), redrawn while it is the first (draw
+ * 16 + r; after 64 equal draws the next parcel of the list); accepted when sigma_T cR /
+ * (sigma_T cR)_max exceeds a uniform; IsotropicScattering (IsotropicScattering.C:61-105) and
+ * the weight correction.  (sigma_T cR)_max grows to the largest value met while the step
+ * accepts against the value it started with.  The background pass draws n n_g max dt
+ * candidates, with the maximum the pair pass left, against Maxwellian partners.
+ * counts[0..3] += pair candidates, pair collisions, background candidates, collisions. */
+int ref_binary(const ref_mesh* m, const ref_pair_params* p, double dt, double* scrmax, double* rem, double* rem_bg,
+               int64_t* counts) {
+    int maxn = 2;
+    for (int cell = 0; cell < m->n_cells; ++cell) {
+        int n = 0;
+        for (int s = 0; s < p->ns; ++s) n += p->cell_start[s][cell + 1] - p->cell_start[s][cell];
+        if (n > maxn) maxn = n;
+    }
+    prec* L = (prec*)malloc(sizeof(prec) * maxn);
+    int* sub = (int*)malloc(sizeof(int) * maxn);
+    int* sl = (int*)malloc(sizeof(int) * maxn);
+    double vth = p->bg ? sqrt(KB * p->TN / p->massN) : 0.0;
+    for (int cell = 0; cell < m->n_cells; ++cell) {
+        int nA;
+        double wA, wB;
+        int n = cell_list(p, cell, L, &nA, &wA, &wB);
+        if (n > 1) {
+            double smax0 = scrmax[cell];
+            double sel = rem[cell] + 0.5 * n * (n - 1) * wA * smax0 * dt / m->V[cell];
+            int nc = (int)sel;
+            rem[cell] = sel - nc;
+            if (counts) counts[0] += nc;
+            if (nc > 0) {
+                /* sub-cells: the octant of the position about the cell centre */
+                v3 cC = ld3(m->C + 3 * cell);
+                for (int r = 0; r < n; ++r) {
+                    int s = L[r].s, i = L[r].idx;
+                    v3 X[4]; int vid[3];
+                    tet_vertices(m, p->tet[s][i], X, vid);
+                    double y1 = p->y[s][3 * i], y2 = p->y[s][3 * i + 1], y3 = p->y[s][3 * i + 2];
+                    double y0 = 1.0 - (y1 + y2 + y3);
+                    v3 pos = vadd(vadd(vscale(y0, X[0]), vscale(y1, X[1])), vadd(vscale(y2, X[2]), vscale(y3, X[3])));
+                    v3 rel = vsub(pos, cC);
+                    sub[r] = (rel.x >= 0.0) + 2 * (rel.y >= 0.0) + 4 * (rel.z >= 0.0);
+                }
+                int sst[9], at = 0;
+                for (int sc = 0; sc < 8; ++sc) {
+                    sst[sc] = at;
+                    for (int r = 0; r < n; ++r) if (sub[r] == sc) sl[at++] = r;
+                }
+                sst[8] = at;
+                for (int k = 0; k < nc; ++k) {
+                    double upick, uacc;
+                    pair_rand(TAG_DSMC, p->seed, p->step, cell, k, 0, &upick, &uacc);
+                    int P = (int)(upick * n);
+                    if (P >= n) P = n - 1;
+                    int sc = sub[P], s0 = sst[sc], nsc = sst[sc + 1] - s0;
+                    int mm = nsc > 1 ? nsc : n;
+                    int Q = -1;
+                    for (int r = 0; r < 64 && Q < 0; ++r) {
+                        double v0, v1;
+                        pair_rand(TAG_DSMC, p->seed, p->step, cell, k, 16 + r, &v0, &v1);
+                        int j = (int)(v0 * mm);
+                        if (j >= mm) j = mm - 1;
+                        int q = nsc > 1 ? sl[s0 + j] : j;
+                        if (q != P) Q = q;
+                    }
+                    if (Q < 0) {
+                        if (nsc > 1) {
+                            int j = 0;
+                            while (sl[s0 + j] != P) ++j;
+                            Q = sl[s0 + (j + 1) % nsc];
+                        } else {
+                            Q = (P + 1) % n;
+                        }
+                    }
+                    int sP = L[P].s, sQ = L[Q].s;
+                    double* UP = p->U[sP] + 3 * L[P].idx;
+                    double* UQ = p->U[sQ] + 3 * L[Q].idx;
+                    double sT = ref_sigma_tcr(p->vhs, p->tref, p->d[sP], p->d[sQ], p->omega[sP], p->omega[sQ],
+                                              p->mass[sP], p->mass[sQ], UP, UQ);
+                    if (sT > scrmax[cell]) scrmax[cell] = sT;
+                    if (sT / smax0 > uacc) {
+                        double u1, u2, preP[3] = {UP[0], UP[1], UP[2]}, preQ[3] = {UQ[0], UQ[1], UQ[2]};
+                        pair_rand(TAG_DSMC, p->seed, p->step, cell, k, 1, &u1, &u2);
+                        ref_ion_isotropic(UP, UQ, p->mass[sP], p->mass[sQ], u1, u2);
+                        if (p->weight_correction) {
+                            double uw, unused;
+                            pair_rand(TAG_DSMC, p->seed, p->step, cell, k, 2, &uw, &unused);
+                            nanbu_weight_correct(uw, p->w[sP], p->w[sQ], UP, UQ, preP, preQ);
+                        }
+                        if (counts) counts[1]++;
+                    }
+                }
+            }
+        }
+        if (p->bg && n > 0) {
+            double smaxb = scrmax[cell];
+            double sel = rem_bg[cell] + n * p->nN * smaxb * dt;
+            int ncb = (int)sel;
+            rem_bg[cell] = sel - ncb;
+            if (counts) counts[2] += ncb;
+            for (int k = 0; k < ncb; ++k) {
+                double upick, uacc, ua, ub, n1, n2, n3, n4;
+                pair_rand(TAG_DSMC_BG, p->seed, p->step, cell, k, 0, &upick, &uacc);
+                int P = (int)(upick * n);
+                if (P >= n) P = n - 1;
+                pair_rand(TAG_DSMC_BG, p->seed, p->step, cell, k, 1, &ua, &ub);
+                box_muller(ua, ub, &n1, &n2);
+                pair_rand(TAG_DSMC_BG, p->seed, p->step, cell, k, 2, &ua, &ub);
+                box_muller(ua, ub, &n3, &n4);
+                double Un[3] = {vth * n1 + p->driftN[0], vth * n2 + p->driftN[1], vth * n3 + p->driftN[2]};
+                int sP = L[P].s;
+                double* UP = p->U[sP] + 3 * L[P].idx;
+                double sT = ref_sigma_tcr(p->vhs, p->tref, p->d[sP], p->dN, p->omega[sP], p->omegaN, p->mass[sP],
+                                          p->massN, UP, Un);
+                if (sT > scrmax[cell]) scrmax[cell] = sT;
+                if (sT / smaxb > uacc) {
+                    double u1, u2;
+                    pair_rand(TAG_DSMC_BG, p->seed, p->step, cell, k, 3, &u1, &u2);
+                    ref_ion_isotropic(UP, Un, p->mass[sP], p->massN, u1, u2);
+                    if (counts) counts[3]++;
+                }
+            }
+        }
+    }
+    free(L); free(sub); free(sl);
+    return 0;
+}
+
+/* IonNeutralCollisionModel::performCollisions (IonNeutralCollisionModel.C:580-713): species
+ * [0, na) the neutrals, [na, ns) the ions.  Per cell rem + n_N n_I w_max (sigma cR)_max dt / V
+ * candidates, each a neutral and an ion drawn with replacement; the ion energy
+ * (gamma - 1) m_I c^2 / e of their relative speed; elastic (IonIsotropicScattering with the
+ * weight correction, the neutral reset when updateNeutralVelocity is false) when a uniform
+ * is at most Qel cR / max, else a charge exchange (the identity switch with probabilities
+ * w_Q / w_P and w_P / w_Q) up to Qt cR / max above the threshold.  counts[0..2] +=
+ * candidates, elastic, charge exchange. */
+int ref_ion_pairs(const ref_mesh* m, const ref_pair_params* p, double dt, double* scrmax, double* rem, int64_t* counts) {
+    const double c = C_LIGHT;
+    int maxn = 2;
+    for (int cell = 0; cell < m->n_cells; ++cell) {
+        int n = 0;
+        for (int s = 0; s < p->ns; ++s) n += p->cell_start[s][cell + 1] - p->cell_start[s][cell];
+        if (n > maxn) maxn = n;
+    }
+    prec* L = (prec*)malloc(sizeof(prec) * maxn);
+    for (int cell = 0; cell < m->n_cells; ++cell) {
+        int nA;
+        double wA, wB;
+        int n = cell_list(p, cell, L, &nA, &wA, &wB);
+        int nB = n - nA;
+        if (nA == 0 || nB == 0) continue;
+        double wm = wB > wA ? wB : wA;
+        double smax0 = scrmax[cell];
+        double sel = rem[cell] + (double)(nA * nB) * wm * smax0 * dt / m->V[cell];
+        int nc = (int)sel;
+        rem[cell] = sel - nc;
+        if (counts) counts[0] += nc;
+        for (int k = 0; k < nc; ++k) {
+            double uP, uQ;
+            pair_rand(TAG_ION_PAIR, p->seed, p->step, cell, k, 0, &uP, &uQ);
+            int P = (int)(uP * nA);
+            if (P >= nA) P = nA - 1;
+            int Q = (int)(uQ * nB);
+            if (Q >= nB) Q = nB - 1;
+            Q += nA;
+            int sP = L[P].s, sQ = L[Q].s;
+            double* UP = p->U[sP] + 3 * L[P].idx;
+            double* UQ = p->U[sQ] + 3 * L[Q].idx;
+            double massI = p->mass[sQ];
+            double cR = vmag(vsub(ld3(UQ), ld3(UP)));
+            double gamma = 1.0 / sqrt(1.0 - ((cR * cR) / (c * c)));
+            double eV = (gamma - 1.0) * massI * c * c / QE;
+            double Qel = sigma_of(p->n_el, p->e_el, p->s_el, eV);
+            double Qcx = sigma_of(p->n_cx, p->e_cx, p->s_cx, eV);
+            double Qt = Qel + Qcx;
+            if (Qt * cR > scrmax[cell]) scrmax[cell] = Qt * cR;
+            double uacc, ucx;
+            pair_rand(TAG_ION_PAIR, p->seed, p->step, cell, k, 1, &uacc, &ucx);
+            double wP = p->w[sP], wQ = p->w[sQ];
+            if (uacc <= Qel * cR / smax0) {
+                double u1, u2, preP[3] = {UP[0], UP[1], UP[2]}, preQ[3] = {UQ[0], UQ[1], UQ[2]};
+                pair_rand(TAG_ION_PAIR, p->seed, p->step, cell, k, 2, &u1, &u2);
+                ref_ion_isotropic(UP, UQ, p->mass[sP], massI, u1, u2);
+                if (p->weight_correction) {
+                    double uw, unused;
+                    pair_rand(TAG_ION_PAIR, p->seed, p->step, cell, k, 3, &uw, &unused);
+                    nanbu_weight_correct(uw, wP, wQ, UP, UQ, preP, preQ);
+                }
+                if (p->keep_neutral) { UP[0] = preP[0]; UP[1] = preP[1]; UP[2] = preP[2]; }
+                if (counts) counts[1]++;
+            } else if (eV >= p->th_cx && uacc <= Qt * cR / smax0) {
+                double tmp[3] = {UP[0], UP[1], UP[2]};
+                if (ucx <= wQ / wP) { UP[0] = UQ[0]; UP[1] = UQ[1]; UP[2] = UQ[2]; }
+                if (ucx <= wP / wQ) { UQ[0] = tmp[0]; UQ[1] = tmp[1]; UQ[2] = tmp[2]; }
+                if (p->keep_neutral) { UP[0] = tmp[0]; UP[1] = tmp[1]; UP[2] = tmp[2]; }
+                if (counts) counts[2]++;
+            }
+        }
+    }
+    free(L);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ Poisson */
 
 struct ref_poisson {

@@ -163,6 +163,35 @@ int ref_ion_mcc(const ref_mesh* m, const ref_ion_mcc_params* p, double dt, doubl
 /* IonIsotropicScattering::updateVelocity on (UP, UQ) in place */
 void ref_ion_isotropic(double* UP, double* UQ, double mP, double mQ, double u1, double u2);
 
+/* parcel pairs: the binary model's neutral DSMC and the ion-neutral model's parcel
+ * collisions.  Species [0, ns) in list order, each sorted by cell (cell_start); the first
+ * na form the first group (the neutrals). */
+typedef struct {
+    int ns, na;
+    double mass[8], w[8], d[8], omega[8];
+    double* U[8];
+    const double* y[8];
+    const int* tet[8];
+    const uint32_t* id[8];
+    const int* cell_start[8];
+    int vhs;
+    double tref;
+    int weight_correction, keep_neutral;
+    int bg;
+    double nN, TN, massN, dN, omegaN, driftN[3];
+    int n_el, n_cx;
+    const double *e_el, *s_el, *e_cx, *s_cx;
+    double th_cx;
+    uint64_t seed;
+    int64_t step;
+} ref_pair_params;
+int ref_binary(const ref_mesh* m, const ref_pair_params* p, double dt, double* sigma_cr_max, double* remainder,
+               double* remainder_bg, int64_t* counts);
+int ref_ion_pairs(const ref_mesh* m, const ref_pair_params* p, double dt, double* sigma_cr_max, double* remainder,
+                  int64_t* counts);
+double ref_sigma_tcr(int vhs, double tref, double dP, double dQ, double omP, double omQ, double mP, double mQ,
+                     const double* UP, const double* UQ);
+
 /* electron-neutral scattering with a sampled partner velocity, in place:
  * NanbuElectronNeutralCollision.C:98-130 (relativistic 0) or
  * RelativisticElectronNeutralCollision.C:161-240 (relativistic 1) */

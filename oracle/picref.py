@@ -55,6 +55,10 @@ def lib():
         _lib.ref_positions.argtypes = [vp, C.c_int, vp, vp, vp]
         _lib.ref_locate.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         _lib.ref_locate.restype = C.c_int
+        _lib.ref_binary.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp]
+        _lib.ref_ion_pairs.argtypes = [vp, vp, C.c_double, vp, vp, vp]
+        _lib.ref_sigma_tcr.argtypes = [C.c_int] + [C.c_double] * 7 + [vp, vp]
+        _lib.ref_sigma_tcr.restype = C.c_double
         _lib.ref_push_one.argtypes = [C.c_int, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
         _lib.ref_interp_E.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
         _lib.ref_move.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_double, C.c_double, C.c_double,
@@ -392,6 +396,85 @@ def ion_isotropic(UP, UQ, mP, mQ, u1, u2):
     P = np.array(UP, np.float64); Q = np.array(UQ, np.float64)
     lib().ref_ion_isotropic(_p(P), _p(Q), float(mP), float(mQ), float(u1), float(u2))
     return P, Q
+
+
+class PairParams(C.Structure):
+    _fields_ = [("ns", C.c_int), ("na", C.c_int), ("mass", C.c_double * 8), ("w", C.c_double * 8),
+                ("d", C.c_double * 8), ("omega", C.c_double * 8), ("U", C.c_void_p * 8), ("y", C.c_void_p * 8),
+                ("tet", C.c_void_p * 8), ("id", C.c_void_p * 8), ("cell_start", C.c_void_p * 8), ("vhs", C.c_int),
+                ("tref", C.c_double), ("weight_correction", C.c_int), ("keep_neutral", C.c_int), ("bg", C.c_int),
+                ("nN", C.c_double), ("TN", C.c_double), ("massN", C.c_double), ("dN", C.c_double),
+                ("omegaN", C.c_double), ("driftN", C.c_double * 3), ("n_el", C.c_int), ("n_cx", C.c_int),
+                ("e_el", C.c_void_p), ("s_el", C.c_void_p), ("e_cx", C.c_void_p), ("s_cx", C.c_void_p),
+                ("th_cx", C.c_double), ("seed", C.c_uint64), ("step", C.c_int64)]
+
+
+def _pair_params(species, na, params):
+    """species: dicts mass, w, d, omega, y [n,3], tet [n], U [n,3], ids [n], cs [n_cells+1], each
+    sorted by cell; params: seed, step and the model's settings.  Returns the structure, the
+    velocity copies the oracle updates and the arrays it must keep alive."""
+    p = PairParams()
+    keep, Us = [], []
+    p.ns, p.na = len(species), na
+    for j, sp in enumerate(species):
+        p.mass[j] = sp["mass"]; p.w[j] = sp["w"]; p.d[j] = sp.get("d", 0.0); p.omega[j] = sp.get("omega", 0.0)
+        U = np.ascontiguousarray(sp["U"], np.float64).reshape(-1, 3).copy()
+        y = np.ascontiguousarray(sp["y"], np.float64).reshape(-1, 3)
+        tet = np.ascontiguousarray(sp["tet"], np.int32)
+        ids = np.ascontiguousarray(sp["ids"], np.uint32)
+        cs = np.ascontiguousarray(sp["cs"], np.int32)
+        keep += [U, y, tet, ids, cs]
+        Us.append(U)
+        p.U[j] = U.ctypes.data; p.y[j] = y.ctypes.data; p.tet[j] = tet.ctypes.data
+        p.id[j] = ids.ctypes.data; p.cell_start[j] = cs.ctypes.data
+    p.vhs = 1 if params.get("TotalCrossSectionModel", "HardSphere") == "VariableHardSphere" else 0
+    p.tref = float(params.get("Tref", 0.0))
+    p.weight_correction = 1 if params.get("WeightCorrection", "none") == "Nanbu" else 0
+    p.keep_neutral = 0 if params.get("updateNeutralVelocity", True) else 1
+    bg = params.get("background")
+    if bg:
+        p.bg = 1
+        p.nN, p.TN, p.massN = float(bg["numberDensity"]), float(bg["temperature"]), float(bg["mass"])
+        p.dN, p.omegaN = float(bg.get("d", 0.0)), float(bg.get("omega", 0.0))
+        for d in range(3):
+            p.driftN[d] = bg.get("velocity", (0.0, 0.0, 0.0))[d]
+    if "xs" in params:
+        for k, (nm_e, nm_s, nm_n) in enumerate((("e_el", "s_el", "n_el"), ("e_cx", "s_cx", "n_cx"))):
+            e, s_ = params["xs"][k]
+            e = np.ascontiguousarray(e, np.float64); s_ = np.ascontiguousarray(s_, np.float64)
+            keep += [e, s_]
+            setattr(p, nm_e, e.ctypes.data); setattr(p, nm_s, s_.ctypes.data); setattr(p, nm_n, len(e))
+        p.th_cx = float(params.get("th_cx", 0.0))
+    p.seed = int(params["seed"]); p.step = int(params["step"])
+    return p, Us, keep
+
+
+def binary(rm: RefMesh, params: dict, dt, species, smax, rem, rem_bg):
+    """BinaryCollisionModel over the neutral species (all of `species`).  Returns (U per
+    species, smax, rem, rem_bg, counts[pair candidates, pair collisions, background
+    candidates, background collisions])."""
+    p, Us, keep = _pair_params(species, len(species), params)
+    smax = np.ascontiguousarray(smax, np.float64).copy(); rem = np.ascontiguousarray(rem, np.float64).copy()
+    rem_bg = np.ascontiguousarray(rem_bg, np.float64).copy()
+    counts = np.zeros(4, np.int64)
+    lib().ref_binary(rm.h, C.byref(p), dt, _p(smax), _p(rem), _p(rem_bg), _p(counts))
+    return Us, smax, rem, rem_bg, counts
+
+
+def ion_pairs(rm: RefMesh, params: dict, dt, neutral, ion, smax, rem):
+    """IonNeutralCollisionModel's collisions of the ions with the neutral parcels.  Returns
+    (U neutral, U ion, smax, rem, counts[candidates, elastic, charge exchange])."""
+    p, Us, keep = _pair_params([neutral, ion], 1, params)
+    smax = np.ascontiguousarray(smax, np.float64).copy(); rem = np.ascontiguousarray(rem, np.float64).copy()
+    counts = np.zeros(3, np.int64)
+    lib().ref_ion_pairs(rm.h, C.byref(p), dt, _p(smax), _p(rem), _p(counts))
+    return Us[0], Us[1], smax, rem, counts
+
+
+def sigma_tcr(vhs, tref, dP, dQ, omP, omQ, mP, mQ, UP, UQ):
+    a = np.ascontiguousarray(UP, np.float64); b = np.ascontiguousarray(UQ, np.float64)
+    return lib().ref_sigma_tcr(int(vhs), float(tref), float(dP), float(dQ), float(omP), float(omQ), float(mP),
+                               float(mQ), _p(a), _p(b))
 
 
 class RefPoisson:

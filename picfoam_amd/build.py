@@ -23,6 +23,7 @@ SOURCES = [
     ("k_field.hip", []),
     ("k_coulomb.hip", []),
     ("k_mcc.hip", []),
+    ("k_dsmc.hip", []),
     ("k_poisson.hip", []),
     ("k_gnn.hip", []),
     ("noalloc.hip", []),

@@ -27,6 +27,7 @@ struct DevBuf {
 
 struct Species {
     double mass = 0, charge = 0, weight = 0;
+    double diameter = 0, omega = 0;          // moleculeProperties diameter and omega
     int solve_movement = 1;
     int64_t capacity = 0;
     int64_t n = 0;  // host mirror of the parcel count
@@ -65,7 +66,13 @@ struct pic_ctx {
     // models
     int pusher = kPushBoris, fw = kCellAverage, cd = kCellAverage;
     bool electrostatic = false, coulomb = false, mcc = false, bg = false;
-    bool ion_mcc = false;                    // IonNeutralCollisionModel against the background
+    bool ion_mcc = false;                    // IonNeutralCollisionModel (neutral parcels, background)
+    bool binary = false;                     // BinaryCollisionModel IsotropicScattering
+    int vhs = 0;                             // TotalCrossSectionModel VariableHardSphere
+    double vhs_tref = 0.0;
+    int binary_wc = 0, ion_wc = 0;           // Nanbu weight correction of the binary / ion-neutral models
+    int ion_keep_neutral = 0;                // IonNeutralCollisionCoeffs.updateNeutralVelocity false
+    int bg_species = -1;                     // the background gas' species
     int coulomb_model = 0, pairing = 0, weight_corr = 0;
     int en_relativistic = 0;
     int precision = PIC_PREC_FP64;
@@ -131,6 +138,10 @@ struct pic_ctx {
     // charge exchange)
     double *d_smax = nullptr, *d_rem = nullptr;
     double *d_smax_i = nullptr, *d_rem_i = nullptr;
+    // ion-neutral parcel pairs; binary model pairs and background (one maximum for both)
+    double *d_smax_ip = nullptr, *d_rem_ip = nullptr;
+    double *d_smax_n = nullptr, *d_rem_n = nullptr, *d_rem_nb = nullptr;
+    DsmcPair* d_dsmc_tab = nullptr;
     static constexpr int kXsProcs = 5;
     double* d_xs_e[kXsProcs] = {};
     double* d_xs_s[kXsProcs] = {};
@@ -664,18 +675,113 @@ int do_ion_mcc_t(pic_ctx* c, double dt) {
     return check_hip(c, launch_mcc(a, c->stream), "ion-neutral mcc kernel");
 }
 
+// the parcel-pair passes' arguments for a list of species (the first na form group A)
+template <class UT>
+PairArgs<UT> pair_args(pic_ctx* c, const std::vector<int>& list, int na, double dt) {
+    PairArgs<UT> a{};
+    a.n_cells = c->hm.n_cells;
+    a.ns = (int)list.size();
+    a.na = na;
+    for (int j = 0; j < a.ns; ++j) {
+        Species& S = c->sp[list[j]];
+        a.p[j] = soa<UT>(S, S.cur);
+        a.start[j] = S.start;
+        a.perm[j] = S.indexed ? S.perm : nullptr;
+        a.id_cell[j] = S.indexed ? S.id_cell : S.id[S.cur];
+        a.w[j] = S.weight;
+        a.mass[j] = S.mass;
+    }
+    a.vol = c->d_V;
+    a.tets = c->d_tets;
+    a.C = c->d_C;
+    a.pts = c->d_points;
+    a.dt = dt;
+    a.seed = c->seed;
+    a.step = c->step;
+    a.big_list = c->d_big_list;
+    a.big_n = c->d_big_n;
+    hipMemsetAsync(c->d_big_n, 0, sizeof(int), c->stream);
+    c->big_n_zero = false;
+    a.flags = c->d_flags;
+    a.counts = c->d_stats + kStatsMcc;
+    return a;
+}
+
+// IonNeutralCollisionModel::performCollisions (IonNeutralCollisionModel.C:580-713): the ion
+// species against the neutral species' parcels
+template <class UT>
+int do_ion_pairs_t(pic_ctx* c, double dt, int neutral, int ion) {
+    PairArgs<UT> a = pair_args<UT>(c, {neutral, ion}, 1, dt);
+    a.smax = c->d_smax_ip;
+    a.rem = c->d_rem_ip;
+    for (int k = 0; k < 2; ++k) {
+        const int p = PIC_XS_ION_ELASTIC + k;
+        a.xs_e[k] = c->d_xs_e[p]; a.xs_s[k] = c->d_xs_s[p]; a.xs_n[k] = c->xs_n[p];
+        a.xs_guide[k] = c->d_xs_guide[p]; a.xs_le0[k] = c->xs_le0[p]; a.xs_idl[k] = c->xs_idl[p];
+    }
+    a.th_cx = c->xs_th[PIC_XS_ION_CHARGE_EXCHANGE];
+    a.keep_neutral = c->ion_keep_neutral;
+    a.wc = c->ion_wc;
+    return check_hip(c, launch_pairs(a, kPairIon, c->stream), "ion-neutral pair kernel");
+}
+
+// the ion species of the ion-neutral model: the background's ion, else the one charged
+// species that is not the electrons (one at most, checked at creation)
+int ion_species_of(const pic_ctx* c) {
+    if (c->ion >= 0) return c->ion;
+    for (int s = 0; s < (int)c->sp.size(); ++s)
+        if (c->sp[s].charge != 0.0 && s != c->electron) return s;
+    return -1;
+}
+
+// IonNeutralCollisionModel::handleCollisions (IonNeutralCollisionModel.C:246-254): the ions
+// with the neutral parcels, then with the background gas
 int do_ion_mcc(pic_ctx* c, double dt) {
     pic_ctx::Scope sc(c, PIC_STAGE_MCC);
     if (c->xs_n[PIC_XS_ION_ELASTIC] + c->xs_n[PIC_XS_ION_CHARGE_EXCHANGE] == 0)
         return c->fail(PIC_ERR_INVALID, "no ion-neutral cross sections set");
-    // IonNeutralCollisionModel::performCollisions pairs ions with neutral parcels; only the
-    // background part is built
-    for (size_t s = 0; s < c->sp.size(); ++s)
-        if (c->sp[s].charge == 0.0 && c->sp[s].n > 0)
-            return c->fail(PIC_ERR_UNSUPPORTED, "ion-neutral collisions with neutral parcels are not implemented "
-                                                "(species " + std::to_string(s) + " holds parcels)");
-    if (c->sp[c->ion].n == 0) return PIC_OK;
+    int neutral = -1;
+    for (int s = 0; s < (int)c->sp.size(); ++s)
+        if (c->sp[s].charge == 0.0) neutral = s;
+    const int ion = ion_species_of(c);
+    if (neutral >= 0 && ion >= 0 && c->sp[neutral].n > 0 && c->sp[ion].n > 0) {
+        const int r = c->mixed() ? do_ion_pairs_t<float>(c, dt, neutral, ion) : do_ion_pairs_t<double>(c, dt, neutral, ion);
+        if (r) return r;
+    }
+    if (!c->bg || c->sp[c->ion].n == 0) return PIC_OK;
     return c->mixed() ? do_ion_mcc_t<float>(c, dt) : do_ion_mcc_t<double>(c, dt);
+}
+
+// BinaryCollisionModel::handleCollisions (BinaryCollisionModel.C:527-535): the neutral
+// species' parcels in pairs, then with the background gas
+template <class UT>
+int do_binary_t(pic_ctx* c, double dt, const std::vector<int>& nl) {
+    PairArgs<UT> a = pair_args<UT>(c, nl, (int)nl.size(), dt);
+    a.smax = c->d_smax_n;
+    a.rem = c->d_rem_n;
+    a.rem_bg = c->d_rem_nb;
+    a.tab = c->d_dsmc_tab;
+    a.vhs = c->vhs;
+    a.wc = c->binary_wc;
+    a.bg = c->bg ? 1 : 0;
+    a.nN = c->bg_n;
+    a.vthN = c->bg ? std::sqrt(kKB * c->bg_T / c->bg_m) : 0.0;
+    for (int d = 0; d < 3; ++d) a.drift[d] = c->bg_v[d];
+    a.massN = c->bg_m;
+    return check_hip(c, launch_pairs(a, kPairDsmc, c->stream), "binary collision kernel");
+}
+
+int do_binary(pic_ctx* c, double dt) {
+    pic_ctx::Scope sc(c, PIC_STAGE_MCC);
+    std::vector<int> nl;
+    bool any = false;
+    for (int s = 0; s < (int)c->sp.size(); ++s)
+        if (c->sp[s].charge == 0.0) {
+            nl.push_back(s);
+            any = any || c->sp[s].n > 0;
+        }
+    if (!any) return PIC_OK;
+    return c->mixed() ? do_binary_t<float>(c, dt, nl) : do_binary_t<double>(c, dt, nl);
 }
 
 int do_mcc(pic_ctx* c, double dt) {
@@ -885,11 +991,36 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->en_relativistic = en == 2 ? 1 : 0;
     c->bg = bgm == 1;
     c->ion_mcc = inm != 0;
-    if (c->ion_mcc && !c->bg) {
-        g_last_error = c->err = "IonNeutralCollisionModel: only collisions with the uniformBackground gas are implemented "
-                                "(IonNeutralCollisionModel.C:447-560); neutral parcels are not";
-        return PIC_ERR_UNSUPPORTED;
+    if (c->ion_mcc) {
+        int wc = 0;
+        if ((r = parse_model(c.get(), models->ion_neutral_weight_correction ? models->ion_neutral_weight_correction : "none",
+                             "WeightCorrectionModel", {"none", "Nanbu"}, &wc)))
+            return r;
+        c->ion_wc = wc;
+        c->ion_keep_neutral = models->ion_neutral_keep_neutral ? 1 : 0;
     }
+    // BinaryCollisionModel (BinaryCollisionModelNew.C:39-61) with its total cross section
+    // (TotalCrossSectionModelNew.C) and weight correction
+    int bin = 0;
+    if ((r = parse_model(c.get(), models->binary_collision ? models->binary_collision : "none", "BinaryCollisionModel",
+                         {"none", "IsotropicScattering"}, &bin)))
+        return r;
+    c->binary = bin != 0;
+    if (c->binary) {
+        int xs = 0, wc = 0;
+        if ((r = parse_model(c.get(), models->total_cross_section ? models->total_cross_section : "HardSphere",
+                             "TotalCrossSectionModel", {"HardSphere", "VariableHardSphere"}, &xs)))
+            return r;
+        if ((r = parse_model(c.get(), models->binary_weight_correction ? models->binary_weight_correction : "none",
+                             "WeightCorrectionModel", {"none", "Nanbu"}, &wc)))
+            return r;
+        c->vhs = xs;
+        c->binary_wc = wc;
+        c->vhs_tref = models->vhs_tref;
+        if (c->vhs && !(c->vhs_tref > 0.0))
+            return c->fail(PIC_ERR_INVALID, "VariableHardSphereCoeffs.Tref must be positive");
+    }
+    c->bg_species = models->bg_species;
     if (c->mcc && !c->bg) {
         g_last_error = c->err = "ElectronNeutralCollisionModel needs the uniformBackground gas model (particle neutrals are not implemented)";
         return PIC_ERR_UNSUPPORTED;
@@ -936,12 +1067,13 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->sort_interval = models->sort_interval > 0 ? models->sort_interval : 1;
     c->poisson_guess = models->poisson_guess;
     if (c->poisson_guess < 0 || c->poisson_guess > 1) return c->fail(PIC_ERR_INVALID, "poisson_guess");
-    if (c->ion_mcc) {
-        // IonNeutralCollisionModel.C:170-181: the background neutral needs its ion species
+    if (c->ion_mcc && c->bg) {
+        // IonNeutralCollisionModel.C:181-188: the background neutral needs its ion species
         if (c->ion < 0 || c->ion >= n_species)
             return c->fail(PIC_ERR_INVALID, "IonNeutralCollisionModel: no ion type for the background gas (SpeciesRelations ionTypeId)");
         if (!(c->bg_m > 0)) return c->fail(PIC_ERR_INVALID, "background gas mass");
     }
+    if (c->binary && c->bg && !(c->bg_m > 0)) return c->fail(PIC_ERR_INVALID, "background gas mass");
     if (c->mcc) {
         if (c->electron < 0 || c->electron >= n_species) return c->fail(PIC_ERR_INVALID, "electronTypeId not set");
         if (c->ion >= n_species) return c->fail(PIC_ERR_INVALID, "ion species out of range");
@@ -1016,6 +1148,8 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         S.mass = species[s].mass;
         S.charge = species[s].charge;
         S.weight = species[s].n_equivalent;
+        S.diameter = species[s].diameter;
+        S.omega = species[s].omega;
         S.solve_movement = species[s].solve_movement;
         S.capacity = species[s].capacity;
         S.sort_interval = species[s].sort_interval > 0 ? species[s].sort_interval : c->sort_interval;
@@ -1025,6 +1159,18 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         c->sp.push_back(S);
     }
     c->Q0 = qmax > 0 ? qmax : 1.0;
+    if (c->ion_mcc) {
+        // IonNeutralCollisionModel.C:170-175: one neutral and one ion species at most
+        int nn = 0, ni = 0;
+        for (int s = 0; s < n_species; ++s) {
+            if (c->sp[s].charge == 0.0) ++nn;
+            else if (s != c->electron) ++ni;
+        }
+        if (nn > 1 || ni > 1)
+            return c->fail(PIC_ERR_UNSUPPORTED, "IonNeutralCollisionModel supports one neutral and one ion species only");
+    }
+    if (c->binary && c->bg && (c->bg_species < 0 || c->bg_species >= n_species || c->sp[c->bg_species].charge != 0.0))
+        return c->fail(PIC_ERR_INVALID, "BinaryCollisionModel with a background gas: bg_species must name a neutral species");
     {
         std::vector<double> ms(n_species), qs(n_species);
         for (int s = 0; s < n_species; ++s) { ms[s] = c->sp[s].mass; qs[s] = c->sp[s].charge; }
@@ -1099,18 +1245,66 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         }
         hipMemcpyAsync(c->d_rem, rem.data(), sizeof(double) * nc, hipMemcpyHostToDevice, c->stream);
     }
-    if (c->ion_mcc) {
-        c->d_smax_i = c->alloc<double>(nc);
-        c->d_rem_i = c->alloc<double>(nc);
-        if (!c->d_smax_i || !c->d_rem_i) return c->fail(PIC_ERR_HIP, "out of device memory (ion-neutral)");
-        // remainders start random in [0,1) (IonNeutralCollisionModel.C:190-194)
+    // candidate remainders start random in [0,1) (IonNeutralCollisionModel.C:190-194,
+    // BinaryCollisionModel.C:155-159), each field from its own salt
+    auto random_remainders = [&](double* d, uint64_t mul, uint64_t salt) {
         std::vector<double> rem(nc);
         for (int i = 0; i < nc; ++i) {
-            uint64_t x = (uint64_t)i * 0xD1B54A32D192ED03ull ^ (c->seed + 0x8CB92BA72F3D8DD7ull);
+            uint64_t x = (uint64_t)i * mul ^ (c->seed + salt);
             x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
             rem[i] = (double)(x >> 11) * (1.0 / 9007199254740992.0);
         }
-        hipMemcpyAsync(c->d_rem_i, rem.data(), sizeof(double) * nc, hipMemcpyHostToDevice, c->stream);
+        hipMemcpyAsync(d, rem.data(), sizeof(double) * nc, hipMemcpyHostToDevice, c->stream);
+        // the host vector dies here: the copy must have read it
+        hipStreamSynchronize(c->stream);
+    };
+    if (c->ion_mcc) {
+        c->d_smax_i = c->alloc<double>(nc);
+        c->d_rem_i = c->alloc<double>(nc);
+        c->d_smax_ip = c->alloc<double>(nc);
+        c->d_rem_ip = c->alloc<double>(nc);
+        if (!c->d_smax_i || !c->d_rem_i || !c->d_smax_ip || !c->d_rem_ip)
+            return c->fail(PIC_ERR_HIP, "out of device memory (ion-neutral)");
+        random_remainders(c->d_rem_i, 0xD1B54A32D192ED03ull, 0x8CB92BA72F3D8DD7ull);
+        random_remainders(c->d_rem_ip, 0xA24BAED4963EE407ull, 0x9FB21C651E98DF25ull);
+    }
+    if (c->binary) {
+        c->d_smax_n = c->alloc<double>(nc);
+        c->d_rem_n = c->alloc<double>(nc);
+        c->d_rem_nb = c->alloc<double>(nc);
+        // pair constants in the order of the neutral species, background row last
+        // (HardSphere.C:64-79, VariableHardSphere.C:64-97)
+        std::vector<int> nl;
+        for (int s = 0; s < n_species; ++s)
+            if (c->sp[s].charge == 0.0) nl.push_back(s);
+        std::vector<DsmcPair> tab((kMaxSpecies + 1) * kMaxSpecies);
+        auto pair = [&](const Species& P, double dQ, double omQ, double mQ) {
+            DsmcPair t{};
+            const double d = 0.5 * (P.diameter + dQ);
+            const double om = 0.5 * (P.omega + omQ);
+            t.hs = kPi * d * d;
+            t.vn = 2.0 * kKB * c->vhs_tref;
+            t.mr = P.mass * mQ / (P.mass + mQ);
+            t.ve = om - 0.5;
+            t.gl = std::exp(std::lgamma(2.5 - om));
+            return t;
+        };
+        for (size_t a = 0; a < nl.size(); ++a) {
+            const Species& P = c->sp[nl[a]];
+            for (size_t b = 0; b < nl.size(); ++b) {
+                const Species& Q = c->sp[nl[b]];
+                tab[a * kMaxSpecies + b] = pair(P, Q.diameter, Q.omega, Q.mass);
+            }
+            if (c->bg) {
+                const Species& G = c->sp[c->bg_species];
+                tab[kMaxSpecies * kMaxSpecies + a] = pair(P, G.diameter, G.omega, c->bg_m);
+            }
+        }
+        c->d_dsmc_tab = c->upload(tab);
+        if (!c->d_smax_n || !c->d_rem_n || !c->d_rem_nb || !c->d_dsmc_tab)
+            return c->fail(PIC_ERR_HIP, "out of device memory (binary collisions)");
+        random_remainders(c->d_rem_n, 0xE7037ED1A0B428DBull, 0x1D8E4E27C47D124Full);
+        random_remainders(c->d_rem_nb, 0x8EBC6AF09C88C6E3ull, 0x589965CC75374CC3ull);
     }
     for (auto& S : c->sp)
         if ((r = alloc_species(c.get(), S))) return r;
@@ -1286,6 +1480,20 @@ int pic_set_cell_field(pic_ctx* c, int32_t field, const double* v) {
             hipMemcpyAsync(field == PIC_FIELD_ION_REMAINDER ? c->d_rem_i : c->d_smax_i, v, sizeof(double) * nc,
                            hipMemcpyHostToDevice, c->stream);
             break;
+        case PIC_FIELD_ION_PARCEL_SIGMA_CR_MAX:
+        case PIC_FIELD_ION_PARCEL_REMAINDER:
+            if (!c->ion_mcc) return c->fail(PIC_ERR_INVALID, "no ion-neutral collision model");
+            hipMemcpyAsync(field == PIC_FIELD_ION_PARCEL_REMAINDER ? c->d_rem_ip : c->d_smax_ip, v, sizeof(double) * nc,
+                           hipMemcpyHostToDevice, c->stream);
+            break;
+        case PIC_FIELD_BINARY_SIGMA_CR_MAX:
+        case PIC_FIELD_BINARY_REMAINDER:
+        case PIC_FIELD_BINARY_BG_REMAINDER:
+            if (!c->binary) return c->fail(PIC_ERR_INVALID, "no binary collision model");
+            hipMemcpyAsync(field == PIC_FIELD_BINARY_SIGMA_CR_MAX ? c->d_smax_n
+                                                                  : (field == PIC_FIELD_BINARY_REMAINDER ? c->d_rem_n : c->d_rem_nb),
+                           v, sizeof(double) * nc, hipMemcpyHostToDevice, c->stream);
+            break;
         default:
             return c->fail(PIC_ERR_INVALID, "field cannot be set");
     }
@@ -1314,6 +1522,20 @@ int pic_get_cell_field(pic_ctx* c, int32_t field, double* v) {
         case PIC_FIELD_ION_REMAINDER:
             if (!c->ion_mcc) return c->fail(PIC_ERR_INVALID, "no ion-neutral collision model");
             src = field == PIC_FIELD_ION_REMAINDER ? c->d_rem_i : c->d_smax_i;
+            n = nc;
+            break;
+        case PIC_FIELD_ION_PARCEL_SIGMA_CR_MAX:
+        case PIC_FIELD_ION_PARCEL_REMAINDER:
+            if (!c->ion_mcc) return c->fail(PIC_ERR_INVALID, "no ion-neutral collision model");
+            src = field == PIC_FIELD_ION_PARCEL_REMAINDER ? c->d_rem_ip : c->d_smax_ip;
+            n = nc;
+            break;
+        case PIC_FIELD_BINARY_SIGMA_CR_MAX:
+        case PIC_FIELD_BINARY_REMAINDER:
+        case PIC_FIELD_BINARY_BG_REMAINDER:
+            if (!c->binary) return c->fail(PIC_ERR_INVALID, "no binary collision model");
+            src = field == PIC_FIELD_BINARY_SIGMA_CR_MAX ? c->d_smax_n
+                                                         : (field == PIC_FIELD_BINARY_REMAINDER ? c->d_rem_n : c->d_rem_nb);
             n = nc;
             break;
         default: return c->fail(PIC_ERR_INVALID, "unknown field");
@@ -1390,11 +1612,16 @@ int pic_mcc_initialize(pic_ctx* c, double emax, double estep) {
     return fill_cells(c, c->d_smax, smax, "mcc initialize");
 }
 
+// the parcel field and the background field from one value: the model has one neutral and
+// one ion species, so both searches run over the same tables with the same ion mass
+// (IonNeutralCollisionModel.C:283-330, :331-379)
 int pic_ion_mcc_initialize(pic_ctx* c, double emax, double estep) {
     if (!c || !c->ion_mcc) return c ? c->fail(PIC_ERR_INVALID, "no ion-neutral model") : PIC_ERR_INVALID;
     if (!(estep > 0)) return c->fail(PIC_ERR_INVALID, "energy step");
-    const double smax = smax_find(c, PIC_XS_ION_ELASTIC, PIC_XS_ION_CHARGE_EXCHANGE + 1, c->sp[c->ion].mass, emax,
-                                  estep);
+    const int ion = ion_species_of(c);
+    if (ion < 0) return c->fail(PIC_ERR_INVALID, "no ion species");
+    const double smax = smax_find(c, PIC_XS_ION_ELASTIC, PIC_XS_ION_CHARGE_EXCHANGE + 1, c->sp[ion].mass, emax, estep);
+    if (int r = fill_cells(c, c->d_smax_ip, smax, "ion-neutral initialize")) return r;
     return fill_cells(c, c->d_smax_i, smax, "ion-neutral initialize");
 }
 
@@ -1408,9 +1635,35 @@ int pic_mcc_initialize_temperature(pic_ctx* c, int32_t model, double temperature
         return fill_cells(c, c->d_smax, smax, "mcc initialize");
     }
     if (!c->ion_mcc) return c->fail(PIC_ERR_INVALID, "no ion-neutral model");
-    const double smax = smax_temperature(c, PIC_XS_ION_ELASTIC, PIC_XS_ION_CHARGE_EXCHANGE + 1, c->sp[c->ion].mass,
+    const int ion = ion_species_of(c);
+    if (ion < 0) return c->fail(PIC_ERR_INVALID, "no ion species");
+    const double smax = smax_temperature(c, PIC_XS_ION_ELASTIC, PIC_XS_ION_CHARGE_EXCHANGE + 1, c->sp[ion].mass,
                                          temperature);
+    if (int r = fill_cells(c, c->d_smax_ip, smax, "ion-neutral initialize")) return r;
     return fill_cells(c, c->d_smax_i, smax, "ion-neutral initialize");
+}
+
+// BinaryCollisionModel::initialize (BinaryCollisionModel.C:540-600)
+int pic_binary_initialize(pic_ctx* c, const double* T, const double* n) {
+    if (!c || !T || !n) return PIC_ERR_INVALID;
+    if (!c->binary) return c->fail(PIC_ERR_INVALID, "no binary collision model");
+    int best = -1;
+    double dens = 0.0;
+    const double bgn = c->bg ? c->bg_n : 0.0;
+    for (int s = 0; s < (int)c->sp.size(); ++s) {
+        if (c->sp[s].charge != 0.0) continue;
+        if (best < 0) best = s;   // the first neutral species when none is denser than zero
+        double d = n[s];
+        if (c->bg && c->bg_species == s) d += bgn;
+        if (d > dens) { dens = d; best = s; }
+    }
+    if (best < 0) return PIC_OK;   // no neutral species: the reference warns and leaves the field
+    double t = T[best];
+    if (c->bg && best == c->bg_species && c->bg_T > t) t = c->bg_T;
+    if (t < 1e-300) t = 293.15;
+    const Species& S = c->sp[best];
+    const double smax = kPi * S.diameter * S.diameter * std::sqrt(2.0 * kKB * t / S.mass);
+    return fill_cells(c, c->d_smax_n, smax, "binary initialize");
 }
 
 int pic_particles_set(pic_ctx* c, int32_t s, int64_t n, const double* bary, const int32_t* tet, const double* U,
@@ -1614,6 +1867,13 @@ int pic_collide_ion_neutral(pic_ctx* c, double dt) {
     return sync_state(c);
 }
 
+int pic_collide_binary(pic_ctx* c, double dt) {
+    if (!c || !c->binary) return c ? c->fail(PIC_ERR_INVALID, "BinaryCollisionModel is none") : PIC_ERR_INVALID;
+    if (int r = do_sort(c)) return r;
+    if (int r = do_binary(c, dt)) return r;
+    return sync_state(c);
+}
+
 int pic_collide_mcc(pic_ctx* c, double dt) {
     if (!c || !c->mcc) return c ? c->fail(PIC_ERR_INVALID, "ElectronNeutralCollisionModel is none") : PIC_ERR_INVALID;
     if (int r = do_sort(c)) return r;
@@ -1646,7 +1906,7 @@ int step_body(pic_ctx* c, double dt, pic_poisson_stats* ps) {
     if (int r = do_move(c, dt)) return r;
     // a species moves into cell order every Species::sort_interval steps; collision models
     // need the cell order every step, and in between get it as an index
-    bool due = c->coulomb || c->mcc || c->ion_mcc;
+    bool due = c->coulomb || c->mcc || c->ion_mcc || c->binary;
     for (const auto& S : c->sp) due = due || (c->step + 1) % S.sort_interval == 0;
     if (due)
         if (int r = do_sort(c, true)) return r;
@@ -1656,6 +1916,8 @@ int step_body(pic_ctx* c, double dt, pic_poisson_stats* ps) {
         if (int r = do_mcc(c, dt)) return r;
     if (c->ion_mcc)
         if (int r = do_ion_mcc(c, dt)) return r;
+    if (c->binary)
+        if (int r = do_binary(c, dt)) return r;
     if (int r = do_deposit_finish(c)) return r;
     if (c->electrostatic)
         if (int r = do_poisson(c, ps)) return r;
@@ -1687,6 +1949,16 @@ void read_step_stats(pic_ctx* c, const pic_poisson_stats& ps, pic_step_stats* st
     st->ion_neutral_candidates = (int64_t)ic[0];
     st->ion_neutral_elastic = (int64_t)ic[1];
     st->ion_neutral_charge_exchange = (int64_t)ic[2];
+    unsigned long long pw[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < kNcollSlots; ++k)
+        for (int j = 0; j < 7; ++j) pw[j] += h[kStatsMcc + k * kNcollStride + kPairWordIon + j];
+    st->ion_parcel_candidates = (int64_t)pw[0];
+    st->ion_parcel_elastic = (int64_t)pw[1];
+    st->ion_parcel_charge_exchange = (int64_t)pw[2];
+    st->binary_candidates = (int64_t)pw[3];
+    st->binary_collisions = (int64_t)pw[4];
+    st->binary_bg_candidates = (int64_t)pw[5];
+    st->binary_bg_collisions = (int64_t)pw[6];
     st->poisson = ps;
 }
 

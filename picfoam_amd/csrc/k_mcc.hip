@@ -14,59 +14,15 @@
 // every parcel sees its collisions in sequence.
 #include <cmath>
 
-#include "device.h"
-#include "kernels.h"
+#include "collide.h"
 
 namespace picmi {
 
 namespace {
 
-// interpolateXY on an ascending table: clamped linear interpolation between e[lo] <= x <
-// e[lo + 1].  With a guide (tables starting above zero) the bracket comes from kXsGuide
-// bins equal in log(e), each holding the last entry at or below its lower edge, and a
-// step or two along the table; otherwise from a bisection.  Both give the same lo.
-__device__ double xs_lookup(const double* __restrict__ e, const double* __restrict__ s, int n,
-                            const int* __restrict__ guide, double le0, double idl, double x) {
-    if (n <= 0) return 0.0;
-    if (!(x > e[0])) return s[0];
-    if (x >= e[n - 1]) return s[n - 1];
-    int lo = 0;
-    if (guide) {
-        int b = (int)((log_pos(x) - le0) * idl);
-        b = b < 0 ? 0 : (b >= kXsGuide ? kXsGuide - 1 : b);
-        lo = guide[b];
-        while (lo > 0 && e[lo] > x) --lo;
-        while (lo + 1 < n && e[lo + 1] <= x) ++lo;
-    } else {
-        int hi = n - 1;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (e[mid] <= x) lo = mid;
-            else hi = mid;
-        }
-    }
-    if (e[lo] == x) return s[lo];
-    return s[lo] + ((x - e[lo]) / (e[lo + 1] - e[lo])) * (s[lo + 1] - s[lo]);
-}
-
-// the draws of candidate k of a cell, keyed by the collision model's tag (kTagMcc for
-// electrons, kTagIonMcc for ions: mcc_tag<REL>)
+// the tag of a scattering model's draws: kTagMcc for electrons, kTagIonMcc for ions
 template <int REL>
 constexpr uint32_t mcc_tag() { return REL == 2 ? kTagIonMcc : kTagMcc; }
-__device__ __forceinline__ void mcc_rand(uint32_t tag, uint64_t seed, int64_t step, int cell, int k, int j, double& a,
-                                         double& b) {
-    uint4 o = rng4(seed, (uint32_t)cell, (uint32_t)step, (uint32_t)k, (tag << 24) | (uint32_t)j);
-    a = u01(o.x, o.y);
-    b = u01(o.z, o.w);
-}
-
-__device__ __forceinline__ void box_muller(double a, double b, double& n1, double& n2) {
-    const double r = sqrt(-2.0 * log_pos(1.0 - a));
-    double sn, cs;
-    sincos_turn(b, &sn, &cs);
-    n1 = r * cs;
-    n2 = r * sn;
-}
 
 // (1 + eV)^u for eV >= 0, u in [0, 1)
 __device__ __forceinline__ double pow1p(double eV, double u) { return exp_small(u * log_pos(1.0 + eV)); }
@@ -320,10 +276,7 @@ __device__ void create_pair(const MccArgs<UT>& a, const MccPair& P, unsigned& c_
 constexpr int kMccIndexBits = 12;
 static_assert(kMccSlotsBig <= (1 << kMccIndexBits), "cell-local index must fit below the order key");
 __device__ __forceinline__ uint64_t mcc_key(uint32_t id, int k) {
-    uint64_t z = (uint64_t)id + 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z ^= z >> 31;
+    const uint64_t z = id_mix(id);
     return ((z >> kMccIndexBits) << kMccIndexBits) | (uint64_t)k;
 }
 

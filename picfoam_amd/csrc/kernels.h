@@ -343,6 +343,66 @@ constexpr int kMccNanbu = 0, kMccRelativistic = 1, kMccIon = 2;
 hipError_t launch_mcc(const MccArgs<double>& a, hipStream_t s);
 hipError_t launch_mcc(const MccArgs<float>& a, hipStream_t s);
 
+// ---------------------------------------------------------------- parcel pairs (k_dsmc.hip)
+// Collisions between parcels of one cell drawn as candidate pairs: the neutral DSMC of
+// BinaryCollisionModel (pairs within the neutral species, then the neutrals with the
+// background gas) and the ions with the neutral parcels of IonNeutralCollisionModel.
+// A cell's parcels are listed species after species (group A first, then group B), each
+// species in the order of a hash of its ids, so that a candidate index names the same
+// parcel whatever order the sort left them in.
+constexpr int kPairIon = 0, kPairDsmc = 1;
+constexpr int kPairSlots = 256, kPairSlotsBig = 1024;   // parcels of a cell per pass
+// per ordered species pair (a, b) of the binary model: pi d_ab^2 (hs) with d_ab the mean
+// diameter, and for the VariableHardSphere model 2 k Tref (vn), the reduced mass (mr),
+// omega_ab - 1/2 (ve) and exp(lgamma(5/2 - omega_ab)) (gl); row kMaxSpecies holds
+// (a, background)
+struct DsmcPair {
+    double hs, vn, mr, ve, gl;
+};
+template <class UT>
+struct PairArgs {
+    int n_cells;
+    int ns, na;                   // species in the cell list; the first na form group A
+    PartSoA<UT> p[kMaxSpecies];   // in list order
+    const int* start[kMaxSpecies];
+    const int* perm[kMaxSpecies];           // sorted slot -> array index, or null
+    const uint32_t* id_cell[kMaxSpecies];   // ids in cell order
+    double w[kMaxSpecies], mass[kMaxSpecies];
+    const double* vol;            // cell volumes
+    double* smax;                 // (sigma cR)_max per cell of the pair pass (and the background pass)
+    double* rem;                  // pair candidate remainder
+    double* rem_bg;               // background candidate remainder (binary model)
+    // binary model
+    const DsmcPair* tab;          // [(kMaxSpecies + 1) * kMaxSpecies]: [a * kMaxSpecies + b], background row
+    int vhs;
+    const TetGeo* tets;           // sub-cells: parcel positions against the cell centre
+    const double* C;
+    const double* pts;
+    int bg;                       // collisions with the uniformBackground gas
+    double nN, vthN, drift[3], massN;
+    // ion-neutral: elastic [0] and charge exchange [1] cross sections
+    const double* xs_e[2];
+    const double* xs_s[2];
+    int xs_n[2];
+    const int* xs_guide[2];
+    double xs_le0[2], xs_idl[2];
+    double th_cx;
+    int keep_neutral;             // updateNeutralVelocity false
+    int wc;                       // Nanbu weight correction
+    double dt;
+    uint64_t seed;
+    int64_t step;
+    int* big_list;                // cells left to the large-cell pass, [n_cells]
+    int* big_n;
+    uint32_t* flags;
+    unsigned long long* counts;   // the MCC counter slots; words kPairWordIon.. / kPairWordDsmc..
+};
+// counter words of an MCC slot: ion pairs (candidates, elastic, charge exchange), binary
+// pairs (candidates, collisions), binary background (candidates, collisions)
+constexpr int kPairWordIon = 7, kPairWordDsmc = 10;
+hipError_t launch_pairs(const PairArgs<double>& a, int mode, hipStream_t s);
+hipError_t launch_pairs(const PairArgs<float>& a, int mode, hipStream_t s);
+
 // ---------------------------------------------------------------- Poisson (k_poisson.hip)
 struct PoissonDev {
     int n, K, ncol;

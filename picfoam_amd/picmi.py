@@ -29,7 +29,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 PIC_BC_ZERO_GRADIENT, PIC_BC_FIXED_VALUE = 0, 1
 POISSON_FP64, POISSON_MIXED_IR = 0, 1
 (FIELD_E, FIELD_B, FIELD_PHI, FIELD_RHO, FIELD_SPACE_CHARGE, FIELD_E_POINT, FIELD_MCC_SMAX, FIELD_MCC_REM,
- FIELD_ION_SMAX, FIELD_ION_REM) = range(10)
+ FIELD_ION_SMAX, FIELD_ION_REM, FIELD_BINARY_SMAX, FIELD_BINARY_REM, FIELD_BINARY_BG_REM, FIELD_ION_PARCEL_SMAX,
+ FIELD_ION_PARCEL_REM) = range(15)
 XS_ELASTIC, XS_EXCITATION, XS_IONIZATION, XS_ION_ELASTIC, XS_ION_CHARGE_EXCHANGE = range(5)
 STAGES = ("move", "sort", "coulomb", "mcc", "deposit", "poisson", "field")
 
@@ -39,7 +40,7 @@ EXPORTS = (
     "pic_mesh_tets", "pic_mesh_geometry", "pic_set_cell_field", "pic_get_cell_field", "pic_set_cross_section",
     "pic_mcc_initialize", "pic_ion_mcc_initialize", "pic_mcc_initialize_temperature", "pic_particles_set", "pic_particles_set_positions", "pic_particles_count",
     "pic_particles_get", "pic_particles_positions", "pic_particles_positions_noalloc", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
-    "pic_collide_coulomb", "pic_collide_mcc", "pic_collide_ion_neutral", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
+    "pic_collide_coulomb", "pic_collide_mcc", "pic_collide_ion_neutral", "pic_collide_binary", "pic_binary_initialize", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
     "pic_run", "pic_cell_moments",
     "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
     "pic_profile_reset", "pic_gnn_init", "pic_gnn_n_params", "pic_gnn_params", "pic_gnn_record",
@@ -69,7 +70,8 @@ class pic_mesh(C.Structure):
 
 class pic_species(C.Structure):
     _fields_ = [("mass", C.c_double), ("charge", C.c_double), ("n_equivalent", C.c_double),
-                ("solve_movement", C.c_int32), ("capacity", C.c_int64), ("sort_interval", C.c_int32)]
+                ("solve_movement", C.c_int32), ("capacity", C.c_int64), ("sort_interval", C.c_int32),
+                ("diameter", C.c_double), ("omega", C.c_double)]
 
 
 class pic_models(C.Structure):
@@ -87,7 +89,10 @@ class pic_models(C.Structure):
                 ("poisson_mode", C.c_int32), ("sort_interval", C.c_int32), ("poisson_guess", C.c_int32),
                 ("debye_species", C.c_uint32), ("debye_ignore_drift", C.c_int32),
                 ("wall_reflection", C.c_char_p), ("diffuse_fraction", C.c_double),
-                ("no_boundary_sync", C.c_int32), ("ion_neutral_collision", C.c_char_p)]
+                ("no_boundary_sync", C.c_int32), ("ion_neutral_collision", C.c_char_p),
+                ("binary_collision", C.c_char_p), ("total_cross_section", C.c_char_p), ("vhs_tref", C.c_double),
+                ("binary_weight_correction", C.c_char_p), ("ion_neutral_weight_correction", C.c_char_p),
+                ("ion_neutral_keep_neutral", C.c_int32), ("bg_species", C.c_int32)]
 
 
 # the keywords Cloud reads from its models dictionary; any other key is an error (the
@@ -98,7 +103,8 @@ MODEL_KEYS = frozenset((
     "allowIntraCollision", "calculate_bmin", "DebyeLength", "coulombLog", "electronTypeId", "ionTypeId",
     "uniformBackground", "createIon", "createElectron", "precision", "hcReferenceForm", "seed",
     "poissonTolerance", "poissonMaxIter", "poissonMode", "sortInterval", "poissonGuess",
-    "WallReflectionModel", "diffuseFraction", "syncVelocityAtBoundary", "IonNeutralCollisionModel"))
+    "WallReflectionModel", "diffuseFraction", "syncVelocityAtBoundary", "IonNeutralCollisionModel",
+    "IonNeutralCollisionCoeffs", "BinaryCollisionModel", "BinaryCollisionCoeffs"))
 
 
 class pic_poisson_stats(C.Structure):
@@ -111,7 +117,11 @@ class pic_step_stats(C.Structure):
                 ("coulomb_collisions", C.c_int64), ("mcc_candidates", C.c_int64), ("mcc_elastic", C.c_int64),
                 ("mcc_excitation", C.c_int64), ("mcc_ionization", C.c_int64), ("poisson", pic_poisson_stats),
                 ("n_wall_hits", C.c_int64), ("ion_neutral_candidates", C.c_int64),
-                ("ion_neutral_elastic", C.c_int64), ("ion_neutral_charge_exchange", C.c_int64)]
+                ("ion_neutral_elastic", C.c_int64), ("ion_neutral_charge_exchange", C.c_int64),
+                ("ion_parcel_candidates", C.c_int64), ("ion_parcel_elastic", C.c_int64),
+                ("ion_parcel_charge_exchange", C.c_int64), ("binary_candidates", C.c_int64),
+                ("binary_collisions", C.c_int64), ("binary_bg_candidates", C.c_int64),
+                ("binary_bg_collisions", C.c_int64)]
 
 
 class pic_diag(C.Structure):
@@ -162,6 +172,8 @@ def load_library(path: str = LIB_PATH):
         "pic_collide_coulomb": (C.c_int, [vp, dbl]),
         "pic_collide_mcc": (C.c_int, [vp, dbl]),
         "pic_collide_ion_neutral": (C.c_int, [vp, dbl]),
+        "pic_collide_binary": (C.c_int, [vp, dbl]),
+        "pic_binary_initialize": (C.c_int, [vp, vp, vp]),
         "pic_deposit": (C.c_int, [vp]),
         "pic_poisson_solve": (C.c_int, [vp, C.POINTER(pic_poisson_stats)]),
         "pic_diag_reduce": (C.c_int, [vp, C.POINTER(pic_diag)]),
@@ -255,6 +267,8 @@ class Cloud:
             sp[i].solve_movement = 1 if s.get("solveMovement", True) else 0
             sp[i].capacity = int(s["capacity"])
             sp[i].sort_interval = int(s.get("sortInterval", 0))
+            sp[i].diameter = float(s.get("diameter", 0.0))
+            sp[i].omega = float(s.get("omega", 0.0))
         unknown = sorted(set(models) - MODEL_KEYS)
         if unknown:
             raise PicError(PIC_ERR_INVALID, f"unknown model keyword(s) {unknown}; valid: {sorted(MODEL_KEYS)}")
@@ -274,6 +288,24 @@ class Cloud:
             v = _enc(models.get(key, default))
             keep.append(v)
             setattr(m, field, v)
+        # BinaryCollisionModel with IsotropicScatteringCoeffs (TotalCrossSectionModel, Tref of
+        # VariableHardSphereCoeffs, WeightCorrection); IonNeutralCollisionCoeffs (WeightCorrection,
+        # updateNeutralVelocity)
+        bc = dict(models.get("BinaryCollisionCoeffs", {}))
+        bad = sorted(set(bc) - {"TotalCrossSectionModel", "Tref", "WeightCorrection"})
+        ic = dict(models.get("IonNeutralCollisionCoeffs", {}))
+        bad += sorted(set(ic) - {"WeightCorrection", "updateNeutralVelocity"})
+        if bad:
+            raise PicError(PIC_ERR_INVALID, f"unknown collision coefficient(s) {bad}")
+        for field, v in (("binary_collision", models.get("BinaryCollisionModel", "none")),
+                         ("total_cross_section", bc.get("TotalCrossSectionModel", "HardSphere")),
+                         ("binary_weight_correction", bc.get("WeightCorrection", "none")),
+                         ("ion_neutral_weight_correction", ic.get("WeightCorrection", "none"))):
+            v = _enc(v)
+            keep.append(v)
+            setattr(m, field, v)
+        m.vhs_tref = float(bc.get("Tref", 0.0))
+        m.ion_neutral_keep_neutral = 0 if ic.get("updateNeutralVelocity", True) else 1
         self._keep = keep
         m.allow_intra_collision = 1 if models.get("allowIntraCollision", True) else 0
         # <model>Coeffs.DebyeLength (CoulombCollisionModel.C:98-141): species (ids), accountForDrift,
@@ -300,6 +332,7 @@ class Cloud:
         m.bg_n_equivalent = bg.get("nEquivalentParticles", 0.0)
         for d, v in enumerate(bg.get("velocity", (0.0, 0.0, 0.0))):
             m.bg_velocity[d] = v
+        m.bg_species = int(bg.get("species", -1))
         m.create_ion = 1 if models.get("createIon", True) else 0
         m.create_electron = 1 if models.get("createElectron", True) else 0
         m.precision = {"fp64": PREC_FP64, "mixed": PREC_MIXED}[models.get("precision", "fp64")]
@@ -371,7 +404,9 @@ class Cloud:
         n = {FIELD_E: 3 * self.n_cells, FIELD_B: 3 * self.n_cells, FIELD_PHI: self.n_cells,
              FIELD_RHO: self.n_cells, FIELD_SPACE_CHARGE: self.n_cells, FIELD_E_POINT: 3 * self.n_points,
              FIELD_MCC_SMAX: self.n_cells, FIELD_MCC_REM: self.n_cells, FIELD_ION_SMAX: self.n_cells,
-             FIELD_ION_REM: self.n_cells}[field]
+             FIELD_ION_REM: self.n_cells, FIELD_BINARY_SMAX: self.n_cells, FIELD_BINARY_REM: self.n_cells,
+             FIELD_BINARY_BG_REM: self.n_cells, FIELD_ION_PARCEL_SMAX: self.n_cells,
+             FIELD_ION_PARCEL_REM: self.n_cells}[field]
         out = np.empty(n)
         self._check(self.lib.pic_get_cell_field(self.h, field, _ptr(out)))
         return out.reshape(-1, 3) if field in (FIELD_E, FIELD_B, FIELD_E_POINT) else out
@@ -399,6 +434,11 @@ class Cloud:
 
     def ion_mcc_initialize(self, maximum_energy, energy_step):
         self._check(self.lib.pic_ion_mcc_initialize(self.h, maximum_energy, energy_step))
+
+    def binary_initialize(self, temperatures, number_densities):
+        """BinaryCollisionModel::initialize: per species temperature [K] and number density"""
+        T = np.ascontiguousarray(temperatures, np.float64); n = np.ascontiguousarray(number_densities, np.float64)
+        self._check(self.lib.pic_binary_initialize(self.h, _ptr(T), _ptr(n)))
 
     def mcc_initialize_temperature(self, model, temperature):
         """model 0 electron-neutral, 1 ion-neutral; temperature [K]"""
@@ -457,6 +497,9 @@ class Cloud:
 
     def collide_ion_neutral(self, dt):
         self._check(self.lib.pic_collide_ion_neutral(self.h, dt))
+
+    def collide_binary(self, dt):
+        self._check(self.lib.pic_collide_binary(self.h, dt))
 
     def deposit(self):
         self._check(self.lib.pic_deposit(self.h))

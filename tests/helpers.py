@@ -37,6 +37,24 @@ def random_parcels(rng, n_tets, n, vmax):
     return y, tet, U, ids
 
 
+def cell_parcels(rm_or_tet_cell, counts, T, mass, rng, id0=0, drift=(0.0, 0.0, 0.0)):
+    """counts[c] parcels in cell c (sorted by cell) at random points of the cell's tets,
+    Maxwellian at T; rm_or_tet_cell is an oracle mesh or its tet -> cell table.  Returns
+    dict(tet, y, U, ids, cs) with cs the cell starts"""
+    tc = rm_or_tet_cell if isinstance(rm_or_tet_cell, np.ndarray) else rm_or_tet_cell.tets()[0]
+    counts = np.asarray(counts, np.int64)
+    n = int(counts.sum())
+    n_cells = len(counts)
+    cell = np.repeat(np.arange(n_cells), counts)
+    tets_of = [np.nonzero(tc == c)[0] for c in range(n_cells)]
+    tet = np.array([tets_of[c][rng.integers(len(tets_of[c]))] for c in cell], np.int32)
+    y = random_bary(rng, n)
+    U = rng.normal(size=(n, 3)) * np.sqrt(KB * T / mass) + np.asarray(drift)
+    ids = (id0 + rng.permutation(n)).astype(np.uint32)
+    cs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    return {"tet": tet, "y": y, "U": U, "ids": ids, "cs": cs}
+
+
 def sort_by_id(ids, *arrays):
     o = np.argsort(ids, kind="stable")
     return [a[o] for a in arrays]

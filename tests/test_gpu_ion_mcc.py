@@ -123,7 +123,8 @@ def test_ion_mcc_parity(oracle, case):
 def test_ion_mcc_in_the_step_counts_and_initialisations(oracle):
     """pic_step with ion-neutral collisions after the move: the step statistics equal the
     oracle's counts for the moved and sorted ions; FindMaximum and Temperature
-    initialisations of (sigma cR)_max equal their closed forms; neutral parcels are refused."""
+    initialisations of (sigma cR)_max equal their closed forms; neutral parcels add the
+    parcel pass."""
     say("ion mcc in the step")
     from picfoam_amd import picmi
     rng = np.random.default_rng(311)
@@ -164,9 +165,14 @@ def test_ion_mcc_in_the_step_counts_and_initialisations(oracle):
     (Ug_s,) = sort_by_id(ig, Ug)
     (Uc_s,) = sort_by_id(ids[order], Uc)
     assert rel_err(Ug_s, Uc_s) < TOL
-    # ion-neutral collisions with neutral parcels are not implemented: refused, not skipped
-    c.set_particles(2, y[:4], tet[:4], np.zeros((4, 3)), np.arange(4, dtype=np.uint32))
-    with pytest.raises(picmi.PicError) as e_:
-        c.step(dt)
-    assert "neutral parcels" in str(e_.value)
+    # with parcels of the neutral species the ions meet them first (IonNeutralCollisionModel.C:
+    # 246-254): the step counts those candidates too
+    # 16 neutrals where the first 16 ions were (cell 0, 60 ions): (sigma cR)_max sized for
+    # about 100 candidates there, n_N n_I w max dt / V
+    c.set_particles(2, y[:16], tet[:16], np.zeros((16, 3)), np.arange(16, dtype=np.uint32))
+    V = rm.geometry()[1][0]
+    dt2 = 1e-8
+    c.set_field(picmi.FIELD_ION_PARCEL_SMAX, np.full(rm.n_cells, 100.0 * V / (16 * 60 * dt2)))
+    st = c.step(dt2)
+    assert 40 < st.ion_parcel_candidates < 200
     c.close()
