@@ -1755,7 +1755,10 @@ int ref_binary(const ref_mesh* m, const ref_pair_params* p, double dt, double* s
                     double y0 = 1.0 - (y1 + y2 + y3);
                     v3 pos = vadd(vadd(vscale(y0, X[0]), vscale(y1, X[1])), vadd(vscale(y2, X[2]), vscale(y3, X[3])));
                     v3 rel = vsub(pos, cC);
-                    sub[r] = (rel.x >= 0.0) + 2 * (rel.y >= 0.0) + 4 * (rel.z >= 0.0);
+                    /* an empty direction counts as zero: reduced-dimension parcels sit on the
+                     * mid-plane, where the sign of the difference is rounding */
+                    int em = m->track_mask;
+                    sub[r] = ((em & 1) || rel.x >= 0.0) + 2 * ((em & 2) || rel.y >= 0.0) + 4 * ((em & 4) || rel.z >= 0.0);
                 }
                 int sst[9], at = 0;
                 for (int sc = 0; sc < 8; ++sc) {

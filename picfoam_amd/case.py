@@ -122,7 +122,8 @@ class Case:
         self._check(self.lib.pic_case_describe(self.h, C.byref(mesh), C.cast(sp, C.c_void_p), C.byref(models)))
         species = [{"name": self.species_names[i], "mass": sp[i].mass, "charge": sp[i].charge,
                     "nEquivalentParticles": sp[i].n_equivalent, "solveMovement": bool(sp[i].solve_movement),
-                    "parcels": int(sp[i].capacity)} for i in range(self.n_species)]
+                    "parcels": int(sp[i].capacity), "diameter": sp[i].diameter, "omega": sp[i].omega}
+                   for i in range(self.n_species)]
         return mesh, species, models
 
     def mesh_dict(self):

@@ -695,6 +695,7 @@ PairArgs<UT> pair_args(pic_ctx* c, const std::vector<int>& list, int na, double 
     a.tets = c->d_tets;
     a.C = c->d_C;
     a.pts = c->d_points;
+    a.empty_mask = c->hm.track_mask;
     a.dt = dt;
     a.seed = c->seed;
     a.step = c->step;

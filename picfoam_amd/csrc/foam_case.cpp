@@ -151,6 +151,12 @@ struct pic_case {
     };
     std::vector<double> smax[2];
     XsInit xs_init[2];
+    // the binary model's picSigmaTcRMax and the ion-neutral parcel field
+    // picSigmaTcRMax_IonNeutral as read from the start time (READ_IF_PRESENT), and the
+    // picInitialise temperatures and number densities BinaryCollisionModel::initialize takes
+    std::vector<double> smax_bin, smax_ip;
+    bool binary_init = false;
+    std::vector<double> init_T, init_n;
     // time
     double start = 0.0, end = 0.0, dt = 0.0, write_interval = 0.0, now = 0.0;
     bool write_steps = false;  // writeControl timeStep
@@ -633,6 +639,10 @@ void read_pic_properties(Case& c, const std::string& path) {
         sp.charge = sd.scalar("charge");
         sp.n_equivalent = sd.scalar("nEquivalentParticles");
         sp.solve_movement = sd.boolean_or("solveMovement", true) ? 1 : 0;
+        // the hard-sphere diameter and viscosity index of the binary collision models
+        // (PICParcelI.H:45-46)
+        sp.diameter = sd.scalar_or("diameter", 0.0);
+        sp.omega = sd.scalar_or("omega", 0.0);
         sp.capacity = 0;
         sp.sort_interval = 0;
         c.species.push_back(sp);
@@ -641,6 +651,7 @@ void read_pic_properties(Case& c, const std::string& path) {
     std::vector<std::pair<const char**, std::string>> names;
     pic_models& m = c.models;
     m = pic_models{};
+    m.bg_species = -1;
     names.push_back({&m.particle_pusher, d.word("ParticlePusher")});
     names.push_back({&m.field_weighting, ss.word("FieldWeigthing")});
     names.push_back({&m.charge_distribution, ss.word("ChargeDistribution")});
@@ -664,7 +675,20 @@ void read_pic_properties(Case& c, const std::string& path) {
     // SolverSettings.syncVelocityAtBoundary, true by default (PICCloud.C:926)
     m.no_boundary_sync = ss.boolean_or("syncVelocityAtBoundary", true) ? 0 : 1;
     const Dict& cm = d.sub_required("CollisionModels");
-    require_none(cm, "BinaryCollisionModel", "neutral DSMC collisions are not implemented");
+    // BinaryCollisionModel (BinaryCollisionModelNew.C:39-61) with <model>Coeffs: the
+    // TotalCrossSectionModel (TotalCrossSectionModelNew.C:39, VariableHardSphereCoeffs.Tref,
+    // VariableHardSphere.C:39-45) and the WeightCorrection (WeightCorrectionModelNew.C:42)
+    const std::string bin = cm.word_or("BinaryCollisionModel", "none");
+    names.push_back({&m.binary_collision, bin});
+    std::string bxs = "HardSphere", bwc = "none";
+    if (bin != "none") {
+        const Dict& bc = cm.sub_required(bin + "Coeffs");
+        bxs = bc.word("TotalCrossSectionModel");
+        bwc = bc.word("WeightCorrection");
+        if (bxs == "VariableHardSphere") m.vhs_tref = bc.sub_required("VariableHardSphereCoeffs").scalar("Tref");
+    }
+    names.push_back({&m.total_cross_section, bxs});
+    names.push_back({&m.binary_weight_correction, bwc});
     // SpeciesRelations (PICCloud.C:96-130)
     m.electron_species = -1;
     m.ion_species = -1;
@@ -737,6 +761,7 @@ void read_pic_properties(Case& c, const std::string& path) {
         for (int k = 0; k < 3; ++k) m.bg_velocity[k] = v[k];
         gas = species_index(c, bc.word("species"), bc.where);
         m.bg_mass = c.species[gas].mass;
+        m.bg_species = gas;
         // the background's ion: the SpeciesRelations entry naming the gas as its neutral
         // (PICCloud::ionTypeId)
         if (const Dict* rel = mp.sub("SpeciesRelations"))
@@ -754,9 +779,9 @@ void read_pic_properties(Case& c, const std::string& path) {
     // but for elastic scattering, threshold; ListCrossSection.C:118-310, energies and values,
     // the ion charge exchange's list under the keyword "value")
     auto read_xs = [&](const Dict& coeffs, const char* what, int first, const char* const* kinds,
-                       const char* const* tags, int nproc) {
+                       const char* const* tags, int nproc, int neutral_sp) {
         const Dict& xm = coeffs.sub_required("CrossSectionModels");
-        const Dict& xg = xm.sub_required(c.species_names[gas]);
+        const Dict& xg = xm.sub_required(c.species_names[neutral_sp]);
         for (int p = 0; p < nproc; ++p) {
             const std::string model = xg.word_or(kinds[p], "none");
             if (model == "none") continue;
@@ -791,18 +816,25 @@ void read_pic_properties(Case& c, const std::string& path) {
         m.create_electron = ec.boolean_or("createElectron", true) ? 1 : 0;
         const char* kinds[3] = {"ElasticCrossSection", "ExcitationCrossSection", "IonizationCrossSection"};
         const char* tags[3] = {"Elastic", "Excitation", "Ionization"};
-        read_xs(ec, "electron-neutral", PIC_XS_ELASTIC, kinds, tags, 3);
+        read_xs(ec, "electron-neutral", PIC_XS_ELASTIC, kinds, tags, 3, gas);
     }
+    std::string iwc = "none";
     if (in != "none") {
-        if (gas < 0)
-            throw FoamError(cm.where + ": ion-neutral collisions with neutral parcels are not implemented; "
-                            "use BackgroundGasModel uniformBackground");
+        // one neutral species (IonNeutralCollisionModel.C:170-175): the background's, else the
+        // species without charge; its cross sections serve the parcels and the background
+        int nsp = gas;
+        if (nsp < 0)
+            for (size_t s = 0; s < c.species.size(); ++s)
+                if (c.species[s].charge == 0.0) { nsp = (int)s; break; }
+        if (nsp < 0) throw FoamError(cm.where + ": IonNeutralCollisionModel needs a neutral species or a background gas");
         const Dict& ic = cm.sub_required("IonNeutralCollisionCoeffs");
-        require_none(ic, "WeightCorrection", "weight correction of ion-neutral collisions (neutral parcels) is not implemented");
+        iwc = ic.word_or("WeightCorrection", "none");
+        m.ion_neutral_keep_neutral = ic.boolean_or("updateNeutralVelocity", true) ? 0 : 1;
         const char* kinds[2] = {"ElasticCrossSection", "ChargeExchangeCrossSection"};
         const char* tags[2] = {"Elastic", "ChargeExchange"};
-        read_xs(ic, "ion-neutral", PIC_XS_ION_ELASTIC, kinds, tags, 2);
+        read_xs(ic, "ion-neutral", PIC_XS_ION_ELASTIC, kinds, tags, 2, nsp);
     }
+    names.push_back({&m.ion_neutral_weight_correction, iwc});
     (void)neutral;
     m.precision = PIC_PREC_FP64;
     m.seed = 12345;
@@ -895,25 +927,32 @@ void open_case(Case& c) {
     }
     read_wall_fields(c, t0);
     if (picmi::foam::file_exists(t0 + "/E")) c.E = read_vector_field(t0 + "/E", c.n_cells);
+    // a (sigma cR)_max field of the start time, if present and positive
+    auto read_smax = [&](const std::string& f, std::vector<double>& out) {
+        out.clear();
+        if (!picmi::foam::file_exists(f)) return;
+        auto d = picmi::foam::read_dict_file(f);
+        Reader r = d->value("internalField");
+        const std::string kind = r.word();
+        if (kind == "uniform") {
+            const double v = r.number();
+            if (v > 0.0) out.assign(c.n_cells, v);
+        } else {
+            r.word();
+            r.list_open();
+            while (!r.accept(')')) out.push_back(r.number());
+            if ((int)out.size() != c.n_cells) r.fail("internalField size differs from the cell count");
+        }
+    };
+    c.smax_bin.clear();
+    c.smax_ip.clear();
+    if (std::string(c.models.binary_collision) != "none") read_smax(t0 + "/picSigmaTcRMax", c.smax_bin);
+    if (std::string(c.models.ion_neutral_collision) != "none") read_smax(t0 + "/picSigmaTcRMax_IonNeutral", c.smax_ip);
     for (int k = 0; k < 2; ++k) {
         const bool on = std::string(k == 0 ? c.models.electron_neutral : c.models.ion_neutral_collision) != "none";
-        const std::string f = t0 + (k == 0 ? "/picSigmaTcRMax" : "/picSigmaTcRMax_IonBackground");
         c.smax[k].clear();
         if (!on) continue;
-        if (picmi::foam::file_exists(f)) {
-            auto d = picmi::foam::read_dict_file(f);
-            Reader r = d->value("internalField");
-            const std::string kind = r.word();
-            if (kind == "uniform") {
-                const double v = r.number();
-                if (v > 0.0) c.smax[k].assign(c.n_cells, v);
-            } else {
-                r.word();
-                r.list_open();
-                while (!r.accept(')')) c.smax[k].push_back(r.number());
-                if ((int)c.smax[k].size() != c.n_cells) r.fail("internalField size differs from the cell count");
-            }
-        }
+        read_smax(t0 + (k == 0 ? "/picSigmaTcRMax_ElectronBackground" : "/picSigmaTcRMax_IonBackground"), c.smax[k]);
         double emax = 0.0;
         for (const Case::CrossSection& x : c.xs)
             if ((x.process >= PIC_XS_ION_ELASTIC) == (k == 1) && x.e.back() < 1e29) emax = std::max(emax, x.e.back());
@@ -1333,6 +1372,20 @@ void initialise(Case& c, const std::string& name, uint64_t seed) {
                 T[s] = c.species[s].mass / (3.0 * kBoltzmann) *
                        (m2 / (double)np - (m1[0] * m1[0] + m1[1] * m1[1] + m1[2] * m1[2]));
             }
+        // number densities from the parcels for the models that compute them
+        // (UniformDisplacement, QuasineutralPlasma, NoParticle, ParticleList; picInitialise.C:
+        // 172-200): the species' real particles over the mesh volume
+        if (model == "UniformDisplacement" || model == "QuasineutralPlasma" || model == "NoParticle" ||
+            model == "ParticleList") {
+            double vol = 0.0;
+            for (int k = 0; k < c.hm.n_cells; ++k) vol += c.hm.V[k];
+            for (size_t s = 0; s < c.species.size(); ++s)
+                n[s] = (double)c.parcels[s].size() * c.species[s].n_equivalent / vol;
+        }
+        // BinaryCollisionModel::initialize (BinaryCollisionModel.C:540-600)
+        c.binary_init = std::string(c.models.binary_collision) != "none";
+        c.init_T = T;
+        c.init_n = n;
         const char* key[2] = {"electronNeutral", "ionNeutral"};
         const int sp[2] = {c.models.electron_species, c.models.ion_species};
         for (int k = 0; k < 2; ++k) {
@@ -1805,6 +1858,17 @@ int pic_case_create(pic_case* c, int32_t precision, double factor, pic_ctx** out
                 r = k == 0 ? pic_mcc_initialize(ctx, xs.emax, xs.estep) : pic_ion_mcc_initialize(ctx, xs.emax, xs.estep);
         }
         if (r != PIC_OK) return fail(r, "MCC (sigma cR)max");
+        // the ion-neutral parcel field as read, and the binary model's field: initialised by
+        // picInitialise, else as read (else zero, as the reference leaves it)
+        if (!c->smax_ip.empty() && std::string(c->models.ion_neutral_collision) != "none")
+            r = pic_set_cell_field(ctx, PIC_FIELD_ION_PARCEL_SIGMA_CR_MAX, c->smax_ip.data());
+        if (r == PIC_OK && std::string(c->models.binary_collision) != "none") {
+            if (c->binary_init)
+                r = pic_binary_initialize(ctx, c->init_T.data(), c->init_n.data());
+            else if (!c->smax_bin.empty())
+                r = pic_set_cell_field(ctx, PIC_FIELD_BINARY_SIGMA_CR_MAX, c->smax_bin.data());
+        }
+        if (r != PIC_OK) return fail(r, "binary / ion-neutral (sigma cR)max");
         if ((r = pic_deposit(ctx)) != PIC_OK) return fail(r, "charge density");
         if (c->post.active) {
             // picInitialise.C:213-237

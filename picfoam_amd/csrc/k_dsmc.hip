@@ -424,10 +424,13 @@ __device__ void pair_cell(const PairArgs<UT>& a, int cell, const Plan& p, PairLd
             const double* p1 = a.pts + 3 * (size_t)g.vid[0];
             const double* p2 = a.pts + 3 * (size_t)g.vid[1];
             const double* p3 = a.pts + 3 * (size_t)g.vid[2];
+            // an empty direction counts as zero (the upper side): the parcels of a
+            // reduced-dimension case sit on the mesh's mid-plane, where the sign of the
+            // difference is rounding
             int sc = 0;
             for (int d = 0; d < 3; ++d) {
                 const double x = (y0 * Cc[d] + y1 * p1[d]) + (y2 * p2[d] + y3 * p3[d]);
-                sc |= (x - Cc[d] >= 0.0 ? 1 : 0) << d;
+                sc |= (((a.empty_mask >> d) & 1u) || x - Cc[d] >= 0.0 ? 1 : 0) << d;
             }
             L.sub[r] = (unsigned char)sc;
         }

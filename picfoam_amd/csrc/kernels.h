@@ -378,6 +378,7 @@ struct PairArgs {
     const TetGeo* tets;           // sub-cells: parcel positions against the cell centre
     const double* C;
     const double* pts;
+    uint32_t empty_mask;          // bit d: direction d is empty (its position component is not tested)
     int bg;                       // collisions with the uniformBackground gas
     double nN, vthN, drift[3], massN;
     // ion-neutral: elastic [0] and charge exchange [1] cross sections

@@ -55,6 +55,24 @@ def cell_parcels(rm_or_tet_cell, counts, T, mass, rng, id0=0, drift=(0.0, 0.0, 0
     return {"tet": tet, "y": y, "U": U, "ids": ids, "cs": cs}
 
 
+def hs_relaxation(n1, n2, m1, m2, d12, T1, T2, t):
+    """Temperatures of a two-species hard-sphere mixture relaxing by isotropic scattering,
+    Maxwellians at T1 and T2: dT1/dt = -(2/3) n2 pi d12^2 (m1 m2 / M^2) 8 sqrt(2/pi) s (T1 - T2)
+    with s^2 = k T1 / m1 + k T2 / m2, and symmetrically for T2 (E[V.g | g] = k (T1 - T2) g^2 /
+    (M s^2) and E|g|^3 = 8 sqrt(2/pi) s^3 for the relative velocity g of the two Maxwellians).
+    Returns [T1(t), T2(t)]."""
+    from scipy import integrate
+    sig = np.pi * d12 ** 2
+    M = m1 + m2
+    k = (2.0 / 3.0) * sig * (m1 * m2 / M ** 2) * 8 * np.sqrt(2 / np.pi)
+
+    def f(_, T):
+        s = np.sqrt(max(KB * T[0] / m1 + KB * T[1] / m2, 0.0))
+        return [-k * n2 * s * (T[0] - T[1]), -k * n1 * s * (T[1] - T[0])]
+
+    return integrate.solve_ivp(f, (0, t[-1]), [T1, T2], t_eval=t, rtol=1e-10, atol=1e-6, max_step=t[-1] / 200).y
+
+
 def sort_by_id(ids, *arrays):
     o = np.argsort(ids, kind="stable")
     return [a[o] for a in arrays]

@@ -194,12 +194,19 @@ def test_pic_properties_map_to_the_models():
         (b"VolumeWeighting", b"VolumeWeighting", b"ElectroStatic")
     assert m.poisson_tolerance == 1e-10    # system/fvSolution solvers.phiE.tolerance
     assert [s["solveMovement"] for s in sp] == [True, False]
+    # the binary collision model and the species' hard-sphere sizes
+    b = case(os.path.join(CASES, "thermalization_binary"))
+    _, sp, m = b.describe()
+    assert [(s["diameter"], s["omega"]) for s in sp] == [(3.08e-10, 1.0), (3.76e-10, 1.0)]
+    assert (m.binary_collision, m.total_cross_section, m.binary_weight_correction) == \
+        (b"IsotropicScattering", b"HardSphere", b"Nanbu")
+    assert b.times == (0.0, 2e-7, 1e-10, 1e-7)
 
 
 @pytest.mark.parametrize("change,needle", [
     ("ParticleMerging PairMerging;\n", "merging"),
-    ("CollisionModels { BinaryCollisionModel VariableHardSphere; }\n", "DSMC"),
-    ("CollisionModels { IonNeutralCollisionModel ElasticCollision; }\n", "ion-neutral"),
+    ("CollisionModels { BinaryCollisionModel IsotropicScattering; }\n", "IsotropicScatteringCoeffs"),
+    ("CollisionModels { IonNeutralCollisionModel ElasticCollision; }\n", "IonNeutralCollisionModel"),
     ("CollisionModels { ElectronNeutralCollisionModel NanbuElectronNeutral; BackgroundGasModel none; }\n",
      "neutral parcels"),
 ])

@@ -7,7 +7,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import rel_err, sort_by_id
+from helpers import hs_relaxation, rel_err, sort_by_id
 
 pytestmark = pytest.mark.gpu
 
@@ -307,5 +307,96 @@ def test_ion_drift_case_matches_oracle_and_drifts(oracle, tmp_path):
     assert abs(drift - 2158.0) < 0.03 * 2158.0
     assert cloud.count(1) > 4900
     for tn in ("1e-07", "1.1e-07"):
+        assert (d / tn / "lagrangian" / "pic" / "U").is_file(), tn
+    cloud.close()
+
+
+def test_binary_thermalization_case_matches_oracle_and_relaxes(oracle, tmp_path):
+    """Neon at 1000 K and argon at 500 K, 1e22 m^-3 each, parcels of 200 and 400 atoms (the
+    reference's thermalization_binaryCollision settings on its 8 x 1 x 8 slab): the
+    Equipartition start and BinaryCollisionModel::initialize ((sigma_T cR)_max = pi d_Ne^2
+    sqrt(2 k 1000 K / m_Ne): neon is the first of the two equally dense species); ten steps
+    of the loop (about 100 collisions) equal the oracle's move, cell order and binary collisions (HardSphere,
+    sub-cells, Nanbu weight correction) parcel by parcel; the run to endTime (2000 steps)
+    relaxes the temperature difference at the closed-form hard-sphere rate, within 6 % of its
+    initial value (about 1 % of T per species is the sampling spread at 17,000 and 8,000
+    parcels)."""
+    from picfoam_amd import picmi
+    from picfoam_amd.case import Case
+    d = copy_case("thermalization_binary", tmp_path)
+    cs = Case(d)
+    cs.initialise(seed=2)
+    _, _, dt, _ = cs.times
+    _, species, models = cs.describe()
+    cloud = cs.create("fp64", capacity_factor=1.0)
+    rm = oracle.RefMesh(cs.mesh_dict())
+    kB = 1.380649e-23
+    m_ne, m_ar = species[0]["mass"], species[1]["mass"]
+    d_ne, d_ar = species[0]["diameter"], species[1]["diameter"]
+    smax0 = cloud.get_field(picmi.FIELD_BINARY_SMAX).copy()
+    assert np.allclose(smax0, np.pi * d_ne ** 2 * np.sqrt(2 * kB * 1000.0 / m_ne), rtol=1e-14)
+    rem = cloud.get_field(picmi.FIELD_BINARY_REM).copy()
+    smax = smax0.copy()
+    st = [cloud.get_particles(s) for s in range(2)]
+    ys, ts, Us, ids = [list(v) for v in zip(*st)]
+    Z = np.zeros((rm.n_cells, 3))
+    P = {"seed": models.seed, "step": 0, "WeightCorrection": "Nanbu"}
+    say("case thermalization_binary: 10 steps against the oracle")
+    coll = 0
+    for step in range(10):
+        stats = cloud.step(dt)
+        css = []
+        for s in range(2):
+            # neutral parcels are not pushed (PICParcel.C:55)
+            ys[s], Us[s], ts[s], _, _ = rm.move("Boris", "CellAverage", ys[s], Us[s], ts[s], 0.0, species[s]["mass"], dt,
+                                               Z, None, Z, do_push=False)
+            order, cs_ = rm.sort_by_cell(ts[s])
+            ys[s], ts[s], Us[s], ids[s] = ys[s][order], ts[s][order], Us[s][order], ids[s][order]
+            css.append(cs_)
+        sp = [{"tet": ts[s], "y": ys[s], "U": Us[s], "ids": ids[s], "cs": css[s], "mass": species[s]["mass"],
+               "w": species[s]["nEquivalentParticles"], "d": species[s]["diameter"], "omega": species[s]["omega"]}
+              for s in range(2)]
+        P["step"] = step
+        Us, smax, rem, _, cnt = oracle.binary(rm, P, dt, sp, smax, rem, rem)
+        assert (stats.binary_candidates, stats.binary_collisions) == (cnt[0], cnt[1])
+        coll += cnt[1]
+    for s in range(2):
+        yg, tg, Ug, ig = cloud.get_particles(s)
+        (Ug_s, ig_s) = sort_by_id(ig, Ug, ig)
+        (Ur_s, ir_s) = sort_by_id(ids[s], Us[s], ids[s])
+        assert np.array_equal(ig_s, ir_s)
+        err = rel_err(Ug_s, Ur_s)
+        say(f"  species {s}: U rel err after 10 steps {err:.2e} ({coll} collisions)")
+        assert err < 1e-11
+    assert coll > 50
+    assert np.allclose(cloud.get_field(picmi.FIELD_BINARY_SMAX), smax, rtol=1e-14)
+    cloud.close()
+    # the loop to endTime, temperatures every 100 steps
+    cloud = cs.create("fp64", capacity_factor=1.0)
+
+    def temps():
+        out = []
+        for s in range(2):
+            _, _, U, _ = cloud.get_particles(s)
+            out.append(species[s]["mass"] / (3 * kB) * np.mean(np.sum((U - U.mean(0)) ** 2, axis=1)))
+        return out
+
+    T = [temps()]
+    for k in range(1, 21):
+        cs.run(cloud, until=k * 100 * dt)
+        T.append(temps())
+    T = np.array(T)
+    vol = rm.geometry()[1].sum()
+    n_ne = cloud.count(0) * species[0]["nEquivalentParticles"] / vol
+    n_ar = cloud.count(1) * species[1]["nEquivalentParticles"] / vol
+    t = 100 * dt * np.arange(21)
+    ref = hs_relaxation(n_ne, n_ar, m_ne, m_ar, 0.5 * (d_ne + d_ar), T[0, 0], T[0, 1], t)
+    dT0 = T[0, 0] - T[0, 1]
+    dev = (T[:, 0] - T[:, 1] - (ref[0] - ref[1])) / dT0
+    say(f"  Ne {T[0, 0]:.1f} -> {T[-1, 0]:.1f} K, Ar {T[0, 1]:.1f} -> {T[-1, 1]:.1f} K; closed form "
+        f"{ref[0, -1]:.1f} / {ref[1, -1]:.1f} K; largest deviation {np.max(np.abs(dev)):.3f} of dT0")
+    assert (ref[0, -1] - ref[1, -1]) / dT0 < 0.5
+    assert np.max(np.abs(dev)) < 0.06
+    for tn in ("1e-07", "2e-07"):
         assert (d / tn / "lagrangian" / "pic" / "U").is_file(), tn
     cloud.close()

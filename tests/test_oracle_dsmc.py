@@ -22,9 +22,8 @@ dT1/dt = -(2/3) n2 pi d12^2 (m1 m2 / M^2) 8 sqrt(2/pi) s (T1 - T2), and symmetri
 import math
 
 import numpy as np
-from scipy import integrate
 
-from helpers import KB, cell_parcels
+from helpers import KB, cell_parcels, hs_relaxation
 
 AMU = 1.66053906660e-27
 M_NE = 3.375350746e-26
@@ -149,16 +148,7 @@ def test_subcell_partners(oracle):
 
 
 def relaxation_ode(n1, n2, m1, m2, d12, T1, T2, t):
-    sig = math.pi * d12 ** 2
-    M = m1 + m2
-    k = (2.0 / 3.0) * sig * (m1 * m2 / M ** 2) * 8 * math.sqrt(2 / math.pi)
-
-    def f(_, T):
-        s = math.sqrt(max(KB * T[0] / m1 + KB * T[1] / m2, 0.0))
-        return [-k * n2 * s * (T[0] - T[1]), -k * n1 * s * (T[1] - T[0])]
-
-    return integrate.solve_ivp(f, (0, t[-1]), [T1, T2], t_eval=t, rtol=1e-10, atol=1e-6,
-                               max_step=t[-1] / 200).y
+    return hs_relaxation(n1, n2, m1, m2, d12, T1, T2, t)
 
 
 def run_mixture(oracle, w_ne, w_ar, steps, dt, seed):
