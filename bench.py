@@ -273,6 +273,40 @@ def run_walls(steps, warmup, ncell=100, ppc=8):
     return out
 
 
+def run_binary(steps, warmup, ncell=100):
+    """Neutral DSMC (BinaryCollisionModel, HardSphere, sub-cells, Nanbu weight correction) on
+    the neon-argon gas of the reference's thermalization_binaryCollision case: 100^3 cells of
+    1.875 um, 26 Ne + 13 Ar parcels per cell (39M parcels), dt 1e-10 s.  The step is the
+    neutral move (no push), the sort and the pair pass (timed as the collision stage)."""
+    from picfoam_amd import cases, picmi
+    mesh, species, models, ex = cases.binary_mixture(ncell=ncell)
+    c = picmi.Cloud(mesh, species, models)
+    for s in range(2):
+        c.init_maxwellian(s, ex["ppc"][s], ex["T"][s], id0=0, seed=21 + s)
+    c.binary_initialize(list(ex["T"]), list(ex["n"]))
+    dt = ex["dt"]
+    n = c.count(0) + c.count(1)
+    c.run(dt, warmup)
+    c.sync()
+    t0 = time.perf_counter()
+    st = c.run(dt, steps)
+    c.sync()
+    el = time.perf_counter() - t0
+    if st.binary_collisions <= 0:
+        raise RuntimeError("no binary collisions happened in the timed steps")
+    c.profile(True)
+    st2 = c.run(dt, steps)
+    c.sync()
+    stages = {k: round(v[0] / steps, 4) for k, v in c.stage_times().items() if v[1]}
+    c.profile(False)
+    c.close()
+    return {"particle_steps_per_s": n * steps / el, "ms_per_step": 1e3 * el / steps, "parcels": n,
+            "cells": ncell ** 3, "candidates_per_step": st.binary_candidates / steps,
+            "collisions_per_step": st.binary_collisions / steps,
+            "stage_ms_per_step_next_window": stages,
+            "note": "the collision stage (\"mcc\") is the binary pair pass; no field solve"}
+
+
 def poisson_benchmark(ncell, ppc, snaps, train_steps, evals, warm, loss_kind=0, guesses=("previous", "extrapolate")):
     """Poisson ms per step (right-hand side, initial guess, PCG to ||r||/||b|| <= 1e-8, gradient)
     on the field-solve workload, for four initial guesses: the previous phi (the reference's
@@ -422,11 +456,12 @@ def main():
     ap.add_argument("--no-tets", action="store_true", help="skip the magnetised tet-lattice workload")
     ap.add_argument("--no-poisson", action="store_true", help="skip the Poisson initial-guess comparison")
     ap.add_argument("--no-walls", action="store_true", help="skip the open-faces / walls comparison")
+    ap.add_argument("--no-binary", action="store_true", help="skip the neutral DSMC workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the mixed headline run alone (for profiling)")
     ap.add_argument("--only-vw-field-solve", action="store_true",
                     help="the VolumeWeighting field-solve workload alone (for profiling the deposit)")
-    ap.add_argument("--only", choices=["thermalisation", "magnetised-tets", "poisson", "walls"],
+    ap.add_argument("--only", choices=["thermalisation", "magnetised-tets", "poisson", "walls", "binary"],
                     help="one sub-workload alone, printed as its JSON object (for profiling): the thermalisation "
                          "in --precision (default fp64), the magnetised tets, or the Poisson initial-guess comparison")
     ap.add_argument("--precision", default=None, choices=["mixed", "fp64"],
@@ -447,7 +482,7 @@ def main():
         sys.exit(2)
     if args.only_headline:
         args.no_fp64 = args.no_thermalisation = args.no_tets = args.no_poisson = args.no_cpu_baseline = True
-        args.no_walls = True
+        args.no_walls = args.no_binary = True
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
     if args.only:
@@ -457,6 +492,8 @@ def main():
             r = run_magnetised_tets(steps=args.steps, warmup=args.warmup)
         elif args.only == "walls":
             r = run_walls(args.steps, args.warmup)
+        elif args.only == "binary":
+            r = run_binary(args.steps, args.warmup)
         else:
             r = poisson_benchmark(args.poisson_ncell, args.poisson_ppc, snaps=48, train_steps=400, evals=24, warm=4,
                                   loss_kind=args.poisson_loss)
@@ -529,6 +566,9 @@ def main():
         print("bench: field solve with open x faces and with MaxwellianThermal walls, 100^3, 8M electrons",
               file=sys.stderr, flush=True)
         wl["walls"] = run_walls(steps=10, warmup=2)
+    if not args.no_binary:
+        print("bench: neutral DSMC, Ne/Ar hard spheres, 100^3 cells, 39M parcels", file=sys.stderr, flush=True)
+        wl["binary_dsmc"] = run_binary(steps=10, warmup=2)
     if wl:
         out["workloads"] = wl
     if not args.no_poisson:

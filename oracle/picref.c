@@ -39,6 +39,16 @@
  *                                                    background Maxwellian width kT/M
  *   relativistic e-n scattering .................. pinned: 4-momentum conservation, Nanbu's
  *                                                    cos chi at v<<c
+ *   ion-neutral MCC (background) ................. pinned: candidate count and remainder, full
+ *                                                    acceptance, elastic share, charge exchange,
+ *                                                    isotropic scattering's conservation laws
+ *   binary DSMC (pairs, sub-cells, background) ... pinned: HardSphere / VariableHardSphere closed
+ *                                                    forms, candidate count, sub-cell partners,
+ *                                                    the closed-form hard-sphere two-temperature
+ *                                                    rate, background rates for HS and VHS
+ *   ion-neutral parcel pairs ..................... pinned: candidate count, acceptance 1/2,
+ *                                                    charge-exchange multiset, conservation,
+ *                                                    updateNeutralVelocity false
  *   Philox4x32-10 ................................ pinned: Random123 known-answer vectors
  *   OpenFOAM-9 Random/shuffle streams, tracking tie-breaks, volPointInterpolation at
  *   non-coupled boundary points

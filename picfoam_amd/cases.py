@@ -84,6 +84,28 @@ def thermalisation(ncell=32, ppc=64, n=1e18, Te_eV=10.0, Ti_eV=1.0, mass_ratio=1
     return mesh, species, models, extra
 
 
+def binary_mixture(ncell=100, ppc_ne=26, ppc_ar=13, n=1e22, T_ne=1000.0, T_ar=500.0, h=1.875e-6, dt=1e-10,
+                   cross_section="HardSphere", precision="fp64"):
+    """Neon and argon relaxing by hard-sphere (or variable-hard-sphere) collisions in an
+    all-cyclic ncell^3 hex box of cells of side h (the reference's
+    thermalization_binaryCollision gas: 1e22 m^-3 of each at 1000 K and 500 K, diameters
+    3.08e-10 and 3.76e-10 m), with parcel weights set by the parcels per cell;
+    IsotropicScattering with the Nanbu weight correction, no field."""
+    mesh = meshgen.hex_box(ncell, ncell, ncell, ncell * h, ncell * h, ncell * h, cyclic=(True, True, True))
+    vol = h ** 3
+    N = ncell ** 3
+    species = [{"name": "Ne", "mass": 3.375350746e-26, "charge": 0.0, "nEquivalentParticles": n * vol / ppc_ne,
+                "capacity": N * ppc_ne, "diameter": 3.08e-10, "omega": 1.0},
+               {"name": "Ar", "mass": 6.681789475e-26, "charge": 0.0, "nEquivalentParticles": n * vol / ppc_ar,
+                "capacity": N * ppc_ar, "diameter": 3.76e-10, "omega": 1.0}]
+    models = {"ParticlePusher": "Boris", "MaxwellSolver": "none", "BinaryCollisionModel": "IsotropicScattering",
+              "BinaryCollisionCoeffs": {"TotalCrossSectionModel": cross_section, "Tref": 273.0,
+                                        "WeightCorrection": "Nanbu" if ppc_ne != ppc_ar else "none"},
+              "precision": precision}
+    extra = {"T": (T_ne, T_ar), "n": (n, n), "dt": dt, "ppc": (ppc_ne, ppc_ar)}
+    return mesh, species, models, extra
+
+
 def spitzer_nu(n, Te_eV, Ti_eV, me, mi, lnL):
     """nu_eq of dT_e/dt = nu_eq (T_i - T_e) for Z=1, n_e = n_i (NRL formulary form)."""
     Te = Te_eV * QE; Ti = Ti_eV * QE
