@@ -58,7 +58,7 @@ enum {
     PIC_FIELD_RHO = 3,          /* [n_cells]   charge density (get only) */
     PIC_FIELD_SPACE_CHARGE = 4, /* [n_cells]   background space charge density */
     PIC_FIELD_E_POINT = 5,      /* [n_points*3] E interpolated to the mesh points (get only) */
-    PIC_FIELD_MCC_SIGMA_CR_MAX = 6, /* [n_cells] background (sigma cR)_max (picSigmaTcRMax) */
+    PIC_FIELD_MCC_SIGMA_CR_MAX = 6, /* [n_cells] electron-background (sigma cR)_max (picSigmaTcRMax_ElectronBackground) */
     PIC_FIELD_MCC_REMAINDER = 7,    /* [n_cells] background candidate remainder */
     PIC_FIELD_ION_SIGMA_CR_MAX = 8, /* [n_cells] ion-background (sigma cR)_max (picSigmaTcRMax_IonBackground) */
     PIC_FIELD_ION_REMAINDER = 9,    /* [n_cells] ion-background candidate remainder */
@@ -219,6 +219,12 @@ typedef struct {
     /* KineticEnergyInfo.C:73-77 relativistic branch: sum (gamma - 1) m c^2, without the
      * parcel weight as in the reference */
     double kinetic_energy_relativistic[8];
+    /* IsotropizationInfo.C:60-66 / UnidirectionalVelocityInfo.C:60-65: sum w U_d U_d per
+     * component; MomentumInfo.C:65-75 relativistic branch: sum m w gamma U; DebyeLengthInfo.C:
+     * 93-99: sum q w */
+    double sum_wUU[8][3];
+    double momentum_relativistic[8][3];
+    double charge[8];
 } pic_diag;
 
 const char* pic_version(void);
@@ -300,6 +306,19 @@ int pic_diag_reduce(pic_ctx* ctx, pic_diag* out);      /* DiagnosticsList::info 
  * species (species = -1), out[n_cells][11]: rhoN, rhoM, picRhoN (parcel count), linearKE,
  * momentum[3], j[3], rhoCharge; densities per m^3.  Sorts the parcels by cell first. */
 int pic_cell_moments(pic_ctx* ctx, int32_t species, double* out);
+/* picFields (postProcess/functionObjects/picFields/picFields.C:97-212) over the time
+ * averages OpenFOAM's fieldAverage keeps of the moment fields: op 0 clears the sums, op 1
+ * adds the current rhoN, rhoM, linearKE and momentum of every species (a sample; the
+ * parcels are sorted by cell first) */
+int pic_fields_average(pic_ctx* ctx, int32_t op);
+/* the means of one species over the samples and the fields picFields derives from them,
+ * out[n_cells][11]: rhoNMean, rhoMMean, linearKEMean, momentumMean[3], UMean[3] =
+ * momentumMean / rhoMMean, translationalT = 2 / (3 k rhoNMean) (linearKEMean - rhoMMean
+ * UMean.UMean / 2), p = k rhoNMean translationalT.  As in the reference the derived fields
+ * are computed only when every cell has rhoNMean and linearKEMean above 1e-300 (*derived =
+ * 1), else they are zero (*derived = 0).  The wall pressure from the force density (fD) is
+ * not built */
+int pic_fields_mean(pic_ctx* ctx, int32_t species, double* out, int32_t* derived);
 int pic_step(pic_ctx* ctx, double dt, pic_step_stats* stats);  /* one PICCloud::evolve() */
 /* n_steps evolve() calls, the time loop of picFoam.C, with one host synchronisation at the
  * end: stats sum the counters over the steps (poisson: the last solve); an error raised on

@@ -127,6 +127,8 @@ struct pic_ctx {
     int64_t* d_nrows = nullptr;
     int* d_big_list = nullptr;               // cells left to the large-cell collision passes
     double* d_moments = nullptr;             // cell moments scratch [n_cells][11]
+    double* d_favg = nullptr;                // picFields time sums [species][n_cells][kFieldAvgVals]
+    int64_t n_favg = 0;                      // samples in them
     int* d_big_n = nullptr;
     PairConst* d_pair = nullptr;             // Coulomb constants per ordered species pair
     void* d_scan_tmp = nullptr;
@@ -1231,7 +1233,7 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     c->d_big_n = c->alloc<int>(1);
     c->scan_bytes = scan_cells_temp_bytes(nc + 1);
     c->d_scan_tmp = c->alloc<char>(c->scan_bytes);
-    c->d_diag_part = c->alloc<double>(1024 * 8);
+    c->d_diag_part = c->alloc<double>(1024 * 12);
     c->d_diag_out = c->alloc<double>(16);
     if (!c->d_tets || !c->d_E || !c->d_diag_out) return c->fail(PIC_ERR_HIP, "out of device memory (mesh)");
     if (c->mcc) {
@@ -1993,7 +1995,7 @@ int pic_diag_reduce(pic_ctx* c, pic_diag* out) {
         hipError_t e = c->mixed() ? launch_diag(soa<float>(S, S.cur), S.n, c->d_diag_part, nb, c->d_diag_out, c->stream)
                                   : launch_diag(soa<double>(S, S.cur), S.n, c->d_diag_part, nb, c->d_diag_out, c->stream);
         if (int r = check_hip(c, e, "diag")) return r;
-        double h[6];
+        double h[12];
         hipMemcpyAsync(h, c->d_diag_out, sizeof(h), hipMemcpyDeviceToHost, c->stream);
         hipStreamSynchronize(c->stream);
         const double w = S.weight;
@@ -2003,6 +2005,11 @@ int pic_diag_reduce(pic_ctx* c, pic_diag* out) {
         out->sum_wU2[s] = h[4] * w;
         out->kinetic_energy[s] = 0.5 * S.mass * h[4] * w;
         out->kinetic_energy_relativistic[s] = h[5] * S.mass * kC * kC;
+        for (int d = 0; d < 3; ++d) {
+            out->sum_wUU[s][d] = h[6 + d] * w;
+            out->momentum_relativistic[s][d] = S.mass * h[9 + d] * w;
+        }
+        out->charge[s] = S.charge * h[0] * w;
     }
     {
         int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (c->hm.n_cells + 255) / 256));
@@ -2016,6 +2023,23 @@ int pic_diag_reduce(pic_ctx* c, pic_diag* out) {
     return PIC_OK;
 }
 
+// the cell moments of species s into c->d_moments (the arrays in cell order first)
+int species_moments(pic_ctx* c, int s) {
+    const int nc = c->hm.n_cells;
+    Species& S = c->sp[s];
+    hipError_t e;
+    if (c->mixed()) {
+        MomentArgs<float> a{nc, soa<float>(S, S.cur), S.start, c->d_V, S.mass, S.charge, S.weight, 4096, c->d_flags,
+                            c->d_moments};
+        e = launch_moments(a, c->stream);
+    } else {
+        MomentArgs<double> a{nc, soa<double>(S, S.cur), S.start, c->d_V, S.mass, S.charge, S.weight, 4096,
+                             c->d_flags, c->d_moments};
+        e = launch_moments(a, c->stream);
+    }
+    return check_hip(c, e, "moments");
+}
+
 int pic_cell_moments(pic_ctx* c, int32_t species, double* out) {
     if (!c || !out || species < -1 || species >= (int)c->sp.size()) return c ? c->fail(PIC_ERR_INVALID, "species") : PIC_ERR_INVALID;
     const int nc = c->hm.n_cells;
@@ -2025,21 +2049,73 @@ int pic_cell_moments(pic_ctx* c, int32_t species, double* out) {
     hipMemsetAsync(c->d_moments, 0, sizeof(double) * (size_t)nc * kMomentVals, c->stream);
     for (int s = 0; s < (int)c->sp.size(); ++s) {
         if (species >= 0 && s != species) continue;
-        Species& S = c->sp[s];
-        hipError_t e;
-        if (c->mixed()) {
-            MomentArgs<float> a{nc, soa<float>(S, S.cur), S.start, c->d_V, S.mass, S.charge, S.weight, 4096, c->d_flags,
-                                c->d_moments};
-            e = launch_moments(a, c->stream);
-        } else {
-            MomentArgs<double> a{nc, soa<double>(S, S.cur), S.start, c->d_V, S.mass, S.charge, S.weight, 4096,
-                                 c->d_flags, c->d_moments};
-            e = launch_moments(a, c->stream);
-        }
-        if (int r = check_hip(c, e, "moments")) return r;
+        if (int r = species_moments(c, s)) return r;
     }
     hipMemcpyAsync(out, c->d_moments, sizeof(double) * (size_t)nc * kMomentVals, hipMemcpyDeviceToHost, c->stream);
     return sync_state(c);
+}
+
+int pic_fields_average(pic_ctx* c, int32_t op) {
+    if (!c || (op != 0 && op != 1)) return c ? c->fail(PIC_ERR_INVALID, "op") : PIC_ERR_INVALID;
+    const int nc = c->hm.n_cells, ns = (int)c->sp.size();
+    const size_t per = (size_t)nc * kFieldAvgVals;
+    if (!c->d_favg) {
+        c->d_favg = c->alloc<double>(per * ns);
+        if (!c->d_favg) return c->fail(PIC_ERR_HIP, "out of device memory (field averages)");
+    }
+    if (op == 0) {
+        hipMemsetAsync(c->d_favg, 0, sizeof(double) * per * ns, c->stream);
+        c->n_favg = 0;
+        return check_hip(c, hipStreamSynchronize(c->stream), "field averages");
+    }
+    if (!c->d_moments) c->d_moments = c->alloc<double>((size_t)nc * kMomentVals);
+    if (!c->d_moments) return c->fail(PIC_ERR_HIP, "out of device memory (moments)");
+    if (int r = do_sort(c)) return r;
+    for (int s = 0; s < ns; ++s) {
+        hipMemsetAsync(c->d_moments, 0, sizeof(double) * (size_t)nc * kMomentVals, c->stream);
+        if (int r = species_moments(c, s)) return r;
+        if (int r = check_hip(c, launch_fields_accumulate(c->d_moments, c->d_favg + per * s, nc, c->stream),
+                              "field averages"))
+            return r;
+    }
+    c->n_favg++;
+    return sync_state(c);
+}
+
+// picFields::write (picFields.C:103-211) from the averages: UMean = momentumMean / rhoMMean,
+// translationalT = 2 / (3 k rhoNMean) (linearKEMean - 1/2 rhoMMean UMean.UMean), p = k
+// rhoNMean translationalT
+int pic_fields_mean(pic_ctx* c, int32_t species, double* out, int32_t* derived) {
+    if (!c || !out || species < 0 || species >= (int)c->sp.size()) return c ? c->fail(PIC_ERR_INVALID, "species") : PIC_ERR_INVALID;
+    if (!c->d_favg || c->n_favg == 0) return c->fail(PIC_ERR_INVALID, "no field averages accumulated");
+    const int nc = c->hm.n_cells;
+    std::vector<double> h((size_t)nc * kFieldAvgVals);
+    hipMemcpyAsync(h.data(), c->d_favg + (size_t)nc * kFieldAvgVals * species, sizeof(double) * h.size(),
+                   hipMemcpyDeviceToHost, c->stream);
+    if (int r = check_hip(c, hipStreamSynchronize(c->stream), "field averages")) return r;
+    const double inv = 1.0 / (double)c->n_favg;
+    bool ok = true;
+    for (int i = 0; i < nc; ++i) {
+        const double* a = &h[(size_t)i * kFieldAvgVals];
+        const double rn = a[0] * inv, rm = a[1] * inv, ke = a[2] * inv;
+        // picFields.C:132: the derived fields are written only when every cell has both
+        ok = ok && std::fabs(rn) > 1e-300 && std::fabs(ke) > 1e-300;
+        double* o = out + (size_t)i * 11;
+        o[0] = rn; o[1] = rm; o[2] = ke;
+        for (int d = 0; d < 3; ++d) o[3 + d] = a[3 + d] * inv;
+    }
+    for (int i = 0; i < nc; ++i) {
+        double* o = out + (size_t)i * 11;
+        double uu = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            o[6 + d] = ok ? o[3 + d] / o[1] : 0.0;
+            uu += o[6 + d] * o[6 + d];
+        }
+        o[9] = ok ? 2.0 / (3.0 * kKB * o[0]) * (o[2] - 0.5 * o[1] * uu) : 0.0;
+        o[10] = ok ? kKB * o[0] * o[9] : 0.0;
+    }
+    if (derived) *derived = ok ? 1 : 0;
+    return PIC_OK;
 }
 
 int pic_sync(pic_ctx* c) {

@@ -330,29 +330,35 @@ hipError_t launch_to_float(const double* src, float* dst, int64_t n, hipStream_t
 }
 
 // ---------------------------------------------------------------- diagnostics
-constexpr int kDiagVals = 6;
-
+// per species: [0] parcels, [1..3] sum U, [4] sum U.U, [5] sum (gamma - 1), [6..8] sum of
+// the squared components, [9..11] sum gamma U (the relativistic momentum per unit mass)
+constexpr int kDiagVals = 12;
 template <class UT>
 __global__ void __launch_bounds__(256) k_diag_partial(PartSoA<UT> p, int64_t n, double* partial) {
-    __shared__ double s[kDiagVals][256];
-    double v[kDiagVals] = {0, 0, 0, 0, 0, 0};
+    __shared__ double s[kDiagVals][4];
+    double v[kDiagVals] = {};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (p.tet[i] < 0) continue;
-        double ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
-        double u2 = ux * ux + uy * uy + uz * uz;
+        const double ux = p.ux[i], uy = p.uy[i], uz = p.uz[i];
+        const double xx = ux * ux, yy = uy * uy, zz = uz * uz;
+        const double u2 = xx + yy + zz;
+        const double b = sqrt(u2) / kC;
+        const double g = 1.0 / sqrt(1.0 - b * b);
         v[0] += 1.0; v[1] += ux; v[2] += uy; v[3] += uz; v[4] += u2;
-        double b = sqrt(u2) / kC;
-        v[5] += 1.0 / sqrt(1.0 - b * b) - 1.0;
+        v[5] += g - 1.0;
+        v[6] += xx; v[7] += yy; v[8] += zz;
+        v[9] += g * ux; v[10] += g * uy; v[11] += g * uz;
     }
-    for (int k = 0; k < kDiagVals; ++k) s[k][threadIdx.x] = v[k];
+    for (int k = 0; k < kDiagVals; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < kDiagVals; ++k) s[k][w] = v[k];
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int k = 0; k < kDiagVals; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + off];
-        __syncthreads();
+    if (threadIdx.x < kDiagVals) {
+        const int k = threadIdx.x;
+        partial[(int64_t)blockIdx.x * kDiagVals + k] = (s[k][0] + s[k][1]) + (s[k][2] + s[k][3]);
     }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < kDiagVals; ++k) partial[(int64_t)blockIdx.x * kDiagVals + k] = s[k][0];
 }
 
 __global__ void k_diag_final(const double* partial, int nb, int nv, double* out) {
@@ -481,5 +487,25 @@ static hipError_t launch_moments_t(const MomentArgs<UT>& a, hipStream_t s) {
 }
 hipError_t launch_moments(const MomentArgs<double>& a, hipStream_t s) { return launch_moments_t(a, s); }
 hipError_t launch_moments(const MomentArgs<float>& a, hipStream_t s) { return launch_moments_t(a, s); }
+
+// the time sums of the picFields averages (the fieldAverage means picFields.C:103-130 reads):
+// acc[c][0..5] += rhoN, rhoM, linearKE, momentum of the cell moments m[c]
+__global__ void k_fields_accumulate(const double* __restrict__ m, double* __restrict__ acc, int n) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double* r = m + (size_t)c * kMomentVals;
+    double* a = acc + (size_t)c * kFieldAvgVals;
+    a[0] += r[0];
+    a[1] += r[1];
+    a[2] += r[3];
+    a[3] += r[4];
+    a[4] += r[5];
+    a[5] += r[6];
+}
+hipError_t launch_fields_accumulate(const double* moments, double* acc, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fields_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, moments, acc, n);
+    return hipGetLastError();
+}
 
 }  // namespace picmi

@@ -233,6 +233,9 @@ struct MomentArgs {
 };
 hipError_t launch_moments(const MomentArgs<double>& a, hipStream_t s);
 hipError_t launch_moments(const MomentArgs<float>& a, hipStream_t s);
+// picFields: per cell and species the time sums of rhoN, rhoM, linearKE and momentum
+constexpr int kFieldAvgVals = 6;
+hipError_t launch_fields_accumulate(const double* moments, double* acc, int n, hipStream_t s);
 
 // ---------------------------------------------------------------- Coulomb (k_coulomb.hip)
 struct CoulombArgs {

@@ -42,7 +42,7 @@ EXPORTS = (
     "pic_particles_get", "pic_particles_positions", "pic_particles_positions_noalloc", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
     "pic_collide_coulomb", "pic_collide_mcc", "pic_collide_ion_neutral", "pic_collide_binary", "pic_binary_initialize", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
     "pic_run", "pic_cell_moments",
-    "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
+    "pic_fields_average", "pic_fields_mean", "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
     "pic_profile_reset", "pic_gnn_init", "pic_gnn_n_params", "pic_gnn_params", "pic_gnn_record",
     "pic_gnn_snapshot_count", "pic_gnn_snapshot", "pic_gnn_grad", "pic_gnn_train", "pic_gnn_eval", "pic_gnn_use",
     "pic_gnn_loss",
@@ -127,7 +127,8 @@ class pic_step_stats(C.Structure):
 class pic_diag(C.Structure):
     _fields_ = [("sum_w", C.c_double * 8), ("sum_wU", (C.c_double * 3) * 8), ("sum_wU2", C.c_double * 8),
                 ("kinetic_energy", C.c_double * 8), ("field_energy", C.c_double), ("n_parcels", C.c_int64 * 8),
-                ("kinetic_energy_relativistic", C.c_double * 8)]
+                ("kinetic_energy_relativistic", C.c_double * 8), ("sum_wUU", (C.c_double * 3) * 8),
+                ("momentum_relativistic", (C.c_double * 3) * 8), ("charge", C.c_double * 8)]
 
 
 _lib = None
@@ -180,6 +181,8 @@ def load_library(path: str = LIB_PATH):
         "pic_step": (C.c_int, [vp, dbl, C.POINTER(pic_step_stats)]),
         "pic_run": (C.c_int, [vp, dbl, i64, C.POINTER(pic_step_stats)]),
         "pic_cell_moments": (C.c_int, [vp, i32, vp]),
+        "pic_fields_average": (C.c_int, [vp, i32]),
+        "pic_fields_mean": (C.c_int, [vp, i32, vp, C.POINTER(i32)]),
         "pic_sync": (C.c_int, [vp]),
         "pic_step_index": (i64, [vp]),
         "pic_set_step_index": (C.c_int, [vp, i64]),
@@ -521,6 +524,18 @@ class Cloud:
         st = pic_step_stats()
         self._check(self.lib.pic_run(self.h, dt, int(n_steps), C.byref(st)))
         return st
+
+    def fields_average(self, reset=False):
+        """picFields: clear the time sums (reset) or add the current moments of every species"""
+        self._check(self.lib.pic_fields_average(self.h, 0 if reset else 1))
+
+    def fields_mean(self, s):
+        """(out [n_cells, 11], derived): rhoNMean, rhoMMean, linearKEMean, momentumMean[3],
+        UMean[3], translationalT, p of species s"""
+        out = np.empty((self.n_cells, 11))
+        ok = C.c_int32()
+        self._check(self.lib.pic_fields_mean(self.h, int(s), _ptr(out), C.byref(ok)))
+        return out, bool(ok.value)
 
     def diag(self):
         d = pic_diag()

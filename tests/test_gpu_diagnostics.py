@@ -117,3 +117,113 @@ def test_diagnostics_models():
     for s in range(ns):
         assert abs(per[s] - ref_per[s]) <= 1e-10 * abs(ref_per[s])
     assert abs(tot - (0.5 * sum(ref_per) + np.sum(phi * sc * V))) <= 1e-10 * abs(tot)
+
+
+def test_remaining_diagnostics_models(tmp_path):
+    """VelocityInfo, MomentumInfo (both branches), CloudCompositionInfo, ChargeInfo (the
+    reference's sum of (rhoCharge + spaceCharge) / V), CellOccupancyInfo, IsotropizationInfo,
+    UnidirectionalVelocityInfo, DebyeLengthInfo, IonizationInfo, PrintParcelInfo sums and
+    DumpInfo against numpy over the same parcels; DiagnosticsList runs the models due at a
+    step (diagnosticControl timeStep with an interval, runTime)."""
+    from picfoam_amd import diagnostics, picmi
+    say("remaining diagnostics")
+    c, species, data, tc = _cloud()
+    _, V = c.geometry()
+    vol = V.sum()
+    drift = diagnostics.velocity(c)
+    P, per = diagnostics.momentum(c)
+    Pr, perr = diagnostics.momentum(c, relativistic=True)
+    nmol, comp = diagnostics.cloud_composition(c)
+    iso = diagnostics.isotropization(c)
+    iso_nd = diagnostics.isotropization(c, account_for_drift=False)
+    uni = diagnostics.unidirectional_velocity(c, (0.0, 1.0, 1.0))
+    ref_P = np.zeros(3)
+    inv = 0.0
+    for s, (tet, U) in enumerate(data):
+        m, q, w = species[s]["mass"], species[s]["charge"], species[s]["nEquivalentParticles"]
+        n = len(U)
+        assert np.allclose(drift[s], U.mean(0), rtol=1e-12, atol=1e-12 * np.abs(U).max())
+        assert np.allclose(per[s], m * w * U.sum(0), rtol=1e-11, atol=1e-12 * m * w * np.abs(U).sum())
+        ref_P += m * w * U.sum(0)
+        g = 1.0 / np.sqrt(1.0 - (U ** 2).sum(1) / C_LIGHT ** 2)
+        assert np.allclose(perr[s], m * w * (g[:, None] * U).sum(0), rtol=1e-11,
+                           atol=1e-12 * m * w * np.abs(g[:, None] * U).sum())
+        assert comp[s][0] == n and abs(comp[s][1] - w * n) <= 1e-12 * w * n
+        assert abs(comp[s][2] - w * n / vol) <= 1e-12 * w * n / vol
+        vd = U.mean(0)
+        tx = m / KB * ((U[:, 0] ** 2).mean() - vd[0] ** 2)
+        tyz = m / (2 * KB) * ((U[:, 1] ** 2 + U[:, 2] ** 2).mean() - vd[1] ** 2 - vd[2] ** 2)
+        assert abs(iso[s][0] - tx) <= 1e-10 * tx and abs(iso[s][1] - tyz) <= 1e-10 * tyz
+        assert abs(iso[s][2] - tyz / tx) <= 1e-10 * tyz / tx
+        assert abs(iso_nd[s][0] - m / KB * (U[:, 0] ** 2).mean()) <= 1e-10 * iso_nd[s][0]
+        assert abs(uni[s] - (U[:, 1] ** 2 + U[:, 2] ** 2).mean()) <= 1e-12 * uni[s]
+        T = m * ((U ** 2).sum(1).mean() - vd @ vd) / (3 * KB)
+        inv += (w * n / vol) * q * q / (EPS0 * KB * T)
+    assert np.allclose(P, ref_P, rtol=1e-11, atol=1e-12 * np.abs(ref_P).max())
+    assert abs(nmol - sum(comp[s][1] for s in range(2))) <= 1e-12 * nmol
+    lam, h = diagnostics.debye_length(c)
+    assert abs(lam - np.sqrt(1.0 / inv)) <= 1e-10 * lam and abs(h - np.mean(np.cbrt(V))) <= 1e-12 * h
+    # charge: after the deposit, with a space charge
+    rng = np.random.default_rng(9)
+    sc = rng.normal(size=c.n_cells) * 1e-4
+    c.set_field(picmi.FIELD_SPACE_CHARGE, sc)
+    c.deposit()
+    rho = c.get_field(picmi.FIELD_RHO)
+    assert abs(diagnostics.charge(c) - np.sum((rho + sc) / V)) <= 1e-12 * np.sum(np.abs((rho + sc) / V))
+    # cell occupancy over all species
+    counts = np.zeros(c.n_cells)
+    for tet, U in data:
+        np.add.at(counts, tc[tet], 1.0)
+    assert diagnostics.cell_occupancy(c) == (counts.sum() / c.n_cells, counts.max(), counts.min())
+    assert diagnostics.ionization_states(c, {1: 3}) == {1: [len(data[1][1]), 0, 0]}
+    # print sums and the dump file
+    sp, su = diagnostics.print_parcel_sums(c)
+    xs = np.concatenate([c.positions(s) for s in range(2)])
+    assert np.allclose(sp, xs.sum(0), rtol=1e-12) and np.allclose(su, sum(U.sum(0) for _, U in data), rtol=1e-11)
+    f = tmp_path / "dump.obj"
+    diagnostics.dump_parcels(c, str(f))
+    lines = f.read_text().splitlines()
+    assert len(lines) == sum(len(U) for _, U in data)
+    first = lines[0].split()
+    assert first[0] == "x" and first[4] == "u" and first[6] == first[7] and first[8] == "t"
+    # the list: interval 2 (timeStep), and the defaults every step
+    dl = diagnostics.DiagnosticsList(c, ["Temperature", "Velocity", "Isotropization"],
+                                     {"VelocityCoeffs": {"diagnosticControl": "timeStep", "diagnosticInterval": 2}})
+    assert set(dl.info(3)) == {"Temperature", "Isotropization"}
+    assert set(dl.info(4)) == {"Temperature", "Velocity", "Isotropization"}
+    with pytest.raises(picmi.PicError):
+        diagnostics.DiagnosticsList(c, ["NoSuchInfo"])
+    c.close()
+
+
+def test_pic_fields_averages():
+    """picFields (picFields.C:103-211): the device's time sums of rhoN, rhoM, linearKE and
+    momentum over sampled steps equal the mean of the per-step cell moments, and UMean,
+    translationalT and p follow from them; with an empty cell the derived fields are not
+    computed, as in the reference."""
+    from picfoam_amd import picmi
+    say("picFields")
+    c, species, data, tc = _cloud()
+    K = 3
+    c.fields_average(reset=True)
+    sums = [np.zeros((c.n_cells, 11)) for _ in range(2)]
+    for k in range(K):
+        c.step(1e-12)
+        c.fields_average()
+        for s in range(2):
+            sums[s] += c.cell_moments(s)
+    kB = 1.380649e-23
+    for s in range(2):
+        out, derived = c.fields_mean(s)
+        mean = sums[s] / K
+        want = np.column_stack([mean[:, 0], mean[:, 1], mean[:, 3], mean[:, 4:7]])
+        assert np.allclose(out[:, :6], want, rtol=1e-12, atol=1e-12 * np.abs(want).max(axis=0))
+        if derived:
+            U = out[:, 3:6] / out[:, 1:2]
+            T = 2.0 / (3.0 * kB * out[:, 0]) * (out[:, 2] - 0.5 * out[:, 1] * (U ** 2).sum(1))
+            assert np.allclose(out[:, 6:9], U, rtol=1e-13)
+            assert np.allclose(out[:, 9], T, rtol=1e-12)
+            assert np.allclose(out[:, 10], kB * out[:, 0] * T, rtol=1e-12)
+        else:
+            assert np.any(mean[:, 0] == 0.0) and np.all(out[:, 6:] == 0.0)
+    c.close()
