@@ -598,11 +598,15 @@ int do_mcc_t(pic_ctx* c, double dt) {
     // in the large-cell pass (48 KiB)
     a.maxp = kMccSlots;
     a.maxp_big = kMccSlotsBig;
-    // up to 150 (300) electrons per cell on average at the last synchronisation, four (two)
-    // cells share a wave with LDS room for 1.5 times the mean plus 40; fuller cells take the
-    // large-cell pass
+    // up to 120 (150, 300) electrons per cell on average at the last synchronisation, eight
+    // (four, two) cells share a wave with LDS room for 1.5 times the mean plus 40; fuller
+    // cells take the large-cell pass.  A cell draws a few candidates per step, so the
+    // candidate rounds run about as fast with 8 lanes per cell as with 16, for twice the cells
     const double ppc = (double)E.n_sync / std::max(1, a.n_cells);
-    if (ppc <= 150.0) {
+    if (ppc <= 120.0) {
+        a.group = 8;
+        a.maxp_group = std::min(256, ((int)(1.5 * ppc) + 40 + 7) & ~7);
+    } else if (ppc <= 150.0) {
         a.group = 16;
         a.maxp_group = std::min(256, ((int)(1.5 * ppc) + 40 + 7) & ~7);
     } else if (ppc <= 300.0) {
@@ -659,7 +663,10 @@ int do_ion_mcc_t(pic_ctx* c, double dt) {
     a.maxp = kMccSlots;
     a.maxp_big = kMccSlotsBig;
     const double ppc = (double)I.n_sync / std::max(1, a.n_cells);
-    if (ppc <= 150.0) {
+    if (ppc <= 120.0) {
+        a.group = 8;
+        a.maxp_group = std::min(256, ((int)(1.5 * ppc) + 40 + 7) & ~7);
+    } else if (ppc <= 150.0) {
         a.group = 16;
         a.maxp_group = std::min(256, ((int)(1.5 * ppc) + 40 + 7) & ~7);
     } else if (ppc <= 300.0) {

@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     mcc_cell<UT, REL, kMccBins>(a, cell, lds, picks, pairs, &npairs);
 }
 
-// Several cells per wave: 64 / L cells, L lanes each (L = 16 or 32), for cells of up to a.maxp_group
+// Several cells per wave: 64 / L cells, L lanes each (L = 8, 16 or 32), for cells of up to a.maxp_group
 // electrons (larger ones go to the large-cell pass).  A cell draws a few candidates per
 // step (about 5 of 100 electrons at the argon case's dt), so one cell per wave leaves most
 // lanes idle while the wave pays the loads, the id ranking and the barriers for that cell
@@ -539,7 +539,10 @@ template <class UT, int REL>
 static hipError_t launch_mcc_t(const MccArgs<UT>& a, hipStream_t s) {
     if (a.n_cells <= 0) return hipSuccess;
     // per cell: maxp keys (8 B) and placements (4 B), and the bucket starts (4 B each)
-    if (a.group == 16) {
+    if (a.group == 8) {
+        const size_t lds = (size_t)8 * (a.maxp_group + a.maxp_group / 2 + 8 * 8 / 2) * sizeof(uint64_t);
+        hipLaunchKernelGGL((k_mcc_group<UT, REL, 8>), dim3((a.n_cells + 7) / 8), dim3(64), lds, s, a);
+    } else if (a.group == 16) {
         const size_t lds = (size_t)4 * (a.maxp_group + a.maxp_group / 2 + 8 * 16 / 2) * sizeof(uint64_t);
         hipLaunchKernelGGL((k_mcc_group<UT, REL, 16>), dim3((a.n_cells + 3) / 4), dim3(64), lds, s, a);
     } else if (a.group == 32) {

@@ -104,12 +104,13 @@ __device__ __forceinline__ vec3<T> interp_E(const TetGeo& g, double y1, double y
     }
 }
 
-// the coordinate held in slot k (2 bits) of (Y0, Y1, Y2, Y3), as selects
-__device__ __forceinline__ double pick_slot(uint32_t k, double y0, double y1, double y2, double y3) {
-    const double lo = (k & 1u) ? y1 : y0;
-    const double hi = (k & 1u) ? y3 : y2;
-    return (k & 2u) ? hi : lo;
-}
+// The four barycentric slots of a lane pass through LDS when the parcel changes tet: the
+// coordinates at the hit are written to the lane's column (slot k at ys[k * kMoveBlock]), the
+// hit slot is overwritten with an exact zero, and the neighbour's slots are read back in
+// its order.  Selecting them with v_cndmask instead costs about 40 VALU instructions per
+// crossing; the LDS round trip costs nine LDS instructions and their latency.  The column
+// layout keeps the lanes of a wave on distinct banks.
+constexpr int kMoveBlock = 256;
 
 // Barycentric tracking through the tet decomposition (PICParcel::moveForward,
 // PICParcel.C:81-144, with OpenFOAM-9 particle::trackToFace re-derived): per tet the
@@ -127,7 +128,8 @@ __device__ __forceinline__ double pick_slot(uint32_t k, double y0, double y1, do
 // PICParcel.C:118-123).
 template <class UT>
 __device__ __forceinline__ int track(const MoveArgs<UT>& a, int t, double& y1, double& y2, double& y3,
-                                     const vec3<UT>& U, double& sf, int& status, bool on_wall = false) {
+                                     const vec3<UT>& U, double& sf, int& status, double* __restrict__ ys,
+                                     bool on_wall = false) {
     const TetGeo* __restrict__ tets = a.tets;
     const double dt = a.dt;
     const uint32_t m = a.track_mask;
@@ -165,19 +167,23 @@ __device__ __forceinline__ int track(const MoveArgs<UT>& a, int t, double& y1, d
         h = T3 < 0.0 && Y3 >= 0.0 && Y3 * bd < -T3 * bn;
         bn = h ? Y3 : bn; bd = h ? -T3 : bd; iH = h ? 3 : iH;
         const double muH = (iH >= 0 || dok) ? bn / bd : 1e300;
-        Y0 = (iH == 0) ? 0.0 : Y0 + muH * T0;
-        Y1 = (iH == 1) ? 0.0 : Y1 + muH * T1;
-        Y2 = (iH == 2) ? 0.0 : Y2 + muH * T2;
-        Y3 = (iH == 3) ? 0.0 : Y3 + muH * T3;
-        if (iH < 0) break;
+        const double Z0 = Y0 + muH * T0, Z1 = Y1 + muH * T1, Z2 = Y2 + muH * T2, Z3 = Y3 + muH * T3;
+        if (iH < 0) {
+            Y0 = Z0; Y1 = Z1; Y2 = Z2; Y3 = Z3;
+            break;
+        }
         double frac = muH * det;
         frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
         sf = sf + f * frac;
         const int nb = (iH & 2) ? ((iH & 1) ? nb4.w : nb4.z) : ((iH & 1) ? nb4.y : nb4.x);
-        if (nb == kTetEmptyFace) break;   // only rounding brings a parcel onto an empty face
         if (nb < 0) {
+            // the parcel stops on the face (an empty face, reached only by rounding; a wall,
+            // where it waits for the wall pass) or leaves: the hit slot is exactly zero
+            Y0 = iH == 0 ? 0.0 : Z0; Y1 = iH == 1 ? 0.0 : Z1;
+            Y2 = iH == 2 ? 0.0 : Z2; Y3 = iH == 3 ? 0.0 : Z3;
+            if (nb == kTetEmptyFace) break;
             if (nb > -1000000 && a.patch_kind && a.patch_kind[-1 - nb] == 1) {
-                status = 3;   // a wall: the parcel waits on the face for the wall pass
+                status = 3;
                 break;
             }
             if (nb <= -1000000) status = 2;
@@ -185,12 +191,16 @@ __device__ __forceinline__ int track(const MoveArgs<UT>& a, int t, double& y1, d
             break;
         }
         // slot j of the neighbour takes the coordinate of this tet's slot src_j
+        ys[0 * kMoveBlock] = Z0;
+        ys[1 * kMoveBlock] = Z1;
+        ys[2 * kMoveBlock] = Z2;
+        ys[3 * kMoveBlock] = Z3;
+        ys[iH * kMoveBlock] = 0.0;
         const uint32_t p = (perm >> (8 * iH)) & 0xffu;
-        const double n0 = pick_slot(p & 3u, Y0, Y1, Y2, Y3);
-        const double n1 = pick_slot((p >> 2) & 3u, Y0, Y1, Y2, Y3);
-        const double n2 = pick_slot((p >> 4) & 3u, Y0, Y1, Y2, Y3);
-        const double n3 = pick_slot(p >> 6, Y0, Y1, Y2, Y3);
-        Y0 = n0; Y1 = n1; Y2 = n2; Y3 = n3;
+        Y0 = ys[(p & 3u) * kMoveBlock];
+        Y1 = ys[((p >> 2) & 3u) * kMoveBlock];
+        Y2 = ys[((p >> 4) & 3u) * kMoveBlock];
+        Y3 = ys[(p >> 6) * kMoveBlock];
         t = nb;
         if (++guard > kMaxTrackSteps) { status = 1; break; }
     }
@@ -323,7 +333,7 @@ __device__ void deposit_vw_block(const TetGeo* __restrict__ tets, int cell, int 
 // One lane per parcel: push (if charged and moving) + track + deposit/count; `block` is
 // the block's index within its species.
 template <class UT, int P, int W, int DEP>
-__device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block, int set, DepLds* dl) {
+__device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block, int set, DepLds* dl, double* ys) {
     const int64_t i = block * blockDim.x + threadIdx.x;
     const int64_t n = a.n_live ? min(*a.n_live, a.n) : a.n;
     // blocks wholly past the live count (the launch bound can be a species' capacity while
@@ -353,7 +363,7 @@ __device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block, 
             if (sqrt(Ux * Ux + Uy * Uy + Uz * Uz) >= kC) atomicOr(a.flags, kFlagSuperluminal);
             if (a.do_push) { a.p.ux[i] = U.x; a.p.uy[i] = U.y; a.p.uz[i] = U.z; }
             double sf = 0.0;
-            t = track(a, t, y1, y2, y3, U, sf, status);
+            t = track(a, t, y1, y2, y3, U, sf, status, ys);
             a.p.y1[i] = y1; a.p.y2[i] = y2; a.p.y3[i] = y3;
             a.p.tet[i] = t;
             if (status == 3) {
@@ -390,15 +400,16 @@ __device__ __forceinline__ void move_body(const MoveArgs<UT>& a, int64_t block, 
 // every moving species in one launch: blocks [block0[s], block0[s+1]) belong to species s,
 // so the short passes of slow species overlap the long tracks of fast ones
 template <class UT, int P, int W, int DEP>
-__global__ void __launch_bounds__(256) k_move(MoveSet<UT> m) {
+__global__ void __launch_bounds__(kMoveBlock) k_move(MoveSet<UT> m) {
+    __shared__ double ys_all[4 * kMoveBlock];
     const int64_t b = blockIdx.x;
     int s = 0;
     while (s + 1 < m.ns && b >= m.block0[s + 1]) ++s;
     if constexpr (DEP == kVolumeWeighting) {
         __shared__ DepLds dl;
-        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], s, &dl);
+        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], s, &dl, ys_all + threadIdx.x);
     } else {
-        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], s, nullptr);
+        move_body<UT, P, W, DEP>(m.a[s], b - m.block0[s], s, nullptr, ys_all + threadIdx.x);
     }
 }
 
@@ -408,7 +419,9 @@ __global__ void __launch_bounds__(256) k_move(MoveSet<UT> m) {
 // others.  A grid-stride loop over the queue length read on the device; the statistics
 // (deleted, guard, wall hits) are summed per wave and added once at the end.
 template <class UT, int P, int W, int DEP>
-__global__ void __launch_bounds__(256) k_wall(MoveSet<UT> m) {
+__global__ void __launch_bounds__(kMoveBlock) k_wall(MoveSet<UT> m) {
+    __shared__ double ys_all[4 * kMoveBlock];
+    double* ys = ys_all + threadIdx.x;
     const MoveArgs<UT>& a0 = m.a[0];
     const unsigned nq = min(*a0.wall_n, a0.wall_cap);
     unsigned long long n_del = 0, n_guard = 0, n_hits = 0;
@@ -425,7 +438,7 @@ __global__ void __launch_bounds__(256) k_wall(MoveSet<UT> m) {
         while (status == 3) {
             const TetGeo& g = a.tets[t];
             wall_hit<UT, P, W>(a, g, y1, y2, y3, sf, -1 - g.nbr[0], U, id, hits++, resyncs);
-            t = track(a, t, y1, y2, y3, U, sf, status, true);
+            t = track(a, t, y1, y2, y3, U, sf, status, ys, true);
             if (status == 3 && hits > kMaxTrackSteps) { status = 1; break; }
         }
         a.p.ux[i] = U.x; a.p.uy[i] = U.y; a.p.uz[i] = U.z;
@@ -629,11 +642,11 @@ static hipError_t launch_move_t(MoveSet<UT> m, int pusher, int weighting, int de
     int64_t blocks = 0;
     for (int k = 0; k < m.ns; ++k) {
         m.block0[k] = blocks;
-        blocks += (m.a[k].n + 255) / 256;
+        blocks += (m.a[k].n + kMoveBlock - 1) / kMoveBlock;
     }
     m.block0[m.ns] = blocks;
     if (blocks <= 0) return hipSuccess;
-    const dim3 blk(256), grd((unsigned)blocks);
+    const dim3 blk(kMoveBlock), grd((unsigned)blocks);
 #define PICMI_MOVE_CASE(P, W, D)                                                     \
     if (pusher == P && weighting == W && dep == D) {                                 \
         hipLaunchKernelGGL((k_move<UT, P, W, D>), grd, blk, 0, s, m);                \
@@ -658,7 +671,7 @@ template <class UT>
 static hipError_t launch_wall_t(const MoveSet<UT>& m, int pusher, int weighting, int dep, hipStream_t s) {
     if (m.ns <= 0 || !m.a[0].wall_q) return hipSuccess;
     // the queue length stays on the device: a fixed grid strides over it
-    const dim3 blk(256), grd(256);
+    const dim3 blk(kMoveBlock), grd(256);
 #define PICMI_WALL_CASE(P, W, D)                                                     \
     if (pusher == P && weighting == W && dep == D) {                                 \
         hipLaunchKernelGGL((k_wall<UT, P, W, D>), grd, blk, 0, s, m);                \
