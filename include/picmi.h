@@ -209,7 +209,39 @@ typedef struct {
     int64_t ion_parcel_candidates, ion_parcel_elastic, ion_parcel_charge_exchange;
     /* BinaryCollisionModel: neutral pairs, then the neutrals with the background gas */
     int64_t binary_candidates, binary_collisions, binary_bg_candidates, binary_bg_collisions;
+    int64_t n_injected;         /* parcels added by the SimpleInjection models */
 } pic_step_stats;
+
+/* One PatchBoundaryModels entry of type SimpleInjection (SimpleInjection.C:34-175) on a
+ * patch of type `patch`.  Every step whose end time t + dt is at most end_time
+ * (SimpleInjection.C:191-197), each listed species gets floor(frequency / nEquivalentParticles
+ * * dt + remainder) new parcels, the fraction carried to the next step (:204-210), placed
+ * and given velocities by the emitter (ParticleEmitter.C:223-349, :357-535): a triangle of
+ * the patch by area (faces fanned from their first point), a uniform point in it moved into
+ * the owner cell by 0.1-0.5 of its normal distance from the cell centre, the components of
+ * empty directions set to zero, and a velocity of the model at temperature eV (in eV) in the
+ * frame of the face.  Charged species are synchronised back by half a step with the field at
+ * their position when the boundary sync is on (ParticleEmitter.C:338).  The new parcels move
+ * their full step in the same pic_step.  interaction[] gives, per species of the cloud, what
+ * a parcel reaching the patch does (SimpleInjection.C:233-292): 0 deletion (the default);
+ * reflection and floating are refused (PIC_ERR_UNSUPPORTED). */
+typedef struct {
+    int32_t patch;
+    int32_t velocity_model;       /* PIC_INJECT_* */
+    int32_t n_species;            /* injectionTypeIds entries */
+    int32_t species[8];           /* injectionTypeIds */
+    double frequency[8];          /* frequencies: real particles per second */
+    double eV[8];                 /* eV: temperature of the emitted distribution */
+    double end_time;              /* injectionEndTime */
+    double time;                  /* the time at step index 0; advanced by dt every step */
+    int32_t interaction[8];       /* per cloud species: 0 deletion */
+} pic_injection;
+enum {
+    PIC_INJECT_BIRD_MAXWELLIAN_FLUX = 0,  /* birdMaxwellianFlux */
+    PIC_INJECT_MOST_PROBABLE_SPEED = 1,   /* mostProbableSpeed */
+    PIC_INJECT_HALF_MAXWELLIAN_FLUX = 2,  /* halfMaxwellianFlux */
+    PIC_INJECT_MAXWELLIAN_FLUX = 3        /* maxwellianFlux */
+};
 
 typedef struct {
     /* TemperatureInfo.C:72-138 / KineticEnergyInfo.C:69-82 sums per species */
@@ -299,6 +331,11 @@ int pic_collide_ion_neutral(pic_ctx* ctx, double dt);  /* IonNeutralCollisionMod
                                                           the ions with the neutral parcels, then with
                                                           the background gas */
 int pic_collide_binary(pic_ctx* ctx, double dt);       /* BinaryCollisionModel::handleCollisions */
+/* BoundaryModelList's SimpleInjection models (BoundaryModelList.C:34-167): replaces the list
+ * (n = 0 removes it; at most 4 models, one patch each) and resets their remainders.
+ * pic_step runs the injection first, as PICCloud::evolve (PICCloud.C:1435) does. */
+int pic_set_injection(pic_ctx* ctx, const pic_injection* models, int32_t n);
+int pic_inject(pic_ctx* ctx, double dt);               /* BoundaryModelList::injection alone */
 int pic_deposit(pic_ctx* ctx);                         /* calculateFields: charge density */
 int pic_poisson_solve(pic_ctx* ctx, pic_poisson_stats* stats); /* MaxwellSolver::solveFields */
 int pic_diag_reduce(pic_ctx* ctx, pic_diag* out);      /* DiagnosticsList::info sums */

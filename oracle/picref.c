@@ -2256,3 +2256,117 @@ void ref_grad(const ref_poisson* P, const double* phi, double* gr) {
     for (int c = 0; c < m->n_cells; ++c)
         for (int d = 0; d < 3; ++d) gr[3 * c + d] /= m->V[c];
 }
+
+/* ------------------------------------------------------------------ SimpleInjection */
+
+#define TAG_INJECT 11u
+
+/* SimpleInjection::injection (SimpleInjection.C:186-229) for one species through
+ * ParticleEmitter::emitParticle (ParticleEmitter.C:223-349) and its velocity models
+ * (:357-535): `count` new parcels of one injector.  The patch's faces are fanned from
+ * their first point into triangles; a triangle is drawn by area (the last whose
+ * cumulative start lies below the drawn area), a point in it by the barycentric
+ * (1 - t, (1 - s) t, s t) with t = sqrt(u) (OpenFOAM barycentric2D01), moved into the
+ * cell by a in [0.1, 0.5) of its normal distance from the owner's centre, its empty
+ * components zeroed (meshTools::constrainDirection) and located in the owner cell.  The
+ * velocity frame: n the inward normal, t1 from the face's first point towards the face
+ * centre, t2 = n x t1.  Charged parcels are pushed by -dt/2 when p->sync is set
+ * (ParticleEmitter.C:338, PICParcel.C:214-227).  The draws are Philox keyed by
+ * (seed; k, step, injector << 16 | species, TAG_INJECT << 24 | draw). */
+void ref_inject(const ref_mesh* m, const ref_inject_params* p, int count, double* y, int* tet, double* U,
+                uint32_t* id) {
+    const int pt = p->patch;
+    int ntri = 0;
+    for (int f = m->pstart[pt]; f < m->pstart[pt] + m->psize[pt]; ++f) ntri += m->fo[f + 1] - m->fo[f] - 2;
+    int* tface = (int*)malloc(sizeof(int) * (ntri > 0 ? ntri : 1));
+    int* tj = (int*)malloc(sizeof(int) * (ntri > 0 ? ntri : 1));
+    double* cum = (double*)malloc(sizeof(double) * (ntri + 1));
+    cum[0] = 0.0;
+    int k = 0;
+    for (int f = m->pstart[pt]; f < m->pstart[pt] + m->psize[pt]; ++f) {
+        const int* fp = m->fp + m->fo[f];
+        const int n = m->fo[f + 1] - m->fo[f];
+        v3 a = ld3(m->points + 3 * fp[0]);
+        for (int j = 1; j + 1 < n; ++j) {
+            v3 b = ld3(m->points + 3 * fp[j]), c = ld3(m->points + 3 * fp[j + 1]);
+            cum[k + 1] = cum[k] + 0.5 * vmag(vcross(vsub(b, a), vsub(c, a)));
+            tface[k] = f; tj[k] = j;
+            ++k;
+        }
+    }
+    const uint32_t ckey = (p->model << 16) | p->species;
+    const double vt = sqrt(KB * p->T / p->mass);
+    const double vmp = sqrt(2.0 * KB * p->T / p->mass);
+    for (int i = 0; i < count; ++i) {
+        uint32_t o[4];
+        rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, TAG_INJECT << 24, o);
+        const double area = u01(o[0], o[1]) * cum[ntri], s = u01(o[2], o[3]);
+        int tri = 0;
+        for (int q = ntri - 1; q >= 0; --q)
+            if (area > cum[q]) { tri = q; break; }
+        const int f = tface[tri];
+        const int* fp = m->fp + m->fo[f];
+        v3 a = ld3(m->points + 3 * fp[0]), b = ld3(m->points + 3 * fp[tj[tri]]),
+           c = ld3(m->points + 3 * fp[tj[tri] + 1]);
+        rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_INJECT << 24) | 1u, o);
+        const double t = sqrt(u01(o[0], o[1])), frac = 0.1 + 0.4 * u01(o[2], o[3]);
+        v3 pf = vadd(vadd(vscale(1.0 - t, a), vscale((1.0 - s) * t, b)), vscale(s * t, c));
+        v3 Sf = ld3(m->Sf + 3 * f);
+        v3 nout = vdiv(Sf, vmag(Sf));
+        const int own = m->owner[f];
+        v3 pc = ld3(m->C + 3 * own);
+        v3 d = vscale(fabs(vdot(vsub(pf, pc), nout)), nout);
+        v3 x = vsub(pf, vscale(frac, d));
+        if (m->track_mask & 1) x.x = 0.0;
+        if (m->track_mask & 2) x.y = 0.0;
+        if (m->track_mask & 4) x.z = 0.0;
+        double xx[3] = {x.x, x.y, x.z};
+        ref_locate(m, 1, xx, &own, y + 3 * i, tet + i);
+        /* the frame of the face (ParticleEmitter.C:308-322) */
+        v3 nin = vscale(-1.0, nout);
+        v3 t1 = vsub(ld3(m->Cf + 3 * f), a);
+        t1 = vdiv(t1, vmag(t1));
+        v3 t2 = vcross(nin, t1);
+        t2 = vdiv(t2, vmag(t2));
+        double n1 = 0.0, n2 = 0.0, vn;
+        if (p->velocity_model != 1) {
+            rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_INJECT << 24) | 2u, o);
+            box_muller(u01(o[0], o[1]), u01(o[2], o[3]), &n1, &n2);
+        }
+        if (p->velocity_model == 0) {
+            /* Bird eqn 12.5: uNormal = 3 (2u - 1) accepted with P = 2 u / sqrt(2) exp(1/2 - u^2) */
+            double un = 0.0;
+            for (uint32_t j = 4; j < 4 + 2000; ++j) {
+                rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_INJECT << 24) | j, o);
+                un = 3.0 * (2.0 * u01(o[0], o[1]) - 1.0);
+                double P = un < 0.0 ? -1.0 : 2.0 * un / sqrt(2.0) * exp(0.5 - un * un);
+                if (!(P < u01(o[2], o[3]))) break;
+            }
+            vn = vmp * un;
+        } else if (p->velocity_model == 1) {
+            vn = vmp;
+        } else {
+            rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_INJECT << 24) | 3u, o);
+            if (p->velocity_model == 2) {
+                double g1, g2;
+                box_muller(u01(o[0], o[1]), u01(o[2], o[3]), &g1, &g2);
+                vn = fabs(g1) * vt;
+            } else {
+                vn = vt * sqrt(-2.0 * log(1.0 - u01(o[0], o[1])));
+            }
+        }
+        v3 u = vadd(vadd(vscale(vt * n1, t1), vscale(vt * n2, t2)), vscale(vn, nin));
+        double uu[3] = {u.x, u.y, u.z};
+        if (p->sync && p->q != 0.0 && tet[i] >= 0) {
+            v3 E = interp_one(m, p->weighting, tet[i], y + 3 * i, p->Ecell, p->Epoint);
+            double e[3] = {E.x, E.y, E.z};
+            double zero[3] = {0.0, 0.0, 0.0};
+            const double* B = p->Bcell ? p->Bcell + 3 * m->tet_cell[tet[i]] : zero;
+            ref_push_one(p->pusher, uu, e, B, p->q, p->mass, -0.5 * p->dt, p->hc_reference_form, p->rotate);
+        }
+        U[3 * i] = uu[0]; U[3 * i + 1] = uu[1]; U[3 * i + 2] = uu[2];
+        rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_NEWID << 24) | 3u, o);
+        if (id) id[i] = o[0];
+    }
+    free(tface); free(tj); free(cum);
+}

@@ -86,6 +86,27 @@ int ref_move_walls(const ref_mesh* m, int pusher, int weighting, int n, double* 
                    int do_push, int hc_reference_form, int rotate, int* n_superluminal,
                    const ref_wall_params* walls);
 
+/* SimpleInjection (SimpleInjection.C:186-229) through ParticleEmitter::emitParticle
+ * (ParticleEmitter.C:223-349, velocity models :357-535): `count` new parcels of one species
+ * from one injector patch.  velocity_model: 0 birdMaxwellianFlux, 1 mostProbableSpeed,
+ * 2 halfMaxwellianFlux, 3 maxwellianFlux.  Writes y[3*count], tet[count], U[3*count] and,
+ * when id is not NULL, id[count]. */
+typedef struct {
+    int patch;
+    int velocity_model;
+    double T;                  /* K: eV * e / k */
+    double mass, q;
+    int sync;                  /* push charged parcels by -dt/2 with the field at their position */
+    int pusher, weighting, hc_reference_form, rotate;
+    double dt;
+    const double *Ecell, *Epoint, *Bcell;
+    uint64_t seed;
+    int64_t step;
+    uint32_t species, model;
+} ref_inject_params;
+void ref_inject(const ref_mesh* m, const ref_inject_params* p, int count, double* y, int* tet, double* U,
+                uint32_t* id);
+
 /* push only (leapfrog initialisation, picInitialise.C:228-237) */
 void ref_push(const ref_mesh* m, int pusher, int weighting, int n, const double* y, double* U, const int* tet,
               double q, double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell,

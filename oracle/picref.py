@@ -25,6 +25,14 @@ class WallParams(C.Structure):
                 ("id", C.c_void_p), ("hits", C.c_void_p)]
 
 
+class InjectParams(C.Structure):
+    _fields_ = [("patch", C.c_int), ("velocity_model", C.c_int), ("T", C.c_double), ("mass", C.c_double),
+                ("q", C.c_double), ("sync", C.c_int), ("pusher", C.c_int), ("weighting", C.c_int),
+                ("hc_reference_form", C.c_int), ("rotate", C.c_int), ("dt", C.c_double), ("Ecell", C.c_void_p),
+                ("Epoint", C.c_void_p), ("Bcell", C.c_void_p), ("seed", C.c_uint64), ("step", C.c_int64),
+                ("species", C.c_uint32), ("model", C.c_uint32)]
+
+
 def build():
     r = subprocess.run(["make", "-s", "-C", HERE, "-B", "libpicref.so"], capture_output=True, text=True)
     if r.returncode != 0:
@@ -56,6 +64,7 @@ def lib():
         _lib.ref_locate.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         _lib.ref_locate.restype = C.c_int
         _lib.ref_binary.argtypes = [vp, vp, C.c_double, vp, vp, vp, vp]
+        _lib.ref_inject.argtypes = [vp, C.POINTER(InjectParams), C.c_int, vp, vp, vp, vp]
         _lib.ref_ion_pairs.argtypes = [vp, vp, C.c_double, vp, vp, vp]
         _lib.ref_sigma_tcr.argtypes = [C.c_int] + [C.c_double] * 7 + [vp, vp]
         _lib.ref_sigma_tcr.restype = C.c_double
@@ -576,3 +585,26 @@ def gnn_forward_backward(nb, x, theta, H, L, dout=None):
     L_.ref_gnn_forward_backward(n, K, _p(nb), x.shape[1], _p(x), H, L, _p(theta), _p(d) if d is not None else None,
                                 _p(g) if g is not None else None, _p(out))
     return out, g
+
+
+def inject(rm: RefMesh, patch, velocity_model, T, mass, count, seed, step, species, model=0, q=0.0, sync=False,
+           pusher=1, weighting=0, dt=0.0, Ecell=None, Epoint=None, Bcell=None, hc_reference_form=0, rotate=0):
+    """SimpleInjection of `count` parcels of one species from one patch (ref_inject); returns
+    (y[count,3], tet[count], U[count,3], id[count])."""
+    keep = []
+
+    def arr(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, np.float64)
+        keep.append(a)
+        return _p(a)
+
+    prm = InjectParams(patch=patch, velocity_model=velocity_model, T=T, mass=mass, q=q, sync=1 if sync else 0,
+                       pusher=pusher, weighting=weighting, hc_reference_form=hc_reference_form, rotate=rotate,
+                       dt=dt, Ecell=arr(Ecell), Epoint=arr(Epoint), Bcell=arr(Bcell), seed=seed, step=step,
+                       species=species, model=model)
+    y = np.zeros((count, 3)); tet = np.zeros(count, np.int32); U = np.zeros((count, 3))
+    ids = np.zeros(count, np.uint32)
+    lib().ref_inject(rm.h, C.byref(prm), count, _p(y), _p(tet), _p(U), _p(ids))
+    return y, tet, U, ids

@@ -93,6 +93,17 @@ struct pic_ctx {
     int electron = -1, ion = -1;
     double bg_n = 0, bg_T = 0, bg_m = 0, bg_w = 0, bg_v[3] = {0, 0, 0};
     int create_ion = 1, create_electron = 1;
+    // SimpleInjection models (BoundaryModelList of PatchBoundaryModels)
+    struct Injector {
+        pic_injection cfg{};
+        double rem[8] = {};            // particleRemainder_ per injected species
+        InjectTri* d_tri = nullptr;
+        double* d_cum = nullptr;
+        int ntri = 0;
+    };
+    std::vector<Injector> inj;
+    double inj_time = 0.0;             // time at the start of the next step
+    int64_t n_injected = 0;            // parcels injected since the statistics were reset
     uint64_t seed = 0;
     double ptol = 1e-8;
     int pmaxit = 10000, pmode = 0;
@@ -189,6 +200,12 @@ struct pic_ctx {
         hipMemsetAsync(p, 0, count * sizeof(T), stream);
         allocs.push_back(p);
         return (T*)p;
+    }
+    void release(void* p) {
+        if (!p) return;
+        for (auto& q : allocs)
+            if (q == p) { q = allocs.back(); allocs.pop_back(); break; }
+        hipFree(p);
     }
     template <class T>
     T* upload(const std::vector<T>& v) {
@@ -925,6 +942,52 @@ int do_poisson(pic_ctx* c, pic_poisson_stats* st) {
 }  // namespace
 
 // ====================================================================== C-ABI
+
+// BoundaryModelList::injection (BoundaryModelList.C:193-200) over the SimpleInjection models
+// (SimpleInjection.C:186-229): the parcel counts are host arithmetic on the frequencies and
+// remainders; the parcels are written on the device after each species' live ones.
+template <class UT>
+int do_inject_t(pic_ctx* c, double dt) {
+    const double t_end = c->inj_time + dt;   // the reference tests the time the step ends at
+    for (size_t m = 0; m < c->inj.size(); ++m) {
+        auto& I = c->inj[m];
+        if (t_end > I.cfg.end_time) continue;
+        for (int e = 0; e < I.cfg.n_species; ++e) {
+            const int s = I.cfg.species[e];
+            Species& S = c->sp[s];
+            const double np = I.cfg.frequency[e] / S.weight * dt + I.rem[e];
+            const int64_t n = (int64_t)np;
+            I.rem[e] = np - (double)n;
+            if (n <= 0) continue;
+            InjectArgs<UT> a{};
+            a.m = move_args<UT>(c, s, dt);
+            a.tri = I.d_tri;
+            a.cum = I.d_cum;
+            a.cell_tet0 = c->d_cell_tet0;
+            a.ntri = I.ntri;
+            a.vmodel = I.cfg.velocity_model;
+            const double T = I.cfg.eV[e] * kQe / kKB;   // SimpleInjection.C:215
+            a.vt = std::sqrt(kKB * T / S.mass);
+            a.model = (uint32_t)m;
+            a.sync = (c->wall_sync && S.charge != 0.0) ? 1 : 0;
+            a.count = n;
+            a.cap = S.capacity;
+            if (int r = check_hip(c, launch_inject(a, c->pusher, c->fw, c->stream), "injection")) return r;
+            S.n = std::min<int64_t>(S.n + n, S.capacity);   // an upper bound until the next synchronisation
+            c->n_injected += n;
+            c->counts_valid = false;
+            c->place_valid = false;
+        }
+    }
+    c->inj_time = t_end;
+    return PIC_OK;
+}
+
+int do_inject(pic_ctx* c, double dt) {
+    if (c->inj.empty()) return PIC_OK;
+    pic_ctx::Scope sc(c, PIC_STAGE_MOVE);
+    return c->mixed() ? do_inject_t<float>(c, dt) : do_inject_t<double>(c, dt);
+}
 
 extern "C" {
 
@@ -1884,6 +1947,91 @@ int pic_collide_binary(pic_ctx* c, double dt) {
     return sync_state(c);
 }
 
+int pic_set_injection(pic_ctx* c, const pic_injection* models, int32_t n) {
+    if (!c || n < 0 || n > 4 || (n > 0 && !models)) return c ? c->fail(PIC_ERR_INVALID, "pic_set_injection: 0-4 models") : PIC_ERR_INVALID;
+    const HostMesh& m = c->hm;
+    const int ns = (int)c->sp.size();
+    std::vector<pic_ctx::Injector> out(n);
+    for (int k = 0; k < n; ++k) {
+        const pic_injection& q = models[k];
+        const std::string tag = "SimpleInjection " + std::to_string(k) + ": ";
+        if (q.patch < 0 || q.patch >= m.n_patches) return c->fail(PIC_ERR_INVALID, tag + "no such patch");
+        // SimpleInjection.C:99-104: the injector must be a plain `patch`
+        if (m.ptype[q.patch] != PIC_PATCH_PATCH) return c->fail(PIC_ERR_INVALID, tag + "the patch is not of type patch");
+        if (m.psize[q.patch] <= 0) return c->fail(PIC_ERR_INVALID, tag + "the patch has no faces");
+        if (q.velocity_model < 0 || q.velocity_model > 3)
+            return c->fail(PIC_ERR_UNKNOWN_MODEL, tag + "valid injection velocity models are: birdMaxwellianFlux mostProbableSpeed maxwellianFlux halfMaxwellianFlux");
+        if (q.n_species < 1 || q.n_species > 8) return c->fail(PIC_ERR_INVALID, tag + "1-8 injectionTypeIds");
+        for (int e = 0; e < q.n_species; ++e) {
+            if (q.species[e] < 0 || q.species[e] >= ns) return c->fail(PIC_ERR_INVALID, tag + "species not defined in cloud");
+            if (!(q.frequency[e] >= 0.0) || !(q.eV[e] >= 0.0)) return c->fail(PIC_ERR_INVALID, tag + "frequencies and eV must be >= 0");
+        }
+        for (int s = 0; s < ns; ++s) {
+            if (q.interaction[s] == 0) continue;
+            return c->fail(PIC_ERR_UNSUPPORTED, tag + "only the deletion interaction is built (reflection and floating are not)");
+        }
+        out[k].cfg = q;
+    }
+    // the patch's faces fanned from their first point (ParticleEmitter.C:78-131 triangulates
+    // them; a fan has the same areas for the planar convex faces of a mesh)
+    hipStreamSynchronize(c->stream);
+    for (auto& I : c->inj) { c->release(I.d_tri); c->release(I.d_cum); }
+    c->inj.clear();
+    for (int k = 0; k < n; ++k) {
+        auto& I = out[k];
+        const int p = I.cfg.patch;
+        std::vector<InjectTri> tri;
+        std::vector<double> cum(1, 0.0);
+        for (int f = m.pstart[p]; f < m.pstart[p] + m.psize[p]; ++f) {
+            const int f0 = m.fo[f], nfp = m.fo[f + 1] - f0;
+            const double* Sf = &m.Sf[3 * f];
+            const double mS = std::sqrt(Sf[0] * Sf[0] + Sf[1] * Sf[1] + Sf[2] * Sf[2]);
+            const int own = m.owner[f];
+            const double* P0 = &m.points[3 * m.fp[f0]];
+            double t1[3], t2[3], nin[3];
+            for (int d = 0; d < 3; ++d) { t1[d] = m.Cf[3 * f + d] - P0[d]; nin[d] = -Sf[d] / mS; }
+            const double it1 = 1.0 / std::sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+            for (double& v : t1) v *= it1;
+            t2[0] = nin[1] * t1[2] - nin[2] * t1[1];
+            t2[1] = nin[2] * t1[0] - nin[0] * t1[2];
+            t2[2] = nin[0] * t1[1] - nin[1] * t1[0];
+            const double it2 = 1.0 / std::sqrt(t2[0] * t2[0] + t2[1] * t2[1] + t2[2] * t2[2]);
+            for (double& v : t2) v *= it2;
+            for (int j = 1; j + 1 < nfp; ++j) {
+                InjectTri T{};
+                const double* B = &m.points[3 * m.fp[f0 + j]];
+                const double* Cc = &m.points[3 * m.fp[f0 + j + 1]];
+                double e1[3], e2[3];
+                for (int d = 0; d < 3; ++d) {
+                    T.a[d] = P0[d]; T.b[d] = B[d]; T.c[d] = Cc[d];
+                    T.n[d] = Sf[d] / mS; T.t1[d] = t1[d]; T.t2[d] = t2[d];
+                    T.pc[d] = m.C[3 * own + d];
+                    e1[d] = B[d] - P0[d]; e2[d] = Cc[d] - P0[d];
+                }
+                T.cell = own;
+                const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                             cz = e1[0] * e2[1] - e1[1] * e2[0];
+                cum.push_back(cum.back() + 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz));
+                tri.push_back(T);
+            }
+        }
+        if (tri.empty()) return c->fail(PIC_ERR_MESH, "SimpleInjection: the patch has no triangles");
+        I.ntri = (int)tri.size();
+        I.d_tri = c->upload(tri);
+        I.d_cum = c->upload(cum);
+        if (!I.d_tri || !I.d_cum) return c->fail(PIC_ERR_HIP, "out of device memory (injection)");
+        c->inj.push_back(I);
+    }
+    if (n > 0) c->inj_time = models[0].time;
+    return PIC_OK;
+}
+
+int pic_inject(pic_ctx* c, double dt) {
+    if (!c || c->inj.empty()) return c ? c->fail(PIC_ERR_INVALID, "no SimpleInjection model set") : PIC_ERR_INVALID;
+    if (int r = do_inject(c, dt)) return r;
+    return sync_state(c);
+}
+
 int pic_collide_mcc(pic_ctx* c, double dt) {
     if (!c || !c->mcc) return c ? c->fail(PIC_ERR_INVALID, "ElectronNeutralCollisionModel is none") : PIC_ERR_INVALID;
     if (int r = do_sort(c)) return r;
@@ -1913,6 +2061,7 @@ int pic_poisson_solve(pic_ctx* c, pic_poisson_stats* st) {
 // PICCloud::evolve (PICCloud.C:1426-1471) enqueued on the context stream without a host
 // round trip, except inside the Poisson solve (convergence checks) and GNN recording
 int step_body(pic_ctx* c, double dt, pic_poisson_stats* ps) {
+    if (int r = do_inject(c, dt)) return r;
     if (int r = do_move(c, dt)) return r;
     // a species moves into cell order every Species::sort_interval steps; collision models
     // need the cell order every step, and in between get it as an index
@@ -1969,11 +2118,13 @@ void read_step_stats(pic_ctx* c, const pic_poisson_stats& ps, pic_step_stats* st
     st->binary_collisions = (int64_t)pw[4];
     st->binary_bg_candidates = (int64_t)pw[5];
     st->binary_bg_collisions = (int64_t)pw[6];
+    st->n_injected = c->n_injected;
     st->poisson = ps;
 }
 
 int pic_step(pic_ctx* c, double dt, pic_step_stats* st) {
     if (!c) return PIC_ERR_INVALID;
+    c->n_injected = 0;
     hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kStatsWords, c->stream);
     pic_poisson_stats ps{};
     if (int r = step_body(c, dt, &ps)) return r;
@@ -1984,6 +2135,7 @@ int pic_step(pic_ctx* c, double dt, pic_step_stats* st) {
 
 int pic_run(pic_ctx* c, double dt, int64_t n_steps, pic_step_stats* st) {
     if (!c || n_steps < 0) return PIC_ERR_INVALID;
+    c->n_injected = 0;
     hipMemsetAsync(c->d_stats, 0, sizeof(unsigned long long) * kStatsWords, c->stream);
     pic_poisson_stats ps{};
     for (int64_t k = 0; k < n_steps; ++k)

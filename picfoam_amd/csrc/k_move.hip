@@ -4,6 +4,7 @@
 // scatter.
 #include "device.h"
 #include "kernels.h"
+#include "collide.h"
 
 namespace picmi {
 
@@ -545,6 +546,121 @@ __global__ void k_locate(LocateArgs<UT> a) {
     a.p.id[o] = a.id[i];
 }
 
+// ParticleEmitter::emitParticle (ParticleEmitter.C:223-349) for parcel k of one injection: a
+// triangle of the patch drawn by area (the cumulative table's last entry below the drawn
+// area), a uniform point in it (barycentric (1 - t, (1 - s) t, s t) with t = sqrt(u)), moved
+// into the cell by a in [0.1, 0.5) of its normal distance from the cell centre, its empty
+// components set to zero (meshTools::constrainDirection), located in the owner cell (the
+// tet with the largest minimum barycentric coordinate, as k_locate), and a velocity from
+// the model (ParticleEmitter.C:357-535) in the frame (inward normal, t1, t2).  The draws
+// are Philox keyed by (seed; k, step, injector and species, tag and draw).
+template <class UT>
+__device__ void emit(const InjectArgs<UT>& a, uint32_t k, int& t, double& y1, double& y2, double& y3, double U[3]) {
+    const MoveArgs<UT>& m = a.m;
+    const uint32_t c = (a.model << 16) | m.species;
+    uint4 o = rng4(m.seed, k, (uint32_t)m.step, c, kTagInject << 24);
+    const double area = u01(o.x, o.y) * a.cum[a.ntri], s = u01(o.z, o.w);
+    int lo = 0, hi = a.ntri - 1;   // the last triangle whose cumulative start lies below the area
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.cum[mid] < area) lo = mid; else hi = mid - 1;
+    }
+    const InjectTri& T = a.tri[lo];
+    o = rng4(m.seed, k, (uint32_t)m.step, c, (kTagInject << 24) | 1u);
+    const double tt = sqrt(u01(o.x, o.y)), frac = 0.1 + 0.4 * u01(o.z, o.w);
+    const double wa = 1.0 - tt, wb = (1.0 - s) * tt, wc = s * tt;
+    double x[3], dn = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        x[d] = wa * T.a[d] + wb * T.b[d] + wc * T.c[d];
+        dn += (x[d] - T.pc[d]) * T.n[d];
+    }
+    dn = fabs(dn);
+    for (int d = 0; d < 3; ++d) {
+        x[d] -= frac * (dn * T.n[d]);
+        if ((m.track_mask >> d) & 1u) x[d] = 0.0;
+    }
+    const int cell = T.cell;
+    const double dx = x[0] - T.pc[0], dy = x[1] - T.pc[1], dz = x[2] - T.pc[2];
+    int best = -1;
+    double bestmin = -1e300;
+    y1 = y2 = y3 = 0.0;
+    for (int q = a.cell_tet0[cell]; q < a.cell_tet0[cell + 1]; ++q) {
+        const TetGeo& g = m.tets[q];
+        const double v1 = (g.r[0] * dx + g.r[1] * dy + g.r[2] * dz) / g.det;
+        const double v2 = (g.r[3] * dx + g.r[4] * dy + g.r[5] * dz) / g.det;
+        const double v3 = (g.r[6] * dx + g.r[7] * dy + g.r[8] * dz) / g.det;
+        const double mn = fmin(fmin(1.0 - (v1 + v2 + v3), v1), fmin(v2, v3));
+        if (mn > bestmin) { bestmin = mn; best = q; y1 = v1; y2 = v2; y3 = v3; }
+    }
+    t = best;
+    // velocity in the frame of the face: n inward, t1, t2 tangential
+    double vn = 0.0, n1 = 0.0, n2 = 0.0;
+    if (a.vmodel != 1) {
+        o = rng4(m.seed, k, (uint32_t)m.step, c, (kTagInject << 24) | 2u);
+        box_muller(u01(o.x, o.y), u01(o.z, o.w), n1, n2);
+        n1 *= a.vt; n2 *= a.vt;
+    }
+    const double vmp = sqrt(2.0) * a.vt;   // maxwellianMostProbableSpeed, PICCloudI.H:541-548
+    if (a.vmodel == 0) {
+        // Bird eqn 12.5 by acceptance-rejection (ParticleEmitter.C:380-398)
+        double un = 0.0;
+        for (uint32_t j = 4; j < 4 + 2000; ++j) {
+            o = rng4(m.seed, k, (uint32_t)m.step, c, (kTagInject << 24) | j);
+            un = 3.0 * (2.0 * u01(o.x, o.y) - 1.0);
+            const double P = un < 0.0 ? -1.0 : 2.0 * un / sqrt(2.0) * exp(0.5 - un * un);
+            if (!(P < u01(o.z, o.w))) break;
+        }
+        vn = vmp * un;
+    } else if (a.vmodel == 1) {
+        vn = vmp;
+    } else {
+        o = rng4(m.seed, k, (uint32_t)m.step, c, (kTagInject << 24) | 3u);
+        if (a.vmodel == 2) {
+            double g1, g2;
+            box_muller(u01(o.x, o.y), u01(o.z, o.w), g1, g2);
+            vn = fabs(g1) * a.vt;
+        } else {
+            vn = a.vt * sqrt(-2.0 * log(1.0 - u01(o.x, o.y)));
+        }
+    }
+    for (int d = 0; d < 3; ++d) U[d] = n1 * T.t1[d] + n2 * T.t2[d] - vn * T.n[d];
+}
+
+// SimpleInjection::injection for one species of one injector: parcels base + k, k < count,
+// after the live ones (base read before any is added; the live count is raised afterwards by
+// k_live_add), with new ids; charged species are synchronised back by half a step
+// (ParticleEmitter.C:338, PICParcel.C:214-227) with the field at their position.
+template <class UT, int P, int W>
+__global__ void __launch_bounds__(256) k_inject(InjectArgs<UT> a) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.count) return;
+    const MoveArgs<UT>& m = a.m;
+    const int64_t i = *m.n_live + k;
+    if (i >= a.cap) { atomicOr(m.flags, kFlagCapacity); return; }
+    int t;
+    double y1, y2, y3, U[3];
+    emit(a, (uint32_t)k, t, y1, y2, y3, U);
+    vec3<UT> u = mk3<UT>((UT)U[0], (UT)U[1], (UT)U[2]);
+    if (a.sync && t >= 0) {
+        const TetGeo& g = m.tets[t];
+        const vec3<UT> E = interp_E<UT, W>(g, y1, y2, y3, m.Ecell, m.Epoint);
+        vec3<UT> B = mk3<UT>(0, 0, 0);
+        if (m.rotate) B = mk3<UT>(m.Bcell[3 * g.cell], m.Bcell[3 * g.cell + 1], m.Bcell[3 * g.cell + 2]);
+        const UT f = (UT)-0.5;
+        u = push_u<UT, P>(u, E, B, m.k * f, m.kv * f, m.rotate != 0, m.hc_ref != 0);
+    }
+    m.p.y1[i] = y1; m.p.y2[i] = y2; m.p.y3[i] = y3;
+    m.p.tet[i] = t;
+    m.p.ux[i] = u.x; m.p.uy[i] = u.y; m.p.uz[i] = u.z;
+    const uint32_t c = (a.model << 16) | m.species;
+    m.p.id[i] = rng4(m.seed, (uint32_t)k, (uint32_t)m.step, c, (kTagNewId << 24) | 3u).x;
+}
+
+// the live count after an injection: raised by the parcels that fitted
+__global__ void k_live_add(int64_t* n, int64_t add, int64_t cap) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *n = min(*n + add, max(cap, *n));
+}
+
 template <class UT>
 __global__ void k_positions(PartSoA<UT> p, int64_t n, const TetGeo* __restrict__ tets, const double* __restrict__ C,
                             const double* __restrict__ pts, double* __restrict__ x) {
@@ -715,6 +831,35 @@ static hipError_t launch_push_t(const MoveArgs<UT>& a, int pusher, int weighting
     PICMI_PUSH_CASE(kPushHC, kCellAverage) PICMI_PUSH_CASE(kPushHC, kVolumeWeighting)
 #undef PICMI_PUSH_CASE
     return hipErrorInvalidValue;
+}
+
+template <class UT>
+static hipError_t launch_inject_t(const InjectArgs<UT>& a, int pusher, int weighting, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    if (a.ntri <= 0 || !a.m.n_live) return hipErrorInvalidValue;
+    const dim3 blk(256), grd((unsigned)((a.count + 255) / 256));
+    bool done = false;
+#define PICMI_INJECT_CASE(P, W)                                                      \
+    if (!done && pusher == P && weighting == W) {                                    \
+        hipLaunchKernelGGL((k_inject<UT, P, W>), grd, blk, 0, s, a);                 \
+        done = true;                                                                 \
+    }
+    PICMI_INJECT_CASE(kPushNone, kCellAverage) PICMI_INJECT_CASE(kPushNone, kVolumeWeighting)
+    PICMI_INJECT_CASE(kPushBoris, kCellAverage) PICMI_INJECT_CASE(kPushBoris, kVolumeWeighting)
+    PICMI_INJECT_CASE(kPushBorisNR, kCellAverage) PICMI_INJECT_CASE(kPushBorisNR, kVolumeWeighting)
+    PICMI_INJECT_CASE(kPushVay, kCellAverage) PICMI_INJECT_CASE(kPushVay, kVolumeWeighting)
+    PICMI_INJECT_CASE(kPushHC, kCellAverage) PICMI_INJECT_CASE(kPushHC, kVolumeWeighting)
+#undef PICMI_INJECT_CASE
+    if (!done) return hipErrorInvalidValue;
+    if (hipError_t e = hipGetLastError()) return e;
+    hipLaunchKernelGGL(k_live_add, dim3(1), dim3(64), 0, s, const_cast<int64_t*>(a.m.n_live), a.count, a.cap);
+    return hipGetLastError();
+}
+hipError_t launch_inject(const InjectArgs<double>& a, int pusher, int weighting, hipStream_t s) {
+    return launch_inject_t(a, pusher, weighting, s);
+}
+hipError_t launch_inject(const InjectArgs<float>& a, int pusher, int weighting, hipStream_t s) {
+    return launch_inject_t(a, pusher, weighting, s);
 }
 
 hipError_t launch_move(const MoveSet<double>& m, int pusher, int weighting, int dep, hipStream_t s) {

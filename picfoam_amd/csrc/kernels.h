@@ -91,6 +91,31 @@ hipError_t launch_move(const MoveSet<float>& m, int pusher, int weighting, int d
 // queued parcel's track, then its count, place and deposit
 hipError_t launch_wall(const MoveSet<double>& m, int pusher, int weighting, int dep, hipStream_t s);
 hipError_t launch_wall(const MoveSet<float>& m, int pusher, int weighting, int dep, hipStream_t s);
+// SimpleInjection (SimpleInjection.C:186-229) through its emitter (ParticleEmitter.C): one
+// triangle of the injector patch's faces, fanned from each face's first point
+struct InjectTri {
+    double a[3], b[3], c[3];    // vertices; a is the face's first point
+    double n[3];                // unit outward normal of the face
+    double t1[3], t2[3];        // wall tangents: t1 from a to the face centre reversed, t2 = -n x t1
+    double pc[3];               // owner cell centre
+    int32_t cell, pad;
+};
+template <class UT>
+struct InjectArgs {
+    MoveArgs<UT> m;             // the species' push constants, fields, tets, seed, step and flags
+    const InjectTri* tri;
+    const double* cum;          // [ntri + 1] cumulative triangle areas from 0
+    const int32_t* cell_tet0;
+    int ntri;
+    int vmodel;                 // 0 birdMaxwellianFlux, 1 mostProbableSpeed, 2 halfMaxwellianFlux, 3 maxwellianFlux
+    double vt;                  // sqrt(k T / m) of the emitted distribution
+    uint32_t model;             // injector index (a key of the draws)
+    int sync;                   // synchronise the new velocity by -dt/2 (charged species with the boundary sync)
+    int64_t count;              // parcels to add after the live ones
+    int64_t cap;                // the species' capacity
+};
+hipError_t launch_inject(const InjectArgs<double>& a, int pusher, int weighting, hipStream_t s);
+hipError_t launch_inject(const InjectArgs<float>& a, int pusher, int weighting, hipStream_t s);
 hipError_t launch_push(const MoveArgs<double>& a, int pusher, int weighting, hipStream_t s);
 hipError_t launch_push(const MoveArgs<float>& a, int pusher, int weighting, hipStream_t s);
 hipError_t launch_deposit(const MoveArgs<double>& a, int dep, hipStream_t s);

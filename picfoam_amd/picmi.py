@@ -40,7 +40,7 @@ EXPORTS = (
     "pic_mesh_tets", "pic_mesh_geometry", "pic_set_cell_field", "pic_get_cell_field", "pic_set_cross_section",
     "pic_mcc_initialize", "pic_ion_mcc_initialize", "pic_mcc_initialize_temperature", "pic_particles_set", "pic_particles_set_positions", "pic_particles_count",
     "pic_particles_get", "pic_particles_positions", "pic_particles_positions_noalloc", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
-    "pic_collide_coulomb", "pic_collide_mcc", "pic_collide_ion_neutral", "pic_collide_binary", "pic_binary_initialize", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
+    "pic_collide_coulomb", "pic_collide_mcc", "pic_collide_ion_neutral", "pic_collide_binary", "pic_binary_initialize", "pic_set_injection", "pic_inject", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
     "pic_run", "pic_cell_moments",
     "pic_fields_average", "pic_fields_mean", "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
     "pic_profile_reset", "pic_gnn_init", "pic_gnn_n_params", "pic_gnn_params", "pic_gnn_record",
@@ -121,7 +121,19 @@ class pic_step_stats(C.Structure):
                 ("ion_parcel_candidates", C.c_int64), ("ion_parcel_elastic", C.c_int64),
                 ("ion_parcel_charge_exchange", C.c_int64), ("binary_candidates", C.c_int64),
                 ("binary_collisions", C.c_int64), ("binary_bg_candidates", C.c_int64),
-                ("binary_bg_collisions", C.c_int64)]
+                ("binary_bg_collisions", C.c_int64), ("n_injected", C.c_int64)]
+
+
+class pic_injection(C.Structure):
+    _fields_ = [("patch", C.c_int32), ("velocity_model", C.c_int32), ("n_species", C.c_int32),
+                ("species", C.c_int32 * 8), ("frequency", C.c_double * 8), ("eV", C.c_double * 8),
+                ("end_time", C.c_double), ("time", C.c_double), ("interaction", C.c_int32 * 8)]
+
+
+# SimpleInjectionCoeffs.velocityModel (SimpleInjection.C:79-97)
+INJECT_VELOCITY_MODELS = {"birdMaxwellianFlux": 0, "mostProbableSpeed": 1, "halfMaxwellianFlux": 2,
+                          "maxwellianFlux": 3}
+INJECT_INTERACTIONS = {"deletion": 0, "reflection": 1, "floating": 3}
 
 
 class pic_diag(C.Structure):
@@ -174,6 +186,8 @@ def load_library(path: str = LIB_PATH):
         "pic_collide_mcc": (C.c_int, [vp, dbl]),
         "pic_collide_ion_neutral": (C.c_int, [vp, dbl]),
         "pic_collide_binary": (C.c_int, [vp, dbl]),
+        "pic_set_injection": (C.c_int, [vp, vp, i32]),
+        "pic_inject": (C.c_int, [vp, dbl]),
         "pic_binary_initialize": (C.c_int, [vp, vp, vp]),
         "pic_deposit": (C.c_int, [vp]),
         "pic_poisson_solve": (C.c_int, [vp, C.POINTER(pic_poisson_stats)]),
@@ -503,6 +517,32 @@ class Cloud:
 
     def collide_binary(self, dt):
         self._check(self.lib.pic_collide_binary(self.h, dt))
+
+    def set_injection(self, models):
+        """PatchBoundaryModels of type SimpleInjection: a list of dicts with patch (index),
+        velocityModel, injectionTypeIds (species indices), frequencies, eV, injectionEndTime,
+        time (the time at step index 0, default 0) and interactions ({species index:
+        "deletion" | "reflection" | "floating"})."""
+        arr = (pic_injection * max(len(models), 1))()
+        for k, m in enumerate(models):
+            q = arr[k]
+            q.patch = int(m["patch"])
+            vm = m.get("velocityModel", "maxwellianFlux")
+            q.velocity_model = INJECT_VELOCITY_MODELS[vm] if isinstance(vm, str) else int(vm)
+            ids = list(m["injectionTypeIds"])
+            q.n_species = len(ids)
+            for e, sp in enumerate(ids):
+                q.species[e] = int(sp)
+                q.frequency[e] = float(m["frequencies"][e])
+                q.eV[e] = float(m["eV"][e])
+            q.end_time = float(m.get("injectionEndTime", 1e300))
+            q.time = float(m.get("time", 0.0))
+            for sp, kind in m.get("interactions", {}).items():
+                q.interaction[int(sp)] = INJECT_INTERACTIONS[kind] if isinstance(kind, str) else int(kind)
+        self._check(self.lib.pic_set_injection(self.h, C.cast(arr, C.c_void_p), len(models)))
+
+    def inject(self, dt):
+        self._check(self.lib.pic_inject(self.h, dt))
 
     def deposit(self):
         self._check(self.lib.pic_deposit(self.h))
