@@ -55,6 +55,12 @@ const char* pic_case_species_name(const pic_case* cs, int32_t species);
 /* constant/polyMesh/boundary name of a patch */
 const char* pic_case_patch_name(const pic_case* cs, int32_t patch);
 
+/* The case's PatchBoundaryModels of type SimpleInjection (BoundaryModelList.C:104-167,
+ * SimpleInjection.C:34-175) as pic_create's context gets them through pic_set_injection:
+ * up to `capacity` entries into out, their number into *n.  Other patch boundary models are
+ * refused when the case is opened. */
+int pic_case_injection(const pic_case* cs, int32_t capacity, pic_injection* out, int32_t* n);
+
 /* The structures pic_create takes, as the case defines them.  The pointers inside *mesh
  * and the model names inside *models stay valid until pic_case_close.  species[] holds
  * n_species entries; its capacities are the parcel counts the case holds so far. */

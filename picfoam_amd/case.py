@@ -17,7 +17,7 @@ from . import picmi
 
 EXPORTS = (
     "pic_case_open", "pic_case_close", "pic_case_error", "pic_case_sizes", "pic_case_times",
-    "pic_case_species_name", "pic_case_patch_name", "pic_case_describe", "pic_case_initialise",
+    "pic_case_species_name", "pic_case_patch_name", "pic_case_describe", "pic_case_injection", "pic_case_initialise",
     "pic_case_read_parcels", "pic_case_parcel_count", "pic_case_parcels", "pic_case_create", "pic_case_write",
     "pic_case_run", "pic_case_time", "pic_case_time_name",
 )
@@ -39,6 +39,7 @@ def _lib():
             "pic_case_species_name": (C.c_char_p, [vp, i32]),
             "pic_case_patch_name": (C.c_char_p, [vp, i32]),
             "pic_case_describe": (C.c_int, [vp, C.POINTER(picmi.pic_mesh), vp, C.POINTER(picmi.pic_models)]),
+            "pic_case_injection": (C.c_int, [vp, i32, vp, C.POINTER(i32)]),
             "pic_case_initialise": (C.c_int, [vp, C.c_char_p, C.c_uint64]),
             "pic_case_read_parcels": (C.c_int, [vp, C.c_char_p]),
             "pic_case_parcel_count": (C.c_int, [vp, i32, C.POINTER(i64)]),
@@ -113,6 +114,21 @@ class Case:
     @property
     def time_name(self):
         return self.lib.pic_case_time_name(self.h).decode()
+
+    def injection(self):
+        """The SimpleInjection models: a list of dicts (patch index, velocity_model, species,
+        frequencies, eV, end_time, time, interaction per cloud species)."""
+        n = C.c_int32()
+        self._check(self.lib.pic_case_injection(self.h, 0, None, C.byref(n)))
+        arr = (picmi.pic_injection * max(n.value, 1))()
+        self._check(self.lib.pic_case_injection(self.h, n.value, C.cast(arr, C.c_void_p), C.byref(n)))
+        out = []
+        for q in arr[:n.value]:
+            k = q.n_species
+            out.append({"patch": q.patch, "velocity_model": q.velocity_model, "species": list(q.species[:k]),
+                        "frequencies": list(q.frequency[:k]), "eV": list(q.eV[:k]), "end_time": q.end_time,
+                        "time": q.time, "interaction": list(q.interaction)})
+        return out
 
     def describe(self):
         """The pic_mesh, the species (as dicts) and the pic_models the case defines."""

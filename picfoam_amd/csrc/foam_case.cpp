@@ -155,6 +155,8 @@ struct pic_case {
     // picSigmaTcRMax_IonNeutral as read from the start time (READ_IF_PRESENT), and the
     // picInitialise temperatures and number densities BinaryCollisionModel::initialize takes
     std::vector<double> smax_bin, smax_ip;
+    // PatchBoundaryModels of type SimpleInjection (time set at creation: the start time)
+    std::vector<pic_injection> injection;
     bool binary_init = false;
     std::vector<double> init_T, init_n;
     // time
@@ -622,6 +624,70 @@ bool list_has_models(const picmi::foam::Entry* e) {
     return entries == 0;   // elements that are not dictionaries: not understood, refused
 }
 
+// PatchBoundaryModels.modelList (BoundaryModelList.C:104-167): "( <patch> { type <model>;
+// <model>Coeffs { ... } } ... )".  SimpleInjection (SimpleInjection.C:34-175) becomes a
+// pic_injection; an entry of type none adds nothing; other models are refused.
+void read_patch_boundary_models(Case& c, const Dict& l) {
+    const picmi::foam::Entry* e = l.find("modelList");
+    if (!e || e->dict) return;
+    Reader r(e->value, l.where + "/modelList");
+    r.list_open();
+    while (!r.accept(')')) {
+        const std::string patch = r.word();
+        r.expect('{');
+        size_t pos = r.pos();
+        auto md = picmi::foam::parse_entries(e->value, pos, l.where + "/modelList/" + patch, true);
+        while (r.pos() < pos) r.next();
+        const std::string type = md->word("type");
+        if (type == "none") continue;
+        if (type != "SimpleInjection")
+            throw FoamError(md->where + ": patch boundary model " + type + " is not implemented (SimpleInjection is)");
+        auto it = std::find(c.patch_names.begin(), c.patch_names.end(), patch);
+        if (it == c.patch_names.end()) throw FoamError(md->where + ": no patch " + patch);
+        const Dict& cf = md->sub_required("SimpleInjectionCoeffs");
+        pic_injection q{};
+        q.patch = (int32_t)(it - c.patch_names.begin());
+        const std::string vm = cf.word("velocityModel");
+        const char* vms[] = {"birdMaxwellianFlux", "mostProbableSpeed", "halfMaxwellianFlux", "maxwellianFlux"};
+        q.velocity_model = -1;
+        for (int k = 0; k < 4; ++k)
+            if (vm == vms[k]) q.velocity_model = k;
+        if (q.velocity_model < 0)
+            cf.fail("velocityModel", "valid injection velocity models are: birdMaxwellianFlux mostProbableSpeed "
+                                     "maxwellianFlux halfMaxwellianFlux");
+        const std::vector<std::string> ids = cf.words("injectionTypeIds");
+        const std::vector<double> fr = cf.scalars("frequencies"), ev = cf.scalars("eV");
+        if (ids.size() != fr.size() || fr.size() != ev.size())
+            throw FoamError(cf.where + ": not the same number of entries in frequencies, eV, injectionTypeIds");
+        if (ids.empty() || ids.size() > 8) throw FoamError(cf.where + ": 1 to 8 injectionTypeIds");
+        q.n_species = (int32_t)ids.size();
+        for (size_t k = 0; k < ids.size(); ++k) {
+            auto sp = std::find(c.species_names.begin(), c.species_names.end(), ids[k]);
+            if (sp == c.species_names.end()) throw FoamError(cf.where + ": species " + ids[k] + " not defined in cloud.");
+            q.species[k] = (int32_t)(sp - c.species_names.begin());
+            q.frequency[k] = fr[k];
+            q.eV[k] = ev[k];
+        }
+        q.end_time = cf.scalar("injectionEndTime");
+        // interactions { <species> { type deletion|reflection|floating; useInitalVelocity b; } }
+        if (const Dict* in = cf.sub("interactions")) {
+            for (const picmi::foam::Entry& ie : in->entries) {
+                if (!ie.dict) continue;
+                auto sp = std::find(c.species_names.begin(), c.species_names.end(), ie.key);
+                if (sp == c.species_names.end()) throw FoamError(in->where + ": Species " + ie.key + " is not defined in cloud");
+                const std::string it2 = ie.dict->word("type");
+                int kind = -1;
+                if (it2 == "deletion") kind = 0;
+                else if (it2 == "reflection") kind = ie.dict->boolean_or("useInitalVelocity", false) ? 2 : 1;
+                else if (it2 == "floating") kind = 3;
+                else throw FoamError(ie.dict->where + ": interaction type " + it2 + " not defined. valid options [deletion/reflection/floating]");
+                q.interaction[sp - c.species_names.begin()] = kind;
+            }
+        }
+        c.injection.push_back(q);
+    }
+}
+
 void read_pic_properties(Case& c, const std::string& path) {
     auto dp = picmi::foam::read_dict_file(path);
     const Dict& d = *dp;
@@ -662,11 +728,12 @@ void read_pic_properties(Case& c, const std::string& path) {
     // models are not implemented
     std::string wall = "SpecularReflection";
     m.diffuse_fraction = 0.0;
+    c.injection.clear();
     if (const Dict* bm = d.sub("BoundaryModels")) {
-        for (const char* k : {"PatchBoundaryModels", "PatchEventModels"})
-            if (const Dict* l = bm->sub(k))
-                if (list_has_models(l->find("modelList")))
-                    throw FoamError(l->where + ": boundary and patch-event models are not implemented");
+        if (const Dict* l = bm->sub("PatchEventModels"))
+            if (list_has_models(l->find("modelList")))
+                throw FoamError(l->where + ": patch-event models are not implemented");
+        if (const Dict* l = bm->sub("PatchBoundaryModels")) read_patch_boundary_models(c, *l);
         wall = bm->word_or("WallReflectionModel", wall);
         if (wall == "MixedDiffuseSpecular")
             m.diffuse_fraction = bm->sub_required("MixedDiffuseSpecularCoeffs").scalar("diffuseFraction");
@@ -1740,6 +1807,16 @@ const char* pic_case_patch_name(const pic_case* c, int32_t p) {
     return c->patch_names[p].c_str();
 }
 
+int pic_case_injection(const pic_case* c, int32_t capacity, pic_injection* out, int32_t* n) {
+    if (!c || !n || capacity < 0 || (capacity > 0 && !out)) return PIC_ERR_INVALID;
+    *n = (int32_t)c->injection.size();
+    for (int32_t k = 0; k < *n && k < capacity; ++k) {
+        out[k] = c->injection[k];
+        out[k].time = c->now;
+    }
+    return PIC_OK;
+}
+
 int pic_case_describe(const pic_case* c, pic_mesh* mesh, pic_species* species, pic_models* models) {
     if (!c) return PIC_ERR_INVALID;
     if (mesh) {
@@ -1869,6 +1946,11 @@ int pic_case_create(pic_case* c, int32_t precision, double factor, pic_ctx** out
                 r = pic_set_cell_field(ctx, PIC_FIELD_BINARY_SIGMA_CR_MAX, c->smax_bin.data());
         }
         if (r != PIC_OK) return fail(r, "binary / ion-neutral (sigma cR)max");
+        if (!c->injection.empty()) {
+            std::vector<pic_injection> inj = c->injection;
+            for (pic_injection& q : inj) q.time = c->now;
+            if ((r = pic_set_injection(ctx, inj.data(), (int32_t)inj.size())) != PIC_OK) return fail(r, "SimpleInjection");
+        }
         if ((r = pic_deposit(ctx)) != PIC_OK) return fail(r, "charge density");
         if (c->post.active) {
             // picInitialise.C:213-237

@@ -268,7 +268,7 @@ def test_wall_case_settings_and_refusals(tmp_path, what):
     elif what == "boundary-model":
         _edit(props, "PatchBoundaryModels { modelList (); }",
               "PatchBoundaryModels { modelList ( anode { type FloatingPotential; } ); }")
-        needle = "boundary and patch-event models"
+        needle = "patch boundary model FloatingPotential is not implemented"
     else:
         # an unknown wall model is refused when the device context is made from the case,
         # with the valid names listed (WallReflectionModelNew.C:47-56), before the GPU is used
@@ -490,3 +490,46 @@ def test_reference_tutorial_cases_load(name, n_cells, per_species):
     assert c.n_cells == n_cells and c.species_names == ["e", "Ar+"]
     c.initialise(seed=1)
     assert [c.parcel_count(s) for s in range(2)] == [per_species, per_species]
+
+
+def test_simple_injection_case():
+    """PatchBoundaryModels.modelList with a SimpleInjection entry (SimpleInjection.C:34-175):
+    patch, velocity model, injectionTypeIds, frequencies, eV, injectionEndTime and the
+    per-species interactions, the start time as the injection's clock; an entry of type
+    none adds nothing."""
+    c = case(os.path.join(CASES, "injection_source"))
+    inj = c.injection()
+    assert len(inj) == 1
+    q = inj[0]
+    assert q["patch"] == 0 and q["velocity_model"] == 3 and q["species"] == [0, 1]
+    assert q["frequencies"] == [8.1e8, 8.1e8] and q["eV"] == [1.0, 0.1]
+    assert q["end_time"] == 1e-10 and q["time"] == 0.0 and q["interaction"] == [0] * 8
+    assert c.mesh_dict()["patches"][0]["type"] == "patch"
+
+
+@pytest.mark.parametrize("what", ["sizes", "species", "velocity", "interaction", "reflection"])
+def test_simple_injection_refusals(tmp_path, what):
+    from picfoam_amd.case import CaseError
+    d = copy_case("injection_source", tmp_path)
+    props = d / "constant" / "picProperties"
+    if what == "sizes":
+        _edit(props, "eV                  (1 0.1);", "eV                  (1);")
+        needle = "not the same number of entries"
+    elif what == "species":
+        _edit(props, "injectionTypeIds    (e Ar+);", "injectionTypeIds    (e Xe+);")
+        needle = "species Xe+ not defined in cloud"
+    elif what == "velocity":
+        _edit(props, "velocityModel       maxwellianFlux;", "velocityModel       fastest;")
+        needle = "valid injection velocity models"
+    elif what == "interaction":
+        _edit(props, "e   { type deletion; }", "e   { type absorb; }")
+        needle = "interaction type absorb not defined"
+    else:
+        # reflection at the injector is read but not built: refused when the context is made
+        _edit(props, "e   { type deletion; }", "e   { type reflection; useInitalVelocity true; }")
+        c = case(d)
+        assert c.injection()[0]["interaction"][0] == 2
+        return
+    with pytest.raises(CaseError) as e:
+        case(d)
+    assert needle in str(e.value)

@@ -400,3 +400,38 @@ def test_binary_thermalization_case_matches_oracle_and_relaxes(oracle, tmp_path)
     for tn in ("1e-07", "2e-07"):
         assert (d / tn / "lagrangian" / "pic" / "U").is_file(), tn
     cloud.close()
+
+
+def test_simple_injection_case_runs(tmp_path):
+    """The injection_source case (the cathode a plain patch with SimpleInjection of e and Ar+,
+    40.5 parcels of each per step until 1e-10 s): the context made from the case injects
+    40 and 41 parcels in turn for each species while the step's end time is within
+    injectionEndTime, none after, and the new parcels take part in the run (the stored
+    count is the injected ones less those that left)."""
+    from picfoam_amd.case import Case
+    d = copy_case("injection_source", tmp_path)
+    say("case injection_source: initialise")
+    cs = Case(d)
+    cs.initialise(seed=2)
+    t0, _, dt, _ = cs.times
+    cloud = cs.create("fp64", capacity_factor=4.0)
+    n0 = [cloud.count(s) for s in range(2)]
+    w = cs.describe()[1][0]["nEquivalentParticles"]
+    expect, rem, t = [], 0.0, t0
+    for _ in range(14):
+        te = t + dt
+        k = 0
+        if te <= 1e-10:
+            np_ = 8.1e8 / w * dt + rem
+            k = int(np_)
+            rem = np_ - k
+        expect.append(2 * k)
+        t = te
+    got, deleted = [], 0
+    for _ in range(14):
+        st = cloud.step(dt)
+        got.append(st.n_injected)
+        deleted += st.n_deleted
+    say("  injected per step", got)
+    assert got == expect and sum(got) > 0
+    assert sum(cloud.count(s) for s in range(2)) == sum(n0) + sum(got) - deleted
