@@ -224,7 +224,12 @@ typedef struct {
  * their position when the boundary sync is on (ParticleEmitter.C:338).  The new parcels move
  * their full step in the same pic_step.  interaction[] gives, per species of the cloud, what
  * a parcel reaching the patch does (SimpleInjection.C:233-292): 0 deletion (the default);
- * reflection and floating are refused (PIC_ERR_UNSUPPORTED). */
+ * 1 reflection: its velocity synchronised to the hit and reflected specularly, not
+ * synchronised back (PICParcel.C:158-161, :347-357); 2 reflection with useInitalVelocity:
+ * for a species the model injects, the parcel is replaced (in its slot, with its id) by one
+ * from the emitter, the triangle drawn uniformly (ParticleEmitter.C:136-221), synchronised
+ * to the hit's step fraction and tracked for the rest of the step, else as 1; 3 floating
+ * is refused (PIC_ERR_UNSUPPORTED: it needs the circuit boundary condition). */
 typedef struct {
     int32_t patch;
     int32_t velocity_model;       /* PIC_INJECT_* */
@@ -234,7 +239,7 @@ typedef struct {
     double eV[8];                 /* eV: temperature of the emitted distribution */
     double end_time;              /* injectionEndTime */
     double time;                  /* the time at step index 0; advanced by dt every step */
-    int32_t interaction[8];       /* per cloud species: 0 deletion */
+    int32_t interaction[8];       /* per cloud species: 0 deletion, 1 reflection, 2 with re-emission */
 } pic_injection;
 enum {
     PIC_INJECT_BIRD_MAXWELLIAN_FLUX = 0,  /* birdMaxwellianFlux */

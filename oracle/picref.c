@@ -714,9 +714,18 @@ struct track_ctx {
     int hits;      /* wall hits of this parcel in this step */
     int resyncs;   /* velocity resyncs after a reflection (PICParcel.C:126-137) */
     int64_t total_hits;
+    /* SimpleInjection re-emission: the injector patch's triangles */
+    const int *r_tface, *r_tj;
+    const double* r_cum;
+    int r_ntri;
 };
 
 #define TAG_WALL 6u
+#define TAG_REEMIT 12u
+static void emit_one(const ref_mesh* m, const ref_inject_params* p, const int* tface, const int* tj,
+                     const double* cum, int ntri, int by_area, uint32_t ka, uint32_t kc, uint32_t tag, double* y,
+                     int* tet, double* Uo);
+static int inject_table(const ref_mesh* m, int pt, int** tface, int** tj, double** cum);
 static inline double u01(uint32_t a, uint32_t b);
 static inline void rng4(uint64_t seed, uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t out[4]);
 
@@ -819,7 +828,44 @@ static int track_one(const ref_mesh* m, double* yy, double* Up, int* tetp, doubl
         int s = cross_face(m, t, iH, y);
         if (s == REF_EMPTY_FACE) break; /* on an empty face (rounding only): stays */
         if (s <= -1000000) { status = 2; break; }
-        if (s < 0 && x && x->walls && m->ptype[-1 - s] == REF_PATCH_WALL) {
+        const int kind = (s < 0 && s > -1000000 && x && x->walls)
+                             ? (x->walls->patch_kind ? x->walls->patch_kind[-1 - s]
+                                                     : (m->ptype[-1 - s] == REF_PATCH_WALL ? 1 : 0))
+                             : 0;
+        if (kind == 2 || kind == 3) {
+            /* SimpleInjection::particleBC on its patch (SimpleInjection.C:241-272) after the
+             * hit's sync of a plain patch (PICParcel.C:158-161): reflection without a resync,
+             * or the parcel replaced by a new one from the emitter at the same step fraction,
+             * synchronised to it (ParticleEmitter.C:136-221) */
+            const int patch = -1 - s;
+            if (kind == 2) {
+                v3 r1 = vcross(e2, e3), r2 = vcross(e3, e1), r3 = vcross(e1, e2);
+                v3 nn = vadd(vadd(r1, r2), r3);
+                nn = vdiv(nn, vmag(nn));
+                sync_velocity(m, x, t, y, Up, sf - 0.5);
+                v3 Uh = ld3(Up);
+                const double un = vdot(Uh, nn);
+                if (un > 0.0) st3(Up, vsub(Uh, vscale(2.0 * un, nn)));
+            } else {
+                const ref_inject_params* rp = x->walls->reemit;
+                double yy3[3];
+                const uint32_t kc = (x->walls->species << 16) | (uint32_t)(x->hits & 0xffff);
+                emit_one(m, rp, x->r_tface, x->r_tj, x->r_cum, x->r_ntri, 0, x->id, kc, TAG_REEMIT, yy3, &t, Up);
+                y[1] = yy3[0]; y[2] = yy3[1]; y[3] = yy3[2];
+                y[0] = 1.0 - (y[1] + y[2] + y[3]);
+                sync_velocity(m, x, t, y, Up, sf - 0.5);
+            }
+            (void)patch;
+            ++x->hits;
+            ++x->total_hits;
+            U = ld3(Up);
+            if (m->track_mask & 1) U.x = 0.0;
+            if (m->track_mask & 2) U.y = 0.0;
+            if (m->track_mask & 4) U.z = 0.0;
+            if (++guard > MAX_TRACK_STEPS) { status = 1; break; }
+            continue;
+        }
+        if (kind == 1) {
             /* PICParcel::hitPatch then the resync of moveForward: the parcel stays on the
              * face in its tet and tracks the rest of the step with the new velocity */
             const int patch = -1 - s;
@@ -853,6 +899,9 @@ int ref_move_walls(const ref_mesh* m, int pusher, int weighting, int n, double* 
                    const ref_wall_params* walls) {
     int guards = 0, sl = 0;
     int64_t hits = 0;
+    int *r_tface = NULL, *r_tj = NULL, r_ntri = 0;
+    double* r_cum = NULL;
+    if (walls && walls->reemit) r_ntri = inject_table(m, walls->reemit->patch, &r_tface, &r_tj, &r_cum);
     /* parcels are independent: spread over the host threads for the CPU baseline */
 #pragma omp parallel for schedule(static) reduction(+ : guards, sl, hits)
     for (int i = 0; i < n; ++i) {
@@ -865,10 +914,12 @@ int ref_move_walls(const ref_mesh* m, int pusher, int weighting, int n, double* 
         }
         if (vmag(ld3(U + 3 * i)) >= C_LIGHT) ++sl; /* PICParcel.C:61-66 */
         struct track_ctx x = {pusher, weighting, hc_reference_form, rotate, do_push, q, mass, dt,
-                              Ecell, Epoint, Bcell, walls, walls ? walls->id[i] : 0u, 0, 0, 0};
+                              Ecell, Epoint, Bcell, walls, walls ? walls->id[i] : 0u, 0, 0, 0,
+                              r_tface, r_tj, r_cum, r_ntri};
         if (track_one(m, y + 3 * i, U + 3 * i, tet + i, dt, &x)) ++guards;
         hits += x.total_hits;
     }
+    free(r_tface); free(r_tj); free(r_cum);
     if (n_superluminal) *n_superluminal = sl;
     if (walls && walls->hits) *walls->hits += hits;
     return guards;
@@ -2273,15 +2324,15 @@ void ref_grad(const ref_poisson* P, const double* phi, double* gr) {
  * centre, t2 = n x t1.  Charged parcels are pushed by -dt/2 when p->sync is set
  * (ParticleEmitter.C:338, PICParcel.C:214-227).  The draws are Philox keyed by
  * (seed; k, step, injector << 16 | species, TAG_INJECT << 24 | draw). */
-void ref_inject(const ref_mesh* m, const ref_inject_params* p, int count, double* y, int* tet, double* U,
-                uint32_t* id) {
-    const int pt = p->patch;
+/* the patch's faces fanned from their first point: triangle -> face, its second point's
+ * index in the face, and the cumulative areas from 0 (ParticleEmitter.C:78-131) */
+static int inject_table(const ref_mesh* m, int pt, int** tface, int** tj, double** cum) {
     int ntri = 0;
     for (int f = m->pstart[pt]; f < m->pstart[pt] + m->psize[pt]; ++f) ntri += m->fo[f + 1] - m->fo[f] - 2;
-    int* tface = (int*)malloc(sizeof(int) * (ntri > 0 ? ntri : 1));
-    int* tj = (int*)malloc(sizeof(int) * (ntri > 0 ? ntri : 1));
-    double* cum = (double*)malloc(sizeof(double) * (ntri + 1));
-    cum[0] = 0.0;
+    *tface = (int*)malloc(sizeof(int) * (ntri > 0 ? ntri : 1));
+    *tj = (int*)malloc(sizeof(int) * (ntri > 0 ? ntri : 1));
+    *cum = (double*)malloc(sizeof(double) * (ntri + 1));
+    (*cum)[0] = 0.0;
     int k = 0;
     for (int f = m->pstart[pt]; f < m->pstart[pt] + m->psize[pt]; ++f) {
         const int* fp = m->fp + m->fo[f];
@@ -2289,82 +2340,105 @@ void ref_inject(const ref_mesh* m, const ref_inject_params* p, int count, double
         v3 a = ld3(m->points + 3 * fp[0]);
         for (int j = 1; j + 1 < n; ++j) {
             v3 b = ld3(m->points + 3 * fp[j]), c = ld3(m->points + 3 * fp[j + 1]);
-            cum[k + 1] = cum[k] + 0.5 * vmag(vcross(vsub(b, a), vsub(c, a)));
-            tface[k] = f; tj[k] = j;
+            (*cum)[k + 1] = (*cum)[k] + 0.5 * vmag(vcross(vsub(b, a), vsub(c, a)));
+            (*tface)[k] = f; (*tj)[k] = j;
             ++k;
         }
     }
-    const uint32_t ckey = (p->model << 16) | p->species;
+    return ntri;
+}
+
+/* One emitted parcel: position, tet and velocity (no synchronisation).  by_area picks the
+ * triangle by area (emitParticle, ParticleEmitter.C:232-262), else uniformly over the
+ * triangles (emitParticleExplicit, :143-146).  Draws keyed (seed; ka, step, kc, tag << 24 |
+ * draw). */
+static void emit_one(const ref_mesh* m, const ref_inject_params* p, const int* tface, const int* tj,
+                     const double* cum, int ntri, int by_area, uint32_t ka, uint32_t kc, uint32_t tag, double* y,
+                     int* tet, double* Uo) {
     const double vt = sqrt(KB * p->T / p->mass);
     const double vmp = sqrt(2.0 * KB * p->T / p->mass);
-    for (int i = 0; i < count; ++i) {
-        uint32_t o[4];
-        rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, TAG_INJECT << 24, o);
-        const double area = u01(o[0], o[1]) * cum[ntri], s = u01(o[2], o[3]);
-        int tri = 0;
+    uint32_t o[4];
+    rng4(p->seed, ka, (uint32_t)p->step, kc, tag << 24, o);
+    const double u = u01(o[0], o[1]), s = u01(o[2], o[3]);
+    int tri = 0;
+    if (by_area) {
+        const double area = u * cum[ntri];
         for (int q = ntri - 1; q >= 0; --q)
             if (area > cum[q]) { tri = q; break; }
-        const int f = tface[tri];
-        const int* fp = m->fp + m->fo[f];
-        v3 a = ld3(m->points + 3 * fp[0]), b = ld3(m->points + 3 * fp[tj[tri]]),
-           c = ld3(m->points + 3 * fp[tj[tri] + 1]);
-        rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_INJECT << 24) | 1u, o);
-        const double t = sqrt(u01(o[0], o[1])), frac = 0.1 + 0.4 * u01(o[2], o[3]);
-        v3 pf = vadd(vadd(vscale(1.0 - t, a), vscale((1.0 - s) * t, b)), vscale(s * t, c));
-        v3 Sf = ld3(m->Sf + 3 * f);
-        v3 nout = vdiv(Sf, vmag(Sf));
-        const int own = m->owner[f];
-        v3 pc = ld3(m->C + 3 * own);
-        v3 d = vscale(fabs(vdot(vsub(pf, pc), nout)), nout);
-        v3 x = vsub(pf, vscale(frac, d));
-        if (m->track_mask & 1) x.x = 0.0;
-        if (m->track_mask & 2) x.y = 0.0;
-        if (m->track_mask & 4) x.z = 0.0;
-        double xx[3] = {x.x, x.y, x.z};
-        ref_locate(m, 1, xx, &own, y + 3 * i, tet + i);
-        /* the frame of the face (ParticleEmitter.C:308-322) */
-        v3 nin = vscale(-1.0, nout);
-        v3 t1 = vsub(ld3(m->Cf + 3 * f), a);
-        t1 = vdiv(t1, vmag(t1));
-        v3 t2 = vcross(nin, t1);
-        t2 = vdiv(t2, vmag(t2));
-        double n1 = 0.0, n2 = 0.0, vn;
-        if (p->velocity_model != 1) {
-            rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_INJECT << 24) | 2u, o);
-            box_muller(u01(o[0], o[1]), u01(o[2], o[3]), &n1, &n2);
+    } else {
+        tri = (int)(u * (double)ntri);
+        if (tri > ntri - 1) tri = ntri - 1;
+    }
+    const int f = tface[tri];
+    const int* fp = m->fp + m->fo[f];
+    v3 a = ld3(m->points + 3 * fp[0]), b = ld3(m->points + 3 * fp[tj[tri]]), c = ld3(m->points + 3 * fp[tj[tri] + 1]);
+    rng4(p->seed, ka, (uint32_t)p->step, kc, (tag << 24) | 1u, o);
+    const double t = sqrt(u01(o[0], o[1])), frac = 0.1 + 0.4 * u01(o[2], o[3]);
+    v3 pf = vadd(vadd(vscale(1.0 - t, a), vscale((1.0 - s) * t, b)), vscale(s * t, c));
+    v3 Sf = ld3(m->Sf + 3 * f);
+    v3 nout = vdiv(Sf, vmag(Sf));
+    const int own = m->owner[f];
+    v3 pc = ld3(m->C + 3 * own);
+    v3 d = vscale(fabs(vdot(vsub(pf, pc), nout)), nout);
+    v3 x = vsub(pf, vscale(frac, d));
+    if (m->track_mask & 1) x.x = 0.0;
+    if (m->track_mask & 2) x.y = 0.0;
+    if (m->track_mask & 4) x.z = 0.0;
+    double xx[3] = {x.x, x.y, x.z};
+    ref_locate(m, 1, xx, &own, y, tet);
+    /* the frame of the face (ParticleEmitter.C:308-322) */
+    v3 nin = vscale(-1.0, nout);
+    v3 t1 = vsub(ld3(m->Cf + 3 * f), a);
+    t1 = vdiv(t1, vmag(t1));
+    v3 t2 = vcross(nin, t1);
+    t2 = vdiv(t2, vmag(t2));
+    double n1 = 0.0, n2 = 0.0, vn;
+    if (p->velocity_model != 1) {
+        rng4(p->seed, ka, (uint32_t)p->step, kc, (tag << 24) | 2u, o);
+        box_muller(u01(o[0], o[1]), u01(o[2], o[3]), &n1, &n2);
+    }
+    if (p->velocity_model == 0) {
+        /* Bird eqn 12.5: uNormal = 3 (2u - 1) accepted with P = 2 u / sqrt(2) exp(1/2 - u^2) */
+        double un = 0.0;
+        for (uint32_t j = 4; j < 4 + 2000; ++j) {
+            rng4(p->seed, ka, (uint32_t)p->step, kc, (tag << 24) | j, o);
+            un = 3.0 * (2.0 * u01(o[0], o[1]) - 1.0);
+            double P = un < 0.0 ? -1.0 : 2.0 * un / sqrt(2.0) * exp(0.5 - un * un);
+            if (!(P < u01(o[2], o[3]))) break;
         }
-        if (p->velocity_model == 0) {
-            /* Bird eqn 12.5: uNormal = 3 (2u - 1) accepted with P = 2 u / sqrt(2) exp(1/2 - u^2) */
-            double un = 0.0;
-            for (uint32_t j = 4; j < 4 + 2000; ++j) {
-                rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_INJECT << 24) | j, o);
-                un = 3.0 * (2.0 * u01(o[0], o[1]) - 1.0);
-                double P = un < 0.0 ? -1.0 : 2.0 * un / sqrt(2.0) * exp(0.5 - un * un);
-                if (!(P < u01(o[2], o[3]))) break;
-            }
-            vn = vmp * un;
-        } else if (p->velocity_model == 1) {
-            vn = vmp;
+        vn = vmp * un;
+    } else if (p->velocity_model == 1) {
+        vn = vmp;
+    } else {
+        rng4(p->seed, ka, (uint32_t)p->step, kc, (tag << 24) | 3u, o);
+        if (p->velocity_model == 2) {
+            double g1, g2;
+            box_muller(u01(o[0], o[1]), u01(o[2], o[3]), &g1, &g2);
+            vn = fabs(g1) * vt;
         } else {
-            rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_INJECT << 24) | 3u, o);
-            if (p->velocity_model == 2) {
-                double g1, g2;
-                box_muller(u01(o[0], o[1]), u01(o[2], o[3]), &g1, &g2);
-                vn = fabs(g1) * vt;
-            } else {
-                vn = vt * sqrt(-2.0 * log(1.0 - u01(o[0], o[1])));
-            }
+            vn = vt * sqrt(-2.0 * log(1.0 - u01(o[0], o[1])));
         }
-        v3 u = vadd(vadd(vscale(vt * n1, t1), vscale(vt * n2, t2)), vscale(vn, nin));
-        double uu[3] = {u.x, u.y, u.z};
+    }
+    v3 uu = vadd(vadd(vscale(vt * n1, t1), vscale(vt * n2, t2)), vscale(vn, nin));
+    st3(Uo, uu);
+}
+
+void ref_inject(const ref_mesh* m, const ref_inject_params* p, int count, double* y, int* tet, double* U,
+                uint32_t* id) {
+    int *tface, *tj;
+    double* cum;
+    const int ntri = inject_table(m, p->patch, &tface, &tj, &cum);
+    const uint32_t ckey = (p->model << 16) | p->species;
+    for (int i = 0; i < count; ++i) {
+        uint32_t o[4];
+        emit_one(m, p, tface, tj, cum, ntri, 1, (uint32_t)i, ckey, TAG_INJECT, y + 3 * i, tet + i, U + 3 * i);
         if (p->sync && p->q != 0.0 && tet[i] >= 0) {
             v3 E = interp_one(m, p->weighting, tet[i], y + 3 * i, p->Ecell, p->Epoint);
             double e[3] = {E.x, E.y, E.z};
             double zero[3] = {0.0, 0.0, 0.0};
             const double* B = p->Bcell ? p->Bcell + 3 * m->tet_cell[tet[i]] : zero;
-            ref_push_one(p->pusher, uu, e, B, p->q, p->mass, -0.5 * p->dt, p->hc_reference_form, p->rotate);
+            ref_push_one(p->pusher, U + 3 * i, e, B, p->q, p->mass, -0.5 * p->dt, p->hc_reference_form, p->rotate);
         }
-        U[3 * i] = uu[0]; U[3 * i + 1] = uu[1]; U[3 * i + 2] = uu[2];
         rng4(p->seed, (uint32_t)i, (uint32_t)p->step, ckey, (TAG_NEWID << 24) | 3u, o);
         if (id) id[i] = o[0];
     }

@@ -80,6 +80,13 @@ typedef struct {
     uint32_t species;
     const uint32_t* id;        /* [n] parcel ids: the keys of the reflection draws */
     int64_t* hits;             /* optional [1]: wall hits counted */
+    /* optional [n_patches]: what a face of each patch does to this species (NULL: walls
+     * reflect by the model, other non-coupled patches delete): 0 delete, 1 the wall model,
+     * 2 SimpleInjection's reflection (hit sync, specular, no resync; SimpleInjection.C:
+     * 245-252, PICParcel.C:347-357), 3 its re-emission with useInitalVelocity (:253-270,
+     * ParticleEmitter.C:136-221) by the emitter `reemit` describes */
+    const int* patch_kind;
+    const struct ref_inject_params_s* reemit;
 } ref_wall_params;
 int ref_move_walls(const ref_mesh* m, int pusher, int weighting, int n, double* y, double* U, int* tet, double q,
                    double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell,
@@ -91,7 +98,7 @@ int ref_move_walls(const ref_mesh* m, int pusher, int weighting, int n, double* 
  * from one injector patch.  velocity_model: 0 birdMaxwellianFlux, 1 mostProbableSpeed,
  * 2 halfMaxwellianFlux, 3 maxwellianFlux.  Writes y[3*count], tet[count], U[3*count] and,
  * when id is not NULL, id[count]. */
-typedef struct {
+typedef struct ref_inject_params_s {
     int patch;
     int velocity_model;
     double T;                  /* K: eV * e / k */

@@ -22,7 +22,7 @@ WALL_MODELS = {"SpecularReflection": 0, "MaxwellianThermal": 1, "MixedDiffuseSpe
 class WallParams(C.Structure):
     _fields_ = [("model", C.c_int), ("diffuse_fraction", C.c_double), ("T", C.c_void_p), ("U", C.c_void_p),
                 ("sync", C.c_int), ("seed", C.c_uint64), ("step", C.c_int64), ("species", C.c_uint32),
-                ("id", C.c_void_p), ("hits", C.c_void_p)]
+                ("id", C.c_void_p), ("hits", C.c_void_p), ("patch_kind", C.c_void_p), ("reemit", C.c_void_p)]
 
 
 class InjectParams(C.Structure):
@@ -200,9 +200,18 @@ class RefMesh:
         wU = np.ascontiguousarray(walls["U"], np.float64)
         ids = np.ascontiguousarray(walls["ids"], np.uint32)
         hits = np.zeros(1, np.int64)
+        pk = walls.get("patch_kind")
+        pk = None if pk is None else np.ascontiguousarray(pk, np.int32)
+        re = walls.get("reemit")
+        rp = None
+        if re is not None:
+            # the injector's emitter for re-emitted parcels: patch, velocity_model, T, mass
+            rp = InjectParams(patch=re["patch"], velocity_model=re["velocity_model"], T=re["T"], mass=mass,
+                              seed=int(walls["seed"]), step=int(walls["step"]), species=int(walls["species"]))
         w = WallParams(WALL_MODELS[walls["model"]], float(walls.get("diffuse_fraction", 0.0)), _p(wT), _p(wU),
                        int(walls.get("sync", True)), int(walls["seed"]), int(walls["step"]), int(walls["species"]),
-                       _p(ids), _p(hits))
+                       _p(ids), _p(hits), None if pk is None else _p(pk),
+                       None if rp is None else C.cast(C.pointer(rp), C.c_void_p))
         g = lib().ref_move_walls(self.h, PUSHERS[pusher], WEIGHTINGS[weighting], len(tet), _p(y), _p(U), _p(tet), q,
                                  mass, dt, _p(Ecell), _p(Epoint), _p(Bcell), int(do_push), int(hc_ref), int(rotate),
                                  C.byref(sl), C.byref(w))

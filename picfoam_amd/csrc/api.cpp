@@ -102,6 +102,15 @@ struct pic_ctx {
         int ntri = 0;
     };
     std::vector<Injector> inj;
+    // per species (rows of kMaxSpecies) and patch: what a parcel reaching the patch does
+    // (0 delete, 1 wall model, 2 specular without resync, 3 re-emission), and per species
+    // the injector that re-emits it
+    std::vector<uint8_t> h_kind;
+    struct Reemit {
+        const InjectTri* tri = nullptr;
+        int ntri = 0, vmodel = 0;
+        double vt = 0.0;
+    } re[kMaxSpecies];
     double inj_time = 0.0;             // time at the start of the next step
     int64_t n_injected = 0;            // parcels injected since the statistics were reset
     uint64_t seed = 0;
@@ -371,8 +380,13 @@ MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
     a.rotate = c->rotate ? 1 : 0;
     a.hc_ref = c->hc_ref;
     a.track_mask = c->hm.track_mask;
+    a.cell_tet0 = c->d_cell_tet0;
     if (c->walls) {
-        a.patch_kind = c->d_patch_kind;
+        a.patch_kind = c->d_patch_kind + (size_t)s * c->hm.n_patches;
+        a.re_tri = c->re[s].tri;
+        a.re_ntri = c->re[s].ntri;
+        a.re_vmodel = c->re[s].vmodel;
+        a.re_vt = c->re[s].vt;
         a.wall_T = c->d_wall_T;
         a.wall_U = c->d_wall_U;
         a.wall_model = c->wall_model;
@@ -1255,15 +1269,16 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
     const int nc = m.n_cells, np = m.n_points;
     c->d_tets = c->upload(m.tets);
     if (c->walls) {
-        std::vector<uint8_t> kind(m.n_patches, 0);
+        std::vector<uint8_t> kind((size_t)kMaxSpecies * m.n_patches, 0);
         std::vector<double> wT(m.n_patches, 0.0), wU(3 * (size_t)m.n_patches, 0.0);
         for (int p = 0; p < m.n_patches; ++p) {
             const pic_patch& pp = mesh->patches[p];
             if (pp.type != PIC_PATCH_WALL) continue;
-            kind[p] = 1;
+            for (int s = 0; s < kMaxSpecies; ++s) kind[(size_t)s * m.n_patches + p] = 1;
             wT[p] = pp.wall_temperature;
             for (int d = 0; d < 3; ++d) wU[3 * p + d] = pp.wall_velocity[d];
         }
+        c->h_kind = kind;
         c->d_patch_kind = c->upload(kind);
         c->d_wall_T = c->upload(wT);
         c->d_wall_U = c->upload(wU);
@@ -1967,8 +1982,9 @@ int pic_set_injection(pic_ctx* c, const pic_injection* models, int32_t n) {
             if (!(q.frequency[e] >= 0.0) || !(q.eV[e] >= 0.0)) return c->fail(PIC_ERR_INVALID, tag + "frequencies and eV must be >= 0");
         }
         for (int s = 0; s < ns; ++s) {
-            if (q.interaction[s] == 0) continue;
-            return c->fail(PIC_ERR_UNSUPPORTED, tag + "only the deletion interaction is built (reflection and floating are not)");
+            if (q.interaction[s] < 0 || q.interaction[s] > 3) return c->fail(PIC_ERR_INVALID, tag + "interaction 0-3");
+            if (q.interaction[s] == 3)
+                return c->fail(PIC_ERR_UNSUPPORTED, tag + "the floating interaction needs the circuit boundary condition, which is not built");
         }
         out[k].cfg = q;
     }
@@ -2023,6 +2039,51 @@ int pic_set_injection(pic_ctx* c, const pic_injection* models, int32_t n) {
         c->inj.push_back(I);
     }
     if (n > 0) c->inj_time = models[0].time;
+    // the interactions on the injector patches (SimpleInjection.C:233-292): 1 reflection
+    // (kind 2), 2 reflection with useInitalVelocity (kind 3, or 2 for a species the model
+    // does not inject); the wall queue and tables are made when the mesh has no walls
+    bool any = false;
+    for (auto& I : c->inj)
+        for (int s = 0; s < ns; ++s) any = any || I.cfg.interaction[s] != 0;
+    if (any && !c->walls) {
+        size_t cap = 0;
+        for (auto& S : c->sp) cap += (size_t)S.capacity;
+        if (cap > 0xFFFFFFFFull) return c->fail(PIC_ERR_CAPACITY, "wall queue: total capacity exceeds 2^32 parcels");
+        c->wall_cap = (unsigned)cap;
+        c->d_wall_q = c->alloc<WallHit>(std::max<size_t>(cap, 1));
+        c->d_wall_n = c->alloc<unsigned>(1);
+        c->h_kind.assign((size_t)kMaxSpecies * m.n_patches, 0);
+        c->d_patch_kind = c->upload(c->h_kind);
+        c->d_wall_T = c->alloc<double>(m.n_patches);
+        c->d_wall_U = c->alloc<double>(3 * (size_t)m.n_patches);
+        if (!c->d_wall_q || !c->d_wall_n || !c->d_patch_kind || !c->d_wall_T || !c->d_wall_U)
+            return c->fail(PIC_ERR_HIP, "out of device memory (wall queue)");
+        c->walls = true;
+    }
+    if (c->walls) {
+        for (int s = 0; s < kMaxSpecies; ++s) c->re[s] = pic_ctx::Reemit{};
+        for (int p = 0; p < m.n_patches; ++p)
+            if (m.ptype[p] == PIC_PATCH_PATCH)
+                for (int s = 0; s < kMaxSpecies; ++s) c->h_kind[(size_t)s * m.n_patches + p] = 0;
+        for (auto& I : c->inj)
+            for (int s = 0; s < ns; ++s) {
+                const int it = I.cfg.interaction[s];
+                if (it == 0) continue;
+                int e = -1;
+                for (int k = 0; k < I.cfg.n_species; ++k)
+                    if (I.cfg.species[k] == s) e = k;
+                uint8_t kind = (it == 2 && e >= 0) ? 3 : 2;
+                if (kind == 3) {
+                    if (c->re[s].tri) return c->fail(PIC_ERR_UNSUPPORTED, "SimpleInjection: a species re-emitted by two models");
+                    c->re[s].tri = I.d_tri;
+                    c->re[s].ntri = I.ntri;
+                    c->re[s].vmodel = I.cfg.velocity_model;
+                    c->re[s].vt = std::sqrt(kKB * (I.cfg.eV[e] * kQe / kKB) / c->sp[s].mass);
+                }
+                c->h_kind[(size_t)s * m.n_patches + I.cfg.patch] = kind;
+            }
+        hipMemcpyAsync(c->d_patch_kind, c->h_kind.data(), c->h_kind.size(), hipMemcpyHostToDevice, c->stream);
+    }
     return PIC_OK;
 }
 

@@ -142,7 +142,7 @@ __device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
 }
 
 constexpr uint32_t kTagShuffle = 1u, kTagCoulomb = 2u, kTagMcc = 3u, kTagNewId = 4u, kTagInit = 5u, kTagWall = 6u,
-                   kTagIonMcc = 7u, kTagIonPair = 8u, kTagDsmc = 9u, kTagDsmcBg = 10u, kTagInject = 11u;
+                   kTagIonMcc = 7u, kTagIonPair = 8u, kTagDsmc = 9u, kTagDsmcBg = 10u, kTagInject = 11u, kTagReemit = 12u;
 
 // error flags raised by kernels (bitmask in device memory)
 enum : uint32_t {

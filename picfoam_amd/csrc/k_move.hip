@@ -183,7 +183,7 @@ __device__ __forceinline__ int track(const MoveArgs<UT>& a, int t, double& y1, d
             Y0 = iH == 0 ? 0.0 : Z0; Y1 = iH == 1 ? 0.0 : Z1;
             Y2 = iH == 2 ? 0.0 : Z2; Y3 = iH == 3 ? 0.0 : Z3;
             if (nb == kTetEmptyFace) break;
-            if (nb > -1000000 && a.patch_kind && a.patch_kind[-1 - nb] == 1) {
+            if (nb > -1000000 && a.patch_kind && a.patch_kind[-1 - nb] != 0) {
                 status = 3;
                 break;
             }
@@ -414,6 +414,34 @@ __global__ void __launch_bounds__(kMoveBlock) k_move(MoveSet<UT> m) {
     }
 }
 
+// SimpleInjection's reflection on its own patch (SimpleInjection.C:245-252 after the hit's
+// sync of a plain patch, PICParcel.C:158-161; specularReflect, PICParcel.C:347-357): the
+// velocity synchronised to the hit time and reflected about the face's outward normal, and
+// not synchronised back (the reference sets no resync for it).
+template <class UT, int P, int W>
+__device__ __forceinline__ void patch_reflect(const MoveArgs<UT>& a, const TetGeo& g, double Y1, double Y2, double Y3,
+                                              double sf, vec3<UT>& U) {
+    double nx = g.r[0] + g.r[3] + g.r[6], ny = g.r[1] + g.r[4] + g.r[7], nz = g.r[2] + g.r[5] + g.r[8];
+    const double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+    nx *= inv; ny *= inv; nz *= inv;
+    if (a.wall_sync && a.do_push) {
+        const vec3<UT> E = interp_E<UT, W>(g, Y1, Y2, Y3, a.Ecell, a.Epoint);
+        vec3<UT> B = mk3<UT>(0, 0, 0);
+        if (a.rotate) B = mk3<UT>(a.Bcell[3 * g.cell], a.Bcell[3 * g.cell + 1], a.Bcell[3 * g.cell + 2]);
+        const UT f = (UT)(sf - 0.5);
+        U = push_u<UT, P>(U, E, B, a.k * f, a.kv * f, a.rotate != 0, a.hc_ref != 0);
+    }
+    double ux = (double)U.x, uy = (double)U.y, uz = (double)U.z;
+    const double un = ux * nx + uy * ny + uz * nz;
+    if (un > 0.0) { ux -= 2.0 * un * nx; uy -= 2.0 * un * ny; uz -= 2.0 * un * nz; }
+    U = mk3<UT>((UT)ux, (UT)uy, (UT)uz);
+}
+
+template <class UT>
+__device__ void emit(const MoveArgs<UT>& m, const InjectTri* __restrict__ tri, const double* __restrict__ cum,
+                     int ntri, const int32_t* __restrict__ cell_tet0, int vmodel, double vt, bool by_area,
+                     uint32_t k, uint32_t c, uint32_t tag, int& t, double& y1, double& y2, double& y3, double U[3]);
+
 // The wall pass: every queued parcel is reflected on its wall (wall_hit), tracks the rest of
 // its step (reaching further walls, reflected here in turn: the second and later ones
 // without the resync), and is then counted, placed and deposited as the move does for the
@@ -438,8 +466,35 @@ __global__ void __launch_bounds__(kMoveBlock) k_wall(MoveSet<UT> m) {
         int status = 3, hits = 0, resyncs = 0;
         while (status == 3) {
             const TetGeo& g = a.tets[t];
-            wall_hit<UT, P, W>(a, g, y1, y2, y3, sf, -1 - g.nbr[0], U, id, hits++, resyncs);
-            t = track(a, t, y1, y2, y3, U, sf, status, ys, true);
+            const int patch = -1 - g.nbr[0];
+            const int kind = a.patch_kind[patch];
+            bool on_face = true;
+            if (kind == 1) {
+                wall_hit<UT, P, W>(a, g, y1, y2, y3, sf, patch, U, id, hits++, resyncs);
+            } else if (kind == 2) {
+                patch_reflect<UT, P, W>(a, g, y1, y2, y3, sf, U);
+                ++hits;
+            } else {
+                // SimpleInjection.C:253-270: a new parcel from the injector's emitter at the hit's
+                // step fraction (ParticleEmitter.C:136-221), synchronised to it, in this slot
+                double Un[3];
+                const uint32_t c = (a.species << 16) | ((uint32_t)hits & 0xffffu);
+                emit(a, a.re_tri, nullptr, a.re_ntri, a.cell_tet0, a.re_vmodel, a.re_vt, false, id, c, kTagReemit, t,
+                     y1, y2, y3, Un);
+                U = mk3<UT>((UT)Un[0], (UT)Un[1], (UT)Un[2]);
+                if (a.wall_sync && a.do_push && t >= 0) {
+                    const TetGeo& h = a.tets[t];
+                    const vec3<UT> E = interp_E<UT, W>(h, y1, y2, y3, a.Ecell, a.Epoint);
+                    vec3<UT> B = mk3<UT>(0, 0, 0);
+                    if (a.rotate) B = mk3<UT>(a.Bcell[3 * h.cell], a.Bcell[3 * h.cell + 1], a.Bcell[3 * h.cell + 2]);
+                    const UT f = (UT)(sf - 0.5);
+                    U = push_u<UT, P>(U, E, B, a.k * f, a.kv * f, a.rotate != 0, a.hc_ref != 0);
+                }
+                ++hits;
+                on_face = false;
+                if (t < 0) { status = 2; break; }
+            }
+            t = track(a, t, y1, y2, y3, U, sf, status, ys, on_face);
             if (status == 3 && hits > kMaxTrackSteps) { status = 1; break; }
         }
         a.p.ux[i] = U.x; a.p.uy[i] = U.y; a.p.uz[i] = U.z;
@@ -554,19 +609,27 @@ __global__ void k_locate(LocateArgs<UT> a) {
 // tet with the largest minimum barycentric coordinate, as k_locate), and a velocity from
 // the model (ParticleEmitter.C:357-535) in the frame (inward normal, t1, t2).  The draws
 // are Philox keyed by (seed; k, step, injector and species, tag and draw).
+// by_area false: the triangle uniformly over the patch's triangles, as the re-emission of a
+// reflected parcel draws it (emitParticleExplicit, ParticleEmitter.C:143-146).
 template <class UT>
-__device__ void emit(const InjectArgs<UT>& a, uint32_t k, int& t, double& y1, double& y2, double& y3, double U[3]) {
-    const MoveArgs<UT>& m = a.m;
-    const uint32_t c = (a.model << 16) | m.species;
-    uint4 o = rng4(m.seed, k, (uint32_t)m.step, c, kTagInject << 24);
-    const double area = u01(o.x, o.y) * a.cum[a.ntri], s = u01(o.z, o.w);
-    int lo = 0, hi = a.ntri - 1;   // the last triangle whose cumulative start lies below the area
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.cum[mid] < area) lo = mid; else hi = mid - 1;
+__device__ void emit(const MoveArgs<UT>& m, const InjectTri* __restrict__ tri, const double* __restrict__ cum,
+                     int ntri, const int32_t* __restrict__ cell_tet0, int vmodel, double vt, bool by_area,
+                     uint32_t k, uint32_t c, uint32_t tag, int& t, double& y1, double& y2, double& y3, double U[3]) {
+    uint4 o = rng4(m.seed, k, (uint32_t)m.step, c, tag << 24);
+    const double u = u01(o.x, o.y), s = u01(o.z, o.w);
+    int lo = 0;
+    if (by_area) {
+        const double area = u * cum[ntri];
+        int hi = ntri - 1;   // the last triangle whose cumulative start lies below the area
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cum[mid] < area) lo = mid; else hi = mid - 1;
+        }
+    } else {
+        lo = min((int)(u * (double)ntri), ntri - 1);
     }
-    const InjectTri& T = a.tri[lo];
-    o = rng4(m.seed, k, (uint32_t)m.step, c, (kTagInject << 24) | 1u);
+    const InjectTri& T = tri[lo];
+    o = rng4(m.seed, k, (uint32_t)m.step, c, (tag << 24) | 1u);
     const double tt = sqrt(u01(o.x, o.y)), frac = 0.1 + 0.4 * u01(o.z, o.w);
     const double wa = 1.0 - tt, wb = (1.0 - s) * tt, wc = s * tt;
     double x[3], dn = 0.0;
@@ -584,7 +647,7 @@ __device__ void emit(const InjectArgs<UT>& a, uint32_t k, int& t, double& y1, do
     int best = -1;
     double bestmin = -1e300;
     y1 = y2 = y3 = 0.0;
-    for (int q = a.cell_tet0[cell]; q < a.cell_tet0[cell + 1]; ++q) {
+    for (int q = cell_tet0[cell]; q < cell_tet0[cell + 1]; ++q) {
         const TetGeo& g = m.tets[q];
         const double v1 = (g.r[0] * dx + g.r[1] * dy + g.r[2] * dz) / g.det;
         const double v2 = (g.r[3] * dx + g.r[4] * dy + g.r[5] * dz) / g.det;
@@ -595,32 +658,32 @@ __device__ void emit(const InjectArgs<UT>& a, uint32_t k, int& t, double& y1, do
     t = best;
     // velocity in the frame of the face: n inward, t1, t2 tangential
     double vn = 0.0, n1 = 0.0, n2 = 0.0;
-    if (a.vmodel != 1) {
-        o = rng4(m.seed, k, (uint32_t)m.step, c, (kTagInject << 24) | 2u);
+    if (vmodel != 1) {
+        o = rng4(m.seed, k, (uint32_t)m.step, c, (tag << 24) | 2u);
         box_muller(u01(o.x, o.y), u01(o.z, o.w), n1, n2);
-        n1 *= a.vt; n2 *= a.vt;
+        n1 *= vt; n2 *= vt;
     }
-    const double vmp = sqrt(2.0) * a.vt;   // maxwellianMostProbableSpeed, PICCloudI.H:541-548
-    if (a.vmodel == 0) {
+    const double vmp = sqrt(2.0) * vt;   // maxwellianMostProbableSpeed, PICCloudI.H:541-548
+    if (vmodel == 0) {
         // Bird eqn 12.5 by acceptance-rejection (ParticleEmitter.C:380-398)
         double un = 0.0;
         for (uint32_t j = 4; j < 4 + 2000; ++j) {
-            o = rng4(m.seed, k, (uint32_t)m.step, c, (kTagInject << 24) | j);
+            o = rng4(m.seed, k, (uint32_t)m.step, c, (tag << 24) | j);
             un = 3.0 * (2.0 * u01(o.x, o.y) - 1.0);
             const double P = un < 0.0 ? -1.0 : 2.0 * un / sqrt(2.0) * exp(0.5 - un * un);
             if (!(P < u01(o.z, o.w))) break;
         }
         vn = vmp * un;
-    } else if (a.vmodel == 1) {
+    } else if (vmodel == 1) {
         vn = vmp;
     } else {
-        o = rng4(m.seed, k, (uint32_t)m.step, c, (kTagInject << 24) | 3u);
-        if (a.vmodel == 2) {
+        o = rng4(m.seed, k, (uint32_t)m.step, c, (tag << 24) | 3u);
+        if (vmodel == 2) {
             double g1, g2;
             box_muller(u01(o.x, o.y), u01(o.z, o.w), g1, g2);
-            vn = fabs(g1) * a.vt;
+            vn = fabs(g1) * vt;
         } else {
-            vn = a.vt * sqrt(-2.0 * log(1.0 - u01(o.x, o.y)));
+            vn = vt * sqrt(-2.0 * log(1.0 - u01(o.x, o.y)));
         }
     }
     for (int d = 0; d < 3; ++d) U[d] = n1 * T.t1[d] + n2 * T.t2[d] - vn * T.n[d];
@@ -639,7 +702,8 @@ __global__ void __launch_bounds__(256) k_inject(InjectArgs<UT> a) {
     if (i >= a.cap) { atomicOr(m.flags, kFlagCapacity); return; }
     int t;
     double y1, y2, y3, U[3];
-    emit(a, (uint32_t)k, t, y1, y2, y3, U);
+    emit(m, a.tri, a.cum, a.ntri, a.cell_tet0, a.vmodel, a.vt, true, (uint32_t)k, (a.model << 16) | m.species,
+         kTagInject, t, y1, y2, y3, U);
     vec3<UT> u = mk3<UT>((UT)U[0], (UT)U[1], (UT)U[2]);
     if (a.sync && t >= 0) {
         const TetGeo& g = m.tets[t];

@@ -13,12 +13,15 @@ struct WallHit {
     int32_t pad;
 };
 
+struct InjectTri;
+
 template <class UT>
 struct MoveArgs {
     PartSoA<UT> p;
     int64_t n;                       // launch bound (>= the live count)
     const int64_t* n_live;           // live parcel count on the device (null: n is exact)
     const TetGeo* tets;
+    const int32_t* cell_tet0;
     const UT* Ecell;
     const UT* Epoint;
     const UT* Bcell;
@@ -26,8 +29,13 @@ struct MoveArgs {
     double dt;
     int do_push, rotate, hc_ref;
     uint32_t track_mask;             // bit d: direction d is empty, its velocity component is not tracked
-    // wall patches (PICParcel.C:153-184, :209-330): kind 1 reflects, 0 deletes (null: no walls)
-    const uint8_t* patch_kind;       // [n_patches]
+    // non-coupled patches (PICParcel.C:153-184, :209-330; SimpleInjection.C:233-292): kind 0
+    // deletes, 1 reflects by the wall model, 2 reflects specularly without the resync, 3
+    // re-emits the parcel from the injector's triangles (null: no walls, every kind 0)
+    const uint8_t* patch_kind;       // [n_patches] of this species
+    const InjectTri* re_tri;         // kind 3: the injector patch's triangles
+    int re_ntri, re_vmodel;
+    double re_vt;
     const double* wall_T;            // [n_patches] boundaryT
     const double* wall_U;            // [n_patches * 3] boundaryU
     int wall_model;                  // 0 SpecularReflection, 1 MaxwellianThermal, 2 MixedDiffuseSpecular

@@ -133,7 +133,9 @@ class pic_injection(C.Structure):
 # SimpleInjectionCoeffs.velocityModel (SimpleInjection.C:79-97)
 INJECT_VELOCITY_MODELS = {"birdMaxwellianFlux": 0, "mostProbableSpeed": 1, "halfMaxwellianFlux": 2,
                           "maxwellianFlux": 3}
-INJECT_INTERACTIONS = {"deletion": 0, "reflection": 1, "floating": 3}
+# interactions { <species> { type deletion|reflection|floating; useInitalVelocity b; } }:
+# reflection with useInitalVelocity true is "reflectionWithInitialVelocity" here
+INJECT_INTERACTIONS = {"deletion": 0, "reflection": 1, "reflectionWithInitialVelocity": 2, "floating": 3}
 
 
 class pic_diag(C.Structure):

@@ -525,7 +525,7 @@ def test_simple_injection_refusals(tmp_path, what):
         _edit(props, "e   { type deletion; }", "e   { type absorb; }")
         needle = "interaction type absorb not defined"
     else:
-        # reflection at the injector is read but not built: refused when the context is made
+        # reflection with the initial velocity is read as interaction 2
         _edit(props, "e   { type deletion; }", "e   { type reflection; useInitalVelocity true; }")
         c = case(d)
         assert c.injection()[0]["interaction"][0] == 2

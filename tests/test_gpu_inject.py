@@ -207,7 +207,7 @@ def test_inject_refusals(oracle):
     c = picmi.Cloud(m, [{"mass": ME, "charge": -QE, "nEquivalentParticles": 1.0, "capacity": 10}], models())
     base = {"patch": 0, "velocityModel": 3, "injectionTypeIds": [0], "frequencies": [1.0], "eV": [1.0]}
     with pytest.raises(picmi.PicError) as e:
-        c.set_injection([dict(base, interactions={0: "reflection"})])
+        c.set_injection([dict(base, interactions={0: "floating"})])
     assert e.value.code == -7   # PIC_ERR_UNSUPPORTED
     with pytest.raises(picmi.PicError):
         c.set_injection([dict(base, patch=patch_index(m, "cyclic"))])
@@ -216,3 +216,51 @@ def test_inject_refusals(oracle):
     assert e.value.code == -2   # PIC_ERR_UNKNOWN_MODEL
     with pytest.raises(picmi.PicError):
         c.inject(1e-12)
+
+
+@pytest.mark.parametrize("interaction,vmodel", [("reflection", 3), ("reflectionWithInitialVelocity", 3),
+                                                ("reflectionWithInitialVelocity", 0)])
+def test_injector_patch_reflections_parity(oracle, interaction, vmodel):
+    """Parcels driven into the injector patch (no parcels injected: frequency 0) reflect
+    there as SimpleInjection::particleBC says, with the hit's sync; those leaving through
+    the other x patch are deleted.  Positions, tets and velocities equal the oracle's move
+    with the same per-patch interactions."""
+    say("injector patch", interaction, vmodel)
+    from picfoam_amd import picmi
+    rng = np.random.default_rng(41)
+    m = inj_mesh("hex")
+    rm = oracle.RefMesh(m)
+    L = np.ptp(np.asarray(m["points"]), axis=0)
+    n, dt, eV = 6000, 1e-10, 3.0
+    c = picmi.Cloud(m, [{"mass": ME, "charge": -QE, "nEquivalentParticles": 1e5, "capacity": 8000}], models())
+    y, tet, U, ids = random_parcels(rng, rm.n_tets, n, 1.5 * L.max() / dt)
+    E = rng.normal(size=(rm.n_cells, 3)) * 2e4
+    c.set_field(0, E)
+    c.set_particles(0, y, tet, U, ids)
+    c.step_index = STEP
+    px = patch_index(m, "xmin")
+    c.set_injection([{"patch": px, "velocityModel": vmodel, "injectionTypeIds": [0], "frequencies": [0.0],
+                      "eV": [eV], "interactions": {0: interaction}}])
+    c.move(dt)
+    yg, tg, Ug, ig = c.get_particles(0)
+    kinds = np.zeros(len(m["patches"]), np.int32)
+    kinds[px] = 2 if interaction == "reflection" else 3
+    P = len(m["patches"])
+    w = {"model": "SpecularReflection", "T": np.zeros(P), "U": np.zeros((P, 3)), "sync": True, "seed": SEED,
+         "step": STEP, "species": 0, "ids": ids, "patch_kind": kinds,
+         "reemit": {"patch": px, "velocity_model": vmodel, "T": eV * QE / KB}}
+    Ep = rm.cell_to_point(E)
+    yr, Ur, tr, guards, _ = rm.move("Boris", "CellAverage", y, U, tet, -QE, ME, dt, E, Ep, np.zeros_like(E), walls=w)
+    assert guards == 0 and np.array_equal(ig, ids)
+    say(f"  {w['hits']} hits, {int((tr < 0).sum())} deleted")
+    assert w["hits"] > n // 10
+    assert np.array_equal(tg < 0, tr < 0)
+    live = np.nonzero(tr >= 0)[0]
+    mism = live[tg[live] != tr[live]]
+    yy = np.column_stack([1 - yr.sum(1), yr])
+    ties = [i for i in mism if yy[i].min() < 1e-9]
+    err_u = rows_err(Ug[live], Ur[live])
+    xg, xr = rm.positions(yg[live], tg[live]), rm.positions(yr[live], tr[live])
+    err_x = float(np.abs(xg - xr).max() / L.max())
+    say(f"  U err {err_u:.2e}, x err {err_x:.2e}, {len(mism)} tets differ ({len(ties)} at a face)")
+    assert err_u < TOL and err_x < TOL and len(ties) == len(mism) and len(mism) <= 2

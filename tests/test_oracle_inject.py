@@ -152,3 +152,67 @@ def test_sync_is_half_a_step_back(oracle, rm):
     y, tet, U1, _ = oracle.inject(rm, 0, 3, T, ME, 100, 21, 4, 0, q=-QE, sync=True, **args)
     expect = U0 - (-QE) * E[0] * dt / (2 * ME)
     assert np.allclose(U1, expect, rtol=1e-13, atol=1e-9)
+
+
+def _reflect_walls(m, kind, ids, vmodel=1, T=1e4):
+    P = len(m["patches"])
+    kinds = np.zeros(P, np.int32)
+    kinds[0] = kind
+    return {"model": "SpecularReflection", "T": np.zeros(P), "U": np.zeros((P, 3)), "sync": False, "seed": 3,
+            "step": 1, "species": 0, "ids": ids, "patch_kind": kinds,
+            "reemit": {"patch": 0, "velocity_model": vmodel, "T": T}}
+
+
+def test_injector_reflection_mirrors_the_track(oracle):
+    """SimpleInjection's reflection on its patch without the sync: a parcel that reaches the
+    xmin plane with velocity U ends where the mirror image of its straight track ends,
+    x -> -(x0 + Ux dt), with Ux reversed; the other components are unchanged."""
+    n, L = 4, 1e-3
+    m = mesh_x(n, L)
+    rm = oracle.RefMesh(m)
+    rng = np.random.default_rng(3)
+    N2, dt = 500, 1e-9
+    x0 = np.column_stack([rng.uniform(0.01, 0.99, N2) * L / n, rng.uniform(0.01, 0.99, N2) * L,
+                          rng.uniform(0.01, 0.99, N2) * L])
+    cell = (np.minimum((x0[:, 1] / (L / n)).astype(int), n - 1) * n +
+            np.minimum((x0[:, 2] / (L / n)).astype(int), n - 1) * n * n)
+    y, tet = rm.locate(x0, cell)[:2]
+    U = np.column_stack([-(x0[:, 0] + rng.uniform(0.01, 0.5, N2) * L / n) / dt, rng.normal(size=N2) * 1e3,
+                         rng.normal(size=N2) * 1e3])
+    ids = np.arange(N2, dtype=np.uint32)
+    Ez = np.zeros((rm.n_cells, 3))
+    w = _reflect_walls(m, 2, ids)
+    y1, U1, t1, g, _ = rm.move("BorisNR", "CellAverage", y, U, tet, -QE, ME, dt, Ez, None, Ez, walls=w)
+    assert g == 0 and w["hits"] == N2 and np.all(t1 >= 0)
+    x1 = rm.positions(y1, t1)
+    assert np.allclose(U1[:, 0], -U[:, 0], rtol=1e-14) and np.array_equal(U1[:, 1:], U[:, 1:])
+    expect = -(x0[:, 0] + U[:, 0] * dt)
+    assert np.allclose(x1[:, 0], expect, rtol=0, atol=1e-12 * L)
+    assert np.allclose(x1[:, 1:], x0[:, 1:] + U[:, 1:] * dt, rtol=0, atol=1e-12 * L)
+
+
+def test_injector_reemission_takes_the_emitted_velocity(oracle):
+    """Re-emission with the mostProbableSpeed model and no sync: every parcel that reached
+    the patch leaves it with exactly the most probable speed along the inward normal, from
+    a point inside a cell of the patch, for the rest of its step."""
+    n, L = 4, 1e-3
+    m = mesh_x(n, L)
+    rm = oracle.RefMesh(m)
+    rng = np.random.default_rng(4)
+    N2, dt, T = 400, 1e-10, 2e4
+    vmp = math.sqrt(2 * KB * T / ME)
+    x0 = np.column_stack([rng.uniform(0.1, 0.9, N2) * L / n, rng.uniform(0.01, 0.99, N2) * L,
+                          rng.uniform(0.01, 0.99, N2) * L])
+    cell = (np.minimum((x0[:, 1] / (L / n)).astype(int), n - 1) * n +
+            np.minimum((x0[:, 2] / (L / n)).astype(int), n - 1) * n * n)
+    y, tet = rm.locate(x0, cell)[:2]
+    U = np.column_stack([-np.full(N2, 2.0 * L / n / dt), np.zeros(N2), np.zeros(N2)])
+    ids = np.arange(N2, dtype=np.uint32)
+    Ez = np.zeros((rm.n_cells, 3))
+    w = _reflect_walls(m, 3, ids, vmodel=1, T=T)
+    y1, U1, t1, g, _ = rm.move("BorisNR", "CellAverage", y, U, tet, -QE, ME, dt, Ez, None, Ez, walls=w)
+    assert g == 0 and w["hits"] == N2
+    assert np.allclose(U1[:, 0], vmp, rtol=1e-14) and np.abs(U1[:, 1:]).max() < 1e-9 * vmp
+    x1 = rm.positions(y1, t1)
+    # emitted 0.05-0.25 h from the plane, then at most (1 - sf) dt vmp further
+    assert x1[:, 0].min() >= 0.05 * L / n * (1 - 1e-12) and x1[:, 0].max() <= 0.25 * L / n + vmp * dt
