@@ -341,6 +341,15 @@ int pic_collide_binary(pic_ctx* ctx, double dt);       /* BinaryCollisionModel::
  * pic_step runs the injection first, as PICCloud::evolve (PICCloud.C:1435) does. */
 int pic_set_injection(pic_ctx* ctx, const pic_injection* models, int32_t n);
 int pic_inject(pic_ctx* ctx, double dt);               /* BoundaryModelList::injection alone */
+/* The wall surface fields of the last move (PICParcel::wallReflection, PICParcel.C:240-339;
+ * the boundary values of rhoN, rhoM, linearKE, momentum, q and fD and of their per-species
+ * fields), per boundary face in face order (face - n_internal_faces).  species -1: 10
+ * values per face (rhoN, rhoM, linearKE, momentum[3], q, fD[3]); species s: 9 values
+ * (rhoN, rhoM, momentum[3], linearKE, fD[3]).  Each hit adds 1/max(|U.n| |Sf|, 1e-300)
+ * weighted by 1, m, m|U|^2/2, m U_t before and after the reflection, and the energy and
+ * momentum lost, times nEquivalentParticles / (dt |Sf|), to q and fD; n is the normal of the
+ * face triangle the parcel hit.  Zero without wall patches. */
+int pic_wall_fields(pic_ctx* ctx, int32_t species, double* out);
 int pic_deposit(pic_ctx* ctx);                         /* calculateFields: charge density */
 int pic_poisson_solve(pic_ctx* ctx, pic_poisson_stats* stats); /* MaxwellSolver::solveFields */
 int pic_diag_reduce(pic_ctx* ctx, pic_diag* out);      /* DiagnosticsList::info sums */

@@ -783,6 +783,23 @@ static void reflect(const ref_mesh* m, struct track_ctx* x, int patch, v3 n, dou
     st3(Up, U);
 }
 
+/* one side of a wall hit in the wall surface fields (PICParcel.C:266-287, :299-320) */
+static void wall_fields_add(const struct track_ctx* x, double* row, v3 n, double fA, v3 U) {
+    const double m = x->mass, un = vdot(U, n);
+    const v3 Ut = vsub(U, vscale(un, n));
+    const double inv = 1.0 / fmax(fabs(un) * fA, 1e-300);
+    const double ke = 0.5 * m * vdot(U, U) * inv;
+    double* sp = row + 10 + 9 * (int)x->walls->species;
+    const double v[6] = {inv, m * inv, ke, m * Ut.x * inv, m * Ut.y * inv, m * Ut.z * inv};
+    const double w[6] = {inv, m * inv, m * Ut.x * inv, m * Ut.y * inv, m * Ut.z * inv, ke};
+    for (int k = 0; k < 6; ++k) {
+#pragma omp atomic
+        row[k] += v[k];
+#pragma omp atomic
+        sp[k] += w[k];
+    }
+}
+
 /* PICParcel::moveForward (PICParcel.C:81-144) with OpenFOAM-9 barycentric tracking
  * (particle::trackToFace / trackToStationaryTet re-derived): per tet, the displacement
  * (1-f)*dt*U is taken to barycentric form Tx_i = r_i.d (det-scaled); the first face with
@@ -873,7 +890,38 @@ static int track_one(const ref_mesh* m, double* yy, double* Up, int* tetp, doubl
             v3 nn = vadd(vadd(r1, r2), r3);
             nn = vdiv(nn, vmag(nn));
             sync_velocity(m, x, t, y, Up, sf - 0.5);
+            const v3 pre = ld3(Up);
+            double* row = NULL;
+            double fA = 0.0;
+            if (x->walls->face_acc) {
+                const int f = m->tet_face[t];
+                row = x->walls->face_acc + (size_t)(f - m->n_internal) * x->walls->acc_stride;
+                fA = vmag(ld3(m->Sf + 3 * f));
+                wall_fields_add(x, row, nn, fA, pre);
+            }
             reflect(m, x, patch, nn, Up);
+            if (row) {
+                const v3 post = ld3(Up);
+                wall_fields_add(x, row, nn, fA, post);
+                const double sc = x->walls->weight / (x->dt * fA), ms = x->mass;
+                const double dq = sc * (0.5 * ms * vdot(pre, pre) - 0.5 * ms * vdot(post, post));
+                const v3 fd = vscale(sc, vsub(vscale(ms, pre), vscale(ms, post)));
+                double* sp = row + 10 + 9 * (int)x->walls->species;
+#pragma omp atomic
+                row[6] += dq;
+#pragma omp atomic
+                row[7] += fd.x;
+#pragma omp atomic
+                row[8] += fd.y;
+#pragma omp atomic
+                row[9] += fd.z;
+#pragma omp atomic
+                sp[6] += fd.x;
+#pragma omp atomic
+                sp[7] += fd.y;
+#pragma omp atomic
+                sp[8] += fd.z;
+            }
             if (x->resyncs++ == 0) sync_velocity(m, x, t, y, Up, 0.5 - sf);
             ++x->hits;
             ++x->total_hits;

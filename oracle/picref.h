@@ -87,6 +87,12 @@ typedef struct {
      * ParticleEmitter.C:136-221) by the emitter `reemit` describes */
     const int* patch_kind;
     const struct ref_inject_params_s* reemit;
+    /* optional: the wall surface fields (PICParcel.C:240-339) added per boundary face
+     * (face - n_internal) in rows of acc_stride doubles: rhoN, rhoM, linearKE, momentum[3],
+     * q, fD[3], then at 10 + 9 * species: rhoN, rhoM, momentum[3], linearKE, fD[3] */
+    double* face_acc;
+    int acc_stride;
+    double weight;             /* nEquivalentParticles */
 } ref_wall_params;
 int ref_move_walls(const ref_mesh* m, int pusher, int weighting, int n, double* y, double* U, int* tet, double q,
                    double mass, double dt, const double* Ecell, const double* Epoint, const double* Bcell,

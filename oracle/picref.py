@@ -22,7 +22,8 @@ WALL_MODELS = {"SpecularReflection": 0, "MaxwellianThermal": 1, "MixedDiffuseSpe
 class WallParams(C.Structure):
     _fields_ = [("model", C.c_int), ("diffuse_fraction", C.c_double), ("T", C.c_void_p), ("U", C.c_void_p),
                 ("sync", C.c_int), ("seed", C.c_uint64), ("step", C.c_int64), ("species", C.c_uint32),
-                ("id", C.c_void_p), ("hits", C.c_void_p), ("patch_kind", C.c_void_p), ("reemit", C.c_void_p)]
+                ("id", C.c_void_p), ("hits", C.c_void_p), ("patch_kind", C.c_void_p), ("reemit", C.c_void_p),
+                ("face_acc", C.c_void_p), ("acc_stride", C.c_int), ("weight", C.c_double)]
 
 
 class InjectParams(C.Structure):
@@ -212,6 +213,12 @@ class RefMesh:
                        int(walls.get("sync", True)), int(walls["seed"]), int(walls["step"]), int(walls["species"]),
                        _p(ids), _p(hits), None if pk is None else _p(pk),
                        None if rp is None else C.cast(C.pointer(rp), C.c_void_p))
+        acc = walls.get("face_acc")   # [n_boundary_faces, stride] float64, added to in place
+        if acc is not None:
+            assert acc.dtype == np.float64 and acc.flags.c_contiguous
+            w.face_acc = _p(acc)
+            w.acc_stride = acc.shape[1]
+            w.weight = float(walls["weight"])
         g = lib().ref_move_walls(self.h, PUSHERS[pusher], WEIGHTINGS[weighting], len(tet), _p(y), _p(U), _p(tet), q,
                                  mass, dt, _p(Ecell), _p(Epoint), _p(Bcell), int(do_push), int(hc_ref), int(rotate),
                                  C.byref(sl), C.byref(w))

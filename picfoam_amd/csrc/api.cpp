@@ -84,7 +84,11 @@ struct pic_ctx {
     double diffuse_fraction = 0.0;
     int wall_sync = 1;                       // syncVelocityAtBoundary
     bool walls = false;                      // the mesh has wall patches
-    uint8_t* d_patch_kind = nullptr;         // per patch: 1 wall, 0 otherwise
+    uint8_t* d_patch_kind = nullptr;         // per species and patch: what a parcel reaching it does
+    double* d_wall_acc = nullptr;            // wall surface fields of the last step (per boundary face)
+    int32_t* d_tet_face = nullptr;
+    double* d_face_area = nullptr;           // |Sf| per boundary face
+    int wall_stride = 0;
     double *d_wall_T = nullptr, *d_wall_U = nullptr;
     WallHit* d_wall_q = nullptr;             // parcels on a wall after the move (all moving species)
     unsigned* d_wall_n = nullptr;
@@ -352,6 +356,10 @@ void zero_deposit(pic_ctx* c) {
     }
     z.p[z.k] = c->d_big_n; z.words[z.k++] = 1;
     if (c->d_wall_n) { z.p[z.k] = c->d_wall_n; z.words[z.k++] = 1; }
+    if (c->d_wall_acc) {
+        z.p[z.k] = c->d_wall_acc;
+        z.words[z.k++] = 2 * (int64_t)std::max(c->hm.n_faces - c->hm.n_internal, 1) * c->wall_stride;
+    }
     launch_zero(z, c->stream);
     c->big_n_zero = true;
 }
@@ -387,6 +395,13 @@ MoveArgs<UT> move_args(pic_ctx* c, int s, double dt) {
         a.re_ntri = c->re[s].ntri;
         a.re_vmodel = c->re[s].vmodel;
         a.re_vt = c->re[s].vt;
+        a.wall_acc = c->d_wall_acc;
+        a.tet_face = c->d_tet_face;
+        a.face_area = c->d_face_area;
+        a.n_internal = c->hm.n_internal;
+        a.wall_stride = c->wall_stride;
+        a.weight = S.weight;
+        a.dt_step = dt;
         a.wall_T = c->d_wall_T;
         a.wall_U = c->d_wall_U;
         a.wall_model = c->wall_model;
@@ -1280,6 +1295,17 @@ int pic_create(const pic_mesh* mesh, const pic_species* species, int32_t n_speci
         }
         c->h_kind = kind;
         c->d_patch_kind = c->upload(kind);
+        // the wall surface fields: 10 totals and 9 per species for every boundary face
+        const int nb = m.n_faces - m.n_internal;
+        c->wall_stride = 10 + 9 * (int)c->sp.size();
+        std::vector<double> fa(std::max(nb, 1), 0.0);
+        for (int f = m.n_internal; f < m.n_faces; ++f)
+            fa[f - m.n_internal] = std::sqrt(m.Sf[3 * f] * m.Sf[3 * f] + m.Sf[3 * f + 1] * m.Sf[3 * f + 1] +
+                                             m.Sf[3 * f + 2] * m.Sf[3 * f + 2]);
+        c->d_face_area = c->upload(fa);
+        c->d_tet_face = c->upload(m.tet_face);
+        c->d_wall_acc = c->alloc<double>((size_t)std::max(nb, 1) * c->wall_stride);
+        if (!c->d_face_area || !c->d_tet_face || !c->d_wall_acc) return c->fail(PIC_ERR_HIP, "out of device memory (wall fields)");
         c->d_wall_T = c->upload(wT);
         c->d_wall_U = c->upload(wU);
         if (!c->d_patch_kind || !c->d_wall_T || !c->d_wall_U) return c->fail(PIC_ERR_HIP, "out of device memory (walls)");
@@ -2084,6 +2110,23 @@ int pic_set_injection(pic_ctx* c, const pic_injection* models, int32_t n) {
             }
         hipMemcpyAsync(c->d_patch_kind, c->h_kind.data(), c->h_kind.size(), hipMemcpyHostToDevice, c->stream);
     }
+    return PIC_OK;
+}
+
+int pic_wall_fields(pic_ctx* c, int32_t species, double* out) {
+    if (!c || !out || species < -1 || species >= (int32_t)c->sp.size()) return c ? c->fail(PIC_ERR_INVALID, "pic_wall_fields: species") : PIC_ERR_INVALID;
+    const int nb = c->hm.n_faces - c->hm.n_internal;
+    const int k = species < 0 ? 10 : 9;
+    if (!c->d_wall_acc) {
+        std::memset(out, 0, sizeof(double) * (size_t)nb * k);
+        return PIC_OK;
+    }
+    std::vector<double> h((size_t)std::max(nb, 1) * c->wall_stride);
+    hipMemcpyAsync(h.data(), c->d_wall_acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream);
+    if (int r = check_hip(c, hipStreamSynchronize(c->stream), "wall fields")) return r;
+    const int off = species < 0 ? 0 : 10 + 9 * species;
+    for (int f = 0; f < nb; ++f)
+        for (int j = 0; j < k; ++j) out[(size_t)f * k + j] = h[(size_t)f * c->wall_stride + off + j];
     return PIC_OK;
 }
 

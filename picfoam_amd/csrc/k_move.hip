@@ -216,9 +216,29 @@ __device__ __forceinline__ int track(const MoveArgs<UT>& a, int t, double& y1, d
 // outward normal (r1 + r2 + r3 of the tet record), and synchronised back by (1/2 - sf) dt,
 // that last push only on the first reflection of the step as in the reference.  The
 // diffuse draws are Philox keyed by (seed; id, step, species and hit, wall tag).
+// One side of a wall hit's contribution to the wall surface fields (PICParcel.C:266-287 before
+// the reflection, :299-320 after it): 1 / max(|U.n| |Sf|, vSmall) weighted by 1, m, m |U|^2 / 2
+// and m times the tangential velocity, for the totals and the species' own columns.
+template <class UT>
+__device__ __forceinline__ void wall_fields_add(const MoveArgs<UT>& a, double* row, double nx, double ny, double nz,
+                                                double fA, double ux, double uy, double uz) {
+    const double m = a.mass;
+    const double un = ux * nx + uy * ny + uz * nz;
+    const double tx = ux - un * nx, ty = uy - un * ny, tz = uz - un * nz;
+    const double inv = 1.0 / fmax(fabs(un) * fA, 1e-300);
+    const double ke = 0.5 * m * (ux * ux + uy * uy + uz * uz) * inv;
+    double* sp = row + 10 + 9 * a.species;
+    atomicAdd(row + 0, inv); atomicAdd(row + 1, m * inv); atomicAdd(row + 2, ke);
+    atomicAdd(row + 3, m * tx * inv); atomicAdd(row + 4, m * ty * inv); atomicAdd(row + 5, m * tz * inv);
+    atomicAdd(sp + 0, inv); atomicAdd(sp + 1, m * inv);
+    atomicAdd(sp + 2, m * tx * inv); atomicAdd(sp + 3, m * ty * inv); atomicAdd(sp + 4, m * tz * inv);
+    atomicAdd(sp + 5, ke);
+}
+
 template <class UT, int P, int W>
 __device__ __forceinline__ void wall_hit(const MoveArgs<UT>& a, const TetGeo& g, double Y1, double Y2, double Y3,
-                                         double sf, int patch, vec3<UT>& U, uint32_t id, int hits, int& resyncs) {
+                                         double sf, int patch, vec3<UT>& U, uint32_t id, int hits, int& resyncs,
+                                         double* row = nullptr, double fA = 0.0) {
     double nx = g.r[0] + g.r[3] + g.r[6], ny = g.r[1] + g.r[4] + g.r[7], nz = g.r[2] + g.r[5] + g.r[8];
     const double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
     nx *= inv; ny *= inv; nz *= inv;
@@ -231,6 +251,8 @@ __device__ __forceinline__ void wall_hit(const MoveArgs<UT>& a, const TetGeo& g,
         U = push_u<UT, P>(U, E, B, a.k * f, a.kv * f, a.rotate != 0, a.hc_ref != 0);
     }
     double ux = (double)U.x, uy = (double)U.y, uz = (double)U.z;
+    const double px = ux, py = uy, pz = uz;
+    if (row) wall_fields_add(a, row, nx, ny, nz, fA, ux, uy, uz);
     const uint32_t c = (a.species << 16) | ((uint32_t)hits & 0xffffu);
     uint4 o = rng4(a.seed, id, (uint32_t)a.step, c, kTagWall << 24);
     const double u_mix = u01(o.x, o.y), u_n = u01(o.z, o.w);
@@ -266,6 +288,18 @@ __device__ __forceinline__ void wall_hit(const MoveArgs<UT>& a, const TetGeo& g,
         uz = s * (n1 * tz + n2 * bz - vn * nz) + a.wall_U[3 * patch + 2];
     }
     U = mk3<UT>((UT)ux, (UT)uy, (UT)uz);
+    if (row) {
+        // the reflected velocity as the parcel keeps it, then the energy and momentum given to
+        // the wall per unit time and area (PICParcel.C:322-337)
+        const double vx = (double)U.x, vy = (double)U.y, vz = (double)U.z;
+        wall_fields_add(a, row, nx, ny, nz, fA, vx, vy, vz);
+        const double s = a.weight / (a.dt_step * fA), m = a.mass;
+        atomicAdd(row + 6, s * (0.5 * m * (px * px + py * py + pz * pz) - 0.5 * m * (vx * vx + vy * vy + vz * vz)));
+        const double fx = s * (m * px - m * vx), fy = s * (m * py - m * vy), fz = s * (m * pz - m * vz);
+        atomicAdd(row + 7, fx); atomicAdd(row + 8, fy); atomicAdd(row + 9, fz);
+        double* sp = row + 10 + 9 * a.species;
+        atomicAdd(sp + 6, fx); atomicAdd(sp + 7, fy); atomicAdd(sp + 8, fz);
+    }
     if (sync && resyncs++ == 0) {
         const UT f = (UT)(0.5 - sf);
         U = push_u<UT, P>(U, E, B, a.k * f, a.kv * f, a.rotate != 0, a.hc_ref != 0);
@@ -470,7 +504,14 @@ __global__ void __launch_bounds__(kMoveBlock) k_wall(MoveSet<UT> m) {
             const int kind = a.patch_kind[patch];
             bool on_face = true;
             if (kind == 1) {
-                wall_hit<UT, P, W>(a, g, y1, y2, y3, sf, patch, U, id, hits++, resyncs);
+                double* row = nullptr;
+                double fA = 0.0;
+                if (a.wall_acc) {
+                    const int bf = a.tet_face[t] - a.n_internal;
+                    row = a.wall_acc + (size_t)bf * a.wall_stride;
+                    fA = a.face_area[bf];
+                }
+                wall_hit<UT, P, W>(a, g, y1, y2, y3, sf, patch, U, id, hits++, resyncs, row, fA);
             } else if (kind == 2) {
                 patch_reflect<UT, P, W>(a, g, y1, y2, y3, sf, U);
                 ++hits;

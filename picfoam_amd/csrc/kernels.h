@@ -36,6 +36,14 @@ struct MoveArgs {
     const InjectTri* re_tri;         // kind 3: the injector patch's triangles
     int re_ntri, re_vmodel;
     double re_vt;
+    // wall surface fields (PICParcel.C:240-339), per boundary face (face - n_internal) a row
+    // of wall_stride doubles: rhoN, rhoM, linearKE, momentum[3], q, fD[3], then per species
+    // rhoN, rhoM, momentum[3], linearKE, fD[3] (null: not kept)
+    double* wall_acc;
+    const int32_t* tet_face;
+    const double* face_area;         // [n_faces - n_internal] |Sf|
+    int n_internal, wall_stride;
+    double weight, dt_step;
     const double* wall_T;            // [n_patches] boundaryT
     const double* wall_U;            // [n_patches * 3] boundaryU
     int wall_model;                  // 0 SpecularReflection, 1 MaxwellianThermal, 2 MixedDiffuseSpecular
@@ -180,8 +188,8 @@ struct ScanSet {
 size_t scan_cells_temp_bytes(int n);
 // up to 12 ranges of 4-byte words zeroed by one launch
 struct ZeroSet {
-    void* p[12];
-    int64_t words[12];
+    void* p[16];
+    int64_t words[16];
     int k;
 };
 hipError_t launch_zero(const ZeroSet& z, hipStream_t s);

@@ -40,7 +40,7 @@ EXPORTS = (
     "pic_mesh_tets", "pic_mesh_geometry", "pic_set_cell_field", "pic_get_cell_field", "pic_set_cross_section",
     "pic_mcc_initialize", "pic_ion_mcc_initialize", "pic_mcc_initialize_temperature", "pic_particles_set", "pic_particles_set_positions", "pic_particles_count",
     "pic_particles_get", "pic_particles_positions", "pic_particles_positions_noalloc", "pic_init_maxwellian", "pic_push", "pic_move", "pic_sort",
-    "pic_collide_coulomb", "pic_collide_mcc", "pic_collide_ion_neutral", "pic_collide_binary", "pic_binary_initialize", "pic_set_injection", "pic_inject", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
+    "pic_collide_coulomb", "pic_collide_mcc", "pic_collide_ion_neutral", "pic_collide_binary", "pic_binary_initialize", "pic_set_injection", "pic_inject", "pic_wall_fields", "pic_deposit", "pic_poisson_solve", "pic_diag_reduce", "pic_step",
     "pic_run", "pic_cell_moments",
     "pic_fields_average", "pic_fields_mean", "pic_sync", "pic_step_index", "pic_set_step_index", "pic_profile_enable", "pic_stage_times",
     "pic_profile_reset", "pic_gnn_init", "pic_gnn_n_params", "pic_gnn_params", "pic_gnn_record",
@@ -190,6 +190,7 @@ def load_library(path: str = LIB_PATH):
         "pic_collide_binary": (C.c_int, [vp, dbl]),
         "pic_set_injection": (C.c_int, [vp, vp, i32]),
         "pic_inject": (C.c_int, [vp, dbl]),
+        "pic_wall_fields": (C.c_int, [vp, i32, vp]),
         "pic_binary_initialize": (C.c_int, [vp, vp, vp]),
         "pic_deposit": (C.c_int, [vp]),
         "pic_poisson_solve": (C.c_int, [vp, C.POINTER(pic_poisson_stats)]),
@@ -545,6 +546,15 @@ class Cloud:
 
     def inject(self, dt):
         self._check(self.lib.pic_inject(self.h, dt))
+
+    def wall_fields(self, species=-1):
+        """The last move's wall surface fields per boundary face: [n_boundary_faces, 10]
+        totals (rhoN, rhoM, linearKE, momentum xyz, q, fD xyz) or [.., 9] for one species
+        (rhoN, rhoM, momentum xyz, linearKE, fD xyz)."""
+        nb = len(self.mesh.owner) - len(self.mesh.neighbour)
+        out = np.zeros((nb, 10 if species < 0 else 9))
+        self._check(self.lib.pic_wall_fields(self.h, int(species), _ptr(out)))
+        return out
 
     def deposit(self):
         self._check(self.lib.pic_deposit(self.h))

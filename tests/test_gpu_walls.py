@@ -230,3 +230,54 @@ def test_wall_step_sort_and_deposit(oracle, weighting):
     rho_r = rm.deposit_finish(weighting, ac, ap, Q0)
     assert np.array_equal(c.get_field(3), rho_r), rel_err(c.get_field(3), rho_r)
     c.close()
+
+
+def test_wall_surface_fields_parity(oracle):
+    """The wall surface fields of one move of two species (rhoN, rhoM, linearKE, momentum, q,
+    fD per wall face, and per species; PICParcel.C:240-339) equal the oracle's sums over the
+    same hits with MixedDiffuseSpecular walls and the sync: 1e-12 of each column's largest
+    value (the device adds with atomics, in another order)."""
+    say("wall surface fields")
+    from picfoam_amd import picmi
+    rng = np.random.default_rng(303)
+    m = wall_mesh("hex")
+    rm = oracle.RefMesh(m)
+    L = np.array(m["lengths"])
+    n, dt = 6000, 1e-10
+    sp = [{"mass": ME, "charge": -QE, "nEquivalentParticles": 1e5, "capacity": 8000},
+          {"mass": 4 * ME, "charge": QE, "nEquivalentParticles": 3e5, "capacity": 8000}]
+    c = picmi.Cloud(m, sp, models_for("Boris", "CellAverage", "MixedDiffuseSpecular", 0.5))
+    E = rng.normal(size=(rm.n_cells, 3)) * 2e4
+    c.set_field(0, E)
+    parcels = []
+    for s in range(2):
+        y, tet, U, ids = random_parcels(rng, rm.n_tets, n, 1.5 * L.max() / dt)
+        c.set_particles(s, y, tet, U, ids)
+        parcels.append((y, tet, U, ids))
+    c.step_index = STEP
+    c.move(dt)
+    nb = len(m["owner"]) - len(m["neighbour"])
+    acc = np.zeros((nb, 10 + 9 * 2))
+    Ep = rm.cell_to_point(E)
+    hits = 0
+    for s in range(2):
+        y, tet, U, ids = parcels[s]
+        w = oracle_walls(m, "MixedDiffuseSpecular", 0.5, ids, species=s)
+        w["face_acc"] = acc
+        w["weight"] = sp[s]["nEquivalentParticles"]
+        _, _, _, g, _ = rm.move("Boris", "CellAverage", y, U, tet, sp[s]["charge"], sp[s]["mass"], dt, E, Ep,
+                                np.zeros_like(E), walls=w)
+        assert g == 0
+        hits += w["hits"]
+    got = c.wall_fields(-1)
+    scale = np.abs(acc[:, :10]).max(axis=0) + 1e-300
+    err = float((np.abs(got - acc[:, :10]) / scale).max())
+    say(f"  {hits} hits, totals rel err {err:.2e}")
+    assert hits > n // 2 and err < 1e-12
+    for s in range(2):
+        gs = c.wall_fields(s)
+        ref = acc[:, 10 + 9 * s:19 + 9 * s]
+        es = float((np.abs(gs - ref) / (np.abs(ref).max(axis=0) + 1e-300)).max())
+        say(f"  species {s} rel err {es:.2e}")
+        assert es < 1e-12 and np.abs(ref).max() > 0
+    c.close()

@@ -158,3 +158,58 @@ def test_mixed_diffuse_specular_fraction(oracle):
     spec = np.all(U2[:, 1:] == U[:, 1:], axis=1) & (U2[:, 0] == -U[:, 0])
     k = n - spec.sum()
     assert abs(k - 0.3 * n) < 5 * math.sqrt(n * 0.3 * 0.7)
+
+
+def test_wall_surface_fields_specular_closed_form(oracle):
+    """Specular walls without the sync (PICParcel.C:240-339): per hit the face gains
+    2/(|U.n| A) in rhoN, m times that in rhoM, m |U|^2/(|U.n| A) in linearKE, 2 m U_t/(|U.n| A)
+    in momentum, nothing in q (the speed is kept) and w 2 m (U.n) n / (dt A) in fD, in the
+    totals and in the species' columns."""
+    import numpy as np
+    from helpers import ME, QE
+    from picfoam_amd import meshgen
+    n, L = 3, 1e-3
+    m = meshgen.hex_box(n, n, n, L, L, L, cyclic=(False, True, True), boundary_type="wall")
+    rm = oracle.RefMesh(m)
+    rng = np.random.default_rng(8)
+    N, dt, w = 300, 1e-9, 7.0
+    h = L / n
+    x0 = np.column_stack([rng.uniform(0.2, 0.8, N) * h, rng.uniform(0.05, 0.95, N) * L, rng.uniform(0.05, 0.95, N) * L])
+    cell = (np.minimum((x0[:, 1] / h).astype(int), n - 1) * n + np.minimum((x0[:, 2] / h).astype(int), n - 1) * n * n)
+    y, tet = rm.locate(x0, cell)[:2]
+    th = rng.uniform(0.5, 0.8, N)                       # the hit's step fraction
+    U = np.column_stack([-x0[:, 0] / (th * dt), rng.normal(size=N) * 2e3, rng.normal(size=N) * 2e3])
+    x_hit = x0 + U * (th * dt)[:, None]
+    P = len(m["patches"])
+    nb = len(m["owner"]) - len(m["neighbour"])
+    acc = np.zeros((nb, 10 + 9 * 2))
+    walls = {"model": "SpecularReflection", "T": np.zeros(P), "U": np.zeros((P, 3)), "sync": False, "seed": 1,
+             "step": 0, "species": 1, "ids": np.arange(N, dtype=np.uint32), "face_acc": acc, "weight": w}
+    Ez = np.zeros((rm.n_cells, 3))
+    _, _, _, g, _ = rm.move("BorisNR", "CellAverage", y, U, tet, -QE, ME, dt, Ez, None, Ez, walls=walls)
+    assert g == 0 and walls["hits"] == N
+    _, _, fc, fa = rm.geometry()
+    nin = len(m["neighbour"])
+    xmin = next(p for p in m["patches"] if p["name"] == "xmin")
+    faces = np.arange(xmin["start"], xmin["start"] + xmin["size"])
+    which = faces[np.argmin(np.linalg.norm(x_hit[:, None, 1:] - fc[faces][None, :, 1:], axis=2), axis=1)]
+    exp = np.zeros((nb, 10))
+    A = np.linalg.norm(fa, axis=1)
+    for i in range(N):
+        f = which[i] - nin
+        a = A[which[i]]
+        un = abs(U[i, 0])
+        ut = np.array([0.0, U[i, 1], U[i, 2]])
+        exp[f, 0] += 2 / (un * a)
+        exp[f, 1] += 2 * ME / (un * a)
+        exp[f, 2] += ME * (U[i] @ U[i]) / (un * a)
+        exp[f, 3:6] += 2 * ME * ut / (un * a)
+        exp[f, 7] += w * 2 * ME * U[i, 0] / (dt * a)   # outward normal -x, U.n = -U_x > 0
+    got = acc[:, :10]
+    scale = np.abs(exp).max(axis=0) + 1e-300
+    assert np.all(np.abs(got - exp) <= 1e-12 * scale)
+    assert np.abs(got[:, 6]).max() <= 1e-12 * np.abs(exp[:, 2]).max() * w / dt
+    sp = acc[:, 10 + 9:10 + 18]
+    assert np.allclose(sp[:, [0, 1, 5]], got[:, [0, 1, 2]], rtol=1e-13, atol=0)
+    assert np.allclose(sp[:, 6:9], got[:, 7:10], rtol=1e-13, atol=0)
+    assert np.all(acc[:, 10:19] == 0.0)
